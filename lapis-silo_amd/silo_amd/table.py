@@ -1,0 +1,101 @@
+"""Column data handed to the engine at upload time (the post-diff ingest form).
+
+A `TableSpec` mirrors what the reference's `Table` holds after ingestion and before `finalize()`
+(`storage/table.h:16-68`, `table.cpp:75-94`): a row layout of 2^16-row chunks, sequence columns in
+reference-diffed form, and int/date/dictionary metadata columns.  Values are dense in row order;
+row `i` of chunk `c` has the global id `c << 16 | i`.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .datagen import CHUNK, Dataset, chunk_sizes
+
+NUC_CHARS = "-ACGTRYSWKMBDHVN"
+AA_CHARS = "-ACDEFGHIKLMNOPQRSTUVWYBJZ*X"
+ALPHABET_NUC = 0
+ALPHABET_AA = 1
+
+
+def alphabet_chars(alphabet: int) -> str:
+    return NUC_CHARS if alphabet == ALPHABET_NUC else AA_CHARS
+
+
+def symbol_of(alphabet: int, ch: str) -> int:
+    c = ch.upper()
+    if alphabet == ALPHABET_NUC and c == "U":
+        c = "T"
+    return alphabet_chars(alphabet).index(c)
+
+
+@dataclass
+class SeqColumnData:
+    name: str
+    alphabet: int
+    reference: np.ndarray  # u8 symbols
+    is_null: np.ndarray
+    cov_start: np.ndarray
+    cov_end: np.ndarray
+    mut_ptr: np.ndarray
+    mut_pos: np.ndarray
+    mut_sym: np.ndarray
+    nrun_ptr: np.ndarray
+    nrun_start: np.ndarray
+    nrun_end: np.ndarray
+
+
+@dataclass
+class IntColumnData:
+    name: str
+    is_date: bool
+    values: np.ndarray  # i32
+    is_null: np.ndarray  # u8
+
+
+@dataclass
+class DictColumnData:
+    name: str
+    ids: np.ndarray  # u32
+    is_null: np.ndarray  # u8
+    names: list
+    lineage_parent: np.ndarray | None = None  # i32 per value, -1 = root
+
+
+@dataclass
+class TableSpec:
+    chunk_sizes: np.ndarray
+    seq_columns: list = field(default_factory=list)
+    int_columns: list = field(default_factory=list)
+    dict_columns: list = field(default_factory=list)
+    bitmaps: list = field(default_factory=list)  # literal bitmaps: arrays of global row ids
+
+    @property
+    def n_rows(self) -> int:
+        return int(np.sum(self.chunk_sizes, dtype=np.uint64))
+
+    def global_ids(self) -> np.ndarray:
+        out = []
+        for c, size in enumerate(self.chunk_sizes.tolist()):
+            out.append((np.uint32(c) << np.uint32(16)) + np.arange(size, dtype=np.uint32))
+        return np.concatenate(out) if out else np.zeros(0, dtype=np.uint32)
+
+
+def spec_from_dataset(ds: Dataset, chunk: int = CHUNK, seq_name: str = "main") -> TableSpec:
+    spec = TableSpec(chunk_sizes=chunk_sizes(ds.n_rows, chunk))
+    spec.seq_columns.append(SeqColumnData(
+        seq_name, ALPHABET_NUC, ds.reference, ds.is_null, ds.cov_start, ds.cov_end, ds.mut_ptr, ds.mut_pos,
+        ds.mut_sym, ds.nrun_ptr, ds.nrun_start, ds.nrun_end))
+    spec.int_columns.append(IntColumnData("date", True, ds.date, ds.date_null))
+    spec.dict_columns.append(DictColumnData("country", ds.country, ds.country_null, list(ds.country_names)))
+    spec.dict_columns.append(DictColumnData("lineage", ds.lineage, ds.lineage_null, list(ds.lineage_names),
+                                            ds.lineage_parent))
+    return spec
+
+
+def encode_names(names: list) -> tuple[bytes, np.ndarray]:
+    blobs = [n.encode("utf-8") for n in names]
+    offsets = np.zeros(len(blobs) + 1, dtype=np.uint32)
+    offsets[1:] = np.cumsum([len(b) for b in blobs], dtype=np.uint64).astype(np.uint32)
+    return b"".join(blobs), offsets
