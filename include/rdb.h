@@ -1,0 +1,134 @@
+/*
+ * rdb.h — C ABI of the MI355X query engine for SILO/RhyDB's filter, Mutations and Aggregated path.
+ *
+ * The engine keeps sequence and metadata columns resident on one GPU (gfx950) and evaluates
+ * filter expressions, Mutations counts and group-by counts with hand-written HIP kernels.  All
+ * arguments are plain pointers and sizes; host arrays are borrowed for the duration of a call.
+ * Every entry point returns an `int` status (RDB_OK on success); the text of the last error of a
+ * context is available from rdb_last_error().  No C++ exception crosses this boundary.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   - storage upload: SequenceColumn::appendChunk/finalize (src/rhydb/storage/column/
+ *     sequence_column.cpp:99-212), Date32Column / Int32Column / DictionaryEncodedColumn appendChunk,
+ *     Table::finalize (src/rhydb/storage/table.cpp:87-94);
+ *   - filter evaluation: computeFilter (src/rhydb/query_engine/operators/compute_filter.cpp:14-21)
+ *     = ScalarExpression::rewrite + compile + Operator::evaluate
+ *     (src/rhydb/query_engine/filter/operators/operator.h:24-37);
+ *   - Mutations action: calculateMutationsPerPosition + addMutationsToOutput
+ *     (src/rhydb/query_engine/operators/mutations_node.cpp:269-366);
+ *   - Aggregated action: BitmapAggregationNode::addToExecPlan
+ *     (src/rhydb/query_engine/operators/bitmap_aggregation_node.cpp:304-356);
+ *   - Count action: CountFilterNode::addToExecPlan (src/rhydb/query_engine/operators/
+ *     count_filter_node.cpp:35-71).
+ */
+#ifndef RDB_H
+#define RDB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum rdb_status {
+   RDB_OK = 0,
+   RDB_EINVAL = 1,   /* malformed argument or ingest data */
+   RDB_ENOMEM = 2,   /* device or host allocation failed */
+   RDB_EDEVICE = 3,  /* HIP runtime error */
+   RDB_EQUERY = 4,   /* illegal query (reference: IllegalQueryException, HTTP 400) */
+   RDB_ECOMPILE = 5  /* operator construction rule violated (reference: QueryCompilationException) */
+};
+
+enum rdb_alphabet { RDB_NUCLEOTIDE = 0, RDB_AMINO_ACID = 1 };
+
+typedef struct rdb_ctx rdb_ctx;       /* one GPU, one stream, device memory pool */
+typedef struct rdb_table rdb_table;   /* row layout + device-resident columns */
+typedef struct rdb_bitmap rdb_bitmap; /* device-resident row set (result of a filter) */
+
+/* ---- context ---------------------------------------------------------------------------------- */
+/* Replaces: nothing in the reference (the reference has no device); owns the HIP stream. */
+int rdb_open(int device, rdb_ctx** out);
+void rdb_close(rdb_ctx* ctx);
+const char* rdb_last_error(const rdb_ctx* ctx);
+
+/* ---- tables ----------------------------------------------------------------------------------- */
+/* Row layout of 2^16-row chunks (storage/column/row_layout.h:27-47, row_id.h:16-37).  chunk_sizes
+ * holds the rows of each chunk (<= 65536); n_chunks < 65535. */
+int rdb_table_create(rdb_ctx* ctx, const uint32_t* chunk_sizes, uint32_t n_chunks, rdb_table** out);
+void rdb_table_free(rdb_table* table);
+
+/* A sequence column in post-diff form (storage/column/sequence_column.h:66-71 BufferedSequence,
+ * common/aligned_sequence.cpp:18-122): per row the covered range [cov_start, cov_end), internal
+ * missing runs [nrun_start, nrun_end) (CSR by nrun_ptr) and mutations against `reference`
+ * (CSR by mut_ptr; positions strictly increasing; symbols in the alphabet's order, never the
+ * missing symbol).  genome_length < 65536. */
+int rdb_table_add_sequence_column(rdb_table* table, const char* name, int alphabet, const uint8_t* reference,
+                                  uint32_t genome_length, const uint8_t* is_null, const uint32_t* cov_start,
+                                  const uint32_t* cov_end, const uint64_t* mut_ptr, const uint32_t* mut_pos,
+                                  const uint8_t* mut_sym, const uint64_t* nrun_ptr, const uint32_t* nrun_start,
+                                  const uint32_t* nrun_end);
+/* int32 or date32 column (storage/column/int_column.h:140, date32_column.h:20-80). */
+int rdb_table_add_int_column(rdb_table* table, const char* name, int is_date, const int32_t* values,
+                             const uint8_t* is_null);
+/* Indexed dictionary column (storage/column/dictionary_encoded_column.h:67-140); names are the
+ * dictionary strings (n_values of them, concatenated, name_offsets[n_values + 1]).  A non-null
+ * lineage_parent (-1 = root) also builds the lineage index (storage/column/lineage_index.h:17-42). */
+int rdb_table_add_dict_column(rdb_table* table, const char* name, const uint32_t* ids, const uint8_t* is_null,
+                              uint32_t n_values, const char* names, const uint32_t* name_offsets,
+                              const int32_t* lineage_parent);
+/* A literal row set usable in filters as (bitmap ID) — the IndexScan over a literal bitmap that the
+ * reference's operator tests use (filter/operators/index_scan.h).  Row ids must lie in existing
+ * chunks. */
+int rdb_table_add_bitmap(rdb_table* table, const uint32_t* rows, uint64_t n, uint32_t* id_out);
+/* Local-reference adaptation, container build and upload (sequence_column.cpp:157-212). */
+int rdb_table_finalize(rdb_table* table);
+/* The local reference after adaptation (SequenceColumn::getLocalReference, sequence_column.h:118). */
+int rdb_table_local_reference(const rdb_table* table, const char* column, uint8_t* out);
+/* Device bytes held by the table and the vertical-index payload statistics of a sequence column. */
+int rdb_table_stats(const rdb_table* table, const char* column, uint64_t* device_bytes, uint64_t* n_containers,
+                    uint64_t* payload_bytes);
+
+/* ---- filters ---------------------------------------------------------------------------------- */
+/* computeFilter (operators/compute_filter.cpp:14-21).  `filter` is the expression tree in the
+ * engine's S-expression form, e.g.
+ *   (and (date_between date 18262 18351) (str_eq country "Switzerland"))
+ *   (nof 2 0 (sym_eq main 122 T) (has_mut main 2000) (not (lineage lineage "B.1" 1)))
+ * The result stays on the device; *cardinality receives its row count. */
+int rdb_filter(rdb_table* table, const char* filter, rdb_bitmap** out, uint64_t* cardinality);
+int rdb_bitmap_cardinality(const rdb_bitmap* bitmap, uint64_t* cardinality);
+/* Global row ids (chunk << 16 | row), ascending (CopyOnWriteBitmap::toRoaring, copy_on_write_bitmap.cpp:352-364). */
+int rdb_bitmap_rows(rdb_bitmap* bitmap, uint32_t* rows, uint64_t capacity, uint64_t* n);
+void rdb_bitmap_free(rdb_bitmap* bitmap);
+/* CountFilterNode: |filter| (count_filter_node.cpp:35-71). */
+int rdb_count(rdb_table* table, const char* filter, uint64_t* count);
+
+/* ---- actions ---------------------------------------------------------------------------------- */
+/* Per-symbol counts counts[symbol][position] (u32, n_symbols x genome_length, symbols of the
+ * column's alphabet) after local-reference accumulation (mutations_node.cpp:206-288). */
+int rdb_mutation_counts(rdb_table* table, const rdb_bitmap* filter, const char* column, uint32_t* counts);
+/* Mutations rows (mutations_node.cpp:292-366) in output order: position ascending (1-based), then
+ * symbol in VALID_MUTATION_SYMBOLS order.  proportion = count / coverage in double. */
+int rdb_mutations(rdb_table* table, const rdb_bitmap* filter, const char* column, double min_proportion,
+                  uint32_t* position, uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage,
+                  double* proportion, uint64_t capacity, uint64_t* n);
+/* Aggregated count group-by (bitmap_aggregation_node.cpp:91-130, 189-253).  dims: comma-separated
+ * dimensions; "name" = indexed dictionary column, "name@pos" = symbol at 0-based position of a
+ * sequence column.  Output: combinations in the reference's depth-first order, n_dims ids each
+ * (dictionary value id or symbol; UINT32_MAX = null group) and their counts. */
+int rdb_aggregate(rdb_table* table, const rdb_bitmap* filter, const char* dims, uint32_t* ids, uint64_t* counts,
+                  uint64_t capacity, uint64_t* n);
+
+/* ---- ingest helper ---------------------------------------------------------------------------- */
+/* extractCoverageAndMutationsFromSequence (common/aligned_sequence.cpp:18-122) with offset 0.
+ * Returns RDB_EINVAL on an illegal character and RDB_ENOMEM when a capacity is too small. */
+int rdb_diff_sequence(int alphabet, const char* sequence, uint32_t length, const char* reference,
+                      uint32_t* start, uint32_t* end, uint32_t* missing, uint64_t missing_capacity,
+                      uint64_t* n_missing, uint32_t* mut_pos, uint8_t* mut_sym, uint64_t mut_capacity,
+                      uint64_t* n_mut);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RDB_H */

@@ -1,0 +1,589 @@
+// The C ABI (include/rdb.h): status codes at the boundary, exceptions inside.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "engine.h"
+#include "query.h"
+
+using namespace rdb;
+
+namespace rdb {
+
+void Context::init(int dev) {
+   device = dev;
+   hipCheck(hipSetDevice(dev), "hipSetDevice");
+   hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+   hipDeviceProp_t prop{};
+   if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) cu_count = prop.multiProcessorCount;
+   hipCheck(hipHostMalloc(&stage_host, kStageSlots * kStageSlotBytes, hipHostMallocDefault), "pinned staging");
+   stage_dev = DeviceBuffer(kStageSlots * kStageSlotBytes);
+   for (auto& e : stage_events) hipCheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+}
+
+Context::~Context() {
+   if (stream) (void)hipStreamSynchronize(stream);
+   for (auto& e : stage_events) {
+      if (e) (void)hipEventDestroy(e);
+   }
+   if (stage_host) (void)hipHostFree(stage_host);
+   if (pinned) (void)hipHostFree(pinned);
+   if (stream) (void)hipStreamDestroy(stream);
+}
+
+void* Context::stage(const void* host, size_t bytes) {
+   if (bytes > kStageSlotBytes) {
+      hipCheck(hipStreamSynchronize(stream), "synchronize");
+      if (stage_big.bytes() < bytes) stage_big = DeviceBuffer(bytes);
+      hipCheck(hipMemcpy(stage_big.as<void>(), host, bytes, hipMemcpyHostToDevice), "upload");
+      return stage_big.as<void>();
+   }
+   const size_t k = stage_next;
+   stage_next = (stage_next + 1) % kStageSlots;
+   if (stage_used[k]) hipCheck(hipEventSynchronize(stage_events[k]), "staging slot wait");
+   auto* h = static_cast<uint8_t*>(stage_host) + k * kStageSlotBytes;
+   auto* d = stage_dev.as<uint8_t>() + k * kStageSlotBytes;
+   std::memcpy(h, host, bytes);
+   hipCheck(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream), "upload");
+   hipCheck(hipEventRecord(stage_events[k], stream), "hipEventRecord");
+   stage_used[k] = true;
+   return d;
+}
+
+}  // namespace rdb
+
+namespace {
+
+thread_local std::string g_error;
+
+template <typename F>
+int guarded(Context* ctx, F&& f) {
+   try {
+      f();
+      return RDB_OK;
+   } catch (const Error& e) {
+      g_error = e.what();
+      if (ctx) ctx->error = e.what();
+      return e.code;
+   } catch (const std::bad_alloc&) {
+      g_error = "host allocation failed";
+      if (ctx) ctx->error = g_error;
+      return RDB_ENOMEM;
+   } catch (const std::exception& e) {
+      g_error = e.what();
+      if (ctx) ctx->error = e.what();
+      return RDB_EINVAL;
+   }
+}
+
+Table& tableOf(rdb_table* t) {
+   if (!t) fail(RDB_EINVAL, "null table");
+   return t->t;
+}
+
+void requireFinalized(const Table& t) {
+   if (!t.finalized) fail(RDB_EINVAL, "table is not finalized (call rdb_table_finalize)");
+}
+
+void requireOpen(const Table& t) {
+   if (t.finalized) fail(RDB_EINVAL, "table is already finalized");
+}
+
+bool nameTaken(const Table& t, const std::string& name) {
+   return t.seqIndex(name) >= 0 || t.intIndex(name) >= 0 || t.dictColumn(name) != nullptr;
+}
+
+const DeviceBitmap& filterOf(const Table& t, const rdb_bitmap* f) {
+   if (!f || !f->b.words) fail(RDB_EINVAL, "null filter bitmap");
+   if (f->b.table != &t) fail(RDB_EINVAL, "filter bitmap belongs to another table");
+   return f->b;
+}
+
+uint32_t mutationGroups(const Table& t, const SeqColumn& col) {
+   const uint32_t target = 2u * static_cast<uint32_t>(t.ctx->cu_count);
+   uint32_t g = (target + col.view.n_blocks - 1) / std::max(1u, col.view.n_blocks);
+   return std::clamp<uint32_t>(g, 1u, std::max(1u, t.n_chunks));
+}
+
+}  // namespace
+
+extern "C" {
+
+int rdb_open(int device, rdb_ctx** out) {
+   return guarded(nullptr, [&] {
+      if (!out) fail(RDB_EINVAL, "null output");
+      int n = 0;
+      hipCheck(hipGetDeviceCount(&n), "hipGetDeviceCount");
+      if (device < 0 || device >= n) fail(RDB_EINVAL, "no such device");
+      auto* c = new rdb_ctx;
+      try {
+         c->c.init(device);
+      } catch (...) {
+         delete c;
+         throw;
+      }
+      *out = c;
+   });
+}
+
+void rdb_close(rdb_ctx* ctx) { delete ctx; }
+
+const char* rdb_last_error(const rdb_ctx* ctx) { return ctx ? ctx->c.error.c_str() : g_error.c_str(); }
+
+int rdb_table_create(rdb_ctx* ctx, const uint32_t* chunk_sizes, uint32_t n_chunks, rdb_table** out) {
+   return guarded(ctx ? &ctx->c : nullptr, [&] {
+      if (!ctx || !out || (n_chunks && !chunk_sizes)) fail(RDB_EINVAL, "null argument");
+      if (n_chunks >= 0xFFFF) fail(RDB_EINVAL, "too many chunks (row_layout.h:44 allows fewer than 65535)");
+      auto* t = new rdb_table;
+      Table& tb = t->t;
+      tb.ctx = &ctx->c;
+      uint64_t off = 0;
+      for (uint32_t c = 0; c < n_chunks; ++c) {
+         if (chunk_sizes[c] > kChunkRows) {
+            delete t;
+            fail(RDB_EINVAL, "a chunk holds at most 65536 rows");
+         }
+         tb.chunk_sizes.push_back(chunk_sizes[c]);
+         tb.chunk_offset.push_back(off);
+         off += chunk_sizes[c];
+      }
+      tb.chunk_offset.push_back(off);
+      tb.n_chunks = n_chunks;
+      tb.num_rows = off;
+      *out = t;
+   });
+}
+
+void rdb_table_free(rdb_table* table) {
+   if (!table) return;
+   Context* ctx = table->t.ctx;
+   std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+   (void)hipStreamSynchronize(ctx->stream);
+   delete table;
+}
+
+int rdb_table_add_sequence_column(rdb_table* table, const char* name, int alphabet, const uint8_t* reference,
+                                  uint32_t genome_length, const uint8_t* is_null, const uint32_t* cov_start,
+                                  const uint32_t* cov_end, const uint64_t* mut_ptr, const uint32_t* mut_pos,
+                                  const uint8_t* mut_sym, const uint64_t* nrun_ptr, const uint32_t* nrun_start,
+                                  const uint32_t* nrun_end) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireOpen(t);
+      if (!name || !reference || !is_null || !cov_start || !cov_end || !mut_ptr || !nrun_ptr) fail(RDB_EINVAL, "null argument");
+      if (nameTaken(t, name)) fail(RDB_EINVAL, std::string("duplicate column '") + name + "'");
+      if (alphabet != RDB_NUCLEOTIDE && alphabet != RDB_AMINO_ACID) fail(RDB_EINVAL, "unknown alphabet");
+      auto col = std::make_unique<SeqColumn>();
+      col->name = name;
+      col->alphabet = alphabet;
+      col->abc = &alphabetFor(alphabet);
+      col->genome_length = genome_length;
+      col->reference.assign(reference, reference + genome_length);
+      buildSequenceColumn(t, *col, is_null, cov_start, cov_end, mut_ptr, mut_pos, mut_sym, nrun_ptr, nrun_start, nrun_end);
+      t.seq.push_back(std::move(col));
+   });
+}
+
+int rdb_table_add_int_column(rdb_table* table, const char* name, int is_date, const int32_t* values, const uint8_t* is_null) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireOpen(t);
+      if (!name || (t.num_rows && (!values || !is_null))) fail(RDB_EINVAL, "null argument");
+      if (nameTaken(t, name)) fail(RDB_EINVAL, std::string("duplicate column '") + name + "'");
+      auto col = std::make_unique<IntColumn>();
+      col->name = name;
+      col->is_date = is_date != 0;
+      buildIntColumn(t, *col, values, is_null);
+      t.ints.push_back(std::move(col));
+   });
+}
+
+int rdb_table_add_dict_column(rdb_table* table, const char* name, const uint32_t* ids, const uint8_t* is_null,
+                              uint32_t n_values, const char* names, const uint32_t* name_offsets,
+                              const int32_t* lineage_parent) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireOpen(t);
+      if (!name || (t.num_rows && (!ids || !is_null)) || (n_values && (!names || !name_offsets))) fail(RDB_EINVAL, "null argument");
+      if (nameTaken(t, name)) fail(RDB_EINVAL, std::string("duplicate column '") + name + "'");
+      auto col = std::make_unique<DictColumn>();
+      col->name = name;
+      for (uint32_t v = 0; v < n_values; ++v) {
+         if (name_offsets[v + 1] < name_offsets[v]) fail(RDB_EINVAL, "bad dictionary offsets");
+         col->dictionary.emplace_back(names + name_offsets[v], names + name_offsets[v + 1]);
+         if (!col->ids.emplace(col->dictionary.back(), v).second) fail(RDB_EINVAL, "duplicate dictionary value");
+      }
+      buildDictColumn(t, *col, ids, is_null, lineage_parent);
+      t.dicts.push_back(std::move(col));
+   });
+}
+
+int rdb_table_add_bitmap(rdb_table* table, const uint32_t* rows, uint64_t n, uint32_t* id_out) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireOpen(t);
+      if ((n && !rows) || !id_out) fail(RDB_EINVAL, "null argument");
+      std::vector<uint32_t> v(rows, rows + n);
+      t.literal_bitmaps.push_back(t.addStoreBitmap(v));
+      *id_out = static_cast<uint32_t>(t.literal_bitmaps.size() - 1);
+   });
+}
+
+int rdb_table_finalize(rdb_table* table) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireOpen(t);
+      finalizeTable(t);
+   });
+}
+
+int rdb_table_local_reference(const rdb_table* table, const char* column, uint8_t* out) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      if (!table || !column || !out) fail(RDB_EINVAL, "null argument");
+      const SeqColumn& col = table->t.seqColumn(column);
+      std::copy(col.local_ref.begin(), col.local_ref.end(), out);
+   });
+}
+
+int rdb_table_stats(const rdb_table* table, const char* column, uint64_t* device_bytes, uint64_t* n_containers,
+                    uint64_t* payload_bytes) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      if (!table) fail(RDB_EINVAL, "null table");
+      const Table& t = table->t;
+      uint64_t bytes = 0;
+      for (const auto& c : t.seq) {
+         for (const auto& b : c->buffers) bytes += b.bytes();
+      }
+      for (const auto& c : t.ints) {
+         for (const auto& b : c->buffers) bytes += b.bytes();
+      }
+      for (const auto& c : t.dicts) bytes += c->labels.bytes();
+      for (const auto& b : t.store_buffers) bytes += b.bytes();
+      if (device_bytes) *device_bytes = bytes;
+      if (column) {
+         const SeqColumn& col = t.seqColumn(column);
+         if (n_containers) *n_containers = col.n_containers;
+         if (payload_bytes) *payload_bytes = col.payload_bytes;
+      }
+   });
+}
+
+int rdb_filter(rdb_table* table, const char* filter, rdb_bitmap** out, uint64_t* cardinality) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireFinalized(t);
+      if (!filter || !out) fail(RDB_EINVAL, "null argument");
+      auto op = query::compileFilter(t, filter);
+      auto* bm = new rdb_bitmap;
+      bm->b = op->evaluate();
+      if (cardinality) {
+         try {
+            *cardinality = bm->b.cardinality();
+         } catch (...) {
+            delete bm;
+            throw;
+         }
+      }
+      *out = bm;
+   });
+}
+
+int rdb_bitmap_cardinality(const rdb_bitmap* bitmap, uint64_t* cardinality) {
+   return guarded(nullptr, [&] {
+      if (!bitmap || !cardinality || !bitmap->b.table) fail(RDB_EINVAL, "null argument");
+      auto& b = const_cast<DeviceBitmap&>(bitmap->b);
+      std::lock_guard<std::recursive_mutex> lock(b.table->ctx->mu);
+      *cardinality = b.cardinality();
+   });
+}
+
+int rdb_bitmap_rows(rdb_bitmap* bitmap, uint32_t* rows, uint64_t capacity, uint64_t* n) {
+   return guarded(nullptr, [&] {
+      if (!bitmap || !n || !bitmap->b.table) fail(RDB_EINVAL, "null argument");
+      std::lock_guard<std::recursive_mutex> lock(bitmap->b.table->ctx->mu);
+      const auto v = bitmap->b.rows();
+      *n = v.size();
+      if (v.size() > capacity) fail(RDB_ENOMEM, "output capacity too small");
+      if (!v.empty()) std::copy(v.begin(), v.end(), rows);
+   });
+}
+
+void rdb_bitmap_free(rdb_bitmap* bitmap) {
+   if (!bitmap) return;
+   if (bitmap->b.table) {
+      std::lock_guard<std::recursive_mutex> lock(bitmap->b.table->ctx->mu);
+      delete bitmap;
+   } else {
+      delete bitmap;
+   }
+}
+
+int rdb_count(rdb_table* table, const char* filter, uint64_t* count) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireFinalized(t);
+      if (!filter || !count) fail(RDB_EINVAL, "null argument");
+      auto op = query::compileFilter(t, filter);
+      DeviceBitmap b = op->evaluate();
+      *count = b.cardinality();
+   });
+}
+
+int rdb_mutation_counts(rdb_table* table, const rdb_bitmap* filter, const char* column, uint32_t* counts) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireFinalized(t);
+      if (!column || !counts) fail(RDB_EINVAL, "null argument");
+      const DeviceBitmap& f = filterOf(t, filter);
+      const int idx = t.seqIndex(column);
+      if (idx < 0) fail(RDB_EQUERY, std::string("Database does not contain the sequence '") + column + "'");
+      const SeqColumn& col = *t.seq[idx];
+      const size_t bytes = static_cast<size_t>(col.abc->count) * col.genome_length * 4;
+      void* dev = t.ctx->pool.get(bytes);
+      try {
+         hipCheck(launchMutations(t.seq_views_dev.as<SeqColumnView>() + idx, col.view, t.n_chunks, mutationGroups(t, col),
+                                  f.words, f.chunk_cards, f.card_dev, t.slot_counts.as<uint32_t>(), t.totals.as<int64_t>(),
+                                  0.0, static_cast<uint32_t*>(dev), nullptr, 0, t.mut_n.as<uint32_t>(), t.ctx->stream),
+                  "mutations kernels");
+         hipCheck(hipMemcpyAsync(counts, dev, bytes, hipMemcpyDeviceToHost, t.ctx->stream), "copy counts");
+         hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
+      } catch (...) {
+         t.ctx->pool.put(dev, bytes);
+         throw;
+      }
+      t.ctx->pool.put(dev, bytes);
+   });
+}
+
+int rdb_mutations(rdb_table* table, const rdb_bitmap* filter, const char* column, double min_proportion,
+                  uint32_t* position, uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage,
+                  double* proportion, uint64_t capacity, uint64_t* n) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireFinalized(t);
+      if (!column || !n) fail(RDB_EINVAL, "null argument");
+      if (!(min_proportion >= 0.0 && min_proportion <= 1.0)) fail(RDB_EQUERY, "minProportion must be in interval [0.0,1.0]");
+      const DeviceBitmap& f = filterOf(t, filter);
+      const int idx = t.seqIndex(column);
+      if (idx < 0) fail(RDB_EQUERY, std::string("Database does not contain the sequence '") + column + "'");
+      const SeqColumn& col = *t.seq[idx];
+      const uint32_t row_cap = static_cast<uint32_t>(t.mut_rows.bytes() / sizeof(MutationRowDev));
+      hipCheck(launchMutations(t.seq_views_dev.as<SeqColumnView>() + idx, col.view, t.n_chunks, mutationGroups(t, col),
+                               f.words, f.chunk_cards, f.card_dev, t.slot_counts.as<uint32_t>(), t.totals.as<int64_t>(),
+                               min_proportion, nullptr, t.mut_rows.as<MutationRowDev>(), row_cap, t.mut_n.as<uint32_t>(),
+                               t.ctx->stream),
+               "mutations kernels");
+      uint32_t rows_n = 0;
+      hipCheck(hipMemcpyAsync(&rows_n, t.mut_n.as<void>(), 4, hipMemcpyDeviceToHost, t.ctx->stream), "copy row count");
+      hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
+      if (rows_n > row_cap) fail(RDB_EDEVICE, "internal: mutation row buffer overflow");
+      std::vector<MutationRowDev> rows(rows_n);
+      if (rows_n) {
+         hipCheck(hipMemcpyAsync(rows.data(), t.mut_rows.as<void>(), rows_n * sizeof(MutationRowDev), hipMemcpyDeviceToHost,
+                                 t.ctx->stream),
+                  "copy rows");
+         hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
+      }
+      // mutations_node.cpp:302-364 emits rows by position, then in VALID_MUTATION_SYMBOLS order
+      std::sort(rows.begin(), rows.end(), [](const MutationRowDev& a, const MutationRowDev& b) {
+         return a.position != b.position ? a.position < b.position : a.symbol < b.symbol;
+      });
+      *n = rows.size();
+      if (rows.size() > capacity) fail(RDB_ENOMEM, "output capacity too small");
+      for (size_t i = 0; i < rows.size(); ++i) {
+         if (position) position[i] = rows[i].position + 1;
+         if (from) from[i] = col.reference[rows[i].position];
+         if (to) to[i] = static_cast<uint8_t>(rows[i].symbol);
+         if (count) count[i] = rows[i].count;
+         if (coverage) coverage[i] = rows[i].coverage;
+         if (proportion) proportion[i] = static_cast<double>(rows[i].count) / static_cast<double>(rows[i].coverage);
+      }
+   });
+}
+
+int rdb_aggregate(rdb_table* table, const rdb_bitmap* filter, const char* dims, uint32_t* ids, uint64_t* counts,
+                  uint64_t capacity, uint64_t* n) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireFinalized(t);
+      if (!dims || !n) fail(RDB_EINVAL, "null argument");
+      const DeviceBitmap& f = filterOf(t, filter);
+      struct Dim {
+         bool is_seq;
+         const DictColumn* dict = nullptr;
+         const SeqColumn* seq = nullptr;
+         uint32_t n_labels;  // including the null label
+      };
+      std::vector<Dim> ds;
+      GroupDims gd{};
+      std::string spec(dims);
+      size_t b = 0;
+      while (b <= spec.size()) {
+         size_t e = spec.find(',', b);
+         if (e == std::string::npos) e = spec.size();
+         const std::string d = spec.substr(b, e - b);
+         b = e + 1;
+         if (d.empty()) continue;
+         if (ds.size() == 4) fail(RDB_EQUERY, "at most four grouping dimensions are supported");
+         const auto at = d.find('@');
+         const size_t k = ds.size();
+         if (at != std::string::npos) {
+            const int idx = t.seqIndex(d.substr(0, at));
+            if (idx < 0) fail(RDB_EQUERY, "Database does not contain the sequence '" + d.substr(0, at) + "'");
+            const SeqColumn& col = *t.seq[idx];
+            char* endp = nullptr;
+            const unsigned long p = std::strtoul(d.c_str() + at + 1, &endp, 10);
+            if (*endp != '\0' || p >= col.genome_length) fail(RDB_EQUERY, "bad sequence position in '" + d + "'");
+            const size_t padded = static_cast<size_t>(t.n_chunks) << 16;
+            if (t.labels_scratch[k].bytes() < padded * 4) t.labels_scratch[k] = DeviceBuffer(padded * 4);
+            const uint32_t blk = col.block_of_pos[p];
+            const uint32_t base = col.block_slot_start[blk];
+            hipCheck(launchSymbolLabels(t.seq_views_dev.as<SeqColumnView>() + idx, t.chunk_sizes_dev.as<uint32_t>(),
+                                        static_cast<uint32_t>(p), t.n_chunks, blk, col.pos_slot_ptr[p] - base,
+                                        col.pos_slot_ptr[p + 1] - base, t.labels_scratch[k].as<uint32_t>(), t.ctx->stream),
+                     "symbol labels");
+            ds.push_back(Dim{true, nullptr, &col, col.abc->count + 1});
+            gd.labels[k] = t.labels_scratch[k].as<uint32_t>();
+         } else {
+            const DictColumn* col = t.dictColumn(d);
+            if (!col) fail(RDB_EQUERY, "The database does not contain the column '" + d + "'");
+            ds.push_back(Dim{false, col, nullptr, static_cast<uint32_t>(col->dictionary.size()) + 1});
+            gd.labels[k] = col->labels.as<uint32_t>();
+         }
+      }
+      gd.n = static_cast<uint32_t>(ds.size());
+      if (ds.empty()) {
+         // partition() at depth 0 == #dimensions emits the single empty combination with |F|
+         // (bitmap_aggregation_node.cpp:91-101), also when |F| is 0
+         *n = 1;
+         if (capacity < 1) fail(RDB_ENOMEM, "output capacity too small");
+         counts[0] = const_cast<DeviceBitmap&>(f).cardinality();
+         return;
+      }
+      uint64_t bins = 1;
+      for (int k = static_cast<int>(ds.size()) - 1; k >= 0; --k) {
+         gd.stride[k] = static_cast<uint32_t>(bins);
+         bins *= ds[k].n_labels;
+         if (bins > (1ULL << 28)) fail(RDB_EQUERY, "too many group combinations");
+      }
+      const size_t hb = bins * 4;
+      void* hist = t.ctx->pool.get(hb);
+      std::vector<uint32_t> h(bins);
+      try {
+         hipCheck(launchGroupHistogram(t.n_chunks, f.words, f.chunk_cards, gd, static_cast<uint32_t>(bins),
+                                       static_cast<uint32_t*>(hist), t.ctx->stream),
+                  "group histogram");
+         hipCheck(hipMemcpyAsync(h.data(), hist, hb, hipMemcpyDeviceToHost, t.ctx->stream), "copy histogram");
+         hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
+      } catch (...) {
+         t.ctx->pool.put(hist, hb);
+         throw;
+      }
+      t.ctx->pool.put(hist, hb);
+      // bitmap_aggregation_node.cpp:91-130, 226-253: per dimension, groups by value string (or
+      // symbol order for a sequence position) with the null group last; depth-first emission
+      struct Combo {
+         uint32_t key[4];
+         uint32_t order[4];
+         uint64_t count;
+      };
+      std::vector<Combo> combos;
+      for (uint64_t bin = 0; bin < bins; ++bin) {
+         if (!h[bin]) continue;
+         Combo c{};
+         uint64_t rem = bin;
+         for (size_t k = 0; k < ds.size(); ++k) {
+            const uint32_t lab = static_cast<uint32_t>(rem / gd.stride[k]);
+            rem %= gd.stride[k];
+            const bool is_null = lab == ds[k].n_labels - 1;
+            c.key[k] = is_null ? kInvalidId : lab;
+            c.order[k] = is_null ? kInvalidId : (ds[k].is_seq ? lab : ds[k].dict->rank[lab]);
+         }
+         c.count = h[bin];
+         combos.push_back(c);
+      }
+      std::sort(combos.begin(), combos.end(), [&](const Combo& a, const Combo& b2) {
+         for (size_t k = 0; k < ds.size(); ++k) {
+            if (a.order[k] != b2.order[k]) return a.order[k] < b2.order[k];
+         }
+         return false;
+      });
+      *n = combos.size();
+      if (combos.size() > capacity) fail(RDB_ENOMEM, "output capacity too small");
+      for (size_t i = 0; i < combos.size(); ++i) {
+         for (size_t k = 0; k < ds.size(); ++k) ids[i * ds.size() + k] = combos[i].key[k];
+         counts[i] = combos[i].count;
+      }
+   });
+}
+
+int rdb_diff_sequence(int alphabet, const char* sequence, uint32_t length, const char* reference, uint32_t* start,
+                      uint32_t* end, uint32_t* missing, uint64_t missing_capacity, uint64_t* n_missing,
+                      uint32_t* mut_pos, uint8_t* mut_sym, uint64_t mut_capacity, uint64_t* n_mut) {
+   return guarded(nullptr, [&] {
+      if (!sequence || !reference || !start || !end || !n_missing || !n_mut) fail(RDB_EINVAL, "null argument");
+      const Alphabet& a = alphabetFor(alphabet);
+      // aligned_sequence.cpp:18-122: N is missing, a symbol equal to the reference's is no mutation,
+      // leading/trailing missing positions define the covered range
+      std::vector<uint32_t> miss;
+      std::vector<std::pair<uint32_t, uint8_t>> muts;
+      for (uint32_t i = 0; i < length; ++i) {
+         const int s = a.fromChar(sequence[i]);
+         if (s < 0) {
+            fail(RDB_EINVAL, std::string("illegal character '") + sequence[i] + "' at position " + std::to_string(i) +
+                                " in the input sequence");
+         }
+         if (s == a.missing) {
+            miss.push_back(i);
+         } else if (s != a.fromChar(reference[i])) {
+            muts.emplace_back(i, static_cast<uint8_t>(s));
+         }
+      }
+      size_t lead = 0;
+      while (lead < miss.size() && miss[lead] == lead) ++lead;
+      size_t trail = 0;
+      while (trail < miss.size() - lead && miss[miss.size() - 1 - trail] == length - 1 - trail) ++trail;
+      if (lead + trail == length) {
+         *start = 0;
+         *end = 0;
+         miss.clear();
+      } else {
+         *start = static_cast<uint32_t>(lead);
+         *end = static_cast<uint32_t>(length - trail);
+         miss.assign(miss.begin() + static_cast<ptrdiff_t>(lead), miss.end() - static_cast<ptrdiff_t>(trail));
+      }
+      *n_missing = miss.size();
+      *n_mut = muts.size();
+      if (miss.size() > missing_capacity || muts.size() > mut_capacity) fail(RDB_ENOMEM, "output capacity too small");
+      std::copy(miss.begin(), miss.end(), missing);
+      for (size_t i = 0; i < muts.size(); ++i) {
+         mut_pos[i] = muts[i].first;
+         mut_sym[i] = muts[i].second;
+      }
+   });
+}
+
+}  // extern "C"

@@ -1,0 +1,251 @@
+// Host side of the engine: context, device-resident tables, and the device row-set type that the
+// filter operators return.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "device_abi.h"
+#include "kernels.h"
+#include "rdb.h"
+
+namespace rdb {
+
+// ---- errors --------------------------------------------------------------------------------------
+struct Error : std::runtime_error {
+   int code;
+   Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+inline void hipCheck(hipError_t e, const char* what) {
+   if (e != hipSuccess) fail(RDB_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Runs f(i) for i in [0, n) on up to `threads` host threads.
+void parallelFor(size_t n, const std::function<void(size_t)>& f, unsigned threads = 0);
+
+// ---- alphabets (common/nucleotide_symbols.h:23-106, common/aa_symbols.h:20-118) ----------------
+struct Alphabet {
+   uint32_t count;
+   uint8_t missing;
+   std::string chars;
+   std::vector<uint8_t> valid;                    // VALID_MUTATION_SYMBOLS
+   std::vector<std::vector<uint8_t>> codes_for;   // CODES_FOR
+   std::vector<std::vector<uint8_t>> ambiguity;   // AMBIGUITY_SYMBOLS (derived)
+   int fromChar(char c) const;
+};
+const Alphabet& alphabetFor(int alphabet);
+
+// ---- device memory ---------------------------------------------------------------------------------
+// Long-lived allocation owned by a table.
+class DeviceBuffer {
+  public:
+   DeviceBuffer() = default;
+   explicit DeviceBuffer(size_t bytes);
+   DeviceBuffer(const void* host, size_t bytes);
+   ~DeviceBuffer();
+   DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) {
+      o.ptr_ = nullptr;
+      o.bytes_ = 0;
+   }
+   DeviceBuffer& operator=(DeviceBuffer&& o) noexcept;
+   DeviceBuffer(const DeviceBuffer&) = delete;
+   DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+   template <typename T>
+   T* as() const {
+      return static_cast<T*>(ptr_);
+   }
+   size_t bytes() const { return bytes_; }
+
+  private:
+   void* ptr_ = nullptr;
+   size_t bytes_ = 0;
+};
+
+// Size-class cache of device allocations for per-query results and scratch.
+class DevicePool {
+  public:
+   ~DevicePool();
+   void* get(size_t bytes);
+   void put(void* p, size_t bytes);
+
+  private:
+   std::mutex mu_;
+   std::map<size_t, std::vector<void*>> free_;
+};
+
+struct Context {
+   static constexpr size_t kStageSlots = 64;
+   static constexpr size_t kStageSlotBytes = 1 << 20;
+
+   int device = 0;
+   hipStream_t stream = nullptr;
+   DevicePool pool;
+   std::recursive_mutex mu;
+   std::string error;
+   int cu_count = 256;
+   // pinned host staging
+   void* pinned = nullptr;
+   size_t pinned_bytes = 0;
+   void* pinnedBuffer(size_t bytes);
+   // ring of (pinned host slot, device slot, event) for small asynchronous uploads
+   void* stage_host = nullptr;
+   DeviceBuffer stage_dev;
+   hipEvent_t stage_events[kStageSlots] = {};
+   bool stage_used[kStageSlots] = {};
+   size_t stage_next = 0;
+   DeviceBuffer stage_big;
+   // Copies `bytes` of host data to device memory that stays valid for work enqueued afterwards on
+   // `stream` until the ring wraps (64 further uploads); large uploads synchronize the stream.
+   void* stage(const void* host, size_t bytes);
+   void init(int dev);
+   ~Context();
+};
+
+// ---- containers (host build) -------------------------------------------------------------------
+struct ArenaBuilder {
+   std::vector<Desc> desc;
+   std::vector<uint16_t> arr;
+   std::vector<uint32_t> runs;
+   std::vector<uint64_t> bits;
+   // Appends the container of the sorted, distinct `rows` of one 2^16 block and returns its descriptor.
+   Desc add(const uint16_t* rows, uint32_t n, uint32_t slot);
+};
+
+// Bytes of a container as the reference stores it: array 2*card, bitset 8192, run 4*runs.
+uint64_t containerPayloadBytes(const Desc& d);
+
+// ---- columns --------------------------------------------------------------------------------------
+struct SeqColumn {
+   std::string name;
+   int alphabet = 0;
+   const Alphabet* abc = nullptr;
+   uint32_t genome_length = 0;
+   std::vector<uint8_t> reference;
+   std::vector<uint8_t> local_ref;
+   // host copies used when compiling filter leaves
+   std::vector<uint32_t> block_pos_start, block_slot_start, pos_slot_ptr, slot_info;
+   std::vector<uint32_t> block_of_pos;
+   uint32_t null_bitmap_id = kInvalidId;
+   bool has_nulls = false;
+   SeqColumnView view{};
+   std::vector<DeviceBuffer> buffers;
+   uint64_t n_containers = 0, payload_bytes = 0, desc_bytes = 0;
+   std::vector<uint64_t> chunk_index_bytes;  // per chunk: descriptors + payloads of stored symbols
+   std::vector<uint64_t> chunk_nruns;        // per chunk: internal missing runs
+   uint32_t n_flipped = 0;
+};
+
+struct IntColumn {
+   std::string name;
+   bool is_date = false;
+   bool is_sorted = true;
+   std::vector<int32_t> values;  // dense row order
+   uint32_t null_bitmap_id = kInvalidId;
+   IntColumnView view{};
+   std::vector<DeviceBuffer> buffers;
+};
+
+struct DictColumn {
+   std::string name;
+   std::vector<std::string> dictionary;
+   std::unordered_map<std::string, uint32_t> ids;
+   std::vector<uint32_t> value_bitmap;      // value id -> store id (kInvalidId when no row has it)
+   std::vector<uint32_t> sublineage_bitmap; // value id -> store id (lineage index)
+   uint32_t null_bitmap_id = kInvalidId;
+   bool has_nulls = false;
+   bool lineage = false;
+   std::vector<uint32_t> rank;              // value id -> rank by string order
+   DeviceBuffer labels;                     // per padded row: id, n_values for null, kInvalidId outside
+};
+
+struct Table {
+   Context* ctx = nullptr;
+   std::vector<uint32_t> chunk_sizes;
+   std::vector<uint64_t> chunk_offset;
+   uint32_t n_chunks = 0;
+   uint64_t num_rows = 0;
+   bool finalized = false;
+
+   std::vector<std::unique_ptr<SeqColumn>> seq;
+   std::vector<std::unique_ptr<IntColumn>> ints;
+   std::vector<std::unique_ptr<DictColumn>> dicts;
+   std::vector<uint32_t> literal_bitmaps;  // literal id -> store id
+
+   // bitmap store: rows per (bitmap, chunk), uploaded at finalize
+   std::vector<std::vector<std::vector<uint16_t>>> store_rows;
+   ArenaBuilder store_builder;
+   std::vector<DeviceBuffer> store_buffers;
+
+   DeviceBuffer chunk_sizes_dev, seq_views_dev, int_views_dev;
+   TableView view{};
+
+   // per-call scratch (sized at finalize)
+   DeviceBuffer slot_counts, totals, mut_rows, mut_n, labels_scratch[4], hist;
+   size_t hist_bins_capacity = 0;
+
+   uint32_t addStoreBitmap(const std::vector<uint32_t>& sorted_global_rows);
+   uint32_t globalRow(uint64_t dense) const;
+
+   const SeqColumn& seqColumn(const std::string& name) const;
+   const IntColumn* intColumn(const std::string& name) const;
+   const DictColumn* dictColumn(const std::string& name) const;
+   int seqIndex(const std::string& name) const;
+   int intIndex(const std::string& name) const;
+};
+
+void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const uint32_t* cov_start,
+                         const uint32_t* cov_end, const uint64_t* mut_ptr, const uint32_t* mut_pos,
+                         const uint8_t* mut_sym, const uint64_t* nrun_ptr, const uint32_t* nrun_start,
+                         const uint32_t* nrun_end);
+void buildIntColumn(Table& t, IntColumn& col, const int32_t* values, const uint8_t* is_null);
+void buildDictColumn(Table& t, DictColumn& col, const uint32_t* ids, const uint8_t* is_null,
+                     const int32_t* lineage_parent);
+void finalizeTable(Table& t);
+
+// ---- device row sets ---------------------------------------------------------------------------
+// The engine's counterpart of CopyOnWriteBitmap (query_engine/copy_on_write_bitmap.h:28-147):
+// dense 1024 x u64 words per chunk on the device, with the total and per-chunk cardinalities.
+struct DeviceBitmap {
+   Table* table = nullptr;
+   uint64_t* words = nullptr;
+   uint32_t* chunk_cards = nullptr;
+   unsigned long long* card_dev = nullptr;
+   size_t bytes = 0;
+   uint64_t card = 0;
+   bool card_known = false;
+
+   DeviceBitmap() = default;
+   explicit DeviceBitmap(Table* t);
+   ~DeviceBitmap();
+   DeviceBitmap(DeviceBitmap&& o) noexcept;
+   DeviceBitmap& operator=(DeviceBitmap&& o) noexcept;
+   DeviceBitmap(const DeviceBitmap&) = delete;
+   DeviceBitmap& operator=(const DeviceBitmap&) = delete;
+
+   uint64_t cardinality();
+   std::vector<uint32_t> rows();
+};
+
+}  // namespace rdb
+
+struct rdb_ctx {
+   rdb::Context c;
+};
+struct rdb_table {
+   rdb::Table t;
+};
+struct rdb_bitmap {
+   rdb::DeviceBitmap b;
+};

@@ -1,0 +1,800 @@
+// HIP kernels for gfx950 (MI355X): filter evaluation, Mutations counting, group-by counting.
+//
+// Everything here is integer, HBM-bandwidth-bound work: no MFMA.  Row sets are dense 1024 x u64
+// words per 2^16-row chunk; a wavefront (64 lanes) owns a 64-word slice (4096 rows), one word per
+// lane, so ballots over 64 consecutive rows map directly onto a word.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_abi.h"
+#include "kernels.h"
+
+namespace rdb {
+
+namespace {
+
+constexpr int kVmWaves = 4;
+constexpr int kVmThreads = kVmWaves * 64;
+constexpr int kMutThreads = 512;
+constexpr int kMutWaves = kMutThreads / 64;
+constexpr int kPosBlock = 1024;
+
+__device__ inline void waveSync() {
+   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+   __builtin_amdgcn_wave_barrier();
+}
+
+__device__ inline uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__device__ inline uint64_t layoutMask(uint32_t chunk_size, uint32_t word) {
+   const uint32_t lo = word * 64;
+   if (chunk_size >= lo + 64) return ~0ULL;
+   if (chunk_size <= lo) return 0ULL;
+   return (1ULL << (chunk_size - lo)) - 1ULL;
+}
+
+__device__ inline uint32_t lowerBound16(const uint16_t* a, uint32_t n, uint32_t x) {
+   uint32_t lo = 0, hi = n;
+   while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (static_cast<uint32_t>(a[mid]) < x) {
+         lo = mid + 1;
+      } else {
+         hi = mid;
+      }
+   }
+   return lo;
+}
+
+// First run whose inclusive end is >= x.
+__device__ inline uint32_t firstRunEndingAtOrAfter(const uint32_t* r, uint32_t n, uint32_t x) {
+   uint32_t lo = 0, hi = n;  // first run with start > x
+   while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if ((r[mid] & 0xFFFFu) <= x) {
+         lo = mid + 1;
+      } else {
+         hi = mid;
+      }
+   }
+   if (lo > 0) {
+      const uint32_t prev = r[lo - 1];
+      const uint32_t end = (prev & 0xFFFFu) + (prev >> 16);
+      if (end >= x) return lo - 1;
+   }
+   return lo;
+}
+
+__device__ inline uint64_t rangeMask(uint32_t a, uint32_t b) {
+   // bits a..b inclusive of one word (0 <= a <= b <= 63)
+   const uint64_t hi = (b == 63) ? ~0ULL : ((1ULL << (b + 1)) - 1ULL);
+   return hi & (~0ULL << a);
+}
+
+// ORs the part of one container inside slice `slice` into the lane's word: bitsets and inline
+// containers directly into `acc`, arrays and runs through the wave's LDS scratch slice (LDS
+// atomics), which the caller zeroes before and reads after.
+__device__ inline void accumulateContainer(const Desc d, const ContainerArenas& ar, uint32_t slice, uint32_t lane,
+                                           uint64_t& acc, uint64_t* scratch) {
+   const uint32_t type = descType(d.w0);
+   const uint32_t card = descCard(d.w0);
+   const uint32_t lo = slice * 4096u;
+   const uint32_t hi = lo + 4096u;
+   const uint32_t my_word = slice * 64u + lane;
+   if (type == CT_INLINE) {
+      for (uint32_t k = 0; k < card; ++k) {
+         const uint32_t v = (d.w1 >> (16 * k)) & 0xFFFFu;
+         if ((v >> 6) == my_word) acc |= 1ULL << (v & 63);
+      }
+   } else if (type == CT_BITSET) {
+      acc |= ar.bits[static_cast<size_t>(d.w1) * kChunkWords + my_word];
+   } else if (type == CT_ARRAY) {
+      const uint16_t* a = ar.arr + d.w1;
+      const uint32_t b = lowerBound16(a, card, lo);
+      const uint32_t e = hi >= kChunkRows ? card : lowerBound16(a, card, hi);
+      for (uint32_t k = b + lane; k < e; k += 64) {
+         const uint32_t v = a[k];
+         atomicOr(reinterpret_cast<unsigned long long*>(&scratch[(v >> 6) & 63]), 1ULL << (v & 63));
+      }
+   } else {
+      const uint32_t* r = ar.runs + d.w1;
+      const uint32_t first = firstRunEndingAtOrAfter(r, card, lo);
+      for (uint32_t k = first + lane; k < card; k += 64) {
+         const uint32_t s = r[k] & 0xFFFFu;
+         const uint32_t e = s + (r[k] >> 16);  // inclusive
+         if (s >= hi) continue;
+         const uint32_t a = s > lo ? s : lo;
+         const uint32_t b = e < hi - 1 ? e : hi - 1;
+         if (a > b) continue;
+         for (uint32_t w = a >> 6; w <= (b >> 6); ++w) {
+            const uint32_t x = w * 64 > a ? 0 : a - w * 64;
+            const uint32_t y = w * 64 + 63 < b ? 63 : b - w * 64;
+            atomicOr(reinterpret_cast<unsigned long long*>(&scratch[w & 63]), rangeMask(x, y));
+         }
+      }
+   }
+}
+
+__device__ inline bool compareInt(uint32_t cmp, int32_t x, int32_t v) {
+   switch (cmp) {
+      case CMP_EQ: return x == v;
+      case CMP_LT: return x < v;
+      case CMP_GT: return x > v;
+      case CMP_LE: return x <= v;
+      case CMP_GE: return x >= v;
+      default: return x != v;
+   }
+}
+
+__device__ inline uint32_t waveSum(uint32_t v) {
+   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+   return v;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Filter VM: one wavefront evaluates one program over one (chunk, slice) item; items of several
+// programs (queries) are spread over the grid.  The operand stack lives in LDS, one 64-word row
+// slice per entry, so any operator tree depth fits that the host sized the launch for.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const ProgramDesc* __restrict__ progs,
+                                                             const Instr* __restrict__ instrs,
+                                                             const uint32_t* __restrict__ ranges, uint32_t n_items,
+                                                             uint32_t depth, uint64_t* __restrict__ out,
+                                                             unsigned long long* __restrict__ cards,
+                                                             uint32_t* __restrict__ chunk_cards) {
+   extern __shared__ uint64_t lds[];
+   const uint32_t wave = threadIdx.x >> 6;
+   const uint32_t lane = threadIdx.x & 63;
+   uint64_t* stack = lds + static_cast<size_t>(wave) * (depth + 1) * 64;
+   uint64_t* scratch = stack + static_cast<size_t>(depth) * 64;
+   const uint32_t n_chunks = tv.n_chunks;
+   const uint32_t per_query = n_chunks * kSlicesPerChunk;
+   for (uint32_t item = uniform(blockIdx.x * kVmWaves + wave); item < n_items; item += gridDim.x * kVmWaves) {
+      const uint32_t qi = item / per_query;
+      const uint32_t rem = item - qi * per_query;
+      const uint32_t chunk = rem / kSlicesPerChunk;
+      const uint32_t slice = rem - chunk * kSlicesPerChunk;
+      const ProgramDesc pd = progs[qi];
+      const uint32_t chunk_size = tv.chunk_sizes[chunk];
+      const uint32_t my_word = slice * 64 + lane;
+      const uint64_t lmask = layoutMask(chunk_size, my_word);
+      const size_t row_base = (static_cast<size_t>(chunk) << 16) + slice * 4096u;
+      uint32_t sp = 0;
+      auto push = [&](uint64_t v) {
+         stack[sp * 64 + lane] = v;
+         ++sp;
+      };
+      auto pop = [&]() -> uint64_t {
+         --sp;
+         return stack[sp * 64 + lane];
+      };
+      for (uint32_t i = 0; i < pd.n_instr; ++i) {
+         const Instr ins = instrs[pd.instr_begin + i];
+         switch (ins.op) {
+            case OP_EMPTY:
+               push(0);
+               break;
+            case OP_FULL:
+               push(lmask);
+               break;
+            case OP_BITMAP: {
+               const Desc d = tv.store.desc[static_cast<size_t>(ins.a) * n_chunks + chunk];
+               uint64_t acc = 0;
+               scratch[lane] = 0;
+               waveSync();
+               accumulateContainer(d, tv.store, slice, lane, acc, scratch);
+               waveSync();
+               push(acc | scratch[lane]);
+               break;
+            }
+            case OP_SYMBOLS: {
+               const SeqColumnView& col = tv.seq[ins.a];
+               const uint32_t tile = ins.b * n_chunks + chunk;
+               const uint32_t t0 = col.tile_ptr[tile];
+               const uint32_t th = col.tile_heavy[tile];
+               const uint32_t t1 = col.tile_ptr[tile + 1];
+               const uint32_t slot_base = col.block_slot_start[ins.b];
+               uint64_t acc = 0;
+               scratch[lane] = 0;
+               waveSync();
+               for (int part = 0; part < 2; ++part) {
+                  uint32_t lo = part == 0 ? t0 : th;
+                  const uint32_t end = part == 0 ? th : t1;
+                  uint32_t hi = end;
+                  while (lo < hi) {
+                     const uint32_t mid = (lo + hi) >> 1;
+                     if (descSlot(col.vert.desc[mid].w0) < ins.c) {
+                        lo = mid + 1;
+                     } else {
+                        hi = mid;
+                     }
+                  }
+                  for (uint32_t k = lo; k < end; ++k) {
+                     const Desc d = col.vert.desc[k];
+                     const uint32_t slot = descSlot(d.w0);
+                     if (slot >= ins.d) break;
+                     const uint32_t sym = col.slot_info[slot_base + slot] >> 16;
+                     if ((ins.e >> sym) & 1u) accumulateContainer(d, col.vert, slice, lane, acc, scratch);
+                  }
+               }
+               waveSync();
+               push(acc | scratch[lane]);
+               break;
+            }
+            case OP_COVERED: {
+               const SeqColumnView& col = tv.seq[ins.a];
+               const uint32_t p = ins.b;
+               uint64_t word = 0;
+               for (uint32_t k = 0; k < 64; ++k) {
+                  const size_t r = row_base + k * 64 + lane;
+                  const uint32_t c = col.cov[r];
+                  bool covered = (c & 0xFFFFu) <= p && p < (c >> 16);
+                  if (covered) {
+                     const uint32_t j1 = col.nrun_ptr[r + 1];
+                     for (uint32_t j = col.nrun_ptr[r]; j < j1; ++j) {
+                        const uint32_t x = col.nruns[j];
+                        if ((x & 0xFFFFu) <= p && p < (x >> 16)) {
+                           covered = false;
+                           break;
+                        }
+                     }
+                  }
+                  const uint64_t m = __ballot(covered);
+                  if (lane == k) word = m;
+               }
+               push(word & lmask);
+               break;
+            }
+            case OP_CMP: {
+               const IntColumnView& col = tv.ints[ins.a];
+               const int32_t v = static_cast<int32_t>(ins.c);
+               uint64_t word = 0;
+               for (uint32_t k = 0; k < 64; ++k) {
+                  const size_t r = row_base + k * 64 + lane;
+                  const uint64_t nb = col.null_bits[static_cast<size_t>(chunk) * kChunkWords + slice * 64 + k];
+                  const bool is_null = (nb >> lane) & 1ULL;
+                  const bool m = is_null ? (ins.d != 0) : compareInt(ins.b, col.values[r], v);
+                  const uint64_t b = __ballot(m);
+                  if (lane == k) word = b;
+               }
+               push(word & lmask);
+               break;
+            }
+            case OP_RANGES: {
+               const uint32_t g0 = (chunk << 16) + my_word * 64u;
+               uint64_t word = 0;
+               for (uint32_t j = 0; j < ins.b; ++j) {
+                  const uint32_t lo = ranges[pd.range_begin + 2 * (ins.a + j)];
+                  const uint32_t hi = ranges[pd.range_begin + 2 * (ins.a + j) + 1];
+                  const uint32_t a = lo > g0 ? lo : g0;
+                  const uint32_t b = hi < g0 + 64 ? hi : g0 + 64;
+                  if (a < b) word |= rangeMask(a - g0, b - 1 - g0);
+               }
+               push(word);
+               break;
+            }
+            case OP_AND: {
+               const uint64_t b = pop();
+               const uint64_t a = pop();
+               push(a & b);
+               break;
+            }
+            case OP_OR: {
+               const uint64_t b = pop();
+               const uint64_t a = pop();
+               push(a | b);
+               break;
+            }
+            case OP_ANDNOT: {
+               const uint64_t b = pop();
+               const uint64_t a = pop();
+               push(a & ~b);
+               break;
+            }
+            case OP_NOT:
+               push(pop() ^ lmask);
+               break;
+            case OP_TH_BEGIN:
+               for (uint32_t j = 0; j <= ins.a; ++j) push(0);
+               break;
+            case OP_TH_ADD: {
+               // Counting that reproduces Threshold's DP (threshold.cpp:64-138) bit for bit: a row
+               // enters the count through `e` (the child, or its complement within the layout
+               // for a negated child) and, once entered, gains one per child through `p` (the
+               // child, or the plain set difference for a negated child).  Inside the layout
+               // e == p and this is a population count of matching children.
+               const uint64_t x = pop();
+               const uint32_t base = sp - (ins.a + 1);
+               const uint64_t started = stack[base * 64 + lane];
+               const uint64_t p = ins.b ? ~x : x;
+               const uint64_t e = ins.b ? (x ^ lmask) : x;
+               uint64_t carry = (started & p) | (~started & e);
+               stack[base * 64 + lane] = started | e;
+               for (uint32_t j = 0; j < ins.a; ++j) {
+                  const uint64_t plane = stack[(base + 1 + j) * 64 + lane];
+                  stack[(base + 1 + j) * 64 + lane] = plane ^ carry;
+                  carry &= plane;
+               }
+               break;
+            }
+            case OP_TH_END: {
+               const uint32_t base = sp - (ins.a + 1);
+               uint64_t gt = 0, eq = ~0ULL;
+               for (int j = static_cast<int>(ins.a) - 1; j >= 0; --j) {
+                  const uint64_t plane = stack[(base + 1 + j) * 64 + lane];
+                  if ((ins.b >> j) & 1u) {
+                     eq &= plane;
+                  } else {
+                     gt |= eq & plane;
+                     eq &= ~plane;
+                  }
+               }
+               sp = base;
+               push(ins.c ? eq : (gt | eq));
+               break;
+            }
+            default:
+               break;
+         }
+      }
+      const uint64_t result = stack[lane];
+      out[static_cast<size_t>(pd.query) * n_chunks * kChunkWords + static_cast<size_t>(chunk) * kChunkWords + my_word] = result;
+      const uint32_t n = waveSum(static_cast<uint32_t>(__popcll(result)));
+      if (lane == 0 && n) {
+         atomicAdd(&cards[pd.query], static_cast<unsigned long long>(n));
+         atomicAdd(&chunk_cards[static_cast<size_t>(pd.query) * n_chunks + chunk], n);
+      }
+   }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Mutations counting.  Work-group (position block b, chunk group g): for every chunk c of the group
+// that the filter touches, the filter's 8 KiB tile of c is staged in LDS and every container of
+// tile (b, c) is AND-popcounted against it; the per-slot sums stay in LDS and are added to the
+// global per-slot counts once at the end.
+// ------------------------------------------------------------------------------------------------
+__device__ inline uint32_t bitAt(const uint64_t* f, uint32_t v) {
+   return static_cast<uint32_t>((f[v >> 6] >> (v & 63)) & 1ULL);
+}
+
+__device__ inline uint32_t rangePopcount(const uint64_t* f, uint32_t a, uint32_t b) {
+   // bits a..b inclusive
+   uint32_t n = 0;
+   const uint32_t wa = a >> 6, wb = b >> 6;
+   for (uint32_t w = wa; w <= wb; ++w) {
+      const uint32_t x = w == wa ? (a & 63) : 0;
+      const uint32_t y = w == wb ? (b & 63) : 63;
+      n += __popcll(f[w] & rangeMask(x, y));
+   }
+   return n;
+}
+
+__global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqColumnView* __restrict__ colp,
+                                                                    uint32_t n_chunks, uint32_t n_groups,
+                                                                    const uint64_t* __restrict__ filter,
+                                                                    const uint32_t* __restrict__ filter_chunk_cards,
+                                                                    uint32_t* __restrict__ slot_counts) {
+   __shared__ uint64_t f[kChunkWords];
+   __shared__ uint32_t cnt[kMaxBlockSlots];
+   const SeqColumnView& col = *colp;
+   const uint32_t b = blockIdx.x;
+   const uint32_t g = blockIdx.y;
+   const uint32_t tid = threadIdx.x;
+   const uint32_t lane = tid & 63;
+   const uint32_t wave = tid >> 6;
+   const uint32_t s0 = col.block_slot_start[b];
+   const uint32_t ns = col.block_slot_start[b + 1] - s0;
+   for (uint32_t i = tid; i < ns; i += kMutThreads) cnt[i] = 0;
+   const Desc* __restrict__ desc = col.vert.desc;
+   for (uint32_t c = g; c < n_chunks; c += n_groups) {
+      if (filter_chunk_cards[c] == 0) continue;
+      const uint32_t tile = b * n_chunks + c;
+      const uint32_t d0 = col.tile_ptr[tile];
+      const uint32_t dh = col.tile_heavy[tile];
+      const uint32_t d1 = col.tile_ptr[tile + 1];
+      if (d0 == d1) continue;
+      __syncthreads();
+      const uint64_t* fc = filter + static_cast<size_t>(c) * kChunkWords;
+      for (uint32_t i = tid; i < kChunkWords; i += kMutThreads) f[i] = fc[i];
+      __syncthreads();
+      // light containers: one per lane
+      for (uint32_t k = d0 + tid; k < dh; k += kMutThreads) {
+         const Desc d = desc[k];
+         const uint32_t type = descType(d.w0);
+         const uint32_t card = descCard(d.w0);
+         uint32_t n = 0;
+         if (type == CT_INLINE) {
+            for (uint32_t j = 0; j < card; ++j) n += bitAt(f, (d.w1 >> (16 * j)) & 0xFFFFu);
+         } else if (type == CT_ARRAY) {
+            const uint16_t* a = col.vert.arr + d.w1;
+            for (uint32_t j = 0; j < card; ++j) n += bitAt(f, a[j]);
+         } else if (type == CT_RUN) {
+            const uint32_t* r = col.vert.runs + d.w1;
+            for (uint32_t j = 0; j < card; ++j) {
+               const uint32_t s = r[j] & 0xFFFFu;
+               n += rangePopcount(f, s, s + (r[j] >> 16));
+            }
+         } else {
+            const uint64_t* bits = col.vert.bits + static_cast<size_t>(d.w1) * kChunkWords;
+            for (uint32_t j = 0; j < kChunkWords; ++j) n += __popcll(bits[j] & f[j]);
+         }
+         if (n) atomicAdd(&cnt[descSlot(d.w0)], n);
+      }
+      // heavy containers: one per wavefront
+      for (uint32_t k = dh + wave; k < d1; k += kMutWaves) {
+         const Desc d = desc[k];
+         const uint32_t type = descType(d.w0);
+         const uint32_t card = descCard(d.w0);
+         uint32_t n = 0;
+         if (type == CT_BITSET) {
+            const ulonglong2* bits = reinterpret_cast<const ulonglong2*>(col.vert.bits + static_cast<size_t>(d.w1) * kChunkWords);
+            for (uint32_t j = lane; j < kChunkWords / 2; j += 64) {
+               const ulonglong2 w = bits[j];
+               n += __popcll(w.x & f[2 * j]) + __popcll(w.y & f[2 * j + 1]);
+            }
+         } else if (type == CT_ARRAY) {
+            const uint16_t* a = col.vert.arr + d.w1;
+            for (uint32_t j = lane; j < card; j += 64) n += bitAt(f, a[j]);
+         } else if (type == CT_RUN) {
+            const uint32_t* r = col.vert.runs + d.w1;
+            for (uint32_t j = lane; j < card; j += 64) {
+               const uint32_t s = r[j] & 0xFFFFu;
+               n += rangePopcount(f, s, s + (r[j] >> 16));
+            }
+         } else {
+            if (lane == 0) {
+               for (uint32_t j = 0; j < card; ++j) n += bitAt(f, (d.w1 >> (16 * j)) & 0xFFFFu);
+            }
+         }
+         n = waveSum(n);
+         if (lane == 0 && n) atomicAdd(&cnt[descSlot(d.w0)], n);
+      }
+   }
+   __syncthreads();
+   for (uint32_t i = tid; i < ns; i += kMutThreads) {
+      if (cnt[i]) atomicAdd(&slot_counts[s0 + i], cnt[i]);
+   }
+}
+
+// Per 1024-position block: totals of the three boundary quantities
+//   A(p) = rows of the filter whose covered range starts at p (p > 0),
+//   B(p) = rows whose covered range ends at p,
+//   C(p) = internal missing runs starting at p minus those ending at p.
+__device__ inline void boundaryAt(const SeqColumnView& col, const uint32_t* slot_counts, uint32_t p, uint32_t& A,
+                                  uint32_t& B, int32_t& C) {
+   A = 0;
+   B = 0;
+   C = 0;
+   const uint32_t e = col.pos_slot_ptr[p + 1];
+   for (uint32_t s = col.pos_slot_ptr[p]; s < e; ++s) {
+      const uint32_t sym = col.slot_info[s] >> 16;
+      if (sym < col.n_symbols) continue;
+      const uint32_t v = slot_counts[s];
+      switch (sym - col.n_symbols) {
+         case BND_COV_START: A += v; break;
+         case BND_COV_END: B += v; break;
+         case BND_N_START: C += static_cast<int32_t>(v); break;
+         default: C -= static_cast<int32_t>(v); break;
+      }
+   }
+}
+
+template <typename T>
+__device__ inline T blockInclusiveScan(T v, T* tmp) {
+   // 1024 threads = 16 waves
+   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   for (int off = 1; off < 64; off <<= 1) {
+      const T y = __shfl_up(v, off, 64);
+      if (lane >= static_cast<uint32_t>(off)) v += y;
+   }
+   if (lane == 63) tmp[wave] = v;
+   __syncthreads();
+   if (wave == 0) {
+      T w = lane < 16 ? tmp[lane] : T(0);
+      for (int off = 1; off < 16; off <<= 1) {
+         const T y = __shfl_up(w, off, 64);
+         if (lane >= static_cast<uint32_t>(off)) w += y;
+      }
+      if (lane < 16) tmp[lane] = w;
+   }
+   __syncthreads();
+   const T prefix = wave > 0 ? tmp[wave - 1] : T(0);
+   __syncthreads();
+   return v + prefix;
+}
+
+__global__ __launch_bounds__(kPosBlock) void mutationsTotalsKernel(const SeqColumnView* __restrict__ colp,
+                                                                   const uint32_t* __restrict__ slot_counts,
+                                                                   int64_t* __restrict__ totals) {
+   __shared__ int64_t tmp[3][16];
+   const SeqColumnView& col = *colp;
+   const uint32_t p = blockIdx.x * kPosBlock + threadIdx.x;
+   uint32_t A = 0, B = 0;
+   int32_t C = 0;
+   if (p < col.genome_length) boundaryAt(col, slot_counts, p, A, B, C);
+   int64_t v[3] = {A, B, C};
+   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   for (int q = 0; q < 3; ++q) {
+      int64_t x = v[q];
+      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+      if (lane == 0) tmp[q][wave] = x;
+   }
+   __syncthreads();
+   if (threadIdx.x < 3) {
+      int64_t s = 0;
+      for (int w = 0; w < 16; ++w) s += tmp[threadIdx.x][w];
+      totals[blockIdx.x * 3 + threadIdx.x] = s;
+   }
+}
+
+__global__ __launch_bounds__(kPosBlock) void mutationsFinalizeKernel(
+   const SeqColumnView* __restrict__ colp, const uint32_t* __restrict__ slot_counts,
+   const int64_t* __restrict__ totals, uint32_t n_pblocks, const unsigned long long* __restrict__ filter_card,
+   double min_proportion, uint32_t* __restrict__ counts_out, MutationRowDev* __restrict__ rows,
+   uint32_t row_capacity, uint32_t* __restrict__ n_rows) {
+   __shared__ int64_t scan_tmp[16];
+   __shared__ int64_t prefix[4];  // previous-block sums of A, B, C and the total of A
+   const SeqColumnView& col = *colp;
+   if (threadIdx.x == 0) {
+      int64_t a = 0, b = 0, c = 0, ta = 0;
+      for (uint32_t k = 0; k < n_pblocks; ++k) {
+         if (k < blockIdx.x) {
+            a += totals[3 * k];
+            b += totals[3 * k + 1];
+            c += totals[3 * k + 2];
+         }
+         ta += totals[3 * k];
+      }
+      prefix[0] = a;
+      prefix[1] = b;
+      prefix[2] = c;
+      prefix[3] = ta;
+   }
+   __syncthreads();
+   const uint32_t L = col.genome_length;
+   const uint32_t p = blockIdx.x * kPosBlock + threadIdx.x;
+   uint32_t A = 0, B = 0;
+   int32_t C = 0;
+   if (p < L) boundaryAt(col, slot_counts, p, A, B, C);
+   const int64_t pa = blockInclusiveScan<int64_t>(A, scan_tmp) + prefix[0];
+   const int64_t pb = blockInclusiveScan<int64_t>(B, scan_tmp) + prefix[1];
+   const int64_t pc = blockInclusiveScan<int64_t>(C, scan_tmp) + prefix[2];
+   if (p >= L) return;
+   const int64_t card = static_cast<int64_t>(*filter_card);
+   // rows covered at p: not starting after p, not ended at or before p, not inside a missing run
+   const int64_t covered = card - (prefix[3] - pa) - pb - pc;
+   const uint32_t s_begin = col.pos_slot_ptr[p];
+   const uint32_t s_end = col.pos_slot_ptr[p + 1];
+   int64_t stored = 0;
+   for (uint32_t s = s_begin; s < s_end; ++s) {
+      if ((col.slot_info[s] >> 16) < col.n_symbols) stored += slot_counts[s];
+   }
+   const uint32_t ref_count = card == 0 ? 0u : static_cast<uint32_t>(covered - stored);
+   const uint32_t local = col.local_ref[p];
+   // per-symbol counts in symbol order: slots of p are sorted by symbol
+   uint32_t total = 0;
+   {
+      uint32_t cur = s_begin;
+      for (uint32_t sym = 0; sym < col.n_symbols; ++sym) {
+         uint32_t v = 0;
+         if (cur < s_end && (col.slot_info[cur] >> 16) == sym) v = slot_counts[cur++];
+         if (sym == local) v += ref_count;
+         if (counts_out) counts_out[static_cast<size_t>(sym) * L + p] = v;
+         if ((col.valid_mask >> sym) & 1u) total += v;
+      }
+   }
+   if (total == 0 || rows == nullptr) return;
+   const uint32_t thr = min_proportion == 0.0
+                           ? 0u
+                           : static_cast<uint32_t>(ceil(static_cast<double>(total) * min_proportion) - 1.0);
+   const uint32_t orig = col.reference[p];
+   uint32_t cur = s_begin;
+   for (uint32_t sym = 0; sym < col.n_symbols; ++sym) {
+      uint32_t v = 0;
+      if (cur < s_end && (col.slot_info[cur] >> 16) == sym) v = slot_counts[cur++];
+      if (sym == local) v += ref_count;
+      if (!((col.valid_mask >> sym) & 1u) || sym == orig) continue;
+      if (v > thr) {
+         const uint32_t idx = atomicAdd(n_rows, 1u);
+         if (idx < row_capacity) rows[idx] = MutationRowDev{p, sym, v, total};
+      }
+   }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Group-by: per-row labels of a sequence position (symbol, missing, or null), then a histogram of
+// combined labels over the filter's rows.
+// ------------------------------------------------------------------------------------------------
+__global__ void symbolLabelsKernel(const SeqColumnView* __restrict__ colp, const uint32_t* __restrict__ chunk_sizes,
+                                   uint32_t position, uint32_t n_rows_padded, uint32_t* __restrict__ labels) {
+   const SeqColumnView& col = *colp;
+   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+   if (r >= n_rows_padded) return;
+   if ((r & 0xFFFFu) >= chunk_sizes[r >> 16]) {
+      labels[r] = kInvalidId;  // not a row of the table
+      return;
+   }
+   const bool is_null = (col.null_bits[r >> 6] >> (r & 63)) & 1ULL;
+   uint32_t label;
+   if (is_null) {
+      label = col.n_symbols;  // null group
+   } else {
+      const uint32_t c = col.cov[r];
+      bool covered = (c & 0xFFFFu) <= position && position < (c >> 16);
+      if (covered) {
+         for (uint32_t j = col.nrun_ptr[r]; j < col.nrun_ptr[r + 1]; ++j) {
+            const uint32_t x = col.nruns[j];
+            if ((x & 0xFFFFu) <= position && position < (x >> 16)) {
+               covered = false;
+               break;
+            }
+         }
+      }
+      label = covered ? col.local_ref[position] : (col.n_symbols == 16 ? 15u : 27u);
+   }
+   labels[r] = label;
+}
+
+// One work-group per chunk: rows stored under a symbol at `position` get that symbol.
+__global__ void symbolScatterKernel(const SeqColumnView* __restrict__ colp, uint32_t n_chunks, uint32_t block,
+                                    uint32_t slot_lo, uint32_t slot_hi, uint32_t* __restrict__ labels) {
+   const SeqColumnView& col = *colp;
+   const uint32_t c = blockIdx.x;
+   const uint32_t tile = block * n_chunks + c;
+   const uint32_t slot_base = col.block_slot_start[block];
+   uint32_t* lab = labels + (static_cast<size_t>(c) << 16);
+   for (uint32_t k = col.tile_ptr[tile]; k < col.tile_ptr[tile + 1]; ++k) {
+      const Desc d = col.vert.desc[k];
+      const uint32_t slot = descSlot(d.w0);
+      if (slot < slot_lo || slot >= slot_hi) continue;
+      const uint32_t sym = col.slot_info[slot_base + slot] >> 16;
+      if (sym >= col.n_symbols) continue;
+      const uint32_t type = descType(d.w0), card = descCard(d.w0);
+      if (type == CT_INLINE) {
+         if (threadIdx.x < card) lab[(d.w1 >> (16 * threadIdx.x)) & 0xFFFFu] = sym;
+      } else if (type == CT_ARRAY) {
+         for (uint32_t j = threadIdx.x; j < card; j += blockDim.x) lab[col.vert.arr[d.w1 + j]] = sym;
+      } else if (type == CT_RUN) {
+         for (uint32_t j = 0; j < card; ++j) {
+            const uint32_t x = col.vert.runs[d.w1 + j];
+            const uint32_t s = x & 0xFFFFu, e = s + (x >> 16);
+            for (uint32_t v = s + threadIdx.x; v <= e; v += blockDim.x) lab[v] = sym;
+         }
+      } else {
+         const uint64_t* bits = col.vert.bits + static_cast<size_t>(d.w1) * kChunkWords;
+         for (uint32_t w = threadIdx.x; w < kChunkWords; w += blockDim.x) {
+            uint64_t x = bits[w];
+            while (x) {
+               lab[w * 64 + __builtin_ctzll(x)] = sym;
+               x &= x - 1;
+            }
+         }
+      }
+   }
+}
+
+constexpr uint32_t kHistLdsBins = 16384;
+
+// One work-group per (chunk, quarter) of rows; labels of up to four dimensions combined with
+// strides into one bin.  The histogram is privatised in LDS when it fits.
+__global__ __launch_bounds__(256) void groupHistogramKernel(uint32_t n_chunks, const uint64_t* __restrict__ filter,
+                                                            const uint32_t* __restrict__ filter_chunk_cards,
+                                                            GroupDims dims, uint32_t n_bins,
+                                                            uint32_t* __restrict__ hist) {
+   __shared__ uint32_t lh[kHistLdsBins];
+   const uint32_t c = blockIdx.x >> 2;
+   const uint32_t quarter = blockIdx.x & 3;
+   if (filter_chunk_cards[c] == 0) return;
+   const bool use_lds = n_bins <= kHistLdsBins;
+   if (use_lds) {
+      for (uint32_t i = threadIdx.x; i < n_bins; i += blockDim.x) lh[i] = 0;
+      __syncthreads();
+   }
+   const uint64_t* f = filter + static_cast<size_t>(c) * kChunkWords + quarter * 256;
+   for (uint32_t w = threadIdx.x; w < 256; w += blockDim.x) {
+      uint64_t x = f[w];
+      while (x) {
+         const uint32_t bit = __builtin_ctzll(x);
+         x &= x - 1;
+         const size_t r = (static_cast<size_t>(c) << 16) + (quarter * 256 + w) * 64 + bit;
+         uint32_t key = 0;
+         bool in_table = true;
+         for (uint32_t d = 0; d < dims.n; ++d) {
+            const uint32_t lab = dims.labels[d][r];
+            in_table &= lab != kInvalidId;
+            key += lab * dims.stride[d];
+         }
+         if (!in_table) continue;
+         if (use_lds) {
+            atomicAdd(&lh[key], 1u);
+         } else {
+            atomicAdd(&hist[key], 1u);
+         }
+      }
+   }
+   if (use_lds) {
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < n_bins; i += blockDim.x) {
+         if (lh[i]) atomicAdd(&hist[i], lh[i]);
+      }
+   }
+}
+
+__global__ void fillKernel(uint32_t* __restrict__ p, uint32_t value, size_t n) {
+   for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+      p[i] = value;
+   }
+}
+
+}  // namespace
+
+// ---- launch wrappers ----------------------------------------------------------------------------
+hipError_t launchFill(uint32_t* p, uint32_t value, size_t n, hipStream_t stream) {
+   if (n == 0) return hipSuccess;
+   size_t blocks = (n + 255) / 256;
+   if (blocks > 4096) blocks = 4096;
+   hipLaunchKernelGGL(fillKernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, stream, p, value, n);
+   return hipGetLastError();
+}
+
+size_t filterLdsBytes(uint32_t depth) { return static_cast<size_t>(kVmWaves) * (depth + 1) * 64 * sizeof(uint64_t); }
+
+hipError_t launchFilter(const TableView& tv, const ProgramDesc* progs, const Instr* instrs, const uint32_t* ranges,
+                        uint32_t n_queries, uint32_t depth, uint64_t* out, unsigned long long* cards,
+                        uint32_t* chunk_cards, uint32_t max_blocks, hipStream_t stream) {
+   const uint32_t n_items = n_queries * tv.n_chunks * kSlicesPerChunk;
+   if (n_items == 0) return hipSuccess;
+   uint32_t blocks = (n_items + kVmWaves - 1) / kVmWaves;
+   if (max_blocks && blocks > max_blocks) blocks = max_blocks;
+   hipLaunchKernelGGL(filterVmKernel, dim3(blocks), dim3(kVmThreads), filterLdsBytes(depth), stream, tv, progs,
+                      instrs, ranges, n_items, depth, out, cards, chunk_cards);
+   return hipGetLastError();
+}
+
+hipError_t launchMutations(const SeqColumnView* col_dev, const SeqColumnView& col_host, uint32_t n_chunks,
+                           uint32_t n_groups, const uint64_t* filter, const uint32_t* filter_chunk_cards,
+                           const unsigned long long* filter_card, uint32_t* slot_counts, int64_t* totals,
+                           double min_proportion, uint32_t* counts_out, MutationRowDev* rows, uint32_t row_capacity,
+                           uint32_t* n_rows, hipStream_t stream) {
+   hipError_t err = launchFill(slot_counts, 0, col_host.n_slots, stream);
+   if (err != hipSuccess) return err;
+   err = launchFill(n_rows, 0, 1, stream);
+   if (err != hipSuccess) return err;
+   hipLaunchKernelGGL(mutationsCountKernel, dim3(col_host.n_blocks, n_groups), dim3(kMutThreads), 0, stream, col_dev,
+                      n_chunks, n_groups, filter, filter_chunk_cards, slot_counts);
+   err = hipGetLastError();
+   if (err != hipSuccess) return err;
+   const uint32_t n_pblocks = (col_host.genome_length + kPosBlock - 1) / kPosBlock;
+   hipLaunchKernelGGL(mutationsTotalsKernel, dim3(n_pblocks), dim3(kPosBlock), 0, stream, col_dev, slot_counts, totals);
+   err = hipGetLastError();
+   if (err != hipSuccess) return err;
+   hipLaunchKernelGGL(mutationsFinalizeKernel, dim3(n_pblocks), dim3(kPosBlock), 0, stream, col_dev, slot_counts,
+                      totals, n_pblocks, filter_card, min_proportion, counts_out, rows, row_capacity, n_rows);
+   return hipGetLastError();
+}
+
+hipError_t launchSymbolLabels(const SeqColumnView* col_dev, const uint32_t* chunk_sizes, uint32_t position,
+                              uint32_t n_chunks, uint32_t block, uint32_t slot_lo, uint32_t slot_hi, uint32_t* labels,
+                              hipStream_t stream) {
+   const uint32_t n = n_chunks << 16;
+   hipLaunchKernelGGL(symbolLabelsKernel, dim3((n + 255) / 256), dim3(256), 0, stream, col_dev, chunk_sizes, position, n,
+                      labels);
+   hipError_t err = hipGetLastError();
+   if (err != hipSuccess) return err;
+   hipLaunchKernelGGL(symbolScatterKernel, dim3(n_chunks), dim3(256), 0, stream, col_dev, n_chunks, block, slot_lo,
+                      slot_hi, labels);
+   return hipGetLastError();
+}
+
+hipError_t launchGroupHistogram(uint32_t n_chunks, const uint64_t* filter, const uint32_t* filter_chunk_cards,
+                                const GroupDims& dims, uint32_t n_bins, uint32_t* hist, hipStream_t stream) {
+   hipError_t err = launchFill(hist, 0, n_bins, stream);
+   if (err != hipSuccess) return err;
+   hipLaunchKernelGGL(groupHistogramKernel, dim3(n_chunks * 4), dim3(256), 0, stream, n_chunks, filter,
+                      filter_chunk_cards, dims, n_bins, hist);
+   return hipGetLastError();
+}
+
+}  // namespace rdb

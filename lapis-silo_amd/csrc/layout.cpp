@@ -1,0 +1,908 @@
+// Building the device-resident layout of a table from post-diff ingest arrays.
+//
+// The sequence-column build restates SequenceColumn::appendChunk / finalize
+// (storage/column/sequence_column.cpp:99-212): containers per (position, chunk, symbol) for every
+// symbol that differs from the local reference, with the per-position local reference adapted to
+// the majority symbol of the covered rows (vertical_sequence_index.cpp:56-164, strict '>' in symbol
+// order, coverage excluding missing rows).  The containers are then cut into (position block,
+// chunk) tiles for the Mutations kernel and uploaded once.
+#include <algorithm>
+#include <bit>
+#include <cstring>
+#include <numeric>
+#include <thread>
+
+#include "engine.h"
+
+namespace rdb {
+
+void parallelFor(size_t n, const std::function<void(size_t)>& f, unsigned threads) {
+   if (threads == 0) threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+   if (n <= 1 || threads <= 1) {
+      for (size_t i = 0; i < n; ++i) f(i);
+      return;
+   }
+   std::atomic<size_t> next{0};
+   std::vector<std::thread> pool;
+   std::exception_ptr error;
+   std::mutex error_mu;
+   for (unsigned k = 0; k < std::min<size_t>(threads, n); ++k) {
+      pool.emplace_back([&] {
+         for (;;) {
+            const size_t i = next.fetch_add(1);
+            if (i >= n) break;
+            try {
+               f(i);
+            } catch (...) {
+               std::lock_guard<std::mutex> lock(error_mu);
+               if (!error) error = std::current_exception();
+            }
+         }
+      });
+   }
+   for (auto& th : pool) th.join();
+   if (error) std::rethrow_exception(error);
+}
+
+// ---- alphabets ---------------------------------------------------------------------------------
+int Alphabet::fromChar(char c) const {
+   char u = c;
+   if (u >= 'a' && u <= 'z') u = static_cast<char>(u - 'a' + 'A');
+   if (count == 16 && u == 'U') u = 'T';
+   const auto p = chars.find(u);
+   return p == std::string::npos ? -1 : static_cast<int>(p);
+}
+
+namespace {
+void deriveAmbiguity(Alphabet& a) {
+   a.ambiguity.assign(a.count, {});
+   for (uint8_t s = 0; s < a.count; ++s) {
+      for (uint8_t cand = 0; cand < a.count; ++cand) {
+         const auto& cs = a.codes_for[s];
+         const auto& cc = a.codes_for[cand];
+         const bool superset =
+            std::all_of(cs.begin(), cs.end(), [&](uint8_t x) { return std::find(cc.begin(), cc.end(), x) != cc.end(); });
+         if (superset) a.ambiguity[s].push_back(cand);
+      }
+   }
+}
+}  // namespace
+
+const Alphabet& alphabetFor(int alphabet) {
+   static const Alphabet nuc = [] {
+      Alphabet a;
+      a.count = 16;
+      a.chars = "-ACGTRYSWKMBDHVN";
+      a.missing = 15;
+      a.valid = {0, 1, 2, 3, 4};
+      a.codes_for = {{0}, {1}, {2}, {3}, {4}, {1, 3}, {2, 4}, {2, 3}, {1, 4}, {3, 4}, {1, 2}, {2, 3, 4}, {1, 3, 4}, {1, 2, 4}, {1, 2, 3}, {}};
+      for (uint8_t s = 0; s < 16; ++s) a.codes_for[15].push_back(s);
+      deriveAmbiguity(a);
+      return a;
+   }();
+   static const Alphabet aa = [] {
+      Alphabet a;
+      a.count = 28;
+      a.chars = "-ACDEFGHIKLMNOPQRSTUVWYBJZ*X";
+      a.missing = 27;
+      for (uint8_t s = 0; s < 23; ++s) a.valid.push_back(s);
+      a.valid.push_back(26);
+      a.codes_for.resize(28);
+      for (uint8_t s = 0; s < 23; ++s) a.codes_for[s] = {s};
+      a.codes_for[23] = {3, 12};
+      a.codes_for[24] = {8, 10};
+      a.codes_for[25] = {4, 15};
+      a.codes_for[26] = {26};
+      for (uint8_t s = 0; s < 28; ++s) a.codes_for[27].push_back(s);
+      deriveAmbiguity(a);
+      return a;
+   }();
+   return alphabet == RDB_AMINO_ACID ? aa : nuc;
+}
+
+// ---- device memory -----------------------------------------------------------------------------
+DeviceBuffer::DeviceBuffer(size_t bytes) : bytes_(bytes) {
+   if (bytes_ == 0) return;
+   if (hipMalloc(&ptr_, bytes_) != hipSuccess) {
+      ptr_ = nullptr;
+      fail(RDB_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+   }
+}
+
+DeviceBuffer::DeviceBuffer(const void* host, size_t bytes) : DeviceBuffer(std::max<size_t>(bytes, 16)) {
+   // empty arrays still get a (small) allocation so that every device pointer is valid
+   if (bytes > 0 && host != nullptr) hipCheck(hipMemcpy(ptr_, host, bytes, hipMemcpyHostToDevice), "upload");
+}
+
+DeviceBuffer::~DeviceBuffer() {
+   if (ptr_) (void)hipFree(ptr_);
+}
+
+DeviceBuffer& DeviceBuffer::operator=(DeviceBuffer&& o) noexcept {
+   if (this != &o) {
+      if (ptr_) (void)hipFree(ptr_);
+      ptr_ = o.ptr_;
+      bytes_ = o.bytes_;
+      o.ptr_ = nullptr;
+      o.bytes_ = 0;
+   }
+   return *this;
+}
+
+DevicePool::~DevicePool() {
+   for (auto& [size, list] : free_) {
+      for (void* p : list) (void)hipFree(p);
+   }
+}
+
+void* DevicePool::get(size_t bytes) {
+   {
+      std::lock_guard<std::mutex> lock(mu_);
+      auto it = free_.find(bytes);
+      if (it != free_.end() && !it->second.empty()) {
+         void* p = it->second.back();
+         it->second.pop_back();
+         return p;
+      }
+   }
+   void* p = nullptr;
+   if (hipMalloc(&p, bytes) != hipSuccess) fail(RDB_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+   return p;
+}
+
+void DevicePool::put(void* p, size_t bytes) {
+   if (!p) return;
+   std::lock_guard<std::mutex> lock(mu_);
+   free_[bytes].push_back(p);
+}
+
+void* Context::pinnedBuffer(size_t bytes) {
+   if (bytes > pinned_bytes) {
+      if (pinned) (void)hipHostFree(pinned);
+      pinned = nullptr;
+      pinned_bytes = 0;
+      hipCheck(hipHostMalloc(&pinned, bytes, hipHostMallocDefault), "pinned allocation");
+      pinned_bytes = bytes;
+   }
+   return pinned;
+}
+
+// ---- containers --------------------------------------------------------------------------------
+Desc ArenaBuilder::add(const uint16_t* rows, uint32_t n, uint32_t slot) {
+   if (n <= 2) {
+      uint32_t w1 = 0;
+      for (uint32_t k = 0; k < n; ++k) w1 |= static_cast<uint32_t>(rows[k]) << (16 * k);
+      Desc d{makeW0(slot, CT_INLINE, n), w1};
+      desc.push_back(d);
+      return d;
+   }
+   uint32_t n_runs = 1;
+   for (uint32_t k = 1; k < n; ++k) n_runs += rows[k] != rows[k - 1] + 1;
+   const uint64_t run_bytes = 4ULL * n_runs;
+   const uint64_t plain_bytes = n <= 4096 ? 2ULL * n : 8192ULL;
+   Desc d{};
+   if (run_bytes < plain_bytes) {
+      d = Desc{makeW0(slot, CT_RUN, n_runs), static_cast<uint32_t>(runs.size())};
+      uint32_t k = 0;
+      while (k < n) {
+         uint32_t j = k;
+         while (j + 1 < n && rows[j + 1] == rows[j] + 1) ++j;
+         runs.push_back(static_cast<uint32_t>(rows[k]) | (static_cast<uint32_t>(rows[j] - rows[k]) << 16));
+         k = j + 1;
+      }
+   } else if (n <= 4096) {
+      if (arr.size() + n > 0xFFFFFFFFULL) fail(RDB_ENOMEM, "array arena exceeds 2^32 entries");
+      d = Desc{makeW0(slot, CT_ARRAY, n), static_cast<uint32_t>(arr.size())};
+      arr.insert(arr.end(), rows, rows + n);
+   } else {
+      d = Desc{makeW0(slot, CT_BITSET, 0), static_cast<uint32_t>(bits.size() / kChunkWords)};
+      const size_t base = bits.size();
+      bits.resize(base + kChunkWords, 0);
+      for (uint32_t k = 0; k < n; ++k) bits[base + (rows[k] >> 6)] |= 1ULL << (rows[k] & 63);
+   }
+   desc.push_back(d);
+   return d;
+}
+
+uint64_t containerPayloadBytes(const Desc& d) {
+   switch (descType(d.w0)) {
+      case CT_INLINE:
+      case CT_ARRAY:
+         return 2ULL * descCard(d.w0);
+      case CT_RUN:
+         return 4ULL * descCard(d.w0);
+      default:
+         return 8192;
+   }
+}
+
+// Every descriptor must point inside its arenas: checked on the host before upload so that a layout
+// bug surfaces as an error and never as an out-of-bounds device access.
+void validateDescriptors(const Desc* desc, size_t n, size_t arr_size, size_t run_size, size_t bit_blocks,
+                         uint32_t slot_limit) {
+   for (size_t k = 0; k < n; ++k) {
+      const Desc& d = desc[k];
+      const uint32_t type = descType(d.w0), card = descCard(d.w0);
+      bool ok = descSlot(d.w0) < slot_limit;
+      switch (type) {
+         case CT_INLINE:
+            ok = ok && card <= 2;
+            break;
+         case CT_ARRAY:
+            ok = ok && card <= 4096 && static_cast<uint64_t>(d.w1) + card <= arr_size;
+            break;
+         case CT_RUN:
+            ok = ok && card > 0 && static_cast<uint64_t>(d.w1) + card <= run_size;
+            break;
+         default:
+            ok = ok && d.w1 < bit_blocks;
+            break;
+      }
+      if (!ok) fail(RDB_EDEVICE, "internal: invalid container descriptor " + std::to_string(k));
+   }
+}
+
+// ---- table helpers -----------------------------------------------------------------------------
+uint32_t Table::globalRow(uint64_t dense) const {
+   const auto it = std::upper_bound(chunk_offset.begin(), chunk_offset.end(), dense) - 1;
+   const uint64_t c = static_cast<uint64_t>(it - chunk_offset.begin());
+   return static_cast<uint32_t>((c << 16) | (dense - *it));
+}
+
+uint32_t Table::addStoreBitmap(const std::vector<uint32_t>& rows) {
+   std::vector<std::vector<uint16_t>> per_chunk(n_chunks);
+   for (uint32_t g : rows) {
+      const uint32_t c = g >> 16;
+      if (c >= n_chunks) fail(RDB_EINVAL, "row id " + std::to_string(g) + " lies outside the table's chunks");
+      per_chunk[c].push_back(static_cast<uint16_t>(g & 0xFFFF));
+   }
+   for (auto& v : per_chunk) {
+      std::sort(v.begin(), v.end());
+      v.erase(std::unique(v.begin(), v.end()), v.end());
+   }
+   store_rows.push_back(std::move(per_chunk));
+   return static_cast<uint32_t>(store_rows.size() - 1);
+}
+
+const SeqColumn& Table::seqColumn(const std::string& name) const {
+   for (const auto& c : seq) {
+      if (c->name == name) return *c;
+   }
+   fail(RDB_EQUERY, "Database does not contain the sequence '" + name + "'");
+}
+
+int Table::seqIndex(const std::string& name) const {
+   for (size_t i = 0; i < seq.size(); ++i) {
+      if (seq[i]->name == name) return static_cast<int>(i);
+   }
+   return -1;
+}
+
+const IntColumn* Table::intColumn(const std::string& name) const {
+   for (const auto& c : ints) {
+      if (c->name == name) return c.get();
+   }
+   return nullptr;
+}
+
+int Table::intIndex(const std::string& name) const {
+   for (size_t i = 0; i < ints.size(); ++i) {
+      if (ints[i]->name == name) return static_cast<int>(i);
+   }
+   return -1;
+}
+
+const DictColumn* Table::dictColumn(const std::string& name) const {
+   for (const auto& c : dicts) {
+      if (c->name == name) return c.get();
+   }
+   return nullptr;
+}
+
+// ---- sequence column ---------------------------------------------------------------------------
+namespace {
+
+struct ChunkContainers {
+   std::vector<uint32_t> key;  // position << 8 | symbol, sorted
+   ArenaBuilder arenas;
+};
+
+constexpr uint32_t kBoundaryCount = 4;
+
+bool inRuns(const uint32_t* runs, uint64_t n, uint32_t p) {
+   // runs sorted, disjoint: start | end << 16
+   uint64_t lo = 0, hi = n;
+   while (lo < hi) {
+      const uint64_t mid = (lo + hi) / 2;
+      if ((runs[mid] & 0xFFFFu) <= p) {
+         lo = mid + 1;
+      } else {
+         hi = mid;
+      }
+   }
+   return lo > 0 && p < (runs[lo - 1] >> 16);
+}
+
+}  // namespace
+
+void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const uint32_t* cov_start,
+                         const uint32_t* cov_end, const uint64_t* mut_ptr, const uint32_t* mut_pos,
+                         const uint8_t* mut_sym, const uint64_t* nrun_ptr, const uint32_t* nrun_start,
+                         const uint32_t* nrun_end) {
+   const uint32_t L = col.genome_length;
+   const Alphabet& A = *col.abc;
+   const uint32_t nsym = A.count;
+   const uint64_t n = t.num_rows;
+   if (L == 0 || L > 65535) fail(RDB_EINVAL, "genome length must be in [1, 65535]");
+   for (uint32_t p = 0; p < L; ++p) {
+      if (col.reference[p] >= nsym) fail(RDB_EINVAL, "invalid reference symbol");
+   }
+
+   // normalized internal missing runs: clipped to [start, end) as insertCoverage does
+   // (horizontal_coverage_index.cpp:34-37), sorted and merged
+   std::vector<uint64_t> nptr(n + 1, 0);
+   std::vector<std::vector<uint32_t>> row_runs_tmp(t.n_chunks);
+   parallelFor(t.n_chunks, [&](size_t c) {
+      auto& out = row_runs_tmp[c];
+      std::vector<std::pair<uint32_t, uint32_t>> runs;
+      for (uint64_t i = t.chunk_offset[c]; i < t.chunk_offset[c + 1]; ++i) {
+         runs.clear();
+         if (!is_null[i]) {
+            if (cov_start[i] > cov_end[i] || cov_end[i] > L) fail(RDB_EINVAL, "invalid coverage range");
+            for (uint64_t k = mut_ptr[i]; k < mut_ptr[i + 1]; ++k) {
+               if (mut_pos[k] >= L) fail(RDB_EINVAL, "mutation position out of range");
+               if (mut_sym[k] >= nsym || mut_sym[k] == A.missing) fail(RDB_EINVAL, "invalid mutation symbol");
+               if (k > mut_ptr[i] && mut_pos[k] <= mut_pos[k - 1]) {
+                  fail(RDB_EINVAL, "mutation positions of a row must be strictly increasing");
+               }
+            }
+            for (uint64_t k = nrun_ptr[i]; k < nrun_ptr[i + 1]; ++k) {
+               const uint32_t a = std::max(nrun_start[k], cov_start[i]);
+               const uint32_t b = std::min(nrun_end[k], cov_end[i]);
+               if (a < b) runs.emplace_back(a, b);
+            }
+            std::sort(runs.begin(), runs.end());
+         }
+         uint32_t count = 0;
+         for (size_t k = 0; k < runs.size(); ++k) {
+            if (count > 0) {
+               uint32_t& last = out.back();
+               const uint32_t le = last >> 16;
+               if (runs[k].first <= le) {
+                  const uint32_t ne = std::max(le, runs[k].second);
+                  last = (last & 0xFFFFu) | (ne << 16);
+                  continue;
+               }
+            }
+            out.push_back(runs[k].first | (runs[k].second << 16));
+            ++count;
+         }
+         nptr[i + 1] = count;
+      }
+   });
+   for (uint64_t i = 0; i < n; ++i) nptr[i + 1] += nptr[i];
+   std::vector<uint32_t> runs_all(nptr[n]);
+   {
+      size_t o = 0;
+      for (auto& v : row_runs_tmp) {
+         std::copy(v.begin(), v.end(), runs_all.begin() + static_cast<ptrdiff_t>(o));
+         o += v.size();
+         std::vector<uint32_t>().swap(v);
+      }
+   }
+
+   // --- local reference adaptation (sequence_column.cpp:157-212) ---------------------------------
+   std::vector<uint32_t> sym_counts(static_cast<size_t>(L) * nsym, 0);
+   std::vector<int64_t> changes(L + 1, 0);
+   for (uint64_t i = 0; i < n; ++i) {
+      if (is_null[i]) continue;
+      changes[cov_start[i]] += 1;
+      changes[cov_end[i]] -= 1;
+      for (uint64_t k = nptr[i]; k < nptr[i + 1]; ++k) {
+         changes[runs_all[k] & 0xFFFFu] -= 1;
+         changes[runs_all[k] >> 16] += 1;
+      }
+      for (uint64_t k = mut_ptr[i]; k < mut_ptr[i + 1]; ++k) sym_counts[static_cast<size_t>(mut_pos[k]) * nsym + mut_sym[k]] += 1;
+   }
+   col.local_ref = col.reference;
+   std::vector<uint8_t> flip_to(L, 0xFF);
+   std::vector<uint32_t> flipped;
+   int64_t coverage = 0;
+   std::vector<uint32_t> counts(nsym);
+   for (uint32_t p = 0; p < L; ++p) {
+      coverage += changes[p];
+      const uint8_t current = col.reference[p];
+      for (uint32_t s = 0; s < nsym; ++s) counts[s] = sym_counts[static_cast<size_t>(p) * nsym + s];
+      uint32_t ref_count = static_cast<uint32_t>(coverage);
+      for (uint32_t s = 0; s < nsym; ++s) {
+         if (s != current) ref_count -= counts[s];
+      }
+      counts[current] = ref_count;
+      uint8_t best = current;
+      uint32_t best_count = counts[current];
+      for (uint32_t s = 0; s < nsym; ++s) {
+         if (s == current) continue;
+         if (counts[s] > best_count) {
+            best = static_cast<uint8_t>(s);
+            best_count = counts[s];
+         }
+      }
+      if (best != current) {
+         flip_to[p] = best;
+         flipped.push_back(p);
+         col.local_ref[p] = best;
+      }
+   }
+   col.n_flipped = static_cast<uint32_t>(flipped.size());
+
+   // --- per-chunk containers ----------------------------------------------------------------------
+   std::vector<ChunkContainers> chunks(t.n_chunks);
+   parallelFor(t.n_chunks, [&](size_t c) {
+      std::vector<uint64_t> entries;
+      const uint64_t off = t.chunk_offset[c];
+      const uint32_t size = t.chunk_sizes[c];
+      auto push = [&](uint32_t p, uint32_t sym, uint32_t r) {
+         entries.push_back((static_cast<uint64_t>(p) << 24) | (static_cast<uint64_t>(sym) << 16) | r);
+      };
+      for (uint32_t r = 0; r < size; ++r) {
+         const uint64_t i = off + r;
+         if (is_null[i]) {
+            push(0, nsym + BND_COV_END, r);  // no covered range: uncovered everywhere
+            continue;
+         }
+         const uint32_t s0 = cov_start[i], e0 = cov_end[i];
+         for (uint64_t k = mut_ptr[i]; k < mut_ptr[i + 1]; ++k) {
+            const uint32_t p = mut_pos[k];
+            if (flip_to[p] != 0xFF && mut_sym[k] == flip_to[p]) continue;
+            push(p, mut_sym[k], r);
+         }
+         for (uint32_t p : flipped) {
+            if (p < s0 || p >= e0) continue;
+            if (inRuns(runs_all.data() + nptr[i], nptr[i + 1] - nptr[i], p)) continue;
+            const uint32_t* mb = mut_pos + mut_ptr[i];
+            const uint32_t* me = mut_pos + mut_ptr[i + 1];
+            const uint32_t* it = std::lower_bound(mb, me, p);
+            if (it != me && *it == p) continue;
+            push(p, col.reference[p], r);
+         }
+         if (s0 > 0) push(s0, nsym + BND_COV_START, r);
+         if (e0 < L) push(e0, nsym + BND_COV_END, r);
+         for (uint64_t k = nptr[i]; k < nptr[i + 1]; ++k) {
+            const uint32_t a = runs_all[k] & 0xFFFFu, b = runs_all[k] >> 16;
+            push(a, nsym + BND_N_START, r);
+            if (b < L) push(b, nsym + BND_N_END, r);
+         }
+      }
+      std::sort(entries.begin(), entries.end());
+      auto& out = chunks[c];
+      std::vector<uint16_t> rows;
+      size_t k = 0;
+      while (k < entries.size()) {
+         const uint64_t key = entries[k] >> 16;
+         rows.clear();
+         while (k < entries.size() && (entries[k] >> 16) == key) {
+            rows.push_back(static_cast<uint16_t>(entries[k] & 0xFFFF));
+            ++k;
+         }
+         out.key.push_back(static_cast<uint32_t>(key));
+         out.arenas.add(rows.data(), static_cast<uint32_t>(rows.size()), 0);
+      }
+   });
+
+   // --- slots: distinct (position, symbol) pairs, sorted ------------------------------------------
+   const uint32_t total_syms = nsym + kBoundaryCount;
+   if (total_syms > 32) fail(RDB_EINVAL, "alphabet too large");
+   std::vector<uint32_t> sym_mask(L, 0);
+   std::vector<uint64_t> pos_bytes(L, 0);
+   col.chunk_index_bytes.assign(t.n_chunks, 0);
+   for (uint32_t c = 0; c < t.n_chunks; ++c) {
+      const auto& ch = chunks[c];
+      for (size_t k = 0; k < ch.key.size(); ++k) {
+         const uint32_t p = ch.key[k] >> 8, s = ch.key[k] & 0xFF;
+         sym_mask[p] |= 1u << s;
+         const uint64_t b = containerPayloadBytes(ch.arenas.desc[k]) + sizeof(Desc);
+         pos_bytes[p] += b;
+         if (s < nsym) {
+            col.chunk_index_bytes[c] += b;
+            col.n_containers += 1;
+            col.payload_bytes += containerPayloadBytes(ch.arenas.desc[k]);
+         }
+      }
+   }
+   col.pos_slot_ptr.assign(L + 1, 0);
+   for (uint32_t p = 0; p < L; ++p) col.pos_slot_ptr[p + 1] = col.pos_slot_ptr[p] + std::popcount(sym_mask[p]);
+   const uint32_t n_slots = col.pos_slot_ptr[L];
+   col.slot_info.resize(n_slots);
+   for (uint32_t p = 0; p < L; ++p) {
+      uint32_t m = sym_mask[p], s = col.pos_slot_ptr[p];
+      while (m) {
+         const uint32_t sym = static_cast<uint32_t>(std::countr_zero(m));
+         m &= m - 1;
+         col.slot_info[s++] = p | (sym << 16);
+      }
+   }
+
+   // --- position blocks: <= 4096 slots and about equal bytes --------------------------------------
+   const uint64_t total_bytes = std::accumulate(pos_bytes.begin(), pos_bytes.end(), uint64_t{0});
+   const uint64_t target = std::max<uint64_t>(total_bytes / 192, 1);
+   col.block_pos_start.assign(1, 0);
+   col.block_of_pos.assign(L, 0);
+   {
+      uint64_t bytes = 0;
+      uint32_t slots = 0;
+      for (uint32_t p = 0; p < L; ++p) {
+         const uint32_t ps = col.pos_slot_ptr[p + 1] - col.pos_slot_ptr[p];
+         if (p > col.block_pos_start.back() && (slots + ps > kMaxBlockSlots || bytes + pos_bytes[p] > target)) {
+            col.block_pos_start.push_back(p);
+            bytes = 0;
+            slots = 0;
+         }
+         bytes += pos_bytes[p];
+         slots += ps;
+         col.block_of_pos[p] = static_cast<uint32_t>(col.block_pos_start.size() - 1);
+      }
+      col.block_pos_start.push_back(L);
+   }
+   const uint32_t n_blocks = static_cast<uint32_t>(col.block_pos_start.size() - 1);
+   col.block_slot_start.resize(n_blocks + 1);
+   for (uint32_t b = 0; b <= n_blocks; ++b) col.block_slot_start[b] = col.pos_slot_ptr[col.block_pos_start[b]];
+
+   // --- tiles -------------------------------------------------------------------------------------
+   std::vector<size_t> arr_base(t.n_chunks + 1, 0), run_base(t.n_chunks + 1, 0), bit_base(t.n_chunks + 1, 0);
+   for (uint32_t c = 0; c < t.n_chunks; ++c) {
+      arr_base[c + 1] = arr_base[c] + chunks[c].arenas.arr.size();
+      run_base[c + 1] = run_base[c] + chunks[c].arenas.runs.size();
+      bit_base[c + 1] = bit_base[c] + chunks[c].arenas.bits.size() / kChunkWords;
+   }
+   if (arr_base[t.n_chunks] > 0xFFFFFFFFULL || run_base[t.n_chunks] > 0xFFFFFFFFULL || bit_base[t.n_chunks] > 0xFFFFFFFFULL) {
+      fail(RDB_ENOMEM, "container arena exceeds 32-bit offsets");
+   }
+   size_t n_desc = 0;
+   for (const auto& ch : chunks) n_desc += ch.key.size();
+   std::vector<Desc> desc;
+   desc.reserve(n_desc);
+   std::vector<uint32_t> tile_ptr(static_cast<size_t>(n_blocks) * t.n_chunks + 1, 0);
+   std::vector<uint32_t> tile_heavy(static_cast<size_t>(n_blocks) * t.n_chunks, 0);
+   std::vector<size_t> cursor(t.n_chunks, 0);
+   auto is_light = [](const Desc& d) {
+      const uint32_t type = descType(d.w0), card = descCard(d.w0);
+      return type == CT_INLINE || (type == CT_ARRAY && card <= 32) || (type == CT_RUN && card <= 4);
+   };
+   std::vector<Desc> heavy;
+   for (uint32_t b = 0; b < n_blocks; ++b) {
+      const uint32_t pend = col.block_pos_start[b + 1];
+      const uint32_t sbase = col.block_slot_start[b];
+      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+         const size_t tile = static_cast<size_t>(b) * t.n_chunks + c;
+         tile_ptr[tile] = static_cast<uint32_t>(desc.size());
+         heavy.clear();
+         auto& ch = chunks[c];
+         size_t& k = cursor[c];
+         while (k < ch.key.size() && (ch.key[k] >> 8) < pend) {
+            const uint32_t p = ch.key[k] >> 8, s = ch.key[k] & 0xFF;
+            const uint32_t slot = col.pos_slot_ptr[p] + std::popcount(sym_mask[p] & ((1u << s) - 1u)) - sbase;
+            Desc d = ch.arenas.desc[k];
+            const uint32_t type = descType(d.w0);
+            d.w0 = makeW0(slot, type, descCard(d.w0));
+            if (type == CT_ARRAY) d.w1 += static_cast<uint32_t>(arr_base[c]);
+            if (type == CT_RUN) d.w1 += static_cast<uint32_t>(run_base[c]);
+            if (type == CT_BITSET) d.w1 += static_cast<uint32_t>(bit_base[c]);
+            if (is_light(d)) {
+               desc.push_back(d);
+            } else {
+               heavy.push_back(d);
+            }
+            ++k;
+         }
+         tile_heavy[tile] = static_cast<uint32_t>(desc.size());
+         desc.insert(desc.end(), heavy.begin(), heavy.end());
+      }
+   }
+   tile_ptr.back() = static_cast<uint32_t>(desc.size());
+   if (desc.size() > 0xFFFFFFFFULL) fail(RDB_ENOMEM, "too many containers");
+
+   std::vector<uint16_t> arr_all(arr_base[t.n_chunks]);
+   std::vector<uint32_t> run_all(run_base[t.n_chunks]);
+   std::vector<uint64_t> bit_all(bit_base[t.n_chunks] * kChunkWords);
+   parallelFor(t.n_chunks, [&](size_t c) {
+      auto& a = chunks[c].arenas;
+      std::copy(a.arr.begin(), a.arr.end(), arr_all.begin() + static_cast<ptrdiff_t>(arr_base[c]));
+      std::copy(a.runs.begin(), a.runs.end(), run_all.begin() + static_cast<ptrdiff_t>(run_base[c]));
+      std::copy(a.bits.begin(), a.bits.end(), bit_all.begin() + static_cast<ptrdiff_t>(bit_base[c] * kChunkWords));
+      ArenaBuilder().desc.swap(a.desc);
+      std::vector<uint16_t>().swap(a.arr);
+      std::vector<uint32_t>().swap(a.runs);
+      std::vector<uint64_t>().swap(a.bits);
+   });
+   chunks.clear();
+   col.desc_bytes = desc.size() * sizeof(Desc);
+
+   // --- row-level arrays (padded to 2^16 rows per chunk) ------------------------------------------
+   const size_t padded = static_cast<size_t>(t.n_chunks) << 16;
+   std::vector<uint32_t> cov(padded, 0), pptr(padded + 1, 0);
+   std::vector<uint64_t> null_bits(static_cast<size_t>(t.n_chunks) * kChunkWords, 0);
+   std::vector<uint32_t> null_rows;
+   col.chunk_nruns.assign(t.n_chunks, 0);
+   for (uint32_t c = 0; c < t.n_chunks; ++c) {
+      for (uint32_t r = 0; r < t.chunk_sizes[c]; ++r) {
+         const uint64_t i = t.chunk_offset[c] + r;
+         const size_t g = (static_cast<size_t>(c) << 16) | r;
+         if (is_null[i]) {
+            null_bits[g >> 6] |= 1ULL << (g & 63);
+            null_rows.push_back(static_cast<uint32_t>(g));
+         } else {
+            cov[g] = cov_start[i] | (cov_end[i] << 16);
+         }
+         col.chunk_nruns[c] += nptr[i + 1] - nptr[i];
+      }
+   }
+   {
+      // pptr over padded rows
+      uint32_t acc = 0;
+      if (nptr[n] > 0xFFFFFFFFULL) fail(RDB_ENOMEM, "too many missing runs");
+      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+         for (uint32_t r = 0; r < kChunkRows; ++r) {
+            const size_t g = (static_cast<size_t>(c) << 16) | r;
+            pptr[g] = acc;
+            if (r < t.chunk_sizes[c]) {
+               const uint64_t i = t.chunk_offset[c] + r;
+               acc += static_cast<uint32_t>(nptr[i + 1] - nptr[i]);
+            }
+         }
+      }
+      pptr[padded] = acc;
+   }
+   col.has_nulls = !null_rows.empty();
+   col.null_bitmap_id = t.addStoreBitmap(null_rows);
+
+   // --- checks ------------------------------------------------------------------------------------
+   validateDescriptors(desc.data(), desc.size(), arr_all.size(), run_all.size(), bit_all.size() / kChunkWords, kMaxBlockSlots);
+   for (uint32_t b = 0; b < n_blocks; ++b) {
+      const uint32_t ns = col.block_slot_start[b + 1] - col.block_slot_start[b];
+      if (ns > kMaxBlockSlots) fail(RDB_EDEVICE, "internal: block with too many slots");
+      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+         const size_t tile = static_cast<size_t>(b) * t.n_chunks + c;
+         if (!(tile_ptr[tile] <= tile_heavy[tile] && tile_heavy[tile] <= tile_ptr[tile + 1])) {
+            fail(RDB_EDEVICE, "internal: tile pointers out of order");
+         }
+         for (uint32_t k = tile_ptr[tile]; k < tile_ptr[tile + 1]; ++k) {
+            if (descSlot(desc[k].w0) >= ns) fail(RDB_EDEVICE, "internal: descriptor slot outside its block");
+         }
+      }
+   }
+   for (size_t k = 0; k < runs_all.size(); ++k) {
+      if ((runs_all[k] & 0xFFFFu) >= (runs_all[k] >> 16) || (runs_all[k] >> 16) > L) fail(RDB_EDEVICE, "internal: bad missing run");
+   }
+
+   // --- upload ------------------------------------------------------------------------------------
+   auto up = [&](const void* p, size_t bytes) -> const void* {
+      col.buffers.emplace_back(p, bytes);
+      return col.buffers.back().as<void>();
+   };
+   SeqColumnView& v = col.view;
+   v.genome_length = L;
+   v.n_symbols = nsym;
+   v.n_blocks = n_blocks;
+   v.n_slots = n_slots;
+   v.n_valid = static_cast<uint32_t>(A.valid.size());
+   v.valid_mask = 0;
+   for (uint8_t s : A.valid) v.valid_mask |= 1u << s;
+   v.vert.desc = static_cast<const Desc*>(up(desc.data(), desc.size() * sizeof(Desc)));
+   v.vert.arr = static_cast<const uint16_t*>(up(arr_all.data(), arr_all.size() * 2));
+   v.vert.runs = static_cast<const uint32_t*>(up(run_all.data(), run_all.size() * 4));
+   v.vert.bits = static_cast<const uint64_t*>(up(bit_all.data(), bit_all.size() * 8));
+   v.block_pos_start = static_cast<const uint32_t*>(up(col.block_pos_start.data(), col.block_pos_start.size() * 4));
+   v.block_slot_start = static_cast<const uint32_t*>(up(col.block_slot_start.data(), col.block_slot_start.size() * 4));
+   v.tile_ptr = static_cast<const uint32_t*>(up(tile_ptr.data(), tile_ptr.size() * 4));
+   v.tile_heavy = static_cast<const uint32_t*>(up(tile_heavy.data(), tile_heavy.size() * 4));
+   v.slot_info = static_cast<const uint32_t*>(up(col.slot_info.data(), col.slot_info.size() * 4));
+   v.pos_slot_ptr = static_cast<const uint32_t*>(up(col.pos_slot_ptr.data(), col.pos_slot_ptr.size() * 4));
+   v.local_ref = static_cast<const uint8_t*>(up(col.local_ref.data(), L));
+   v.reference = static_cast<const uint8_t*>(up(col.reference.data(), L));
+   v.cov = static_cast<const uint32_t*>(up(cov.data(), cov.size() * 4));
+   v.nrun_ptr = static_cast<const uint32_t*>(up(pptr.data(), pptr.size() * 4));
+   v.nruns = static_cast<const uint32_t*>(up(runs_all.data(), runs_all.size() * 4));
+   v.null_bits = static_cast<const uint64_t*>(up(null_bits.data(), null_bits.size() * 8));
+}
+
+// ---- metadata columns --------------------------------------------------------------------------
+void buildIntColumn(Table& t, IntColumn& col, const int32_t* values, const uint8_t* is_null) {
+   const size_t padded = static_cast<size_t>(t.n_chunks) << 16;
+   std::vector<int32_t> vals(padded, 0);
+   std::vector<uint64_t> null_bits(static_cast<size_t>(t.n_chunks) * kChunkWords, 0);
+   std::vector<uint32_t> null_rows;
+   col.values.assign(values, values + t.num_rows);
+   bool have_last = false;
+   int32_t last = 0;
+   for (uint32_t c = 0; c < t.n_chunks; ++c) {
+      for (uint32_t r = 0; r < t.chunk_sizes[c]; ++r) {
+         const uint64_t i = t.chunk_offset[c] + r;
+         const size_t g = (static_cast<size_t>(c) << 16) | r;
+         if (is_null[i]) {
+            null_bits[g >> 6] |= 1ULL << (g & 63);
+            null_rows.push_back(static_cast<uint32_t>(g));
+            col.values[i] = 0;
+            col.is_sorted = false;  // date32_column.cpp:27-32
+         } else {
+            vals[g] = values[i];
+            if (have_last && values[i] < last) col.is_sorted = false;
+            last = values[i];
+            have_last = true;
+         }
+      }
+   }
+   col.null_bitmap_id = t.addStoreBitmap(null_rows);
+   col.buffers.emplace_back(vals.data(), vals.size() * 4);
+   col.view.values = col.buffers.back().as<int32_t>();
+   col.buffers.emplace_back(null_bits.data(), null_bits.size() * 8);
+   col.view.null_bits = col.buffers.back().as<uint64_t>();
+}
+
+void buildDictColumn(Table& t, DictColumn& col, const uint32_t* ids, const uint8_t* is_null,
+                     const int32_t* lineage_parent) {
+   const uint32_t nv = static_cast<uint32_t>(col.dictionary.size());
+   std::vector<std::vector<uint32_t>> rows(nv);
+   std::vector<uint32_t> null_rows;
+   const size_t padded = static_cast<size_t>(t.n_chunks) << 16;
+   std::vector<uint32_t> labels(padded, kInvalidId);
+   for (uint32_t c = 0; c < t.n_chunks; ++c) {
+      for (uint32_t r = 0; r < t.chunk_sizes[c]; ++r) {
+         const uint64_t i = t.chunk_offset[c] + r;
+         const uint32_t g = (c << 16) | r;
+         if (is_null[i]) {
+            null_rows.push_back(g);
+            labels[g] = nv;
+         } else {
+            if (ids[i] >= nv) fail(RDB_EINVAL, "dictionary id out of range");
+            rows[ids[i]].push_back(g);
+            labels[g] = ids[i];
+         }
+      }
+   }
+   col.value_bitmap.assign(nv, kInvalidId);
+   for (uint32_t v = 0; v < nv; ++v) {
+      if (!rows[v].empty()) col.value_bitmap[v] = t.addStoreBitmap(rows[v]);
+   }
+   col.has_nulls = !null_rows.empty();
+   col.null_bitmap_id = t.addStoreBitmap(null_rows);
+   if (lineage_parent) {
+      // lineage_index.h:17-42: a value's rows together with those of all its descendants
+      col.lineage = true;
+      col.sublineage_bitmap.assign(nv, kInvalidId);
+      std::vector<std::vector<uint32_t>> children(nv);
+      for (uint32_t v = 0; v < nv; ++v) {
+         const int32_t p = lineage_parent[v];
+         if (p >= 0 && static_cast<uint32_t>(p) < nv && static_cast<uint32_t>(p) != v) children[p].push_back(v);
+      }
+      for (uint32_t v = 0; v < nv; ++v) {
+         std::vector<uint32_t> acc;
+         std::vector<uint32_t> stack{v};
+         std::vector<uint8_t> seen(nv, 0);
+         while (!stack.empty()) {
+            const uint32_t w = stack.back();
+            stack.pop_back();
+            if (seen[w]) continue;
+            seen[w] = 1;
+            acc.insert(acc.end(), rows[w].begin(), rows[w].end());
+            for (uint32_t ch : children[w]) stack.push_back(ch);
+         }
+         if (!acc.empty()) {
+            std::sort(acc.begin(), acc.end());
+            col.sublineage_bitmap[v] = t.addStoreBitmap(acc);
+         }
+      }
+   }
+   std::vector<uint32_t> order(nv);
+   std::iota(order.begin(), order.end(), 0);
+   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return col.dictionary[a] < col.dictionary[b]; });
+   col.rank.assign(nv, 0);
+   for (uint32_t k = 0; k < nv; ++k) col.rank[order[k]] = k;
+   col.labels = DeviceBuffer(labels.data(), labels.size() * 4);
+}
+
+// ---- finalize: bitmap store + views + scratch ----------------------------------------------------
+void finalizeTable(Table& t) {
+   // bitmap store
+   ArenaBuilder& sb = t.store_builder;
+   sb = ArenaBuilder();
+   for (const auto& bm : t.store_rows) {
+      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+         const auto& r = bm[c];
+         sb.add(r.data(), static_cast<uint32_t>(r.size()), 0);
+      }
+   }
+   if (sb.desc.size() != t.store_rows.size() * t.n_chunks) fail(RDB_EDEVICE, "internal: bitmap store size mismatch");
+   validateDescriptors(sb.desc.data(), sb.desc.size(), sb.arr.size(), sb.runs.size(), sb.bits.size() / kChunkWords, 1);
+   t.store_buffers.clear();
+   auto up = [&](const void* p, size_t bytes) -> void* {
+      t.store_buffers.emplace_back(p, bytes);
+      return t.store_buffers.back().as<void>();
+   };
+   t.view.store.desc = static_cast<const Desc*>(up(sb.desc.data(), sb.desc.size() * sizeof(Desc)));
+   t.view.store.arr = static_cast<const uint16_t*>(up(sb.arr.data(), sb.arr.size() * 2));
+   t.view.store.runs = static_cast<const uint32_t*>(up(sb.runs.data(), sb.runs.size() * 4));
+   t.view.store.bits = static_cast<const uint64_t*>(up(sb.bits.data(), sb.bits.size() * 8));
+   sb = ArenaBuilder();
+   std::vector<SeqColumnView> sv;
+   for (const auto& c : t.seq) sv.push_back(c->view);
+   std::vector<IntColumnView> iv;
+   for (const auto& c : t.ints) iv.push_back(c->view);
+   t.chunk_sizes_dev = DeviceBuffer(t.chunk_sizes.data(), t.chunk_sizes.size() * 4);
+   t.seq_views_dev = DeviceBuffer(sv.data(), sv.size() * sizeof(SeqColumnView));
+   t.int_views_dev = DeviceBuffer(iv.data(), iv.size() * sizeof(IntColumnView));
+   t.view.n_chunks = t.n_chunks;
+   t.view.chunk_sizes = t.chunk_sizes_dev.as<uint32_t>();
+   t.view.seq = t.seq_views_dev.as<SeqColumnView>();
+   t.view.ints = t.int_views_dev.as<IntColumnView>();
+   uint32_t max_slots = 1, max_L = 1;
+   for (const auto& c : t.seq) {
+      max_slots = std::max(max_slots, c->view.n_slots);
+      max_L = std::max(max_L, c->genome_length);
+   }
+   t.slot_counts = DeviceBuffer(static_cast<size_t>(max_slots) * 4);
+   t.totals = DeviceBuffer(static_cast<size_t>((max_L + 1023) / 1024) * 3 * sizeof(int64_t));
+   t.mut_rows = DeviceBuffer(static_cast<size_t>(max_L) * 32 * sizeof(MutationRowDev));
+   t.mut_n = DeviceBuffer(64);
+   t.finalized = true;
+}
+
+// ---- device bitmaps ----------------------------------------------------------------------------
+DeviceBitmap::DeviceBitmap(Table* t) : table(t) {
+   const size_t n_words = static_cast<size_t>(t->n_chunks) * kChunkWords;
+   bytes = n_words * 8 + static_cast<size_t>(t->n_chunks) * 4 + 64;
+   auto* base = static_cast<uint8_t*>(t->ctx->pool.get(bytes));
+   words = reinterpret_cast<uint64_t*>(base);
+   card_dev = reinterpret_cast<unsigned long long*>(base + n_words * 8);
+   chunk_cards = reinterpret_cast<uint32_t*>(base + n_words * 8 + 64);
+}
+
+DeviceBitmap::~DeviceBitmap() {
+   if (words && table) table->ctx->pool.put(words, bytes);
+}
+
+DeviceBitmap::DeviceBitmap(DeviceBitmap&& o) noexcept { *this = std::move(o); }
+
+DeviceBitmap& DeviceBitmap::operator=(DeviceBitmap&& o) noexcept {
+   if (this != &o) {
+      if (words && table) table->ctx->pool.put(words, bytes);
+      table = o.table;
+      words = o.words;
+      chunk_cards = o.chunk_cards;
+      card_dev = o.card_dev;
+      bytes = o.bytes;
+      card = o.card;
+      card_known = o.card_known;
+      o.words = nullptr;
+      o.table = nullptr;
+   }
+   return *this;
+}
+
+uint64_t DeviceBitmap::cardinality() {
+   if (!card_known) {
+      unsigned long long v = 0;
+      hipCheck(hipMemcpyAsync(&v, card_dev, sizeof(v), hipMemcpyDeviceToHost, table->ctx->stream), "copy cardinality");
+      hipCheck(hipStreamSynchronize(table->ctx->stream), "synchronize");
+      card = v;
+      card_known = true;
+   }
+   return card;
+}
+
+std::vector<uint32_t> DeviceBitmap::rows() {
+   const size_t n_words = static_cast<size_t>(table->n_chunks) * kChunkWords;
+   std::vector<uint64_t> host(n_words);
+   hipCheck(hipMemcpyAsync(host.data(), words, n_words * 8, hipMemcpyDeviceToHost, table->ctx->stream), "copy bitmap");
+   hipCheck(hipStreamSynchronize(table->ctx->stream), "synchronize");
+   std::vector<uint32_t> out;
+   for (size_t w = 0; w < n_words; ++w) {
+      uint64_t x = host[w];
+      while (x) {
+         out.push_back(static_cast<uint32_t>(w * 64 + static_cast<size_t>(std::countr_zero(x))));
+         x &= x - 1;
+      }
+   }
+   return out;
+}
+
+}  // namespace rdb

@@ -1,0 +1,287 @@
+"""ctypes binding of the engine's C ABI (include/rdb.h).
+
+`Engine` owns one GPU context; `DeviceTable` uploads a `TableSpec` and answers filter, count,
+Mutations and Aggregated queries on the device.  There is no CPU fallback: without the built
+library or a GPU these calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+from .table import TableSpec, alphabet_chars
+
+RDB_OK = 0
+_STATUS = {1: "RDB_EINVAL", 2: "RDB_ENOMEM", 3: "RDB_EDEVICE", 4: "RDB_EQUERY", 5: "RDB_ECOMPILE"}
+
+_vp, _u32, _u64, _i32, _f64, _cp, _int = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32,
+                                          ctypes.c_double, ctypes.c_char_p, ctypes.c_int)
+
+# (name, restype, argtypes) of every entry point declared in include/rdb.h
+SIGNATURES = [
+    ("rdb_open", _int, [_int, ctypes.POINTER(_vp)]),
+    ("rdb_close", None, [_vp]),
+    ("rdb_last_error", _cp, [_vp]),
+    ("rdb_table_create", _int, [_vp, _vp, _u32, ctypes.POINTER(_vp)]),
+    ("rdb_table_free", None, [_vp]),
+    ("rdb_table_add_sequence_column", _int, [_vp, _cp, _int, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("rdb_table_add_int_column", _int, [_vp, _cp, _int, _vp, _vp]),
+    ("rdb_table_add_dict_column", _int, [_vp, _cp, _vp, _vp, _u32, _cp, _vp, _vp]),
+    ("rdb_table_add_bitmap", _int, [_vp, _vp, _u64, _vp]),
+    ("rdb_table_finalize", _int, [_vp]),
+    ("rdb_table_local_reference", _int, [_vp, _cp, _vp]),
+    ("rdb_table_stats", _int, [_vp, _cp, _vp, _vp, _vp]),
+    ("rdb_filter", _int, [_vp, _cp, ctypes.POINTER(_vp), _vp]),
+    ("rdb_bitmap_cardinality", _int, [_vp, _vp]),
+    ("rdb_bitmap_rows", _int, [_vp, _vp, _u64, _vp]),
+    ("rdb_bitmap_free", None, [_vp]),
+    ("rdb_count", _int, [_vp, _cp, _vp]),
+    ("rdb_mutation_counts", _int, [_vp, _vp, _cp, _vp]),
+    ("rdb_mutations", _int, [_vp, _vp, _cp, _f64, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
+    ("rdb_aggregate", _int, [_vp, _vp, _cp, _vp, _vp, _u64, _vp]),
+    ("rdb_diff_sequence", _int, [_int, _cp, _u32, _cp, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _u64, _vp]),
+]
+
+_bound = False
+
+
+def lib() -> ctypes.CDLL:
+    global _bound
+    L = _native.load(_native.ENGINE_LIB)
+    if not _bound:
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _bound = True
+    return L
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{_STATUS.get(code, code)}: {message}")
+        self.code = code
+        self.status = _STATUS.get(code, str(code))
+
+
+def _check(rc: int, ctx) -> None:
+    if rc != RDB_OK:
+        raise EngineError(rc, lib().rdb_last_error(ctx).decode(errors="replace"))
+
+
+def diff_sequence(sequence: str, reference: str, alphabet: int = 0):
+    """The engine's restatement of extractCoverageAndMutationsFromSequence (offset 0)."""
+    n = len(sequence)
+    start, end = _u32(), _u32()
+    miss = np.zeros(n + 1, np.uint32)
+    mpos = np.zeros(n + 1, np.uint32)
+    msym = np.zeros(n + 1, np.uint8)
+    nm, nmut = _u64(), _u64()
+    rc = lib().rdb_diff_sequence(alphabet, sequence.encode(), n, reference.encode(), ctypes.byref(start),
+                                 ctypes.byref(end), _p(miss), n + 1, ctypes.byref(nm), _p(mpos), _p(msym), n + 1,
+                                 ctypes.byref(nmut))
+    _check(rc, None)
+    return (start.value, end.value, miss[: nm.value].tolist(),
+            list(zip(mpos[: nmut.value].tolist(), msym[: nmut.value].tolist())))
+
+
+class Engine:
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(lib().rdb_open(device, ctypes.byref(h)), None)
+        self._h = h
+        self._tables = []
+
+    @property
+    def handle(self):
+        return self._h
+
+    def table(self, spec: TableSpec) -> "DeviceTable":
+        t = DeviceTable(self, spec)
+        self._tables.append(t)
+        return t
+
+    def close(self) -> None:
+        for t in self._tables:
+            t.close()
+        self._tables = []
+        if self._h:
+            lib().rdb_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Bitmap:
+    """A filter result resident on the device (the engine's CopyOnWriteBitmap)."""
+
+    def __init__(self, table: "DeviceTable", handle, cardinality: int):
+        self.table = table
+        self._h = handle
+        self.cardinality = cardinality
+
+    @property
+    def handle(self):
+        return self._h
+
+    def rows(self) -> np.ndarray:
+        out = np.zeros(max(self.cardinality, 1), np.uint32)
+        n = _u64()
+        _check(lib().rdb_bitmap_rows(self._h, _p(out), len(out), ctypes.byref(n)), self.table.engine.handle)
+        return out[: n.value]
+
+    def free(self) -> None:
+        if self._h:
+            lib().rdb_bitmap_free(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+
+class DeviceTable:
+    def __init__(self, engine: Engine, spec: TableSpec):
+        self.engine = engine
+        L = lib()
+        ctx = engine.handle
+        sizes = np.ascontiguousarray(spec.chunk_sizes, dtype=np.uint32)
+        h = _vp()
+        _check(L.rdb_table_create(ctx, _p(sizes), len(sizes), ctypes.byref(h)), ctx)
+        self._h = h
+        self.seq_lengths = {}
+        self.alphabets = {}
+        self.dict_names = {}
+        try:
+            for c in spec.seq_columns:
+                arrs = [np.ascontiguousarray(x) for x in (
+                    np.asarray(c.reference, np.uint8), np.asarray(c.is_null, np.uint8), np.asarray(c.cov_start, np.uint32),
+                    np.asarray(c.cov_end, np.uint32), np.asarray(c.mut_ptr, np.uint64), np.asarray(c.mut_pos, np.uint32),
+                    np.asarray(c.mut_sym, np.uint8), np.asarray(c.nrun_ptr, np.uint64), np.asarray(c.nrun_start, np.uint32),
+                    np.asarray(c.nrun_end, np.uint32))]
+                _check(L.rdb_table_add_sequence_column(h, c.name.encode(), c.alphabet, _p(arrs[0]), len(c.reference),
+                                                       *[_p(a) for a in arrs[1:]]), ctx)
+                self.seq_lengths[c.name] = len(c.reference)
+                self.alphabets[c.name] = c.alphabet
+            for c in spec.int_columns:
+                v = np.ascontiguousarray(c.values, dtype=np.int32)
+                nl = np.ascontiguousarray(c.is_null, dtype=np.uint8)
+                _check(L.rdb_table_add_int_column(h, c.name.encode(), int(c.is_date), _p(v), _p(nl)), ctx)
+            for c in spec.dict_columns:
+                ids = np.ascontiguousarray(c.ids, dtype=np.uint32)
+                nl = np.ascontiguousarray(c.is_null, dtype=np.uint8)
+                blobs = [s.encode() for s in c.names]
+                offs = np.zeros(len(blobs) + 1, dtype=np.uint32)
+                offs[1:] = np.cumsum([len(b) for b in blobs])
+                parent = None if c.lineage_parent is None else np.ascontiguousarray(c.lineage_parent, dtype=np.int32)
+                _check(L.rdb_table_add_dict_column(h, c.name.encode(), _p(ids), _p(nl), len(c.names), b"".join(blobs),
+                                                   _p(offs), None if parent is None else _p(parent)), ctx)
+                self.dict_names[c.name] = list(c.names)
+            for b in spec.bitmaps:
+                arr = np.ascontiguousarray(b, dtype=np.uint32)
+                bid = _u32()
+                _check(L.rdb_table_add_bitmap(h, _p(arr), len(arr), ctypes.byref(bid)), ctx)
+            _check(L.rdb_table_finalize(h), ctx)
+        except Exception:
+            self.close()
+            raise
+        self.n_rows = spec.n_rows
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            lib().rdb_table_free(self._h)
+            self._h = None
+
+    def _ctx(self):
+        return self.engine.handle
+
+    def bitmap(self, expr: str) -> Bitmap:
+        h = _vp()
+        card = _u64()
+        _check(lib().rdb_filter(self._h, expr.encode(), ctypes.byref(h), ctypes.byref(card)), self._ctx())
+        return Bitmap(self, h, card.value)
+
+    def filter(self, expr: str) -> np.ndarray:
+        with self.bitmap(expr) as b:
+            return b.rows()
+
+    def count(self, expr: str) -> int:
+        n = _u64()
+        _check(lib().rdb_count(self._h, expr.encode(), ctypes.byref(n)), self._ctx())
+        return n.value
+
+    def local_reference(self, column: str) -> np.ndarray:
+        out = np.zeros(self.seq_lengths[column], np.uint8)
+        _check(lib().rdb_table_local_reference(self._h, column.encode(), _p(out)), self._ctx())
+        return out
+
+    def stats(self, column: str | None = None) -> dict:
+        db, nc, pb = _u64(), _u64(), _u64()
+        _check(lib().rdb_table_stats(self._h, None if column is None else column.encode(), ctypes.byref(db),
+                                     ctypes.byref(nc), ctypes.byref(pb)), self._ctx())
+        return {"device_bytes": db.value, "containers": nc.value, "payload_bytes": pb.value}
+
+    def _with_bitmap(self, filt, fn):
+        if isinstance(filt, Bitmap):
+            return fn(filt)
+        with self.bitmap(filt) as b:
+            return fn(b)
+
+    def mutation_counts(self, filt, column: str) -> np.ndarray:
+        n_sym = len(alphabet_chars(self.alphabets[column]))
+        out = np.zeros((n_sym, self.seq_lengths[column]), np.uint32)
+
+        def run(b):
+            _check(lib().rdb_mutation_counts(self._h, b.handle, column.encode(), _p(out)), self._ctx())
+            return out
+        return self._with_bitmap(filt, run)
+
+    def mutations(self, filt, column: str, min_proportion: float) -> list:
+        cap = 32 * self.seq_lengths[column] + 16
+        pos = np.zeros(cap, np.uint32)
+        fr = np.zeros(cap, np.uint8)
+        to = np.zeros(cap, np.uint8)
+        cnt = np.zeros(cap, np.uint32)
+        cov = np.zeros(cap, np.uint32)
+        prop = np.zeros(cap, np.float64)
+        n = _u64()
+
+        def run(b):
+            _check(lib().rdb_mutations(self._h, b.handle, column.encode(), float(min_proportion), _p(pos), _p(fr), _p(to),
+                                       _p(cnt), _p(cov), _p(prop), cap, ctypes.byref(n)), self._ctx())
+            k = n.value
+            return [(int(pos[i]), int(fr[i]), int(to[i]), int(cnt[i]), int(cov[i]), float(prop[i])) for i in range(k)]
+        return self._with_bitmap(filt, run)
+
+    def aggregate(self, filt, dims: list) -> list:
+        cap = 1 << 20
+        nd = max(len(dims), 1)
+        ids = np.zeros(cap * nd, np.uint32)
+        counts = np.zeros(cap, np.uint64)
+        n = _u64()
+
+        def run(b):
+            _check(lib().rdb_aggregate(self._h, b.handle, ",".join(dims).encode(), _p(ids), _p(counts), cap,
+                                       ctypes.byref(n)), self._ctx())
+            out = []
+            for i in range(n.value):
+                key = tuple(None if int(x) == 0xFFFFFFFF else int(x) for x in ids[i * len(dims):(i + 1) * len(dims)])
+                out.append((key, int(counts[i])))
+            return out
+        return self._with_bitmap(filt, run)

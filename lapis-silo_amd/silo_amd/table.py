@@ -99,3 +99,41 @@ def encode_names(names: list) -> tuple[bytes, np.ndarray]:
     offsets = np.zeros(len(blobs) + 1, dtype=np.uint32)
     offsets[1:] = np.cumsum([len(b) for b in blobs], dtype=np.uint64).astype(np.uint32)
     return b"".join(blobs), offsets
+
+
+def seq_column_from_strings(name: str, reference: str, rows: list, alphabet: int = ALPHABET_NUC) -> SeqColumnData:
+    """Diffs aligned sequence strings (None = null row) with the engine's restatement of
+    `extractCoverageAndMutationsFromSequence` and returns the ingest arrays."""
+    from .engine import diff_sequence
+
+    n = len(rows)
+    is_null = np.zeros(n, np.uint8)
+    cs = np.zeros(n, np.uint32)
+    ce = np.zeros(n, np.uint32)
+    mptr = np.zeros(n + 1, np.uint64)
+    nptr = np.zeros(n + 1, np.uint64)
+    mpos, msym, ns, ne = [], [], [], []
+    for i, s in enumerate(rows):
+        if s is None:
+            is_null[i] = 1
+        else:
+            if len(s) != len(reference):
+                raise ValueError("aligned sequences must have the reference's length")
+            start, end, missing, muts = diff_sequence(s, reference, alphabet)
+            cs[i], ce[i] = start, end
+            for p, sym in muts:
+                mpos.append(p)
+                msym.append(sym)
+            k = 0
+            while k < len(missing):
+                j = k
+                while j + 1 < len(missing) and missing[j + 1] == missing[j] + 1:
+                    j += 1
+                ns.append(missing[k])
+                ne.append(missing[j] + 1)
+                k = j + 1
+        mptr[i + 1] = len(mpos)
+        nptr[i + 1] = len(ns)
+    ref = np.array([symbol_of(alphabet, c) for c in reference], np.uint8)
+    return SeqColumnData(name, alphabet, ref, is_null, cs, ce, mptr, np.array(mpos, np.uint32), np.array(msym, np.uint8),
+                         nptr, np.array(ns, np.uint32), np.array(ne, np.uint32))
