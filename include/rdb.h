@@ -119,6 +119,19 @@ int rdb_mutations(rdb_table* table, const rdb_bitmap* filter, const char* column
 int rdb_aggregate(rdb_table* table, const rdb_bitmap* filter, const char* dims, uint32_t* ids, uint64_t* counts,
                   uint64_t capacity, uint64_t* n);
 
+/* ---- measurement ------------------------------------------------------------------------------ */
+/* No reference counterpart (the reference's evobench probes time CPU scopes, src/evobench/
+ * evobench.hpp:144-230).  Enables (and resets) HIP-event timing of the Mutations counting kernel
+ * and the filter kernel on the context's stream; timing adds a wait after each timed launch. */
+int rdb_profile(rdb_ctx* ctx, int enable);
+int rdb_profile_read(rdb_ctx* ctx, double* mutations_ms, uint64_t* mutations_launches, double* filter_ms,
+                     uint64_t* filter_launches);
+/* Algorithmic bytes of one Mutations evaluation under `filter`: stored container bytes (array 2*card,
+ * bitset 8192, run 4*runs, plus an 8-byte descriptor each) of the chunks the filter touches, 8192
+ * per such chunk (filter tile), 8 per row and 4 per internal missing run of those chunks (coverage),
+ * and 4 * n_symbols * genome_length (counts). */
+int rdb_mutations_traffic(rdb_table* table, const rdb_bitmap* filter, const char* column, uint64_t* bytes);
+
 /* ---- ingest helper ---------------------------------------------------------------------------- */
 /* extractCoverageAndMutationsFromSequence (common/aligned_sequence.cpp:18-122) with offset 0.
  * Returns RDB_EINVAL on an illegal character and RDB_ENOMEM when a capacity is too small. */

@@ -20,11 +20,23 @@ void Context::init(int dev) {
    hipCheck(hipHostMalloc(&stage_host, kStageSlots * kStageSlotBytes, hipHostMallocDefault), "pinned staging");
    stage_dev = DeviceBuffer(kStageSlots * kStageSlotBytes);
    for (auto& e : stage_events) hipCheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+   for (auto& e : prof_events) hipCheck(hipEventCreate(&e), "hipEventCreate");
+}
+
+void Context::profileAccumulate(int which) {
+   hipCheck(hipEventSynchronize(prof_events[2 * which + 1]), "profile wait");
+   float ms = 0;
+   hipCheck(hipEventElapsedTime(&ms, prof_events[2 * which], prof_events[2 * which + 1]), "profile time");
+   prof_ms[which] += ms;
+   prof_launches[which] += 1;
 }
 
 Context::~Context() {
    if (stream) (void)hipStreamSynchronize(stream);
    for (auto& e : stage_events) {
+      if (e) (void)hipEventDestroy(e);
+   }
+   for (auto& e : prof_events) {
       if (e) (void)hipEventDestroy(e);
    }
    if (stage_host) (void)hipHostFree(stage_host);
@@ -392,8 +404,10 @@ int rdb_mutations(rdb_table* table, const rdb_bitmap* filter, const char* column
       hipCheck(launchMutations(t.seq_views_dev.as<SeqColumnView>() + idx, col.view, t.n_chunks, mutationGroups(t, col),
                                f.words, f.chunk_cards, f.card_dev, t.slot_counts.as<uint32_t>(), t.totals.as<int64_t>(),
                                min_proportion, nullptr, t.mut_rows.as<MutationRowDev>(), row_cap, t.mut_n.as<uint32_t>(),
-                               t.ctx->stream),
+                               t.ctx->stream, t.ctx->profiling ? t.ctx->prof_events[0] : nullptr,
+                               t.ctx->profiling ? t.ctx->prof_events[1] : nullptr),
                "mutations kernels");
+      if (t.ctx->profiling) t.ctx->profileAccumulate(0);
       uint32_t rows_n = 0;
       hipCheck(hipMemcpyAsync(&rows_n, t.mut_n.as<void>(), 4, hipMemcpyDeviceToHost, t.ctx->stream), "copy row count");
       hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
@@ -537,6 +551,49 @@ int rdb_aggregate(rdb_table* table, const rdb_bitmap* filter, const char* dims, 
          for (size_t k = 0; k < ds.size(); ++k) ids[i * ds.size() + k] = combos[i].key[k];
          counts[i] = combos[i].count;
       }
+   });
+}
+
+int rdb_profile(rdb_ctx* ctx, int enable) {
+   return guarded(ctx ? &ctx->c : nullptr, [&] {
+      if (!ctx) fail(RDB_EINVAL, "null context");
+      std::lock_guard<std::recursive_mutex> lock(ctx->c.mu);
+      ctx->c.profiling = enable != 0;
+      ctx->c.prof_ms[0] = ctx->c.prof_ms[1] = 0;
+      ctx->c.prof_launches[0] = ctx->c.prof_launches[1] = 0;
+   });
+}
+
+int rdb_profile_read(rdb_ctx* ctx, double* mutations_ms, uint64_t* mutations_launches, double* filter_ms,
+                     uint64_t* filter_launches) {
+   return guarded(ctx ? &ctx->c : nullptr, [&] {
+      if (!ctx) fail(RDB_EINVAL, "null context");
+      std::lock_guard<std::recursive_mutex> lock(ctx->c.mu);
+      if (mutations_ms) *mutations_ms = ctx->c.prof_ms[0];
+      if (mutations_launches) *mutations_launches = ctx->c.prof_launches[0];
+      if (filter_ms) *filter_ms = ctx->c.prof_ms[1];
+      if (filter_launches) *filter_launches = ctx->c.prof_launches[1];
+   });
+}
+
+int rdb_mutations_traffic(rdb_table* table, const rdb_bitmap* filter, const char* column, uint64_t* bytes) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireFinalized(t);
+      if (!column || !bytes) fail(RDB_EINVAL, "null argument");
+      const DeviceBitmap& f = filterOf(t, filter);
+      const SeqColumn& col = t.seqColumn(column);
+      std::vector<uint32_t> cards(t.n_chunks);
+      hipCheck(hipMemcpyAsync(cards.data(), f.chunk_cards, cards.size() * 4, hipMemcpyDeviceToHost, t.ctx->stream), "copy");
+      hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
+      uint64_t b = 4ULL * col.abc->count * col.genome_length;
+      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+         if (!cards[c]) continue;
+         b += col.chunk_index_bytes[c] + 8192ULL + 8ULL * t.chunk_sizes[c] + 4ULL * col.chunk_nruns[c];
+      }
+      *bytes = b;
    });
 }
 

@@ -109,6 +109,12 @@ struct Context {
    // Copies `bytes` of host data to device memory that stays valid for work enqueued afterwards on
    // `stream` until the ring wraps (64 further uploads); large uploads synchronize the stream.
    void* stage(const void* host, size_t bytes);
+   // kernel timing with HIP events on `stream` (enabled by rdb_profile)
+   bool profiling = false;
+   hipEvent_t prof_events[4] = {};
+   double prof_ms[2] = {0, 0};     // [0] Mutations counting kernel, [1] filter kernel
+   uint64_t prof_launches[2] = {0, 0};
+   void profileAccumulate(int which);  // waits for the pair's stop event and adds its time
    void init(int dev);
    ~Context();
 };

@@ -33,7 +33,8 @@ hipError_t launchMutations(const SeqColumnView* col_dev, const SeqColumnView& co
                            uint32_t n_groups, const uint64_t* filter, const uint32_t* filter_chunk_cards,
                            const unsigned long long* filter_card, uint32_t* slot_counts, int64_t* totals,
                            double min_proportion, uint32_t* counts_out, MutationRowDev* rows, uint32_t row_capacity,
-                           uint32_t* n_rows, hipStream_t stream);
+                           uint32_t* n_rows, hipStream_t stream, hipEvent_t count_start = nullptr,
+                           hipEvent_t count_stop = nullptr);
 
 hipError_t launchSymbolLabels(const SeqColumnView* col_dev, const uint32_t* chunk_sizes, uint32_t position,
                               uint32_t n_chunks, uint32_t block, uint32_t slot_lo, uint32_t slot_hi, uint32_t* labels,

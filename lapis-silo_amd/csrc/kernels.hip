@@ -757,15 +757,17 @@ hipError_t launchMutations(const SeqColumnView* col_dev, const SeqColumnView& co
                            uint32_t n_groups, const uint64_t* filter, const uint32_t* filter_chunk_cards,
                            const unsigned long long* filter_card, uint32_t* slot_counts, int64_t* totals,
                            double min_proportion, uint32_t* counts_out, MutationRowDev* rows, uint32_t row_capacity,
-                           uint32_t* n_rows, hipStream_t stream) {
+                           uint32_t* n_rows, hipStream_t stream, hipEvent_t count_start, hipEvent_t count_stop) {
    hipError_t err = launchFill(slot_counts, 0, col_host.n_slots, stream);
    if (err != hipSuccess) return err;
    err = launchFill(n_rows, 0, 1, stream);
    if (err != hipSuccess) return err;
+   if (count_start && (err = hipEventRecord(count_start, stream)) != hipSuccess) return err;
    hipLaunchKernelGGL(mutationsCountKernel, dim3(col_host.n_blocks, n_groups), dim3(kMutThreads), 0, stream, col_dev,
                       n_chunks, n_groups, filter, filter_chunk_cards, slot_counts);
    err = hipGetLastError();
    if (err != hipSuccess) return err;
+   if (count_stop && (err = hipEventRecord(count_stop, stream)) != hipSuccess) return err;
    const uint32_t n_pblocks = (col_host.genome_length + kPosBlock - 1) / kPosBlock;
    hipLaunchKernelGGL(mutationsTotalsKernel, dim3(n_pblocks), dim3(kPosBlock), 0, stream, col_dev, slot_counts, totals);
    err = hipGetLastError();

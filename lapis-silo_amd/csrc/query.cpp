@@ -57,10 +57,15 @@ DeviceBitmap runProgram(Table& t, const Emitter& program) {
    std::memcpy(blob.data() + sizeof(pd), program.code.data(), program.code.size() * sizeof(Instr));
    std::memcpy(blob.data() + sizeof(pd) + program.code.size() * sizeof(Instr), program.ranges.data(), program.ranges.size() * 4);
    auto* dev = static_cast<uint8_t*>(ctx.stage(blob.data(), blob.size()));
+   if (ctx.profiling) hipCheck(hipEventRecord(ctx.prof_events[2], ctx.stream), "hipEventRecord");
    hipCheck(launchFilter(t.view, reinterpret_cast<const ProgramDesc*>(dev), reinterpret_cast<const Instr*>(dev + sizeof(pd)),
                          reinterpret_cast<const uint32_t*>(dev + sizeof(pd) + program.code.size() * sizeof(Instr)), 1, depth,
                          out.words, out.card_dev, out.chunk_cards, 0, ctx.stream),
             "filter kernel");
+   if (ctx.profiling) {
+      hipCheck(hipEventRecord(ctx.prof_events[3], ctx.stream), "hipEventRecord");
+      ctx.profileAccumulate(1);
+   }
    return out;
 }
 
@@ -497,6 +502,17 @@ class Parser {
          e = std::make_unique<Expression>(K::COVERED);
          e->column = atom().second;
          e->position = position();
+      } else if (op == "op_threshold") {
+         e = std::make_unique<Expression>(K::OP_THRESHOLD);
+         e->number = static_cast<int>(integer());
+         e->exact = integer() != 0;
+         rest(*e);
+      } else if (op == "op_intersection" || op == "op_union") {
+         e = std::make_unique<Expression>(op == "op_union" ? K::OP_UNION : K::OP_INTERSECTION);
+         rest(*e);
+      } else if (op == "op_complement") {
+         e = std::make_unique<Expression>(K::OP_COMPLEMENT);
+         e->children.push_back(expr());
       } else {
          bad("unknown expression '" + op + "'");
       }
@@ -515,7 +531,11 @@ ExpressionPtr rewrite(const Table& t, const Expression& e) {
    switch (e.kind) {
       case K::AND:
       case K::NOF:
-      case K::NOT: {
+      case K::NOT:
+      case K::OP_THRESHOLD:
+      case K::OP_INTERSECTION:
+      case K::OP_UNION:
+      case K::OP_COMPLEMENT: {
          // and.cpp:139-147, nof.cpp:249-257 (mode NONE keeps an exact NOf), negation.cpp
          auto r = e.shallowCopy();
          for (const auto& c : e.children) r->children.push_back(rewrite(t, *c));
@@ -926,6 +946,28 @@ OperatorPtr compile(Table& t, const Expression& e) {
          p.push_back(std::make_unique<IsInCoveredRegion>(tp, idx, e.position, IsInCoveredRegion::Comparator::IS_COVERED));
          return std::make_unique<Selection>(tp, std::nullopt, std::move(p));
       }
+      case K::OP_THRESHOLD:
+      case K::OP_INTERSECTION: {
+         OperatorVector a, b;
+         for (const auto& ce : e.children) {
+            auto op = compile(t, *ce);
+            if (op->type() == COMPLEMENT) {
+               b.push_back(Operator::negate(std::move(op)));
+            } else {
+               a.push_back(std::move(op));
+            }
+         }
+         if (e.kind == K::OP_INTERSECTION) return std::make_unique<Intersection>(tp, std::move(a), std::move(b));
+         if (e.number < 0) fail(RDB_ECOMPILE, "Compilation Error: number_of_matchers must be greater than zero");
+         return std::make_unique<Threshold>(tp, std::move(a), std::move(b), static_cast<uint32_t>(e.number), e.exact);
+      }
+      case K::OP_UNION: {
+         OperatorVector a;
+         for (const auto& ce : e.children) a.push_back(compile(t, *ce));
+         return std::make_unique<Union>(tp, std::move(a));
+      }
+      case K::OP_COMPLEMENT:
+         return std::make_unique<Complement>(tp, compile(t, *e.children[0]));
       default:
          fail(RDB_ECOMPILE, "expression must be rewritten before compilation");
    }

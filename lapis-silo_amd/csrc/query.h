@@ -238,7 +238,10 @@ using ExpressionPtr = std::unique_ptr<Expression>;
 struct Expression {
    enum Kind {
       TRUE_, FALSE_, AND, OR, NOT, NOF, SYMBOL_EQUALS, HAS_MUTATION, SYMBOL_IN_SET, STRING_EQUALS, LINEAGE,
-      DATE_BETWEEN, INT_BETWEEN, INT_EQUALS, BITMAP, COVERED
+      DATE_BETWEEN, INT_BETWEEN, INT_EQUALS, BITMAP, COVERED,
+      // operator-level forms (the reference's operator tests build operators directly with their
+      // constructor rules)
+      OP_THRESHOLD, OP_INTERSECTION, OP_UNION, OP_COMPLEMENT
    };
    Kind kind;
    std::vector<ExpressionPtr> children;

@@ -41,6 +41,9 @@ SIGNATURES = [
     ("rdb_mutation_counts", _int, [_vp, _vp, _cp, _vp]),
     ("rdb_mutations", _int, [_vp, _vp, _cp, _f64, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
     ("rdb_aggregate", _int, [_vp, _vp, _cp, _vp, _vp, _u64, _vp]),
+    ("rdb_profile", _int, [_vp, _int]),
+    ("rdb_profile_read", _int, [_vp, _vp, _vp, _vp, _vp]),
+    ("rdb_mutations_traffic", _int, [_vp, _vp, _cp, _vp]),
     ("rdb_diff_sequence", _int, [_int, _cp, _u32, _cp, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _u64, _vp]),
 ]
 
@@ -106,6 +109,17 @@ class Engine:
         t = DeviceTable(self, spec)
         self._tables.append(t)
         return t
+
+    def profile(self, enable: bool) -> None:
+        """HIP-event timing of the Mutations counting kernel and the filter kernel (resets totals)."""
+        _check(lib().rdb_profile(self._h, int(enable)), self._h)
+
+    def profile_read(self) -> dict:
+        mm, mn, fm, fn = _f64(), _u64(), _f64(), _u64()
+        _check(lib().rdb_profile_read(self._h, ctypes.byref(mm), ctypes.byref(mn), ctypes.byref(fm), ctypes.byref(fn)),
+               self._h)
+        return {"mutations_ms": mm.value, "mutations_launches": mn.value, "filter_ms": fm.value,
+                "filter_launches": fn.value}
 
     def close(self) -> None:
         for t in self._tables:
@@ -242,6 +256,15 @@ class DeviceTable:
             return fn(filt)
         with self.bitmap(filt) as b:
             return fn(b)
+
+    def mutations_traffic(self, filt, column: str) -> int:
+        """Algorithmic bytes of one Mutations evaluation under the filter."""
+        n = _u64()
+
+        def run(b):
+            _check(lib().rdb_mutations_traffic(self._h, b.handle, column.encode(), ctypes.byref(n)), self._ctx())
+            return n.value
+        return self._with_bitmap(filt, run)
 
     def mutation_counts(self, filt, column: str) -> np.ndarray:
         n_sym = len(alphabet_chars(self.alphabets[column]))

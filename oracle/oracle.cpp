@@ -773,7 +773,10 @@ using ExprPtr = std::unique_ptr<Expr>;
 struct Expr {
    enum Kind {
       TRUE_, FALSE_, AND, OR, NOT, NOF, SYM_EQ, HAS_MUT, SYM_IN, STR_EQ, LINEAGE,
-      DATE_BETWEEN, INT_BETWEEN, INT_EQ, BITMAP, COVERED
+      DATE_BETWEEN, INT_BETWEEN, INT_EQ, BITMAP, COVERED,
+      // operator-level forms used by the reference's operator tests: they build the operator
+      // directly with its constructor rules instead of through an expression's compile()
+      OP_THRESHOLD, OP_INTERSECTION, OP_UNION, OP_COMPLEMENT
    } kind;
    std::vector<ExprPtr> children;
    std::string column;
@@ -938,6 +941,17 @@ struct Reader {
          e = std::make_unique<Expr>(Expr::COVERED);
          e->column = name();
          e->position = static_cast<uint32_t>(integer());
+      } else if (op == "op_threshold") {
+         e = std::make_unique<Expr>(Expr::OP_THRESHOLD);
+         e->number = static_cast<int>(integer());
+         e->exact = integer() != 0;
+         children(*e);
+      } else if (op == "op_intersection" || op == "op_union") {
+         e = std::make_unique<Expr>(op == "op_union" ? Expr::OP_UNION : Expr::OP_INTERSECTION);
+         children(*e);
+      } else if (op == "op_complement") {
+         e = std::make_unique<Expr>(Expr::OP_COMPLEMENT);
+         e->children.push_back(expr());
       } else {
          throw QueryError("malformed filter: unknown expression '" + op + "'");
       }
@@ -965,7 +979,11 @@ ExprPtr rewrite(const Table& t, const Expr& e) {
    switch (e.kind) {
       case Expr::AND:
       case Expr::NOF:
-      case Expr::NOT: {
+      case Expr::NOT:
+      case Expr::OP_THRESHOLD:
+      case Expr::OP_INTERSECTION:
+      case Expr::OP_UNION:
+      case Expr::OP_COMPLEMENT: {
          // and.cpp:139-147, nof.cpp:249-257 (NONE keeps exact NOf), negation.cpp:18-21
          auto r = std::make_unique<Expr>(e, 0);
          for (const auto& c : e.children) r->children.push_back(rewrite(t, *c));
@@ -1373,6 +1391,28 @@ OperatorPtr compile(const Table& t, const Expr& e) {
          p.push_back(std::make_unique<IsInCoveredRegion>(tp, &col, e.position, true));
          return std::make_unique<Selection>(tp, std::nullopt, std::move(p));
       }
+      case Expr::OP_THRESHOLD:
+      case Expr::OP_INTERSECTION: {
+         OperatorVector a, b;
+         for (const auto& c : e.children) {
+            auto op = compile(t, *c);
+            if (op->type() == OpType::COMPLEMENT) {
+               b.push_back(Operator::negate(std::move(op)));
+            } else {
+               a.push_back(std::move(op));
+            }
+         }
+         if (e.kind == Expr::OP_INTERSECTION) return std::make_unique<Intersection>(tp, std::move(a), std::move(b));
+         if (e.number < 0) throw QueryError("Compilation Error: number_of_matchers must be greater than zero");
+         return std::make_unique<Threshold>(tp, std::move(a), std::move(b), static_cast<uint32_t>(e.number), e.exact);
+      }
+      case Expr::OP_UNION: {
+         OperatorVector a;
+         for (const auto& c : e.children) a.push_back(compile(t, *c));
+         return std::make_unique<Union>(tp, std::move(a));
+      }
+      case Expr::OP_COMPLEMENT:
+         return std::make_unique<Complement>(tp, compile(t, *e.children[0]));
       default:
          throw QueryError("expression must be rewritten before compilation");
    }
