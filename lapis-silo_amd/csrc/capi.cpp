@@ -112,10 +112,12 @@ const DeviceBitmap& filterOf(const Table& t, const rdb_bitmap* f) {
    return f->b;
 }
 
+// Chunk groups of the Mutations kernel: enough work-groups for three per CU, at most one group
+// per chunk and at most kMaxGroups partial sums.
 uint32_t mutationGroups(const Table& t, const SeqColumn& col) {
-   const uint32_t target = 2u * static_cast<uint32_t>(t.ctx->cu_count);
-   uint32_t g = (target + col.view.n_blocks - 1) / std::max(1u, col.view.n_blocks);
-   return std::clamp<uint32_t>(g, 1u, std::max(1u, t.n_chunks));
+   const uint32_t target = 3u * static_cast<uint32_t>(t.ctx->cu_count);
+   const uint32_t g = (target + col.view.n_blocks - 1) / std::max(1u, col.view.n_blocks);
+   return std::clamp<uint32_t>(g, 1u, std::min<uint32_t>(kMaxGroups, std::max(1u, t.n_chunks)));
 }
 
 }  // namespace
@@ -374,7 +376,7 @@ int rdb_mutation_counts(rdb_table* table, const rdb_bitmap* filter, const char* 
       try {
          hipCheck(launchMutations(t.seq_views_dev.as<SeqColumnView>() + idx, col.view, t.n_chunks, mutationGroups(t, col),
                                   f.words, f.chunk_cards, f.card_dev, t.slot_counts.as<uint32_t>(), t.totals.as<int64_t>(),
-                                  0.0, static_cast<uint32_t*>(dev), nullptr, 0, t.mut_n.as<uint32_t>(), t.ctx->stream),
+                                  0.0, static_cast<uint32_t*>(dev), nullptr, 0, t.mut_rows.as<uint32_t>(), t.ctx->stream),
                   "mutations kernels");
          hipCheck(hipMemcpyAsync(counts, dev, bytes, hipMemcpyDeviceToHost, t.ctx->stream), "copy counts");
          hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
@@ -400,22 +402,32 @@ int rdb_mutations(rdb_table* table, const rdb_bitmap* filter, const char* column
       const int idx = t.seqIndex(column);
       if (idx < 0) fail(RDB_EQUERY, std::string("Database does not contain the sequence '") + column + "'");
       const SeqColumn& col = *t.seq[idx];
-      const uint32_t row_cap = static_cast<uint32_t>(t.mut_rows.bytes() / sizeof(MutationRowDev));
+      // device buffer: a 16-byte header holding the row count, then the rows
+      auto* base = t.mut_rows.as<uint8_t>();
+      auto* n_dev = reinterpret_cast<uint32_t*>(base);
+      auto* rows_dev = reinterpret_cast<MutationRowDev*>(base + 16);
+      const uint32_t row_cap = static_cast<uint32_t>((t.mut_rows.bytes() - 16) / sizeof(MutationRowDev));
       hipCheck(launchMutations(t.seq_views_dev.as<SeqColumnView>() + idx, col.view, t.n_chunks, mutationGroups(t, col),
                                f.words, f.chunk_cards, f.card_dev, t.slot_counts.as<uint32_t>(), t.totals.as<int64_t>(),
-                               min_proportion, nullptr, t.mut_rows.as<MutationRowDev>(), row_cap, t.mut_n.as<uint32_t>(),
-                               t.ctx->stream, t.ctx->profiling ? t.ctx->prof_events[0] : nullptr,
+                               min_proportion, nullptr, rows_dev, row_cap, n_dev, t.ctx->stream,
+                               t.ctx->profiling ? t.ctx->prof_events[0] : nullptr,
                                t.ctx->profiling ? t.ctx->prof_events[1] : nullptr),
                "mutations kernels");
       if (t.ctx->profiling) t.ctx->profileAccumulate(0);
-      uint32_t rows_n = 0;
-      hipCheck(hipMemcpyAsync(&rows_n, t.mut_n.as<void>(), 4, hipMemcpyDeviceToHost, t.ctx->stream), "copy row count");
+      // one copy brings the count and the first rows; a second one only for long outputs
+      constexpr size_t kFirst = 2048;
+      const size_t first_bytes = 16 + kFirst * sizeof(MutationRowDev);
+      auto* host = static_cast<uint8_t*>(t.ctx->pinnedBuffer(first_bytes));
+      hipCheck(hipMemcpyAsync(host, base, first_bytes, hipMemcpyDeviceToHost, t.ctx->stream), "copy rows");
       hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
+      uint32_t rows_n = 0;
+      std::memcpy(&rows_n, host, 4);
       if (rows_n > row_cap) fail(RDB_EDEVICE, "internal: mutation row buffer overflow");
       std::vector<MutationRowDev> rows(rows_n);
-      if (rows_n) {
-         hipCheck(hipMemcpyAsync(rows.data(), t.mut_rows.as<void>(), rows_n * sizeof(MutationRowDev), hipMemcpyDeviceToHost,
-                                 t.ctx->stream),
+      std::memcpy(rows.data(), host + 16, std::min<size_t>(rows_n, kFirst) * sizeof(MutationRowDev));
+      if (rows_n > kFirst) {
+         hipCheck(hipMemcpyAsync(rows.data() + kFirst, rows_dev + kFirst, (rows_n - kFirst) * sizeof(MutationRowDev),
+                                 hipMemcpyDeviceToHost, t.ctx->stream),
                   "copy rows");
          hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
       }

@@ -12,10 +12,11 @@
 //    `card` is the element count of INLINE/ARRAY and the run count of RUN.  An INLINE descriptor
 //    with card 0 is the empty container.
 //  * Vertical index of a sequence column — positions are cut into blocks (contiguous position
-//    ranges with at most 4096 distinct (position, symbol) "slots" and roughly equal payload
-//    bytes).  Tile (block b, chunk c) holds the descriptors of the block's containers in chunk c:
-//    first the light ones (few elements, processed one per lane), then the heavy ones (processed
-//    one per wavefront), each part sorted by slot.  Besides the stored (non-local-reference)
+//    ranges with at most 4095 distinct (position, symbol) "slots" and roughly equal payload
+//    bytes).  Tile (block b, chunk c) holds the block's containers in chunk c in two parts:
+//    light containers (at most 32 rows) flattened into u32 entries `slot << 16 | row`, sorted by
+//    slot then row and padded to a multiple of four with entries of the unused slot 4095; and
+//    heavy containers as descriptors (processed one per wavefront).  Besides the stored (non-local-reference)
 //    symbols, four boundary pseudo-symbols hold, per position p, the rows whose covered range
 //    starts at p (p > 0), ends at p (p < L), and whose internal missing runs start / end at p —
 //    the vertical form of the reference's horizontal coverage index.
@@ -37,7 +38,10 @@ constexpr uint32_t kChunkRows = 1u << 16;
 constexpr uint32_t kChunkWords = 1024;
 constexpr uint32_t kSliceWords = 64;
 constexpr uint32_t kSlicesPerChunk = kChunkWords / kSliceWords;
-constexpr uint32_t kMaxBlockSlots = 4096;
+constexpr uint32_t kMaxBlockSlots = 4095;   // slot 4095 is the padding slot of light entries
+constexpr uint32_t kPadSlot = 4095;
+constexpr uint32_t kLightMaxRows = 32;
+constexpr uint32_t kMaxGroups = 16;         // chunk groups of the Mutations kernel
 constexpr uint32_t kInvalidId = 0xFFFFFFFFu;
 
 enum ContainerType : uint32_t { CT_INLINE = 0, CT_ARRAY = 1, CT_RUN = 2, CT_BITSET = 3 };
@@ -74,8 +78,9 @@ struct SeqColumnView {
    ContainerArenas vert;
    const uint32_t* block_pos_start;   // [n_blocks + 1]
    const uint32_t* block_slot_start;  // [n_blocks + 1]
-   const uint32_t* tile_ptr;          // [n_blocks * n_chunks + 1]
-   const uint32_t* tile_heavy;        // [n_blocks * n_chunks]
+   const uint32_t* light_ptr;         // [n_blocks * n_chunks + 1], multiples of 4
+   const uint32_t* light;             // slot << 16 | row
+   const uint32_t* heavy_ptr;         // [n_blocks * n_chunks + 1] into vert.desc
    const uint32_t* slot_info;         // [n_slots] position | symbol << 16
    const uint32_t* pos_slot_ptr;      // [genome_length + 1]
    const uint8_t* local_ref;          // [genome_length]

@@ -27,11 +27,12 @@ hipError_t launchFilter(const TableView& tv, const ProgramDesc* progs, const Ins
                         uint32_t n_queries, uint32_t depth, uint64_t* out, unsigned long long* cards,
                         uint32_t* chunk_cards, uint32_t max_blocks, hipStream_t stream);
 
-// Mutations for one filter: per-slot AND-popcounts, boundary prefix sums, local-reference
+// Mutations for one filter: per-slot AND-popcounts (one partial sum per chunk group, n_groups <=
+// kMaxGroups, `partial` holds n_groups * n_slots u32), boundary prefix sums, local-reference
 // accumulation and thresholding.  counts_out ([n_symbols][L]) and rows may be null.
 hipError_t launchMutations(const SeqColumnView* col_dev, const SeqColumnView& col_host, uint32_t n_chunks,
                            uint32_t n_groups, const uint64_t* filter, const uint32_t* filter_chunk_cards,
-                           const unsigned long long* filter_card, uint32_t* slot_counts, int64_t* totals,
+                           const unsigned long long* filter_card, uint32_t* partial, int64_t* totals,
                            double min_proportion, uint32_t* counts_out, MutationRowDev* rows, uint32_t row_capacity,
                            uint32_t* n_rows, hipStream_t stream, hipEvent_t count_start = nullptr,
                            hipEvent_t count_stop = nullptr);

@@ -16,8 +16,8 @@ namespace {
 
 constexpr int kVmWaves = 4;
 constexpr int kVmThreads = kVmWaves * 64;
-constexpr int kMutThreads = 512;
-constexpr int kMutWaves = kMutThreads / 64;
+constexpr int kMutWaves = 4;
+constexpr int kMutThreads = kMutWaves * 64;
 constexpr int kPosBlock = 1024;
 
 __device__ inline void waveSync() {
@@ -191,24 +191,41 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
             case OP_SYMBOLS: {
                const SeqColumnView& col = tv.seq[ins.a];
                const uint32_t tile = ins.b * n_chunks + chunk;
-               const uint32_t t0 = col.tile_ptr[tile];
-               const uint32_t th = col.tile_heavy[tile];
-               const uint32_t t1 = col.tile_ptr[tile + 1];
                const uint32_t slot_base = col.block_slot_start[ins.b];
                uint64_t acc = 0;
                scratch[lane] = 0;
                waveSync();
-               for (int part = 0; part < 2; ++part) {
-                  uint32_t lo = part == 0 ? t0 : th;
-                  const uint32_t end = part == 0 ? th : t1;
+               {
+                  // light entries of the slots [c, d): slot << 16 | row, sorted
+                  const uint32_t* le = col.light;
+                  uint32_t lo = col.light_ptr[tile], hi = col.light_ptr[tile + 1];
+                  uint32_t a = lo, b = hi;
+                  while (a < b) {
+                     const uint32_t mid = (a + b) >> 1;
+                     if ((le[mid] >> 16) < ins.c) a = mid + 1; else b = mid;
+                  }
+                  uint32_t e = a;
+                  b = hi;
+                  while (e < b) {
+                     const uint32_t mid = (e + b) >> 1;
+                     if ((le[mid] >> 16) < ins.d) e = mid + 1; else b = mid;
+                  }
+                  for (uint32_t k = a + lane; k < e; k += 64) {
+                     const uint32_t x = le[k];
+                     const uint32_t row = x & 0xFFFFu;
+                     const uint32_t sym = col.slot_info[slot_base + (x >> 16)] >> 16;
+                     if (((ins.e >> sym) & 1u) && (row >> 12) == slice) {
+                        atomicOr(reinterpret_cast<unsigned long long*>(&scratch[(row >> 6) & 63]), 1ULL << (row & 63));
+                     }
+                  }
+               }
+               {
+                  uint32_t lo = col.heavy_ptr[tile];
+                  const uint32_t end = col.heavy_ptr[tile + 1];
                   uint32_t hi = end;
                   while (lo < hi) {
                      const uint32_t mid = (lo + hi) >> 1;
-                     if (descSlot(col.vert.desc[mid].w0) < ins.c) {
-                        lo = mid + 1;
-                     } else {
-                        hi = mid;
-                     }
+                     if (descSlot(col.vert.desc[mid].w0) < ins.c) lo = mid + 1; else hi = mid;
                   }
                   for (uint32_t k = lo; k < end; ++k) {
                      const Desc d = col.vert.desc[k];
@@ -374,9 +391,9 @@ __global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqCol
                                                                     uint32_t n_chunks, uint32_t n_groups,
                                                                     const uint64_t* __restrict__ filter,
                                                                     const uint32_t* __restrict__ filter_chunk_cards,
-                                                                    uint32_t* __restrict__ slot_counts) {
-   __shared__ uint64_t f[kChunkWords];
-   __shared__ uint32_t cnt[kMaxBlockSlots];
+                                                                    uint32_t* __restrict__ partial) {
+   __shared__ uint64_t fs[kMutWaves][kChunkWords];
+   __shared__ uint32_t cnt[kMaxBlockSlots + 1];
    const SeqColumnView& col = *colp;
    const uint32_t b = blockIdx.x;
    const uint32_t g = blockIdx.y;
@@ -385,84 +402,96 @@ __global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqCol
    const uint32_t wave = tid >> 6;
    const uint32_t s0 = col.block_slot_start[b];
    const uint32_t ns = col.block_slot_start[b + 1] - s0;
-   for (uint32_t i = tid; i < ns; i += kMutThreads) cnt[i] = 0;
+   for (uint32_t i = tid; i <= kMaxBlockSlots; i += kMutThreads) cnt[i] = 0;
+   __syncthreads();
+   uint64_t* f = fs[wave];
    const Desc* __restrict__ desc = col.vert.desc;
-   for (uint32_t c = g; c < n_chunks; c += n_groups) {
+   // each wavefront takes its own chunks of the group; no block-wide barrier until the end
+   for (uint32_t k = wave;; k += kMutWaves) {
+      const uint32_t c = g + k * n_groups;
+      if (c >= n_chunks) break;
       if (filter_chunk_cards[c] == 0) continue;
       const uint32_t tile = b * n_chunks + c;
-      const uint32_t d0 = col.tile_ptr[tile];
-      const uint32_t dh = col.tile_heavy[tile];
-      const uint32_t d1 = col.tile_ptr[tile + 1];
-      if (d0 == d1) continue;
-      __syncthreads();
-      const uint64_t* fc = filter + static_cast<size_t>(c) * kChunkWords;
-      for (uint32_t i = tid; i < kChunkWords; i += kMutThreads) f[i] = fc[i];
-      __syncthreads();
-      // light containers: one per lane
-      for (uint32_t k = d0 + tid; k < dh; k += kMutThreads) {
-         const Desc d = desc[k];
-         const uint32_t type = descType(d.w0);
-         const uint32_t card = descCard(d.w0);
-         uint32_t n = 0;
-         if (type == CT_INLINE) {
-            for (uint32_t j = 0; j < card; ++j) n += bitAt(f, (d.w1 >> (16 * j)) & 0xFFFFu);
-         } else if (type == CT_ARRAY) {
-            const uint16_t* a = col.vert.arr + d.w1;
-            for (uint32_t j = 0; j < card; ++j) n += bitAt(f, a[j]);
-         } else if (type == CT_RUN) {
-            const uint32_t* r = col.vert.runs + d.w1;
-            for (uint32_t j = 0; j < card; ++j) {
-               const uint32_t s = r[j] & 0xFFFFu;
-               n += rangePopcount(f, s, s + (r[j] >> 16));
+      const uint32_t l0 = col.light_ptr[tile], l1 = col.light_ptr[tile + 1];
+      const uint32_t h0 = col.heavy_ptr[tile], h1 = col.heavy_ptr[tile + 1];
+      if (l0 == l1 && h0 == h1) continue;
+      const ulonglong2* src = reinterpret_cast<const ulonglong2*>(filter + static_cast<size_t>(c) * kChunkWords);
+      for (uint32_t j = lane; j < kChunkWords / 2; j += 64) {
+         const ulonglong2 w = src[j];
+         f[2 * j] = w.x;
+         f[2 * j + 1] = w.y;
+      }
+      waveSync();
+      // light entries, four per lane per step; consecutive entries of a slot are summed locally
+      {
+         const uint4* e4 = reinterpret_cast<const uint4*>(col.light + l0);
+         const uint32_t n4 = (l1 - l0) >> 2;
+         uint32_t cur = kPadSlot, acc = 0;
+         auto take = [&](uint32_t x) {
+            const uint32_t slot = x >> 16;
+            const uint32_t row = x & 0xFFFFu;
+            if (slot != cur) {
+               if (acc) atomicAdd(&cnt[cur], acc);
+               cur = slot;
+               acc = 0;
             }
-         } else {
-            const uint64_t* bits = col.vert.bits + static_cast<size_t>(d.w1) * kChunkWords;
-            for (uint32_t j = 0; j < kChunkWords; ++j) n += __popcll(bits[j] & f[j]);
+            acc += static_cast<uint32_t>((f[row >> 6] >> (row & 63)) & 1ULL);
+         };
+         for (uint32_t j = lane; j < n4; j += 64) {
+            const uint4 e = e4[j];
+            take(e.x);
+            take(e.y);
+            take(e.z);
+            take(e.w);
          }
-         if (n) atomicAdd(&cnt[descSlot(d.w0)], n);
+         if (acc) atomicAdd(&cnt[cur], acc);
       }
       // heavy containers: one per wavefront
-      for (uint32_t k = dh + wave; k < d1; k += kMutWaves) {
-         const Desc d = desc[k];
-         const uint32_t type = descType(d.w0);
-         const uint32_t card = descCard(d.w0);
+      for (uint32_t d = h0; d < h1; ++d) {
+         const Desc dd = desc[d];
+         const uint32_t type = descType(dd.w0);
+         const uint32_t card = descCard(dd.w0);
          uint32_t n = 0;
          if (type == CT_BITSET) {
-            const ulonglong2* bits = reinterpret_cast<const ulonglong2*>(col.vert.bits + static_cast<size_t>(d.w1) * kChunkWords);
+            const ulonglong2* bits = reinterpret_cast<const ulonglong2*>(col.vert.bits + static_cast<size_t>(dd.w1) * kChunkWords);
             for (uint32_t j = lane; j < kChunkWords / 2; j += 64) {
                const ulonglong2 w = bits[j];
                n += __popcll(w.x & f[2 * j]) + __popcll(w.y & f[2 * j + 1]);
             }
          } else if (type == CT_ARRAY) {
-            const uint16_t* a = col.vert.arr + d.w1;
+            const uint16_t* a = col.vert.arr + dd.w1;
             for (uint32_t j = lane; j < card; j += 64) n += bitAt(f, a[j]);
          } else if (type == CT_RUN) {
-            const uint32_t* r = col.vert.runs + d.w1;
+            const uint32_t* r = col.vert.runs + dd.w1;
             for (uint32_t j = lane; j < card; j += 64) {
-               const uint32_t s = r[j] & 0xFFFFu;
-               n += rangePopcount(f, s, s + (r[j] >> 16));
+               const uint32_t st = r[j] & 0xFFFFu;
+               n += rangePopcount(f, st, st + (r[j] >> 16));
             }
-         } else {
-            if (lane == 0) {
-               for (uint32_t j = 0; j < card; ++j) n += bitAt(f, (d.w1 >> (16 * j)) & 0xFFFFu);
-            }
+         } else if (lane == 0) {
+            for (uint32_t j = 0; j < card; ++j) n += bitAt(f, (dd.w1 >> (16 * j)) & 0xFFFFu);
          }
          n = waveSum(n);
-         if (lane == 0 && n) atomicAdd(&cnt[descSlot(d.w0)], n);
+         if (lane == 0 && n) atomicAdd(&cnt[descSlot(dd.w0)], n);
       }
+      waveSync();
    }
    __syncthreads();
-   for (uint32_t i = tid; i < ns; i += kMutThreads) {
-      if (cnt[i]) atomicAdd(&slot_counts[s0 + i], cnt[i]);
-   }
+   uint32_t* out = partial + static_cast<size_t>(g) * col.n_slots + s0;
+   for (uint32_t i = tid; i < ns; i += kMutThreads) out[i] = cnt[i];
+}
+
+__device__ inline uint32_t slotValue(const uint32_t* partial, uint32_t n_groups, uint32_t n_slots, uint32_t s) {
+   uint32_t v = 0;
+   for (uint32_t g = 0; g < n_groups; ++g) v += partial[static_cast<size_t>(g) * n_slots + s];
+   return v;
 }
 
 // Per 1024-position block: totals of the three boundary quantities
 //   A(p) = rows of the filter whose covered range starts at p (p > 0),
 //   B(p) = rows whose covered range ends at p,
 //   C(p) = internal missing runs starting at p minus those ending at p.
-__device__ inline void boundaryAt(const SeqColumnView& col, const uint32_t* slot_counts, uint32_t p, uint32_t& A,
-                                  uint32_t& B, int32_t& C) {
+__device__ inline void boundaryAt(const SeqColumnView& col, const uint32_t* partial, uint32_t n_groups, uint32_t p,
+                                  uint32_t& A, uint32_t& B, int32_t& C) {
    A = 0;
    B = 0;
    C = 0;
@@ -470,7 +499,7 @@ __device__ inline void boundaryAt(const SeqColumnView& col, const uint32_t* slot
    for (uint32_t s = col.pos_slot_ptr[p]; s < e; ++s) {
       const uint32_t sym = col.slot_info[s] >> 16;
       if (sym < col.n_symbols) continue;
-      const uint32_t v = slot_counts[s];
+      const uint32_t v = slotValue(partial, n_groups, col.n_slots, s);
       switch (sym - col.n_symbols) {
          case BND_COV_START: A += v; break;
          case BND_COV_END: B += v; break;
@@ -505,14 +534,16 @@ __device__ inline T blockInclusiveScan(T v, T* tmp) {
 }
 
 __global__ __launch_bounds__(kPosBlock) void mutationsTotalsKernel(const SeqColumnView* __restrict__ colp,
-                                                                   const uint32_t* __restrict__ slot_counts,
-                                                                   int64_t* __restrict__ totals) {
+                                                                   const uint32_t* __restrict__ partial,
+                                                                   uint32_t n_groups, int64_t* __restrict__ totals,
+                                                                   uint32_t* __restrict__ n_rows) {
    __shared__ int64_t tmp[3][16];
    const SeqColumnView& col = *colp;
+   if (blockIdx.x == 0 && threadIdx.x == 0) *n_rows = 0;  // the finalize kernel appends rows after this one
    const uint32_t p = blockIdx.x * kPosBlock + threadIdx.x;
    uint32_t A = 0, B = 0;
    int32_t C = 0;
-   if (p < col.genome_length) boundaryAt(col, slot_counts, p, A, B, C);
+   if (p < col.genome_length) boundaryAt(col, partial, n_groups, p, A, B, C);
    int64_t v[3] = {A, B, C};
    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    for (int q = 0; q < 3; ++q) {
@@ -529,7 +560,7 @@ __global__ __launch_bounds__(kPosBlock) void mutationsTotalsKernel(const SeqColu
 }
 
 __global__ __launch_bounds__(kPosBlock) void mutationsFinalizeKernel(
-   const SeqColumnView* __restrict__ colp, const uint32_t* __restrict__ slot_counts,
+   const SeqColumnView* __restrict__ colp, const uint32_t* __restrict__ partial, uint32_t n_groups,
    const int64_t* __restrict__ totals, uint32_t n_pblocks, const unsigned long long* __restrict__ filter_card,
    double min_proportion, uint32_t* __restrict__ counts_out, MutationRowDev* __restrict__ rows,
    uint32_t row_capacity, uint32_t* __restrict__ n_rows) {
@@ -556,7 +587,7 @@ __global__ __launch_bounds__(kPosBlock) void mutationsFinalizeKernel(
    const uint32_t p = blockIdx.x * kPosBlock + threadIdx.x;
    uint32_t A = 0, B = 0;
    int32_t C = 0;
-   if (p < L) boundaryAt(col, slot_counts, p, A, B, C);
+   if (p < L) boundaryAt(col, partial, n_groups, p, A, B, C);
    const int64_t pa = blockInclusiveScan<int64_t>(A, scan_tmp) + prefix[0];
    const int64_t pb = blockInclusiveScan<int64_t>(B, scan_tmp) + prefix[1];
    const int64_t pc = blockInclusiveScan<int64_t>(C, scan_tmp) + prefix[2];
@@ -568,7 +599,7 @@ __global__ __launch_bounds__(kPosBlock) void mutationsFinalizeKernel(
    const uint32_t s_end = col.pos_slot_ptr[p + 1];
    int64_t stored = 0;
    for (uint32_t s = s_begin; s < s_end; ++s) {
-      if ((col.slot_info[s] >> 16) < col.n_symbols) stored += slot_counts[s];
+      if ((col.slot_info[s] >> 16) < col.n_symbols) stored += slotValue(partial, n_groups, col.n_slots, s);
    }
    const uint32_t ref_count = card == 0 ? 0u : static_cast<uint32_t>(covered - stored);
    const uint32_t local = col.local_ref[p];
@@ -578,7 +609,7 @@ __global__ __launch_bounds__(kPosBlock) void mutationsFinalizeKernel(
       uint32_t cur = s_begin;
       for (uint32_t sym = 0; sym < col.n_symbols; ++sym) {
          uint32_t v = 0;
-         if (cur < s_end && (col.slot_info[cur] >> 16) == sym) v = slot_counts[cur++];
+         if (cur < s_end && (col.slot_info[cur] >> 16) == sym) v = slotValue(partial, n_groups, col.n_slots, cur++);
          if (sym == local) v += ref_count;
          if (counts_out) counts_out[static_cast<size_t>(sym) * L + p] = v;
          if ((col.valid_mask >> sym) & 1u) total += v;
@@ -592,7 +623,7 @@ __global__ __launch_bounds__(kPosBlock) void mutationsFinalizeKernel(
    uint32_t cur = s_begin;
    for (uint32_t sym = 0; sym < col.n_symbols; ++sym) {
       uint32_t v = 0;
-      if (cur < s_end && (col.slot_info[cur] >> 16) == sym) v = slot_counts[cur++];
+      if (cur < s_end && (col.slot_info[cur] >> 16) == sym) v = slotValue(partial, n_groups, col.n_slots, cur++);
       if (sym == local) v += ref_count;
       if (!((col.valid_mask >> sym) & 1u) || sym == orig) continue;
       if (v > thr) {
@@ -644,7 +675,14 @@ __global__ void symbolScatterKernel(const SeqColumnView* __restrict__ colp, uint
    const uint32_t tile = block * n_chunks + c;
    const uint32_t slot_base = col.block_slot_start[block];
    uint32_t* lab = labels + (static_cast<size_t>(c) << 16);
-   for (uint32_t k = col.tile_ptr[tile]; k < col.tile_ptr[tile + 1]; ++k) {
+   for (uint32_t k = col.light_ptr[tile] + threadIdx.x; k < col.light_ptr[tile + 1]; k += blockDim.x) {
+      const uint32_t x = col.light[k];
+      const uint32_t slot = x >> 16;
+      if (slot < slot_lo || slot >= slot_hi) continue;
+      const uint32_t sym = col.slot_info[slot_base + slot] >> 16;
+      if (sym < col.n_symbols) lab[x & 0xFFFFu] = sym;
+   }
+   for (uint32_t k = col.heavy_ptr[tile]; k < col.heavy_ptr[tile + 1]; ++k) {
       const Desc d = col.vert.desc[k];
       const uint32_t slot = descSlot(d.w0);
       if (slot < slot_lo || slot >= slot_hi) continue;
@@ -658,8 +696,8 @@ __global__ void symbolScatterKernel(const SeqColumnView* __restrict__ colp, uint
       } else if (type == CT_RUN) {
          for (uint32_t j = 0; j < card; ++j) {
             const uint32_t x = col.vert.runs[d.w1 + j];
-            const uint32_t s = x & 0xFFFFu, e = s + (x >> 16);
-            for (uint32_t v = s + threadIdx.x; v <= e; v += blockDim.x) lab[v] = sym;
+            const uint32_t st = x & 0xFFFFu, e = st + (x >> 16);
+            for (uint32_t v = st + threadIdx.x; v <= e; v += blockDim.x) lab[v] = sym;
          }
       } else {
          const uint64_t* bits = col.vert.bits + static_cast<size_t>(d.w1) * kChunkWords;
@@ -755,24 +793,23 @@ hipError_t launchFilter(const TableView& tv, const ProgramDesc* progs, const Ins
 
 hipError_t launchMutations(const SeqColumnView* col_dev, const SeqColumnView& col_host, uint32_t n_chunks,
                            uint32_t n_groups, const uint64_t* filter, const uint32_t* filter_chunk_cards,
-                           const unsigned long long* filter_card, uint32_t* slot_counts, int64_t* totals,
+                           const unsigned long long* filter_card, uint32_t* partial, int64_t* totals,
                            double min_proportion, uint32_t* counts_out, MutationRowDev* rows, uint32_t row_capacity,
                            uint32_t* n_rows, hipStream_t stream, hipEvent_t count_start, hipEvent_t count_stop) {
-   hipError_t err = launchFill(slot_counts, 0, col_host.n_slots, stream);
-   if (err != hipSuccess) return err;
-   err = launchFill(n_rows, 0, 1, stream);
-   if (err != hipSuccess) return err;
+   if (n_groups == 0 || n_groups > kMaxGroups) return hipErrorInvalidValue;
+   hipError_t err = hipSuccess;
    if (count_start && (err = hipEventRecord(count_start, stream)) != hipSuccess) return err;
    hipLaunchKernelGGL(mutationsCountKernel, dim3(col_host.n_blocks, n_groups), dim3(kMutThreads), 0, stream, col_dev,
-                      n_chunks, n_groups, filter, filter_chunk_cards, slot_counts);
+                      n_chunks, n_groups, filter, filter_chunk_cards, partial);
    err = hipGetLastError();
    if (err != hipSuccess) return err;
    if (count_stop && (err = hipEventRecord(count_stop, stream)) != hipSuccess) return err;
    const uint32_t n_pblocks = (col_host.genome_length + kPosBlock - 1) / kPosBlock;
-   hipLaunchKernelGGL(mutationsTotalsKernel, dim3(n_pblocks), dim3(kPosBlock), 0, stream, col_dev, slot_counts, totals);
+   hipLaunchKernelGGL(mutationsTotalsKernel, dim3(n_pblocks), dim3(kPosBlock), 0, stream, col_dev, partial, n_groups,
+                      totals, n_rows);
    err = hipGetLastError();
    if (err != hipSuccess) return err;
-   hipLaunchKernelGGL(mutationsFinalizeKernel, dim3(n_pblocks), dim3(kPosBlock), 0, stream, col_dev, slot_counts,
+   hipLaunchKernelGGL(mutationsFinalizeKernel, dim3(n_pblocks), dim3(kPosBlock), 0, stream, col_dev, partial, n_groups,
                       totals, n_pblocks, filter_card, min_proportion, counts_out, rows, row_capacity, n_rows);
    return hipGetLastError();
 }

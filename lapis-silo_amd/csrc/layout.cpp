@@ -524,7 +524,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
 
    // --- position blocks: <= 4096 slots and about equal bytes --------------------------------------
    const uint64_t total_bytes = std::accumulate(pos_bytes.begin(), pos_bytes.end(), uint64_t{0});
-   const uint64_t target = std::max<uint64_t>(total_bytes / 192, 1);
+   const uint64_t target = std::max<uint64_t>(total_bytes / 64, 1);
    col.block_pos_start.assign(1, 0);
    col.block_of_pos.assign(L, 0);
    {
@@ -557,49 +557,61 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    if (arr_base[t.n_chunks] > 0xFFFFFFFFULL || run_base[t.n_chunks] > 0xFFFFFFFFULL || bit_base[t.n_chunks] > 0xFFFFFFFFULL) {
       fail(RDB_ENOMEM, "container arena exceeds 32-bit offsets");
    }
-   size_t n_desc = 0;
-   for (const auto& ch : chunks) n_desc += ch.key.size();
+   // Light containers (at most kLightMaxRows rows) become u32 entries slot << 16 | row; the rest stay
+   // descriptors into the concatenated arenas.
+   const size_t n_tiles = static_cast<size_t>(n_blocks) * t.n_chunks;
+   std::vector<uint32_t> light_ptr(n_tiles + 1, 0), heavy_ptr(n_tiles + 1, 0);
+   std::vector<uint32_t> light;
    std::vector<Desc> desc;
-   desc.reserve(n_desc);
-   std::vector<uint32_t> tile_ptr(static_cast<size_t>(n_blocks) * t.n_chunks + 1, 0);
-   std::vector<uint32_t> tile_heavy(static_cast<size_t>(n_blocks) * t.n_chunks, 0);
    std::vector<size_t> cursor(t.n_chunks, 0);
-   auto is_light = [](const Desc& d) {
-      const uint32_t type = descType(d.w0), card = descCard(d.w0);
-      return type == CT_INLINE || (type == CT_ARRAY && card <= 32) || (type == CT_RUN && card <= 4);
-   };
-   std::vector<Desc> heavy;
    for (uint32_t b = 0; b < n_blocks; ++b) {
       const uint32_t pend = col.block_pos_start[b + 1];
       const uint32_t sbase = col.block_slot_start[b];
       for (uint32_t c = 0; c < t.n_chunks; ++c) {
          const size_t tile = static_cast<size_t>(b) * t.n_chunks + c;
-         tile_ptr[tile] = static_cast<uint32_t>(desc.size());
-         heavy.clear();
+         light_ptr[tile] = static_cast<uint32_t>(light.size());
+         heavy_ptr[tile] = static_cast<uint32_t>(desc.size());
          auto& ch = chunks[c];
          size_t& k = cursor[c];
          while (k < ch.key.size() && (ch.key[k] >> 8) < pend) {
             const uint32_t p = ch.key[k] >> 8, s = ch.key[k] & 0xFF;
             const uint32_t slot = col.pos_slot_ptr[p] + std::popcount(sym_mask[p] & ((1u << s) - 1u)) - sbase;
             Desc d = ch.arenas.desc[k];
-            const uint32_t type = descType(d.w0);
-            d.w0 = makeW0(slot, type, descCard(d.w0));
-            if (type == CT_ARRAY) d.w1 += static_cast<uint32_t>(arr_base[c]);
-            if (type == CT_RUN) d.w1 += static_cast<uint32_t>(run_base[c]);
-            if (type == CT_BITSET) d.w1 += static_cast<uint32_t>(bit_base[c]);
-            if (is_light(d)) {
-               desc.push_back(d);
+            const uint32_t type = descType(d.w0), card = descCard(d.w0);
+            uint32_t rows = card;
+            if (type == CT_RUN) {
+               rows = 0;
+               for (uint32_t j = 0; j < card; ++j) rows += (ch.arenas.runs[d.w1 + j] >> 16) + 1;
+            } else if (type == CT_BITSET) {
+               rows = kChunkRows;
+            }
+            if (rows <= kLightMaxRows) {
+               const uint32_t hi = slot << 16;
+               if (type == CT_INLINE) {
+                  for (uint32_t j = 0; j < card; ++j) light.push_back(hi | ((d.w1 >> (16 * j)) & 0xFFFFu));
+               } else if (type == CT_ARRAY) {
+                  for (uint32_t j = 0; j < card; ++j) light.push_back(hi | ch.arenas.arr[d.w1 + j]);
+               } else {
+                  for (uint32_t j = 0; j < card; ++j) {
+                     const uint32_t x = ch.arenas.runs[d.w1 + j];
+                     for (uint32_t r = (x & 0xFFFFu); r <= (x & 0xFFFFu) + (x >> 16); ++r) light.push_back(hi | r);
+                  }
+               }
             } else {
-               heavy.push_back(d);
+               d.w0 = makeW0(slot, type, card);
+               if (type == CT_ARRAY) d.w1 += static_cast<uint32_t>(arr_base[c]);
+               if (type == CT_RUN) d.w1 += static_cast<uint32_t>(run_base[c]);
+               if (type == CT_BITSET) d.w1 += static_cast<uint32_t>(bit_base[c]);
+               desc.push_back(d);
             }
             ++k;
          }
-         tile_heavy[tile] = static_cast<uint32_t>(desc.size());
-         desc.insert(desc.end(), heavy.begin(), heavy.end());
+         while (light.size() % 4 != 0) light.push_back(kPadSlot << 16);
+         if (light.size() > 0xFFFFFFF0ULL || desc.size() > 0xFFFFFFF0ULL) fail(RDB_ENOMEM, "too many containers");
       }
    }
-   tile_ptr.back() = static_cast<uint32_t>(desc.size());
-   if (desc.size() > 0xFFFFFFFFULL) fail(RDB_ENOMEM, "too many containers");
+   light_ptr[n_tiles] = static_cast<uint32_t>(light.size());
+   heavy_ptr[n_tiles] = static_cast<uint32_t>(desc.size());
 
    std::vector<uint16_t> arr_all(arr_base[t.n_chunks]);
    std::vector<uint32_t> run_all(run_base[t.n_chunks]);
@@ -615,7 +627,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
       std::vector<uint64_t>().swap(a.bits);
    });
    chunks.clear();
-   col.desc_bytes = desc.size() * sizeof(Desc);
+   col.desc_bytes = desc.size() * sizeof(Desc) + light.size() * 4;
 
    // --- row-level arrays (padded to 2^16 rows per chunk) ------------------------------------------
    const size_t padded = static_cast<size_t>(t.n_chunks) << 16;
@@ -662,11 +674,16 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
       if (ns > kMaxBlockSlots) fail(RDB_EDEVICE, "internal: block with too many slots");
       for (uint32_t c = 0; c < t.n_chunks; ++c) {
          const size_t tile = static_cast<size_t>(b) * t.n_chunks + c;
-         if (!(tile_ptr[tile] <= tile_heavy[tile] && tile_heavy[tile] <= tile_ptr[tile + 1])) {
+         if (light_ptr[tile] > light_ptr[tile + 1] || heavy_ptr[tile] > heavy_ptr[tile + 1] || light_ptr[tile] % 4 != 0) {
             fail(RDB_EDEVICE, "internal: tile pointers out of order");
          }
-         for (uint32_t k = tile_ptr[tile]; k < tile_ptr[tile + 1]; ++k) {
+         for (uint32_t k = heavy_ptr[tile]; k < heavy_ptr[tile + 1]; ++k) {
             if (descSlot(desc[k].w0) >= ns) fail(RDB_EDEVICE, "internal: descriptor slot outside its block");
+         }
+         for (uint32_t k = light_ptr[tile]; k < light_ptr[tile + 1]; ++k) {
+            const uint32_t slot = light[k] >> 16;
+            if (slot >= ns && slot != kPadSlot) fail(RDB_EDEVICE, "internal: light entry slot outside its block");
+            if (k > light_ptr[tile] && light[k] < light[k - 1]) fail(RDB_EDEVICE, "internal: light entries not sorted");
          }
       }
    }
@@ -693,8 +710,9 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    v.vert.bits = static_cast<const uint64_t*>(up(bit_all.data(), bit_all.size() * 8));
    v.block_pos_start = static_cast<const uint32_t*>(up(col.block_pos_start.data(), col.block_pos_start.size() * 4));
    v.block_slot_start = static_cast<const uint32_t*>(up(col.block_slot_start.data(), col.block_slot_start.size() * 4));
-   v.tile_ptr = static_cast<const uint32_t*>(up(tile_ptr.data(), tile_ptr.size() * 4));
-   v.tile_heavy = static_cast<const uint32_t*>(up(tile_heavy.data(), tile_heavy.size() * 4));
+   v.light_ptr = static_cast<const uint32_t*>(up(light_ptr.data(), light_ptr.size() * 4));
+   v.light = static_cast<const uint32_t*>(up(light.data(), light.size() * 4));
+   v.heavy_ptr = static_cast<const uint32_t*>(up(heavy_ptr.data(), heavy_ptr.size() * 4));
    v.slot_info = static_cast<const uint32_t*>(up(col.slot_info.data(), col.slot_info.size() * 4));
    v.pos_slot_ptr = static_cast<const uint32_t*>(up(col.pos_slot_ptr.data(), col.pos_slot_ptr.size() * 4));
    v.local_ref = static_cast<const uint8_t*>(up(col.local_ref.data(), L));
@@ -839,9 +857,9 @@ void finalizeTable(Table& t) {
       max_slots = std::max(max_slots, c->view.n_slots);
       max_L = std::max(max_L, c->genome_length);
    }
-   t.slot_counts = DeviceBuffer(static_cast<size_t>(max_slots) * 4);
+   t.slot_counts = DeviceBuffer(static_cast<size_t>(max_slots) * 4 * kMaxGroups);
    t.totals = DeviceBuffer(static_cast<size_t>((max_L + 1023) / 1024) * 3 * sizeof(int64_t));
-   t.mut_rows = DeviceBuffer(static_cast<size_t>(max_L) * 32 * sizeof(MutationRowDev));
+   t.mut_rows = DeviceBuffer(16 + static_cast<size_t>(max_L) * 32 * sizeof(MutationRowDev));
    t.mut_n = DeviceBuffer(64);
    t.finalized = true;
 }
