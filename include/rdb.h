@@ -112,6 +112,11 @@ int rdb_mutation_counts(rdb_table* table, const rdb_bitmap* filter, const char* 
 int rdb_mutations(rdb_table* table, const rdb_bitmap* filter, const char* column, double min_proportion,
                   uint32_t* position, uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage,
                   double* proportion, uint64_t capacity, uint64_t* n);
+/* MutationsNode::addToExecPlan as one call (mutations_node.cpp:372-428): computeFilter on the
+ * device, then the Mutations rows as rdb_mutations returns them, with a single wait. */
+int rdb_mutations_query(rdb_table* table, const char* filter, const char* column, double min_proportion,
+                        uint32_t* position, uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage,
+                        double* proportion, uint64_t capacity, uint64_t* n);
 /* Aggregated count group-by (bitmap_aggregation_node.cpp:91-130, 189-253).  dims: comma-separated
  * dimensions; "name" = indexed dictionary column, "name@pos" = symbol at 0-based position of a
  * sequence column.  Output: combinations in the reference's depth-first order, n_dims ids each

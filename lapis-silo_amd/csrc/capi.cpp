@@ -112,12 +112,11 @@ const DeviceBitmap& filterOf(const Table& t, const rdb_bitmap* f) {
    return f->b;
 }
 
-// Chunk groups of the Mutations kernel: enough work-groups for three per CU, at most one group
-// per chunk and at most kMaxGroups partial sums.
-uint32_t mutationGroups(const Table& t, const SeqColumn& col) {
-   const uint32_t target = 3u * static_cast<uint32_t>(t.ctx->cu_count);
-   const uint32_t g = (target + col.view.n_blocks - 1) / std::max(1u, col.view.n_blocks);
-   return std::clamp<uint32_t>(g, 1u, std::min<uint32_t>(kMaxGroups, std::max(1u, t.n_chunks)));
+// Chunk groups of the Mutations kernel: a group per four chunks (one chunk per wavefront of the
+// work-group), at most kMaxGroups partial sums.
+uint32_t mutationGroups(const Table& t, const SeqColumn& /*col*/) {
+   const uint32_t g = (t.n_chunks + 3) / 4;
+   return std::clamp<uint32_t>(g, 1u, kMaxGroups);
 }
 
 }  // namespace
@@ -388,6 +387,69 @@ int rdb_mutation_counts(rdb_table* table, const rdb_bitmap* filter, const char* 
    });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Mutations rows under a device filter: kernels, one readback, reference output order
+// (mutations_node.cpp:292-366).
+void runMutations(Table& t, const DeviceBitmap& f, const char* column, double min_proportion, uint32_t* position,
+                  uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage, double* proportion,
+                  uint64_t capacity, uint64_t* n) {
+   if (!(min_proportion >= 0.0 && min_proportion <= 1.0)) fail(RDB_EQUERY, "minProportion must be in interval [0.0,1.0]");
+   const int idx = t.seqIndex(column);
+   if (idx < 0) fail(RDB_EQUERY, std::string("Database does not contain the sequence '") + column + "'");
+   const SeqColumn& col = *t.seq[idx];
+   // device buffer: a 16-byte header holding the row count, then the rows
+   auto* base = t.mut_rows.as<uint8_t>();
+   auto* n_dev = reinterpret_cast<uint32_t*>(base);
+   auto* rows_dev = reinterpret_cast<MutationRowDev*>(base + 16);
+   const uint32_t row_cap = static_cast<uint32_t>((t.mut_rows.bytes() - 16) / sizeof(MutationRowDev));
+   hipCheck(launchMutations(t.seq_views_dev.as<SeqColumnView>() + idx, col.view, t.n_chunks, mutationGroups(t, col),
+                            f.words, f.chunk_cards, f.card_dev, t.slot_counts.as<uint32_t>(), t.totals.as<int64_t>(),
+                            min_proportion, nullptr, rows_dev, row_cap, n_dev, t.ctx->stream,
+                            t.ctx->profiling ? t.ctx->prof_events[0] : nullptr,
+                            t.ctx->profiling ? t.ctx->prof_events[1] : nullptr),
+            "mutations kernels");
+   if (t.ctx->profiling) t.ctx->profileAccumulate(0);
+   // one copy brings the count and the first rows; a second one only for long outputs
+   constexpr size_t kFirst = 2048;
+   const size_t first_bytes = std::min(16 + kFirst * sizeof(MutationRowDev), t.mut_rows.bytes());
+   auto* host = static_cast<uint8_t*>(t.ctx->pinnedBuffer(first_bytes));
+   hipCheck(hipMemcpyAsync(host, base, first_bytes, hipMemcpyDeviceToHost, t.ctx->stream), "copy rows");
+   hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
+   uint32_t rows_n = 0;
+   std::memcpy(&rows_n, host, 4);
+   if (rows_n > row_cap) fail(RDB_EDEVICE, "internal: mutation row buffer overflow");
+   std::vector<MutationRowDev> rows(rows_n);
+   const size_t in_first = std::min<size_t>(rows_n, (first_bytes - 16) / sizeof(MutationRowDev));
+   std::memcpy(rows.data(), host + 16, in_first * sizeof(MutationRowDev));
+   if (rows_n > in_first) {
+      hipCheck(hipMemcpyAsync(rows.data() + in_first, rows_dev + in_first, (rows_n - in_first) * sizeof(MutationRowDev),
+                              hipMemcpyDeviceToHost, t.ctx->stream),
+               "copy rows");
+      hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
+   }
+   // mutations_node.cpp:302-364 emits rows by position, then in VALID_MUTATION_SYMBOLS order
+   std::sort(rows.begin(), rows.end(), [](const MutationRowDev& a, const MutationRowDev& b) {
+      return a.position != b.position ? a.position < b.position : a.symbol < b.symbol;
+   });
+   *n = rows.size();
+   if (rows.size() > capacity) fail(RDB_ENOMEM, "output capacity too small");
+   for (size_t i = 0; i < rows.size(); ++i) {
+      if (position) position[i] = rows[i].position + 1;
+      if (from) from[i] = col.reference[rows[i].position];
+      if (to) to[i] = static_cast<uint8_t>(rows[i].symbol);
+      if (count) count[i] = rows[i].count;
+      if (coverage) coverage[i] = rows[i].coverage;
+      if (proportion) proportion[i] = static_cast<double>(rows[i].count) / static_cast<double>(rows[i].coverage);
+   }
+}
+
+}  // namespace
+
+extern "C" {
+
 int rdb_mutations(rdb_table* table, const rdb_bitmap* filter, const char* column, double min_proportion,
                   uint32_t* position, uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage,
                   double* proportion, uint64_t capacity, uint64_t* n) {
@@ -397,54 +459,22 @@ int rdb_mutations(rdb_table* table, const rdb_bitmap* filter, const char* column
       std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
       requireFinalized(t);
       if (!column || !n) fail(RDB_EINVAL, "null argument");
-      if (!(min_proportion >= 0.0 && min_proportion <= 1.0)) fail(RDB_EQUERY, "minProportion must be in interval [0.0,1.0]");
-      const DeviceBitmap& f = filterOf(t, filter);
-      const int idx = t.seqIndex(column);
-      if (idx < 0) fail(RDB_EQUERY, std::string("Database does not contain the sequence '") + column + "'");
-      const SeqColumn& col = *t.seq[idx];
-      // device buffer: a 16-byte header holding the row count, then the rows
-      auto* base = t.mut_rows.as<uint8_t>();
-      auto* n_dev = reinterpret_cast<uint32_t*>(base);
-      auto* rows_dev = reinterpret_cast<MutationRowDev*>(base + 16);
-      const uint32_t row_cap = static_cast<uint32_t>((t.mut_rows.bytes() - 16) / sizeof(MutationRowDev));
-      hipCheck(launchMutations(t.seq_views_dev.as<SeqColumnView>() + idx, col.view, t.n_chunks, mutationGroups(t, col),
-                               f.words, f.chunk_cards, f.card_dev, t.slot_counts.as<uint32_t>(), t.totals.as<int64_t>(),
-                               min_proportion, nullptr, rows_dev, row_cap, n_dev, t.ctx->stream,
-                               t.ctx->profiling ? t.ctx->prof_events[0] : nullptr,
-                               t.ctx->profiling ? t.ctx->prof_events[1] : nullptr),
-               "mutations kernels");
-      if (t.ctx->profiling) t.ctx->profileAccumulate(0);
-      // one copy brings the count and the first rows; a second one only for long outputs
-      constexpr size_t kFirst = 2048;
-      const size_t first_bytes = 16 + kFirst * sizeof(MutationRowDev);
-      auto* host = static_cast<uint8_t*>(t.ctx->pinnedBuffer(first_bytes));
-      hipCheck(hipMemcpyAsync(host, base, first_bytes, hipMemcpyDeviceToHost, t.ctx->stream), "copy rows");
-      hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
-      uint32_t rows_n = 0;
-      std::memcpy(&rows_n, host, 4);
-      if (rows_n > row_cap) fail(RDB_EDEVICE, "internal: mutation row buffer overflow");
-      std::vector<MutationRowDev> rows(rows_n);
-      std::memcpy(rows.data(), host + 16, std::min<size_t>(rows_n, kFirst) * sizeof(MutationRowDev));
-      if (rows_n > kFirst) {
-         hipCheck(hipMemcpyAsync(rows.data() + kFirst, rows_dev + kFirst, (rows_n - kFirst) * sizeof(MutationRowDev),
-                                 hipMemcpyDeviceToHost, t.ctx->stream),
-                  "copy rows");
-         hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
-      }
-      // mutations_node.cpp:302-364 emits rows by position, then in VALID_MUTATION_SYMBOLS order
-      std::sort(rows.begin(), rows.end(), [](const MutationRowDev& a, const MutationRowDev& b) {
-         return a.position != b.position ? a.position < b.position : a.symbol < b.symbol;
-      });
-      *n = rows.size();
-      if (rows.size() > capacity) fail(RDB_ENOMEM, "output capacity too small");
-      for (size_t i = 0; i < rows.size(); ++i) {
-         if (position) position[i] = rows[i].position + 1;
-         if (from) from[i] = col.reference[rows[i].position];
-         if (to) to[i] = static_cast<uint8_t>(rows[i].symbol);
-         if (count) count[i] = rows[i].count;
-         if (coverage) coverage[i] = rows[i].coverage;
-         if (proportion) proportion[i] = static_cast<double>(rows[i].count) / static_cast<double>(rows[i].coverage);
-      }
+      runMutations(t, filterOf(t, filter), column, min_proportion, position, from, to, count, coverage, proportion,
+                   capacity, n);
+   });
+}
+
+int rdb_mutations_query(rdb_table* table, const char* filter, const char* column, double min_proportion,
+                        uint32_t* position, uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage,
+                        double* proportion, uint64_t capacity, uint64_t* n) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireFinalized(t);
+      if (!filter || !column || !n) fail(RDB_EINVAL, "null argument");
+      DeviceBitmap f = query::compileFilter(t, filter)->evaluate();
+      runMutations(t, f, column, min_proportion, position, from, to, count, coverage, proportion, capacity, n);
    });
 }
 

@@ -15,8 +15,10 @@
 //    ranges with at most 4095 distinct (position, symbol) "slots" and roughly equal payload
 //    bytes).  Tile (block b, chunk c) holds the block's containers in chunk c in two parts:
 //    light containers (at most 32 rows) flattened into u32 entries `slot << 16 | row`, sorted by
-//    slot then row and padded to a multiple of four with entries of the unused slot 4095; and
-//    heavy containers as descriptors (processed one per wavefront).  Besides the stored (non-local-reference)
+//    slot then row and padded to a multiple of four with entries of the unused slot 4095; larger
+//    arrays as groups of eight rows (16 bytes, unused places zero) with a u16 header per group
+//    `slot | (rows_in_group - 1) << 12`; and bitset and run containers as descriptors (processed
+//    one per wavefront).  Besides the stored (non-local-reference)
 //    symbols, four boundary pseudo-symbols hold, per position p, the rows whose covered range
 //    starts at p (p > 0), ends at p (p < L), and whose internal missing runs start / end at p —
 //    the vertical form of the reference's horizontal coverage index.
@@ -80,6 +82,9 @@ struct SeqColumnView {
    const uint32_t* block_slot_start;  // [n_blocks + 1]
    const uint32_t* light_ptr;         // [n_blocks * n_chunks + 1], multiples of 4
    const uint32_t* light;             // slot << 16 | row
+   const uint32_t* group_ptr;         // [n_blocks * n_chunks + 1] into groups / group_hdr
+   const uint16_t* groups;            // eight u16 rows per group (16-byte aligned)
+   const uint16_t* group_hdr;         // slot | (rows - 1) << 12
    const uint32_t* heavy_ptr;         // [n_blocks * n_chunks + 1] into vert.desc
    const uint32_t* slot_info;         // [n_slots] position | symbol << 16
    const uint32_t* pos_slot_ptr;      // [genome_length + 1]

@@ -220,6 +220,33 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                   }
                }
                {
+                  // groups of eight rows of the slots [c, d)
+                  const uint16_t* gh = col.group_hdr;
+                  uint32_t a = col.group_ptr[tile], b = col.group_ptr[tile + 1];
+                  while (a < b) {
+                     const uint32_t mid = (a + b) >> 1;
+                     if ((gh[mid] & 0xFFFu) < ins.c) a = mid + 1; else b = mid;
+                  }
+                  uint32_t e = a;
+                  b = col.group_ptr[tile + 1];
+                  while (e < b) {
+                     const uint32_t mid = (e + b) >> 1;
+                     if ((gh[mid] & 0xFFFu) < ins.d) e = mid + 1; else b = mid;
+                  }
+                  for (uint32_t k = a + lane; k < e; k += 64) {
+                     const uint32_t hdr = gh[k];
+                     const uint32_t sym = col.slot_info[slot_base + (hdr & 0xFFFu)] >> 16;
+                     if (!((ins.e >> sym) & 1u)) continue;
+                     const uint32_t m = (hdr >> 12) + 1;
+                     for (uint32_t q = 0; q < m; ++q) {
+                        const uint32_t row = col.groups[static_cast<size_t>(k) * 8 + q];
+                        if ((row >> 12) == slice) {
+                           atomicOr(reinterpret_cast<unsigned long long*>(&scratch[(row >> 6) & 63]), 1ULL << (row & 63));
+                        }
+                     }
+                  }
+               }
+               {
                   uint32_t lo = col.heavy_ptr[tile];
                   const uint32_t end = col.heavy_ptr[tile + 1];
                   uint32_t hi = end;
@@ -414,16 +441,25 @@ __global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqCol
       const uint32_t tile = b * n_chunks + c;
       const uint32_t l0 = col.light_ptr[tile], l1 = col.light_ptr[tile + 1];
       const uint32_t h0 = col.heavy_ptr[tile], h1 = col.heavy_ptr[tile + 1];
-      if (l0 == l1 && h0 == h1) continue;
-      const ulonglong2* src = reinterpret_cast<const ulonglong2*>(filter + static_cast<size_t>(c) * kChunkWords);
-      for (uint32_t j = lane; j < kChunkWords / 2; j += 64) {
-         const ulonglong2 w = src[j];
-         f[2 * j] = w.x;
-         f[2 * j + 1] = w.y;
+      const uint32_t g0 = col.group_ptr[tile], g1 = col.group_ptr[tile + 1];
+      if (l0 == l1 && h0 == h1 && g0 == g1) continue;
+      {
+         // stage the 8 KiB filter tile: eight independent 16-byte loads per lane
+         const ulonglong2* src = reinterpret_cast<const ulonglong2*>(filter + static_cast<size_t>(c) * kChunkWords);
+         ulonglong2 w[8];
+#pragma unroll
+         for (uint32_t u = 0; u < 8; ++u) w[u] = src[lane + 64 * u];
+#pragma unroll
+         for (uint32_t u = 0; u < 8; ++u) {
+            f[2 * (lane + 64 * u)] = w[u].x;
+            f[2 * (lane + 64 * u) + 1] = w[u].y;
+         }
       }
       waveSync();
-      // light entries, four per lane per step; consecutive entries of a slot are summed locally
+      // light entries: each lane walks a contiguous span of 16-byte groups, loading kUnroll groups
+      // before using them; consecutive entries of a slot are summed before one LDS atomic
       {
+         constexpr uint32_t kUnroll = 8;
          const uint4* e4 = reinterpret_cast<const uint4*>(col.light + l0);
          const uint32_t n4 = (l1 - l0) >> 2;
          uint32_t cur = kPadSlot, acc = 0;
@@ -437,16 +473,59 @@ __global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqCol
             }
             acc += static_cast<uint32_t>((f[row >> 6] >> (row & 63)) & 1ULL);
          };
-         for (uint32_t j = lane; j < n4; j += 64) {
-            const uint4 e = e4[j];
-            take(e.x);
-            take(e.y);
-            take(e.z);
-            take(e.w);
+         for (uint32_t base = lane * kUnroll; base < n4; base += 64 * kUnroll) {
+            uint4 e[kUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+               e[u] = base + u < n4 ? e4[base + u] : make_uint4(kPadSlot << 16, kPadSlot << 16, kPadSlot << 16, kPadSlot << 16);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+               take(e[u].x);
+               take(e[u].y);
+               take(e[u].z);
+               take(e[u].w);
+            }
          }
          if (acc) atomicAdd(&cnt[cur], acc);
       }
-      // heavy containers: one per wavefront
+      // larger arrays: groups of eight rows sharing one slot; lanes walk contiguous spans of groups
+      {
+         constexpr uint32_t kUnroll = 8;
+         const uint4* gr = reinterpret_cast<const uint4*>(col.groups) + g0;
+         const uint16_t* gh = col.group_hdr + g0;
+         const uint32_t ng = g1 - g0;
+         uint32_t cur = kPadSlot, acc = 0;
+         for (uint32_t base = lane * kUnroll; base < ng; base += 64 * kUnroll) {
+            uint4 rows[kUnroll];
+            uint32_t hdr[kUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+               const bool ok = base + u < ng;
+               rows[u] = ok ? gr[base + u] : make_uint4(0, 0, 0, 0);
+               hdr[u] = ok ? gh[base + u] : 0xFFFFu;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+               if (hdr[u] == 0xFFFFu) continue;
+               const uint32_t slot = hdr[u] & 0xFFFu;
+               const uint32_t m = (hdr[u] >> 12) + 1;
+               const uint32_t r[8] = {rows[u].x & 0xFFFFu, rows[u].x >> 16, rows[u].y & 0xFFFFu, rows[u].y >> 16,
+                                      rows[u].z & 0xFFFFu, rows[u].z >> 16, rows[u].w & 0xFFFFu, rows[u].w >> 16};
+               uint32_t n = 0;
+#pragma unroll
+               for (uint32_t q = 0; q < 8; ++q) n += q < m ? bitAt(f, r[q]) : 0u;
+               if (slot != cur) {
+                  if (acc) atomicAdd(&cnt[cur], acc);
+                  cur = slot;
+                  acc = 0;
+               }
+               acc += n;
+            }
+         }
+         if (acc) atomicAdd(&cnt[cur], acc);
+      }
+      // bitset and run containers: one per wavefront
       for (uint32_t d = h0; d < h1; ++d) {
          const Desc dd = desc[d];
          const uint32_t type = descType(dd.w0);
@@ -454,13 +533,25 @@ __global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqCol
          uint32_t n = 0;
          if (type == CT_BITSET) {
             const ulonglong2* bits = reinterpret_cast<const ulonglong2*>(col.vert.bits + static_cast<size_t>(dd.w1) * kChunkWords);
-            for (uint32_t j = lane; j < kChunkWords / 2; j += 64) {
-               const ulonglong2 w = bits[j];
-               n += __popcll(w.x & f[2 * j]) + __popcll(w.y & f[2 * j + 1]);
+            ulonglong2 w[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) w[u] = bits[lane + 64 * u];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+               const uint32_t j = lane + 64 * u;
+               n += __popcll(w[u].x & f[2 * j]) + __popcll(w[u].y & f[2 * j + 1]);
             }
          } else if (type == CT_ARRAY) {
             const uint16_t* a = col.vert.arr + dd.w1;
-            for (uint32_t j = lane; j < card; j += 64) n += bitAt(f, a[j]);
+            for (uint32_t j0 = lane; j0 < card; j0 += 256) {
+               uint32_t v[4];
+#pragma unroll
+               for (uint32_t u = 0; u < 4; ++u) v[u] = j0 + 64 * u < card ? a[j0 + 64 * u] : 0xFFFFFFFFu;
+#pragma unroll
+               for (uint32_t u = 0; u < 4; ++u) {
+                  if (v[u] != 0xFFFFFFFFu) n += bitAt(f, v[u]);
+               }
+            }
          } else if (type == CT_RUN) {
             const uint32_t* r = col.vert.runs + dd.w1;
             for (uint32_t j = lane; j < card; j += 64) {
@@ -681,6 +772,14 @@ __global__ void symbolScatterKernel(const SeqColumnView* __restrict__ colp, uint
       if (slot < slot_lo || slot >= slot_hi) continue;
       const uint32_t sym = col.slot_info[slot_base + slot] >> 16;
       if (sym < col.n_symbols) lab[x & 0xFFFFu] = sym;
+   }
+   for (uint32_t k = col.group_ptr[tile] + threadIdx.x; k < col.group_ptr[tile + 1]; k += blockDim.x) {
+      const uint32_t hdr = col.group_hdr[k];
+      const uint32_t slot = hdr & 0xFFFu;
+      if (slot < slot_lo || slot >= slot_hi) continue;
+      const uint32_t sym = col.slot_info[slot_base + slot] >> 16;
+      if (sym >= col.n_symbols) continue;
+      for (uint32_t q = 0; q <= (hdr >> 12); ++q) lab[col.groups[static_cast<size_t>(k) * 8 + q]] = sym;
    }
    for (uint32_t k = col.heavy_ptr[tile]; k < col.heavy_ptr[tile + 1]; ++k) {
       const Desc d = col.vert.desc[k];

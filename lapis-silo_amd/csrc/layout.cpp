@@ -560,8 +560,10 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    // Light containers (at most kLightMaxRows rows) become u32 entries slot << 16 | row; the rest stay
    // descriptors into the concatenated arenas.
    const size_t n_tiles = static_cast<size_t>(n_blocks) * t.n_chunks;
-   std::vector<uint32_t> light_ptr(n_tiles + 1, 0), heavy_ptr(n_tiles + 1, 0);
+   std::vector<uint32_t> light_ptr(n_tiles + 1, 0), heavy_ptr(n_tiles + 1, 0), group_ptr(n_tiles + 1, 0);
    std::vector<uint32_t> light;
+   std::vector<uint16_t> group_rows;  // 8 per group
+   std::vector<uint16_t> group_hdr;
    std::vector<Desc> desc;
    std::vector<size_t> cursor(t.n_chunks, 0);
    for (uint32_t b = 0; b < n_blocks; ++b) {
@@ -571,6 +573,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
          const size_t tile = static_cast<size_t>(b) * t.n_chunks + c;
          light_ptr[tile] = static_cast<uint32_t>(light.size());
          heavy_ptr[tile] = static_cast<uint32_t>(desc.size());
+         group_ptr[tile] = static_cast<uint32_t>(group_hdr.size());
          auto& ch = chunks[c];
          size_t& k = cursor[c];
          while (k < ch.key.size() && (ch.key[k] >> 8) < pend) {
@@ -597,6 +600,13 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
                      for (uint32_t r = (x & 0xFFFFu); r <= (x & 0xFFFFu) + (x >> 16); ++r) light.push_back(hi | r);
                   }
                }
+            } else if (type == CT_ARRAY) {
+               const uint16_t* a = ch.arenas.arr.data() + d.w1;
+               for (uint32_t j = 0; j < card; j += 8) {
+                  const uint32_t m = std::min<uint32_t>(8, card - j);
+                  for (uint32_t u = 0; u < 8; ++u) group_rows.push_back(u < m ? a[j + u] : 0);
+                  group_hdr.push_back(static_cast<uint16_t>(slot | ((m - 1) << 12)));
+               }
             } else {
                d.w0 = makeW0(slot, type, card);
                if (type == CT_ARRAY) d.w1 += static_cast<uint32_t>(arr_base[c]);
@@ -607,11 +617,14 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
             ++k;
          }
          while (light.size() % 4 != 0) light.push_back(kPadSlot << 16);
-         if (light.size() > 0xFFFFFFF0ULL || desc.size() > 0xFFFFFFF0ULL) fail(RDB_ENOMEM, "too many containers");
+         if (light.size() > 0xFFFFFFF0ULL || desc.size() > 0xFFFFFFF0ULL || group_hdr.size() > 0xFFFFFFF0ULL) {
+            fail(RDB_ENOMEM, "too many containers");
+         }
       }
    }
    light_ptr[n_tiles] = static_cast<uint32_t>(light.size());
    heavy_ptr[n_tiles] = static_cast<uint32_t>(desc.size());
+   group_ptr[n_tiles] = static_cast<uint32_t>(group_hdr.size());
 
    std::vector<uint16_t> arr_all(arr_base[t.n_chunks]);
    std::vector<uint32_t> run_all(run_base[t.n_chunks]);
@@ -627,7 +640,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
       std::vector<uint64_t>().swap(a.bits);
    });
    chunks.clear();
-   col.desc_bytes = desc.size() * sizeof(Desc) + light.size() * 4;
+   col.desc_bytes = desc.size() * sizeof(Desc) + light.size() * 4 + group_hdr.size() * 18;
 
    // --- row-level arrays (padded to 2^16 rows per chunk) ------------------------------------------
    const size_t padded = static_cast<size_t>(t.n_chunks) << 16;
@@ -680,6 +693,13 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
          for (uint32_t k = heavy_ptr[tile]; k < heavy_ptr[tile + 1]; ++k) {
             if (descSlot(desc[k].w0) >= ns) fail(RDB_EDEVICE, "internal: descriptor slot outside its block");
          }
+         if (group_ptr[tile] > group_ptr[tile + 1]) fail(RDB_EDEVICE, "internal: group pointers out of order");
+         for (uint32_t k = group_ptr[tile]; k < group_ptr[tile + 1]; ++k) {
+            if ((group_hdr[k] & 0xFFFu) >= ns) fail(RDB_EDEVICE, "internal: group slot outside its block");
+            if (k > group_ptr[tile] && (group_hdr[k] & 0xFFFu) < (group_hdr[k - 1] & 0xFFFu)) {
+               fail(RDB_EDEVICE, "internal: groups not sorted");
+            }
+         }
          for (uint32_t k = light_ptr[tile]; k < light_ptr[tile + 1]; ++k) {
             const uint32_t slot = light[k] >> 16;
             if (slot >= ns && slot != kPadSlot) fail(RDB_EDEVICE, "internal: light entry slot outside its block");
@@ -713,6 +733,9 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    v.light_ptr = static_cast<const uint32_t*>(up(light_ptr.data(), light_ptr.size() * 4));
    v.light = static_cast<const uint32_t*>(up(light.data(), light.size() * 4));
    v.heavy_ptr = static_cast<const uint32_t*>(up(heavy_ptr.data(), heavy_ptr.size() * 4));
+   v.group_ptr = static_cast<const uint32_t*>(up(group_ptr.data(), group_ptr.size() * 4));
+   v.groups = static_cast<const uint16_t*>(up(group_rows.data(), group_rows.size() * 2));
+   v.group_hdr = static_cast<const uint16_t*>(up(group_hdr.data(), group_hdr.size() * 2));
    v.slot_info = static_cast<const uint32_t*>(up(col.slot_info.data(), col.slot_info.size() * 4));
    v.pos_slot_ptr = static_cast<const uint32_t*>(up(col.pos_slot_ptr.data(), col.pos_slot_ptr.size() * 4));
    v.local_ref = static_cast<const uint8_t*>(up(col.local_ref.data(), L));
@@ -859,7 +882,8 @@ void finalizeTable(Table& t) {
    }
    t.slot_counts = DeviceBuffer(static_cast<size_t>(max_slots) * 4 * kMaxGroups);
    t.totals = DeviceBuffer(static_cast<size_t>((max_L + 1023) / 1024) * 3 * sizeof(int64_t));
-   t.mut_rows = DeviceBuffer(16 + static_cast<size_t>(max_L) * 32 * sizeof(MutationRowDev));
+   // at least the 2048 rows the Mutations path copies back in its first transfer
+   t.mut_rows = DeviceBuffer(16 + std::max<size_t>(static_cast<size_t>(max_L) * 32, 2048) * sizeof(MutationRowDev));
    t.mut_n = DeviceBuffer(64);
    t.finalized = true;
 }

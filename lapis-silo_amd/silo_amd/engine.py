@@ -40,6 +40,7 @@ SIGNATURES = [
     ("rdb_count", _int, [_vp, _cp, _vp]),
     ("rdb_mutation_counts", _int, [_vp, _vp, _cp, _vp]),
     ("rdb_mutations", _int, [_vp, _vp, _cp, _f64, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
+    ("rdb_mutations_query", _int, [_vp, _cp, _cp, _f64, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
     ("rdb_aggregate", _int, [_vp, _vp, _cp, _vp, _vp, _u64, _vp]),
     ("rdb_profile", _int, [_vp, _int]),
     ("rdb_profile_read", _int, [_vp, _vp, _vp, _vp, _vp]),
@@ -275,21 +276,36 @@ class DeviceTable:
             return out
         return self._with_bitmap(filt, run)
 
-    def mutations(self, filt, column: str, min_proportion: float) -> list:
+    def _row_buffers(self, column: str):
         cap = 32 * self.seq_lengths[column] + 16
-        pos = np.zeros(cap, np.uint32)
-        fr = np.zeros(cap, np.uint8)
-        to = np.zeros(cap, np.uint8)
-        cnt = np.zeros(cap, np.uint32)
-        cov = np.zeros(cap, np.uint32)
-        prop = np.zeros(cap, np.float64)
+        key = (column, cap)
+        bufs = getattr(self, "_bufs", {})
+        if key not in bufs:
+            bufs[key] = (np.zeros(cap, np.uint32), np.zeros(cap, np.uint8), np.zeros(cap, np.uint8),
+                         np.zeros(cap, np.uint32), np.zeros(cap, np.uint32), np.zeros(cap, np.float64))
+            self._bufs = bufs
+        return cap, bufs[key]
+
+    def mutations_query(self, expr: str, column: str, min_proportion: float) -> list:
+        """Filter + Mutations in one call (one device round trip)."""
+        cap, (pos, fr, to, cnt, cov, prop) = self._row_buffers(column)
+        n = _u64()
+        _check(lib().rdb_mutations_query(self._h, expr.encode(), column.encode(), float(min_proportion), _p(pos), _p(fr),
+                                         _p(to), _p(cnt), _p(cov), _p(prop), cap, ctypes.byref(n)), self._ctx())
+        k = n.value
+        return list(zip(pos[:k].tolist(), fr[:k].tolist(), to[:k].tolist(), cnt[:k].tolist(), cov[:k].tolist(),
+                        prop[:k].tolist()))
+
+    def mutations(self, filt, column: str, min_proportion: float) -> list:
+        cap, (pos, fr, to, cnt, cov, prop) = self._row_buffers(column)
         n = _u64()
 
         def run(b):
             _check(lib().rdb_mutations(self._h, b.handle, column.encode(), float(min_proportion), _p(pos), _p(fr), _p(to),
                                        _p(cnt), _p(cov), _p(prop), cap, ctypes.byref(n)), self._ctx())
             k = n.value
-            return [(int(pos[i]), int(fr[i]), int(to[i]), int(cnt[i]), int(cov[i]), float(prop[i])) for i in range(k)]
+            return list(zip(pos[:k].tolist(), fr[:k].tolist(), to[:k].tolist(), cnt[:k].tolist(), cov[:k].tolist(),
+                            prop[:k].tolist()))
         return self._with_bitmap(filt, run)
 
     def aggregate(self, filt, dims: list) -> list:

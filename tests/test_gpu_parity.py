@@ -57,7 +57,9 @@ def test_mutations(small):
     for e in exprs:
         assert np.array_equal(et.mutation_counts(e, "main"), ot.mutation_counts(e, "main")), e
         for p in (0.0, 0.05, 0.5, 1.0):
-            assert et.mutations(e, "main", p) == ot.mutations(e, "main", p), (e, p)
+            want = ot.mutations(e, "main", p)
+            assert et.mutations(e, "main", p) == want, (e, p)
+            assert et.mutations_query(e, "main", p) == want, (e, p)
 
 
 def test_aggregate(small):
