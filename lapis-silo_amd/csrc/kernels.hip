@@ -267,25 +267,42 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                break;
             }
             case OP_COVERED: {
+               // each lane builds its own word from its 64 consecutive rows
                const SeqColumnView& col = tv.seq[ins.a];
                const uint32_t p = ins.b;
+               const size_t r0 = row_base + static_cast<size_t>(lane) * 64;
+               const uint4* cv = reinterpret_cast<const uint4*>(col.cov + r0);
+               const uint4* pv = reinterpret_cast<const uint4*>(col.nrun_ptr + r0);
                uint64_t word = 0;
-               for (uint32_t k = 0; k < 64; ++k) {
-                  const size_t r = row_base + k * 64 + lane;
-                  const uint32_t c = col.cov[r];
-                  bool covered = (c & 0xFFFFu) <= p && p < (c >> 16);
-                  if (covered) {
-                     const uint32_t j1 = col.nrun_ptr[r + 1];
-                     for (uint32_t j = col.nrun_ptr[r]; j < j1; ++j) {
-                        const uint32_t x = col.nruns[j];
-                        if ((x & 0xFFFFu) <= p && p < (x >> 16)) {
-                           covered = false;
-                           break;
+#pragma unroll
+               for (uint32_t h = 0; h < 2; ++h) {
+                  uint4 c4[8], p4[8];
+#pragma unroll
+                  for (uint32_t u = 0; u < 8; ++u) {
+                     c4[u] = cv[8 * h + u];
+                     p4[u] = pv[8 * h + u];
+                  }
+                  const uint32_t last = col.nrun_ptr[r0 + 32 * h + 32];
+#pragma unroll
+                  for (uint32_t u = 0; u < 8; ++u) {
+                     const uint32_t cs[4] = {c4[u].x, c4[u].y, c4[u].z, c4[u].w};
+                     const uint32_t ps[5] = {p4[u].x, p4[u].y, p4[u].z, p4[u].w,
+                                             u + 1 < 8 ? p4[u + 1 < 8 ? u + 1 : 7].x : last};
+#pragma unroll
+                     for (uint32_t q = 0; q < 4; ++q) {
+                        bool covered = (cs[q] & 0xFFFFu) <= p && p < (cs[q] >> 16);
+                        if (covered) {
+                           for (uint32_t j = ps[q]; j < ps[q + 1]; ++j) {
+                              const uint32_t x = col.nruns[j];
+                              if ((x & 0xFFFFu) <= p && p < (x >> 16)) {
+                                 covered = false;
+                                 break;
+                              }
+                           }
                         }
+                        if (covered) word |= 1ULL << (32 * h + 4 * u + q);
                      }
                   }
-                  const uint64_t m = __ballot(covered);
-                  if (lane == k) word = m;
                }
                push(word & lmask);
                break;
@@ -293,14 +310,25 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
             case OP_CMP: {
                const IntColumnView& col = tv.ints[ins.a];
                const int32_t v = static_cast<int32_t>(ins.c);
+               const uint64_t nulls = col.null_bits[static_cast<size_t>(chunk) * kChunkWords + my_word];
+               const int4* vals = reinterpret_cast<const int4*>(col.values + row_base + static_cast<size_t>(lane) * 64);
+               const bool with_nulls = ins.d != 0;
                uint64_t word = 0;
-               for (uint32_t k = 0; k < 64; ++k) {
-                  const size_t r = row_base + k * 64 + lane;
-                  const uint64_t nb = col.null_bits[static_cast<size_t>(chunk) * kChunkWords + slice * 64 + k];
-                  const bool is_null = (nb >> lane) & 1ULL;
-                  const bool m = is_null ? (ins.d != 0) : compareInt(ins.b, col.values[r], v);
-                  const uint64_t b = __ballot(m);
-                  if (lane == k) word = b;
+#pragma unroll
+               for (uint32_t h = 0; h < 2; ++h) {
+                  int4 x[8];
+#pragma unroll
+                  for (uint32_t u = 0; u < 8; ++u) x[u] = vals[8 * h + u];
+#pragma unroll
+                  for (uint32_t u = 0; u < 8; ++u) {
+                     const int32_t xs[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+#pragma unroll
+                     for (uint32_t q = 0; q < 4; ++q) {
+                        const uint32_t bit = 32 * h + 4 * u + q;
+                        const bool m = ((nulls >> bit) & 1ULL) ? with_nulls : compareInt(ins.b, xs[q], v);
+                        word |= static_cast<uint64_t>(m) << bit;
+                     }
+                  }
                }
                push(word & lmask);
                break;
