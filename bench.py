@@ -1,18 +1,30 @@
 #!/usr/bin/env python3
 """Mutations-query benchmark on one MI355X.
 
-A step is one pass of the configuration's query workload (config 1: 64 Mutations queries with
-minProportion 0.05 over 4 M sequences x 29,903 nt; filters on date range, country or lineage with
-selectivities 1e-4..0.5).  Inputs are generated, uploaded and resident in HBM before timing; each
-query is filter + Mutations through the C ABI and returns its output rows to the host.  Prints one
-JSON line.  The CPU baseline is the oracle (a single-threaded restatement of the reference's CPU
-operators) timed on a bounded sample of the same queries, whose results are also compared with the
-engine's bit for bit.
+A step is one pass of the configuration's query workload over a table resident in HBM:
+
+* config 1 (default): 64 Mutations queries (minProportion 0.05) over 4 M sequences x 29,903 nt,
+  filters on date range, country or lineage with selectivities 1e-4..0.5;
+* config 2: 1,024 heterogeneous filter trees (depth 1-6 over And/Or/Not/NOf with sequence and
+  metadata leaves, exact selectivities log-uniform over 1e-6..0.5), alternately Mutations and
+  Aggregated by country, each also returning its cardinality;
+* config 0: the 100 k-row CPU-runnable case.
+
+Two paths run the same workload:
+
+* plain: one C-ABI call per query (filter kernel + action kernels + one host round trip);
+* batched: the whole step as one batch (rdb_batch_*): filters of all queries in one launch,
+  dense or sparse Mutations chosen per query on the device, HIP-graph replay, with the kernel
+  knobs picked by the autotuner beforehand (outside the timed region).
+
+`value` is the batched path's queries/s; the plain path's figures sit beside it.  Every query's
+batched result is compared with its plain result.  The CPU baseline is the oracle (a
+single-threaded restatement of the reference's CPU operators) on a bounded sample of the same
+queries, whose results are also compared with the engine's bit for bit.  Prints one JSON line.
 """
 import argparse
 import json
 import os
-import statistics
 import sys
 import time
 
@@ -21,6 +33,7 @@ sys.path.insert(0, os.path.join(ROOT, "lapis-silo_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "Mutations-query queries/s + p50/p99 latency @4M seqs, bit-exact; GPU vs CPU"
 
 
 def log(msg):
@@ -33,6 +46,30 @@ def percentile(xs, q):
     return s[k]
 
 
+def copy_calibration(mib):
+    """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel divided by mib * 1024 gives
+    the factor by which the counters' KiB relate to bytes moved."""
+    import torch
+
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copied {mib} MiB five times")
+
+
+def run_plain_query(et, q):
+    if q.get("action", "mutations") == "mutations":
+        return et.mutations_query(q["filter"], q.get("column", "main"), q["min_proportion"])
+    if q["action"] == "aggregate":
+        with et.bitmap(q["filter"]) as b:
+            return et.aggregate(b, q["dims"])
+    return et.count(q["filter"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,20 +79,25 @@ def main():
     ap.add_argument("--rows", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the per-query path (the batched one still runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (scales PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs on a single MI355X (no multi-GPU path); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
-    from silo_amd import datagen, table, workloads
+    from silo_amd import autotune, datagen, table, workloads
     from silo_amd.engine import Engine
 
     t0 = time.time()
     ds = datagen.generate_config(args.config, n_rows=args.rows)
     spec = table.spec_from_dataset(ds)
-    queries = workloads.config0_queries(ds) if args.config == 0 else workloads.config1_queries(ds)
-    log(f"generated {ds.n_rows} rows x {ds.genome_length} nt in {time.time() - t0:.1f}s; {len(queries)} queries")
+    log(f"generated {ds.n_rows} rows x {ds.genome_length} nt in {time.time() - t0:.1f}s")
 
     eng = Engine(0)
     t0 = time.time()
@@ -64,47 +106,95 @@ def main():
     st = et.stats("main")
     log(f"index built and uploaded in {upload_s:.1f}s: {st}")
 
-    def run_pass(record=None):
-        out = []
-        for q in queries:
-            t = time.perf_counter()
-            rows = et.mutations_query(q["filter"], "main", q["min_proportion"])
-            if record is not None:
-                record.append(time.perf_counter() - t)
-            out.append(rows)
-        return out
+    if args.config == 0:
+        queries = workloads.config0_queries(ds)
+    elif args.config == 2:
+        t0 = time.time()
 
-    for _ in range(args.warmup):
-        run_pass()
-    eng.profile(True)
-    lat = []
-    t_start = time.perf_counter()
-    results = None
-    for _ in range(args.steps):
-        results = run_pass(lat)
-    elapsed = time.perf_counter() - t_start
-    prof = eng.profile_read()
-    eng.profile(False)
-    n_q = args.steps * len(queries)
-    qps = n_q / elapsed
-    log(f"timed {n_q} queries in {elapsed:.3f}s: {qps:.1f} queries/s; kernel profile {prof}")
+        def count(filters):
+            out = []
+            for i in range(0, len(filters), 2048):
+                out += et.batch([{"filter": f, "action": "count"} for f in filters[i:i + 2048]])
+            return [c if isinstance(c, int) else -1 for c in out]
+        queries = workloads.config2_queries(ds, count)
+        log(f"config-2 workload: {len(queries)} trees calibrated in {time.time() - t0:.1f}s")
+    else:
+        queries = workloads.config1_queries(ds)
+    for q in queries:
+        q.setdefault("action", "mutations")
+        q.setdefault("column", "main")
+    n_mut = sum(q["action"] == "mutations" for q in queries)
+    log(f"{len(queries)} queries ({n_mut} Mutations)")
 
-    # algorithmic bytes per query (outside the timed region)
-    traffic = []
+    # ---- plain path: one C-ABI call per query -----------------------------------------------------
+    plain = None
+    plain_results = None
+    prof = None
+    if not args.no_plain:
+        def plain_pass(record=None):
+            out = []
+            for q in queries:
+                t = time.perf_counter()
+                out.append(run_plain_query(et, q))
+                if record is not None:
+                    record.append(time.perf_counter() - t)
+            return out
+
+        for _ in range(args.warmup):
+            plain_pass()
+        lat = []
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            plain_results = plain_pass(lat)
+        elapsed = time.perf_counter() - t_start
+        plain = {"queries_per_s": round(args.steps * len(queries) / elapsed, 3),
+                 "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+                 "p50_ms": round(percentile(lat, 50) * 1e3, 4), "p99_ms": round(percentile(lat, 99) * 1e3, 4)}
+        # kernel durations: HIP events around every dense Mutations counting and filter launch, in a
+        # pass of its own (the events make the host wait after each timed launch)
+        eng.profile(True)
+        plain_pass()
+        prof = eng.profile_read()
+        eng.profile(False)
+        log(f"plain path: {plain}; kernel profile {prof}")
+
+    # algorithmic bytes of the dense Mutations kernel (outside the timed regions)
+    traffic = 0
     cards = []
     for q in queries:
         with et.bitmap(q["filter"]) as b:
-            traffic.append(et.mutations_traffic(b, "main"))
+            if q["action"] == "mutations":
+                traffic += et.mutations_traffic(b, "main")
             cards.append(b.cardinality)
-    bytes_per_pass = sum(traffic)
-    mut_launch_bytes = bytes_per_pass * args.steps
-    achieved = (mut_launch_bytes / (prof["mutations_ms"] * 1e-3)) / 1e9 if prof["mutations_ms"] > 0 else 0.0
-    pmc = None
-    pmc_path = os.path.join(ROOT, "profiles", "mutations_count_pmc.json")
-    if os.path.exists(pmc_path):
-        with open(pmc_path) as f:
-            pmc = json.load(f).get("hbm_bytes_per_launch")
 
+    # ---- batched path: autotune, then the whole step as one batch ---------------------------------
+    t0 = time.time()
+    tuned = autotune.autotune(eng, et, autotune.probe_filters(ds), log=log) if ds.n_rows >= 1 << 20 else None
+    log(f"autotune {time.time() - t0:.1f}s")
+    traffic_tuned = 0  # Mutations bytes by the method each query takes under the tuned switch
+    for q in queries:
+        if q["action"] == "mutations":
+            with et.bitmap(q["filter"]) as b:
+                traffic_tuned += et.mutations_traffic(b, "main")
+    eng.set_tuning("batch_max_queries", min(max(len(queries), 1), 256))
+    for _ in range(max(args.warmup, 1)):
+        et.batch(queries)
+    blat = []
+    t_start = time.perf_counter()
+    batch_results = None
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        batch_results = et.batch(queries)
+        blat.append(time.perf_counter() - t)
+    b_elapsed = time.perf_counter() - t_start
+    b_qps = args.steps * len(queries) / b_elapsed
+    log(f"batched path: {b_qps:.1f} queries/s, {b_elapsed / args.steps * 1e3:.2f} ms per step")
+    identical_paths = None
+    if plain_results is not None:
+        identical_paths = all(a == b for a, b in zip(plain_results, batch_results))
+        log(f"batched results identical to plain: {identical_paths}")
+
+    # ---- CPU baseline ------------------------------------------------------------------------------
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -118,9 +208,15 @@ def main():
         identical = True
         t0 = time.perf_counter()
         while time.perf_counter() - t0 < args.cpu_seconds and done < len(queries):
-            q = queries[(done * 7) % len(queries)]
-            rows = ot.mutations(q["filter"], "main", q["min_proportion"])
-            identical &= rows == results[(done * 7) % len(queries)]
+            i = (done * 7) % len(queries)
+            q = queries[i]
+            if q["action"] == "mutations":
+                want = ot.mutations(q["filter"], "main", q["min_proportion"])
+            elif q["action"] == "aggregate":
+                want = ot.aggregate(q["filter"], q["dims"])
+            else:
+                want = ot.count(q["filter"])
+            identical &= want == batch_results[i]
             done += 1
         cpu_s = time.perf_counter() - t0
         cpu = {"value": round(done / cpu_s, 4), "unit": "queries/s", "cores": 1, "kind": "port",
@@ -130,42 +226,51 @@ def main():
         log(f"cpu baseline {cpu}; parity {parity}")
         ot.close()
 
+    pmc = None
+    pmc_path = os.path.join(ROOT, "profiles", "mutations_count_pmc.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as f:
+            pmc = json.load(f).get("hbm_bytes_per_launch")
+    roofline = None
+    if prof is not None and prof["mutations_launches"] > 0:
+        n_dense = prof["mutations_launches"]
+        per_launch = traffic / max(n_mut, 1)
+        achieved = per_launch * n_dense / (prof["mutations_ms"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": "mutationsCountKernel", "achieved": round(achieved, 2),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc,
+                    "algorithmic_bytes_per_launch_mean": int(per_launch), "launches": n_dense,
+                    "avg_launch_ms": round(prof["mutations_ms"] / n_dense, 5)}
+
     line = {
-        "metric": "Mutations-query queries/s + p50/p99 latency @4M seqs, bit-exact; GPU vs CPU",
-        "value": round(qps, 3),
+        "metric": METRIC,
+        "value": round(b_qps, 3),
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "ms_per_step": round(b_elapsed / args.steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": (f"config {args.config}: {ds.n_rows} sequences x {ds.genome_length} nt, "
-                         f"{len(queries)} Mutations(minProportion 0.05) queries per step"),
+            "workload": (f"config {args.config}: {ds.n_rows} sequences x {ds.genome_length} nt, {len(queries)} queries "
+                         f"per step ({n_mut} Mutations with minProportion 0.05)"),
             "rows": ds.n_rows,
             "genome_length": ds.genome_length,
             "queries_per_step": len(queries),
-            "path": "plain (per query: filter kernel + Mutations kernels, one host round trip)",
+            "path": "batched (one rdb_batch per step, autotuned knobs, HIP-graph replay)",
         },
-        "p50_ms": round(percentile(lat, 50) * 1e3, 4),
-        "p99_ms": round(percentile(lat, 99) * 1e3, 4),
-        "bitmap_GBps": round(mut_launch_bytes / elapsed / 1e9, 2),
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "mutationsCountKernel",
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": pmc,
-            "algorithmic_bytes_per_launch_mean": int(bytes_per_pass / len(queries)),
-            "launches": prof["mutations_launches"],
-            "avg_launch_ms": round(prof["mutations_ms"] / max(prof["mutations_launches"], 1), 5),
-        },
+        "p50_ms": round(percentile(blat, 50) * 1e3, 4),
+        "p99_ms": round(percentile(blat, 99) * 1e3, 4),
+        "latency_note": "batched path: a query's latency is its batch's submit-to-results time",
+        "plain": plain,
+        "batched_vs_plain": round(b_qps / plain["queries_per_s"], 3) if plain else None,
+        "batched_identical_to_plain": identical_paths,
+        "tuning": tuned,
+        "bitmap_GBps": round(traffic_tuned * args.steps / b_elapsed / 1e9, 2),
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,
         "upload_s": round(upload_s, 2),

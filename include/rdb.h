@@ -131,11 +131,60 @@ int rdb_aggregate(rdb_table* table, const rdb_bitmap* filter, const char* dims, 
 int rdb_profile(rdb_ctx* ctx, int enable);
 int rdb_profile_read(rdb_ctx* ctx, double* mutations_ms, uint64_t* mutations_launches, double* filter_ms,
                      uint64_t* filter_launches);
-/* Algorithmic bytes of one Mutations evaluation under `filter`: stored container bytes (array 2*card,
- * bitset 8192, run 4*runs, plus an 8-byte descriptor each) of the chunks the filter touches, 8192
- * per such chunk (filter tile), 8 per row and 4 per internal missing run of those chunks (coverage),
- * and 4 * n_symbols * genome_length (counts). */
+/* Algorithmic bytes of one Mutations evaluation under `filter` by the method the context's tuning
+ * selects.  Dense: stored container bytes (array 2*card, bitset 8192, run 4*runs, plus an 8-byte
+ * descriptor each) of the chunks the filter touches, 8192 per such chunk (filter tile), 8 per row
+ * and 4 per internal missing run of those chunks (coverage), and 4 * n_symbols * genome_length
+ * (counts).  Sparse: the filter words, 20 bytes per filter row plus 4 per mutation and per missing
+ * run of those rows, and the counts and boundary arrays read and cleared. */
 int rdb_mutations_traffic(rdb_table* table, const rdb_bitmap* filter, const char* column, uint64_t* bytes);
+
+/* ---- batched queries ------------------------------------------------------------------------- */
+/* No single reference counterpart: the reference runs one query per worker thread
+ * (src/rhydb/api/query_handler.cpp:38-99), each as computeFilter + the action's node.  A batch runs
+ * many such queries with one launch sequence per lane job; each query's results are those of the
+ * single-query entry points (rdb_count, rdb_mutations_query, rdb_aggregate) on the same filter. */
+enum rdb_action { RDB_ACTION_COUNT = 0, RDB_ACTION_MUTATIONS = 1, RDB_ACTION_AGGREGATE = 2 };
+
+typedef struct rdb_query {
+   const char* filter;     /* filter expression, as for rdb_filter */
+   int action;             /* rdb_action */
+   const char* column;     /* Mutations: sequence column */
+   double min_proportion;  /* Mutations: minProportion */
+   const char* dims;       /* Aggregated: comma-separated dictionary columns */
+} rdb_query;
+
+typedef struct rdb_batch rdb_batch;
+
+/* Compiles the n queries and enqueues them; returns without waiting for the GPU.  A query whose
+ * filter or action is invalid does not fail the batch: its getters return its own status. */
+int rdb_batch_submit(rdb_table* table, const rdb_query* queries, uint32_t n, rdb_batch** out);
+/* Waits for every job of the batch (the getters wait too). */
+int rdb_batch_wait(rdb_batch* batch);
+/* Cardinality of query q's filter (any action): CountFilterNode (count_filter_node.cpp:35-71). */
+int rdb_batch_count(rdb_batch* batch, uint32_t q, uint64_t* count);
+/* Rows of Mutations query q, as rdb_mutations. */
+int rdb_batch_mutations(rdb_batch* batch, uint32_t q, uint32_t* position, uint8_t* from, uint8_t* to,
+                        uint32_t* count, uint32_t* coverage, double* proportion, uint64_t capacity, uint64_t* n);
+/* Groups of Aggregated query q, as rdb_aggregate. */
+int rdb_batch_aggregate(rdb_batch* batch, uint32_t q, uint32_t* ids, uint64_t* counts, uint64_t capacity,
+                        uint64_t* n);
+void rdb_batch_free(rdb_batch* batch);
+
+/* ---- tuning ----------------------------------------------------------------------------------- */
+/* No reference counterpart: knobs of the engine's own kernels, set by hand or by the autotuner.
+ *   "sparse_max_rows"  Mutations under a filter of at most this many rows walk the rows' own
+ *                      mutations instead of streaming the index (-1, the default: never);
+ *   "mutation_groups"  chunk groups (partial sums) of the dense Mutations kernel, 0 = automatic;
+ *   "mutation_unroll"  16-byte loads in flight per lane in the dense Mutations kernel, 8 or 16;
+ *   "batch_max_queries" queries per lane job of the batched path (1..4096, default 256);
+ *   "batch_lanes"      lanes (HIP streams with their own buffers) per table (1..8, default 2);
+ *   "batch_graphs"     1: replay each lane's launch sequence from a captured HIP graph (default);
+ *   "batch_row_capacity" Mutations rows per query kept by a lane job (more: the query runs again
+ *                      alone), default 4096.
+ * Results never depend on them.  RDB_EINVAL for an unknown key or an out-of-range value. */
+int rdb_set_tuning(rdb_ctx* ctx, const char* key, int64_t value);
+int rdb_get_tuning(rdb_ctx* ctx, const char* key, int64_t* value);
 
 /* ---- ingest helper ---------------------------------------------------------------------------- */
 /* extractCoverageAndMutationsFromSequence (common/aligned_sequence.cpp:18-122) with offset 0.

@@ -1,0 +1,584 @@
+// Batched execution of filter + Count / Mutations / Aggregated queries (see batch.h).
+#include "batch.h"
+
+#include <algorithm>
+#include <cstring>
+
+#include "capi_internal.h"
+#include "query.h"
+
+namespace rdb {
+
+namespace {
+
+constexpr size_t kAlign = 256;
+size_t alignUp(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+// Layout of a lane's staged inputs (device and pinned host alike).
+struct ParamsLayout {
+   size_t header = 0;        // u32: n_queries, n_agg, total_bins, pad
+   size_t is_mutations = 0;  // u32 [max_queries]
+   size_t min_prop = 0;      // f64 [max_queries]
+   size_t agg_query = 0;     // u32 [max_queries]
+   size_t agg_bins = 0;      // u32 [max_queries]
+   size_t agg_offset = 0;    // u32 [max_queries]
+   size_t dims = 0;          // GroupDims [max_queries]
+   size_t progs = 0;         // ProgramDesc [max_queries]
+   size_t instrs = 0;        // Instr [instr_capacity]
+   size_t ranges = 0;        // u32 [range_capacity]
+   size_t bytes = 0;
+
+   ParamsLayout(uint32_t q, uint32_t instr_cap, uint32_t range_cap) {
+      size_t o = 0;
+      auto take = [&](size_t n) {
+         const size_t at = o;
+         o = alignUp(o + n);
+         return at;
+      };
+      header = take(16);
+      is_mutations = take(4ULL * q);
+      min_prop = take(8ULL * q);
+      agg_query = take(4ULL * q);
+      agg_bins = take(4ULL * q);
+      agg_offset = take(4ULL * q);
+      dims = take(sizeof(GroupDims) * q);
+      progs = take(sizeof(ProgramDesc) * q);
+      instrs = take(sizeof(Instr) * instr_cap);
+      ranges = take(4ULL * range_cap);
+      bytes = o;
+   }
+};
+
+// Layout of a lane's results: per-query header {n_rows, offset, card lo, card hi}, histogram
+// words, packed Mutations rows.
+struct OutLayout {
+   size_t header = 0;
+   size_t hist = 0;
+   size_t rows = 0;
+   size_t bytes = 0;
+   OutLayout(uint32_t q, uint32_t hist_cap, uint32_t row_cap) {
+      header = 0;
+      hist = alignUp(16ULL * q);
+      rows = alignUp(hist + 4ULL * hist_cap);
+      bytes = rows + sizeof(MutationRowDev) * static_cast<size_t>(q) * row_cap;
+   }
+};
+
+struct CompiledQuery {
+   query::Emitter program;
+   uint32_t depth = 1;
+};
+
+// Stack depth a lane's filter launch is sized for: rounded up to a multiple of 8 so that few graph
+// shapes cover all programs.
+uint32_t depthBucket(uint32_t depth, uint32_t max_depth) { return std::min((depth + 7) / 8 * 8, max_depth); }
+
+BatchLane& acquireLane(Table& t, int column);
+void retire(BatchLane& lane);
+
+void setupLane(Table& t, BatchLane& lane, int column) {
+   Context& ctx = *t.ctx;
+   lane.table = &t;
+   lane.column = column;
+   lane.max_queries = ctx.batch_max_queries;
+   lane.row_capacity = ctx.batch_row_capacity;
+   lane.instr_capacity = lane.max_queries * 256;
+   lane.range_capacity = lane.max_queries * 256;
+   lane.hist_capacity = std::max<uint32_t>(lane.max_queries * 64, 1u << 20);
+   hipCheck(hipStreamCreateWithFlags(&lane.stream, hipStreamNonBlocking), "hipStreamCreate");
+   hipCheck(hipEventCreateWithFlags(&lane.done, hipEventDisableTiming), "hipEventCreate");
+   const uint32_t Q = lane.max_queries;
+   const ParamsLayout pl(Q, lane.instr_capacity, lane.range_capacity);
+   const OutLayout ol(Q, lane.hist_capacity, lane.row_capacity);
+   lane.params_bytes = pl.bytes;
+   lane.out_bytes = ol.bytes;
+   lane.params = DeviceBuffer(pl.bytes);
+   lane.out = DeviceBuffer(ol.bytes);
+   hipCheck(hipHostMalloc(&lane.host_params, pl.bytes, hipHostMallocDefault), "pinned batch inputs");
+   hipCheck(hipHostMalloc(&lane.host_out, ol.bytes, hipHostMallocDefault), "pinned batch results");
+   const size_t words = static_cast<size_t>(t.n_chunks) * kChunkWords;
+   lane.filters = DeviceBuffer(words * 8 * Q);
+   lane.cards = DeviceBuffer(8ULL * Q + 4ULL * Q * t.n_chunks);
+   lane.lists = DeviceBuffer(8ULL * Q);
+   lane.list_counts = DeviceBuffer(16);
+   lane.n_rows = DeviceBuffer(4ULL * Q);
+   lane.rows = DeviceBuffer(sizeof(MutationRowDev) * static_cast<size_t>(Q) * lane.row_capacity);
+   if (column >= 0) {
+      const SeqColumn& col = *t.seq[column];
+      const size_t L = col.genome_length;
+      lane.partial = DeviceBuffer(4ULL * Q * col.view.n_slots);
+      lane.sparse_counts = DeviceBuffer(4ULL * Q * col.abc->count * L);
+      lane.sparse_bnd = DeviceBuffer(12ULL * Q * L);
+      lane.totals = DeviceBuffer(2ULL * Q * ((L + kMutPosBlock - 1) / kMutPosBlock) * 3 * sizeof(int64_t));
+      hipCheck(hipMemset(lane.sparse_counts.as<void>(), 0, lane.sparse_counts.bytes()), "clear sparse counts");
+      hipCheck(hipMemset(lane.sparse_bnd.as<void>(), 0, lane.sparse_bnd.bytes()), "clear sparse boundaries");
+   }
+}
+
+// The lane's fixed kernel sequence for programs of at most `depth` stack entries.
+void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth) {
+   Context& ctx = *t.ctx;
+   const uint32_t Q = lane.max_queries;
+   const ParamsLayout pl(Q, lane.instr_capacity, lane.range_capacity);
+   const OutLayout ol(Q, lane.hist_capacity, lane.row_capacity);
+   auto* pd = lane.params.as<uint8_t>();
+   auto* od = lane.out.as<uint8_t>();
+   const auto* n_queries = reinterpret_cast<const uint32_t*>(pd + pl.header);
+   auto* cards = lane.cards.as<unsigned long long>();
+   auto* chunk_cards = reinterpret_cast<uint32_t*>(lane.cards.as<uint8_t>() + 8ULL * Q);
+   hipCheck(launchFill(reinterpret_cast<uint32_t*>(cards), 0, lane.cards.bytes() / 4, lane.stream), "clear cardinalities");
+   hipCheck(launchFilter(t.view, reinterpret_cast<const ProgramDesc*>(pd + pl.progs), reinterpret_cast<const Instr*>(pd + pl.instrs),
+                         reinterpret_cast<const uint32_t*>(pd + pl.ranges), Q, depth, lane.filters.as<uint64_t>(), cards,
+                         chunk_cards, static_cast<uint32_t>(ctx.cu_count) * 8, lane.stream, n_queries),
+            "filter kernel");
+   if (lane.column >= 0) {
+      const SeqColumn& col = *t.seq[lane.column];
+      MutationsBatch m;
+      m.col_dev = t.seq_views_dev.as<SeqColumnView>() + lane.column;
+      m.col_host = &col.view;
+      m.chunk_sizes = t.chunk_sizes_dev.as<uint32_t>();
+      m.n_chunks = t.n_chunks;
+      m.max_queries = Q;
+      m.n_queries = n_queries;
+      m.filters = lane.filters.as<uint64_t>();
+      m.chunk_cards = chunk_cards;
+      m.cards = cards;
+      m.is_mutations = reinterpret_cast<const uint32_t*>(pd + pl.is_mutations);
+      m.min_proportion = reinterpret_cast<const double*>(pd + pl.min_prop);
+      m.sparse_max = ctx.sparse_max_rows;
+      m.unroll = ctx.mutation_unroll;
+      m.lists = lane.lists.as<uint32_t>();
+      m.list_counts = lane.list_counts.as<uint32_t>();
+      m.partial = lane.partial.as<uint32_t>();
+      m.sparse_counts = lane.sparse_counts.as<uint32_t>();
+      m.sparse_bnd = lane.sparse_bnd.as<int32_t>();
+      m.totals = lane.totals.as<int64_t>();
+      m.rows = lane.rows.as<MutationRowDev>();
+      m.row_capacity = lane.row_capacity;
+      m.n_rows = lane.n_rows.as<uint32_t>();
+      hipCheck(launchMutationsBatch(m, lane.stream), "batched Mutations kernels");
+   } else {
+      hipCheck(launchFill(lane.n_rows.as<uint32_t>(), 0, Q, lane.stream), "clear row counts");
+   }
+   AggregateBatch a;
+   a.n_chunks = t.n_chunks;
+   a.max_entries = Q;
+   a.n_agg = n_queries + 1;
+   a.total_bins = n_queries + 2;
+   a.max_total_bins = lane.hist_capacity;
+   a.agg_query = reinterpret_cast<const uint32_t*>(pd + pl.agg_query);
+   a.bins = reinterpret_cast<const uint32_t*>(pd + pl.agg_bins);
+   a.hist_offset = reinterpret_cast<const uint32_t*>(pd + pl.agg_offset);
+   a.dims = reinterpret_cast<const GroupDims*>(pd + pl.dims);
+   a.filters = lane.filters.as<uint64_t>();
+   a.chunk_cards = chunk_cards;
+   a.hist = reinterpret_cast<uint32_t*>(od + ol.hist);
+   hipCheck(launchAggregateBatch(a, lane.stream), "batched group histogram");
+   hipCheck(launchPackRows(n_queries, Q, lane.n_rows.as<uint32_t>(), cards, lane.rows.as<MutationRowDev>(), lane.row_capacity,
+                           reinterpret_cast<uint32_t*>(od + ol.header), reinterpret_cast<MutationRowDev*>(od + ol.rows),
+                           lane.stream),
+            "pack rows");
+}
+
+void launchLane(Table& t, BatchLane& lane, uint32_t depth) {
+   Context& ctx = *t.ctx;
+   const uint32_t bucket = depthBucket(depth, maxFilterDepth(ctx));
+   if (!ctx.batch_graphs) {
+      enqueueKernels(t, lane, bucket);
+      return;
+   }
+   const uint64_t key = static_cast<uint64_t>(bucket) | (static_cast<uint64_t>(ctx.mutation_unroll) << 16) |
+                        (static_cast<uint64_t>(ctx.sparse_max_rows + 1) << 24);
+   auto it = lane.graphs.find(key);
+   if (it == lane.graphs.end()) {
+      hipGraph_t graph = nullptr;
+      hipCheck(hipStreamBeginCapture(lane.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+      try {
+         enqueueKernels(t, lane, bucket);
+      } catch (...) {
+         (void)hipStreamEndCapture(lane.stream, &graph);
+         if (graph) (void)hipGraphDestroy(graph);
+         throw;
+      }
+      hipCheck(hipStreamEndCapture(lane.stream, &graph), "end capture");
+      hipGraphExec_t exec = nullptr;
+      const hipError_t err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      hipCheck(err, "graph instantiate");
+      it = lane.graphs.emplace(key, exec).first;
+   }
+   hipCheck(hipGraphLaunch(it->second, lane.stream), "graph launch");
+}
+
+struct JobPlan {
+   std::vector<uint32_t> queries;  // batch indices
+   uint32_t instrs = 0, ranges = 0, bins = 0;
+   int column = -1;
+};
+
+void enqueueJob(Table& t, BatchResult& b, const std::vector<CompiledQuery>& cq, const JobPlan& job) {
+   BatchLane& lane = acquireLane(t, job.column);
+   const uint32_t Q = lane.max_queries;
+   const ParamsLayout pl(Q, lane.instr_capacity, lane.range_capacity);
+   const OutLayout ol(Q, lane.hist_capacity, lane.row_capacity);
+   auto* hp = static_cast<uint8_t*>(lane.host_params);
+   auto* is_mut = reinterpret_cast<uint32_t*>(hp + pl.is_mutations);
+   auto* min_prop = reinterpret_cast<double*>(hp + pl.min_prop);
+   auto* agg_query = reinterpret_cast<uint32_t*>(hp + pl.agg_query);
+   auto* agg_bins = reinterpret_cast<uint32_t*>(hp + pl.agg_bins);
+   auto* agg_offset = reinterpret_cast<uint32_t*>(hp + pl.agg_offset);
+   auto* dims = reinterpret_cast<GroupDims*>(hp + pl.dims);
+   auto* progs = reinterpret_cast<ProgramDesc*>(hp + pl.progs);
+   auto* instrs = reinterpret_cast<Instr*>(hp + pl.instrs);
+   auto* ranges = reinterpret_cast<uint32_t*>(hp + pl.ranges);
+   uint32_t n_instr = 0, n_range = 0, n_agg = 0, total_bins = 0, depth = 1;
+   const uint32_t n = static_cast<uint32_t>(job.queries.size());
+   for (uint32_t slot = 0; slot < n; ++slot) {
+      const uint32_t qi = job.queries[slot];
+      const BatchQueryResult& r = b.queries[qi];
+      const query::Emitter& e = cq[qi].program;
+      progs[slot] = ProgramDesc{n_instr, static_cast<uint32_t>(e.code.size()), n_range, slot};
+      std::memcpy(instrs + n_instr, e.code.data(), e.code.size() * sizeof(Instr));
+      std::memcpy(ranges + n_range, e.ranges.data(), e.ranges.size() * 4);
+      n_instr += static_cast<uint32_t>(e.code.size());
+      n_range += static_cast<uint32_t>(e.ranges.size());
+      depth = std::max(depth, cq[qi].depth);
+      is_mut[slot] = r.status == RDB_OK && r.action == RDB_ACTION_MUTATIONS ? 1u : 0u;
+      min_prop[slot] = r.min_proportion;
+      if (r.status == RDB_OK && r.action == RDB_ACTION_AGGREGATE) {
+         const auto ds = capi::parseDims(t, r.dims.c_str());
+         GroupDims gd{};
+         gd.n = static_cast<uint32_t>(ds.size());
+         const uint64_t bins = capi::dimStrides(ds, gd.stride);
+         for (size_t k = 0; k < ds.size(); ++k) gd.labels[k] = ds[k].dict->labels.as<uint32_t>();
+         agg_query[n_agg] = slot;
+         agg_bins[n_agg] = static_cast<uint32_t>(bins);
+         agg_offset[n_agg] = total_bins;
+         dims[n_agg] = gd;
+         total_bins += static_cast<uint32_t>(bins);
+         ++n_agg;
+      }
+   }
+   auto* header = reinterpret_cast<uint32_t*>(hp + pl.header);
+   header[0] = n;
+   header[1] = n_agg;
+   header[2] = total_bins;
+   header[3] = 0;
+   lane.busy = true;
+   lane.owner = &b;
+   lane.job_queries = job.queries;
+   lane.job_total_bins = total_bins;
+   // header and histograms always; the packed rows up to 1 MiB (the rest, if any, on retirement)
+   lane.job_first_bytes = std::min(lane.out_bytes, ol.rows + (size_t{1} << 20));
+   b.pending.push_back(&lane);
+   hipCheck(hipMemcpyAsync(lane.params.as<void>(), hp, pl.bytes, hipMemcpyHostToDevice, lane.stream), "stage batch");
+   launchLane(t, lane, depth);
+   hipCheck(hipMemcpyAsync(lane.host_out, lane.out.as<void>(), lane.job_first_bytes, hipMemcpyDeviceToHost, lane.stream),
+            "copy batch results");
+   hipCheck(hipEventRecord(lane.done, lane.stream), "hipEventRecord");
+}
+
+BatchLane& acquireLane(Table& t, int column) {
+   Context& ctx = *t.ctx;
+   for (auto& l : t.lanes) {
+      if (!l->busy && l->column == column && l->max_queries == ctx.batch_max_queries &&
+          l->row_capacity == ctx.batch_row_capacity) {
+         return *l;
+      }
+   }
+   // drop idle lanes of another shape, then create one if the budget allows
+   t.lanes.erase(std::remove_if(t.lanes.begin(), t.lanes.end(),
+                                [&](const std::shared_ptr<BatchLane>& l) {
+                                   return !l->busy && (l->column != column || l->max_queries != ctx.batch_max_queries ||
+                                                       l->row_capacity != ctx.batch_row_capacity);
+                                }),
+                 t.lanes.end());
+   if (t.lanes.size() < std::max<uint32_t>(ctx.batch_lanes, 1)) {
+      auto lane = std::make_shared<BatchLane>();
+      setupLane(t, *lane, column);
+      t.lanes.push_back(lane);
+      return *lane;
+   }
+   // every lane is busy: retire the one whose job was enqueued first
+   BatchLane* oldest = t.lanes.front().get();
+   hipCheck(hipEventSynchronize(oldest->done), "batch wait");
+   retire(*oldest);
+   return acquireLane(t, column);
+}
+
+// Copies a finished job's results into its batch and frees the lane.
+void retire(BatchLane& lane) {
+   if (!lane.busy) return;
+   Table& t = *lane.table;
+   BatchResult& b = *lane.owner;
+   const uint32_t Q = lane.max_queries;
+   const OutLayout ol(Q, lane.hist_capacity, lane.row_capacity);
+   auto* ho = static_cast<uint8_t*>(lane.host_out);
+   const auto* header = reinterpret_cast<const uint32_t*>(ho + ol.header);
+   const uint32_t n = static_cast<uint32_t>(lane.job_queries.size());
+   // rows beyond the first copy
+   size_t kept = 0;
+   for (uint32_t s = 0; s < n; ++s) kept += std::min(header[4 * s], lane.row_capacity);
+   const size_t rows_bytes = kept * sizeof(MutationRowDev);
+   if (ol.rows + rows_bytes > lane.job_first_bytes) {
+      hipCheck(hipMemcpyAsync(ho + lane.job_first_bytes, lane.out.as<uint8_t>() + lane.job_first_bytes,
+                              ol.rows + rows_bytes - lane.job_first_bytes, hipMemcpyDeviceToHost, lane.stream),
+               "copy batch rows");
+      hipCheck(hipStreamSynchronize(lane.stream), "synchronize");
+   }
+   const auto* packed = reinterpret_cast<const MutationRowDev*>(ho + ol.rows);
+   const auto* hist = reinterpret_cast<const uint32_t*>(ho + ol.hist);
+   uint32_t agg_offset = 0;
+   std::vector<uint32_t> overflow;
+   for (uint32_t s = 0; s < n; ++s) {
+      BatchQueryResult& r = b.queries[lane.job_queries[s]];
+      if (r.status != RDB_OK) continue;
+      r.cardinality = static_cast<uint64_t>(header[4 * s + 2]) | (static_cast<uint64_t>(header[4 * s + 3]) << 32);
+      if (r.action == RDB_ACTION_MUTATIONS) {
+         const uint32_t nr = header[4 * s];
+         if (nr > lane.row_capacity) {
+            overflow.push_back(lane.job_queries[s]);
+            continue;
+         }
+         r.rows.assign(packed + header[4 * s + 1], packed + header[4 * s + 1] + nr);
+         capi::sortMutationRows(r.rows);
+      } else if (r.action == RDB_ACTION_AGGREGATE) {
+         const auto ds = capi::parseDims(t, r.dims.c_str());
+         uint32_t stride[4] = {};
+         const uint64_t bins = capi::dimStrides(ds, stride);
+         if (ds.empty()) {
+            // a single empty combination with |F| (bitmap_aggregation_node.cpp:91-101)
+            r.agg_counts.assign(1, r.cardinality);
+         } else {
+            capi::decodeHistogram(ds, stride, hist + agg_offset, bins, r.agg_ids, r.agg_counts);
+         }
+         agg_offset += static_cast<uint32_t>(bins);
+      }
+   }
+   lane.busy = false;
+   lane.owner = nullptr;
+   lane.job_queries.clear();
+   b.pending.erase(std::remove(b.pending.begin(), b.pending.end(), &lane), b.pending.end());
+   // queries whose rows did not fit the lane's capacity run again on the single-query path
+   for (uint32_t qi : overflow) {
+      BatchQueryResult& r = b.queries[qi];
+      DeviceBitmap f = query::compileFilter(t, r.filter)->evaluate();
+      r.rows = capi::mutationRows(t, f, r.column, r.min_proportion);
+   }
+}
+
+}  // namespace
+
+BatchLane::~BatchLane() {
+   if (stream) (void)hipStreamSynchronize(stream);
+   for (auto& [k, g] : graphs) (void)hipGraphExecDestroy(g);
+   if (done) (void)hipEventDestroy(done);
+   if (host_params) (void)hipHostFree(host_params);
+   if (host_out) (void)hipHostFree(host_out);
+   if (stream) (void)hipStreamDestroy(stream);
+}
+
+void batchSubmit(Table& t, BatchResult& b) {
+   Context& ctx = *t.ctx;
+   const uint32_t n = static_cast<uint32_t>(b.queries.size());
+   std::vector<CompiledQuery> cq(n);
+   // host side: parse, rewrite and compile every filter; a query that fails keeps its status and
+   // runs as the empty program
+   for (uint32_t i = 0; i < n; ++i) {
+      BatchQueryResult& r = b.queries[i];
+      try {
+         if (r.action == RDB_ACTION_MUTATIONS) {
+            r.column = capi::mutationColumn(t, r.column_name.c_str(), r.min_proportion);
+         } else if (r.action == RDB_ACTION_AGGREGATE) {
+            for (const auto& d : capi::parseDims(t, r.dims.c_str())) {
+               if (d.is_seq) fail(RDB_EQUERY, "batched Aggregated queries group by dictionary columns only");
+            }
+         } else if (r.action != RDB_ACTION_COUNT) {
+            fail(RDB_EINVAL, "unknown action " + std::to_string(r.action));
+         }
+         auto op = query::compileFilter(t, r.filter);
+         op->emit(cq[i].program);
+         if (cq[i].program.sp != 1) fail(RDB_ECOMPILE, "internal: filter program leaves a wrong stack");
+         cq[i].depth = std::max<uint32_t>(cq[i].program.max_sp, 1);
+         if (cq[i].depth > maxFilterDepth(ctx)) fail(RDB_EQUERY, "filter expression nests too deeply");
+      } catch (const Error& e) {
+         r.status = e.code;
+         r.message = e.what();
+      }
+      if (r.status != RDB_OK) {
+         cq[i].program = query::Emitter();
+         cq[i].program.leaf(Instr{OP_EMPTY, 0, 0, 0, 0, 0, 0, 0});
+         cq[i].depth = 1;
+      }
+   }
+   // cut into lane jobs: one Mutations column per job, capacities of the lane
+   const uint32_t max_q = ctx.batch_max_queries;
+   const uint32_t instr_cap = max_q * 256, range_cap = max_q * 256;
+   const uint32_t hist_cap = std::max<uint32_t>(max_q * 64, 1u << 20);
+   std::vector<JobPlan> jobs;
+   JobPlan cur;
+   auto flush = [&] {
+      if (!cur.queries.empty()) jobs.push_back(std::move(cur));
+      cur = JobPlan();
+   };
+   for (uint32_t i = 0; i < n; ++i) {
+      const BatchQueryResult& r = b.queries[i];
+      const uint32_t ni = static_cast<uint32_t>(cq[i].program.code.size());
+      const uint32_t nr = static_cast<uint32_t>(cq[i].program.ranges.size());
+      uint32_t bins = 0;
+      if (r.status == RDB_OK && r.action == RDB_ACTION_AGGREGATE) {
+         uint32_t stride[4];
+         bins = static_cast<uint32_t>(capi::dimStrides(capi::parseDims(t, r.dims.c_str()), stride));
+      }
+      if (ni > instr_cap || nr > range_cap || bins > hist_cap) {
+         b.queries[i].status = RDB_EQUERY;
+         b.queries[i].message = "query too large for the batched path";
+         continue;
+      }
+      const int col = r.status == RDB_OK && r.action == RDB_ACTION_MUTATIONS ? r.column : -1;
+      if (cur.queries.size() == max_q || cur.instrs + ni > instr_cap || cur.ranges + nr > range_cap ||
+          cur.bins + bins > hist_cap || (col >= 0 && cur.column >= 0 && col != cur.column)) {
+         flush();
+      }
+      if (col >= 0) cur.column = col;
+      cur.queries.push_back(i);
+      cur.instrs += ni;
+      cur.ranges += nr;
+      cur.bins += bins;
+   }
+   flush();
+   for (const auto& job : jobs) enqueueJob(t, b, cq, job);
+}
+
+void batchWait(BatchResult& b) {
+   Context& ctx = *b.table->ctx;
+   while (!b.pending.empty()) {
+      BatchLane* lane = b.pending.front();
+      hipEvent_t ev = lane->done;
+      ctx.mu.unlock();
+      const hipError_t err = hipEventSynchronize(ev);
+      ctx.mu.lock();
+      hipCheck(err, "batch wait");
+      if (lane->busy && lane->owner == &b) retire(*lane);
+   }
+}
+
+}  // namespace rdb
+
+struct rdb_batch {
+   rdb::BatchResult b;
+};
+
+using rdb::capi::guarded;
+using rdb::capi::requireFinalized;
+using rdb::capi::tableOf;
+
+namespace {
+
+// The results of query q after every job of the batch has finished; a query that failed returns
+// its own status and message.
+rdb::BatchQueryResult& batchQuery(rdb_batch* batch, uint32_t q) {
+   if (!batch) rdb::fail(RDB_EINVAL, "null batch");
+   rdb::batchWait(batch->b);
+   if (q >= batch->b.queries.size()) rdb::fail(RDB_EINVAL, "query index out of range");
+   rdb::BatchQueryResult& r = batch->b.queries[q];
+   if (r.status != RDB_OK) rdb::fail(r.status, r.message);
+   return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rdb_batch_submit(rdb_table* table, const rdb_query* queries, uint32_t n, rdb_batch** out) {
+   rdb::Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      rdb::Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireFinalized(t);
+      if (!out || (n && !queries)) rdb::fail(RDB_EINVAL, "null argument");
+      auto batch = std::make_unique<rdb_batch>();
+      batch->b.table = &t;
+      batch->b.queries.resize(n);
+      for (uint32_t i = 0; i < n; ++i) {
+         rdb::BatchQueryResult& r = batch->b.queries[i];
+         const rdb_query& q = queries[i];
+         r.action = q.action;
+         r.filter = q.filter ? q.filter : "";
+         r.column_name = q.column ? q.column : "";
+         r.min_proportion = q.min_proportion;
+         r.dims = q.dims ? q.dims : "";
+         if (!q.filter) {
+            r.status = RDB_EINVAL;
+            r.message = "null filter";
+         }
+      }
+      rdb::batchSubmit(t, batch->b);
+      *out = batch.release();
+   });
+}
+
+int rdb_batch_wait(rdb_batch* batch) {
+   rdb::Context* ctx = batch && batch->b.table ? batch->b.table->ctx : nullptr;
+   return guarded(ctx, [&] {
+      if (!batch) rdb::fail(RDB_EINVAL, "null batch");
+      std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+      rdb::batchWait(batch->b);
+   });
+}
+
+int rdb_batch_count(rdb_batch* batch, uint32_t q, uint64_t* count) {
+   rdb::Context* ctx = batch && batch->b.table ? batch->b.table->ctx : nullptr;
+   return guarded(ctx, [&] {
+      if (!batch || !count) rdb::fail(RDB_EINVAL, "null argument");
+      std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+      *count = batchQuery(batch, q).cardinality;
+   });
+}
+
+int rdb_batch_mutations(rdb_batch* batch, uint32_t q, uint32_t* position, uint8_t* from, uint8_t* to, uint32_t* count,
+                        uint32_t* coverage, double* proportion, uint64_t capacity, uint64_t* n) {
+   rdb::Context* ctx = batch && batch->b.table ? batch->b.table->ctx : nullptr;
+   return guarded(ctx, [&] {
+      if (!batch || !n) rdb::fail(RDB_EINVAL, "null argument");
+      std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+      rdb::BatchQueryResult& r = batchQuery(batch, q);
+      if (r.action != RDB_ACTION_MUTATIONS) rdb::fail(RDB_EINVAL, "query is not a Mutations query");
+      rdb::capi::emitMutationRows(*batch->b.table->seq[r.column], r.rows, position, from, to, count, coverage,
+                                  proportion, capacity, n);
+   });
+}
+
+int rdb_batch_aggregate(rdb_batch* batch, uint32_t q, uint32_t* ids, uint64_t* counts, uint64_t capacity, uint64_t* n) {
+   rdb::Context* ctx = batch && batch->b.table ? batch->b.table->ctx : nullptr;
+   return guarded(ctx, [&] {
+      if (!batch || !n) rdb::fail(RDB_EINVAL, "null argument");
+      std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+      rdb::BatchQueryResult& r = batchQuery(batch, q);
+      if (r.action != RDB_ACTION_AGGREGATE) rdb::fail(RDB_EINVAL, "query is not an Aggregated query");
+      *n = r.agg_counts.size();
+      if (r.agg_counts.size() > capacity) rdb::fail(RDB_ENOMEM, "output capacity too small");
+      if (!r.agg_ids.empty()) std::copy(r.agg_ids.begin(), r.agg_ids.end(), ids);
+      std::copy(r.agg_counts.begin(), r.agg_counts.end(), counts);
+   });
+}
+
+void rdb_batch_free(rdb_batch* batch) {
+   if (!batch) return;
+   if (batch->b.table) {
+      rdb::Context* ctx = batch->b.table->ctx;
+      std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+      try {
+         rdb::batchWait(batch->b);
+      } catch (...) {
+         // the lanes stay valid; their results are dropped with the batch
+         for (rdb::BatchLane* l : batch->b.pending) {
+            l->busy = false;
+            l->owner = nullptr;
+         }
+      }
+   }
+   delete batch;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Batched execution: many queries evaluated by one launch sequence on one of a table's lanes.
+//
+// A lane is a HIP stream plus device buffers sized for `max_queries` queries (filter words,
+// cardinalities, Mutations scratch, packed output) and pinned host buffers for the staged inputs
+// and the results.  A batch is cut into lane jobs; a job is staged with one host-to-device copy,
+// run as one fixed kernel sequence (captured once per lane and shape into a HIP graph and replayed
+// afterwards), and brought back with one device-to-host copy.  Lanes run concurrently on their own
+// streams, so jobs submitted while others are on the GPU overlap with them.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "kernels.h"
+
+namespace rdb {
+
+struct BatchResult;
+
+struct BatchLane {
+   Table* table = nullptr;
+   int column = -1;  // sequence column of the lane's Mutations scratch (-1: none)
+   hipStream_t stream = nullptr;
+   hipEvent_t done = nullptr;
+   uint32_t max_queries = 0;
+   uint32_t row_capacity = 0;
+   uint32_t instr_capacity = 0;
+   uint32_t range_capacity = 0;
+   uint32_t hist_capacity = 0;
+   size_t params_bytes = 0;
+   size_t out_bytes = 0;
+   DeviceBuffer params, filters, cards, lists, list_counts, partial, sparse_counts, sparse_bnd, totals, rows, n_rows, out;
+   void* host_params = nullptr;
+   void* host_out = nullptr;
+   std::map<uint64_t, hipGraphExec_t> graphs;  // by (depth bucket, sparse switch, unroll)
+
+   // the job on the lane (busy until retired)
+   bool busy = false;
+   BatchResult* owner = nullptr;
+   std::vector<uint32_t> job_queries;  // batch query index of each lane slot
+   uint32_t job_total_bins = 0;
+   size_t job_first_bytes = 0;
+
+   BatchLane() = default;
+   BatchLane(const BatchLane&) = delete;
+   BatchLane& operator=(const BatchLane&) = delete;
+   ~BatchLane();
+};
+
+// Results of one query of a batch.
+struct BatchQueryResult {
+   int status = RDB_OK;
+   std::string message;
+   int action = 0;
+   std::string column_name;
+   int column = -1;
+   double min_proportion = 0.0;
+   std::string filter;
+   std::string dims;
+   uint64_t cardinality = 0;
+   std::vector<MutationRowDev> rows;
+   std::vector<uint32_t> agg_ids;
+   std::vector<uint64_t> agg_counts;
+};
+
+struct BatchResult {
+   Table* table = nullptr;
+   std::vector<BatchQueryResult> queries;
+   std::vector<BatchLane*> pending;  // lanes holding this batch's unretired jobs
+};
+
+// Compiles and enqueues the queries; results appear in `out` once every pending job is retired.
+void batchSubmit(Table& t, BatchResult& out);
+// Waits for and retires every pending job of `b`.
+void batchWait(BatchResult& b);
+
+}  // namespace rdb

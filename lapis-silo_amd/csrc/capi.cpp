@@ -4,6 +4,7 @@
 #include <cstring>
 #include <string>
 
+#include "capi_internal.h"
 #include "engine.h"
 #include "query.h"
 
@@ -17,6 +18,14 @@ void Context::init(int dev) {
    hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
    hipDeviceProp_t prop{};
    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) cu_count = prop.multiProcessorCount;
+   int lds = 0;
+   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds >= 64 * 1024) {
+      max_lds_bytes = static_cast<size_t>(lds);
+   }
+   if (configureFilterKernel(max_lds_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      max_lds_bytes = 64 * 1024;
+   }
    hipCheck(hipHostMalloc(&stage_host, kStageSlots * kStageSlotBytes, hipHostMallocDefault), "pinned staging");
    stage_dev = DeviceBuffer(kStageSlots * kStageSlotBytes);
    for (auto& e : stage_events) hipCheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
@@ -63,40 +72,20 @@ void* Context::stage(const void* host, size_t bytes) {
    return d;
 }
 
+uint32_t maxFilterDepth(const Context& ctx) {
+   uint32_t d = 1;
+   while (filterLdsBytes(d + 1) <= ctx.max_lds_bytes) ++d;
+   return d;
+}
+
 }  // namespace rdb
 
 namespace {
 
-thread_local std::string g_error;
-
-template <typename F>
-int guarded(Context* ctx, F&& f) {
-   try {
-      f();
-      return RDB_OK;
-   } catch (const Error& e) {
-      g_error = e.what();
-      if (ctx) ctx->error = e.what();
-      return e.code;
-   } catch (const std::bad_alloc&) {
-      g_error = "host allocation failed";
-      if (ctx) ctx->error = g_error;
-      return RDB_ENOMEM;
-   } catch (const std::exception& e) {
-      g_error = e.what();
-      if (ctx) ctx->error = e.what();
-      return RDB_EINVAL;
-   }
-}
-
-Table& tableOf(rdb_table* t) {
-   if (!t) fail(RDB_EINVAL, "null table");
-   return t->t;
-}
-
-void requireFinalized(const Table& t) {
-   if (!t.finalized) fail(RDB_EINVAL, "table is not finalized (call rdb_table_finalize)");
-}
+using capi::g_error;
+using capi::guarded;
+using capi::requireFinalized;
+using capi::tableOf;
 
 void requireOpen(const Table& t) {
    if (t.finalized) fail(RDB_EINVAL, "table is already finalized");
@@ -115,8 +104,31 @@ const DeviceBitmap& filterOf(const Table& t, const rdb_bitmap* f) {
 // Chunk groups of the Mutations kernel: a group per four chunks (one chunk per wavefront of the
 // work-group), at most kMaxGroups partial sums.
 uint32_t mutationGroups(const Table& t, const SeqColumn& /*col*/) {
+   if (t.ctx->mutation_groups) return std::clamp<uint32_t>(t.ctx->mutation_groups, 1u, kMaxGroups);
    const uint32_t g = (t.n_chunks + 3) / 4;
    return std::clamp<uint32_t>(g, 1u, kMaxGroups);
+}
+
+// Launch description of the Mutations kernels for column `idx` under filter `f`, with the
+// context's dense/sparse switch and the table's scratch buffers.
+MutationsLaunch mutationsLaunch(Table& t, int idx, const DeviceBitmap& f) {
+   SeqColumn& col = *t.seq[idx];
+   MutationsLaunch m;
+   m.col_dev = t.seq_views_dev.as<SeqColumnView>() + idx;
+   m.col_host = &col.view;
+   m.chunk_sizes = t.chunk_sizes_dev.as<uint32_t>();
+   m.n_chunks = t.n_chunks;
+   m.n_groups = mutationGroups(t, col);
+   m.unroll = t.ctx->mutation_unroll;
+   m.filter = f.words;
+   m.filter_chunk_cards = f.chunk_cards;
+   m.filter_card = f.card_dev;
+   m.sparse_max = t.ctx->sparse_max_rows;
+   m.partial = t.slot_counts.as<uint32_t>();
+   m.sparse_counts = col.sparse_counts.as<uint32_t>();
+   m.sparse_bnd = col.sparse_bnd.as<int32_t>();
+   m.totals = t.totals.as<int64_t>();
+   return m;
 }
 
 }  // namespace
@@ -165,6 +177,61 @@ int rdb_table_create(rdb_ctx* ctx, const uint32_t* chunk_sizes, uint32_t n_chunk
       tb.n_chunks = n_chunks;
       tb.num_rows = off;
       *out = t;
+   });
+}
+
+int rdb_set_tuning(rdb_ctx* ctx, const char* key, int64_t value) {
+   return guarded(ctx ? &ctx->c : nullptr, [&] {
+      if (!ctx || !key) fail(RDB_EINVAL, "null argument");
+      std::lock_guard<std::recursive_mutex> lock(ctx->c.mu);
+      const std::string k(key);
+      if (k == "sparse_max_rows") {
+         ctx->c.sparse_max_rows = value < 0 ? -1 : static_cast<long long>(value);
+      } else if (k == "mutation_groups") {
+         if (value < 0 || value > kMaxGroups) fail(RDB_EINVAL, "mutation_groups must be in [0, 16]");
+         ctx->c.mutation_groups = static_cast<uint32_t>(value);
+      } else if (k == "mutation_unroll") {
+         if (value != 8 && value != 16) fail(RDB_EINVAL, "mutation_unroll must be 8 or 16");
+         ctx->c.mutation_unroll = static_cast<uint32_t>(value);
+      } else if (k == "batch_max_queries") {
+         if (value < 1 || value > 4096) fail(RDB_EINVAL, "batch_max_queries must be in [1, 4096]");
+         ctx->c.batch_max_queries = static_cast<uint32_t>(value);
+      } else if (k == "batch_lanes") {
+         if (value < 1 || value > 8) fail(RDB_EINVAL, "batch_lanes must be in [1, 8]");
+         ctx->c.batch_lanes = static_cast<uint32_t>(value);
+      } else if (k == "batch_graphs") {
+         ctx->c.batch_graphs = value != 0;
+      } else if (k == "batch_row_capacity") {
+         if (value < 16 || value > (1 << 20)) fail(RDB_EINVAL, "batch_row_capacity must be in [16, 2^20]");
+         ctx->c.batch_row_capacity = static_cast<uint32_t>(value);
+      } else {
+         fail(RDB_EINVAL, "unknown tuning key '" + k + "'");
+      }
+   });
+}
+
+int rdb_get_tuning(rdb_ctx* ctx, const char* key, int64_t* value) {
+   return guarded(ctx ? &ctx->c : nullptr, [&] {
+      if (!ctx || !key || !value) fail(RDB_EINVAL, "null argument");
+      std::lock_guard<std::recursive_mutex> lock(ctx->c.mu);
+      const std::string k(key);
+      if (k == "sparse_max_rows") {
+         *value = ctx->c.sparse_max_rows;
+      } else if (k == "mutation_groups") {
+         *value = ctx->c.mutation_groups;
+      } else if (k == "mutation_unroll") {
+         *value = ctx->c.mutation_unroll;
+      } else if (k == "batch_max_queries") {
+         *value = ctx->c.batch_max_queries;
+      } else if (k == "batch_lanes") {
+         *value = ctx->c.batch_lanes;
+      } else if (k == "batch_graphs") {
+         *value = ctx->c.batch_graphs;
+      } else if (k == "batch_row_capacity") {
+         *value = ctx->c.batch_row_capacity;
+      } else {
+         fail(RDB_EINVAL, "unknown tuning key '" + k + "'");
+      }
    });
 }
 
@@ -373,10 +440,10 @@ int rdb_mutation_counts(rdb_table* table, const rdb_bitmap* filter, const char* 
       const size_t bytes = static_cast<size_t>(col.abc->count) * col.genome_length * 4;
       void* dev = t.ctx->pool.get(bytes);
       try {
-         hipCheck(launchMutations(t.seq_views_dev.as<SeqColumnView>() + idx, col.view, t.n_chunks, mutationGroups(t, col),
-                                  f.words, f.chunk_cards, f.card_dev, t.slot_counts.as<uint32_t>(), t.totals.as<int64_t>(),
-                                  0.0, static_cast<uint32_t*>(dev), nullptr, 0, t.mut_rows.as<uint32_t>(), t.ctx->stream),
-                  "mutations kernels");
+         MutationsLaunch m = mutationsLaunch(t, idx, f);
+         m.counts_out = static_cast<uint32_t*>(dev);
+         m.n_rows = t.mut_rows.as<uint32_t>();
+         hipCheck(launchMutations(m, t.ctx->stream), "mutations kernels");
          hipCheck(hipMemcpyAsync(counts, dev, bytes, hipMemcpyDeviceToHost, t.ctx->stream), "copy counts");
          hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
       } catch (...) {
@@ -389,28 +456,26 @@ int rdb_mutation_counts(rdb_table* table, const rdb_bitmap* filter, const char* 
 
 }  // extern "C"
 
-namespace {
+namespace rdb::capi {
 
 // Mutations rows under a device filter: kernels, one readback, reference output order
 // (mutations_node.cpp:292-366).
-void runMutations(Table& t, const DeviceBitmap& f, const char* column, double min_proportion, uint32_t* position,
-                  uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage, double* proportion,
-                  uint64_t capacity, uint64_t* n) {
-   if (!(min_proportion >= 0.0 && min_proportion <= 1.0)) fail(RDB_EQUERY, "minProportion must be in interval [0.0,1.0]");
-   const int idx = t.seqIndex(column);
-   if (idx < 0) fail(RDB_EQUERY, std::string("Database does not contain the sequence '") + column + "'");
-   const SeqColumn& col = *t.seq[idx];
+std::vector<MutationRowDev> mutationRows(Table& t, const DeviceBitmap& f, int idx, double min_proportion) {
    // device buffer: a 16-byte header holding the row count, then the rows
    auto* base = t.mut_rows.as<uint8_t>();
    auto* n_dev = reinterpret_cast<uint32_t*>(base);
    auto* rows_dev = reinterpret_cast<MutationRowDev*>(base + 16);
    const uint32_t row_cap = static_cast<uint32_t>((t.mut_rows.bytes() - 16) / sizeof(MutationRowDev));
-   hipCheck(launchMutations(t.seq_views_dev.as<SeqColumnView>() + idx, col.view, t.n_chunks, mutationGroups(t, col),
-                            f.words, f.chunk_cards, f.card_dev, t.slot_counts.as<uint32_t>(), t.totals.as<int64_t>(),
-                            min_proportion, nullptr, rows_dev, row_cap, n_dev, t.ctx->stream,
-                            t.ctx->profiling ? t.ctx->prof_events[0] : nullptr,
-                            t.ctx->profiling ? t.ctx->prof_events[1] : nullptr),
-            "mutations kernels");
+   MutationsLaunch m = mutationsLaunch(t, idx, f);
+   m.min_proportion = min_proportion;
+   m.rows = rows_dev;
+   m.row_capacity = row_cap;
+   m.n_rows = n_dev;
+   if (t.ctx->profiling) {
+      m.count_start = t.ctx->prof_events[0];
+      m.count_stop = t.ctx->prof_events[1];
+   }
+   hipCheck(launchMutations(m, t.ctx->stream), "mutations kernels");
    if (t.ctx->profiling) t.ctx->profileAccumulate(0);
    // one copy brings the count and the first rows; a second one only for long outputs
    constexpr size_t kFirst = 2048;
@@ -430,10 +495,20 @@ void runMutations(Table& t, const DeviceBitmap& f, const char* column, double mi
                "copy rows");
       hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
    }
+   sortMutationRows(rows);
+   return rows;
+}
+
+void sortMutationRows(std::vector<MutationRowDev>& rows) {
    // mutations_node.cpp:302-364 emits rows by position, then in VALID_MUTATION_SYMBOLS order
    std::sort(rows.begin(), rows.end(), [](const MutationRowDev& a, const MutationRowDev& b) {
       return a.position != b.position ? a.position < b.position : a.symbol < b.symbol;
    });
+}
+
+void emitMutationRows(const SeqColumn& col, const std::vector<MutationRowDev>& rows, uint32_t* position, uint8_t* from,
+                      uint8_t* to, uint32_t* count, uint32_t* coverage, double* proportion, uint64_t capacity,
+                      uint64_t* n) {
    *n = rows.size();
    if (rows.size() > capacity) fail(RDB_ENOMEM, "output capacity too small");
    for (size_t i = 0; i < rows.size(); ++i) {
@@ -444,6 +519,25 @@ void runMutations(Table& t, const DeviceBitmap& f, const char* column, double mi
       if (coverage) coverage[i] = rows[i].coverage;
       if (proportion) proportion[i] = static_cast<double>(rows[i].count) / static_cast<double>(rows[i].coverage);
    }
+}
+
+int mutationColumn(const Table& t, const char* column, double min_proportion) {
+   if (!(min_proportion >= 0.0 && min_proportion <= 1.0)) fail(RDB_EQUERY, "minProportion must be in interval [0.0,1.0]");
+   const int idx = t.seqIndex(column);
+   if (idx < 0) fail(RDB_EQUERY, std::string("Database does not contain the sequence '") + column + "'");
+   return idx;
+}
+
+}  // namespace rdb::capi
+
+namespace {
+
+void runMutations(Table& t, const DeviceBitmap& f, const char* column, double min_proportion, uint32_t* position,
+                  uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage, double* proportion,
+                  uint64_t capacity, uint64_t* n) {
+   const int idx = capi::mutationColumn(t, column, min_proportion);
+   const auto rows = capi::mutationRows(t, f, idx, min_proportion);
+   capi::emitMutationRows(*t.seq[idx], rows, position, from, to, count, coverage, proportion, capacity, n);
 }
 
 }  // namespace
@@ -487,47 +581,23 @@ int rdb_aggregate(rdb_table* table, const rdb_bitmap* filter, const char* dims, 
       requireFinalized(t);
       if (!dims || !n) fail(RDB_EINVAL, "null argument");
       const DeviceBitmap& f = filterOf(t, filter);
-      struct Dim {
-         bool is_seq;
-         const DictColumn* dict = nullptr;
-         const SeqColumn* seq = nullptr;
-         uint32_t n_labels;  // including the null label
-      };
-      std::vector<Dim> ds;
+      const std::vector<capi::AggDim> ds = capi::parseDims(t, dims);
       GroupDims gd{};
-      std::string spec(dims);
-      size_t b = 0;
-      while (b <= spec.size()) {
-         size_t e = spec.find(',', b);
-         if (e == std::string::npos) e = spec.size();
-         const std::string d = spec.substr(b, e - b);
-         b = e + 1;
-         if (d.empty()) continue;
-         if (ds.size() == 4) fail(RDB_EQUERY, "at most four grouping dimensions are supported");
-         const auto at = d.find('@');
-         const size_t k = ds.size();
-         if (at != std::string::npos) {
-            const int idx = t.seqIndex(d.substr(0, at));
-            if (idx < 0) fail(RDB_EQUERY, "Database does not contain the sequence '" + d.substr(0, at) + "'");
-            const SeqColumn& col = *t.seq[idx];
-            char* endp = nullptr;
-            const unsigned long p = std::strtoul(d.c_str() + at + 1, &endp, 10);
-            if (*endp != '\0' || p >= col.genome_length) fail(RDB_EQUERY, "bad sequence position in '" + d + "'");
+      for (size_t k = 0; k < ds.size(); ++k) {
+         if (ds[k].is_seq) {
+            const SeqColumn& col = *ds[k].seq;
+            const uint32_t p = ds[k].position;
             const size_t padded = static_cast<size_t>(t.n_chunks) << 16;
             if (t.labels_scratch[k].bytes() < padded * 4) t.labels_scratch[k] = DeviceBuffer(padded * 4);
             const uint32_t blk = col.block_of_pos[p];
             const uint32_t base = col.block_slot_start[blk];
-            hipCheck(launchSymbolLabels(t.seq_views_dev.as<SeqColumnView>() + idx, t.chunk_sizes_dev.as<uint32_t>(),
-                                        static_cast<uint32_t>(p), t.n_chunks, blk, col.pos_slot_ptr[p] - base,
-                                        col.pos_slot_ptr[p + 1] - base, t.labels_scratch[k].as<uint32_t>(), t.ctx->stream),
+            hipCheck(launchSymbolLabels(t.seq_views_dev.as<SeqColumnView>() + ds[k].seq_index, t.chunk_sizes_dev.as<uint32_t>(),
+                                        p, t.n_chunks, blk, col.pos_slot_ptr[p] - base, col.pos_slot_ptr[p + 1] - base,
+                                        t.labels_scratch[k].as<uint32_t>(), t.ctx->stream),
                      "symbol labels");
-            ds.push_back(Dim{true, nullptr, &col, col.abc->count + 1});
             gd.labels[k] = t.labels_scratch[k].as<uint32_t>();
          } else {
-            const DictColumn* col = t.dictColumn(d);
-            if (!col) fail(RDB_EQUERY, "The database does not contain the column '" + d + "'");
-            ds.push_back(Dim{false, col, nullptr, static_cast<uint32_t>(col->dictionary.size()) + 1});
-            gd.labels[k] = col->labels.as<uint32_t>();
+            gd.labels[k] = ds[k].dict->labels.as<uint32_t>();
          }
       }
       gd.n = static_cast<uint32_t>(ds.size());
@@ -539,12 +609,7 @@ int rdb_aggregate(rdb_table* table, const rdb_bitmap* filter, const char* dims, 
          counts[0] = const_cast<DeviceBitmap&>(f).cardinality();
          return;
       }
-      uint64_t bins = 1;
-      for (int k = static_cast<int>(ds.size()) - 1; k >= 0; --k) {
-         gd.stride[k] = static_cast<uint32_t>(bins);
-         bins *= ds[k].n_labels;
-         if (bins > (1ULL << 28)) fail(RDB_EQUERY, "too many group combinations");
-      }
+      const uint64_t bins = capi::dimStrides(ds, gd.stride);
       const size_t hb = bins * 4;
       void* hist = t.ctx->pool.get(hb);
       std::vector<uint32_t> h(bins);
@@ -559,40 +624,13 @@ int rdb_aggregate(rdb_table* table, const rdb_bitmap* filter, const char* dims, 
          throw;
       }
       t.ctx->pool.put(hist, hb);
-      // bitmap_aggregation_node.cpp:91-130, 226-253: per dimension, groups by value string (or
-      // symbol order for a sequence position) with the null group last; depth-first emission
-      struct Combo {
-         uint32_t key[4];
-         uint32_t order[4];
-         uint64_t count;
-      };
-      std::vector<Combo> combos;
-      for (uint64_t bin = 0; bin < bins; ++bin) {
-         if (!h[bin]) continue;
-         Combo c{};
-         uint64_t rem = bin;
-         for (size_t k = 0; k < ds.size(); ++k) {
-            const uint32_t lab = static_cast<uint32_t>(rem / gd.stride[k]);
-            rem %= gd.stride[k];
-            const bool is_null = lab == ds[k].n_labels - 1;
-            c.key[k] = is_null ? kInvalidId : lab;
-            c.order[k] = is_null ? kInvalidId : (ds[k].is_seq ? lab : ds[k].dict->rank[lab]);
-         }
-         c.count = h[bin];
-         combos.push_back(c);
-      }
-      std::sort(combos.begin(), combos.end(), [&](const Combo& a, const Combo& b2) {
-         for (size_t k = 0; k < ds.size(); ++k) {
-            if (a.order[k] != b2.order[k]) return a.order[k] < b2.order[k];
-         }
-         return false;
-      });
-      *n = combos.size();
-      if (combos.size() > capacity) fail(RDB_ENOMEM, "output capacity too small");
-      for (size_t i = 0; i < combos.size(); ++i) {
-         for (size_t k = 0; k < ds.size(); ++k) ids[i * ds.size() + k] = combos[i].key[k];
-         counts[i] = combos[i].count;
-      }
+      std::vector<uint32_t> out_ids;
+      std::vector<uint64_t> out_counts;
+      capi::decodeHistogram(ds, gd.stride, h.data(), bins, out_ids, out_counts);
+      *n = out_counts.size();
+      if (out_counts.size() > capacity) fail(RDB_ENOMEM, "output capacity too small");
+      std::copy(out_ids.begin(), out_ids.end(), ids);
+      std::copy(out_counts.begin(), out_counts.end(), counts);
    });
 }
 
@@ -630,10 +668,25 @@ int rdb_mutations_traffic(rdb_table* table, const rdb_bitmap* filter, const char
       std::vector<uint32_t> cards(t.n_chunks);
       hipCheck(hipMemcpyAsync(cards.data(), f.chunk_cards, cards.size() * 4, hipMemcpyDeviceToHost, t.ctx->stream), "copy");
       hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
-      uint64_t b = 4ULL * col.abc->count * col.genome_length;
-      for (uint32_t c = 0; c < t.n_chunks; ++c) {
-         if (!cards[c]) continue;
-         b += col.chunk_index_bytes[c] + 8192ULL + 8ULL * t.chunk_sizes[c] + 4ULL * col.chunk_nruns[c];
+      uint64_t card = 0;
+      for (uint32_t c : cards) card += c;
+      const uint64_t counts_bytes = 4ULL * col.abc->count * col.genome_length;
+      uint64_t b = 0;
+      if (t.ctx->sparse_max_rows >= 0 && card <= static_cast<uint64_t>(t.ctx->sparse_max_rows)) {
+         // sparse method: the filter words, then per filter row its coverage word, two CSR offset
+         // pairs, its mutations and its missing runs; the counts are read and cleared
+         DeviceBitmap& fm = const_cast<DeviceBitmap&>(f);
+         b = 8ULL * t.n_chunks * kChunkWords + 2 * counts_bytes + 2ULL * 12 * col.genome_length;
+         for (uint32_t r : fm.rows()) {
+            b += 4 + 16 + 4ULL * (col.row_mut_ptr[r + 1] - col.row_mut_ptr[r]) +
+                 4ULL * (col.row_nrun_ptr[r + 1] - col.row_nrun_ptr[r]);
+         }
+      } else {
+         b = counts_bytes;
+         for (uint32_t c = 0; c < t.n_chunks; ++c) {
+            if (!cards[c]) continue;
+            b += col.chunk_index_bytes[c] + 8192ULL + 8ULL * t.chunk_sizes[c] + 4ULL * col.chunk_nruns[c];
+         }
       }
       *bytes = b;
    });

@@ -24,6 +24,8 @@
 //    the vertical form of the reference's horizontal coverage index.
 //  * Row-level arrays for filter leaves: coverage (start | end << 16), internal missing runs (CSR),
 //    int/date values, dictionary ids and null bitsets.
+//  * Row-major mutations of a sequence column (CSR over padded rows, `position | symbol << 16`,
+//    relative to the original reference): the input of the sparse Mutations kernel.
 #pragma once
 
 #include <cstdint>
@@ -45,6 +47,7 @@ constexpr uint32_t kPadSlot = 4095;
 constexpr uint32_t kLightMaxRows = 32;
 constexpr uint32_t kMaxGroups = 16;         // chunk groups of the Mutations kernel
 constexpr uint32_t kInvalidId = 0xFFFFFFFFu;
+constexpr uint32_t kMutPosBlock = 256;      // positions per work-group of the Mutations finalize kernels
 
 enum ContainerType : uint32_t { CT_INLINE = 0, CT_ARRAY = 1, CT_RUN = 2, CT_BITSET = 3 };
 
@@ -94,6 +97,8 @@ struct SeqColumnView {
    const uint32_t* nrun_ptr;          // [n_chunks * 2^16 + 1]
    const uint32_t* nruns;             // start | end << 16, end exclusive
    const uint64_t* null_bits;         // [n_chunks * 1024]
+   const uint32_t* row_mut_ptr;       // [n_chunks * 2^16 + 1] row-major mutations (sparse path)
+   const uint32_t* row_muts;          // position | symbol << 16, original-reference frame
 };
 
 // One row of Mutations output before the host computes the proportion.

@@ -95,6 +95,7 @@ struct Context {
    std::recursive_mutex mu;
    std::string error;
    int cu_count = 256;
+   size_t max_lds_bytes = 64 * 1024;  // LDS a work-group may allocate
    // pinned host staging
    void* pinned = nullptr;
    size_t pinned_bytes = 0;
@@ -115,6 +116,14 @@ struct Context {
    double prof_ms[2] = {0, 0};     // [0] Mutations counting kernel, [1] filter kernel
    uint64_t prof_launches[2] = {0, 0};
    void profileAccumulate(int which);  // waits for the pair's stop event and adds its time
+   // tuning knobs (rdb_set_tuning)
+   long long sparse_max_rows = -1;     // filters with at most this many rows take the sparse path
+   uint32_t mutation_groups = 0;       // chunk groups of the dense Mutations kernel (0 = automatic)
+   uint32_t mutation_unroll = 8;       // 16-byte loads in flight per lane in the dense Mutations kernel
+   uint32_t batch_max_queries = 256;   // queries per lane job of the batched path
+   uint32_t batch_lanes = 2;           // lanes (streams with their own buffers) per table
+   uint32_t batch_graphs = 1;          // capture each lane's launch sequence into a HIP graph
+   uint32_t batch_row_capacity = 4096; // Mutations rows per query a lane job keeps (more: rerun alone)
    void init(int dev);
    ~Context();
 };
@@ -151,6 +160,9 @@ struct SeqColumn {
    std::vector<uint64_t> chunk_index_bytes;  // per chunk: descriptors + payloads of stored symbols
    std::vector<uint64_t> chunk_nruns;        // per chunk: internal missing runs
    uint32_t n_flipped = 0;
+   uint64_t row_major_bytes = 0;             // row-major mutation CSR (sparse path)
+   std::vector<uint32_t> row_mut_ptr, row_nrun_ptr;  // host copies over padded rows (traffic accounting)
+   DeviceBuffer sparse_counts, sparse_bnd;   // zero between calls; cleared by the sparse finalize
 };
 
 struct IntColumn {
@@ -176,6 +188,11 @@ struct DictColumn {
    DeviceBuffer labels;                     // per padded row: id, n_values for null, kInvalidId outside
 };
 
+struct BatchLane;
+
+// Deepest filter program (operand stack entries) the filter kernel's LDS holds on this device.
+uint32_t maxFilterDepth(const Context& ctx);
+
 struct Table {
    Context* ctx = nullptr;
    std::vector<uint32_t> chunk_sizes;
@@ -200,6 +217,8 @@ struct Table {
    // per-call scratch (sized at finalize)
    DeviceBuffer slot_counts, totals, mut_rows, mut_n, labels_scratch[4], hist;
    size_t hist_bins_capacity = 0;
+   // lanes of the batched path (created on demand)
+   std::vector<std::shared_ptr<BatchLane>> lanes;
 
    uint32_t addStoreBitmap(const std::vector<uint32_t>& sorted_global_rows);
    uint32_t globalRow(uint64_t dense) const;

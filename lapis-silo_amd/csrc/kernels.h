@@ -19,23 +19,113 @@ struct GroupDims {
 hipError_t launchFill(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
 
 size_t filterLdsBytes(uint32_t depth);
+// Lets the filter kernel allocate up to max_lds_bytes of dynamic LDS (deep operator trees).
+hipError_t configureFilterKernel(size_t max_lds_bytes);
 
-// Evaluates n_queries programs over every (chunk, 4096-row slice) of the table.  Output words of
+// Evaluates n_queries programs over every (chunk, 4096-row slice) of the table (when n_queries_dev
+// is set, the number of programs is read from it and n_queries only sizes the grid).  Output words of
 // query q start at out + q * n_chunks * 1024; cards[q] and chunk_cards[q * n_chunks + c] must be
 // zero on entry and receive the popcounts.
 hipError_t launchFilter(const TableView& tv, const ProgramDesc* progs, const Instr* instrs, const uint32_t* ranges,
                         uint32_t n_queries, uint32_t depth, uint64_t* out, unsigned long long* cards,
-                        uint32_t* chunk_cards, uint32_t max_blocks, hipStream_t stream);
+                        uint32_t* chunk_cards, uint32_t max_blocks, hipStream_t stream,
+                        const uint32_t* n_queries_dev = nullptr);
 
-// Mutations for one filter: per-slot AND-popcounts (one partial sum per chunk group, n_groups <=
-// kMaxGroups, `partial` holds n_groups * n_slots u32), boundary prefix sums, local-reference
-// accumulation and thresholding.  counts_out ([n_symbols][L]) and rows may be null.
-hipError_t launchMutations(const SeqColumnView* col_dev, const SeqColumnView& col_host, uint32_t n_chunks,
-                           uint32_t n_groups, const uint64_t* filter, const uint32_t* filter_chunk_cards,
-                           const unsigned long long* filter_card, uint32_t* partial, int64_t* totals,
-                           double min_proportion, uint32_t* counts_out, MutationRowDev* rows, uint32_t row_capacity,
-                           uint32_t* n_rows, hipStream_t stream, hipEvent_t count_start = nullptr,
-                           hipEvent_t count_stop = nullptr);
+// Mutations for one filter.  Two methods, both launched; the filter's device-side cardinality
+// picks one:
+//  * dense (cardinality > sparse_max, or sparse_max < 0): per-slot AND-popcounts of the vertical
+//    index against the filter (one partial sum per chunk group, n_groups <= kMaxGroups, `partial`
+//    holds n_groups * n_slots u32);
+//  * sparse (cardinality <= sparse_max): each filter row adds its own mutations and boundaries;
+//    sparse_counts ([n_symbols][L] u32) and sparse_bnd ([3][L] i32) must be zero on entry and are
+//    zero again on exit.
+// Then boundary prefix sums, the implicit (reference) count and thresholding.  counts_out
+// ([n_symbols][L]) and rows may be null.
+struct MutationsLaunch {
+   const SeqColumnView* col_dev = nullptr;
+   const SeqColumnView* col_host = nullptr;
+   const uint32_t* chunk_sizes = nullptr;  // device
+   uint32_t n_chunks = 0;
+   uint32_t n_groups = 1;
+   uint32_t unroll = 8;                    // 16-byte loads in flight per lane in the dense kernel (8 or 16)
+   const uint64_t* filter = nullptr;
+   const uint32_t* filter_chunk_cards = nullptr;
+   const unsigned long long* filter_card = nullptr;
+   long long sparse_max = -1;
+   uint32_t* partial = nullptr;
+   uint32_t* sparse_counts = nullptr;
+   int32_t* sparse_bnd = nullptr;
+   int64_t* totals = nullptr;
+   double min_proportion = 0.0;
+   uint32_t* counts_out = nullptr;
+   MutationRowDev* rows = nullptr;
+   uint32_t row_capacity = 0;
+   uint32_t* n_rows = nullptr;
+   hipEvent_t count_start = nullptr;  // recorded around the dense counting kernel when set
+   hipEvent_t count_stop = nullptr;
+};
+
+hipError_t launchMutations(const MutationsLaunch& m, hipStream_t stream);
+
+// Mutations for a batch of queries whose filters were evaluated together (query q's words at
+// filters + q * n_chunks * 1024).  A planning kernel sorts the Mutations queries by their
+// device-side cardinality into a dense list and a sparse list (empty filters in neither); the
+// counting kernels then work through the lists, each list entry owning its scratch (dense: n_slots
+// partial sums; sparse: the zeroed counts and boundary arrays), and the finalize kernels append
+// query q's rows at rows + q * row_capacity with their number in n_rows[q] (which may exceed the
+// capacity).  Every grid is sized for max_queries and reads the actual count from the device, so
+// the launch sequence can be captured once and replayed for any batch of up to max_queries.
+struct MutationsBatch {
+   const SeqColumnView* col_dev = nullptr;
+   const SeqColumnView* col_host = nullptr;
+   const uint32_t* chunk_sizes = nullptr;
+   uint32_t n_chunks = 0;
+   uint32_t max_queries = 0;
+   const uint32_t* n_queries = nullptr;        // device
+   const uint64_t* filters = nullptr;
+   const uint32_t* chunk_cards = nullptr;      // [q * n_chunks + c]
+   const unsigned long long* cards = nullptr;  // [q]
+   const uint32_t* is_mutations = nullptr;     // [q] 1 for Mutations queries on this column
+   const double* min_proportion = nullptr;     // [q]
+   long long sparse_max = -1;
+   uint32_t unroll = 8;
+   uint32_t* lists = nullptr;                  // [2 * max_queries]: dense ids, then sparse ids
+   uint32_t* list_counts = nullptr;            // [2]
+   uint32_t* partial = nullptr;                // [max_queries * n_slots]
+   uint32_t* sparse_counts = nullptr;          // [max_queries * n_symbols * L], zero
+   int32_t* sparse_bnd = nullptr;              // [max_queries * 3 * L], zero
+   int64_t* totals = nullptr;                  // [2 * max_queries * position blocks * 3]
+   MutationRowDev* rows = nullptr;
+   uint32_t row_capacity = 0;
+   uint32_t* n_rows = nullptr;                 // [max_queries]
+};
+
+hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream);
+
+// Aggregated group counts for a batch: entry e (e < *n_agg) counts the rows of query agg_query[e]
+// into hist[hist_offset[e] .. + bins[e]), which the launch zeroes first (*total_bins words).
+struct AggregateBatch {
+   uint32_t n_chunks = 0;
+   uint32_t max_entries = 0;
+   const uint32_t* n_agg = nullptr;         // device
+   const uint32_t* total_bins = nullptr;    // device
+   uint32_t max_total_bins = 0;
+   const uint32_t* agg_query = nullptr;
+   const uint32_t* bins = nullptr;
+   const uint32_t* hist_offset = nullptr;
+   const GroupDims* dims = nullptr;
+   const uint64_t* filters = nullptr;
+   const uint32_t* chunk_cards = nullptr;
+   uint32_t* hist = nullptr;
+};
+
+hipError_t launchAggregateBatch(const AggregateBatch& a, hipStream_t stream);
+
+// Packs the Mutations rows of a batch: header[q] = {n_rows[q], offset, card low, card high} and
+// query q's first min(n_rows[q], row_capacity) rows at packed[offset ..].
+hipError_t launchPackRows(const uint32_t* n_queries, uint32_t max_queries, const uint32_t* n_rows,
+                          const unsigned long long* cards, const MutationRowDev* rows, uint32_t row_capacity,
+                          uint32_t* header, MutationRowDev* packed, hipStream_t stream);
 
 hipError_t launchSymbolLabels(const SeqColumnView* col_dev, const uint32_t* chunk_sizes, uint32_t position,
                               uint32_t n_chunks, uint32_t block, uint32_t slot_lo, uint32_t slot_hi, uint32_t* labels,

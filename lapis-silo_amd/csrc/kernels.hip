@@ -18,7 +18,8 @@ constexpr int kVmWaves = 4;
 constexpr int kVmThreads = kVmWaves * 64;
 constexpr int kMutWaves = 4;
 constexpr int kMutThreads = kMutWaves * 64;
-constexpr int kPosBlock = 1024;
+constexpr int kPosBlock = static_cast<int>(kMutPosBlock);
+constexpr int kPosWaves = kPosBlock / 64;
 
 __device__ inline void waveSync() {
    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -45,6 +46,30 @@ __device__ inline uint32_t lowerBound16(const uint16_t* a, uint32_t n, uint32_t 
       }
    }
    return lo;
+}
+
+// Lower bound by the whole wavefront: the first index i in [lo, hi) with !before(i), for a predicate
+// that holds on a prefix.  Each round samples 64 evenly spaced indices with one load per lane and
+// keeps the interval between the last sample before the bound and the first one after it, so a
+// search takes about log64(n) dependent loads instead of log2(n).  Must be called by all 64 lanes
+// with the same arguments; returns the same value in every lane.
+template <typename Before>
+__device__ inline uint32_t waveLowerBound(uint32_t lo, uint32_t hi, Before before) {
+   const uint32_t lane = threadIdx.x & 63;
+   while (hi - lo > 64) {
+      const uint32_t step = (hi - lo + 63) / 64;
+      const uint32_t idx = lo + lane * step;
+      const bool b = idx < hi && before(idx);
+      const uint32_t cnt = static_cast<uint32_t>(__popcll(__ballot(b)));
+      if (cnt == 0) return lo;
+      const uint32_t last_before = lo + (cnt - 1) * step;
+      const uint32_t next = lo + cnt * step;
+      lo = last_before + 1;
+      if (next < hi) hi = next;
+   }
+   const uint32_t idx = lo + lane;
+   const bool b = idx < hi && before(idx);
+   return lo + static_cast<uint32_t>(__popcll(__ballot(b)));
 }
 
 // First run whose inclusive end is >= x.
@@ -91,8 +116,9 @@ __device__ inline void accumulateContainer(const Desc d, const ContainerArenas& 
       acc |= ar.bits[static_cast<size_t>(d.w1) * kChunkWords + my_word];
    } else if (type == CT_ARRAY) {
       const uint16_t* a = ar.arr + d.w1;
-      const uint32_t b = lowerBound16(a, card, lo);
-      const uint32_t e = hi >= kChunkRows ? card : lowerBound16(a, card, hi);
+      const uint32_t b = waveLowerBound(0, card, [&](uint32_t i) { return static_cast<uint32_t>(a[i]) < lo; });
+      const uint32_t e = hi >= kChunkRows ? card
+                                          : waveLowerBound(b, card, [&](uint32_t i) { return static_cast<uint32_t>(a[i]) < hi; });
       for (uint32_t k = b + lane; k < e; k += 64) {
          const uint32_t v = a[k];
          atomicOr(reinterpret_cast<unsigned long long*>(&scratch[(v >> 6) & 63]), 1ULL << (v & 63));
@@ -142,7 +168,8 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                                                              const uint32_t* __restrict__ ranges, uint32_t n_items,
                                                              uint32_t depth, uint64_t* __restrict__ out,
                                                              unsigned long long* __restrict__ cards,
-                                                             uint32_t* __restrict__ chunk_cards) {
+                                                             uint32_t* __restrict__ chunk_cards,
+                                                             const uint32_t* __restrict__ n_queries_dev) {
    extern __shared__ uint64_t lds[];
    const uint32_t wave = threadIdx.x >> 6;
    const uint32_t lane = threadIdx.x & 63;
@@ -150,6 +177,7 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
    uint64_t* scratch = stack + static_cast<size_t>(depth) * 64;
    const uint32_t n_chunks = tv.n_chunks;
    const uint32_t per_query = n_chunks * kSlicesPerChunk;
+   if (n_queries_dev) n_items = uniform(*n_queries_dev * per_query);
    for (uint32_t item = uniform(blockIdx.x * kVmWaves + wave); item < n_items; item += gridDim.x * kVmWaves) {
       const uint32_t qi = item / per_query;
       const uint32_t rem = item - qi * per_query;
@@ -198,18 +226,9 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                {
                   // light entries of the slots [c, d): slot << 16 | row, sorted
                   const uint32_t* le = col.light;
-                  uint32_t lo = col.light_ptr[tile], hi = col.light_ptr[tile + 1];
-                  uint32_t a = lo, b = hi;
-                  while (a < b) {
-                     const uint32_t mid = (a + b) >> 1;
-                     if ((le[mid] >> 16) < ins.c) a = mid + 1; else b = mid;
-                  }
-                  uint32_t e = a;
-                  b = hi;
-                  while (e < b) {
-                     const uint32_t mid = (e + b) >> 1;
-                     if ((le[mid] >> 16) < ins.d) e = mid + 1; else b = mid;
-                  }
+                  const uint32_t lo = col.light_ptr[tile], hi = col.light_ptr[tile + 1];
+                  const uint32_t a = waveLowerBound(lo, hi, [&](uint32_t i) { return (le[i] >> 16) < ins.c; });
+                  const uint32_t e = waveLowerBound(a, hi, [&](uint32_t i) { return (le[i] >> 16) < ins.d; });
                   for (uint32_t k = a + lane; k < e; k += 64) {
                      const uint32_t x = le[k];
                      const uint32_t row = x & 0xFFFFu;
@@ -222,17 +241,9 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                {
                   // groups of eight rows of the slots [c, d)
                   const uint16_t* gh = col.group_hdr;
-                  uint32_t a = col.group_ptr[tile], b = col.group_ptr[tile + 1];
-                  while (a < b) {
-                     const uint32_t mid = (a + b) >> 1;
-                     if ((gh[mid] & 0xFFFu) < ins.c) a = mid + 1; else b = mid;
-                  }
-                  uint32_t e = a;
-                  b = col.group_ptr[tile + 1];
-                  while (e < b) {
-                     const uint32_t mid = (e + b) >> 1;
-                     if ((gh[mid] & 0xFFFu) < ins.d) e = mid + 1; else b = mid;
-                  }
+                  const uint32_t g_hi = col.group_ptr[tile + 1];
+                  const uint32_t a = waveLowerBound(col.group_ptr[tile], g_hi, [&](uint32_t i) { return (gh[i] & 0xFFFu) < ins.c; });
+                  const uint32_t e = waveLowerBound(a, g_hi, [&](uint32_t i) { return (gh[i] & 0xFFFu) < ins.d; });
                   for (uint32_t k = a + lane; k < e; k += 64) {
                      const uint32_t hdr = gh[k];
                      const uint32_t sym = col.slot_info[slot_base + (hdr & 0xFFFu)] >> 16;
@@ -247,13 +258,9 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                   }
                }
                {
-                  uint32_t lo = col.heavy_ptr[tile];
                   const uint32_t end = col.heavy_ptr[tile + 1];
-                  uint32_t hi = end;
-                  while (lo < hi) {
-                     const uint32_t mid = (lo + hi) >> 1;
-                     if (descSlot(col.vert.desc[mid].w0) < ins.c) lo = mid + 1; else hi = mid;
-                  }
+                  const uint32_t lo = waveLowerBound(col.heavy_ptr[tile], end,
+                                                     [&](uint32_t i) { return descSlot(col.vert.desc[i].w0) < ins.c; });
                   for (uint32_t k = lo; k < end; ++k) {
                      const Desc d = col.vert.desc[k];
                      const uint32_t slot = descSlot(d.w0);
@@ -308,28 +315,30 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                break;
             }
             case OP_CMP: {
+               // one or two comparisons (g & 1) of the column, conjoined; a null row matches a
+               // comparison iff its with_nulls flag is set
                const IntColumnView& col = tv.ints[ins.a];
                const int32_t v = static_cast<int32_t>(ins.c);
+               const int32_t v2 = static_cast<int32_t>(ins.f);
+               const bool two = (ins.g & 1u) != 0;
                const uint64_t nulls = col.null_bits[static_cast<size_t>(chunk) * kChunkWords + my_word];
                const int4* vals = reinterpret_cast<const int4*>(col.values + row_base + static_cast<size_t>(lane) * 64);
-               const bool with_nulls = ins.d != 0;
+               int4 x[16];
+#pragma unroll
+               for (uint32_t u = 0; u < 16; ++u) x[u] = vals[u];
                uint64_t word = 0;
 #pragma unroll
-               for (uint32_t h = 0; h < 2; ++h) {
-                  int4 x[8];
+               for (uint32_t u = 0; u < 16; ++u) {
+                  const int32_t xs[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
 #pragma unroll
-                  for (uint32_t u = 0; u < 8; ++u) x[u] = vals[8 * h + u];
-#pragma unroll
-                  for (uint32_t u = 0; u < 8; ++u) {
-                     const int32_t xs[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
-#pragma unroll
-                     for (uint32_t q = 0; q < 4; ++q) {
-                        const uint32_t bit = 32 * h + 4 * u + q;
-                        const bool m = ((nulls >> bit) & 1ULL) ? with_nulls : compareInt(ins.b, xs[q], v);
-                        word |= static_cast<uint64_t>(m) << bit;
-                     }
+                  for (uint32_t q = 0; q < 4; ++q) {
+                     word |= static_cast<uint64_t>(compareInt(ins.b, xs[q], v) && (!two || compareInt(ins.e, xs[q], v2)))
+                             << (4 * u + q);
                   }
                }
+               // null rows: the stored value is ignored
+               const uint64_t null_match = (ins.d != 0) && (!two || (ins.g & 2u) != 0) ? nulls : 0ULL;
+               word = (word & ~nulls) | null_match;
                push(word & lmask);
                break;
             }
@@ -426,6 +435,16 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
 // tile (b, c) is AND-popcounted against it; the per-slot sums stay in LDS and are added to the
 // global per-slot counts once at the end.
 // ------------------------------------------------------------------------------------------------
+// The dense (index-streaming) and sparse (row-walking) Mutations kernels are both launched; the
+// filter's cardinality, known on the device, activates exactly one of them.  sparse_max < 0
+// disables the sparse path.
+__device__ inline bool sparseActive(const unsigned long long* card, long long sparse_max) {
+   return sparse_max >= 0 && *card <= static_cast<unsigned long long>(sparse_max);
+}
+__device__ inline bool denseActive(const unsigned long long* card, long long sparse_max) {
+   return !sparseActive(card, sparse_max);
+}
+
 __device__ inline uint32_t bitAt(const uint64_t* f, uint32_t v) {
    return static_cast<uint32_t>((f[v >> 6] >> (v & 63)) & 1ULL);
 }
@@ -442,16 +461,13 @@ __device__ inline uint32_t rangePopcount(const uint64_t* f, uint32_t a, uint32_t
    return n;
 }
 
-__global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqColumnView* __restrict__ colp,
-                                                                    uint32_t n_chunks, uint32_t n_groups,
-                                                                    const uint64_t* __restrict__ filter,
-                                                                    const uint32_t* __restrict__ filter_chunk_cards,
-                                                                    uint32_t* __restrict__ partial) {
-   __shared__ uint64_t fs[kMutWaves][kChunkWords];
-   __shared__ uint32_t cnt[kMaxBlockSlots + 1];
-   const SeqColumnView& col = *colp;
-   const uint32_t b = blockIdx.x;
-   const uint32_t g = blockIdx.y;
+// Per-slot AND-popcounts of position block b over the chunks {g + k * n_groups} against one filter;
+// writes the block's slot sums to out[0 .. slots of b).  Called by a whole work-group.
+template <uint32_t kUnroll>
+__device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b, uint32_t g, uint32_t n_chunks,
+                                           uint32_t n_groups, const uint64_t* __restrict__ filter,
+                                           const uint32_t* __restrict__ filter_chunk_cards, uint32_t* __restrict__ out,
+                                           uint64_t (*fs)[kChunkWords], uint32_t* cnt) {
    const uint32_t tid = threadIdx.x;
    const uint32_t lane = tid & 63;
    const uint32_t wave = tid >> 6;
@@ -487,7 +503,6 @@ __global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqCol
       // light entries: each lane walks a contiguous span of 16-byte groups, loading kUnroll groups
       // before using them; consecutive entries of a slot are summed before one LDS atomic
       {
-         constexpr uint32_t kUnroll = 8;
          const uint4* e4 = reinterpret_cast<const uint4*>(col.light + l0);
          const uint32_t n4 = (l1 - l0) >> 2;
          uint32_t cur = kPadSlot, acc = 0;
@@ -519,7 +534,6 @@ __global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqCol
       }
       // larger arrays: groups of eight rows sharing one slot; lanes walk contiguous spans of groups
       {
-         constexpr uint32_t kUnroll = 8;
          const uint4* gr = reinterpret_cast<const uint4*>(col.groups) + g0;
          const uint16_t* gh = col.group_hdr + g0;
          const uint32_t ng = g1 - g0;
@@ -595,8 +609,24 @@ __global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqCol
       waveSync();
    }
    __syncthreads();
-   uint32_t* out = partial + static_cast<size_t>(g) * col.n_slots + s0;
-   for (uint32_t i = tid; i < ns; i += kMutThreads) out[i] = cnt[i];
+   for (uint32_t i = tid; i < ns; i += kMutThreads) out[s0 + i] = cnt[i];
+}
+
+
+template <uint32_t kUnroll>
+__global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqColumnView* __restrict__ colp,
+                                                                    uint32_t n_chunks, uint32_t n_groups,
+                                                                    const uint64_t* __restrict__ filter,
+                                                                    const uint32_t* __restrict__ filter_chunk_cards,
+                                                                    const unsigned long long* __restrict__ filter_card,
+                                                                    long long sparse_max,
+                                                                    uint32_t* __restrict__ partial) {
+   __shared__ uint64_t fs[kMutWaves][kChunkWords];
+   __shared__ uint32_t cnt[kMaxBlockSlots + 1];
+   if (!denseActive(filter_card, sparse_max)) return;
+   const SeqColumnView& col = *colp;
+   countBlock<kUnroll>(col, blockIdx.x, blockIdx.y, n_chunks, n_groups, filter, filter_chunk_cards,
+              partial + static_cast<size_t>(blockIdx.y) * col.n_slots, fs, cnt);
 }
 
 __device__ inline uint32_t slotValue(const uint32_t* partial, uint32_t n_groups, uint32_t n_slots, uint32_t s) {
@@ -605,151 +635,405 @@ __device__ inline uint32_t slotValue(const uint32_t* partial, uint32_t n_groups,
    return v;
 }
 
-// Per 1024-position block: totals of the three boundary quantities
-//   A(p) = rows of the filter whose covered range starts at p (p > 0),
-//   B(p) = rows whose covered range ends at p,
-//   C(p) = internal missing runs starting at p minus those ending at p.
-__device__ inline void boundaryAt(const SeqColumnView& col, const uint32_t* partial, uint32_t n_groups, uint32_t p,
-                                  uint32_t& A, uint32_t& B, int32_t& C) {
-   A = 0;
-   B = 0;
-   C = 0;
-   const uint32_t e = col.pos_slot_ptr[p + 1];
-   for (uint32_t s = col.pos_slot_ptr[p]; s < e; ++s) {
-      const uint32_t sym = col.slot_info[s] >> 16;
-      if (sym < col.n_symbols) continue;
-      const uint32_t v = slotValue(partial, n_groups, col.n_slots, s);
-      switch (sym - col.n_symbols) {
-         case BND_COV_START: A += v; break;
-         case BND_COV_END: B += v; break;
-         case BND_N_START: C += static_cast<int32_t>(v); break;
-         default: C -= static_cast<int32_t>(v); break;
+// ------------------------------------------------------------------------------------------------
+// Sparse Mutations counting: each filter row adds its own stored mutations (row-major CSR in the
+// original-reference frame), its covered-range boundaries and its internal missing runs with
+// global atomics.  Work is proportional to the filter's rows, not to the index.
+//   counts[s * L + p]  rows of F with symbol s at p (s != reference[p])
+//   bnd[p], bnd[L + p], bnd[2L + p]  A(p), B(p), C(p) as in the dense path
+// Both arrays must be zero on entry; the sparse finalize kernel clears what it reads.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void sparseCountKernel(const SeqColumnView* __restrict__ colp,
+                                                         const uint32_t* __restrict__ chunk_sizes, uint32_t n_words,
+                                                         const uint64_t* __restrict__ filter,
+                                                         const unsigned long long* __restrict__ filter_card,
+                                                         long long sparse_max, uint32_t* __restrict__ counts,
+                                                         int32_t* __restrict__ bnd) {
+   if (!sparseActive(filter_card, sparse_max)) return;
+   const SeqColumnView& col = *colp;
+   const uint32_t L = col.genome_length;
+   for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n_words; w += gridDim.x * blockDim.x) {
+      uint64_t x = filter[w];
+      if (x == 0) continue;
+      const uint32_t size = chunk_sizes[w >> 10];
+      while (x) {
+         const uint32_t r = (w << 6) | static_cast<uint32_t>(__builtin_ctzll(x));
+         x &= x - 1;
+         // bits outside the row layout (left by Threshold's complement of negated children) have
+         // no entries in the index: the dense path counts them covered with the local reference
+         if ((r & 0xFFFFu) >= size) continue;
+         const uint32_t cv = col.cov[r];
+         const uint32_t st = cv & 0xFFFFu, en = cv >> 16;
+         if (st > 0) atomicAdd(&bnd[st], 1);
+         if (en < L) atomicAdd(&bnd[L + en], 1);
+         for (uint32_t k = col.nrun_ptr[r], e = col.nrun_ptr[r + 1]; k < e; ++k) {
+            const uint32_t run = col.nruns[k];
+            atomicAdd(&bnd[2 * L + (run & 0xFFFFu)], 1);
+            if ((run >> 16) < L) atomicAdd(&bnd[2 * L + (run >> 16)], -1);
+         }
+         for (uint32_t k = col.row_mut_ptr[r], e = col.row_mut_ptr[r + 1]; k < e; ++k) {
+            const uint32_t m = col.row_muts[k];
+            atomicAdd(&counts[(m >> 16) * L + (m & 0xFFFFu)], 1u);
+         }
       }
    }
 }
 
+// Sums the chunk groups' partial slot counts into group 0, in place (one thread per slot).
+__global__ __launch_bounds__(256) void reducePartialsKernel(uint32_t* __restrict__ partial, uint32_t n_groups,
+                                                            uint32_t n_slots,
+                                                            const unsigned long long* __restrict__ filter_card,
+                                                            long long sparse_max) {
+   if (!denseActive(filter_card, sparse_max)) return;
+   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+   if (s >= n_slots) return;
+   uint32_t v = partial[s];
+   for (uint32_t g = 1; g < n_groups; ++g) v += partial[static_cast<size_t>(g) * n_slots + s];
+   partial[s] = v;
+}
+
+// Sources of the per-position quantities for the totals and finalize kernels.
+//  * SlotSource: per-slot partial sums of the dense kernel; the implicit symbol (rows covered at p
+//    with no stored entry) is the local reference.
+//  * SparseSource: per-(symbol, position) counts of the sparse kernel in the original-reference
+//    frame; the implicit symbol is the original reference.
+struct SlotSource {
+   const SeqColumnView* col;
+   const uint32_t* partial;
+   uint32_t n_groups;
+
+   __device__ void boundary(uint32_t p, uint32_t& A, uint32_t& B, int32_t& C) const {
+      A = 0;
+      B = 0;
+      C = 0;
+      const uint32_t e = col->pos_slot_ptr[p + 1];
+      for (uint32_t s = col->pos_slot_ptr[p]; s < e; ++s) {
+         const uint32_t sym = col->slot_info[s] >> 16;
+         if (sym < col->n_symbols) continue;
+         const uint32_t v = slotValue(partial, n_groups, col->n_slots, s);
+         switch (sym - col->n_symbols) {
+            case BND_COV_START: A += v; break;
+            case BND_COV_END: B += v; break;
+            case BND_N_START: C += static_cast<int32_t>(v); break;
+            default: C -= static_cast<int32_t>(v); break;
+         }
+      }
+   }
+   __device__ uint32_t cursor(uint32_t p) const { return col->pos_slot_ptr[p]; }
+   // stored count of symbol `sym` at p; symbols must be visited in increasing order
+   __device__ uint32_t stored(uint32_t p, uint32_t sym, uint32_t& cur) const {
+      if (cur < col->pos_slot_ptr[p + 1] && (col->slot_info[cur] >> 16) == sym) {
+         return slotValue(partial, n_groups, col->n_slots, cur++);
+      }
+      return 0;
+   }
+   __device__ uint32_t implicitSymbol(uint32_t p) const { return col->local_ref[p]; }
+   __device__ void clear(uint32_t) const {}
+};
+
+struct SparseSource {
+   const SeqColumnView* col;
+   uint32_t* counts;
+   int32_t* bnd;
+
+   __device__ void boundary(uint32_t p, uint32_t& A, uint32_t& B, int32_t& C) const {
+      const uint32_t L = col->genome_length;
+      A = static_cast<uint32_t>(bnd[p]);
+      B = static_cast<uint32_t>(bnd[L + p]);
+      C = bnd[2 * L + p];
+   }
+   __device__ uint32_t cursor(uint32_t) const { return 0; }
+   __device__ uint32_t stored(uint32_t p, uint32_t sym, uint32_t&) const {
+      return counts[static_cast<size_t>(sym) * col->genome_length + p];
+   }
+   __device__ uint32_t implicitSymbol(uint32_t p) const { return col->reference[p]; }
+   __device__ void clear(uint32_t p) const {
+      const uint32_t L = col->genome_length;
+      for (uint32_t s = 0; s < col->n_symbols; ++s) counts[static_cast<size_t>(s) * L + p] = 0;
+      bnd[p] = 0;
+      bnd[L + p] = 0;
+      bnd[2 * L + p] = 0;
+   }
+};
+
+__device__ inline bool sourceActive(const SlotSource&, const unsigned long long* card, long long sparse_max) {
+   return denseActive(card, sparse_max);
+}
+__device__ inline bool sourceActive(const SparseSource&, const unsigned long long* card, long long sparse_max) {
+   return sparseActive(card, sparse_max);
+}
+
 template <typename T>
-__device__ inline T blockInclusiveScan(T v, T* tmp) {
-   // 1024 threads = 16 waves
-   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+__device__ inline T waveInclusiveScan(T v) {
+   const uint32_t lane = threadIdx.x & 63;
    for (int off = 1; off < 64; off <<= 1) {
       const T y = __shfl_up(v, off, 64);
       if (lane >= static_cast<uint32_t>(off)) v += y;
    }
+   return v;
+}
+
+template <typename T>
+__device__ inline T waveTotal(T v) {
+   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+   return v;
+}
+
+// Inclusive scan over the kPosBlock threads of a work-group; tmp holds kPosWaves values.
+template <typename T>
+__device__ inline T blockInclusiveScan(T v, T* tmp) {
+   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+   v = waveInclusiveScan(v);
    if (lane == 63) tmp[wave] = v;
    __syncthreads();
-   if (wave == 0) {
-      T w = lane < 16 ? tmp[lane] : T(0);
-      for (int off = 1; off < 16; off <<= 1) {
-         const T y = __shfl_up(w, off, 64);
-         if (lane >= static_cast<uint32_t>(off)) w += y;
-      }
-      if (lane < 16) tmp[lane] = w;
-   }
-   __syncthreads();
-   const T prefix = wave > 0 ? tmp[wave - 1] : T(0);
+   T prefix = 0;
+   for (uint32_t w = 0; w < wave; ++w) prefix += tmp[w];
    __syncthreads();
    return v + prefix;
 }
 
-__global__ __launch_bounds__(kPosBlock) void mutationsTotalsKernel(const SeqColumnView* __restrict__ colp,
-                                                                   const uint32_t* __restrict__ partial,
-                                                                   uint32_t n_groups, int64_t* __restrict__ totals,
-                                                                   uint32_t* __restrict__ n_rows) {
-   __shared__ int64_t tmp[3][16];
-   const SeqColumnView& col = *colp;
-   if (blockIdx.x == 0 && threadIdx.x == 0) *n_rows = 0;  // the finalize kernel appends rows after this one
-   const uint32_t p = blockIdx.x * kPosBlock + threadIdx.x;
+// Per kPosBlock-position block: totals of the three boundary quantities
+//   A(p) = rows of the filter whose covered range starts at p (p > 0),
+//   B(p) = rows whose covered range ends at p,
+//   C(p) = internal missing runs starting at p minus those ending at p.
+template <typename Src>
+__device__ __forceinline__ void totalsBody(const Src& src, uint32_t pblock, int64_t* __restrict__ totals) {
+   __shared__ int64_t tmp[3][kPosWaves];
+   const SeqColumnView& col = *src.col;
+   const uint32_t p = pblock * kPosBlock + threadIdx.x;
    uint32_t A = 0, B = 0;
    int32_t C = 0;
-   if (p < col.genome_length) boundaryAt(col, partial, n_groups, p, A, B, C);
-   int64_t v[3] = {A, B, C};
+   if (p < col.genome_length) src.boundary(p, A, B, C);
+   const int64_t v[3] = {A, B, C};
    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    for (int q = 0; q < 3; ++q) {
-      int64_t x = v[q];
-      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+      const int64_t x = waveTotal(v[q]);
       if (lane == 0) tmp[q][wave] = x;
    }
    __syncthreads();
    if (threadIdx.x < 3) {
-      int64_t s = 0;
-      for (int w = 0; w < 16; ++w) s += tmp[threadIdx.x][w];
-      totals[blockIdx.x * 3 + threadIdx.x] = s;
+      int64_t sum = 0;
+      for (int w = 0; w < kPosWaves; ++w) sum += tmp[threadIdx.x][w];
+      totals[pblock * 3 + threadIdx.x] = sum;
    }
 }
 
-__global__ __launch_bounds__(kPosBlock) void mutationsFinalizeKernel(
-   const SeqColumnView* __restrict__ colp, const uint32_t* __restrict__ partial, uint32_t n_groups,
-   const int64_t* __restrict__ totals, uint32_t n_pblocks, const unsigned long long* __restrict__ filter_card,
-   double min_proportion, uint32_t* __restrict__ counts_out, MutationRowDev* __restrict__ rows,
-   uint32_t row_capacity, uint32_t* __restrict__ n_rows) {
-   __shared__ int64_t scan_tmp[16];
+template <typename Src>
+__global__ __launch_bounds__(kPosBlock) void mutationsTotalsKernel(Src src, const unsigned long long* __restrict__ filter_card,
+                                                                   long long sparse_max, int64_t* __restrict__ totals,
+                                                                   uint32_t* __restrict__ n_rows) {
+   if (!sourceActive(src, filter_card, sparse_max)) return;
+   if (blockIdx.x == 0 && threadIdx.x == 0) *n_rows = 0;  // the finalize kernel appends rows after this one
+   totalsBody(src, blockIdx.x, totals);
+}
+
+template <typename Src>
+__device__ __forceinline__ void finalizeBody(const Src& src, uint32_t pblock, const int64_t* __restrict__ totals,
+                                             uint32_t n_pblocks, uint64_t filter_card, double min_proportion,
+                                             uint32_t* __restrict__ counts_out, MutationRowDev* __restrict__ rows,
+                                             uint32_t row_capacity, uint32_t* __restrict__ n_rows) {
+   __shared__ int64_t scan_tmp[kPosWaves];
+   __shared__ int64_t part[4][kPosWaves];
    __shared__ int64_t prefix[4];  // previous-block sums of A, B, C and the total of A
-   const SeqColumnView& col = *colp;
-   if (threadIdx.x == 0) {
+   const SeqColumnView& col = *src.col;
+   {
+      // block totals: thread k < n_pblocks (<= kPosBlock for genome lengths below 2^16) takes block k
+      const uint32_t k = threadIdx.x;
       int64_t a = 0, b = 0, c = 0, ta = 0;
-      for (uint32_t k = 0; k < n_pblocks; ++k) {
-         if (k < blockIdx.x) {
-            a += totals[3 * k];
-            b += totals[3 * k + 1];
-            c += totals[3 * k + 2];
+      if (k < n_pblocks) {
+         ta = totals[3 * k];
+         if (k < pblock) {
+            a = ta;
+            b = totals[3 * k + 1];
+            c = totals[3 * k + 2];
          }
-         ta += totals[3 * k];
       }
-      prefix[0] = a;
-      prefix[1] = b;
-      prefix[2] = c;
-      prefix[3] = ta;
+      const int64_t v[4] = {a, b, c, ta};
+      const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+      for (int q = 0; q < 4; ++q) {
+         const int64_t x = waveTotal(v[q]);
+         if (lane == 0) part[q][wave] = x;
+      }
+      __syncthreads();
+      if (threadIdx.x < 4) {
+         int64_t sum = 0;
+         for (int w = 0; w < kPosWaves; ++w) sum += part[threadIdx.x][w];
+         prefix[threadIdx.x] = sum;
+      }
+      __syncthreads();
    }
-   __syncthreads();
    const uint32_t L = col.genome_length;
-   const uint32_t p = blockIdx.x * kPosBlock + threadIdx.x;
+   const uint32_t p = pblock * kPosBlock + threadIdx.x;
    uint32_t A = 0, B = 0;
    int32_t C = 0;
-   if (p < L) boundaryAt(col, partial, n_groups, p, A, B, C);
+   if (p < L) src.boundary(p, A, B, C);
    const int64_t pa = blockInclusiveScan<int64_t>(A, scan_tmp) + prefix[0];
    const int64_t pb = blockInclusiveScan<int64_t>(B, scan_tmp) + prefix[1];
    const int64_t pc = blockInclusiveScan<int64_t>(C, scan_tmp) + prefix[2];
    if (p >= L) return;
-   const int64_t card = static_cast<int64_t>(*filter_card);
+   const int64_t card = static_cast<int64_t>(filter_card);
    // rows covered at p: not starting after p, not ended at or before p, not inside a missing run
    const int64_t covered = card - (prefix[3] - pa) - pb - pc;
-   const uint32_t s_begin = col.pos_slot_ptr[p];
-   const uint32_t s_end = col.pos_slot_ptr[p + 1];
    int64_t stored = 0;
-   for (uint32_t s = s_begin; s < s_end; ++s) {
-      if ((col.slot_info[s] >> 16) < col.n_symbols) stored += slotValue(partial, n_groups, col.n_slots, s);
+   {
+      uint32_t cur = src.cursor(p);
+      for (uint32_t sym = 0; sym < col.n_symbols; ++sym) stored += src.stored(p, sym, cur);
    }
-   const uint32_t ref_count = card == 0 ? 0u : static_cast<uint32_t>(covered - stored);
-   const uint32_t local = col.local_ref[p];
-   // per-symbol counts in symbol order: slots of p are sorted by symbol
+   const uint32_t implicit_count = card == 0 ? 0u : static_cast<uint32_t>(covered - stored);
+   const uint32_t implicit = src.implicitSymbol(p);
+   // per-symbol counts in symbol order
    uint32_t total = 0;
    {
-      uint32_t cur = s_begin;
+      uint32_t cur = src.cursor(p);
       for (uint32_t sym = 0; sym < col.n_symbols; ++sym) {
-         uint32_t v = 0;
-         if (cur < s_end && (col.slot_info[cur] >> 16) == sym) v = slotValue(partial, n_groups, col.n_slots, cur++);
-         if (sym == local) v += ref_count;
+         uint32_t v = src.stored(p, sym, cur);
+         if (sym == implicit) v += implicit_count;
          if (counts_out) counts_out[static_cast<size_t>(sym) * L + p] = v;
          if ((col.valid_mask >> sym) & 1u) total += v;
       }
    }
-   if (total == 0 || rows == nullptr) return;
-   const uint32_t thr = min_proportion == 0.0
-                           ? 0u
-                           : static_cast<uint32_t>(ceil(static_cast<double>(total) * min_proportion) - 1.0);
-   const uint32_t orig = col.reference[p];
-   uint32_t cur = s_begin;
-   for (uint32_t sym = 0; sym < col.n_symbols; ++sym) {
-      uint32_t v = 0;
-      if (cur < s_end && (col.slot_info[cur] >> 16) == sym) v = slotValue(partial, n_groups, col.n_slots, cur++);
-      if (sym == local) v += ref_count;
-      if (!((col.valid_mask >> sym) & 1u) || sym == orig) continue;
-      if (v > thr) {
-         const uint32_t idx = atomicAdd(n_rows, 1u);
-         if (idx < row_capacity) rows[idx] = MutationRowDev{p, sym, v, total};
+   if (total != 0 && rows != nullptr) {
+      const uint32_t thr = min_proportion == 0.0
+                              ? 0u
+                              : static_cast<uint32_t>(ceil(static_cast<double>(total) * min_proportion) - 1.0);
+      const uint32_t orig = col.reference[p];
+      uint32_t cur = src.cursor(p);
+      for (uint32_t sym = 0; sym < col.n_symbols; ++sym) {
+         uint32_t v = src.stored(p, sym, cur);
+         if (sym == implicit) v += implicit_count;
+         if (!((col.valid_mask >> sym) & 1u) || sym == orig) continue;
+         if (v > thr) {
+            const uint32_t idx = atomicAdd(n_rows, 1u);
+            if (idx < row_capacity) rows[idx] = MutationRowDev{p, sym, v, total};
+         }
       }
    }
+   src.clear(p);
+}
+
+template <typename Src>
+__global__ __launch_bounds__(kPosBlock) void mutationsFinalizeKernel(
+   Src src, const int64_t* __restrict__ totals, uint32_t n_pblocks, const unsigned long long* __restrict__ filter_card,
+   long long sparse_max, double min_proportion, uint32_t* __restrict__ counts_out, MutationRowDev* __restrict__ rows,
+   uint32_t row_capacity, uint32_t* __restrict__ n_rows) {
+   if (!sourceActive(src, filter_card, sparse_max)) return;
+   finalizeBody(src, blockIdx.x, totals, n_pblocks, *filter_card, min_proportion, counts_out, rows, row_capacity, n_rows);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Batched Mutations (see MutationsBatch).
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void batchPlanKernel(MutationsBatch m) {
+   __shared__ uint32_t n_dense, n_sparse;
+   if (threadIdx.x == 0) {
+      n_dense = 0;
+      n_sparse = 0;
+   }
+   __syncthreads();
+   const uint32_t nq = *m.n_queries;
+   for (uint32_t q = threadIdx.x; q < nq && q < m.max_queries; q += blockDim.x) {
+      m.n_rows[q] = 0;
+      if (!m.is_mutations[q] || m.cards[q] == 0) continue;
+      if (m.sparse_max >= 0 && m.cards[q] <= static_cast<unsigned long long>(m.sparse_max)) {
+         m.lists[m.max_queries + atomicAdd(&n_sparse, 1u)] = q;
+      } else {
+         m.lists[atomicAdd(&n_dense, 1u)] = q;
+      }
+   }
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      m.list_counts[0] = n_dense;
+      m.list_counts[1] = n_sparse;
+   }
+}
+
+// Work-group w handles position block b of dense list entry i.  Blocks are dealt to the eight XCDs
+// (work-group w runs on XCD w % 8) so that the entries sharing a block run on one XCD, close in
+// time, and its L2 serves the block's tiles to all but the first of them.
+template <uint32_t kUnroll>
+__global__ __launch_bounds__(kMutThreads) void mutationsCountBatchKernel(MutationsBatch m) {
+   __shared__ uint64_t fs[kMutWaves][kChunkWords];
+   __shared__ uint32_t cnt[kMaxBlockSlots + 1];
+   const uint32_t n_dense = m.list_counts[0];
+   if (n_dense == 0) return;
+   const SeqColumnView& col = *m.col_dev;
+   const uint32_t w = blockIdx.x;
+   const uint32_t xcd = w & 7u;
+   const uint32_t i = w >> 3;
+   const uint32_t b = xcd + 8u * (i / n_dense);
+   if (b >= col.n_blocks) return;
+   const uint32_t e = i % n_dense;
+   const uint32_t q = m.lists[e];
+   countBlock<kUnroll>(col, b, 0, m.n_chunks, 1, m.filters + static_cast<size_t>(q) * m.n_chunks * kChunkWords,
+                       m.chunk_cards + static_cast<size_t>(q) * m.n_chunks, m.partial + static_cast<size_t>(e) * col.n_slots,
+                       fs, cnt);
+}
+
+__global__ __launch_bounds__(256) void sparseCountBatchKernel(MutationsBatch m) {
+   const uint32_t e = blockIdx.y;
+   if (e >= m.list_counts[1]) return;
+   const uint32_t q = m.lists[m.max_queries + e];
+   const SeqColumnView& col = *m.col_dev;
+   const size_t L = col.genome_length;
+   uint32_t* counts = m.sparse_counts + static_cast<size_t>(e) * col.n_symbols * L;
+   int32_t* bnd = m.sparse_bnd + static_cast<size_t>(e) * 3 * L;
+   const uint64_t* filter = m.filters + static_cast<size_t>(q) * m.n_chunks * kChunkWords;
+   const uint32_t n_words = m.n_chunks * kChunkWords;
+   for (uint32_t wd = blockIdx.x * blockDim.x + threadIdx.x; wd < n_words; wd += gridDim.x * blockDim.x) {
+      uint64_t x = filter[wd];
+      if (x == 0) continue;
+      const uint32_t size = m.chunk_sizes[wd >> 10];
+      while (x) {
+         const uint32_t r = (wd << 6) | static_cast<uint32_t>(__builtin_ctzll(x));
+         x &= x - 1;
+         if ((r & 0xFFFFu) >= size) continue;
+         const uint32_t cv = col.cov[r];
+         const uint32_t st = cv & 0xFFFFu, en = cv >> 16;
+         if (st > 0) atomicAdd(&bnd[st], 1);
+         if (en < L) atomicAdd(&bnd[L + en], 1);
+         for (uint32_t k = col.nrun_ptr[r], ke = col.nrun_ptr[r + 1]; k < ke; ++k) {
+            const uint32_t run = col.nruns[k];
+            atomicAdd(&bnd[2 * L + (run & 0xFFFFu)], 1);
+            if ((run >> 16) < L) atomicAdd(&bnd[2 * L + (run >> 16)], -1);
+         }
+         for (uint32_t k = col.row_mut_ptr[r], ke = col.row_mut_ptr[r + 1]; k < ke; ++k) {
+            const uint32_t mu = col.row_muts[k];
+            atomicAdd(&counts[(mu >> 16) * L + (mu & 0xFFFFu)], 1u);
+         }
+      }
+   }
+}
+
+__device__ inline SlotSource batchSource(const MutationsBatch& m, uint32_t e, SlotSource*) {
+   return SlotSource{m.col_dev, m.partial + static_cast<size_t>(e) * m.col_dev->n_slots, 1};
+}
+__device__ inline SparseSource batchSource(const MutationsBatch& m, uint32_t e, SparseSource*) {
+   const size_t L = m.col_dev->genome_length;
+   return SparseSource{m.col_dev, m.sparse_counts + static_cast<size_t>(e) * m.col_dev->n_symbols * L,
+                       m.sparse_bnd + static_cast<size_t>(e) * 3 * L};
+}
+__device__ inline uint32_t batchList(const SlotSource*) { return 0; }
+__device__ inline uint32_t batchList(const SparseSource*) { return 1; }
+
+template <typename Src>
+__global__ __launch_bounds__(kPosBlock) void mutationsTotalsBatchKernel(MutationsBatch m, uint32_t n_pblocks) {
+   const uint32_t list = batchList(static_cast<Src*>(nullptr));
+   const uint32_t e = blockIdx.y;
+   if (e >= m.list_counts[list]) return;
+   const Src src = batchSource(m, e, static_cast<Src*>(nullptr));
+   totalsBody(src, blockIdx.x, m.totals + (static_cast<size_t>(list) * m.max_queries + e) * n_pblocks * 3);
+}
+
+template <typename Src>
+__global__ __launch_bounds__(kPosBlock) void mutationsFinalizeBatchKernel(MutationsBatch m, uint32_t n_pblocks) {
+   const uint32_t list = batchList(static_cast<Src*>(nullptr));
+   const uint32_t e = blockIdx.y;
+   if (e >= m.list_counts[list]) return;
+   const uint32_t q = m.lists[list * m.max_queries + e];
+   const Src src = batchSource(m, e, static_cast<Src*>(nullptr));
+   finalizeBody(src, blockIdx.x, m.totals + (static_cast<size_t>(list) * m.max_queries + e) * n_pblocks * 3, n_pblocks,
+                m.cards[q], m.min_proportion[q], nullptr, m.rows + static_cast<size_t>(q) * m.row_capacity,
+                m.row_capacity, m.n_rows + q);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -886,6 +1170,84 @@ __global__ __launch_bounds__(256) void groupHistogramKernel(uint32_t n_chunks, c
    }
 }
 
+__global__ __launch_bounds__(256) void groupHistogramBatchKernel(AggregateBatch a) {
+   __shared__ uint32_t lh[kHistLdsBins];
+   const uint32_t e = blockIdx.y;
+   if (e >= *a.n_agg) return;
+   const uint32_t q = a.agg_query[e];
+   const uint32_t c = blockIdx.x >> 2;
+   const uint32_t quarter = blockIdx.x & 3;
+   if (a.chunk_cards[static_cast<size_t>(q) * a.n_chunks + c] == 0) return;
+   const uint32_t n_bins = a.bins[e];
+   const GroupDims dims = a.dims[e];
+   uint32_t* hist = a.hist + a.hist_offset[e];
+   const bool use_lds = n_bins <= kHistLdsBins;
+   if (use_lds) {
+      for (uint32_t i = threadIdx.x; i < n_bins; i += blockDim.x) lh[i] = 0;
+      __syncthreads();
+   }
+   const uint64_t* f = a.filters + static_cast<size_t>(q) * a.n_chunks * kChunkWords + static_cast<size_t>(c) * kChunkWords +
+                       quarter * 256;
+   for (uint32_t w = threadIdx.x; w < 256; w += blockDim.x) {
+      uint64_t x = f[w];
+      while (x) {
+         const uint32_t bit = __builtin_ctzll(x);
+         x &= x - 1;
+         const size_t r = (static_cast<size_t>(c) << 16) + (quarter * 256 + w) * 64 + bit;
+         uint32_t key = 0;
+         bool in_table = true;
+         for (uint32_t d = 0; d < dims.n; ++d) {
+            const uint32_t lab = dims.labels[d][r];
+            in_table &= lab != kInvalidId;
+            key += lab * dims.stride[d];
+         }
+         if (!in_table) continue;
+         if (use_lds) {
+            atomicAdd(&lh[key], 1u);
+         } else {
+            atomicAdd(&hist[key], 1u);
+         }
+      }
+   }
+   if (use_lds) {
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < n_bins; i += blockDim.x) {
+         if (lh[i]) atomicAdd(&hist[i], lh[i]);
+      }
+   }
+}
+
+__global__ void fillCountKernel(uint32_t* __restrict__ p, uint32_t value, const uint32_t* __restrict__ n_dev) {
+   const uint32_t n = *n_dev;
+   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = value;
+}
+
+__global__ __launch_bounds__(256) void packRowsKernel(const uint32_t* __restrict__ n_queries, const uint32_t* __restrict__ n_rows,
+                                                      const unsigned long long* __restrict__ cards,
+                                                      const MutationRowDev* __restrict__ rows, uint32_t row_capacity,
+                                                      uint32_t* __restrict__ header, MutationRowDev* __restrict__ packed) {
+   __shared__ uint32_t part[4];
+   const uint32_t q = blockIdx.x;
+   if (q >= *n_queries) return;
+   // offset of query q: rows kept by the queries before it
+   uint32_t acc = 0;
+   for (uint32_t j = threadIdx.x; j < q; j += blockDim.x) acc += min(n_rows[j], row_capacity);
+   acc = waveTotal(acc);
+   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+   __syncthreads();
+   const uint32_t offset = part[0] + part[1] + part[2] + part[3];
+   const uint32_t n = min(n_rows[q], row_capacity);
+   if (threadIdx.x == 0) {
+      const unsigned long long card = cards[q];
+      header[4 * q] = n_rows[q];
+      header[4 * q + 1] = offset;
+      header[4 * q + 2] = static_cast<uint32_t>(card);
+      header[4 * q + 3] = static_cast<uint32_t>(card >> 32);
+   }
+   const MutationRowDev* src = rows + static_cast<size_t>(q) * row_capacity;
+   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) packed[offset + i] = src[i];
+}
+
 __global__ void fillKernel(uint32_t* __restrict__ p, uint32_t value, size_t n) {
    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
         i += static_cast<size_t>(gridDim.x) * blockDim.x) {
@@ -904,40 +1266,124 @@ hipError_t launchFill(uint32_t* p, uint32_t value, size_t n, hipStream_t stream)
    return hipGetLastError();
 }
 
+hipError_t configureFilterKernel(size_t max_lds_bytes) {
+   if (max_lds_bytes <= 64 * 1024) return hipSuccess;
+   return hipFuncSetAttribute(reinterpret_cast<const void*>(&filterVmKernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(max_lds_bytes));
+}
+
 size_t filterLdsBytes(uint32_t depth) { return static_cast<size_t>(kVmWaves) * (depth + 1) * 64 * sizeof(uint64_t); }
 
 hipError_t launchFilter(const TableView& tv, const ProgramDesc* progs, const Instr* instrs, const uint32_t* ranges,
                         uint32_t n_queries, uint32_t depth, uint64_t* out, unsigned long long* cards,
-                        uint32_t* chunk_cards, uint32_t max_blocks, hipStream_t stream) {
+                        uint32_t* chunk_cards, uint32_t max_blocks, hipStream_t stream,
+                        const uint32_t* n_queries_dev) {
    const uint32_t n_items = n_queries * tv.n_chunks * kSlicesPerChunk;
    if (n_items == 0) return hipSuccess;
    uint32_t blocks = (n_items + kVmWaves - 1) / kVmWaves;
    if (max_blocks && blocks > max_blocks) blocks = max_blocks;
    hipLaunchKernelGGL(filterVmKernel, dim3(blocks), dim3(kVmThreads), filterLdsBytes(depth), stream, tv, progs,
-                      instrs, ranges, n_items, depth, out, cards, chunk_cards);
+                      instrs, ranges, n_items, depth, out, cards, chunk_cards, n_queries_dev);
    return hipGetLastError();
 }
 
-hipError_t launchMutations(const SeqColumnView* col_dev, const SeqColumnView& col_host, uint32_t n_chunks,
-                           uint32_t n_groups, const uint64_t* filter, const uint32_t* filter_chunk_cards,
-                           const unsigned long long* filter_card, uint32_t* partial, int64_t* totals,
-                           double min_proportion, uint32_t* counts_out, MutationRowDev* rows, uint32_t row_capacity,
-                           uint32_t* n_rows, hipStream_t stream, hipEvent_t count_start, hipEvent_t count_stop) {
-   if (n_groups == 0 || n_groups > kMaxGroups) return hipErrorInvalidValue;
+hipError_t launchMutations(const MutationsLaunch& m, hipStream_t stream) {
+   if (m.n_groups == 0 || m.n_groups > kMaxGroups) return hipErrorInvalidValue;
+   if (m.sparse_max >= 0 && (m.sparse_counts == nullptr || m.sparse_bnd == nullptr)) return hipErrorInvalidValue;
    hipError_t err = hipSuccess;
-   if (count_start && (err = hipEventRecord(count_start, stream)) != hipSuccess) return err;
-   hipLaunchKernelGGL(mutationsCountKernel, dim3(col_host.n_blocks, n_groups), dim3(kMutThreads), 0, stream, col_dev,
-                      n_chunks, n_groups, filter, filter_chunk_cards, partial);
-   err = hipGetLastError();
+   const SeqColumnView& h = *m.col_host;
+   if (m.count_start && (err = hipEventRecord(m.count_start, stream)) != hipSuccess) return err;
+   if (m.unroll == 16) {
+      hipLaunchKernelGGL(mutationsCountKernel<16>, dim3(h.n_blocks, m.n_groups), dim3(kMutThreads), 0, stream, m.col_dev,
+                         m.n_chunks, m.n_groups, m.filter, m.filter_chunk_cards, m.filter_card, m.sparse_max, m.partial);
+   } else {
+      hipLaunchKernelGGL(mutationsCountKernel<8>, dim3(h.n_blocks, m.n_groups), dim3(kMutThreads), 0, stream, m.col_dev,
+                         m.n_chunks, m.n_groups, m.filter, m.filter_chunk_cards, m.filter_card, m.sparse_max, m.partial);
+   }
+   if ((err = hipGetLastError()) != hipSuccess) return err;
+   if (m.count_stop && (err = hipEventRecord(m.count_stop, stream)) != hipSuccess) return err;
+   if (m.n_groups > 1) {
+      hipLaunchKernelGGL(reducePartialsKernel, dim3((h.n_slots + 255) / 256), dim3(256), 0, stream, m.partial,
+                         m.n_groups, h.n_slots, m.filter_card, m.sparse_max);
+      if ((err = hipGetLastError()) != hipSuccess) return err;
+   }
+   const uint32_t n_pblocks = (h.genome_length + kPosBlock - 1) / kPosBlock;
+   const SlotSource dense{m.col_dev, m.partial, 1};
+   hipLaunchKernelGGL(mutationsTotalsKernel<SlotSource>, dim3(n_pblocks), dim3(kPosBlock), 0, stream, dense,
+                      m.filter_card, m.sparse_max, m.totals, m.n_rows);
+   if ((err = hipGetLastError()) != hipSuccess) return err;
+   hipLaunchKernelGGL(mutationsFinalizeKernel<SlotSource>, dim3(n_pblocks), dim3(kPosBlock), 0, stream, dense,
+                      m.totals, n_pblocks, m.filter_card, m.sparse_max, m.min_proportion, m.counts_out, m.rows,
+                      m.row_capacity, m.n_rows);
+   if ((err = hipGetLastError()) != hipSuccess) return err;
+   if (m.sparse_max < 0) return hipSuccess;
+   const uint32_t n_words = m.n_chunks * kChunkWords;
+   hipLaunchKernelGGL(sparseCountKernel, dim3((n_words + 255) / 256), dim3(256), 0, stream, m.col_dev, m.chunk_sizes,
+                      n_words, m.filter, m.filter_card, m.sparse_max, m.sparse_counts, m.sparse_bnd);
+   if ((err = hipGetLastError()) != hipSuccess) return err;
+   const SparseSource sparse{m.col_dev, m.sparse_counts, m.sparse_bnd};
+   hipLaunchKernelGGL(mutationsTotalsKernel<SparseSource>, dim3(n_pblocks), dim3(kPosBlock), 0, stream, sparse,
+                      m.filter_card, m.sparse_max, m.totals, m.n_rows);
+   if ((err = hipGetLastError()) != hipSuccess) return err;
+   hipLaunchKernelGGL(mutationsFinalizeKernel<SparseSource>, dim3(n_pblocks), dim3(kPosBlock), 0, stream, sparse,
+                      m.totals, n_pblocks, m.filter_card, m.sparse_max, m.min_proportion, m.counts_out, m.rows,
+                      m.row_capacity, m.n_rows);
+   return hipGetLastError();
+}
+
+hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
+   if (m.max_queries == 0) return hipSuccess;
+   if (m.max_queries > 65535) return hipErrorInvalidValue;
+   const SeqColumnView& h = *m.col_host;
+   hipError_t err = hipSuccess;
+   hipLaunchKernelGGL(batchPlanKernel, dim3(1), dim3(1024), 0, stream, m);
+   if ((err = hipGetLastError()) != hipSuccess) return err;
+   const uint32_t n_xblocks = 8 * ((h.n_blocks + 7) / 8);
+   const dim3 count_grid(n_xblocks * m.max_queries);
+   if (m.unroll == 16) {
+      hipLaunchKernelGGL(mutationsCountBatchKernel<16>, count_grid, dim3(kMutThreads), 0, stream, m);
+   } else {
+      hipLaunchKernelGGL(mutationsCountBatchKernel<8>, count_grid, dim3(kMutThreads), 0, stream, m);
+   }
+   if ((err = hipGetLastError()) != hipSuccess) return err;
+   const uint32_t n_pblocks = (h.genome_length + kPosBlock - 1) / kPosBlock;
+   hipLaunchKernelGGL(mutationsTotalsBatchKernel<SlotSource>, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0, stream,
+                      m, n_pblocks);
+   if ((err = hipGetLastError()) != hipSuccess) return err;
+   hipLaunchKernelGGL(mutationsFinalizeBatchKernel<SlotSource>, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0,
+                      stream, m, n_pblocks);
+   if ((err = hipGetLastError()) != hipSuccess) return err;
+   if (m.sparse_max < 0) return hipSuccess;
+   const uint32_t n_words = m.n_chunks * kChunkWords;
+   hipLaunchKernelGGL(sparseCountBatchKernel, dim3((n_words + 255) / 256, m.max_queries), dim3(256), 0, stream, m);
+   if ((err = hipGetLastError()) != hipSuccess) return err;
+   hipLaunchKernelGGL(mutationsTotalsBatchKernel<SparseSource>, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0,
+                      stream, m, n_pblocks);
+   if ((err = hipGetLastError()) != hipSuccess) return err;
+   hipLaunchKernelGGL(mutationsFinalizeBatchKernel<SparseSource>, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0,
+                      stream, m, n_pblocks);
+   return hipGetLastError();
+}
+
+hipError_t launchAggregateBatch(const AggregateBatch& a, hipStream_t stream) {
+   if (a.max_entries == 0) return hipSuccess;
+   if (a.max_entries > 65535) return hipErrorInvalidValue;
+   uint32_t blocks = (a.max_total_bins + 255) / 256;
+   if (blocks > 1024) blocks = 1024;
+   if (blocks == 0) blocks = 1;
+   hipLaunchKernelGGL(fillCountKernel, dim3(blocks), dim3(256), 0, stream, a.hist, 0u, a.total_bins);
+   hipError_t err = hipGetLastError();
    if (err != hipSuccess) return err;
-   if (count_stop && (err = hipEventRecord(count_stop, stream)) != hipSuccess) return err;
-   const uint32_t n_pblocks = (col_host.genome_length + kPosBlock - 1) / kPosBlock;
-   hipLaunchKernelGGL(mutationsTotalsKernel, dim3(n_pblocks), dim3(kPosBlock), 0, stream, col_dev, partial, n_groups,
-                      totals, n_rows);
-   err = hipGetLastError();
-   if (err != hipSuccess) return err;
-   hipLaunchKernelGGL(mutationsFinalizeKernel, dim3(n_pblocks), dim3(kPosBlock), 0, stream, col_dev, partial, n_groups,
-                      totals, n_pblocks, filter_card, min_proportion, counts_out, rows, row_capacity, n_rows);
+   hipLaunchKernelGGL(groupHistogramBatchKernel, dim3(a.n_chunks * 4, a.max_entries), dim3(256), 0, stream, a);
+   return hipGetLastError();
+}
+
+hipError_t launchPackRows(const uint32_t* n_queries, uint32_t max_queries, const uint32_t* n_rows,
+                          const unsigned long long* cards, const MutationRowDev* rows, uint32_t row_capacity,
+                          uint32_t* header, MutationRowDev* packed, hipStream_t stream) {
+   if (max_queries == 0) return hipSuccess;
+   hipLaunchKernelGGL(packRowsKernel, dim3(max_queries), dim3(256), 0, stream, n_queries, n_rows, cards, rows,
+                      row_capacity, header, packed);
    return hipGetLastError();
 }
 

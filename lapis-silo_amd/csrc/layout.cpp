@@ -378,6 +378,19 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
             ++count;
          }
          nptr[i + 1] = count;
+         if (!is_null[i]) {
+            // the diff form of aligned_sequence.cpp:18-122: a mutation lies inside the covered
+            // range, outside every missing run, and differs from the reference symbol
+            const uint32_t* rr = out.data() + (out.size() - count);
+            uint32_t j = 0;
+            for (uint64_t k = mut_ptr[i]; k < mut_ptr[i + 1]; ++k) {
+               const uint32_t p = mut_pos[k];
+               if (p < cov_start[i] || p >= cov_end[i]) fail(RDB_EINVAL, "mutation outside the covered range");
+               if (mut_sym[k] == col.reference[p]) fail(RDB_EINVAL, "mutation equal to the reference symbol");
+               while (j < count && (rr[j] >> 16) <= p) ++j;
+               if (j < count && (rr[j] & 0xFFFFu) <= p) fail(RDB_EINVAL, "mutation inside a missing run");
+            }
+         }
       }
    });
    for (uint64_t i = 0; i < n; ++i) nptr[i + 1] += nptr[i];
@@ -450,7 +463,9 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
             push(0, nsym + BND_COV_END, r);  // no covered range: uncovered everywhere
             continue;
          }
-         const uint32_t s0 = cov_start[i], e0 = cov_end[i];
+         // an empty covered range is stored as [0, 0): uncovered from position 0 on
+         const bool has_cov = cov_start[i] < cov_end[i];
+         const uint32_t s0 = has_cov ? cov_start[i] : 0, e0 = has_cov ? cov_end[i] : 0;
          for (uint64_t k = mut_ptr[i]; k < mut_ptr[i + 1]; ++k) {
             const uint32_t p = mut_pos[k];
             if (flip_to[p] != 0xFF && mut_sym[k] == flip_to[p]) continue;
@@ -656,7 +671,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
             null_bits[g >> 6] |= 1ULL << (g & 63);
             null_rows.push_back(static_cast<uint32_t>(g));
          } else {
-            cov[g] = cov_start[i] | (cov_end[i] << 16);
+            if (cov_start[i] < cov_end[i]) cov[g] = cov_start[i] | (cov_end[i] << 16);  // empty range: [0, 0)
          }
          col.chunk_nruns[c] += nptr[i + 1] - nptr[i];
       }
@@ -679,6 +694,29 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    }
    col.has_nulls = !null_rows.empty();
    col.null_bitmap_id = t.addStoreBitmap(null_rows);
+
+   // row-major mutations over padded rows (null rows have none)
+   std::vector<uint32_t> rmptr(padded + 1, 0), rmuts;
+   {
+      uint64_t total = 0;
+      for (uint64_t i = 0; i < n; ++i) {
+         if (!is_null[i]) total += mut_ptr[i + 1] - mut_ptr[i];
+      }
+      if (total > 0xFFFFFFFFULL) fail(RDB_ENOMEM, "too many mutations for 32-bit offsets");
+      rmuts.resize(total);
+      uint32_t acc = 0;
+      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+         for (uint32_t r = 0; r < kChunkRows; ++r) {
+            const size_t g = (static_cast<size_t>(c) << 16) | r;
+            rmptr[g] = acc;
+            if (r >= t.chunk_sizes[c]) continue;
+            const uint64_t i = t.chunk_offset[c] + r;
+            if (is_null[i]) continue;
+            for (uint64_t k = mut_ptr[i]; k < mut_ptr[i + 1]; ++k) rmuts[acc++] = mut_pos[k] | (static_cast<uint32_t>(mut_sym[k]) << 16);
+         }
+      }
+      rmptr[padded] = acc;
+   }
 
    // --- checks ------------------------------------------------------------------------------------
    validateDescriptors(desc.data(), desc.size(), arr_all.size(), run_all.size(), bit_all.size() / kChunkWords, kMaxBlockSlots);
@@ -744,6 +782,11 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    v.nrun_ptr = static_cast<const uint32_t*>(up(pptr.data(), pptr.size() * 4));
    v.nruns = static_cast<const uint32_t*>(up(runs_all.data(), runs_all.size() * 4));
    v.null_bits = static_cast<const uint64_t*>(up(null_bits.data(), null_bits.size() * 8));
+   v.row_mut_ptr = static_cast<const uint32_t*>(up(rmptr.data(), rmptr.size() * 4));
+   v.row_muts = static_cast<const uint32_t*>(up(rmuts.data(), rmuts.size() * 4));
+   col.row_major_bytes = (rmptr.size() + rmuts.size()) * 4;
+   col.row_mut_ptr = std::move(rmptr);
+   col.row_nrun_ptr = std::move(pptr);
 }
 
 // ---- metadata columns --------------------------------------------------------------------------
@@ -881,10 +924,18 @@ void finalizeTable(Table& t) {
       max_L = std::max(max_L, c->genome_length);
    }
    t.slot_counts = DeviceBuffer(static_cast<size_t>(max_slots) * 4 * kMaxGroups);
-   t.totals = DeviceBuffer(static_cast<size_t>((max_L + 1023) / 1024) * 3 * sizeof(int64_t));
+   t.totals = DeviceBuffer(static_cast<size_t>((max_L + kMutPosBlock - 1) / kMutPosBlock) * 3 * sizeof(int64_t));
    // at least the 2048 rows the Mutations path copies back in its first transfer
    t.mut_rows = DeviceBuffer(16 + std::max<size_t>(static_cast<size_t>(max_L) * 32, 2048) * sizeof(MutationRowDev));
    t.mut_n = DeviceBuffer(64);
+   for (auto& c : t.seq) {
+      // sparse-path accumulators start (and are left) zero
+      const size_t L = c->genome_length;
+      c->sparse_counts = DeviceBuffer(L * c->abc->count * 4);
+      c->sparse_bnd = DeviceBuffer(L * 3 * 4);
+      hipCheck(hipMemset(c->sparse_counts.as<void>(), 0, c->sparse_counts.bytes()), "clear sparse counts");
+      hipCheck(hipMemset(c->sparse_bnd.as<void>(), 0, c->sparse_bnd.bytes()), "clear sparse boundaries");
+   }
    t.finalized = true;
 }
 

@@ -47,7 +47,7 @@ DeviceBitmap runProgram(Table& t, const Emitter& program) {
    Context& ctx = *t.ctx;
    if (program.sp != 1) fail(RDB_ECOMPILE, "internal: filter program leaves " + std::to_string(program.sp) + " results");
    const uint32_t depth = std::max<uint32_t>(program.max_sp, 1);
-   if (filterLdsBytes(depth) > 64 * 1024) fail(RDB_EQUERY, "filter expression nests too deeply");
+   if (depth > maxFilterDepth(ctx)) fail(RDB_EQUERY, "filter expression nests too deeply");
    DeviceBitmap out(&t);
    hipCheck(launchFill(reinterpret_cast<uint32_t*>(out.card_dev), 0, (64 + 4 * static_cast<size_t>(t.n_chunks)) / 4, ctx.stream),
             "clear cardinalities");
@@ -96,6 +96,14 @@ std::unique_ptr<Predicate> CompareToValueSelection::negate() const {
 
 void CompareToValueSelection::emit(Emitter& e) const {
    e.leaf(Instr{OP_CMP, static_cast<uint32_t>(column_), cmpCode(cmp_), static_cast<uint32_t>(value_), with_nulls_ ? 1u : 0u, 0, 0, 0});
+}
+
+bool CompareToValueSelection::emitPair(Emitter& e, const Predicate* other) const {
+   const auto* o = dynamic_cast<const CompareToValueSelection*>(other);
+   if (o == nullptr || o->column_ != column_) return false;
+   e.leaf(Instr{OP_CMP, static_cast<uint32_t>(column_), cmpCode(cmp_), static_cast<uint32_t>(value_), with_nulls_ ? 1u : 0u,
+                cmpCode(o->cmp_), static_cast<uint32_t>(o->value_), 1u | (o->with_nulls_ ? 2u : 0u)});
+   return true;
 }
 
 std::string IsInCoveredRegion::toString() const {
@@ -237,15 +245,21 @@ std::string Selection::toString() const {
 // selection.cpp:94-141: rows of the child (when present) that satisfy every predicate
 void Selection::emit(Emitter& e) const {
    if (predicates.empty()) fail(RDB_ECOMPILE, "Selection without predicates");
+   // predicates are conjoined; two adjacent comparisons of one column become one instruction
+   auto emitFrom = [&](size_t i) -> size_t {
+      const auto* c = dynamic_cast<const CompareToValueSelection*>(predicates[i].get());
+      if (c != nullptr && i + 1 < predicates.size() && c->emitPair(e, predicates[i + 1].get())) return i + 2;
+      predicates[i]->emit(e);
+      return i + 1;
+   };
    size_t i = 0;
    if (child_operator.has_value()) {
       (*child_operator)->emit(e);
    } else {
-      predicates[0]->emit(e);
-      i = 1;
+      i = emitFrom(0);
    }
-   for (; i < predicates.size(); ++i) {
-      predicates[i]->emit(e);
+   while (i < predicates.size()) {
+      i = emitFrom(i);
       e.binary(OP_AND);
    }
 }

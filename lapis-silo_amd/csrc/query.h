@@ -80,6 +80,9 @@ class CompareToValueSelection : public Predicate {
    }
    std::unique_ptr<Predicate> negate() const override;
    void emit(Emitter& e) const override;
+   // one instruction for this predicate and, when `other` compares the same column, that one too
+   // (their conjunction: a value range reads the column once)
+   bool emitPair(Emitter& e, const Predicate* other) const;
 
   private:
    const Table* table_;

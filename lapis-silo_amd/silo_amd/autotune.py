@@ -1,0 +1,106 @@
+"""Autotuner of the engine's kernel knobs on a resident table (rdb_set_tuning).
+
+It measures, on the GPU and on the table at hand:
+
+* the dense Mutations kernel's shape: loads in flight per lane ("mutation_unroll") and chunk groups
+  ("mutation_groups"), timed with HIP events on a filter that touches every chunk;
+* the dense/sparse switch ("sparse_max_rows"): the dense method costs about the same for any filter
+  that touches every chunk, the sparse one grows with the filter's rows; both are timed on filters
+  of several cardinalities, the sparse time is fitted as a + b * rows, and the switch is put where
+  the fitted line meets the dense time.
+
+No knob changes a result; the tuner only picks the fastest settings.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+
+def _time_query(table, expr: str, column: str, reps: int) -> float:
+    table.mutations_query(expr, column, 0.05)
+    t = time.perf_counter()
+    for _ in range(reps):
+        table.mutations_query(expr, column, 0.05)
+    return (time.perf_counter() - t) / reps
+
+
+def _kernel_us(engine, table, expr: str, column: str, reps: int) -> float:
+    engine.profile(True)
+    for _ in range(reps):
+        table.mutations_query(expr, column, 0.05)
+    p = engine.profile_read()
+    engine.profile(False)
+    return p["mutations_ms"] / max(p["mutations_launches"], 1) * 1e3
+
+
+def autotune(engine, table, probes: list, column: str = "main", reps: int = 5, log=None) -> dict:
+    """`probes`: filter expressions spanning small to large cardinalities (at least one touching
+    every chunk).  Returns the chosen knobs and the measurements; the knobs are applied."""
+    say = log or (lambda *_: None)
+    cards = []
+    for e in probes:
+        with table.bitmap(e) as b:
+            cards.append(b.cardinality)
+    order = np.argsort(cards)
+    probes = [probes[i] for i in order]
+    cards = [cards[i] for i in order]
+    broad = probes[-1]
+    engine.set_tuning("sparse_max_rows", -1)
+    # 1. dense kernel shape
+    shapes = {}
+    for unroll in (8, 16):
+        for groups in (0, 8, 4):
+            engine.set_tuning("mutation_unroll", unroll)
+            engine.set_tuning("mutation_groups", groups)
+            shapes[(unroll, groups)] = _kernel_us(engine, table, broad, column, reps)
+    best = min(shapes, key=shapes.get)
+    engine.set_tuning("mutation_unroll", best[0])
+    engine.set_tuning("mutation_groups", best[1])
+    say(f"dense kernel shapes (unroll, groups) -> us: {shapes}; chose {best}")
+    # 2. dense/sparse switch from per-query wall times
+    dense = []
+    sparse = []
+    for e in probes:
+        engine.set_tuning("sparse_max_rows", -1)
+        dense.append(_time_query(table, e, column, reps))
+        engine.set_tuning("sparse_max_rows", 1 << 40)
+        sparse.append(_time_query(table, e, column, reps))
+    engine.set_tuning("sparse_max_rows", -1)
+    x = np.asarray(cards, dtype=np.float64)
+    ys = np.asarray(sparse)
+    # least squares a + b * rows over the probes
+    A = np.stack([np.ones_like(x), x], axis=1)
+    (a, b), *_ = np.linalg.lstsq(A, ys, rcond=None)
+    t_dense = float(np.median(dense))
+    if b <= 0:
+        threshold = int(x.max())
+    else:
+        threshold = int(max(0.0, (t_dense - a) / b))
+    # never above the largest measured filter on which sparse still won
+    won = [c for c, d, s in zip(cards, dense, sparse) if s < d]
+    if won:
+        threshold = int(min(max(threshold, max(won)), 2 * max(won)))
+    else:
+        threshold = 0
+    engine.set_tuning("sparse_max_rows", threshold)
+    say(f"dense/sparse per-query seconds: {[(c, round(d * 1e6, 1), round(s * 1e6, 1)) for c, d, s in zip(cards, dense, sparse)]}; "
+        f"sparse fit a={a * 1e6:.1f}us b={b * 1e9:.2f}ns/row; switch at {threshold} rows")
+    return {"mutation_unroll": best[0], "mutation_groups": best[1], "sparse_max_rows": threshold,
+            "dense_shapes_us": {f"{k[0]}x{k[1]}": round(v, 2) for k, v in shapes.items()},
+            "probe_cards": [int(c) for c in cards],
+            "dense_us": [round(d * 1e6, 1) for d in dense], "sparse_us": [round(s * 1e6, 1) for s in sparse]}
+
+
+def probe_filters(ds, n: int = 8) -> list:
+    """Lineage and country equality filters whose expected selectivities spread log-uniformly
+    over 1e-5 .. 0.5 (the generator's own value frequencies), plus the full table."""
+    lw = ds.lineage_weight / ds.lineage_weight.sum()
+    out = []
+    for s in np.logspace(math.log10(1e-5), math.log10(0.05), n):
+        v = int(np.argmin(np.abs(np.log(lw) - math.log(s))))
+        out.append(f'(lineage lineage "{ds.lineage_names[v]}" 0)')
+    out.append("(true)")
+    return list(dict.fromkeys(out))

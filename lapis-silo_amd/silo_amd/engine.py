@@ -45,8 +45,24 @@ SIGNATURES = [
     ("rdb_profile", _int, [_vp, _int]),
     ("rdb_profile_read", _int, [_vp, _vp, _vp, _vp, _vp]),
     ("rdb_mutations_traffic", _int, [_vp, _vp, _cp, _vp]),
+    ("rdb_batch_submit", _int, [_vp, _vp, _u32, ctypes.POINTER(_vp)]),
+    ("rdb_batch_wait", _int, [_vp]),
+    ("rdb_batch_count", _int, [_vp, _u32, _vp]),
+    ("rdb_batch_mutations", _int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
+    ("rdb_batch_aggregate", _int, [_vp, _u32, _vp, _vp, _u64, _vp]),
+    ("rdb_batch_free", None, [_vp]),
+    ("rdb_set_tuning", _int, [_vp, _cp, ctypes.c_int64]),
+    ("rdb_get_tuning", _int, [_vp, _cp, _vp]),
     ("rdb_diff_sequence", _int, [_int, _cp, _u32, _cp, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _u64, _vp]),
 ]
+
+class RdbQuery(ctypes.Structure):
+    """struct rdb_query of include/rdb.h."""
+    _fields_ = [("filter", ctypes.c_char_p), ("action", ctypes.c_int), ("column", ctypes.c_char_p),
+                ("min_proportion", ctypes.c_double), ("dims", ctypes.c_char_p)]
+
+
+ACTION_COUNT, ACTION_MUTATIONS, ACTION_AGGREGATE = 0, 1, 2
 
 _bound = False
 
@@ -121,6 +137,15 @@ class Engine:
                self._h)
         return {"mutations_ms": mm.value, "mutations_launches": mn.value, "filter_ms": fm.value,
                 "filter_launches": fn.value}
+
+    def set_tuning(self, key: str, value: int) -> None:
+        """Kernel knobs: "sparse_max_rows" (-1 = dense only), "mutation_groups" (0 = automatic)."""
+        _check(lib().rdb_set_tuning(self._h, key.encode(), int(value)), self._h)
+
+    def get_tuning(self, key: str) -> int:
+        v = ctypes.c_int64()
+        _check(lib().rdb_get_tuning(self._h, key.encode(), ctypes.byref(v)), self._h)
+        return v.value
 
     def close(self) -> None:
         for t in self._tables:
@@ -277,7 +302,7 @@ class DeviceTable:
         return self._with_bitmap(filt, run)
 
     def _row_buffers(self, column: str):
-        cap = 32 * self.seq_lengths[column] + 16
+        cap = 32 * self.seq_lengths.get(column, 0) + 16  # an unknown column fails in the call itself
         key = (column, cap)
         bufs = getattr(self, "_bufs", {})
         if key not in bufs:
@@ -324,3 +349,88 @@ class DeviceTable:
                 out.append((key, int(counts[i])))
             return out
         return self._with_bitmap(filt, run)
+
+    # ---- batched path --------------------------------------------------------------------------
+    def batch_submit(self, queries: list) -> "Batch":
+        """Submits dicts {"filter", "action": "count"|"mutations"|"aggregate", "column",
+        "min_proportion", "dims": [...]} as one batch; returns without waiting."""
+        arr = (RdbQuery * max(len(queries), 1))()
+        keep = []
+        for i, q in enumerate(queries):
+            act = {"count": ACTION_COUNT, "mutations": ACTION_MUTATIONS, "aggregate": ACTION_AGGREGATE}[q["action"]]
+            f = q["filter"].encode()
+            col = q.get("column", "").encode()
+            dims = ",".join(q.get("dims", [])).encode()
+            keep += [f, col, dims]
+            arr[i] = RdbQuery(f, act, col, float(q.get("min_proportion", 0.0)), dims)
+        h = _vp()
+        _check(lib().rdb_batch_submit(self._h, arr, len(queries), ctypes.byref(h)), self._ctx())
+        return Batch(self, h, queries)
+
+    def batch(self, queries: list) -> list:
+        """Runs a batch and returns per query: its cardinality, Mutations rows or groups (or the
+        EngineError of a query that failed)."""
+        with self.batch_submit(queries) as b:
+            return b.results()
+
+
+class Batch:
+    def __init__(self, table: DeviceTable, handle, queries: list):
+        self.table = table
+        self._h = handle
+        self.queries = queries
+
+    def wait(self) -> None:
+        _check(lib().rdb_batch_wait(self._h), self.table._ctx())
+
+    def count(self, q: int) -> int:
+        n = _u64()
+        _check(lib().rdb_batch_count(self._h, q, ctypes.byref(n)), self.table._ctx())
+        return n.value
+
+    def mutations(self, q: int) -> list:
+        cap, (pos, fr, to, cnt, cov, prop) = self.table._row_buffers(self.queries[q]["column"])
+        n = _u64()
+        _check(lib().rdb_batch_mutations(self._h, q, _p(pos), _p(fr), _p(to), _p(cnt), _p(cov), _p(prop), cap,
+                                         ctypes.byref(n)), self.table._ctx())
+        k = n.value
+        return list(zip(pos[:k].tolist(), fr[:k].tolist(), to[:k].tolist(), cnt[:k].tolist(), cov[:k].tolist(),
+                        prop[:k].tolist()))
+
+    def aggregate(self, q: int) -> list:
+        dims = self.queries[q].get("dims", [])
+        cap = 1 << 20
+        ids = np.zeros(cap * max(len(dims), 1), np.uint32)
+        counts = np.zeros(cap, np.uint64)
+        n = _u64()
+        _check(lib().rdb_batch_aggregate(self._h, q, _p(ids), _p(counts), cap, ctypes.byref(n)), self.table._ctx())
+        out = []
+        for i in range(n.value):
+            key = tuple(None if int(x) == 0xFFFFFFFF else int(x) for x in ids[i * len(dims):(i + 1) * len(dims)])
+            out.append((key, int(counts[i])))
+        return out
+
+    def results(self) -> list:
+        out = []
+        for i, q in enumerate(self.queries):
+            try:
+                if q["action"] == "mutations":
+                    out.append(self.mutations(i))
+                elif q["action"] == "aggregate":
+                    out.append(self.aggregate(i))
+                else:
+                    out.append(self.count(i))
+            except EngineError as e:
+                out.append(e)
+        return out
+
+    def free(self) -> None:
+        if self._h:
+            lib().rdb_batch_free(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()

@@ -1,0 +1,133 @@
+"""The batched path (rdb_batch_*) against the oracle: every query of a batch, whatever its action,
+lane job, dense or sparse method and graph replay, returns what the oracle returns for it alone."""
+import numpy as np
+import pytest
+
+from helpers import ExprGen, random_bitmaps, small_spec
+
+pytestmark = pytest.mark.gpu
+
+KNOBS = ("sparse_max_rows", "batch_max_queries", "batch_lanes", "batch_graphs", "batch_row_capacity")
+
+
+@pytest.fixture(scope="module")
+def small(engine):
+    from oracle.oracle import OracleTable
+
+    ds, ts = small_spec(seed=21, n_rows=5000, L=500, chunk=2048)
+    ts.bitmaps = random_bitmaps(ts, seed=6)
+    ot = OracleTable(ts)
+    et = engine.table(ts)
+    yield ds, ts, ot, et
+    et.close()
+    ot.close()
+
+
+@pytest.fixture
+def tuning(engine):
+    saved = {k: engine.get_tuning(k) for k in KNOBS}
+    yield engine
+    for k, v in saved.items():
+        engine.set_tuning(k, v)
+
+
+def make_queries(ds, seed, n, bitmaps=0):
+    gen = ExprGen(ds, seed=seed, bitmaps=bitmaps)
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        e = gen.expr(int(rng.integers(1, 4)))
+        kind = i % 4
+        if kind == 0:
+            out.append({"filter": e, "action": "count"})
+        elif kind in (1, 2):
+            out.append({"filter": e, "action": "mutations", "column": "main",
+                        "min_proportion": float(rng.choice([0.0, 0.02, 0.05, 0.5, 1.0]))})
+        else:
+            dims = [["country"], ["lineage", "country"], []][int(rng.integers(0, 3))]
+            out.append({"filter": e, "action": "aggregate", "dims": dims})
+    return out
+
+
+def expected(ot, q):
+    if q["action"] == "count":
+        return ot.count(q["filter"])
+    if q["action"] == "mutations":
+        return ot.mutations(q["filter"], q["column"], q["min_proportion"])
+    return ot.aggregate(q["filter"], q["dims"])
+
+
+def check(et, ot, queries):
+    got = et.batch(queries)
+    assert len(got) == len(queries)
+    for q, g in zip(queries, got):
+        assert g == expected(ot, q), q
+
+
+@pytest.mark.parametrize("graphs", [0, 1])
+@pytest.mark.parametrize("sparse_max", [-1, 300, 1 << 40])
+def test_batch_matches_oracle(small, tuning, graphs, sparse_max):
+    ds, ts, ot, et = small
+    tuning.set_tuning("batch_graphs", graphs)
+    tuning.set_tuning("sparse_max_rows", sparse_max)
+    tuning.set_tuning("batch_max_queries", 16)  # several lane jobs per batch
+    check(et, ot, make_queries(ds, seed=100 + graphs, n=60, bitmaps=len(ts.bitmaps)))
+
+
+def test_batch_replays_stay_exact(small, tuning):
+    """The same captured graph replayed for several batches, with single-query work between them."""
+    ds, ts, ot, et = small
+    tuning.set_tuning("batch_graphs", 1)
+    tuning.set_tuning("sparse_max_rows", 500)
+    tuning.set_tuning("batch_max_queries", 32)
+    queries = make_queries(ds, seed=7, n=32)
+    want = [expected(ot, q) for q in queries]
+    for rep in range(4):
+        assert et.batch(queries) == want, rep
+        assert et.count("(has_mut main 17)") == ot.count("(has_mut main 17)")
+    other = make_queries(ds, seed=8, n=20)
+    assert et.batch(other) == [expected(ot, q) for q in other]
+    assert et.batch(queries) == want
+
+
+def test_batch_overlapping_submissions(small, tuning):
+    """Batches submitted before earlier ones finished (more jobs than lanes) keep their results."""
+    ds, ts, ot, et = small
+    tuning.set_tuning("batch_lanes", 2)
+    tuning.set_tuning("batch_max_queries", 8)
+    tuning.set_tuning("sparse_max_rows", 200)
+    qsets = [make_queries(ds, seed=200 + k, n=20) for k in range(4)]
+    handles = [et.batch_submit(qs) for qs in qsets]
+    try:
+        # collected newest first: older batches' jobs were retired by later submissions
+        for qs, h in zip(reversed(qsets), reversed(handles)):
+            assert h.results() == [expected(ot, q) for q in qs]
+    finally:
+        for h in handles:
+            h.free()
+
+
+def test_batch_row_overflow_reruns(small, tuning):
+    ds, ts, ot, et = small
+    tuning.set_tuning("batch_row_capacity", 16)
+    queries = [{"filter": "(true)", "action": "mutations", "column": "main", "min_proportion": 0.0},
+               {"filter": "(has_mut main 17)", "action": "mutations", "column": "main", "min_proportion": 0.0}]
+    check(et, ot, queries)
+
+
+def test_batch_query_errors_are_per_query(small):
+    from silo_amd.engine import EngineError
+
+    ds, ts, ot, et = small
+    queries = [{"filter": "(has_mut main 17)", "action": "count"},
+               {"filter": "(sym_eq nosuch 3 A)", "action": "count"},
+               {"filter": "(true)", "action": "mutations", "column": "nosuch", "min_proportion": 0.05},
+               {"filter": "(true", "action": "count"},
+               {"filter": "(true)", "action": "mutations", "column": "main", "min_proportion": 1.5},
+               {"filter": "(true)", "action": "aggregate", "dims": ["main@3"]},
+               {"filter": "(str_eq country null)", "action": "aggregate", "dims": ["lineage"]}]
+    got = et.batch(queries)
+    assert got[0] == ot.count("(has_mut main 17)")
+    for g in got[1:6]:
+        assert isinstance(g, EngineError) and g.status == "RDB_EQUERY", g
+    assert got[6] == ot.aggregate("(str_eq country null)", ["lineage"])

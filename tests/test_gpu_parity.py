@@ -69,3 +69,93 @@ def test_aggregate(small):
     for e in exprs:
         for dims in (["country"], ["lineage", "country"], ["main@33"], ["country", "main@5"], []):
             assert et.aggregate(e, dims) == ot.aggregate(e, dims), (e, dims)
+
+
+@pytest.mark.parametrize("sparse_max", [0, 40, 1 << 40])
+def test_mutations_sparse_switch(small, engine, sparse_max):
+    """The row-walking method (filters of at most sparse_max rows) and the index-streaming one give
+    the same counts and rows; 0 sends only empty filters to it, 2^40 sends every filter."""
+    ds, ts, ot, et = small
+    gen = ExprGen(ds, seed=9)
+    exprs = ["(true)", "(false)", f'(str_eq country "{ds.country_names[0]}")', "(has_mut main 17)",
+             "(nof 2 0 (has_mut main 17) (has_mut main 40) (sym_eq main 33 N))"] + [gen.expr(2) for _ in range(25)]
+    engine.set_tuning("sparse_max_rows", sparse_max)
+    try:
+        assert engine.get_tuning("sparse_max_rows") == sparse_max
+        for e in exprs:
+            assert np.array_equal(et.mutation_counts(e, "main"), ot.mutation_counts(e, "main")), e
+            for p in (0.0, 0.05, 1.0):
+                want = ot.mutations(e, "main", p)
+                assert et.mutations(e, "main", p) == want, (e, p)
+                assert et.mutations_query(e, "main", p) == want, (e, p)
+    finally:
+        engine.set_tuning("sparse_max_rows", -1)
+
+
+def test_tuning_keys(engine):
+    from silo_amd.engine import EngineError
+
+    with pytest.raises(EngineError):
+        engine.set_tuning("no_such_knob", 1)
+    with pytest.raises(EngineError):
+        engine.set_tuning("mutation_groups", 17)
+    assert engine.get_tuning("mutation_groups") == 0
+
+
+def test_rejects_malformed_diffs(engine):
+    """The engine takes the diff form aligned_sequence.cpp produces: mutations inside the covered
+    range, outside missing runs and different from the reference."""
+    from silo_amd import table
+    from silo_amd.engine import EngineError
+
+    ref = "ACGTACGTAC"
+    for rows in (["ACGTACGTAC"], ["NNGTACGTAC"]):
+        spec = table.TableSpec(chunk_sizes=np.asarray([1], np.uint32))
+        spec.seq_columns.append(table.seq_column_from_strings("main", ref, rows, 0))
+        engine.table(spec).close()
+    bad = []
+    for pos, sym in ((0, 3), (9, 3), (4, 1)):  # before the covered range, after it, equal to the reference
+        c = table.seq_column_from_strings("main", ref, ["NNGTACGTNN"], 0)
+        c.mut_ptr = np.asarray([0, 1], np.uint64)
+        c.mut_pos = np.asarray([pos], np.uint32)
+        c.mut_sym = np.asarray([sym], np.uint8)
+        bad.append(c)
+    c = table.seq_column_from_strings("main", ref, ["ACGTNNGTAC"], 0)  # inside the missing run [4, 6)
+    c.mut_ptr = np.asarray([0, 1], np.uint64)
+    c.mut_pos = np.asarray([5], np.uint32)
+    c.mut_sym = np.asarray([1], np.uint8)
+    bad.append(c)
+    for c in bad:
+        spec = table.TableSpec(chunk_sizes=np.asarray([1], np.uint32))
+        spec.seq_columns.append(c)
+        with pytest.raises(EngineError) as e:
+            engine.table(spec)
+        assert e.value.status == "RDB_EINVAL"
+
+
+@pytest.mark.parametrize("sparse_max", [-1, 1 << 40])
+def test_empty_covered_ranges(engine, sparse_max):
+    """Rows with no covered position, whether given as [0, 0) or [L, L), count as missing
+    everywhere on both Mutations methods."""
+    from oracle.oracle import OracleTable
+    from silo_amd import table
+
+    ref = "ACGTACGTAC"
+    rows = ["NNNNNNNNNN", "ACGTACGTAC", "ACGTTCGTAN", "NNGTACGTAC", "ACGAACGTAC"]
+    spec = table.TableSpec(chunk_sizes=np.asarray([len(rows)], np.uint32))
+    col = table.seq_column_from_strings("main", ref, rows, 0)
+    col.cov_start = np.asarray(col.cov_start, np.uint32).copy()
+    col.cov_end = np.asarray(col.cov_end, np.uint32).copy()
+    col.cov_start[0] = col.cov_end[0] = len(ref)
+    spec.seq_columns.append(col)
+    ot = OracleTable(spec)
+    et = engine.table(spec)
+    engine.set_tuning("sparse_max_rows", sparse_max)
+    try:
+        for e in ("(true)", "(sym_eq main 3 A)", "(not (covered main 5))"):
+            assert np.array_equal(et.mutation_counts(e, "main"), ot.mutation_counts(e, "main")), e
+            assert et.mutations(e, "main", 0.0) == ot.mutations(e, "main", 0.0), e
+    finally:
+        engine.set_tuning("sparse_max_rows", -1)
+        et.close()
+        ot.close()
