@@ -8,6 +8,13 @@ A step is one pass of the configuration's query workload over a table resident i
 * config 2: 1,024 heterogeneous filter trees (depth 1-6 over And/Or/Not/NOf with sequence and
   metadata leaves, exact selectivities log-uniform over 1e-6..0.5), alternately Mutations and
   Aggregated by country, each also returning its cardinality;
+* config 3: the config-2 workload with the learned planner: the estimator is trained on the
+  engine's exact node cardinalities of seeded trees (--planner-trees) and judged on held-out ones
+  (--planner-eval) by q-error against the independence estimate; the batched path then runs with
+  the learned plan, and its throughput and results are compared with the reference plan's;
+* config 4: serving: 16 M sequences plus ten amino-acid columns resident, requests of the mix
+  (60 % Mutations, 25 % Aggregated by lineage x country, 15 % n-of-m over 20 positions) arriving
+  as a Poisson process, served by continuous batching; p50/p99 latency and throughput;
 * config 0: the 100 k-row CPU-runnable case.
 
 Two paths run the same workload:
@@ -27,6 +34,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "lapis-silo_amd"))
@@ -70,6 +79,64 @@ def run_plain_query(et, q):
     return et.count(q["filter"])
 
 
+def serve(args):
+    """Config 4: 16 M sequences + 10 amino-acid columns resident; Poisson arrivals of the serving
+    mix; continuous batching; p50/p99 latency and throughput, results checked against the same
+    queries run alone."""
+    from silo_amd import datagen, serving, table
+    from silo_amd.engine import Engine
+
+    t0 = time.time()
+    ds = datagen.generate_config(4, n_rows=args.rows, keep_handle=True)
+    layout = datagen.gene_layout(ds.genome_length)
+    aa = datagen.translate(ds, layout)
+    spec = table.spec_from_dataset(ds, aa_columns=aa)
+    log(f"generated {ds.n_rows} rows x {ds.genome_length} nt + {len(aa)} amino-acid columns "
+        f"({sum(n for _, _, n in layout)} residues) in {time.time() - t0:.1f}s")
+    eng = Engine(0)
+    t0 = time.time()
+    et = eng.table(spec)
+    upload_s = time.time() - t0
+    log(f"index built and uploaded in {upload_s:.1f}s: {et.stats()}")
+    genes = [g for g, _, _ in layout]
+    reqs = serving.config4_requests(ds, genes, n=args.requests, seed=41)
+    arrivals = serving.poisson_arrivals(len(reqs), rate=args.rate, seed=42)
+    eng.set_tuning("sparse_max_rows", 2000)
+    srv = serving.Server(eng, et, max_batch=256, row_capacity=65536)
+    log(f"serving with {srv.lanes} lanes of {srv.lane_bytes / 2**20:.0f} MiB ({srv.free_bytes / 2**30:.1f} GiB free)")
+    warm = serving.config4_requests(ds, genes, n=256, seed=40)
+    srv.run(warm, serving.poisson_arrivals(len(warm), rate=args.rate, seed=39))
+    results, latency, wall, sizes = srv.run(reqs, arrivals)
+    qps = len(reqs) / wall
+    log(f"served {len(reqs)} requests in {wall:.2f}s: {qps:.1f}/s, p50 {np.percentile(latency, 50) * 1e3:.2f} ms, "
+        f"p99 {np.percentile(latency, 99) * 1e3:.2f} ms, {len(sizes)} batches (mean {np.mean(sizes):.1f})")
+    # the same queries alone (single-query path): a bounded sample, bit for bit
+    sample = list(range(0, len(reqs), max(1, len(reqs) // 200)))
+    t0 = time.perf_counter()
+    identical = all(run_plain_query(et, reqs[i]) == results[i] for i in sample)
+    plain_qps = len(sample) / (time.perf_counter() - t0)
+    log(f"alone == in the mix for {len(sample)} sampled requests: {identical}; single-query path {plain_qps:.1f}/s")
+    mix = {k: sum(q["action"] == k for q in reqs) for k in ("mutations", "aggregate", "count")}
+    line = {
+        "metric": METRIC, "value": round(qps, 3), "unit": "queries/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic",
+        "config": {"workload": (f"config 4: {ds.n_rows} sequences x {ds.genome_length} nt + {len(aa)} amino-acid columns, "
+                                f"{len(reqs)} requests, Poisson arrivals at {args.rate:.0f}/s"),
+                   "rows": ds.n_rows, "mix": mix, "path": "serving loop (continuous batching over lanes)"},
+        "p50_ms": round(float(np.percentile(latency, 50)) * 1e3, 4),
+        "p99_ms": round(float(np.percentile(latency, 99)) * 1e3, 4),
+        "offered_rate": args.rate, "lanes": srv.lanes, "lane_MiB": round(srv.lane_bytes / 2**20, 1),
+        "batches": len(sizes), "mean_batch": round(float(np.mean(sizes)), 2),
+        "alone_equals_mix": {"sampled": len(sample), "identical": bool(identical)},
+        "single_query_path_qps": round(plain_qps, 2), "upload_s": round(upload_s, 2),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    et.close()
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +147,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-query path (the batched one still runs)")
+    ap.add_argument("--planner-trees", type=int, default=20000, help="config 3: training trees (200000 in full)")
+    ap.add_argument("--planner-eval", type=int, default=2000, help="config 3: held-out trees (10000 in full)")
+    ap.add_argument("--planner-epochs", type=int, default=30)
+    ap.add_argument("--requests", type=int, default=4000, help="config 4: requests served")
+    ap.add_argument("--rate", type=float, default=3000.0, help="config 4: Poisson arrival rate (requests/s)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (scales PMC byte counters)")
     args = ap.parse_args()
@@ -89,6 +161,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.config == 4:
+        serve(args)
         return
 
     from silo_amd import autotune, datagen, table, workloads
@@ -108,7 +183,7 @@ def main():
 
     if args.config == 0:
         queries = workloads.config0_queries(ds)
-    elif args.config == 2:
+    elif args.config in (2, 3):
         t0 = time.time()
 
         def count(filters):
@@ -176,6 +251,26 @@ def main():
         if q["action"] == "mutations":
             with et.bitmap(q["filter"]) as b:
                 traffic_tuned += et.mutations_traffic(b, "main")
+    planner = None
+    if args.config == 3:
+        from silo_amd import planner_train
+
+        t0 = time.time()
+        train_trees = planner_train.trees(ds, args.planner_trees, seed=3)
+        held_out = planner_train.trees(ds, args.planner_eval, seed=4)
+        x, y = planner_train.samples(et, train_trees, log=log)
+        t_samples = time.time() - t0
+        mlp = eng.mlp(n_in=32, n_hidden=128, n_hidden_layers=3, seed=7)
+        t0 = time.time()
+        loss = mlp.train(x, y, epochs=args.planner_epochs, batch=4096, lr=1e-3, seed=8)
+        t_train = time.time() - t0
+        ev = planner_train.evaluate(et, mlp, held_out)
+        sp = planner_train.speedup(eng, et, queries, mlp)
+        planner = {"training_trees": len(train_trees), "training_nodes": int(len(y)), "held_out": ev,
+                   "train_mse": loss, "epochs": args.planner_epochs, "samples_s": round(t_samples, 1),
+                   "train_s": round(t_train, 1), "workload": sp}
+        log(f"planner: {planner}")
+        eng.set_planner("learned", mlp)
     eng.set_tuning("batch_max_queries", min(max(len(queries), 1), 256))
     for _ in range(max(args.warmup, 1)):
         et.batch(queries)
@@ -260,7 +355,8 @@ def main():
             "rows": ds.n_rows,
             "genome_length": ds.genome_length,
             "queries_per_step": len(queries),
-            "path": "batched (one rdb_batch per step, autotuned knobs, HIP-graph replay)",
+            "path": ("batched (one rdb_batch per step, autotuned knobs, HIP-graph replay"
+                     + (", learned planner)" if args.config == 3 else ")")),
         },
         "p50_ms": round(percentile(blat, 50) * 1e3, 4),
         "p99_ms": round(percentile(blat, 99) * 1e3, 4),
@@ -269,6 +365,7 @@ def main():
         "batched_vs_plain": round(b_qps / plain["queries_per_s"], 3) if plain else None,
         "batched_identical_to_plain": identical_paths,
         "tuning": tuned,
+        "planner": planner,
         "bitmap_GBps": round(traffic_tuned * args.steps / b_elapsed / 1e9, 2),
         "roofline": roofline,
         "cpu_baseline": cpu,

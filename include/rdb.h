@@ -161,6 +161,13 @@ typedef struct rdb_batch rdb_batch;
 int rdb_batch_submit(rdb_table* table, const rdb_query* queries, uint32_t n, rdb_batch** out);
 /* Waits for every job of the batch (the getters wait too). */
 int rdb_batch_wait(rdb_batch* batch);
+/* Collects the batch's finished jobs without waiting; *done = 1 once all have finished. */
+int rdb_batch_poll(rdb_batch* batch, int* done);
+/* Device memory one lane holds for jobs of up to max_queries queries keeping up to row_capacity
+ * records each: the basis of admission by free device memory. */
+int rdb_batch_lane_bytes(rdb_table* table, uint32_t max_queries, uint32_t row_capacity, uint64_t* bytes);
+/* Free and total device memory of the context's GPU (hipMemGetInfo). */
+int rdb_device_memory(rdb_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 /* Cardinality of query q's filter (any action): CountFilterNode (count_filter_node.cpp:35-71). */
 int rdb_batch_count(rdb_batch* batch, uint32_t q, uint64_t* count);
 /* Rows of Mutations query q, as rdb_mutations. */
@@ -170,6 +177,48 @@ int rdb_batch_mutations(rdb_batch* batch, uint32_t q, uint32_t* position, uint8_
 int rdb_batch_aggregate(rdb_batch* batch, uint32_t q, uint32_t* ids, uint64_t* counts, uint64_t capacity,
                         uint64_t* n);
 void rdb_batch_free(rdb_batch* batch);
+
+/* ---- learned planner -------------------------------------------------------------------------- */
+/* No reference counterpart: the reference plans with fixed rules (Intersection sorts evaluated
+ * children by cardinality, intersection.cpp:74-86; Selection orders predicates by a fixed
+ * selectivity guess, selection.cpp:37, and switches to per-row matching for small children,
+ * selection.cpp:105).  Here a multilayer perceptron (ReLU hidden layers, BF16 parameters, FP32
+ * accumulation; forward, backward and Adam as HIP kernels) estimates every operator node's
+ * cardinality from per-node features, and the batched path uses the estimates to order
+ * conjuncts / disjuncts and to skip the rest of an And (Or) in every 4096-row slice where it is
+ * already empty (full).  No plan changes a result. */
+typedef struct rdb_mlp rdb_mlp;
+
+/* Parameters: per layer W (out x in, row-major) then b; inputs -> n_hidden_layers x n_hidden -> 1.
+ * He-uniform initial weights from `seed`, zero biases. */
+int rdb_mlp_create(rdb_ctx* ctx, uint32_t n_in, uint32_t n_hidden, uint32_t n_hidden_layers, uint64_t seed,
+                   rdb_mlp** out);
+void rdb_mlp_free(rdb_mlp* mlp);
+int rdb_mlp_parameter_count(rdb_mlp* mlp, uint64_t* n);
+/* The BF16 parameter values the kernels use, as float. */
+int rdb_mlp_get_parameters(rdb_mlp* mlp, float* out, uint64_t n);
+/* Sets the FP32 master parameters (rounded to BF16 for the products) and resets Adam. */
+int rdb_mlp_set_parameters(rdb_mlp* mlp, const float* p, uint64_t n);
+/* y[i] = network(x[i * n_in ..]) for n rows. */
+int rdb_mlp_forward(rdb_mlp* mlp, const float* x, uint32_t n, float* y);
+/* Mean squared error over n rows and its gradient with respect to every parameter. */
+int rdb_mlp_gradients(rdb_mlp* mlp, const float* x, const float* y, uint32_t n, float* loss, float* grads);
+/* Adam (lr, 0.9, 0.999, 1e-8) over `epochs` shuffled passes of minibatches; the last pass's MSE. */
+int rdb_mlp_train(rdb_mlp* mlp, const float* x, const float* y, uint64_t n, uint32_t epochs, uint32_t batch, float lr,
+                  uint64_t seed, float* final_loss);
+/* Training rows of the planner: one per operator node of every filter's compiled tree (post-order),
+ * 32 features with the children's exact values in the child-summary features, and the target:
+ * log(1 + rows) / log(1 + table rows) of the node's exact cardinality (evaluated on the device).
+ * tree[i] = the filter the row belongs to.  Any output may be null. */
+int rdb_plan_samples(rdb_table* table, const char* const* filters, uint32_t n_filters, float* features, float* targets,
+                     uint32_t* tree, uint64_t capacity, uint64_t* n);
+/* Per node of every filter: the learned estimate (rows; 0 when mlp is null), the independence
+ * estimate (rows) and the exact cardinality (exact may be null to skip evaluating). */
+int rdb_plan_estimates(rdb_table* table, rdb_mlp* mlp, const char* const* filters, uint32_t n_filters, float* estimate,
+                       float* independence, uint64_t* exact, uint32_t* tree, uint64_t capacity, uint64_t* n);
+/* Planner of the batched path: 0 = the reference's order, nothing skipped (default); 1 = order by
+ * independence estimates, short-circuit on; 2 = order by the estimator `mlp`, short-circuit on. */
+int rdb_set_planner(rdb_ctx* ctx, int mode, rdb_mlp* mlp);
 
 /* ---- tuning ----------------------------------------------------------------------------------- */
 /* No reference counterpart: knobs of the engine's own kernels, set by hand or by the autotuner.

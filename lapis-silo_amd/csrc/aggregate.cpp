@@ -56,14 +56,23 @@ uint64_t dimStrides(const std::vector<AggDim>& ds, uint32_t* stride) {
 
 void decodeHistogram(const std::vector<AggDim>& ds, const uint32_t* stride, const uint32_t* hist, uint64_t bins,
                      std::vector<uint32_t>& ids, std::vector<uint64_t>& counts) {
+   std::vector<std::pair<uint32_t, uint32_t>> nonzero;
+   for (uint64_t bin = 0; bin < bins; ++bin) {
+      if (hist[bin]) nonzero.emplace_back(static_cast<uint32_t>(bin), hist[bin]);
+   }
+   decodeBins(ds, stride, nonzero, ids, counts);
+}
+
+void decodeBins(const std::vector<AggDim>& ds, const uint32_t* stride, const std::vector<std::pair<uint32_t, uint32_t>>& bins,
+                std::vector<uint32_t>& ids, std::vector<uint64_t>& counts) {
    struct Combo {
       uint32_t key[4];
       uint32_t order[4];
       uint64_t count;
    };
    std::vector<Combo> combos;
-   for (uint64_t bin = 0; bin < bins; ++bin) {
-      if (!hist[bin]) continue;
+   for (const auto& [bin, count] : bins) {
+      if (!count) continue;
       Combo c{};
       uint64_t rem = bin;
       for (size_t k = 0; k < ds.size(); ++k) {
@@ -73,7 +82,7 @@ void decodeHistogram(const std::vector<AggDim>& ds, const uint32_t* stride, cons
          c.key[k] = is_null ? kInvalidId : lab;
          c.order[k] = is_null ? kInvalidId : (ds[k].is_seq ? lab : ds[k].dict->rank[lab]);
       }
-      c.count = hist[bin];
+      c.count = count;
       combos.push_back(c);
    }
    std::sort(combos.begin(), combos.end(), [&](const Combo& a, const Combo& b) {

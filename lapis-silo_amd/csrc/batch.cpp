@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "capi_internal.h"
+#include "planner.h"
 #include "query.h"
 
 namespace rdb {
@@ -49,17 +50,15 @@ struct ParamsLayout {
    }
 };
 
-// Layout of a lane's results: per-query header {n_rows, offset, card lo, card hi}, histogram
-// words, packed Mutations rows.
+// Layout of a lane's results: per-query header {n_rows, offset, card lo, card hi}, then the packed
+// 16-byte records: Mutations rows, or {bin, count} of the non-empty groups of Aggregated queries.
 struct OutLayout {
    size_t header = 0;
-   size_t hist = 0;
    size_t rows = 0;
    size_t bytes = 0;
-   OutLayout(uint32_t q, uint32_t hist_cap, uint32_t row_cap) {
+   OutLayout(uint32_t q, uint32_t /*hist_cap*/, uint32_t row_cap) {
       header = 0;
-      hist = alignUp(16ULL * q);
-      rows = alignUp(hist + 4ULL * hist_cap);
+      rows = alignUp(16ULL * q);
       bytes = rows + sizeof(MutationRowDev) * static_cast<size_t>(q) * row_cap;
    }
 };
@@ -71,20 +70,41 @@ struct CompiledQuery {
 
 // Stack depth a lane's filter launch is sized for: rounded up to a multiple of 8 so that few graph
 // shapes cover all programs.
+// Histogram words of a lane: room for every query of a job to group by up to 2^17 combinations.
+uint32_t histCapacity(uint32_t max_queries) {
+   return static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(uint64_t{max_queries} << 17, 1u << 20), 1u << 30));
+}
+
 uint32_t depthBucket(uint32_t depth, uint32_t max_depth) { return std::min((depth + 7) / 8 * 8, max_depth); }
 
-BatchLane& acquireLane(Table& t, int column);
+BatchLane& acquireLane(Table& t);
 void retire(BatchLane& lane);
 
-void setupLane(Table& t, BatchLane& lane, int column) {
+// Mutations scratch of one lane slot, sized for the largest sequence column so that a lane serves
+// any column.
+struct MutScratch {
+   size_t slots = 0, counts = 0, bnd = 0, pblocks = 0;
+};
+MutScratch mutScratch(const Table& t) {
+   MutScratch m;
+   for (const auto& c : t.seq) {
+      const size_t L = c->genome_length;
+      m.slots = std::max<size_t>(m.slots, c->view.n_slots);
+      m.counts = std::max<size_t>(m.counts, L * c->abc->count);
+      m.bnd = std::max<size_t>(m.bnd, 3 * L);
+      m.pblocks = std::max<size_t>(m.pblocks, (L + kMutPosBlock - 1) / kMutPosBlock);
+   }
+   return m;
+}
+
+void setupLane(Table& t, BatchLane& lane) {
    Context& ctx = *t.ctx;
    lane.table = &t;
-   lane.column = column;
    lane.max_queries = ctx.batch_max_queries;
    lane.row_capacity = ctx.batch_row_capacity;
    lane.instr_capacity = lane.max_queries * 256;
    lane.range_capacity = lane.max_queries * 256;
-   lane.hist_capacity = std::max<uint32_t>(lane.max_queries * 64, 1u << 20);
+   lane.hist_capacity = histCapacity(lane.max_queries);
    hipCheck(hipStreamCreateWithFlags(&lane.stream, hipStreamNonBlocking), "hipStreamCreate");
    hipCheck(hipEventCreateWithFlags(&lane.done, hipEventDisableTiming), "hipEventCreate");
    const uint32_t Q = lane.max_queries;
@@ -102,21 +122,22 @@ void setupLane(Table& t, BatchLane& lane, int column) {
    lane.lists = DeviceBuffer(8ULL * Q);
    lane.list_counts = DeviceBuffer(16);
    lane.n_rows = DeviceBuffer(4ULL * Q);
+   lane.hist = DeviceBuffer(4ULL * lane.hist_capacity);
    lane.rows = DeviceBuffer(sizeof(MutationRowDev) * static_cast<size_t>(Q) * lane.row_capacity);
-   if (column >= 0) {
-      const SeqColumn& col = *t.seq[column];
-      const size_t L = col.genome_length;
-      lane.partial = DeviceBuffer(4ULL * Q * col.view.n_slots);
-      lane.sparse_counts = DeviceBuffer(4ULL * Q * col.abc->count * L);
-      lane.sparse_bnd = DeviceBuffer(12ULL * Q * L);
-      lane.totals = DeviceBuffer(2ULL * Q * ((L + kMutPosBlock - 1) / kMutPosBlock) * 3 * sizeof(int64_t));
+   if (!t.seq.empty()) {
+      const MutScratch ms = mutScratch(t);
+      lane.partial = DeviceBuffer(4ULL * Q * ms.slots);
+      lane.sparse_counts = DeviceBuffer(4ULL * Q * ms.counts);
+      lane.sparse_bnd = DeviceBuffer(4ULL * Q * ms.bnd);
+      lane.totals = DeviceBuffer(2ULL * Q * ms.pblocks * 3 * sizeof(int64_t));
       hipCheck(hipMemset(lane.sparse_counts.as<void>(), 0, lane.sparse_counts.bytes()), "clear sparse counts");
       hipCheck(hipMemset(lane.sparse_bnd.as<void>(), 0, lane.sparse_bnd.bytes()), "clear sparse boundaries");
    }
 }
 
-// The lane's fixed kernel sequence for programs of at most `depth` stack entries.
-void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth) {
+// The lane's fixed kernel sequence for programs of at most `depth` stack entries whose Mutations
+// queries read sequence column `column` (-1: none).
+void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column) {
    Context& ctx = *t.ctx;
    const uint32_t Q = lane.max_queries;
    const ParamsLayout pl(Q, lane.instr_capacity, lane.range_capacity);
@@ -131,10 +152,10 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth) {
                          reinterpret_cast<const uint32_t*>(pd + pl.ranges), Q, depth, lane.filters.as<uint64_t>(), cards,
                          chunk_cards, static_cast<uint32_t>(ctx.cu_count) * 8, lane.stream, n_queries),
             "filter kernel");
-   if (lane.column >= 0) {
-      const SeqColumn& col = *t.seq[lane.column];
+   if (column >= 0) {
+      const SeqColumn& col = *t.seq[column];
       MutationsBatch m;
-      m.col_dev = t.seq_views_dev.as<SeqColumnView>() + lane.column;
+      m.col_dev = t.seq_views_dev.as<SeqColumnView>() + column;
       m.col_host = &col.view;
       m.chunk_sizes = t.chunk_sizes_dev.as<uint32_t>();
       m.n_chunks = t.n_chunks;
@@ -172,7 +193,10 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth) {
    a.dims = reinterpret_cast<const GroupDims*>(pd + pl.dims);
    a.filters = lane.filters.as<uint64_t>();
    a.chunk_cards = chunk_cards;
-   a.hist = reinterpret_cast<uint32_t*>(od + ol.hist);
+   a.hist = lane.hist.as<uint32_t>();
+   a.rows = lane.rows.as<MutationRowDev>();
+   a.row_capacity = lane.row_capacity;
+   a.n_rows = lane.n_rows.as<uint32_t>();
    hipCheck(launchAggregateBatch(a, lane.stream), "batched group histogram");
    hipCheck(launchPackRows(n_queries, Q, lane.n_rows.as<uint32_t>(), cards, lane.rows.as<MutationRowDev>(), lane.row_capacity,
                            reinterpret_cast<uint32_t*>(od + ol.header), reinterpret_cast<MutationRowDev*>(od + ol.rows),
@@ -180,21 +204,21 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth) {
             "pack rows");
 }
 
-void launchLane(Table& t, BatchLane& lane, uint32_t depth) {
+void launchLane(Table& t, BatchLane& lane, uint32_t depth, int column) {
    Context& ctx = *t.ctx;
    const uint32_t bucket = depthBucket(depth, maxFilterDepth(ctx));
    if (!ctx.batch_graphs) {
-      enqueueKernels(t, lane, bucket);
+      enqueueKernels(t, lane, bucket, column);
       return;
    }
-   const uint64_t key = static_cast<uint64_t>(bucket) | (static_cast<uint64_t>(ctx.mutation_unroll) << 16) |
-                        (static_cast<uint64_t>(ctx.sparse_max_rows + 1) << 24);
+   const uint64_t key = static_cast<uint64_t>(bucket) | (static_cast<uint64_t>(ctx.mutation_unroll) << 10) |
+                        (static_cast<uint64_t>(column + 1) << 16) | (static_cast<uint64_t>(ctx.sparse_max_rows + 1) << 24);
    auto it = lane.graphs.find(key);
    if (it == lane.graphs.end()) {
       hipGraph_t graph = nullptr;
       hipCheck(hipStreamBeginCapture(lane.stream, hipStreamCaptureModeThreadLocal), "begin capture");
       try {
-         enqueueKernels(t, lane, bucket);
+         enqueueKernels(t, lane, bucket, column);
       } catch (...) {
          (void)hipStreamEndCapture(lane.stream, &graph);
          if (graph) (void)hipGraphDestroy(graph);
@@ -217,7 +241,7 @@ struct JobPlan {
 };
 
 void enqueueJob(Table& t, BatchResult& b, const std::vector<CompiledQuery>& cq, const JobPlan& job) {
-   BatchLane& lane = acquireLane(t, job.column);
+   BatchLane& lane = acquireLane(t);
    const uint32_t Q = lane.max_queries;
    const ParamsLayout pl(Q, lane.instr_capacity, lane.range_capacity);
    const OutLayout ol(Q, lane.hist_capacity, lane.row_capacity);
@@ -237,11 +261,13 @@ void enqueueJob(Table& t, BatchResult& b, const std::vector<CompiledQuery>& cq, 
       const uint32_t qi = job.queries[slot];
       const BatchQueryResult& r = b.queries[qi];
       const query::Emitter& e = cq[qi].program;
-      progs[slot] = ProgramDesc{n_instr, static_cast<uint32_t>(e.code.size()), n_range, slot};
+      progs[slot] = ProgramDesc{n_instr, static_cast<uint32_t>(e.code.size()), n_range, slot,
+                                n_range + static_cast<uint32_t>(e.ranges.size()), static_cast<uint32_t>(e.prefetch.size())};
       std::memcpy(instrs + n_instr, e.code.data(), e.code.size() * sizeof(Instr));
       std::memcpy(ranges + n_range, e.ranges.data(), e.ranges.size() * 4);
+      std::memcpy(ranges + n_range + e.ranges.size(), e.prefetch.data(), e.prefetch.size() * 4);
       n_instr += static_cast<uint32_t>(e.code.size());
-      n_range += static_cast<uint32_t>(e.ranges.size());
+      n_range += static_cast<uint32_t>(e.ranges.size() + e.prefetch.size());
       depth = std::max(depth, cq[qi].depth);
       is_mut[slot] = r.status == RDB_OK && r.action == RDB_ACTION_MUTATIONS ? 1u : 0u;
       min_prop[slot] = r.min_proportion;
@@ -266,44 +292,45 @@ void enqueueJob(Table& t, BatchResult& b, const std::vector<CompiledQuery>& cq, 
    header[3] = 0;
    lane.busy = true;
    lane.owner = &b;
+   lane.job_seq = ++t.job_counter;
    lane.job_queries = job.queries;
    lane.job_total_bins = total_bins;
-   // header and histograms always; the packed rows up to 1 MiB (the rest, if any, on retirement)
+   // the header and up to 1 MiB of packed records (the rest, if any, on retirement)
    lane.job_first_bytes = std::min(lane.out_bytes, ol.rows + (size_t{1} << 20));
    b.pending.push_back(&lane);
    hipCheck(hipMemcpyAsync(lane.params.as<void>(), hp, pl.bytes, hipMemcpyHostToDevice, lane.stream), "stage batch");
-   launchLane(t, lane, depth);
+   launchLane(t, lane, depth, job.column);
    hipCheck(hipMemcpyAsync(lane.host_out, lane.out.as<void>(), lane.job_first_bytes, hipMemcpyDeviceToHost, lane.stream),
             "copy batch results");
    hipCheck(hipEventRecord(lane.done, lane.stream), "hipEventRecord");
 }
 
-BatchLane& acquireLane(Table& t, int column) {
+BatchLane& acquireLane(Table& t) {
    Context& ctx = *t.ctx;
+   auto shaped = [&](const BatchLane& l) {
+      return l.max_queries == ctx.batch_max_queries && l.row_capacity == ctx.batch_row_capacity;
+   };
    for (auto& l : t.lanes) {
-      if (!l->busy && l->column == column && l->max_queries == ctx.batch_max_queries &&
-          l->row_capacity == ctx.batch_row_capacity) {
-         return *l;
-      }
+      if (!l->busy && shaped(*l)) return *l;
    }
    // drop idle lanes of another shape, then create one if the budget allows
    t.lanes.erase(std::remove_if(t.lanes.begin(), t.lanes.end(),
-                                [&](const std::shared_ptr<BatchLane>& l) {
-                                   return !l->busy && (l->column != column || l->max_queries != ctx.batch_max_queries ||
-                                                       l->row_capacity != ctx.batch_row_capacity);
-                                }),
+                                [&](const std::shared_ptr<BatchLane>& l) { return !l->busy && !shaped(*l); }),
                  t.lanes.end());
    if (t.lanes.size() < std::max<uint32_t>(ctx.batch_lanes, 1)) {
       auto lane = std::make_shared<BatchLane>();
-      setupLane(t, *lane, column);
+      setupLane(t, *lane);
       t.lanes.push_back(lane);
       return *lane;
    }
    // every lane is busy: retire the one whose job was enqueued first
    BatchLane* oldest = t.lanes.front().get();
+   for (auto& l : t.lanes) {
+      if (l->busy && l->job_seq < oldest->job_seq) oldest = l.get();
+   }
    hipCheck(hipEventSynchronize(oldest->done), "batch wait");
    retire(*oldest);
-   return acquireLane(t, column);
+   return acquireLane(t);
 }
 
 // Copies a finished job's results into its batch and frees the lane.
@@ -327,8 +354,6 @@ void retire(BatchLane& lane) {
       hipCheck(hipStreamSynchronize(lane.stream), "synchronize");
    }
    const auto* packed = reinterpret_cast<const MutationRowDev*>(ho + ol.rows);
-   const auto* hist = reinterpret_cast<const uint32_t*>(ho + ol.hist);
-   uint32_t agg_offset = 0;
    std::vector<uint32_t> overflow;
    for (uint32_t s = 0; s < n; ++s) {
       BatchQueryResult& r = b.queries[lane.job_queries[s]];
@@ -344,26 +369,40 @@ void retire(BatchLane& lane) {
          capi::sortMutationRows(r.rows);
       } else if (r.action == RDB_ACTION_AGGREGATE) {
          const auto ds = capi::parseDims(t, r.dims.c_str());
-         uint32_t stride[4] = {};
-         const uint64_t bins = capi::dimStrides(ds, stride);
          if (ds.empty()) {
             // a single empty combination with |F| (bitmap_aggregation_node.cpp:91-101)
             r.agg_counts.assign(1, r.cardinality);
-         } else {
-            capi::decodeHistogram(ds, stride, hist + agg_offset, bins, r.agg_ids, r.agg_counts);
+            continue;
          }
-         agg_offset += static_cast<uint32_t>(bins);
+         const uint32_t nb = header[4 * s];
+         if (nb > lane.row_capacity) {
+            overflow.push_back(lane.job_queries[s]);
+            continue;
+         }
+         uint32_t stride[4] = {};
+         capi::dimStrides(ds, stride);
+         std::vector<std::pair<uint32_t, uint32_t>> bins;
+         bins.reserve(nb);
+         for (uint32_t k = 0; k < nb; ++k) {
+            const MutationRowDev& rec = packed[header[4 * s + 1] + k];
+            bins.emplace_back(rec.position, rec.symbol);
+         }
+         capi::decodeBins(ds, stride, bins, r.agg_ids, r.agg_counts);
       }
    }
    lane.busy = false;
    lane.owner = nullptr;
    lane.job_queries.clear();
    b.pending.erase(std::remove(b.pending.begin(), b.pending.end(), &lane), b.pending.end());
-   // queries whose rows did not fit the lane's capacity run again on the single-query path
+   // queries whose records did not fit the lane's capacity run again on the single-query path
    for (uint32_t qi : overflow) {
       BatchQueryResult& r = b.queries[qi];
       DeviceBitmap f = query::compileFilter(t, r.filter)->evaluate();
-      r.rows = capi::mutationRows(t, f, r.column, r.min_proportion);
+      if (r.action == RDB_ACTION_MUTATIONS) {
+         r.rows = capi::mutationRows(t, f, r.column, r.min_proportion);
+      } else {
+         capi::aggregateRows(t, f, r.dims.c_str(), r.agg_ids, r.agg_counts);
+      }
    }
 }
 
@@ -378,10 +417,13 @@ BatchLane::~BatchLane() {
    if (stream) (void)hipStreamDestroy(stream);
 }
 
+void submitCompiled(Table& t, BatchResult& b, std::vector<CompiledQuery>& cq);
+
 void batchSubmit(Table& t, BatchResult& b) {
    Context& ctx = *t.ctx;
    const uint32_t n = static_cast<uint32_t>(b.queries.size());
    std::vector<CompiledQuery> cq(n);
+   std::vector<query::OperatorPtr> ops(n);
    // host side: parse, rewrite and compile every filter; a query that fails keeps its status and
    // runs as the empty program
    for (uint32_t i = 0; i < n; ++i) {
@@ -396,14 +438,32 @@ void batchSubmit(Table& t, BatchResult& b) {
          } else if (r.action != RDB_ACTION_COUNT) {
             fail(RDB_EINVAL, "unknown action " + std::to_string(r.action));
          }
-         auto op = query::compileFilter(t, r.filter);
-         op->emit(cq[i].program);
-         if (cq[i].program.sp != 1) fail(RDB_ECOMPILE, "internal: filter program leaves a wrong stack");
-         cq[i].depth = std::max<uint32_t>(cq[i].program.max_sp, 1);
-         if (cq[i].depth > maxFilterDepth(ctx)) fail(RDB_EQUERY, "filter expression nests too deeply");
+         ops[i] = query::compileFilter(t, r.filter);
       } catch (const Error& e) {
          r.status = e.code;
          r.message = e.what();
+      }
+   }
+   // the planner orders conjuncts and disjuncts and turns on short-circuit (results unchanged)
+   if (ctx.plan_mode != plan::REFERENCE) {
+      std::vector<query::Operator*> roots;
+      for (auto& op : ops) {
+         if (op) roots.push_back(op.get());
+      }
+      plan::planAll(t, roots, static_cast<plan::Mode>(ctx.plan_mode), ctx.planner);
+   }
+   for (uint32_t i = 0; i < n; ++i) {
+      BatchQueryResult& r = b.queries[i];
+      if (r.status == RDB_OK) {
+         try {
+            ops[i]->emit(cq[i].program);
+            if (cq[i].program.sp != 1) fail(RDB_ECOMPILE, "internal: filter program leaves a wrong stack");
+            cq[i].depth = std::max<uint32_t>(cq[i].program.max_sp, 1);
+            if (cq[i].depth > maxFilterDepth(ctx)) fail(RDB_EQUERY, "filter expression nests too deeply");
+         } catch (const Error& e) {
+            r.status = e.code;
+            r.message = e.what();
+         }
       }
       if (r.status != RDB_OK) {
          cq[i].program = query::Emitter();
@@ -411,10 +471,16 @@ void batchSubmit(Table& t, BatchResult& b) {
          cq[i].depth = 1;
       }
    }
+   submitCompiled(t, b, cq);
+}
+
+void submitCompiled(Table& t, BatchResult& b, std::vector<CompiledQuery>& cq) {
+   Context& ctx = *t.ctx;
+   const uint32_t n = static_cast<uint32_t>(b.queries.size());
    // cut into lane jobs: one Mutations column per job, capacities of the lane
    const uint32_t max_q = ctx.batch_max_queries;
    const uint32_t instr_cap = max_q * 256, range_cap = max_q * 256;
-   const uint32_t hist_cap = std::max<uint32_t>(max_q * 64, 1u << 20);
+   const uint32_t hist_cap = histCapacity(max_q);
    std::vector<JobPlan> jobs;
    JobPlan cur;
    auto flush = [&] {
@@ -424,7 +490,7 @@ void batchSubmit(Table& t, BatchResult& b) {
    for (uint32_t i = 0; i < n; ++i) {
       const BatchQueryResult& r = b.queries[i];
       const uint32_t ni = static_cast<uint32_t>(cq[i].program.code.size());
-      const uint32_t nr = static_cast<uint32_t>(cq[i].program.ranges.size());
+      const uint32_t nr = static_cast<uint32_t>(cq[i].program.ranges.size() + cq[i].program.prefetch.size());
       uint32_t bins = 0;
       if (r.status == RDB_OK && r.action == RDB_ACTION_AGGREGATE) {
          uint32_t stride[4];
@@ -448,6 +514,56 @@ void batchSubmit(Table& t, BatchResult& b) {
    }
    flush();
    for (const auto& job : jobs) enqueueJob(t, b, cq, job);
+}
+
+std::vector<uint64_t> countPrograms(Table& t, const std::vector<query::Emitter>& programs) {
+   BatchResult b;
+   b.table = &t;
+   b.queries.resize(programs.size());
+   std::vector<CompiledQuery> cq(programs.size());
+   for (size_t i = 0; i < programs.size(); ++i) {
+      b.queries[i].action = RDB_ACTION_COUNT;
+      cq[i].program = programs[i];
+      cq[i].depth = std::max<uint32_t>(programs[i].max_sp, 1);
+      if (cq[i].depth > maxFilterDepth(*t.ctx)) fail(RDB_EQUERY, "filter expression nests too deeply");
+   }
+   submitCompiled(t, b, cq);
+   batchWait(b);
+   std::vector<uint64_t> out(programs.size());
+   for (size_t i = 0; i < programs.size(); ++i) {
+      if (b.queries[i].status != RDB_OK) fail(b.queries[i].status, b.queries[i].message);
+      out[i] = b.queries[i].cardinality;
+   }
+   return out;
+}
+
+bool batchPoll(BatchResult& b) {
+   std::vector<BatchLane*> lanes = b.pending;
+   for (BatchLane* l : lanes) {
+      const hipError_t err = hipEventQuery(l->done);
+      if (err == hipErrorNotReady) return false;
+      hipCheck(err, "batch poll");
+      if (l->busy && l->owner == &b) retire(*l);
+   }
+   return b.pending.empty();
+}
+
+size_t laneBytes(const Table& t, uint32_t max_queries, uint32_t row_capacity) {
+   const uint32_t Q = max_queries;
+   const uint32_t instr_cap = Q * 256, range_cap = Q * 256;
+   const uint32_t hist_cap = histCapacity(Q);
+   const ParamsLayout pl(Q, instr_cap, range_cap);
+   const OutLayout ol(Q, hist_cap, row_capacity);
+   size_t b = pl.bytes + ol.bytes;                                            // staged inputs, results
+   b += static_cast<size_t>(t.n_chunks) * kChunkWords * 8 * Q;                // filter words
+   b += 8ULL * Q + 4ULL * Q * t.n_chunks + 8ULL * Q + 16 + 4ULL * Q;         // cardinalities, lists
+   b += sizeof(MutationRowDev) * static_cast<size_t>(Q) * row_capacity;      // rows before packing
+   b += 4ULL * hist_cap;
+   if (!t.seq.empty()) {
+      const MutScratch ms = mutScratch(t);
+      b += 4ULL * Q * (ms.slots + ms.counts + ms.bnd) + 2ULL * Q * ms.pblocks * 3 * sizeof(int64_t);
+   }
+   return b;
 }
 
 void batchWait(BatchResult& b) {
@@ -524,6 +640,26 @@ int rdb_batch_wait(rdb_batch* batch) {
       if (!batch) rdb::fail(RDB_EINVAL, "null batch");
       std::lock_guard<std::recursive_mutex> lock(ctx->mu);
       rdb::batchWait(batch->b);
+   });
+}
+
+int rdb_batch_poll(rdb_batch* batch, int* done) {
+   rdb::Context* ctx = batch && batch->b.table ? batch->b.table->ctx : nullptr;
+   return guarded(ctx, [&] {
+      if (!batch || !done) rdb::fail(RDB_EINVAL, "null argument");
+      std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+      *done = rdb::batchPoll(batch->b) ? 1 : 0;
+   });
+}
+
+int rdb_batch_lane_bytes(rdb_table* table, uint32_t max_queries, uint32_t row_capacity, uint64_t* bytes) {
+   rdb::Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      rdb::Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireFinalized(t);
+      if (!bytes || max_queries == 0) rdb::fail(RDB_EINVAL, "bad argument");
+      *bytes = rdb::laneBytes(t, max_queries, row_capacity);
    });
 }
 

@@ -22,7 +22,6 @@ struct BatchResult;
 
 struct BatchLane {
    Table* table = nullptr;
-   int column = -1;  // sequence column of the lane's Mutations scratch (-1: none)
    hipStream_t stream = nullptr;
    hipEvent_t done = nullptr;
    uint32_t max_queries = 0;
@@ -32,13 +31,15 @@ struct BatchLane {
    uint32_t hist_capacity = 0;
    size_t params_bytes = 0;
    size_t out_bytes = 0;
-   DeviceBuffer params, filters, cards, lists, list_counts, partial, sparse_counts, sparse_bnd, totals, rows, n_rows, out;
+   DeviceBuffer params, filters, cards, lists, list_counts, partial, sparse_counts, sparse_bnd, totals, rows, n_rows, hist,
+      out;
    void* host_params = nullptr;
    void* host_out = nullptr;
-   std::map<uint64_t, hipGraphExec_t> graphs;  // by (depth bucket, sparse switch, unroll)
+   std::map<uint64_t, hipGraphExec_t> graphs;  // by (depth bucket, Mutations column, sparse switch, unroll)
 
    // the job on the lane (busy until retired)
    bool busy = false;
+   uint64_t job_seq = 0;  // submission order of the job
    BatchResult* owner = nullptr;
    std::vector<uint32_t> job_queries;  // batch query index of each lane slot
    uint32_t job_total_bins = 0;
@@ -76,5 +77,15 @@ struct BatchResult {
 void batchSubmit(Table& t, BatchResult& out);
 // Waits for and retires every pending job of `b`.
 void batchWait(BatchResult& b);
+// Retires the jobs of `b` that have finished, without waiting; true when none is left.
+bool batchPoll(BatchResult& b);
+// Device bytes of one lane (its Mutations scratch serves every sequence column).
+size_t laneBytes(const Table& t, uint32_t max_queries, uint32_t row_capacity);
+
+namespace query {
+class Emitter;
+}
+// Cardinalities of compiled filter programs, evaluated as one batch.
+std::vector<uint64_t> countPrograms(Table& t, const std::vector<query::Emitter>& programs);
 
 }  // namespace rdb

@@ -180,6 +180,18 @@ int rdb_table_create(rdb_ctx* ctx, const uint32_t* chunk_sizes, uint32_t n_chunk
    });
 }
 
+int rdb_device_memory(rdb_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+   return guarded(ctx ? &ctx->c : nullptr, [&] {
+      if (!ctx || !free_bytes || !total_bytes) fail(RDB_EINVAL, "null argument");
+      std::lock_guard<std::recursive_mutex> lock(ctx->c.mu);
+      size_t f = 0, tot = 0;
+      hipCheck(hipSetDevice(ctx->c.device), "hipSetDevice");
+      hipCheck(hipMemGetInfo(&f, &tot), "hipMemGetInfo");
+      *free_bytes = f;
+      *total_bytes = tot;
+   });
+}
+
 int rdb_set_tuning(rdb_ctx* ctx, const char* key, int64_t value) {
    return guarded(ctx ? &ctx->c : nullptr, [&] {
       if (!ctx || !key) fail(RDB_EINVAL, "null argument");
@@ -572,6 +584,61 @@ int rdb_mutations_query(rdb_table* table, const char* filter, const char* column
    });
 }
 
+}  // extern "C"
+
+namespace rdb::capi {
+
+void aggregateRows(Table& t, const DeviceBitmap& f, const char* dims, std::vector<uint32_t>& ids,
+                   std::vector<uint64_t>& counts) {
+   const std::vector<capi::AggDim> ds = capi::parseDims(t, dims);
+   GroupDims gd{};
+   for (size_t k = 0; k < ds.size(); ++k) {
+      if (ds[k].is_seq) {
+         const SeqColumn& col = *ds[k].seq;
+         const uint32_t p = ds[k].position;
+         const size_t padded = static_cast<size_t>(t.n_chunks) << 16;
+         if (t.labels_scratch[k].bytes() < padded * 4) t.labels_scratch[k] = DeviceBuffer(padded * 4);
+         const uint32_t blk = col.block_of_pos[p];
+         const uint32_t base = col.block_slot_start[blk];
+         hipCheck(launchSymbolLabels(t.seq_views_dev.as<SeqColumnView>() + ds[k].seq_index, t.chunk_sizes_dev.as<uint32_t>(),
+                                     p, t.n_chunks, blk, col.pos_slot_ptr[p] - base, col.pos_slot_ptr[p + 1] - base,
+                                     t.labels_scratch[k].as<uint32_t>(), t.ctx->stream),
+                  "symbol labels");
+         gd.labels[k] = t.labels_scratch[k].as<uint32_t>();
+      } else {
+         gd.labels[k] = ds[k].dict->labels.as<uint32_t>();
+      }
+   }
+   gd.n = static_cast<uint32_t>(ds.size());
+   if (ds.empty()) {
+      // partition() at depth 0 == #dimensions emits the single empty combination with |F|
+      // (bitmap_aggregation_node.cpp:91-101), also when |F| is 0
+      ids.clear();
+      counts.assign(1, const_cast<DeviceBitmap&>(f).cardinality());
+      return;
+   }
+   const uint64_t bins = capi::dimStrides(ds, gd.stride);
+   const size_t hb = bins * 4;
+   void* hist = t.ctx->pool.get(hb);
+   std::vector<uint32_t> h(bins);
+   try {
+      hipCheck(launchGroupHistogram(t.n_chunks, f.words, f.chunk_cards, gd, static_cast<uint32_t>(bins),
+                                    static_cast<uint32_t*>(hist), t.ctx->stream),
+               "group histogram");
+      hipCheck(hipMemcpyAsync(h.data(), hist, hb, hipMemcpyDeviceToHost, t.ctx->stream), "copy histogram");
+      hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
+   } catch (...) {
+      t.ctx->pool.put(hist, hb);
+      throw;
+   }
+   t.ctx->pool.put(hist, hb);
+   decodeHistogram(ds, gd.stride, h.data(), bins, ids, counts);
+}
+
+}  // namespace rdb::capi
+
+extern "C" {
+
 int rdb_aggregate(rdb_table* table, const rdb_bitmap* filter, const char* dims, uint32_t* ids, uint64_t* counts,
                   uint64_t capacity, uint64_t* n) {
    Context* ctx = table ? table->t.ctx : nullptr;
@@ -581,52 +648,9 @@ int rdb_aggregate(rdb_table* table, const rdb_bitmap* filter, const char* dims, 
       requireFinalized(t);
       if (!dims || !n) fail(RDB_EINVAL, "null argument");
       const DeviceBitmap& f = filterOf(t, filter);
-      const std::vector<capi::AggDim> ds = capi::parseDims(t, dims);
-      GroupDims gd{};
-      for (size_t k = 0; k < ds.size(); ++k) {
-         if (ds[k].is_seq) {
-            const SeqColumn& col = *ds[k].seq;
-            const uint32_t p = ds[k].position;
-            const size_t padded = static_cast<size_t>(t.n_chunks) << 16;
-            if (t.labels_scratch[k].bytes() < padded * 4) t.labels_scratch[k] = DeviceBuffer(padded * 4);
-            const uint32_t blk = col.block_of_pos[p];
-            const uint32_t base = col.block_slot_start[blk];
-            hipCheck(launchSymbolLabels(t.seq_views_dev.as<SeqColumnView>() + ds[k].seq_index, t.chunk_sizes_dev.as<uint32_t>(),
-                                        p, t.n_chunks, blk, col.pos_slot_ptr[p] - base, col.pos_slot_ptr[p + 1] - base,
-                                        t.labels_scratch[k].as<uint32_t>(), t.ctx->stream),
-                     "symbol labels");
-            gd.labels[k] = t.labels_scratch[k].as<uint32_t>();
-         } else {
-            gd.labels[k] = ds[k].dict->labels.as<uint32_t>();
-         }
-      }
-      gd.n = static_cast<uint32_t>(ds.size());
-      if (ds.empty()) {
-         // partition() at depth 0 == #dimensions emits the single empty combination with |F|
-         // (bitmap_aggregation_node.cpp:91-101), also when |F| is 0
-         *n = 1;
-         if (capacity < 1) fail(RDB_ENOMEM, "output capacity too small");
-         counts[0] = const_cast<DeviceBitmap&>(f).cardinality();
-         return;
-      }
-      const uint64_t bins = capi::dimStrides(ds, gd.stride);
-      const size_t hb = bins * 4;
-      void* hist = t.ctx->pool.get(hb);
-      std::vector<uint32_t> h(bins);
-      try {
-         hipCheck(launchGroupHistogram(t.n_chunks, f.words, f.chunk_cards, gd, static_cast<uint32_t>(bins),
-                                       static_cast<uint32_t*>(hist), t.ctx->stream),
-                  "group histogram");
-         hipCheck(hipMemcpyAsync(h.data(), hist, hb, hipMemcpyDeviceToHost, t.ctx->stream), "copy histogram");
-         hipCheck(hipStreamSynchronize(t.ctx->stream), "synchronize");
-      } catch (...) {
-         t.ctx->pool.put(hist, hb);
-         throw;
-      }
-      t.ctx->pool.put(hist, hb);
       std::vector<uint32_t> out_ids;
       std::vector<uint64_t> out_counts;
-      capi::decodeHistogram(ds, gd.stride, h.data(), bins, out_ids, out_counts);
+      capi::aggregateRows(t, f, dims, out_ids, out_counts);
       *n = out_counts.size();
       if (out_counts.size() > capacity) fail(RDB_ENOMEM, "output capacity too small");
       std::copy(out_ids.begin(), out_ids.end(), ids);

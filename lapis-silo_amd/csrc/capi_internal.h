@@ -60,6 +60,12 @@ uint64_t dimStrides(const std::vector<AggDim>& ds, uint32_t* stride);
 // sequence position) with the null group (UINT32_MAX) last, depth first.
 void decodeHistogram(const std::vector<AggDim>& ds, const uint32_t* stride, const uint32_t* hist, uint64_t bins,
                      std::vector<uint32_t>& ids, std::vector<uint64_t>& counts);
+// The same from (bin, count) pairs in any order.
+void decodeBins(const std::vector<AggDim>& ds, const uint32_t* stride, const std::vector<std::pair<uint32_t, uint32_t>>& bins,
+                std::vector<uint32_t>& ids, std::vector<uint64_t>& counts);
+// The Aggregated action under a device filter (single-query path).
+void aggregateRows(Table& t, const DeviceBitmap& f, const char* dims, std::vector<uint32_t>& ids,
+                   std::vector<uint64_t>& counts);
 
 // Mutations of column `idx` under a device filter, rows sorted in the reference's output order.
 std::vector<MutationRowDev> mutationRows(Table& t, const DeviceBitmap& f, int idx, double min_proportion);

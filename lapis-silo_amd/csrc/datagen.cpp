@@ -96,6 +96,15 @@ struct Dataset {
    std::vector<int32_t> date;
    std::vector<uint8_t> country_null, date_null, lineage_null;
    std::string names_buf;
+   // amino-acid columns (dg_translate)
+   struct AaColumn {
+      std::vector<uint8_t> reference;
+      std::vector<uint32_t> cov_start, cov_end;
+      std::vector<uint64_t> mut_ptr, nrun_ptr;
+      std::vector<uint32_t> mut_pos, nrun_start, nrun_end;
+      std::vector<uint8_t> mut_sym;
+   };
+   std::vector<AaColumn> aa;
 };
 
 uint8_t otherBase(std::mt19937_64& rng, uint8_t current) {
@@ -409,6 +418,28 @@ void* dg_generate(
 
 void dg_free(void* handle) { delete static_cast<Dataset*>(handle); }
 
+/// Amino-acid columns derived from the nucleotide rows: gene g covers the codons at nucleotide
+/// positions gene_start[g] + 3 i .. + 2 for residues i < gene_len[g].  A residue is missing ('X')
+/// when any of its three nucleotides is outside the row's covered range or inside a missing run;
+/// a deletion ('-') when all three are '-'; also missing when only some are; otherwise the codon's
+/// amino acid in the standard genetic code, a mutation when it differs from the reference's.
+/// Leading and trailing missing residues bound the residue coverage, the others become missing
+/// runs, as for nucleotides.  Null rows stay null.  Outputs are in the generator's CSR layout per
+/// gene and stay owned by the dataset (dg_aa_* getters).
+int dg_translate(void* handle, uint32_t n_genes, const uint32_t* gene_start, const uint32_t* gene_len);
+uint32_t dg_aa_genes(void* h);
+const uint8_t* dg_aa_reference(void* h, uint32_t g);
+const uint32_t* dg_aa_cov_start(void* h, uint32_t g);
+const uint32_t* dg_aa_cov_end(void* h, uint32_t g);
+const uint64_t* dg_aa_mut_ptr(void* h, uint32_t g);
+const uint32_t* dg_aa_mut_pos(void* h, uint32_t g);
+const uint8_t* dg_aa_mut_sym(void* h, uint32_t g);
+uint64_t dg_aa_num_mutations(void* h, uint32_t g);
+const uint64_t* dg_aa_nrun_ptr(void* h, uint32_t g);
+const uint32_t* dg_aa_nrun_start(void* h, uint32_t g);
+const uint32_t* dg_aa_nrun_end(void* h, uint32_t g);
+uint64_t dg_aa_num_nruns(void* h, uint32_t g);
+
 uint32_t dg_num_rows(void* h) { return static_cast<Dataset*>(h)->p.n_rows; }
 uint32_t dg_genome_length(void* h) { return static_cast<Dataset*>(h)->p.genome_length; }
 uint32_t dg_num_lineages(void* h) { return static_cast<uint32_t>(static_cast<Dataset*>(h)->parent.size()); }
@@ -457,5 +488,172 @@ int dg_materialize(void* h, uint32_t row, char* out) {
    }
    return 0;
 }
+
+}  // extern "C"
+
+namespace {
+
+// Standard genetic code; codon index 16 b1 + 4 b2 + b3 with T = 0, C = 1, A = 2, G = 3.
+constexpr char kCode[] = "FFLLSSSSYY**CC*WLLLLPPPPHHQQRRRRIIIMTTTTNNKKSSRRVVVVAAAADDEEGGGG";
+constexpr char kAaChars[] = "-ACDEFGHIKLMNOPQRSTUVWYBJZ*X";
+constexpr uint8_t kAaGap = 0;
+constexpr uint8_t kAaMissing = 27;
+
+uint8_t aaIndex(char c) {
+   for (uint8_t i = 0; kAaChars[i]; ++i) {
+      if (kAaChars[i] == c) return i;
+   }
+   return kAaMissing;
+}
+
+// nucleotide symbol (A=1, C=2, G=3, T=4) -> code index digit (T=0, C=1, A=2, G=3)
+int codeDigit(uint8_t nuc) {
+   switch (nuc) {
+      case 4: return 0;
+      case 2: return 1;
+      case 1: return 2;
+      case 3: return 3;
+      default: return -1;
+   }
+}
+
+uint8_t translate(uint8_t a, uint8_t b, uint8_t c) {
+   if (a == kGap && b == kGap && c == kGap) return kAaGap;
+   const int x = codeDigit(a), y = codeDigit(b), z = codeDigit(c);
+   if (x < 0 || y < 0 || z < 0) return kAaMissing;  // partial deletion or an ambiguous base
+   return aaIndex(kCode[16 * x + 4 * y + z]);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dg_translate(void* handle, uint32_t n_genes, const uint32_t* gene_start, const uint32_t* gene_len) {
+   auto* ds = static_cast<Dataset*>(handle);
+   const uint32_t L = ds->p.genome_length;
+   const uint32_t n = ds->p.n_rows;
+   for (uint32_t g = 0; g < n_genes; ++g) {
+      if (gene_len[g] == 0 || gene_start[g] + 3ULL * gene_len[g] > L) return -1;
+   }
+   ds->aa.assign(n_genes, {});
+   for (uint32_t g = 0; g < n_genes; ++g) {
+      auto& col = ds->aa[g];
+      const uint32_t gs = gene_start[g], gl = gene_len[g];
+      col.reference.resize(gl);
+      for (uint32_t i = 0; i < gl; ++i) {
+         col.reference[i] = translate(ds->reference[gs + 3 * i], ds->reference[gs + 3 * i + 1], ds->reference[gs + 3 * i + 2]);
+      }
+      struct RowAa {
+         uint32_t start = 0, end = 0;
+         std::vector<std::pair<uint32_t, uint32_t>> runs;
+         std::vector<std::pair<uint32_t, uint8_t>> muts;
+      };
+      std::vector<RowAa> rows(n);
+#pragma omp parallel for schedule(dynamic, 4096)
+      for (int64_t r = 0; r < static_cast<int64_t>(n); ++r) {
+         RowAa& out = rows[r];
+         if (ds->is_null[r] || ds->cov_start[r] >= ds->cov_end[r]) continue;
+         const uint32_t cs = ds->cov_start[r], ce = ds->cov_end[r];
+         // missing residues: codons not fully inside [cs, ce) or touching an internal N run
+         std::vector<uint8_t> missing(gl, 0);
+         for (uint32_t i = 0; i < gl; ++i) {
+            const uint32_t a = gs + 3 * i;
+            if (a < cs || a + 3 > ce) missing[i] = 1;
+         }
+         for (uint64_t k = ds->nrun_ptr[r]; k < ds->nrun_ptr[r + 1]; ++k) {
+            const uint32_t a = ds->nrun_start[k], b = ds->nrun_end[k];
+            if (b <= gs || a >= gs + 3 * gl) continue;
+            const uint32_t i0 = a > gs ? (a - gs) / 3 : 0;
+            const uint32_t i1 = std::min<uint32_t>(gl, (b - gs + 2) / 3);
+            for (uint32_t i = i0; i < i1; ++i) missing[i] = 1;
+         }
+         // residues whose codon carries a nucleotide mutation
+         const uint32_t* mp = ds->mut_pos.data() + ds->mut_ptr[r];
+         const uint32_t* me = ds->mut_pos.data() + ds->mut_ptr[r + 1];
+         const uint8_t* ms = ds->mut_sym.data() + ds->mut_ptr[r];
+         auto nucAt = [&](uint32_t pos) -> uint8_t {
+            const uint32_t* it = std::lower_bound(mp, me, pos);
+            return (it != me && *it == pos) ? ms[it - mp] : ds->reference[pos];
+         };
+         const uint32_t* it = std::lower_bound(mp, me, gs);
+         uint32_t last = UINT32_MAX;
+         for (; it != me && *it < gs + 3 * gl; ++it) {
+            const uint32_t i = (*it - gs) / 3;
+            if (i == last || missing[i]) continue;
+            last = i;
+            const uint32_t a = gs + 3 * i;
+            const uint8_t aa = translate(nucAt(a), nucAt(a + 1), nucAt(a + 2));
+            if (aa == kAaMissing) {
+               missing[i] = 1;
+            } else if (aa != col.reference[i]) {
+               out.muts.emplace_back(i, aa);
+            }
+         }
+         // coverage: leading and trailing missing residues
+         uint32_t s0 = 0, e0 = gl;
+         while (s0 < gl && missing[s0]) ++s0;
+         while (e0 > s0 && missing[e0 - 1]) --e0;
+         if (s0 >= e0) continue;  // nothing covered
+         out.start = s0;
+         out.end = e0;
+         for (uint32_t i = s0; i < e0;) {
+            if (!missing[i]) {
+               ++i;
+               continue;
+            }
+            uint32_t j = i;
+            while (j < e0 && missing[j]) ++j;
+            out.runs.emplace_back(i, j);
+            i = j;
+         }
+         out.muts.erase(std::remove_if(out.muts.begin(), out.muts.end(),
+                                       [&](const auto& m) { return missing[m.first] != 0; }),
+                        out.muts.end());
+      }
+      col.cov_start.resize(n);
+      col.cov_end.resize(n);
+      col.mut_ptr.assign(n + 1, 0);
+      col.nrun_ptr.assign(n + 1, 0);
+      for (uint32_t r = 0; r < n; ++r) {
+         col.mut_ptr[r + 1] = col.mut_ptr[r] + rows[r].muts.size();
+         col.nrun_ptr[r + 1] = col.nrun_ptr[r] + rows[r].runs.size();
+      }
+      col.mut_pos.resize(col.mut_ptr[n]);
+      col.mut_sym.resize(col.mut_ptr[n]);
+      col.nrun_start.resize(col.nrun_ptr[n]);
+      col.nrun_end.resize(col.nrun_ptr[n]);
+#pragma omp parallel for schedule(static)
+      for (int64_t r = 0; r < static_cast<int64_t>(n); ++r) {
+         col.cov_start[r] = rows[r].start;
+         col.cov_end[r] = rows[r].end;
+         uint64_t o = col.mut_ptr[r];
+         for (const auto& [i, a] : rows[r].muts) {
+            col.mut_pos[o] = i;
+            col.mut_sym[o] = a;
+            ++o;
+         }
+         o = col.nrun_ptr[r];
+         for (const auto& [a, b] : rows[r].runs) {
+            col.nrun_start[o] = a;
+            col.nrun_end[o] = b;
+            ++o;
+         }
+      }
+   }
+   return 0;
+}
+
+uint32_t dg_aa_genes(void* h) { return static_cast<uint32_t>(static_cast<Dataset*>(h)->aa.size()); }
+const uint8_t* dg_aa_reference(void* h, uint32_t g) { return static_cast<Dataset*>(h)->aa.at(g).reference.data(); }
+const uint32_t* dg_aa_cov_start(void* h, uint32_t g) { return static_cast<Dataset*>(h)->aa.at(g).cov_start.data(); }
+const uint32_t* dg_aa_cov_end(void* h, uint32_t g) { return static_cast<Dataset*>(h)->aa.at(g).cov_end.data(); }
+const uint64_t* dg_aa_mut_ptr(void* h, uint32_t g) { return static_cast<Dataset*>(h)->aa.at(g).mut_ptr.data(); }
+const uint32_t* dg_aa_mut_pos(void* h, uint32_t g) { return static_cast<Dataset*>(h)->aa.at(g).mut_pos.data(); }
+const uint8_t* dg_aa_mut_sym(void* h, uint32_t g) { return static_cast<Dataset*>(h)->aa.at(g).mut_sym.data(); }
+uint64_t dg_aa_num_mutations(void* h, uint32_t g) { return static_cast<Dataset*>(h)->aa.at(g).mut_pos.size(); }
+const uint64_t* dg_aa_nrun_ptr(void* h, uint32_t g) { return static_cast<Dataset*>(h)->aa.at(g).nrun_ptr.data(); }
+const uint32_t* dg_aa_nrun_start(void* h, uint32_t g) { return static_cast<Dataset*>(h)->aa.at(g).nrun_start.data(); }
+const uint32_t* dg_aa_nrun_end(void* h, uint32_t g) { return static_cast<Dataset*>(h)->aa.at(g).nrun_end.data(); }
+uint64_t dg_aa_num_nruns(void* h, uint32_t g) { return static_cast<Dataset*>(h)->aa.at(g).nrun_start.size(); }
 
 }  // extern "C"

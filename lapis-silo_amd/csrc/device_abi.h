@@ -97,6 +97,8 @@ struct SeqColumnView {
    const uint32_t* nrun_ptr;          // [n_chunks * 2^16 + 1]
    const uint32_t* nruns;             // start | end << 16, end exclusive
    const uint64_t* null_bits;         // [n_chunks * 1024]
+   const uint32_t* pos_ranges;        // [(chunk * L + position) * 6]: [lo, hi) of the position's light
+                                      // entries, groups and descriptors in its tile
    const uint32_t* row_mut_ptr;       // [n_chunks * 2^16 + 1] row-major mutations (sparse path)
    const uint32_t* row_muts;          // position | symbol << 16, original-reference frame
 };
@@ -126,8 +128,9 @@ struct TableView {
 enum OpCode : uint32_t {
    OP_EMPTY = 0,
    OP_FULL,
-   OP_BITMAP,    // a = bitmap-store id
-   OP_SYMBOLS,   // a = seq column, b = block, c = slot lo, d = slot hi, e = symbol mask
+   OP_BITMAP,    // a = bitmap-store id, b = prefetch slot
+   OP_SYMBOLS,   // a = seq column, b = block, c = slot lo, d = slot hi, e = symbol mask, f = position,
+                 // g = prefetch slot
    OP_COVERED,   // a = seq column, b = position
    OP_CMP,       // a = int column, b = comparator, c = value (bits), d = with_nulls
    OP_RANGES,    // a = first range, b = count (global [lo, hi) pairs in the program's range table)
@@ -138,6 +141,8 @@ enum OpCode : uint32_t {
    OP_TH_BEGIN,  // a = planes: push started-mask and `a` counter planes
    OP_TH_ADD,    // a = planes, b = negated child
    OP_TH_END,    // a = planes, b = n, c = exact
+   OP_SKIP_IF_EMPTY,  // a = target: jump when the top word is zero in every lane (rest of an And)
+   OP_SKIP_IF_FULL,   // a = target: jump when the top word is all ones in every lane (rest of an Or)
 };
 
 enum CmpCode : uint32_t { CMP_EQ = 0, CMP_LT, CMP_GT, CMP_LE, CMP_GE, CMP_NE };
@@ -146,11 +151,20 @@ struct Instr {
    uint32_t op, a, b, c, d, e, f, g;
 };
 
+// Leaves whose per-chunk records (a sequence position's six tile offsets, a store bitmap's
+// descriptor) a wavefront loads together, one per lane, before running the program: codes
+// column << 16 | position, or 1 << 31 | store id.  Leaf instructions name their prefetch slot
+// (OP_SYMBOLS in g, OP_BITMAP in b); kNoPrefetch loads the record at the leaf instead.
+constexpr uint32_t kPrefetchSlots = 64;
+constexpr uint32_t kNoPrefetch = 0xFFFFu;
+
 struct ProgramDesc {
    uint32_t instr_begin;
    uint32_t n_instr;
    uint32_t range_begin;
    uint32_t query;       // index into the per-query outputs
+   uint32_t pf_begin;    // prefetch codes at ranges[pf_begin ..]
+   uint32_t n_pf;
 };
 
 }  // namespace rdb

@@ -124,6 +124,9 @@ struct Context {
    uint32_t batch_lanes = 2;           // lanes (streams with their own buffers) per table
    uint32_t batch_graphs = 1;          // capture each lane's launch sequence into a HIP graph
    uint32_t batch_row_capacity = 4096; // Mutations rows per query a lane job keeps (more: rerun alone)
+   // planner of the batched path: 0 reference order, 1 independence estimates, 2 learned estimator
+   int plan_mode = 0;
+   struct Mlp* planner = nullptr;
    void init(int dev);
    ~Context();
 };
@@ -160,6 +163,8 @@ struct SeqColumn {
    std::vector<uint64_t> chunk_index_bytes;  // per chunk: descriptors + payloads of stored symbols
    std::vector<uint64_t> chunk_nruns;        // per chunk: internal missing runs
    uint32_t n_flipped = 0;
+   std::vector<uint32_t> slot_rows;          // rows per slot over all chunks (planner statistics)
+   std::vector<uint32_t> coverage;           // covered rows per position (planner statistics)
    uint64_t row_major_bytes = 0;             // row-major mutation CSR (sparse path)
    std::vector<uint32_t> row_mut_ptr, row_nrun_ptr;  // host copies over padded rows (traffic accounting)
    DeviceBuffer sparse_counts, sparse_bnd;   // zero between calls; cleared by the sparse finalize
@@ -170,6 +175,8 @@ struct IntColumn {
    bool is_date = false;
    bool is_sorted = true;
    std::vector<int32_t> values;  // dense row order
+   std::vector<int32_t> sorted_values;  // non-null values, sorted (planner statistics)
+   uint64_t null_count = 0;
    uint32_t null_bitmap_id = kInvalidId;
    IntColumnView view{};
    std::vector<DeviceBuffer> buffers;
@@ -208,6 +215,7 @@ struct Table {
 
    // bitmap store: rows per (bitmap, chunk), uploaded at finalize
    std::vector<std::vector<std::vector<uint16_t>>> store_rows;
+   std::vector<uint64_t> store_card;  // rows of each store bitmap (planner statistics)
    ArenaBuilder store_builder;
    std::vector<DeviceBuffer> store_buffers;
 
@@ -219,6 +227,7 @@ struct Table {
    size_t hist_bins_capacity = 0;
    // lanes of the batched path (created on demand)
    std::vector<std::shared_ptr<BatchLane>> lanes;
+   uint64_t job_counter = 0;
 
    uint32_t addStoreBitmap(const std::vector<uint32_t>& sorted_global_rows);
    uint32_t globalRow(uint64_t dense) const;

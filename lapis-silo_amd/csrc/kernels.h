@@ -103,7 +103,8 @@ struct MutationsBatch {
 hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream);
 
 // Aggregated group counts for a batch: entry e (e < *n_agg) counts the rows of query agg_query[e]
-// into hist[hist_offset[e] .. + bins[e]), which the launch zeroes first (*total_bins words).
+// into hist[hist_offset[e] .. + bins[e]), which the launch zeroes first (*total_bins words), then
+// compacts the non-empty bins into the query's row slots.
 struct AggregateBatch {
    uint32_t n_chunks = 0;
    uint32_t max_entries = 0;
@@ -117,6 +118,11 @@ struct AggregateBatch {
    const uint64_t* filters = nullptr;
    const uint32_t* chunk_cards = nullptr;
    uint32_t* hist = nullptr;
+   // non-empty bins of entry e go to rows + agg_query[e] * row_capacity as {bin, count, 0, 0}, their
+   // number to n_rows[agg_query[e]] (which must be zero; it may exceed the capacity)
+   MutationRowDev* rows = nullptr;
+   uint32_t row_capacity = 0;
+   uint32_t* n_rows = nullptr;
 };
 
 hipError_t launchAggregateBatch(const AggregateBatch& a, hipStream_t stream);

@@ -35,19 +35,6 @@ __device__ inline uint64_t layoutMask(uint32_t chunk_size, uint32_t word) {
    return (1ULL << (chunk_size - lo)) - 1ULL;
 }
 
-__device__ inline uint32_t lowerBound16(const uint16_t* a, uint32_t n, uint32_t x) {
-   uint32_t lo = 0, hi = n;
-   while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (static_cast<uint32_t>(a[mid]) < x) {
-         lo = mid + 1;
-      } else {
-         hi = mid;
-      }
-   }
-   return lo;
-}
-
 // Lower bound by the whole wavefront: the first index i in [lo, hi) with !before(i), for a predicate
 // that holds on a prefix.  Each round samples 64 evenly spaced indices with one load per lane and
 // keeps the interval between the last sample before the bound and the first one after it, so a
@@ -175,6 +162,8 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
    const uint32_t lane = threadIdx.x & 63;
    uint64_t* stack = lds + static_cast<size_t>(wave) * (depth + 1) * 64;
    uint64_t* scratch = stack + static_cast<size_t>(depth) * 64;
+   uint32_t* pf = reinterpret_cast<uint32_t*>(lds + static_cast<size_t>(kVmWaves) * (depth + 1) * 64) +
+                  static_cast<size_t>(wave) * kPrefetchSlots * 8;
    const uint32_t n_chunks = tv.n_chunks;
    const uint32_t per_query = n_chunks * kSlicesPerChunk;
    if (n_queries_dev) n_items = uniform(*n_queries_dev * per_query);
@@ -188,6 +177,29 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
       const uint32_t my_word = slice * 64 + lane;
       const uint64_t lmask = layoutMask(chunk_size, my_word);
       const size_t row_base = (static_cast<size_t>(chunk) << 16) + slice * 4096u;
+      {
+         // the program's leaf records for this chunk, one per lane, in one round of loads
+         waveSync();  // the previous item's reads of the records are done
+         const uint32_t n_pf = pd.n_pf < kPrefetchSlots ? pd.n_pf : kPrefetchSlots;
+         if (lane < n_pf) {
+            const uint32_t code = ranges[pd.pf_begin + lane];
+            uint32_t* dst = pf + lane * 8;
+            if (code >> 31) {
+               const Desc d = tv.store.desc[static_cast<size_t>(code & 0x7FFFFFFFu) * n_chunks + chunk];
+               dst[0] = d.w0;
+               dst[1] = d.w1;
+            } else {
+               const SeqColumnView& c = tv.seq[code >> 16];
+               const uint32_t* r = c.pos_ranges + (static_cast<size_t>(chunk) * c.genome_length + (code & 0xFFFFu)) * 6;
+               uint32_t v[6];
+#pragma unroll
+               for (uint32_t k = 0; k < 6; ++k) v[k] = r[k];
+#pragma unroll
+               for (uint32_t k = 0; k < 6; ++k) dst[k] = v[k];
+            }
+         }
+         waveSync();
+      }
       uint32_t sp = 0;
       auto push = [&](uint64_t v) {
          stack[sp * 64 + lane] = v;
@@ -207,7 +219,8 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                push(lmask);
                break;
             case OP_BITMAP: {
-               const Desc d = tv.store.desc[static_cast<size_t>(ins.a) * n_chunks + chunk];
+               const Desc d = ins.b < kPrefetchSlots ? Desc{pf[ins.b * 8], pf[ins.b * 8 + 1]}
+                                                     : tv.store.desc[static_cast<size_t>(ins.a) * n_chunks + chunk];
                uint64_t acc = 0;
                scratch[lane] = 0;
                waveSync();
@@ -217,57 +230,45 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                break;
             }
             case OP_SYMBOLS: {
+               // the position's entries in this chunk's tile, located by one record of six offsets
                const SeqColumnView& col = tv.seq[ins.a];
-               const uint32_t tile = ins.b * n_chunks + chunk;
                const uint32_t slot_base = col.block_slot_start[ins.b];
+               const uint32_t* pr = ins.g < kPrefetchSlots
+                                       ? pf + ins.g * 8
+                                       : col.pos_ranges + (static_cast<size_t>(chunk) * col.genome_length + ins.f) * 6;
+               const uint32_t l_lo = uniform(pr[0]), l_hi = uniform(pr[1]);
+               const uint32_t g_lo = uniform(pr[2]), g_hi = uniform(pr[3]);
+               const uint32_t h_lo = uniform(pr[4]), h_hi = uniform(pr[5]);
                uint64_t acc = 0;
                scratch[lane] = 0;
                waveSync();
-               {
-                  // light entries of the slots [c, d): slot << 16 | row, sorted
-                  const uint32_t* le = col.light;
-                  const uint32_t lo = col.light_ptr[tile], hi = col.light_ptr[tile + 1];
-                  const uint32_t a = waveLowerBound(lo, hi, [&](uint32_t i) { return (le[i] >> 16) < ins.c; });
-                  const uint32_t e = waveLowerBound(a, hi, [&](uint32_t i) { return (le[i] >> 16) < ins.d; });
-                  for (uint32_t k = a + lane; k < e; k += 64) {
-                     const uint32_t x = le[k];
-                     const uint32_t row = x & 0xFFFFu;
-                     const uint32_t sym = col.slot_info[slot_base + (x >> 16)] >> 16;
-                     if (((ins.e >> sym) & 1u) && (row >> 12) == slice) {
-                        atomicOr(reinterpret_cast<unsigned long long*>(&scratch[(row >> 6) & 63]), 1ULL << (row & 63));
+               for (uint32_t k = l_lo + lane; k < l_hi; k += 64) {
+                  const uint32_t x = col.light[k];
+                  const uint32_t row = x & 0xFFFFu;
+                  const uint32_t sym = col.slot_info[slot_base + (x >> 16)] >> 16;
+                  if (((ins.e >> sym) & 1u) && (row >> 12) == slice) {
+                     atomicOr(reinterpret_cast<unsigned long long*>(&scratch[(row >> 6) & 63]), 1ULL << (row & 63));
+                  }
+               }
+               for (uint32_t k = g_lo + lane; k < g_hi; k += 64) {
+                  const uint32_t hdr = col.group_hdr[k];
+                  const uint32_t sym = col.slot_info[slot_base + (hdr & 0xFFFu)] >> 16;
+                  if (!((ins.e >> sym) & 1u)) continue;
+                  const uint4 rows = reinterpret_cast<const uint4*>(col.groups)[k];
+                  const uint32_t r8[8] = {rows.x & 0xFFFFu, rows.x >> 16, rows.y & 0xFFFFu, rows.y >> 16,
+                                          rows.z & 0xFFFFu, rows.z >> 16, rows.w & 0xFFFFu, rows.w >> 16};
+                  const uint32_t m = (hdr >> 12) + 1;
+#pragma unroll
+                  for (uint32_t q = 0; q < 8; ++q) {
+                     if (q < m && (r8[q] >> 12) == slice) {
+                        atomicOr(reinterpret_cast<unsigned long long*>(&scratch[(r8[q] >> 6) & 63]), 1ULL << (r8[q] & 63));
                      }
                   }
                }
-               {
-                  // groups of eight rows of the slots [c, d)
-                  const uint16_t* gh = col.group_hdr;
-                  const uint32_t g_hi = col.group_ptr[tile + 1];
-                  const uint32_t a = waveLowerBound(col.group_ptr[tile], g_hi, [&](uint32_t i) { return (gh[i] & 0xFFFu) < ins.c; });
-                  const uint32_t e = waveLowerBound(a, g_hi, [&](uint32_t i) { return (gh[i] & 0xFFFu) < ins.d; });
-                  for (uint32_t k = a + lane; k < e; k += 64) {
-                     const uint32_t hdr = gh[k];
-                     const uint32_t sym = col.slot_info[slot_base + (hdr & 0xFFFu)] >> 16;
-                     if (!((ins.e >> sym) & 1u)) continue;
-                     const uint32_t m = (hdr >> 12) + 1;
-                     for (uint32_t q = 0; q < m; ++q) {
-                        const uint32_t row = col.groups[static_cast<size_t>(k) * 8 + q];
-                        if ((row >> 12) == slice) {
-                           atomicOr(reinterpret_cast<unsigned long long*>(&scratch[(row >> 6) & 63]), 1ULL << (row & 63));
-                        }
-                     }
-                  }
-               }
-               {
-                  const uint32_t end = col.heavy_ptr[tile + 1];
-                  const uint32_t lo = waveLowerBound(col.heavy_ptr[tile], end,
-                                                     [&](uint32_t i) { return descSlot(col.vert.desc[i].w0) < ins.c; });
-                  for (uint32_t k = lo; k < end; ++k) {
-                     const Desc d = col.vert.desc[k];
-                     const uint32_t slot = descSlot(d.w0);
-                     if (slot >= ins.d) break;
-                     const uint32_t sym = col.slot_info[slot_base + slot] >> 16;
-                     if ((ins.e >> sym) & 1u) accumulateContainer(d, col.vert, slice, lane, acc, scratch);
-                  }
+               for (uint32_t k = h_lo; k < h_hi; ++k) {
+                  const Desc d = col.vert.desc[k];
+                  const uint32_t sym = col.slot_info[slot_base + descSlot(d.w0)] >> 16;
+                  if ((ins.e >> sym) & 1u) accumulateContainer(d, col.vert, slice, lane, acc, scratch);
                }
                waveSync();
                push(acc | scratch[lane]);
@@ -415,6 +416,13 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                push(ins.c ? eq : (gt | eq));
                break;
             }
+            case OP_SKIP_IF_EMPTY:
+               // the slice's partial conjunction is empty: the remaining conjuncts cannot change it
+               if (__ballot(stack[(sp - 1) * 64 + lane] != 0ULL) == 0ULL) i = ins.a - 1;
+               break;
+            case OP_SKIP_IF_FULL:
+               if (__ballot(stack[(sp - 1) * 64 + lane] != ~0ULL) == 0ULL) i = ins.a - 1;
+               break;
             default:
                break;
          }
@@ -1217,6 +1225,22 @@ __global__ __launch_bounds__(256) void groupHistogramBatchKernel(AggregateBatch 
    }
 }
 
+// Non-empty bins of every entry as {bin, count} records in its query's row slots (any order).
+__global__ __launch_bounds__(256) void compactHistogramKernel(AggregateBatch a) {
+   const uint32_t e = blockIdx.y;
+   if (e >= *a.n_agg) return;
+   const uint32_t q = a.agg_query[e];
+   const uint32_t n_bins = a.bins[e];
+   const uint32_t* hist = a.hist + a.hist_offset[e];
+   MutationRowDev* out = a.rows + static_cast<size_t>(q) * a.row_capacity;
+   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_bins; i += gridDim.x * blockDim.x) {
+      const uint32_t v = hist[i];
+      if (v == 0) continue;
+      const uint32_t k = atomicAdd(&a.n_rows[q], 1u);
+      if (k < a.row_capacity) out[k] = MutationRowDev{i, v, 0, 0};
+   }
+}
+
 __global__ void fillCountKernel(uint32_t* __restrict__ p, uint32_t value, const uint32_t* __restrict__ n_dev) {
    const uint32_t n = *n_dev;
    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = value;
@@ -1272,7 +1296,10 @@ hipError_t configureFilterKernel(size_t max_lds_bytes) {
                               static_cast<int>(max_lds_bytes));
 }
 
-size_t filterLdsBytes(uint32_t depth) { return static_cast<size_t>(kVmWaves) * (depth + 1) * 64 * sizeof(uint64_t); }
+size_t filterLdsBytes(uint32_t depth) {
+   // per wave: depth stack entries and a scratch entry of 64 words, and the prefetched leaf records
+   return static_cast<size_t>(kVmWaves) * ((depth + 1) * 64 * sizeof(uint64_t) + kPrefetchSlots * 8 * sizeof(uint32_t));
+}
 
 hipError_t launchFilter(const TableView& tv, const ProgramDesc* progs, const Instr* instrs, const uint32_t* ranges,
                         uint32_t n_queries, uint32_t depth, uint64_t* out, unsigned long long* cards,
@@ -1375,6 +1402,8 @@ hipError_t launchAggregateBatch(const AggregateBatch& a, hipStream_t stream) {
    hipError_t err = hipGetLastError();
    if (err != hipSuccess) return err;
    hipLaunchKernelGGL(groupHistogramBatchKernel, dim3(a.n_chunks * 4, a.max_entries), dim3(256), 0, stream, a);
+   if ((err = hipGetLastError()) != hipSuccess) return err;
+   hipLaunchKernelGGL(compactHistogramKernel, dim3(64, a.max_entries), dim3(256), 0, stream, a);
    return hipGetLastError();
 }
 

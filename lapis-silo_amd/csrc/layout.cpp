@@ -304,6 +304,7 @@ namespace {
 
 struct ChunkContainers {
    std::vector<uint32_t> key;  // position << 8 | symbol, sorted
+   std::vector<uint32_t> rows; // rows of each container
    ArenaBuilder arenas;
 };
 
@@ -422,8 +423,10 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    std::vector<uint32_t> flipped;
    int64_t coverage = 0;
    std::vector<uint32_t> counts(nsym);
+   col.coverage.assign(L, 0);
    for (uint32_t p = 0; p < L; ++p) {
       coverage += changes[p];
+      col.coverage[p] = static_cast<uint32_t>(coverage);
       const uint8_t current = col.reference[p];
       for (uint32_t s = 0; s < nsym; ++s) counts[s] = sym_counts[static_cast<size_t>(p) * nsym + s];
       uint32_t ref_count = static_cast<uint32_t>(coverage);
@@ -500,6 +503,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
             ++k;
          }
          out.key.push_back(static_cast<uint32_t>(key));
+         out.rows.push_back(static_cast<uint32_t>(rows.size()));
          out.arenas.add(rows.data(), static_cast<uint32_t>(rows.size()), 0);
       }
    });
@@ -534,6 +538,17 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
          const uint32_t sym = static_cast<uint32_t>(std::countr_zero(m));
          m &= m - 1;
          col.slot_info[s++] = p | (sym << 16);
+      }
+   }
+
+   // rows per slot over all chunks (planner statistics)
+   col.slot_rows.assign(n_slots, 0);
+   for (uint32_t c = 0; c < t.n_chunks; ++c) {
+      const auto& ch = chunks[c];
+      for (size_t k = 0; k < ch.key.size(); ++k) {
+         const uint32_t p = ch.key[k] >> 8, sym = ch.key[k] & 0xFF;
+         const uint32_t slot = col.pos_slot_ptr[p] + static_cast<uint32_t>(std::popcount(sym_mask[p] & ((1u << sym) - 1u)));
+         col.slot_rows[slot] += ch.rows[k];
       }
    }
 
@@ -581,6 +596,8 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    std::vector<uint16_t> group_hdr;
    std::vector<Desc> desc;
    std::vector<size_t> cursor(t.n_chunks, 0);
+   // per (chunk, position): [lo, hi) of the position's light entries, groups and descriptors
+   std::vector<uint32_t> pos_ranges(static_cast<size_t>(t.n_chunks) * L * 6, 0);
    for (uint32_t b = 0; b < n_blocks; ++b) {
       const uint32_t pend = col.block_pos_start[b + 1];
       const uint32_t sbase = col.block_slot_start[b];
@@ -591,8 +608,16 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
          group_ptr[tile] = static_cast<uint32_t>(group_hdr.size());
          auto& ch = chunks[c];
          size_t& k = cursor[c];
+         uint32_t range_pos = kInvalidId;
          while (k < ch.key.size() && (ch.key[k] >> 8) < pend) {
             const uint32_t p = ch.key[k] >> 8, s = ch.key[k] & 0xFF;
+            uint32_t* pr = pos_ranges.data() + (static_cast<size_t>(c) * L + p) * 6;
+            if (p != range_pos) {
+               range_pos = p;
+               pr[0] = static_cast<uint32_t>(light.size());
+               pr[2] = static_cast<uint32_t>(group_hdr.size());
+               pr[4] = static_cast<uint32_t>(desc.size());
+            }
             const uint32_t slot = col.pos_slot_ptr[p] + std::popcount(sym_mask[p] & ((1u << s) - 1u)) - sbase;
             Desc d = ch.arenas.desc[k];
             const uint32_t type = descType(d.w0), card = descCard(d.w0);
@@ -629,6 +654,9 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
                if (type == CT_BITSET) d.w1 += static_cast<uint32_t>(bit_base[c]);
                desc.push_back(d);
             }
+            pr[1] = static_cast<uint32_t>(light.size());
+            pr[3] = static_cast<uint32_t>(group_hdr.size());
+            pr[5] = static_cast<uint32_t>(desc.size());
             ++k;
          }
          while (light.size() % 4 != 0) light.push_back(kPadSlot << 16);
@@ -782,6 +810,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    v.nrun_ptr = static_cast<const uint32_t*>(up(pptr.data(), pptr.size() * 4));
    v.nruns = static_cast<const uint32_t*>(up(runs_all.data(), runs_all.size() * 4));
    v.null_bits = static_cast<const uint64_t*>(up(null_bits.data(), null_bits.size() * 8));
+   v.pos_ranges = static_cast<const uint32_t*>(up(pos_ranges.data(), pos_ranges.size() * 4));
    v.row_mut_ptr = static_cast<const uint32_t*>(up(rmptr.data(), rmptr.size() * 4));
    v.row_muts = static_cast<const uint32_t*>(up(rmuts.data(), rmuts.size() * 4));
    col.row_major_bytes = (rmptr.size() + rmuts.size()) * 4;
@@ -816,6 +845,14 @@ void buildIntColumn(Table& t, IntColumn& col, const int32_t* values, const uint8
       }
    }
    col.null_bitmap_id = t.addStoreBitmap(null_rows);
+   // sorted non-null values: the planner's exact counts of value predicates
+   col.null_count = null_rows.size();
+   col.sorted_values.clear();
+   col.sorted_values.reserve(t.num_rows - null_rows.size());
+   for (uint64_t i = 0; i < t.num_rows; ++i) {
+      if (!is_null[i]) col.sorted_values.push_back(values[i]);
+   }
+   std::sort(col.sorted_values.begin(), col.sorted_values.end());
    col.buffers.emplace_back(vals.data(), vals.size() * 4);
    col.view.values = col.buffers.back().as<int32_t>();
    col.buffers.emplace_back(null_bits.data(), null_bits.size() * 8);
@@ -896,6 +933,11 @@ void finalizeTable(Table& t) {
       }
    }
    if (sb.desc.size() != t.store_rows.size() * t.n_chunks) fail(RDB_EDEVICE, "internal: bitmap store size mismatch");
+   t.store_card.assign(t.store_rows.size(), 0);
+   for (size_t b = 0; b < t.store_rows.size(); ++b) {
+      for (const auto& r : t.store_rows[b]) t.store_card[b] += r.size();
+   }
+
    validateDescriptors(sb.desc.data(), sb.desc.size(), sb.arr.size(), sb.runs.size(), sb.bits.size() / kChunkWords, 1);
    t.store_buffers.clear();
    auto up = [&](const void* p, size_t bytes) -> void* {

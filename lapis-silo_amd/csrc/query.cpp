@@ -51,11 +51,15 @@ DeviceBitmap runProgram(Table& t, const Emitter& program) {
    DeviceBitmap out(&t);
    hipCheck(launchFill(reinterpret_cast<uint32_t*>(out.card_dev), 0, (64 + 4 * static_cast<size_t>(t.n_chunks)) / 4, ctx.stream),
             "clear cardinalities");
-   std::vector<uint8_t> blob(sizeof(ProgramDesc) + program.code.size() * sizeof(Instr) + program.ranges.size() * 4 + 16);
-   ProgramDesc pd{0, static_cast<uint32_t>(program.code.size()), 0, 0};
+   std::vector<uint8_t> blob(sizeof(ProgramDesc) + program.code.size() * sizeof(Instr) +
+                             (program.ranges.size() + program.prefetch.size()) * 4 + 16);
+   ProgramDesc pd{0, static_cast<uint32_t>(program.code.size()), 0, 0, static_cast<uint32_t>(program.ranges.size()),
+                  static_cast<uint32_t>(program.prefetch.size())};
    std::memcpy(blob.data(), &pd, sizeof(pd));
    std::memcpy(blob.data() + sizeof(pd), program.code.data(), program.code.size() * sizeof(Instr));
    std::memcpy(blob.data() + sizeof(pd) + program.code.size() * sizeof(Instr), program.ranges.data(), program.ranges.size() * 4);
+   std::memcpy(blob.data() + sizeof(pd) + program.code.size() * sizeof(Instr) + program.ranges.size() * 4,
+               program.prefetch.data(), program.prefetch.size() * 4);
    auto* dev = static_cast<uint8_t*>(ctx.stage(blob.data(), blob.size()));
    if (ctx.profiling) hipCheck(hipEventRecord(ctx.prof_events[2], ctx.stream), "hipEventRecord");
    hipCheck(launchFilter(t.view, reinterpret_cast<const ProgramDesc*>(dev), reinterpret_cast<const Instr*>(dev + sizeof(pd)),
@@ -106,6 +110,28 @@ bool CompareToValueSelection::emitPair(Emitter& e, const Predicate* other) const
    return true;
 }
 
+uint64_t CompareToValueSelection::tableCount(const Table& t) const {
+   const IntColumn& c = *t.ints[column_];
+   const auto& v = c.sorted_values;
+   const auto lo = [&](int32_t x) { return static_cast<uint64_t>(std::lower_bound(v.begin(), v.end(), x) - v.begin()); };
+   const auto hi = [&](int32_t x) { return static_cast<uint64_t>(std::upper_bound(v.begin(), v.end(), x) - v.begin()); };
+   uint64_t n = 0;
+   switch (cmp_) {
+      case Comparator::EQUALS: n = hi(value_) - lo(value_); break;
+      case Comparator::NOT_EQUALS: n = v.size() - (hi(value_) - lo(value_)); break;
+      case Comparator::LESS: n = lo(value_); break;
+      case Comparator::LESS_OR_EQUALS: n = hi(value_); break;
+      case Comparator::HIGHER: n = v.size() - hi(value_); break;
+      case Comparator::HIGHER_OR_EQUALS: n = v.size() - lo(value_); break;
+   }
+   return n + (with_nulls_ ? c.null_count : 0);
+}
+
+uint64_t IsInCoveredRegion::tableCount(const Table& t) const {
+   const uint64_t covered = t.seq[column_]->coverage[position_];
+   return cmp_ == Comparator::IS_COVERED ? covered : t.num_rows - covered;
+}
+
 std::string IsInCoveredRegion::toString() const {
    return std::string(cmp_ == Comparator::IS_COVERED ? "" : "!") + "IsInCoveredRegion(" + std::to_string(position_) + ")";
 }
@@ -131,7 +157,7 @@ std::string IndexScan::toString() const {
 
 void IndexScan::emit(Emitter& e) const {
    if (source.kind == BitmapSource::STORE) {
-      e.leaf(Instr{OP_BITMAP, source.store_id, 0, 0, 0, 0, 0, 0});
+      e.leaf(Instr{OP_BITMAP, source.store_id, e.prefetchSlot((1u << 31) | source.store_id), 0, 0, 0, 0, 0});
       return;
    }
    const SeqColumn& col = *table->seq[source.column];
@@ -139,7 +165,8 @@ void IndexScan::emit(Emitter& e) const {
    const uint32_t b = col.block_of_pos[p];
    const uint32_t base = col.block_slot_start[b];
    e.leaf(Instr{OP_SYMBOLS, static_cast<uint32_t>(source.column), b, col.pos_slot_ptr[p] - base,
-                col.pos_slot_ptr[p + 1] - base, source.symbol_mask, 0, 0});
+                col.pos_slot_ptr[p + 1] - base, source.symbol_mask, p,
+                e.prefetchSlot((static_cast<uint32_t>(source.column) << 16) | p)});
 }
 
 // intersection.cpp:18-41
@@ -159,15 +186,19 @@ std::string Intersection::toString() const {
 
 // intersection.cpp:59-106: AND of the children, then AND NOT of the negated children
 void Intersection::emit(Emitter& e) const {
+   std::vector<size_t> skips;
    children[0]->emit(e);
    for (size_t i = 1; i < children.size(); ++i) {
+      if (short_circuit) skips.push_back(e.skip(OP_SKIP_IF_EMPTY));
       children[i]->emit(e);
       e.binary(OP_AND);
    }
    for (const auto& n : negated_children) {
+      if (short_circuit) skips.push_back(e.skip(OP_SKIP_IF_EMPTY));
       n->emit(e);
       e.binary(OP_ANDNOT);
    }
+   for (size_t at : skips) e.patch(at, e.code.size());
 }
 
 // complement.cpp:51-56
@@ -303,11 +334,14 @@ void Union::emit(Emitter& e) const {
       e.leaf(Instr{OP_EMPTY, 0, 0, 0, 0, 0, 0, 0});
       return;
    }
+   std::vector<size_t> skips;
    children[0]->emit(e);
    for (size_t i = 1; i < children.size(); ++i) {
+      if (short_circuit) skips.push_back(e.skip(OP_SKIP_IF_FULL));
       children[i]->emit(e);
       e.binary(OP_OR);
    }
+   for (size_t at : skips) e.patch(at, e.code.size());
 }
 
 // operator.cpp:23-67 with the per-type negate rules

@@ -21,8 +21,19 @@ class Emitter {
   public:
    std::vector<Instr> code;
    std::vector<uint32_t> ranges;  // global [lo, hi) pairs
+   std::vector<uint32_t> prefetch;  // leaf record codes loaded before the program runs
    uint32_t sp = 0;
    uint32_t max_sp = 0;
+
+   // prefetch slot of a leaf record (the same record shares one slot), kNoPrefetch when full
+   uint32_t prefetchSlot(uint32_t code) {
+      for (size_t i = 0; i < prefetch.size(); ++i) {
+         if (prefetch[i] == code) return static_cast<uint32_t>(i);
+      }
+      if (prefetch.size() >= kPrefetchSlots) return kNoPrefetch;
+      prefetch.push_back(code);
+      return static_cast<uint32_t>(prefetch.size() - 1);
+   }
 
    void leaf(const Instr& ins) {
       code.push_back(ins);
@@ -45,6 +56,12 @@ class Emitter {
       code.push_back(Instr{OP_TH_END, planes, n, exact ? 1u : 0u, 0, 0, 0, 0});
       sp -= planes;
    }
+   // a forward jump whose target is set by patch(); leaves the stack as it is
+   size_t skip(uint32_t op) {
+      code.push_back(Instr{op, 0, 0, 0, 0, 0, 0, 0});
+      return code.size() - 1;
+   }
+   void patch(size_t at, size_t target) { code[at].a = static_cast<uint32_t>(target); }
 
   private:
    void bump(uint32_t k) {
@@ -64,6 +81,8 @@ class Predicate {
    virtual ~Predicate() = default;
    virtual std::string toString() const = 0;
    virtual double estimateSelectivity() const { return 0.5; }
+   // rows of the whole table the predicate matches, from host-side column statistics
+   virtual uint64_t tableCount(const Table& t) const = 0;
    virtual std::unique_ptr<Predicate> copy() const = 0;
    virtual std::unique_ptr<Predicate> negate() const = 0;
    virtual void emit(Emitter& e) const = 0;
@@ -83,6 +102,7 @@ class CompareToValueSelection : public Predicate {
    // one instruction for this predicate and, when `other` compares the same column, that one too
    // (their conjunction: a value range reads the column once)
    bool emitPair(Emitter& e, const Predicate* other) const;
+   uint64_t tableCount(const Table& t) const override;
 
   private:
    const Table* table_;
@@ -104,6 +124,7 @@ class IsInCoveredRegion : public Predicate {
    }
    std::unique_ptr<Predicate> negate() const override;
    void emit(Emitter& e) const override;
+   uint64_t tableCount(const Table& t) const override;
 
   private:
    const Table* table_;
@@ -177,6 +198,8 @@ class Intersection : public Operator {
    void emit(Emitter& e) const override;
    OperatorVector children;
    OperatorVector negated_children;
+   // planner: skip the remaining children of a slice once the partial result is empty
+   bool short_circuit = false;
 };
 
 class Complement : public Operator {
@@ -232,6 +255,8 @@ class Union : public Operator {
    std::string toString() const override;
    void emit(Emitter& e) const override;
    OperatorVector children;
+   // planner: skip the remaining children of a slice once the partial result is all ones
+   bool short_circuit = false;
 };
 
 // ---- expressions ----------------------------------------------------------------------------------

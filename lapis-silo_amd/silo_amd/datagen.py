@@ -92,6 +92,15 @@ def _lib() -> ctypes.CDLL:
             getattr(lib, name).argtypes = [vp]
         lib.dg_materialize.restype = ctypes.c_int
         lib.dg_materialize.argtypes = [vp, u32, ctypes.c_char_p]
+        lib.dg_translate.restype = ctypes.c_int
+        lib.dg_translate.argtypes = [vp, u32, vp, vp]
+        for name in ("dg_aa_reference", "dg_aa_cov_start", "dg_aa_cov_end", "dg_aa_mut_ptr", "dg_aa_mut_pos",
+                     "dg_aa_mut_sym", "dg_aa_nrun_ptr", "dg_aa_nrun_start", "dg_aa_nrun_end"):
+            getattr(lib, name).restype = vp
+            getattr(lib, name).argtypes = [vp, u32]
+        for name in ("dg_aa_num_mutations", "dg_aa_num_nruns"):
+            getattr(lib, name).restype = u64
+            getattr(lib, name).argtypes = [vp, u32]
         _SIGNATURES = True
     return lib
 
@@ -160,6 +169,55 @@ class Dataset:
             self.release()
         except Exception:
             pass
+
+
+# The ten amino-acid columns of the serving workload: names and residue counts (9,700 in total).
+AA_GENES = [("ORF1a", 4405), ("ORF1b", 2728), ("S", 1273), ("ORF3a", 275), ("E", 75), ("M", 222), ("ORF6", 61),
+            ("ORF7a", 121), ("ORF8", 121), ("N", 419)]
+
+
+def gene_layout(genome_length: int, genes=AA_GENES, seed: int = BASE_SEED):
+    """Seeded, non-overlapping codon start positions of the genes in genome order: the spare
+    nucleotides (genome length minus three per residue) are cut at seeded points into gaps."""
+    total = 3 * sum(n for _, n in genes)
+    spare = genome_length - total
+    if spare < 0:
+        raise ValueError("genes do not fit the genome")
+    rng = np.random.default_rng(seed)
+    cuts = np.sort(rng.integers(0, spare + 1, len(genes)))
+    starts = []
+    pos = 0
+    prev = 0
+    for (name, n), c in zip(genes, cuts):
+        pos += int(c - prev)
+        prev = int(c)
+        starts.append(pos)
+        pos += 3 * n
+    return [(name, s, n) for (name, n), s in zip(genes, starts)]
+
+
+def translate(ds: "Dataset", layout) -> list:
+    """Amino-acid columns of the dataset's rows (the generator keeps its handle for this): per gene
+    (name, reference, cov_start, cov_end, mut_ptr, mut_pos, mut_sym, nrun_ptr, nrun_start, nrun_end)."""
+    if ds._handle is None:
+        raise RuntimeError("generate with keep_handle=True to translate")
+    lib = _lib()
+    starts = np.asarray([s for _, s, _ in layout], np.uint32)
+    lens = np.asarray([n for _, _, n in layout], np.uint32)
+    if lib.dg_translate(ds._handle, len(layout), starts.ctypes.data, lens.ctypes.data) != 0:
+        raise ValueError("gene layout outside the genome")
+    h = ds._handle
+    n = ds.n_rows
+    out = []
+    for g, (name, _, gl) in enumerate(layout):
+        nm = lib.dg_aa_num_mutations(h, g)
+        nr = lib.dg_aa_num_nruns(h, g)
+        out.append((name, _copy(lib.dg_aa_reference(h, g), gl, np.uint8), _copy(lib.dg_aa_cov_start(h, g), n, np.uint32),
+                    _copy(lib.dg_aa_cov_end(h, g), n, np.uint32), _copy(lib.dg_aa_mut_ptr(h, g), n + 1, np.uint64),
+                    _copy(lib.dg_aa_mut_pos(h, g), nm, np.uint32), _copy(lib.dg_aa_mut_sym(h, g), nm, np.uint8),
+                    _copy(lib.dg_aa_nrun_ptr(h, g), n + 1, np.uint64), _copy(lib.dg_aa_nrun_start(h, g), nr, np.uint32),
+                    _copy(lib.dg_aa_nrun_end(h, g), nr, np.uint32)))
+    return out
 
 
 def generate(spec: GenSpec, keep_handle: bool = False) -> Dataset:

@@ -47,10 +47,24 @@ SIGNATURES = [
     ("rdb_mutations_traffic", _int, [_vp, _vp, _cp, _vp]),
     ("rdb_batch_submit", _int, [_vp, _vp, _u32, ctypes.POINTER(_vp)]),
     ("rdb_batch_wait", _int, [_vp]),
+    ("rdb_batch_poll", _int, [_vp, _vp]),
+    ("rdb_batch_lane_bytes", _int, [_vp, _u32, _u32, _vp]),
+    ("rdb_device_memory", _int, [_vp, _vp, _vp]),
     ("rdb_batch_count", _int, [_vp, _u32, _vp]),
     ("rdb_batch_mutations", _int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
     ("rdb_batch_aggregate", _int, [_vp, _u32, _vp, _vp, _u64, _vp]),
     ("rdb_batch_free", None, [_vp]),
+    ("rdb_mlp_create", _int, [_vp, _u32, _u32, _u32, _u64, ctypes.POINTER(_vp)]),
+    ("rdb_mlp_free", None, [_vp]),
+    ("rdb_mlp_parameter_count", _int, [_vp, _vp]),
+    ("rdb_mlp_get_parameters", _int, [_vp, _vp, _u64]),
+    ("rdb_mlp_set_parameters", _int, [_vp, _vp, _u64]),
+    ("rdb_mlp_forward", _int, [_vp, _vp, _u32, _vp]),
+    ("rdb_mlp_gradients", _int, [_vp, _vp, _vp, _u32, _vp, _vp]),
+    ("rdb_mlp_train", _int, [_vp, _vp, _vp, _u64, _u32, _u32, ctypes.c_float, _u64, _vp]),
+    ("rdb_plan_samples", _int, [_vp, _vp, _u32, _vp, _vp, _vp, _u64, _vp]),
+    ("rdb_plan_estimates", _int, [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _u64, _vp]),
+    ("rdb_set_planner", _int, [_vp, _int, _vp]),
     ("rdb_set_tuning", _int, [_vp, _cp, ctypes.c_int64]),
     ("rdb_get_tuning", _int, [_vp, _cp, _vp]),
     ("rdb_diff_sequence", _int, [_int, _cp, _u32, _cp, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _u64, _vp]),
@@ -138,6 +152,12 @@ class Engine:
         return {"mutations_ms": mm.value, "mutations_launches": mn.value, "filter_ms": fm.value,
                 "filter_launches": fn.value}
 
+    def device_memory(self) -> tuple:
+        """(free, total) bytes of the GPU."""
+        f, t = _u64(), _u64()
+        _check(lib().rdb_device_memory(self._h, ctypes.byref(f), ctypes.byref(t)), self._h)
+        return f.value, t.value
+
     def set_tuning(self, key: str, value: int) -> None:
         """Kernel knobs: "sparse_max_rows" (-1 = dense only), "mutation_groups" (0 = automatic)."""
         _check(lib().rdb_set_tuning(self._h, key.encode(), int(value)), self._h)
@@ -146,6 +166,14 @@ class Engine:
         v = ctypes.c_int64()
         _check(lib().rdb_get_tuning(self._h, key.encode(), ctypes.byref(v)), self._h)
         return v.value
+
+    def mlp(self, n_in: int = 32, n_hidden: int = 128, n_hidden_layers: int = 3, seed: int = 0) -> "Mlp":
+        return Mlp(self, n_in, n_hidden, n_hidden_layers, seed)
+
+    def set_planner(self, mode: str, mlp: "Mlp | None" = None) -> None:
+        """mode: "reference" (default), "independence" or "learned" (with the estimator `mlp`)."""
+        m = {"reference": 0, "independence": 1, "learned": 2}[mode]
+        _check(lib().rdb_set_planner(self._h, m, None if mlp is None else mlp.handle), self._h)
 
     def close(self) -> None:
         for t in self._tables:
@@ -350,6 +378,11 @@ class DeviceTable:
             return out
         return self._with_bitmap(filt, run)
 
+    def lane_bytes(self, max_queries: int, row_capacity: int) -> int:
+        n = _u64()
+        _check(lib().rdb_batch_lane_bytes(self._h, max_queries, row_capacity, ctypes.byref(n)), self._ctx())
+        return n.value
+
     # ---- batched path --------------------------------------------------------------------------
     def batch_submit(self, queries: list) -> "Batch":
         """Submits dicts {"filter", "action": "count"|"mutations"|"aggregate", "column",
@@ -382,6 +415,12 @@ class Batch:
 
     def wait(self) -> None:
         _check(lib().rdb_batch_wait(self._h), self.table._ctx())
+
+    def poll(self) -> bool:
+        """True once every job of the batch has finished (collects finished jobs, never waits)."""
+        d = ctypes.c_int()
+        _check(lib().rdb_batch_poll(self._h, ctypes.byref(d)), self.table._ctx())
+        return bool(d.value)
 
     def count(self, q: int) -> int:
         n = _u64()
@@ -434,3 +473,99 @@ class Batch:
 
     def __exit__(self, *exc):
         self.free()
+
+
+class Mlp:
+    """The planner's cardinality estimator (BF16 parameters, FP32 accumulation, HIP kernels)."""
+
+    def __init__(self, engine: Engine, n_in: int, n_hidden: int, n_hidden_layers: int, seed: int):
+        self.engine = engine
+        self.n_in = n_in
+        h = _vp()
+        _check(lib().rdb_mlp_create(engine.handle, n_in, n_hidden, n_hidden_layers, seed, ctypes.byref(h)), engine.handle)
+        self._h = h
+        n = _u64()
+        _check(lib().rdb_mlp_parameter_count(h, ctypes.byref(n)), engine.handle)
+        self.n_params = n.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def parameters(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        _check(lib().rdb_mlp_get_parameters(self._h, _p(p), self.n_params), self.engine.handle)
+        return p
+
+    def set_parameters(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().rdb_mlp_set_parameters(self._h, _p(p), len(p)), self.engine.handle)
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(len(x), np.float32)
+        _check(lib().rdb_mlp_forward(self._h, _p(x), len(x), _p(y)), self.engine.handle)
+        return y
+
+    def gradients(self, x: np.ndarray, y: np.ndarray):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().rdb_mlp_gradients(self._h, _p(x), _p(y), len(x), ctypes.byref(loss), _p(g)), self.engine.handle)
+        return loss.value, g
+
+    def train(self, x: np.ndarray, y: np.ndarray, epochs: int, batch: int = 4096, lr: float = 1e-3, seed: int = 0) -> float:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(lib().rdb_mlp_train(self._h, _p(x), _p(y), len(x), epochs, batch, ctypes.c_float(lr), seed,
+                                   ctypes.byref(loss)), self.engine.handle)
+        return loss.value
+
+    def free(self) -> None:
+        if self._h:
+            lib().rdb_mlp_free(self._h)
+            self._h = None
+
+
+def _cstrings(items):
+    arr = (ctypes.c_char_p * max(len(items), 1))()
+    enc = [s.encode() for s in items]
+    for i, s in enumerate(enc):
+        arr[i] = s
+    return arr, enc
+
+
+def plan_samples(table: DeviceTable, filters: list):
+    """Training rows of the planner: (features [n, 32], targets [n], tree index [n])."""
+    arr, keep = _cstrings(filters)
+    n = _u64()
+    rc = lib().rdb_plan_samples(table.handle, arr, len(filters), None, None, None, 0, ctypes.byref(n))
+    if rc not in (RDB_OK, 2):
+        _check(rc, table._ctx())
+    cap = n.value
+    f = np.zeros((cap, 32), np.float32)
+    y = np.zeros(cap, np.float32)
+    tr = np.zeros(cap, np.uint32)
+    _check(lib().rdb_plan_samples(table.handle, arr, len(filters), _p(f), _p(y), _p(tr), cap, ctypes.byref(n)),
+           table._ctx())
+    return f, y, tr
+
+
+def plan_estimates(table: DeviceTable, filters: list, mlp: "Mlp | None" = None, exact: bool = True):
+    """Per node of every filter: learned estimate, independence estimate, exact rows, tree index."""
+    arr, keep = _cstrings(filters)
+    n = _u64()
+    rc = lib().rdb_plan_estimates(table.handle, None if mlp is None else mlp.handle, arr, len(filters), None, None,
+                                  None, None, 0, ctypes.byref(n))
+    if rc not in (RDB_OK, 2):
+        _check(rc, table._ctx())
+    cap = n.value
+    est = np.zeros(cap, np.float32)
+    ind = np.zeros(cap, np.float32)
+    ex = np.zeros(cap, np.uint64)
+    tr = np.zeros(cap, np.uint32)
+    _check(lib().rdb_plan_estimates(table.handle, None if mlp is None else mlp.handle, arr, len(filters), _p(est), _p(ind),
+                                    _p(ex) if exact else None, _p(tr), cap, ctypes.byref(n)), table._ctx())
+    return est, ind, (ex if exact else None), tr

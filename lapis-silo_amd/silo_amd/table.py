@@ -82,11 +82,14 @@ class TableSpec:
         return np.concatenate(out) if out else np.zeros(0, dtype=np.uint32)
 
 
-def spec_from_dataset(ds: Dataset, chunk: int = CHUNK, seq_name: str = "main") -> TableSpec:
+def spec_from_dataset(ds: Dataset, chunk: int = CHUNK, seq_name: str = "main", aa_columns: list | None = None) -> TableSpec:
+    """The dataset as ingest columns; `aa_columns` (datagen.translate) adds amino-acid columns."""
     spec = TableSpec(chunk_sizes=chunk_sizes(ds.n_rows, chunk))
     spec.seq_columns.append(SeqColumnData(
         seq_name, ALPHABET_NUC, ds.reference, ds.is_null, ds.cov_start, ds.cov_end, ds.mut_ptr, ds.mut_pos,
         ds.mut_sym, ds.nrun_ptr, ds.nrun_start, ds.nrun_end))
+    for (name, ref, cs, ce, mptr, mpos, msym, nptr, ns, ne) in aa_columns or []:
+        spec.seq_columns.append(SeqColumnData(name, ALPHABET_AA, ref, ds.is_null, cs, ce, mptr, mpos, msym, nptr, ns, ne))
     spec.int_columns.append(IntColumnData("date", True, ds.date, ds.date_null))
     spec.dict_columns.append(DictColumnData("country", ds.country, ds.country_null, list(ds.country_names)))
     spec.dict_columns.append(DictColumnData("lineage", ds.lineage, ds.lineage_null, list(ds.lineage_names),
