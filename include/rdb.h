@@ -224,11 +224,16 @@ int rdb_set_planner(rdb_ctx* ctx, int mode, rdb_mlp* mlp);
 /* No reference counterpart: knobs of the engine's own kernels, set by hand or by the autotuner.
  *   "sparse_max_rows"  Mutations under a filter of at most this many rows walk the rows' own
  *                      mutations instead of streaming the index (-1, the default: never);
- *   "mutation_groups"  chunk groups (partial sums) of the dense Mutations kernel, 0 = automatic;
+ *   "mutation_groups"  chunk groups (partial sums) of the dense Mutations kernel, 0 = automatic
+ *                      (at most 32);
+ *   "mutation_packed"  1: the dense Mutations kernel keeps two 16-bit slot counters per LDS word
+ *                      (default; used when a group holds at most 15 chunks), 0: one 32-bit;
  *   "mutation_unroll"  16-byte loads in flight per lane in the dense Mutations kernel, 8 or 16;
  *   "batch_max_queries" queries per lane job of the batched path (1..4096, default 256);
  *   "batch_lanes"      lanes (HIP streams with their own buffers) per table (1..8, default 2);
  *   "batch_graphs"     1: replay each lane's launch sequence from a captured HIP graph (default);
+ *   "batch_shared_scan" 1: a job's dense Mutations queries share passes over the index, four per
+ *                      pass; 0: one pass per query (default);
  *   "batch_row_capacity" Mutations rows per query kept by a lane job (more: the query runs again
  *                      alone), default 4096.
  * Results never depend on them.  RDB_EINVAL for an unknown key or an out-of-range value. */

@@ -31,8 +31,8 @@ struct BatchLane {
    uint32_t hist_capacity = 0;
    size_t params_bytes = 0;
    size_t out_bytes = 0;
-   DeviceBuffer params, filters, cards, lists, list_counts, partial, sparse_counts, sparse_bnd, totals, rows, n_rows, hist,
-      out;
+   DeviceBuffer params, filters, cards, lists, list_counts, partial, heavy_sums, sparse_counts, sparse_bnd, totals, rows,
+      n_rows, hist, out;
    void* host_params = nullptr;
    void* host_out = nullptr;
    std::map<uint64_t, hipGraphExec_t> graphs;  // by (depth bucket, Mutations column, sparse switch, unroll)

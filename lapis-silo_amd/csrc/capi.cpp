@@ -102,11 +102,12 @@ const DeviceBitmap& filterOf(const Table& t, const rdb_bitmap* f) {
 }
 
 // Chunk groups of the Mutations kernel: a group per four chunks (one chunk per wavefront of the
-// work-group), at most kMaxGroups partial sums.
+// work-group), at most kMaxGroups partial sums, and at least enough groups for the packed counters
+// (at most 15 chunks per group).
 uint32_t mutationGroups(const Table& t, const SeqColumn& /*col*/) {
    if (t.ctx->mutation_groups) return std::clamp<uint32_t>(t.ctx->mutation_groups, 1u, kMaxGroups);
-   const uint32_t g = (t.n_chunks + 3) / 4;
-   return std::clamp<uint32_t>(g, 1u, kMaxGroups);
+   const uint32_t at_least = std::max<uint32_t>(1, (t.n_chunks + 14) / 15);
+   return std::clamp<uint32_t>((t.n_chunks + 3) / 4, at_least, kMaxGroups);
 }
 
 // Launch description of the Mutations kernels for column `idx` under filter `f`, with the
@@ -125,6 +126,7 @@ MutationsLaunch mutationsLaunch(Table& t, int idx, const DeviceBitmap& f) {
    m.filter_card = f.card_dev;
    m.sparse_max = t.ctx->sparse_max_rows;
    m.partial = t.slot_counts.as<uint32_t>();
+   m.heavy_sums = t.ctx->mutation_packed ? t.heavy_sums.as<uint32_t>() : nullptr;
    m.sparse_counts = col.sparse_counts.as<uint32_t>();
    m.sparse_bnd = col.sparse_bnd.as<int32_t>();
    m.totals = t.totals.as<int64_t>();
@@ -200,11 +202,13 @@ int rdb_set_tuning(rdb_ctx* ctx, const char* key, int64_t value) {
       if (k == "sparse_max_rows") {
          ctx->c.sparse_max_rows = value < 0 ? -1 : static_cast<long long>(value);
       } else if (k == "mutation_groups") {
-         if (value < 0 || value > kMaxGroups) fail(RDB_EINVAL, "mutation_groups must be in [0, 16]");
+         if (value < 0 || value > kMaxGroups) fail(RDB_EINVAL, "mutation_groups must be in [0, 32]");
          ctx->c.mutation_groups = static_cast<uint32_t>(value);
       } else if (k == "mutation_unroll") {
          if (value != 8 && value != 16) fail(RDB_EINVAL, "mutation_unroll must be 8 or 16");
          ctx->c.mutation_unroll = static_cast<uint32_t>(value);
+      } else if (k == "mutation_packed") {
+         ctx->c.mutation_packed = value != 0;
       } else if (k == "batch_max_queries") {
          if (value < 1 || value > 4096) fail(RDB_EINVAL, "batch_max_queries must be in [1, 4096]");
          ctx->c.batch_max_queries = static_cast<uint32_t>(value);
@@ -213,6 +217,8 @@ int rdb_set_tuning(rdb_ctx* ctx, const char* key, int64_t value) {
          ctx->c.batch_lanes = static_cast<uint32_t>(value);
       } else if (k == "batch_graphs") {
          ctx->c.batch_graphs = value != 0;
+      } else if (k == "batch_shared_scan") {
+         ctx->c.batch_shared_scan = value != 0;
       } else if (k == "batch_row_capacity") {
          if (value < 16 || value > (1 << 20)) fail(RDB_EINVAL, "batch_row_capacity must be in [16, 2^20]");
          ctx->c.batch_row_capacity = static_cast<uint32_t>(value);
@@ -233,12 +239,16 @@ int rdb_get_tuning(rdb_ctx* ctx, const char* key, int64_t* value) {
          *value = ctx->c.mutation_groups;
       } else if (k == "mutation_unroll") {
          *value = ctx->c.mutation_unroll;
+      } else if (k == "mutation_packed") {
+         *value = ctx->c.mutation_packed;
       } else if (k == "batch_max_queries") {
          *value = ctx->c.batch_max_queries;
       } else if (k == "batch_lanes") {
          *value = ctx->c.batch_lanes;
       } else if (k == "batch_graphs") {
          *value = ctx->c.batch_graphs;
+      } else if (k == "batch_shared_scan") {
+         *value = ctx->c.batch_shared_scan;
       } else if (k == "batch_row_capacity") {
          *value = ctx->c.batch_row_capacity;
       } else {

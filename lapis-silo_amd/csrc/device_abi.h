@@ -45,9 +45,11 @@ constexpr uint32_t kSlicesPerChunk = kChunkWords / kSliceWords;
 constexpr uint32_t kMaxBlockSlots = 4095;   // slot 4095 is the padding slot of light entries
 constexpr uint32_t kPadSlot = 4095;
 constexpr uint32_t kLightMaxRows = 32;
-constexpr uint32_t kMaxGroups = 16;         // chunk groups of the Mutations kernel
+constexpr uint32_t kMaxGroups = 32;         // chunk groups of the Mutations kernel
 constexpr uint32_t kInvalidId = 0xFFFFFFFFu;
 constexpr uint32_t kMutPosBlock = 256;      // positions per work-group of the Mutations finalize kernels
+constexpr uint32_t kRunBucket = 256;        // positions per start bucket of the missing-run index
+constexpr uint32_t kShortRun = 1024;        // longer missing runs go to a per-chunk list of their own
 
 enum ContainerType : uint32_t { CT_INLINE = 0, CT_ARRAY = 1, CT_RUN = 2, CT_BITSET = 3 };
 
@@ -96,6 +98,14 @@ struct SeqColumnView {
    const uint32_t* cov;               // [n_chunks * 2^16] start | end << 16
    const uint32_t* nrun_ptr;          // [n_chunks * 2^16 + 1]
    const uint32_t* nruns;             // start | end << 16, end exclusive
+   // missing-run index by position: list (chunk c, bucket k, slice s) holds the runs of slice s's
+   // rows (row | start << 16 | end << 32, row within the chunk, in row order) that start in
+   // positions [k * kRunBucket, (k + 1) * kRunBucket) and are at most kShortRun long; bucket
+   // n_run_buckets holds the longer runs.  A run covering p starts in one of the buckets of
+   // [p - kShortRun + 1, p], or is long.
+   uint32_t n_run_buckets;
+   const uint32_t* run_index_ptr;     // [(c * (n_run_buckets + 1) + k) * 16 + s], and one more
+   const uint64_t* run_index;
    const uint64_t* null_bits;         // [n_chunks * 1024]
    const uint32_t* pos_ranges;        // [(chunk * L + position) * 6]: [lo, hi) of the position's light
                                       // entries, groups and descriptors in its tile

@@ -120,10 +120,12 @@ struct Context {
    long long sparse_max_rows = -1;     // filters with at most this many rows take the sparse path
    uint32_t mutation_groups = 0;       // chunk groups of the dense Mutations kernel (0 = automatic)
    uint32_t mutation_unroll = 8;       // 16-byte loads in flight per lane in the dense Mutations kernel
+   uint32_t mutation_packed = 1;       // dense Mutations kernel with 16-bit packed LDS counters
    uint32_t batch_max_queries = 256;   // queries per lane job of the batched path
    uint32_t batch_lanes = 2;           // lanes (streams with their own buffers) per table
    uint32_t batch_graphs = 1;          // capture each lane's launch sequence into a HIP graph
    uint32_t batch_row_capacity = 4096; // Mutations rows per query a lane job keeps (more: rerun alone)
+   uint32_t batch_shared_scan = 0;     // dense Mutations of a job: four queries per pass over the index
    // planner of the batched path: 0 reference order, 1 independence estimates, 2 learned estimator
    int plan_mode = 0;
    struct Mlp* planner = nullptr;
@@ -223,7 +225,7 @@ struct Table {
    TableView view{};
 
    // per-call scratch (sized at finalize)
-   DeviceBuffer slot_counts, totals, mut_rows, mut_n, labels_scratch[4], hist;
+   DeviceBuffer slot_counts, heavy_sums, totals, mut_rows, mut_n, labels_scratch[4], hist;
    size_t hist_bins_capacity = 0;
    // lanes of the batched path (created on demand)
    std::vector<std::shared_ptr<BatchLane>> lanes;

@@ -53,6 +53,7 @@ struct MutationsLaunch {
    const unsigned long long* filter_card = nullptr;
    long long sparse_max = -1;
    uint32_t* partial = nullptr;
+   uint32_t* heavy_sums = nullptr;         // [n_slots], zero; set: 16-bit packed LDS counters (more CU occupancy)
    uint32_t* sparse_counts = nullptr;
    int32_t* sparse_bnd = nullptr;
    int64_t* totals = nullptr;
@@ -92,6 +93,7 @@ struct MutationsBatch {
    uint32_t* lists = nullptr;                  // [2 * max_queries]: dense ids, then sparse ids
    uint32_t* list_counts = nullptr;            // [2]
    uint32_t* partial = nullptr;                // [max_queries * n_slots]
+   uint32_t* heavy_sums = nullptr;             // [max_queries * n_slots], zero: set = shared-scan kernel
    uint32_t* sparse_counts = nullptr;          // [max_queries * n_symbols * L], zero
    int32_t* sparse_bnd = nullptr;              // [max_queries * 3 * L], zero
    int64_t* totals = nullptr;                  // [2 * max_queries * position blocks * 3]

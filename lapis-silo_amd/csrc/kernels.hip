@@ -275,44 +275,74 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                break;
             }
             case OP_COVERED: {
-               // each lane builds its own word from its 64 consecutive rows
+               // rows whose covered range holds p (each lane its 64 consecutive rows), minus the rows
+               // with a missing run over p, found in the missing-run index: the runs of this slice's
+               // rows that start in the buckets of [p - kShortRun + 1, p], and the long ones
                const SeqColumnView& col = tv.seq[ins.a];
                const uint32_t p = ins.b;
                const size_t r0 = row_base + static_cast<size_t>(lane) * 64;
                const uint4* cv = reinterpret_cast<const uint4*>(col.cov + r0);
-               const uint4* pv = reinterpret_cast<const uint4*>(col.nrun_ptr + r0);
+               uint4 c4[16];
+#pragma unroll
+               for (uint32_t u = 0; u < 16; ++u) c4[u] = cv[u];
+               // candidate lists: lane j < n_lists loads list j's bounds
+               const uint32_t nb = col.n_run_buckets;
+               const uint32_t k_hi = p / kRunBucket;
+               const uint32_t k_lo = p + 1 >= kShortRun ? (p + 1 - kShortRun) / kRunBucket : 0;
+               const uint32_t n_lists = k_hi - k_lo + 2;  // the buckets and the long runs
+               uint32_t my_lo = 0, my_len = 0;
+               if (lane < n_lists) {
+                  const uint32_t k = lane + 1 < n_lists ? k_lo + lane : nb;
+                  const uint32_t* rp = col.run_index_ptr + (static_cast<size_t>(chunk) * (nb + 1) + k) * kSlicesPerChunk + slice;
+                  my_lo = rp[0];
+                  my_len = rp[1] - rp[0];
+               }
+               scratch[lane] = 0;
                uint64_t word = 0;
 #pragma unroll
-               for (uint32_t h = 0; h < 2; ++h) {
-                  uint4 c4[8], p4[8];
+               for (uint32_t u = 0; u < 16; ++u) {
+                  const uint32_t cs[4] = {c4[u].x, c4[u].y, c4[u].z, c4[u].w};
 #pragma unroll
-                  for (uint32_t u = 0; u < 8; ++u) {
-                     c4[u] = cv[8 * h + u];
-                     p4[u] = pv[8 * h + u];
+                  for (uint32_t q = 0; q < 4; ++q) {
+                     if ((cs[q] & 0xFFFFu) <= p && p < (cs[q] >> 16)) word |= 1ULL << (4 * u + q);
                   }
-                  const uint32_t last = col.nrun_ptr[r0 + 32 * h + 32];
+               }
+               // flat index over the lists' entries
+               uint32_t lo[8], start[9];
+               start[0] = 0;
 #pragma unroll
-                  for (uint32_t u = 0; u < 8; ++u) {
-                     const uint32_t cs[4] = {c4[u].x, c4[u].y, c4[u].z, c4[u].w};
-                     const uint32_t ps[5] = {p4[u].x, p4[u].y, p4[u].z, p4[u].w,
-                                             u + 1 < 8 ? p4[u + 1 < 8 ? u + 1 : 7].x : last};
+               for (uint32_t j = 0; j < 8; ++j) {
+                  lo[j] = __shfl(my_lo, j, 64);
+                  const uint32_t len = j < n_lists ? __shfl(my_len, j, 64) : 0u;
+                  start[j + 1] = start[j] + len;
+               }
+               const uint32_t total = start[8];
+               waveSync();
+               for (uint32_t base = 0; base < total; base += 64 * 4) {
+                  uint64_t e[4];
 #pragma unroll
-                     for (uint32_t q = 0; q < 4; ++q) {
-                        bool covered = (cs[q] & 0xFFFFu) <= p && p < (cs[q] >> 16);
-                        if (covered) {
-                           for (uint32_t j = ps[q]; j < ps[q + 1]; ++j) {
-                              const uint32_t x = col.nruns[j];
-                              if ((x & 0xFFFFu) <= p && p < (x >> 16)) {
-                                 covered = false;
-                                 break;
-                              }
-                           }
-                        }
-                        if (covered) word |= 1ULL << (32 * h + 4 * u + q);
+                  for (uint32_t u = 0; u < 4; ++u) {
+                     const uint32_t idx = base + u * 64 + lane;
+                     e[u] = 0;  // row 0, an empty run [0, 0)
+                     if (idx < total) {
+                        uint32_t j = 0;
+#pragma unroll
+                        for (uint32_t t = 1; t < 8; ++t) j += idx >= start[t] ? 1u : 0u;
+                        e[u] = col.run_index[lo[j] + (idx - start[j])];
+                     }
+                  }
+#pragma unroll
+                  for (uint32_t u = 0; u < 4; ++u) {
+                     const uint32_t a = static_cast<uint32_t>(e[u] >> 16) & 0xFFFFu;
+                     const uint32_t b = static_cast<uint32_t>(e[u] >> 32) & 0xFFFFu;
+                     if (a <= p && p < b) {
+                        const uint32_t row = static_cast<uint32_t>(e[u]) & 0xFFFu;  // within the slice
+                        atomicOr(reinterpret_cast<unsigned long long*>(&scratch[row >> 6]), 1ULL << (row & 63));
                      }
                   }
                }
-               push(word & lmask);
+               waveSync();
+               push(word & ~scratch[lane] & lmask);
                break;
             }
             case OP_CMP: {
@@ -471,18 +501,34 @@ __device__ inline uint32_t rangePopcount(const uint64_t* f, uint32_t a, uint32_t
 
 // Per-slot AND-popcounts of position block b over the chunks {g + k * n_groups} against one filter;
 // writes the block's slot sums to out[0 .. slots of b).  Called by a whole work-group.
-template <uint32_t kUnroll>
+//
+// kPacked: two slots share one 32-bit LDS counter (16 bits each), halving the counter array so
+// that four work-groups fit a CU.  A field gathers light entries and eight-row groups only (at
+// most 4096 rows per slot and chunk), over at most 15 chunks per work-group (the caller's chunk
+// grouping guarantees it), so it stays below 2^16; bitset and run containers, which may hold all
+// 65,536 rows of a chunk, add to the global `heavy` sums instead.
+constexpr uint32_t kPackedMaxChunks = 15;
+
+template <uint32_t kUnroll, bool kPacked>
 __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b, uint32_t g, uint32_t n_chunks,
                                            uint32_t n_groups, const uint64_t* __restrict__ filter,
                                            const uint32_t* __restrict__ filter_chunk_cards, uint32_t* __restrict__ out,
-                                           uint64_t (*fs)[kChunkWords], uint32_t* cnt) {
+                                           uint64_t (*fs)[kChunkWords], uint32_t* cnt, uint32_t* __restrict__ heavy) {
    const uint32_t tid = threadIdx.x;
    const uint32_t lane = tid & 63;
    const uint32_t wave = tid >> 6;
    const uint32_t s0 = col.block_slot_start[b];
    const uint32_t ns = col.block_slot_start[b + 1] - s0;
-   for (uint32_t i = tid; i <= kMaxBlockSlots; i += kMutThreads) cnt[i] = 0;
+   constexpr uint32_t kWords = kPacked ? (kMaxBlockSlots + 1) / 2 : kMaxBlockSlots + 1;
+   for (uint32_t i = tid; i < kWords; i += kMutThreads) cnt[i] = 0;
    __syncthreads();
+   auto addCount = [&](uint32_t slot, uint32_t v) {
+      if (kPacked) {
+         atomicAdd(&cnt[slot >> 1], v << ((slot & 1u) << 4));
+      } else {
+         atomicAdd(&cnt[slot], v);
+      }
+   };
    uint64_t* f = fs[wave];
    const Desc* __restrict__ desc = col.vert.desc;
    // each wavefront takes its own chunks of the group; no block-wide barrier until the end
@@ -518,7 +564,7 @@ __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b,
             const uint32_t slot = x >> 16;
             const uint32_t row = x & 0xFFFFu;
             if (slot != cur) {
-               if (acc) atomicAdd(&cnt[cur], acc);
+               if (acc) addCount(cur, acc);
                cur = slot;
                acc = 0;
             }
@@ -538,7 +584,7 @@ __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b,
                take(e[u].w);
             }
          }
-         if (acc) atomicAdd(&cnt[cur], acc);
+         if (acc) addCount(cur, acc);
       }
       // larger arrays: groups of eight rows sharing one slot; lanes walk contiguous spans of groups
       {
@@ -566,14 +612,14 @@ __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b,
 #pragma unroll
                for (uint32_t q = 0; q < 8; ++q) n += q < m ? bitAt(f, r[q]) : 0u;
                if (slot != cur) {
-                  if (acc) atomicAdd(&cnt[cur], acc);
+                  if (acc) addCount(cur, acc);
                   cur = slot;
                   acc = 0;
                }
                acc += n;
             }
          }
-         if (acc) atomicAdd(&cnt[cur], acc);
+         if (acc) addCount(cur, acc);
       }
       // bitset and run containers: one per wavefront
       for (uint32_t d = h0; d < h1; ++d) {
@@ -612,29 +658,38 @@ __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b,
             for (uint32_t j = 0; j < card; ++j) n += bitAt(f, (dd.w1 >> (16 * j)) & 0xFFFFu);
          }
          n = waveSum(n);
-         if (lane == 0 && n) atomicAdd(&cnt[descSlot(dd.w0)], n);
+         if (lane == 0 && n) {
+            if (kPacked) {
+               atomicAdd(&heavy[s0 + descSlot(dd.w0)], n);
+            } else {
+               atomicAdd(&cnt[descSlot(dd.w0)], n);
+            }
+         }
       }
       waveSync();
    }
    __syncthreads();
-   for (uint32_t i = tid; i < ns; i += kMutThreads) out[s0 + i] = cnt[i];
+   for (uint32_t i = tid; i < ns; i += kMutThreads) {
+      out[s0 + i] = kPacked ? (cnt[i >> 1] >> ((i & 1u) << 4)) & 0xFFFFu : cnt[i];
+   }
 }
 
 
-template <uint32_t kUnroll>
+template <uint32_t kUnroll, bool kPacked>
 __global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqColumnView* __restrict__ colp,
                                                                     uint32_t n_chunks, uint32_t n_groups,
                                                                     const uint64_t* __restrict__ filter,
                                                                     const uint32_t* __restrict__ filter_chunk_cards,
                                                                     const unsigned long long* __restrict__ filter_card,
                                                                     long long sparse_max,
-                                                                    uint32_t* __restrict__ partial) {
+                                                                    uint32_t* __restrict__ partial,
+                                                                    uint32_t* __restrict__ heavy) {
    __shared__ uint64_t fs[kMutWaves][kChunkWords];
-   __shared__ uint32_t cnt[kMaxBlockSlots + 1];
+   __shared__ uint32_t cnt[kPacked ? (kMaxBlockSlots + 1) / 2 : kMaxBlockSlots + 1];
    if (!denseActive(filter_card, sparse_max)) return;
    const SeqColumnView& col = *colp;
-   countBlock<kUnroll>(col, blockIdx.x, blockIdx.y, n_chunks, n_groups, filter, filter_chunk_cards,
-              partial + static_cast<size_t>(blockIdx.y) * col.n_slots, fs, cnt);
+   countBlock<kUnroll, kPacked>(col, blockIdx.x, blockIdx.y, n_chunks, n_groups, filter, filter_chunk_cards,
+                                partial + static_cast<size_t>(blockIdx.y) * col.n_slots, fs, cnt, heavy);
 }
 
 __device__ inline uint32_t slotValue(const uint32_t* partial, uint32_t n_groups, uint32_t n_slots, uint32_t s) {
@@ -691,12 +746,16 @@ __global__ __launch_bounds__(256) void sparseCountKernel(const SeqColumnView* __
 __global__ __launch_bounds__(256) void reducePartialsKernel(uint32_t* __restrict__ partial, uint32_t n_groups,
                                                             uint32_t n_slots,
                                                             const unsigned long long* __restrict__ filter_card,
-                                                            long long sparse_max) {
+                                                            long long sparse_max, uint32_t* __restrict__ heavy) {
    if (!denseActive(filter_card, sparse_max)) return;
    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
    if (s >= n_slots) return;
    uint32_t v = partial[s];
    for (uint32_t g = 1; g < n_groups; ++g) v += partial[static_cast<size_t>(g) * n_slots + s];
+   if (heavy) {
+      v += heavy[s];
+      heavy[s] = 0;
+   }
    partial[s] = v;
 }
 
@@ -972,9 +1031,231 @@ __global__ __launch_bounds__(kMutThreads) void mutationsCountBatchKernel(Mutatio
    if (b >= col.n_blocks) return;
    const uint32_t e = i % n_dense;
    const uint32_t q = m.lists[e];
-   countBlock<kUnroll>(col, b, 0, m.n_chunks, 1, m.filters + static_cast<size_t>(q) * m.n_chunks * kChunkWords,
+   countBlock<kUnroll, false>(col, b, 0, m.n_chunks, 1, m.filters + static_cast<size_t>(q) * m.n_chunks * kChunkWords,
                        m.chunk_cards + static_cast<size_t>(q) * m.n_chunks, m.partial + static_cast<size_t>(e) * col.n_slots,
-                       fs, cnt);
+                              fs, cnt, nullptr);
+}
+
+// ---- shared scan: four dense queries per pass over the index --------------------------------------
+// Work-group (block b, quad j) streams block b's tiles of every chunk once for the dense list
+// entries 4j .. 4j+3.  Per chunk, the four filters' words are interleaved into a nibble per row
+// (bit q: the row is in entry 4j+q's filter), staged in LDS (32 KiB per chunk); each light entry
+// or group row then costs one LDS byte read, and one 64-bit counter per slot holds the four
+// queries' counts as 16-bit fields (added with one LDS atomic per run of a slot).  A field
+// gathers at most 15 chunks x 4096 rows before the work-group adds it to the query's slot sums
+// (which it owns), so it cannot overflow.  Bitset and run containers (a few thousand) are counted
+// per query from the filters' words and added with global atomics to a second sum array.
+constexpr uint32_t kQuadThreads = 512;
+constexpr uint32_t kQuadFlushChunks = 15;
+
+// bit i of the low 16 bits of x moves to bit 4 i
+__device__ inline uint64_t spread4(uint64_t x) {
+   x &= 0xFFFFull;
+   x = (x | (x << 24)) & 0x000000FF000000FFull;
+   x = (x | (x << 12)) & 0x000F000F000F000Full;
+   x = (x | (x << 6)) & 0x0303030303030303ull;
+   x = (x | (x << 3)) & 0x1111111111111111ull;
+   return x;
+}
+
+// a row's nibble as four 16-bit counters
+__device__ inline uint64_t nibbleCounts(uint32_t n) {
+   return static_cast<uint64_t>(n & 1u) | (static_cast<uint64_t>(n & 2u) << 15) | (static_cast<uint64_t>(n & 4u) << 30) |
+          (static_cast<uint64_t>(n & 8u) << 45);
+}
+
+__device__ inline uint32_t rowNibble(const uint8_t* mask, uint32_t row) {
+   return (static_cast<uint32_t>(mask[row >> 1]) >> ((row & 1u) << 2)) & 0xFu;
+}
+
+template <uint32_t kUnroll>
+__global__ __launch_bounds__(kQuadThreads) void mutationsCountQuadKernel(MutationsBatch m, uint32_t* heavy_sums) {
+   __shared__ __align__(16) uint8_t mask[kChunkRows / 2];
+   __shared__ uint64_t cnt[kMaxBlockSlots + 1];
+   const uint32_t n_dense = m.list_counts[0];
+   const SeqColumnView& col = *m.col_dev;
+   const uint32_t b = blockIdx.x % col.n_blocks;
+   const uint32_t j = blockIdx.x / col.n_blocks;
+   if (4 * j >= n_dense) return;
+   const uint32_t tid = threadIdx.x;
+   const uint32_t lane = tid & 63;
+   const uint32_t n_chunks = m.n_chunks;
+   const uint32_t s0 = col.block_slot_start[b];
+   const uint32_t ns = col.block_slot_start[b + 1] - s0;
+   uint32_t q[4];
+   bool valid[4];
+#pragma unroll
+   for (uint32_t k = 0; k < 4; ++k) {
+      valid[k] = 4 * j + k < n_dense;
+      q[k] = valid[k] ? m.lists[4 * j + k] : 0;
+   }
+   for (uint32_t i = tid; i <= kMaxBlockSlots; i += kQuadThreads) cnt[i] = 0;
+   uint32_t since_flush = 0;
+   bool first_flush = true;
+   auto flush = [&]() {
+      __syncthreads();
+      for (uint32_t i = tid; i < ns; i += kQuadThreads) {
+         const uint64_t v = cnt[i];
+         cnt[i] = 0;
+#pragma unroll
+         for (uint32_t k = 0; k < 4; ++k) {
+            if (!valid[k]) continue;
+            uint32_t* out = m.partial + static_cast<size_t>(4 * j + k) * col.n_slots + s0 + i;
+            const uint32_t add = static_cast<uint32_t>((v >> (16 * k)) & 0xFFFFu);
+            *out = first_flush ? add : *out + add;
+         }
+      }
+      first_flush = false;
+      since_flush = 0;
+      __syncthreads();
+   };
+   for (uint32_t c = 0; c < n_chunks; ++c) {
+      bool any = false;
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) any |= valid[k] && m.chunk_cards[static_cast<size_t>(q[k]) * n_chunks + c] != 0;
+      if (!any) continue;
+      const uint32_t tile = b * n_chunks + c;
+      const uint32_t l0 = col.light_ptr[tile], l1 = col.light_ptr[tile + 1];
+      const uint32_t h0 = col.heavy_ptr[tile], h1 = col.heavy_ptr[tile + 1];
+      const uint32_t g0 = col.group_ptr[tile], g1 = col.group_ptr[tile + 1];
+      if (l0 == l1 && h0 == h1 && g0 == g1) continue;
+      if (since_flush == kQuadFlushChunks) flush();
+      ++since_flush;
+      __syncthreads();  // the previous chunk's readers of the mask are done
+      // stage: 1024 words x 4 filters -> 64 nibbles (32 bytes) per word
+      for (uint32_t w = tid; w < kChunkWords; w += kQuadThreads) {
+         uint64_t x[4];
+#pragma unroll
+         for (uint32_t k = 0; k < 4; ++k) {
+            x[k] = valid[k] ? m.filters[(static_cast<size_t>(q[k]) * n_chunks + c) * kChunkWords + w] : 0ull;
+         }
+         uint64_t out[4];
+#pragma unroll
+         for (uint32_t h = 0; h < 4; ++h) {
+            out[h] = spread4(x[0] >> (16 * h)) | (spread4(x[1] >> (16 * h)) << 1) | (spread4(x[2] >> (16 * h)) << 2) |
+                     (spread4(x[3] >> (16 * h)) << 3);
+         }
+         uint4* dst = reinterpret_cast<uint4*>(mask + static_cast<size_t>(w) * 32);
+         dst[0] = make_uint4(static_cast<uint32_t>(out[0]), static_cast<uint32_t>(out[0] >> 32), static_cast<uint32_t>(out[1]),
+                             static_cast<uint32_t>(out[1] >> 32));
+         dst[1] = make_uint4(static_cast<uint32_t>(out[2]), static_cast<uint32_t>(out[2] >> 32), static_cast<uint32_t>(out[3]),
+                             static_cast<uint32_t>(out[3] >> 32));
+      }
+      __syncthreads();
+      // light entries: contiguous spans per thread, runs of a slot summed before one LDS atomic
+      {
+         const uint4* e4 = reinterpret_cast<const uint4*>(col.light + l0);
+         const uint32_t n4 = (l1 - l0) >> 2;
+         uint32_t cur = kPadSlot;
+         uint64_t acc = 0;
+         auto take = [&](uint32_t x) {
+            const uint32_t slot = x >> 16;
+            if (slot != cur) {
+               if (acc) atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[cur]), static_cast<unsigned long long>(acc));
+               cur = slot;
+               acc = 0;
+            }
+            acc += nibbleCounts(rowNibble(mask, x & 0xFFFFu));
+         };
+         for (uint32_t base = tid * kUnroll; base < n4; base += kQuadThreads * kUnroll) {
+            uint4 e[kUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+               e[u] = base + u < n4 ? e4[base + u] : make_uint4(kPadSlot << 16, kPadSlot << 16, kPadSlot << 16, kPadSlot << 16);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+               take(e[u].x);
+               take(e[u].y);
+               take(e[u].z);
+               take(e[u].w);
+            }
+         }
+         // slot 4095 is the padding slot: its counter is never read
+         if (acc) atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[cur]), static_cast<unsigned long long>(acc));
+      }
+      // groups of eight rows of one slot
+      {
+         const uint4* gr = reinterpret_cast<const uint4*>(col.groups) + g0;
+         const uint16_t* gh = col.group_hdr + g0;
+         const uint32_t ng = g1 - g0;
+         uint32_t cur = kPadSlot;
+         uint64_t acc = 0;
+         for (uint32_t base = tid * kUnroll; base < ng; base += kQuadThreads * kUnroll) {
+            uint4 rows[kUnroll];
+            uint32_t hdr[kUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+               const bool ok = base + u < ng;
+               rows[u] = ok ? gr[base + u] : make_uint4(0, 0, 0, 0);
+               hdr[u] = ok ? gh[base + u] : 0xFFFFu;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+               if (hdr[u] == 0xFFFFu) continue;
+               const uint32_t slot = hdr[u] & 0xFFFu;
+               const uint32_t mm = (hdr[u] >> 12) + 1;
+               const uint32_t r[8] = {rows[u].x & 0xFFFFu, rows[u].x >> 16, rows[u].y & 0xFFFFu, rows[u].y >> 16,
+                                      rows[u].z & 0xFFFFu, rows[u].z >> 16, rows[u].w & 0xFFFFu, rows[u].w >> 16};
+               uint64_t add = 0;
+#pragma unroll
+               for (uint32_t t = 0; t < 8; ++t) add += t < mm ? nibbleCounts(rowNibble(mask, r[t])) : 0ull;
+               if (slot != cur) {
+                  if (acc) atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[cur]), static_cast<unsigned long long>(acc));
+                  cur = slot;
+                  acc = 0;
+               }
+               acc += add;
+            }
+         }
+         if (acc) atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[cur]), static_cast<unsigned long long>(acc));
+      }
+      // bitset and run containers: one per wavefront, per query from the filter's words
+      for (uint32_t d = h0 + (tid >> 6); d < h1; d += kQuadThreads / 64) {
+         const Desc dd = col.vert.desc[d];
+         const uint32_t type = descType(dd.w0);
+         const uint32_t card = descCard(dd.w0);
+         const uint32_t slot = descSlot(dd.w0);
+#pragma unroll
+         for (uint32_t k = 0; k < 4; ++k) {
+            if (!valid[k]) continue;
+            const uint64_t* f = m.filters + (static_cast<size_t>(q[k]) * n_chunks + c) * kChunkWords;
+            uint32_t n = 0;
+            if (type == CT_BITSET) {
+               const uint64_t* bits = col.vert.bits + static_cast<size_t>(dd.w1) * kChunkWords;
+               for (uint32_t w = lane; w < kChunkWords; w += 64) n += __popcll(bits[w] & f[w]);
+            } else if (type == CT_RUN) {
+               const uint32_t* rr = col.vert.runs + dd.w1;
+               for (uint32_t t = lane; t < card; t += 64) {
+                  const uint32_t st = rr[t] & 0xFFFFu;
+                  n += rangePopcount(f, st, st + (rr[t] >> 16));
+               }
+            } else if (type == CT_ARRAY) {
+               const uint16_t* a = col.vert.arr + dd.w1;
+               for (uint32_t t = lane; t < card; t += 64) n += bitAt(f, a[t]);
+            } else if (lane == 0) {
+               for (uint32_t t = 0; t < card; ++t) n += bitAt(f, (dd.w1 >> (16 * t)) & 0xFFFFu);
+            }
+            n = waveSum(n);
+            if (lane == 0 && n) atomicAdd(&heavy_sums[static_cast<size_t>(4 * j + k) * col.n_slots + s0 + slot], n);
+         }
+      }
+   }
+   flush();
+}
+
+// partial[e][s] += heavy_sums[e][s] and clears heavy_sums, for every dense list entry
+__global__ __launch_bounds__(256) void addHeavySumsKernel(MutationsBatch m, uint32_t* heavy_sums, uint32_t n_slots) {
+   const uint32_t e = blockIdx.y;
+   if (e >= m.list_counts[0]) return;
+   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n_slots; s += gridDim.x * blockDim.x) {
+      const size_t i = static_cast<size_t>(e) * n_slots + s;
+      const uint32_t h = heavy_sums[i];
+      if (h) {
+         m.partial[i] += h;
+         heavy_sums[i] = 0;
+      }
+   }
 }
 
 __global__ __launch_bounds__(256) void sparseCountBatchKernel(MutationsBatch m) {
@@ -1320,18 +1601,32 @@ hipError_t launchMutations(const MutationsLaunch& m, hipStream_t stream) {
    hipError_t err = hipSuccess;
    const SeqColumnView& h = *m.col_host;
    if (m.count_start && (err = hipEventRecord(m.count_start, stream)) != hipSuccess) return err;
+   // packed counters need at most kPackedMaxChunks chunks per work-group and the heavy sums
+   const bool packed = m.heavy_sums != nullptr && (m.n_chunks + m.n_groups - 1) / m.n_groups <= kPackedMaxChunks;
+   uint32_t* heavy = packed ? m.heavy_sums : nullptr;
+   const dim3 grid(h.n_blocks, m.n_groups);
+#define RDB_LAUNCH_K1(U, P)                                                                                            \
+   hipLaunchKernelGGL((mutationsCountKernel<U, P>), grid, dim3(kMutThreads), 0, stream, m.col_dev, m.n_chunks, m.n_groups, \
+                      m.filter, m.filter_chunk_cards, m.filter_card, m.sparse_max, m.partial, heavy)
    if (m.unroll == 16) {
-      hipLaunchKernelGGL(mutationsCountKernel<16>, dim3(h.n_blocks, m.n_groups), dim3(kMutThreads), 0, stream, m.col_dev,
-                         m.n_chunks, m.n_groups, m.filter, m.filter_chunk_cards, m.filter_card, m.sparse_max, m.partial);
+      if (packed) {
+         RDB_LAUNCH_K1(16, true);
+      } else {
+         RDB_LAUNCH_K1(16, false);
+      }
    } else {
-      hipLaunchKernelGGL(mutationsCountKernel<8>, dim3(h.n_blocks, m.n_groups), dim3(kMutThreads), 0, stream, m.col_dev,
-                         m.n_chunks, m.n_groups, m.filter, m.filter_chunk_cards, m.filter_card, m.sparse_max, m.partial);
+      if (packed) {
+         RDB_LAUNCH_K1(8, true);
+      } else {
+         RDB_LAUNCH_K1(8, false);
+      }
    }
+#undef RDB_LAUNCH_K1
    if ((err = hipGetLastError()) != hipSuccess) return err;
    if (m.count_stop && (err = hipEventRecord(m.count_stop, stream)) != hipSuccess) return err;
-   if (m.n_groups > 1) {
+   if ((m.n_groups > 1 || heavy) && h.n_slots > 0) {
       hipLaunchKernelGGL(reducePartialsKernel, dim3((h.n_slots + 255) / 256), dim3(256), 0, stream, m.partial,
-                         m.n_groups, h.n_slots, m.filter_card, m.sparse_max);
+                         m.n_groups, h.n_slots, m.filter_card, m.sparse_max, heavy);
       if ((err = hipGetLastError()) != hipSuccess) return err;
    }
    const uint32_t n_pblocks = (h.genome_length + kPosBlock - 1) / kPosBlock;
@@ -1365,12 +1660,26 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
    hipError_t err = hipSuccess;
    hipLaunchKernelGGL(batchPlanKernel, dim3(1), dim3(1024), 0, stream, m);
    if ((err = hipGetLastError()) != hipSuccess) return err;
-   const uint32_t n_xblocks = 8 * ((h.n_blocks + 7) / 8);
-   const dim3 count_grid(n_xblocks * m.max_queries);
-   if (m.unroll == 16) {
-      hipLaunchKernelGGL(mutationsCountBatchKernel<16>, count_grid, dim3(kMutThreads), 0, stream, m);
+   if (m.heavy_sums != nullptr) {
+      // shared scan, four dense queries per pass
+      const dim3 grid(h.n_blocks * ((m.max_queries + 3) / 4));
+      if (m.unroll == 16) {
+         hipLaunchKernelGGL(mutationsCountQuadKernel<4>, grid, dim3(kQuadThreads), 0, stream, m, m.heavy_sums);
+      } else {
+         hipLaunchKernelGGL(mutationsCountQuadKernel<2>, grid, dim3(kQuadThreads), 0, stream, m, m.heavy_sums);
+      }
+      if ((err = hipGetLastError()) != hipSuccess) return err;
+      const uint32_t sblocks = (h.n_slots + 255) / 256;
+      hipLaunchKernelGGL(addHeavySumsKernel, dim3(sblocks == 0 ? 1 : (sblocks < 64 ? sblocks : 64), m.max_queries),
+                         dim3(256), 0, stream, m, m.heavy_sums, h.n_slots);
    } else {
-      hipLaunchKernelGGL(mutationsCountBatchKernel<8>, count_grid, dim3(kMutThreads), 0, stream, m);
+      const uint32_t n_xblocks = 8 * ((h.n_blocks + 7) / 8);
+      const dim3 count_grid(n_xblocks * m.max_queries);
+      if (m.unroll == 16) {
+         hipLaunchKernelGGL(mutationsCountBatchKernel<16>, count_grid, dim3(kMutThreads), 0, stream, m);
+      } else {
+         hipLaunchKernelGGL(mutationsCountBatchKernel<8>, count_grid, dim3(kMutThreads), 0, stream, m);
+      }
    }
    if ((err = hipGetLastError()) != hipSuccess) return err;
    const uint32_t n_pblocks = (h.genome_length + kPosBlock - 1) / kPosBlock;

@@ -720,6 +720,38 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
       }
       pptr[padded] = acc;
    }
+   // missing-run index by (chunk, start bucket or "long", slice), rows in order
+   const uint32_t n_rb = (L + kRunBucket - 1) / kRunBucket;
+   const size_t n_lists = static_cast<size_t>(t.n_chunks) * (n_rb + 1) * kSlicesPerChunk;
+   std::vector<uint32_t> ri_ptr(n_lists + 1, 0);
+   std::vector<uint64_t> ri;
+   {
+      auto listOf = [&](uint32_t c, uint32_t r, uint32_t run) {
+         const uint32_t a = run & 0xFFFFu, b = run >> 16;
+         const uint32_t k = b - a > kShortRun ? n_rb : a / kRunBucket;
+         return (static_cast<size_t>(c) * (n_rb + 1) + k) * kSlicesPerChunk + (r >> 12);
+      };
+      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+         for (uint32_t r = 0; r < t.chunk_sizes[c]; ++r) {
+            const uint64_t i = t.chunk_offset[c] + r;
+            for (uint64_t k = nptr[i]; k < nptr[i + 1]; ++k) ri_ptr[listOf(c, r, runs_all[k]) + 1] += 1;
+         }
+      }
+      for (size_t l = 0; l < n_lists; ++l) ri_ptr[l + 1] += ri_ptr[l];
+      if (ri_ptr[n_lists] != nptr[n]) fail(RDB_EDEVICE, "internal: missing-run index size");
+      ri.resize(ri_ptr[n_lists]);
+      std::vector<uint32_t> fill(ri_ptr.begin(), ri_ptr.end() - 1);
+      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+         for (uint32_t r = 0; r < t.chunk_sizes[c]; ++r) {
+            const uint64_t i = t.chunk_offset[c] + r;
+            for (uint64_t k = nptr[i]; k < nptr[i + 1]; ++k) {
+               const uint32_t run = runs_all[k];
+               ri[fill[listOf(c, r, run)]++] = r | (static_cast<uint64_t>(run & 0xFFFFu) << 16) |
+                                              (static_cast<uint64_t>(run >> 16) << 32);
+            }
+         }
+      }
+   }
    col.has_nulls = !null_rows.empty();
    col.null_bitmap_id = t.addStoreBitmap(null_rows);
 
@@ -809,6 +841,9 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    v.cov = static_cast<const uint32_t*>(up(cov.data(), cov.size() * 4));
    v.nrun_ptr = static_cast<const uint32_t*>(up(pptr.data(), pptr.size() * 4));
    v.nruns = static_cast<const uint32_t*>(up(runs_all.data(), runs_all.size() * 4));
+   v.n_run_buckets = n_rb;
+   v.run_index_ptr = static_cast<const uint32_t*>(up(ri_ptr.data(), ri_ptr.size() * 4));
+   v.run_index = static_cast<const uint64_t*>(up(ri.data(), ri.size() * 8));
    v.null_bits = static_cast<const uint64_t*>(up(null_bits.data(), null_bits.size() * 8));
    v.pos_ranges = static_cast<const uint32_t*>(up(pos_ranges.data(), pos_ranges.size() * 4));
    v.row_mut_ptr = static_cast<const uint32_t*>(up(rmptr.data(), rmptr.size() * 4));
@@ -966,6 +1001,8 @@ void finalizeTable(Table& t) {
       max_L = std::max(max_L, c->genome_length);
    }
    t.slot_counts = DeviceBuffer(static_cast<size_t>(max_slots) * 4 * kMaxGroups);
+   t.heavy_sums = DeviceBuffer(static_cast<size_t>(max_slots) * 4);
+   hipCheck(hipMemset(t.heavy_sums.as<void>(), 0, t.heavy_sums.bytes()), "clear heavy sums");
    t.totals = DeviceBuffer(static_cast<size_t>((max_L + kMutPosBlock - 1) / kMutPosBlock) * 3 * sizeof(int64_t));
    // at least the 2048 rows the Mutations path copies back in its first transfer
    t.mut_rows = DeviceBuffer(16 + std::max<size_t>(static_cast<size_t>(max_L) * 32, 2048) * sizeof(MutationRowDev));

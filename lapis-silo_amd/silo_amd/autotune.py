@@ -2,8 +2,9 @@
 
 It measures, on the GPU and on the table at hand:
 
-* the dense Mutations kernel's shape: loads in flight per lane ("mutation_unroll") and chunk groups
-  ("mutation_groups"), timed with HIP events on a filter that touches every chunk;
+* the dense Mutations kernel's shape: loads in flight per lane ("mutation_unroll"), chunk groups
+  ("mutation_groups") and 16-bit packed counters ("mutation_packed"), timed with HIP events on a
+  filter that touches every chunk;
 * the dense/sparse switch ("sparse_max_rows"): the dense method costs about the same for any filter
   that touches every chunk, the sparse one grows with the filter's rows; both are timed on filters
   of several cardinalities, the sparse time is fitted as a + b * rows, and the switch is put where
@@ -49,17 +50,20 @@ def autotune(engine, table, probes: list, column: str = "main", reps: int = 5, l
     cards = [cards[i] for i in order]
     broad = probes[-1]
     engine.set_tuning("sparse_max_rows", -1)
-    # 1. dense kernel shape
+    # 1. dense kernel shape: loads in flight, counter packing (work-groups per CU), chunk groups
     shapes = {}
-    for unroll in (8, 16):
-        for groups in (0, 8, 4):
-            engine.set_tuning("mutation_unroll", unroll)
-            engine.set_tuning("mutation_groups", groups)
-            shapes[(unroll, groups)] = _kernel_us(engine, table, broad, column, reps)
+    for packed in (1, 0):
+        for unroll in (8, 16):
+            for groups in (0, 12, 16, 24, 32):
+                engine.set_tuning("mutation_packed", packed)
+                engine.set_tuning("mutation_unroll", unroll)
+                engine.set_tuning("mutation_groups", groups)
+                shapes[(packed, unroll, groups)] = _kernel_us(engine, table, broad, column, reps)
     best = min(shapes, key=shapes.get)
-    engine.set_tuning("mutation_unroll", best[0])
-    engine.set_tuning("mutation_groups", best[1])
-    say(f"dense kernel shapes (unroll, groups) -> us: {shapes}; chose {best}")
+    engine.set_tuning("mutation_packed", best[0])
+    engine.set_tuning("mutation_unroll", best[1])
+    engine.set_tuning("mutation_groups", best[2])
+    say(f"dense kernel shapes (packed, unroll, groups) -> us: {shapes}; chose {best}")
     # 2. dense/sparse switch from per-query wall times
     dense = []
     sparse = []
@@ -88,8 +92,8 @@ def autotune(engine, table, probes: list, column: str = "main", reps: int = 5, l
     engine.set_tuning("sparse_max_rows", threshold)
     say(f"dense/sparse per-query seconds: {[(c, round(d * 1e6, 1), round(s * 1e6, 1)) for c, d, s in zip(cards, dense, sparse)]}; "
         f"sparse fit a={a * 1e6:.1f}us b={b * 1e9:.2f}ns/row; switch at {threshold} rows")
-    return {"mutation_unroll": best[0], "mutation_groups": best[1], "sparse_max_rows": threshold,
-            "dense_shapes_us": {f"{k[0]}x{k[1]}": round(v, 2) for k, v in shapes.items()},
+    return {"mutation_packed": best[0], "mutation_unroll": best[1], "mutation_groups": best[2], "sparse_max_rows": threshold,
+            "dense_shapes_us": {f"p{k[0]}u{k[1]}g{k[2]}": round(v, 2) for k, v in shapes.items()},
             "probe_cards": [int(c) for c in cards],
             "dense_us": [round(d * 1e6, 1) for d in dense], "sparse_us": [round(s * 1e6, 1) for s in sparse]}
 

@@ -7,7 +7,7 @@ from helpers import ExprGen, random_bitmaps, small_spec
 
 pytestmark = pytest.mark.gpu
 
-KNOBS = ("sparse_max_rows", "batch_max_queries", "batch_lanes", "batch_graphs", "batch_row_capacity")
+KNOBS = ("sparse_max_rows", "batch_max_queries", "batch_lanes", "batch_graphs", "batch_row_capacity", "batch_shared_scan")
 
 
 @pytest.fixture(scope="module")
@@ -64,11 +64,13 @@ def check(et, ot, queries):
         assert g == expected(ot, q), q
 
 
+@pytest.mark.parametrize("shared", [0, 1])
 @pytest.mark.parametrize("graphs", [0, 1])
 @pytest.mark.parametrize("sparse_max", [-1, 300, 1 << 40])
-def test_batch_matches_oracle(small, tuning, graphs, sparse_max):
+def test_batch_matches_oracle(small, tuning, graphs, sparse_max, shared):
     ds, ts, ot, et = small
     tuning.set_tuning("batch_graphs", graphs)
+    tuning.set_tuning("batch_shared_scan", shared)
     tuning.set_tuning("sparse_max_rows", sparse_max)
     tuning.set_tuning("batch_max_queries", 16)  # several lane jobs per batch
     check(et, ot, make_queries(ds, seed=100 + graphs, n=60, bitmaps=len(ts.bitmaps)))

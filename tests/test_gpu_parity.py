@@ -98,7 +98,7 @@ def test_tuning_keys(engine):
     with pytest.raises(EngineError):
         engine.set_tuning("no_such_knob", 1)
     with pytest.raises(EngineError):
-        engine.set_tuning("mutation_groups", 17)
+        engine.set_tuning("mutation_groups", 33)
     assert engine.get_tuning("mutation_groups") == 0
 
 
