@@ -147,6 +147,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-query path (the batched one still runs)")
+    ap.add_argument("--no-batched", action="store_true", help="only the per-query path (for profiling its kernels)")
     ap.add_argument("--planner-trees", type=int, default=20000, help="config 3: training trees (200000 in full)")
     ap.add_argument("--planner-eval", type=int, default=2000, help="config 3: held-out trees (10000 in full)")
     ap.add_argument("--planner-epochs", type=int, default=30)
@@ -241,6 +242,16 @@ def main():
             if q["action"] == "mutations":
                 traffic += et.mutations_traffic(b, "main")
             cards.append(b.cardinality)
+
+    if args.no_batched:
+        n_dense = prof["mutations_launches"] if prof else 0
+        print(json.dumps({"metric": METRIC, "value": plain["queries_per_s"] if plain else None, "unit": "queries/s",
+                          "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "plain": plain,
+                          "kernel_profile": prof, "dense_launches": n_dense,
+                          "algorithmic_bytes_per_dense_launch": int(traffic / max(n_mut, 1))}), flush=True)
+        et.close()
+        eng.close()
+        return
 
     # ---- batched path: autotune, then the whole step as one batch ---------------------------------
     t0 = time.time()

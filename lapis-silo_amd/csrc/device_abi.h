@@ -153,6 +153,8 @@ enum OpCode : uint32_t {
    OP_TH_END,    // a = planes, b = n, c = exact
    OP_SKIP_IF_EMPTY,  // a = target: jump when the top word is zero in every lane (rest of an And)
    OP_SKIP_IF_FULL,   // a = target: jump when the top word is all ones in every lane (rest of an Or)
+   OP_TH_SKIP,        // a = target (the OP_TH_END), b = planes, c = n, d = exact, e = children left:
+                      // jump when every row's count is decided whatever the children left add
 };
 
 enum CmpCode : uint32_t { CMP_EQ = 0, CMP_LT, CMP_GT, CMP_LE, CMP_GE, CMP_NE };

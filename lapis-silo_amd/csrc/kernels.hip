@@ -446,6 +446,34 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                push(ins.c ? eq : (gt | eq));
                break;
             }
+            case OP_TH_SKIP: {
+               // counts of the rows (bit-sliced over ins.b planes): decided when count >= n (> n
+               // for an exact match) or when count + left < n
+               const uint32_t base = sp - (ins.b + 1);
+               auto compare = [&](uint32_t k, uint64_t& gt, uint64_t& eq) {
+                  gt = 0;
+                  eq = ~0ULL;
+                  for (int j = static_cast<int>(ins.b) - 1; j >= 0; --j) {
+                     const uint64_t plane = stack[(base + 1 + j) * 64 + lane];
+                     if ((k >> j) & 1u) {
+                        eq &= plane;
+                     } else {
+                        gt |= eq & plane;
+                        eq &= ~plane;
+                     }
+                  }
+               };
+               uint64_t gt_n, eq_n;
+               compare(ins.c, gt_n, eq_n);
+               uint64_t decided = ins.d ? gt_n : (gt_n | eq_n);
+               if (ins.c > ins.e) {
+                  uint64_t gt_k, eq_k;
+                  compare(ins.c - ins.e, gt_k, eq_k);
+                  decided |= ~(gt_k | eq_k);  // count < n - left
+               }
+               if (__ballot(decided != ~0ULL) == 0ULL) i = ins.a - 1;
+               break;
+            }
             case OP_SKIP_IF_EMPTY:
                // the slice's partial conjunction is empty: the remaining conjuncts cannot change it
                if (__ballot(stack[(sp - 1) * 64 + lane] != 0ULL) == 0ULL) i = ins.a - 1;
@@ -483,8 +511,9 @@ __device__ inline bool denseActive(const unsigned long long* card, long long spa
    return !sparseActive(card, sparse_max);
 }
 
+// bit v of a filter tile: one 32-bit LDS read (a 64-bit read spans two banks)
 __device__ inline uint32_t bitAt(const uint64_t* f, uint32_t v) {
-   return static_cast<uint32_t>((f[v >> 6] >> (v & 63)) & 1ULL);
+   return (reinterpret_cast<const uint32_t*>(f)[v >> 5] >> (v & 31)) & 1u;
 }
 
 __device__ inline uint32_t rangePopcount(const uint64_t* f, uint32_t a, uint32_t b) {
@@ -568,7 +597,7 @@ __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b,
                cur = slot;
                acc = 0;
             }
-            acc += static_cast<uint32_t>((f[row >> 6] >> (row & 63)) & 1ULL);
+            acc += bitAt(f, row);
          };
          for (uint32_t base = lane * kUnroll; base < n4; base += 64 * kUnroll) {
             uint4 e[kUnroll];
@@ -1413,76 +1442,102 @@ __global__ void symbolScatterKernel(const SeqColumnView* __restrict__ colp, uint
 }
 
 constexpr uint32_t kHistLdsBins = 16384;
+// Bin ranges counted in LDS by separate work-groups; bins beyond the last range go to global atomics.
+constexpr uint32_t kHistRanges = 8;
 
-// One work-group per (chunk, quarter) of rows; labels of up to four dimensions combined with
-// strides into one bin.  The histogram is privatised in LDS when it fits.
+// Rows of one quarter chunk (16,384 rows) of a filter, each visited once: for a sparse quarter by
+// iterating the set bits of its words, for a dense one by walking the rows in order (consecutive
+// threads on consecutive rows, so the label loads coalesce).
+template <typename F>
+__device__ inline void forQuarterRows(const uint64_t* __restrict__ f, uint32_t chunk_card, size_t row_base,
+                                      uint64_t* words_lds, F visit) {
+   if (chunk_card > 8192) {
+      for (uint32_t w = threadIdx.x; w < 256; w += blockDim.x) words_lds[w] = f[w];
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < 16384; i += blockDim.x) {
+         if ((words_lds[i >> 6] >> (i & 63)) & 1ULL) visit(row_base + i);
+      }
+   } else {
+      for (uint32_t w = threadIdx.x; w < 256; w += blockDim.x) {
+         uint64_t x = f[w];
+         while (x) {
+            const uint32_t bit = __builtin_ctzll(x);
+            x &= x - 1;
+            visit(row_base + w * 64 + bit);
+         }
+      }
+   }
+}
+
+
+// Work-group (chunk quarter, bin range y): labels of up to four dimensions combined with strides
+// into one bin; the rows whose bin falls in [y * kHistLdsBins, (y + 1) * kHistLdsBins) are counted
+// in LDS, and the last range also takes every bin above it with global atomics.
 __global__ __launch_bounds__(256) void groupHistogramKernel(uint32_t n_chunks, const uint64_t* __restrict__ filter,
                                                             const uint32_t* __restrict__ filter_chunk_cards,
                                                             GroupDims dims, uint32_t n_bins,
                                                             uint32_t* __restrict__ hist) {
    __shared__ uint32_t lh[kHistLdsBins];
+   __shared__ uint64_t words[256];
    const uint32_t c = blockIdx.x >> 2;
    const uint32_t quarter = blockIdx.x & 3;
-   if (filter_chunk_cards[c] == 0) return;
-   const bool use_lds = n_bins <= kHistLdsBins;
-   if (use_lds) {
-      for (uint32_t i = threadIdx.x; i < n_bins; i += blockDim.x) lh[i] = 0;
-      __syncthreads();
-   }
+   const uint32_t lo = blockIdx.y * kHistLdsBins;
+   const uint32_t card = filter_chunk_cards[c];
+   if (card == 0 || lo >= n_bins) return;
+   const uint32_t span = n_bins - lo < kHistLdsBins ? n_bins - lo : kHistLdsBins;
+   const bool last = blockIdx.y + 1 == gridDim.y;
+   for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) lh[i] = 0;
+   __syncthreads();
    const uint64_t* f = filter + static_cast<size_t>(c) * kChunkWords + quarter * 256;
-   for (uint32_t w = threadIdx.x; w < 256; w += blockDim.x) {
-      uint64_t x = f[w];
-      while (x) {
-         const uint32_t bit = __builtin_ctzll(x);
-         x &= x - 1;
-         const size_t r = (static_cast<size_t>(c) << 16) + (quarter * 256 + w) * 64 + bit;
-         uint32_t key = 0;
-         bool in_table = true;
-         for (uint32_t d = 0; d < dims.n; ++d) {
-            const uint32_t lab = dims.labels[d][r];
-            in_table &= lab != kInvalidId;
-            key += lab * dims.stride[d];
-         }
-         if (!in_table) continue;
-         if (use_lds) {
-            atomicAdd(&lh[key], 1u);
-         } else {
-            atomicAdd(&hist[key], 1u);
-         }
+   forQuarterRows(f, card, (static_cast<size_t>(c) << 16) + quarter * 16384u, words, [&](size_t r) {
+      uint32_t key = 0;
+      bool in_table = true;
+      for (uint32_t d = 0; d < dims.n; ++d) {
+         const uint32_t lab = dims.labels[d][r];
+         in_table &= lab != kInvalidId;
+         key += lab * dims.stride[d];
       }
-   }
-   if (use_lds) {
-      __syncthreads();
-      for (uint32_t i = threadIdx.x; i < n_bins; i += blockDim.x) {
-         if (lh[i]) atomicAdd(&hist[i], lh[i]);
+      if (!in_table || key < lo) return;
+      if (key - lo < kHistLdsBins) {
+         atomicAdd(&lh[key - lo], 1u);
+      } else if (last) {
+         atomicAdd(&hist[key], 1u);
       }
+   });
+   __syncthreads();
+   for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) {
+      if (lh[i]) atomicAdd(&hist[lo + i], lh[i]);
    }
 }
 
+// Work-group (quarter chunk, entry e): the rows are counted in passes over bin ranges
+// [z * kHistLdsBins, (z + 1) * kHistLdsBins), each in LDS; the last pass also takes every bin above
+// its range, with global atomics.  The passes run inside the work-group so that the grid (fixed for
+// graph replay) does not grow with the bin count.
 __global__ __launch_bounds__(256) void groupHistogramBatchKernel(AggregateBatch a) {
    __shared__ uint32_t lh[kHistLdsBins];
+   __shared__ uint64_t words[256];
    const uint32_t e = blockIdx.y;
    if (e >= *a.n_agg) return;
+   const uint32_t n_bins = a.bins[e];
    const uint32_t q = a.agg_query[e];
    const uint32_t c = blockIdx.x >> 2;
    const uint32_t quarter = blockIdx.x & 3;
-   if (a.chunk_cards[static_cast<size_t>(q) * a.n_chunks + c] == 0) return;
-   const uint32_t n_bins = a.bins[e];
+   const uint32_t card = a.chunk_cards[static_cast<size_t>(q) * a.n_chunks + c];
+   if (card == 0 || n_bins == 0) return;
    const GroupDims dims = a.dims[e];
    uint32_t* hist = a.hist + a.hist_offset[e];
-   const bool use_lds = n_bins <= kHistLdsBins;
-   if (use_lds) {
-      for (uint32_t i = threadIdx.x; i < n_bins; i += blockDim.x) lh[i] = 0;
-      __syncthreads();
-   }
    const uint64_t* f = a.filters + static_cast<size_t>(q) * a.n_chunks * kChunkWords + static_cast<size_t>(c) * kChunkWords +
                        quarter * 256;
-   for (uint32_t w = threadIdx.x; w < 256; w += blockDim.x) {
-      uint64_t x = f[w];
-      while (x) {
-         const uint32_t bit = __builtin_ctzll(x);
-         x &= x - 1;
-         const size_t r = (static_cast<size_t>(c) << 16) + (quarter * 256 + w) * 64 + bit;
+   uint32_t passes = (n_bins + kHistLdsBins - 1) / kHistLdsBins;
+   passes = passes > kHistRanges ? kHistRanges : passes;
+   for (uint32_t z = 0; z < passes; ++z) {
+      const uint32_t lo = z * kHistLdsBins;
+      const uint32_t span = n_bins - lo < kHistLdsBins ? n_bins - lo : kHistLdsBins;
+      const bool last = z + 1 == passes;
+      for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) lh[i] = 0;
+      __syncthreads();
+      forQuarterRows(f, card, (static_cast<size_t>(c) << 16) + quarter * 16384u, words, [&](size_t r) {
          uint32_t key = 0;
          bool in_table = true;
          for (uint32_t d = 0; d < dims.n; ++d) {
@@ -1490,19 +1545,18 @@ __global__ __launch_bounds__(256) void groupHistogramBatchKernel(AggregateBatch 
             in_table &= lab != kInvalidId;
             key += lab * dims.stride[d];
          }
-         if (!in_table) continue;
-         if (use_lds) {
-            atomicAdd(&lh[key], 1u);
-         } else {
+         if (!in_table || key < lo) return;
+         if (key - lo < kHistLdsBins) {
+            atomicAdd(&lh[key - lo], 1u);
+         } else if (last) {
             atomicAdd(&hist[key], 1u);
          }
-      }
-   }
-   if (use_lds) {
+      });
       __syncthreads();
-      for (uint32_t i = threadIdx.x; i < n_bins; i += blockDim.x) {
-         if (lh[i]) atomicAdd(&hist[i], lh[i]);
+      for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) {
+         if (lh[i]) atomicAdd(&hist[lo + i], lh[i]);
       }
+      __syncthreads();
    }
 }
 
@@ -1640,6 +1694,7 @@ hipError_t launchMutations(const MutationsLaunch& m, hipStream_t stream) {
    if ((err = hipGetLastError()) != hipSuccess) return err;
    if (m.sparse_max < 0) return hipSuccess;
    const uint32_t n_words = m.n_chunks * kChunkWords;
+   if (n_words == 0) return hipSuccess;  // no chunks: no rows
    hipLaunchKernelGGL(sparseCountKernel, dim3((n_words + 255) / 256), dim3(256), 0, stream, m.col_dev, m.chunk_sizes,
                       n_words, m.filter, m.filter_card, m.sparse_max, m.sparse_counts, m.sparse_bnd);
    if ((err = hipGetLastError()) != hipSuccess) return err;
@@ -1691,6 +1746,7 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
    if ((err = hipGetLastError()) != hipSuccess) return err;
    if (m.sparse_max < 0) return hipSuccess;
    const uint32_t n_words = m.n_chunks * kChunkWords;
+   if (n_words == 0) return hipSuccess;  // no chunks: no rows
    hipLaunchKernelGGL(sparseCountBatchKernel, dim3((n_words + 255) / 256, m.max_queries), dim3(256), 0, stream, m);
    if ((err = hipGetLastError()) != hipSuccess) return err;
    hipLaunchKernelGGL(mutationsTotalsBatchKernel<SparseSource>, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0,
@@ -1710,6 +1766,7 @@ hipError_t launchAggregateBatch(const AggregateBatch& a, hipStream_t stream) {
    hipLaunchKernelGGL(fillCountKernel, dim3(blocks), dim3(256), 0, stream, a.hist, 0u, a.total_bins);
    hipError_t err = hipGetLastError();
    if (err != hipSuccess) return err;
+   if (a.n_chunks == 0) return hipSuccess;  // no rows: the histograms stay zero
    hipLaunchKernelGGL(groupHistogramBatchKernel, dim3(a.n_chunks * 4, a.max_entries), dim3(256), 0, stream, a);
    if ((err = hipGetLastError()) != hipSuccess) return err;
    hipLaunchKernelGGL(compactHistogramKernel, dim3(64, a.max_entries), dim3(256), 0, stream, a);
@@ -1729,6 +1786,7 @@ hipError_t launchSymbolLabels(const SeqColumnView* col_dev, const uint32_t* chun
                               uint32_t n_chunks, uint32_t block, uint32_t slot_lo, uint32_t slot_hi, uint32_t* labels,
                               hipStream_t stream) {
    const uint32_t n = n_chunks << 16;
+   if (n_chunks == 0) return hipSuccess;
    hipLaunchKernelGGL(symbolLabelsKernel, dim3((n + 255) / 256), dim3(256), 0, stream, col_dev, chunk_sizes, position, n,
                       labels);
    hipError_t err = hipGetLastError();
@@ -1742,7 +1800,10 @@ hipError_t launchGroupHistogram(uint32_t n_chunks, const uint64_t* filter, const
                                 const GroupDims& dims, uint32_t n_bins, uint32_t* hist, hipStream_t stream) {
    hipError_t err = launchFill(hist, 0, n_bins, stream);
    if (err != hipSuccess) return err;
-   hipLaunchKernelGGL(groupHistogramKernel, dim3(n_chunks * 4), dim3(256), 0, stream, n_chunks, filter,
+   if (n_chunks == 0) return hipSuccess;
+   uint32_t ranges = (n_bins + kHistLdsBins - 1) / kHistLdsBins;
+   ranges = ranges == 0 ? 1 : (ranges > kHistRanges ? kHistRanges : ranges);
+   hipLaunchKernelGGL(groupHistogramKernel, dim3(n_chunks * 4, ranges), dim3(256), 0, stream, n_chunks, filter,
                       filter_chunk_cards, dims, n_bins, hist);
    return hipGetLastError();
 }

@@ -293,6 +293,16 @@ void apply(std::vector<Node>& nodes, const float* estimates) {
          auto& o = static_cast<Union&>(*n.op);
          std::stable_sort(o.children.begin(), o.children.end(), by(false));
          o.short_circuit = true;
+      } else if (n.op->type() == THRESHOLD) {
+         // only the non-negated children are reordered: their counting does not depend on the
+         // order (the negated ones' handling of rows outside the layout does)
+         auto& o = static_cast<Threshold&>(*n.op);
+         const size_t m = o.non_negated_children.size() + o.negated_children.size();
+         // few matchers needed: the largest children first reach n soonest; many: the smallest
+         // first rule rows out soonest
+         const bool largest_first = 2 * o.number_of_matchers <= m;
+         std::stable_sort(o.non_negated_children.begin(), o.non_negated_children.end(), by(!largest_first));
+         o.short_circuit = true;
       }
    }
 }

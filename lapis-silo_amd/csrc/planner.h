@@ -5,7 +5,8 @@
 //   * orders the children of Intersection (ascending estimated rows) and Union (descending), both
 //     commutative in the filter kernel's word arithmetic, and
 //   * turns on their short-circuit, with which the filter kernel skips the remaining children of a
-//     4096-row slice once the partial conjunction is empty (or the partial disjunction full).
+//     4096-row slice once the partial conjunction is empty (or the partial disjunction full), or
+//     once every row's n-of-m count is decided whatever the remaining children add.
 // Without an estimator the compiled order stands and nothing is skipped: the reference's plan,
 // which evaluates every child (intersection.cpp:59-106, union.cpp:34-42).
 #pragma once
@@ -45,7 +46,8 @@ void compose(const Table& t, std::vector<Node>& nodes, size_t k, const float* va
 // And, complement rule for Or, binomial tail for n-of-m).  Also one of the network's features.
 float independenceEstimate(const Table& t, const std::vector<Node>& nodes, size_t k, const float* values);
 
-// Orders the children of every Intersection and Union by the estimates and turns on short-circuit.
+// Orders the children of every Intersection, Union and Threshold (non-negated ones) by the
+// estimates and turns on their short-circuit.
 void apply(std::vector<Node>& nodes, const float* estimates);
 
 enum Mode { REFERENCE = 0, INDEPENDENCE = 1, LEARNED = 2 };

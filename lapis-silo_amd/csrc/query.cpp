@@ -315,14 +315,28 @@ void Threshold::emit(Emitter& e) const {
    const uint32_t m = static_cast<uint32_t>(non_negated_children.size() + negated_children.size());
    const uint32_t planes = 32u - static_cast<uint32_t>(std::countl_zero(m));
    e.thresholdBegin(planes);
+   // with short-circuit, before each later child: skip the rest once every row's count c is such
+   // that c + (children left) cannot change the comparison (each child adds at most one)
+   std::vector<size_t> skips;
+   uint32_t done = 0;
+   auto skipPoint = [&] {
+      if (!short_circuit || done == 0) return;
+      e.code.push_back(Instr{OP_TH_SKIP, 0, planes, number_of_matchers, match_exactly ? 1u : 0u, m - done, 0, 0});
+      skips.push_back(e.code.size() - 1);
+   };
    for (const auto& c : non_negated_children) {
+      skipPoint();
       c->emit(e);
       e.thresholdAdd(planes, false);
+      ++done;
    }
    for (const auto& c : negated_children) {
+      skipPoint();
       c->emit(e);
       e.thresholdAdd(planes, true);
+      ++done;
    }
+   for (size_t at : skips) e.patch(at, e.code.size());
    e.thresholdEnd(planes, number_of_matchers, match_exactly);
 }
 

@@ -246,6 +246,8 @@ class Threshold : public Operator {
    OperatorVector negated_children;
    uint32_t number_of_matchers;
    bool match_exactly;
+   // planner: skip the remaining children of a slice once every row's outcome is decided
+   bool short_circuit = false;
 };
 
 class Union : public Operator {

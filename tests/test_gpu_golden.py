@@ -64,3 +64,34 @@ def test_local_reference_kats(engine, case):
         assert "".join(symbol_chars(0)[s] for s in et.local_reference(case["column"])) == case["expect"]
     finally:
         et.close()
+
+
+@pytest.mark.parametrize("mode", ["independence", "learned"])
+def test_kat_counts_under_the_planner(engine, mode):
+    """Planned evaluation (reordered children, short-circuit And / Or / n-of-m skips) of every
+    operator and query known-answer filter gives the known cardinality."""
+    mlp = engine.mlp(seed=13) if mode == "learned" else None
+    try:
+        engine.set_planner(mode, mlp)
+        for case in KATS["operator_cases"]:
+            if "expect_error" in case:
+                continue
+            et = engine.table(operator_spec(case))
+            try:
+                assert et.batch([{"filter": case["filter"], "action": "count"}]) == [len(case["expect"])], case["name"]
+            finally:
+                et.close()
+        for case in KATS["query_cases"]:
+            if "expect_error" in case:
+                continue
+            spec = dataset_spec(case["dataset"])
+            et = engine.table(spec)
+            try:
+                want = case["expect_count"] if "expect_count" in case else len(case["expect_rows"])
+                assert et.batch([{"filter": case["filter"], "action": "count"}]) == [want], case["name"]
+            finally:
+                et.close()
+    finally:
+        engine.set_planner("reference")
+        if mlp is not None:
+            mlp.free()
