@@ -255,7 +255,11 @@ def main():
 
     # ---- batched path: autotune, then the whole step as one batch ---------------------------------
     t0 = time.time()
-    tuned = autotune.autotune(eng, et, autotune.probe_filters(ds), log=log) if ds.n_rows >= 1 << 20 else None
+    tuned = None
+    if ds.n_rows >= 1 << 20:
+        probes = autotune.probe_filters(ds)
+        tuned = autotune.autotune(eng, et, probes, log=log)
+        tuned.update(autotune.autotune_batch(eng, et, probes, log=log))
     log(f"autotune {time.time() - t0:.1f}s")
     traffic_tuned = 0  # Mutations bytes by the method each query takes under the tuned switch
     for q in queries:

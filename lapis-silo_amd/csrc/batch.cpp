@@ -128,6 +128,7 @@ void setupLane(Table& t, BatchLane& lane) {
       const MutScratch ms = mutScratch(t);
       lane.partial = DeviceBuffer(4ULL * Q * ms.slots);
       lane.heavy_sums = DeviceBuffer(4ULL * Q * ms.slots);
+      lane.partial_b = DeviceBuffer(4ULL * Q * ms.slots);
       hipCheck(hipMemset(lane.heavy_sums.as<void>(), 0, lane.heavy_sums.bytes()), "clear heavy sums");
       lane.sparse_counts = DeviceBuffer(4ULL * Q * ms.counts);
       lane.sparse_bnd = DeviceBuffer(4ULL * Q * ms.bnd);
@@ -170,10 +171,12 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column) {
       m.min_proportion = reinterpret_cast<const double*>(pd + pl.min_prop);
       m.sparse_max = ctx.sparse_max_rows;
       m.unroll = ctx.mutation_unroll;
+      m.coop = ctx.mutation_coop != 0;
       m.lists = lane.lists.as<uint32_t>();
       m.list_counts = lane.list_counts.as<uint32_t>();
       m.partial = lane.partial.as<uint32_t>();
       m.heavy_sums = ctx.batch_shared_scan ? lane.heavy_sums.as<uint32_t>() : nullptr;
+      m.partial_b = lane.partial_b.as<uint32_t>();
       m.sparse_counts = lane.sparse_counts.as<uint32_t>();
       m.sparse_bnd = lane.sparse_bnd.as<int32_t>();
       m.totals = lane.totals.as<int64_t>();
@@ -214,9 +217,8 @@ void launchLane(Table& t, BatchLane& lane, uint32_t depth, int column) {
       enqueueKernels(t, lane, bucket, column);
       return;
    }
-   const uint64_t key = static_cast<uint64_t>(bucket) | (static_cast<uint64_t>(ctx.mutation_unroll) << 10) |
-                        (static_cast<uint64_t>(ctx.batch_shared_scan) << 15) |
-                        (static_cast<uint64_t>(column + 1) << 16) | (static_cast<uint64_t>(ctx.sparse_max_rows + 1) << 24);
+   const uint32_t knobs = ctx.mutation_unroll | (ctx.batch_shared_scan ? 1u << 8 : 0u) | (ctx.mutation_coop ? 1u << 9 : 0u);
+   const auto key = std::make_tuple(bucket, column, static_cast<long long>(ctx.sparse_max_rows), knobs);
    auto it = lane.graphs.find(key);
    if (it == lane.graphs.end()) {
       hipGraph_t graph = nullptr;
@@ -565,7 +567,7 @@ size_t laneBytes(const Table& t, uint32_t max_queries, uint32_t row_capacity) {
    b += 4ULL * hist_cap;
    if (!t.seq.empty()) {
       const MutScratch ms = mutScratch(t);
-      b += 4ULL * Q * (2 * ms.slots + ms.counts + ms.bnd) + 2ULL * Q * ms.pblocks * 3 * sizeof(int64_t);
+      b += 4ULL * Q * (3 * ms.slots + ms.counts + ms.bnd) + 2ULL * Q * ms.pblocks * 3 * sizeof(int64_t);
    }
    return b;
 }

@@ -11,6 +11,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "engine.h"
@@ -31,11 +32,12 @@ struct BatchLane {
    uint32_t hist_capacity = 0;
    size_t params_bytes = 0;
    size_t out_bytes = 0;
-   DeviceBuffer params, filters, cards, lists, list_counts, partial, heavy_sums, sparse_counts, sparse_bnd, totals, rows,
+   DeviceBuffer params, filters, cards, lists, list_counts, partial, partial_b, heavy_sums, sparse_counts, sparse_bnd, totals, rows,
       n_rows, hist, out;
    void* host_params = nullptr;
    void* host_out = nullptr;
-   std::map<uint64_t, hipGraphExec_t> graphs;  // by (depth bucket, Mutations column, sparse switch, unroll)
+   // by (depth bucket, Mutations column, sparse switch, kernel knobs): what the captured launches bake in
+   std::map<std::tuple<uint32_t, int, long long, uint32_t>, hipGraphExec_t> graphs;
 
    // the job on the lane (busy until retired)
    bool busy = false;

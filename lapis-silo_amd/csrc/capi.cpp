@@ -121,6 +121,7 @@ MutationsLaunch mutationsLaunch(Table& t, int idx, const DeviceBitmap& f) {
    m.n_chunks = t.n_chunks;
    m.n_groups = mutationGroups(t, col);
    m.unroll = t.ctx->mutation_unroll;
+   m.coop = t.ctx->mutation_coop != 0;
    m.filter = f.words;
    m.filter_chunk_cards = f.chunk_cards;
    m.filter_card = f.card_dev;
@@ -209,6 +210,8 @@ int rdb_set_tuning(rdb_ctx* ctx, const char* key, int64_t value) {
          ctx->c.mutation_unroll = static_cast<uint32_t>(value);
       } else if (k == "mutation_packed") {
          ctx->c.mutation_packed = value != 0;
+      } else if (k == "mutation_coop") {
+         ctx->c.mutation_coop = value != 0;
       } else if (k == "batch_max_queries") {
          if (value < 1 || value > 4096) fail(RDB_EINVAL, "batch_max_queries must be in [1, 4096]");
          ctx->c.batch_max_queries = static_cast<uint32_t>(value);
@@ -241,6 +244,8 @@ int rdb_get_tuning(rdb_ctx* ctx, const char* key, int64_t* value) {
          *value = ctx->c.mutation_unroll;
       } else if (k == "mutation_packed") {
          *value = ctx->c.mutation_packed;
+      } else if (k == "mutation_coop") {
+         *value = ctx->c.mutation_coop;
       } else if (k == "batch_max_queries") {
          *value = ctx->c.batch_max_queries;
       } else if (k == "batch_lanes") {

@@ -48,6 +48,7 @@ struct MutationsLaunch {
    uint32_t n_chunks = 0;
    uint32_t n_groups = 1;
    uint32_t unroll = 8;                    // 16-byte loads in flight per lane in the dense kernel (8 or 16)
+   bool coop = false;                      // a work-group's wavefronts count each chunk tile together
    const uint64_t* filter = nullptr;
    const uint32_t* filter_chunk_cards = nullptr;
    const unsigned long long* filter_card = nullptr;
@@ -90,10 +91,12 @@ struct MutationsBatch {
    const double* min_proportion = nullptr;     // [q]
    long long sparse_max = -1;
    uint32_t unroll = 8;
+   bool coop = false;                          // per-query dense kernel: wavefronts share each tile
    uint32_t* lists = nullptr;                  // [2 * max_queries]: dense ids, then sparse ids
    uint32_t* list_counts = nullptr;            // [2]
    uint32_t* partial = nullptr;                // [max_queries * n_slots]
    uint32_t* heavy_sums = nullptr;             // [max_queries * n_slots], zero: set = shared-scan kernel
+   uint32_t* partial_b = nullptr;              // [max_queries * n_slots]: the shared scan's second part
    uint32_t* sparse_counts = nullptr;          // [max_queries * n_symbols * L], zero
    int32_t* sparse_bnd = nullptr;              // [max_queries * 3 * L], zero
    int64_t* totals = nullptr;                  // [2 * max_queries * position blocks * 3]

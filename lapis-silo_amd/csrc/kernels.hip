@@ -511,6 +511,22 @@ __device__ inline bool denseActive(const unsigned long long* card, long long spa
    return !sparseActive(card, sparse_max);
 }
 
+// Index arrays are read through global-address-space pointers: the column view holds them as
+// generic pointers, which would compile to flat loads (counted against the LDS counter too, so every
+// LDS wait would also wait for them).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef const u32x4 __attribute__((address_space(1))) GlobalU32x4;
+typedef const uint16_t __attribute__((address_space(1))) GlobalU16;
+typedef const uint32_t __attribute__((address_space(1))) GlobalU32;
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+typedef const u64x2 __attribute__((address_space(1))) GlobalU64x2;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef const u32x2 __attribute__((address_space(1))) GlobalU32x2;
+template <typename G>
+__device__ inline const G* toGlobal(const void* p) {
+   return (const G*)(p);
+}
+
 // bit v of a filter tile: one 32-bit LDS read (a 64-bit read spans two banks)
 __device__ inline uint32_t bitAt(const uint64_t* f, uint32_t v) {
    return (reinterpret_cast<const uint32_t*>(f)[v >> 5] >> (v & 31)) & 1u;
@@ -538,7 +554,7 @@ __device__ inline uint32_t rangePopcount(const uint64_t* f, uint32_t a, uint32_t
 // 65,536 rows of a chunk, add to the global `heavy` sums instead.
 constexpr uint32_t kPackedMaxChunks = 15;
 
-template <uint32_t kUnroll, bool kPacked>
+template <uint32_t kUnroll, bool kPacked, bool kCoop>
 __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b, uint32_t g, uint32_t n_chunks,
                                            uint32_t n_groups, const uint64_t* __restrict__ filter,
                                            const uint32_t* __restrict__ filter_chunk_cards, uint32_t* __restrict__ out,
@@ -546,6 +562,10 @@ __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b,
    const uint32_t tid = threadIdx.x;
    const uint32_t lane = tid & 63;
    const uint32_t wave = tid >> 6;
+   // kCoop: the work-group's four wavefronts count each chunk's tile together (one filter tile in
+   // LDS, entries split over 256 threads); otherwise each wavefront takes its own chunks
+   constexpr uint32_t kThreads = kCoop ? kMutThreads : 64;
+   const uint32_t me = kCoop ? tid : lane;
    const uint32_t s0 = col.block_slot_start[b];
    const uint32_t ns = col.block_slot_start[b + 1] - s0;
    constexpr uint32_t kWords = kPacked ? (kMaxBlockSlots + 1) / 2 : kMaxBlockSlots + 1;
@@ -558,144 +578,246 @@ __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b,
          atomicAdd(&cnt[slot], v);
       }
    };
-   uint64_t* f = fs[wave];
+   uint64_t* f = fs[kCoop ? 0 : wave];
+   const uint32_t* fw = reinterpret_cast<const uint32_t*>(f);
    const Desc* __restrict__ desc = col.vert.desc;
-   // each wavefront takes its own chunks of the group; no block-wide barrier until the end
-   for (uint32_t k = wave;; k += kMutWaves) {
-      const uint32_t c = g + k * n_groups;
-      if (c >= n_chunks) break;
-      if (filter_chunk_cards[c] == 0) continue;
-      const uint32_t tile = b * n_chunks + c;
-      const uint32_t l0 = col.light_ptr[tile], l1 = col.light_ptr[tile + 1];
-      const uint32_t h0 = col.heavy_ptr[tile], h1 = col.heavy_ptr[tile + 1];
-      const uint32_t g0 = col.group_ptr[tile], g1 = col.group_ptr[tile + 1];
-      if (l0 == l1 && h0 == h1 && g0 == g1) continue;
-      {
-         // stage the 8 KiB filter tile: eight independent 16-byte loads per lane
-         const ulonglong2* src = reinterpret_cast<const ulonglong2*>(filter + static_cast<size_t>(c) * kChunkWords);
-         ulonglong2 w[8];
-#pragma unroll
-         for (uint32_t u = 0; u < 8; ++u) w[u] = src[lane + 64 * u];
-#pragma unroll
-         for (uint32_t u = 0; u < 8; ++u) {
-            f[2 * (lane + 64 * u)] = w[u].x;
-            f[2 * (lane + 64 * u) + 1] = w[u].y;
-         }
-      }
-      waveSync();
-      // light entries: each lane walks a contiguous span of 16-byte groups, loading kUnroll groups
-      // before using them; consecutive entries of a slot are summed before one LDS atomic
-      {
-         const uint4* e4 = reinterpret_cast<const uint4*>(col.light + l0);
-         const uint32_t n4 = (l1 - l0) >> 2;
-         uint32_t cur = kPadSlot, acc = 0;
-         auto take = [&](uint32_t x) {
-            const uint32_t slot = x >> 16;
-            const uint32_t row = x & 0xFFFFu;
-            if (slot != cur) {
-               if (acc) addCount(cur, acc);
-               cur = slot;
-               acc = 0;
-            }
-            acc += bitAt(f, row);
-         };
-         for (uint32_t base = lane * kUnroll; base < n4; base += 64 * kUnroll) {
-            uint4 e[kUnroll];
-#pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) {
-               e[u] = base + u < n4 ? e4[base + u] : make_uint4(kPadSlot << 16, kPadSlot << 16, kPadSlot << 16, kPadSlot << 16);
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) {
-               take(e[u].x);
-               take(e[u].y);
-               take(e[u].z);
-               take(e[u].w);
-            }
-         }
-         if (acc) addCount(cur, acc);
-      }
-      // larger arrays: groups of eight rows sharing one slot; lanes walk contiguous spans of groups
-      {
-         const uint4* gr = reinterpret_cast<const uint4*>(col.groups) + g0;
-         const uint16_t* gh = col.group_hdr + g0;
-         const uint32_t ng = g1 - g0;
-         uint32_t cur = kPadSlot, acc = 0;
-         for (uint32_t base = lane * kUnroll; base < ng; base += 64 * kUnroll) {
-            uint4 rows[kUnroll];
-            uint32_t hdr[kUnroll];
-#pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) {
-               const bool ok = base + u < ng;
-               rows[u] = ok ? gr[base + u] : make_uint4(0, 0, 0, 0);
-               hdr[u] = ok ? gh[base + u] : 0xFFFFu;
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) {
-               if (hdr[u] == 0xFFFFu) continue;
-               const uint32_t slot = hdr[u] & 0xFFFu;
-               const uint32_t m = (hdr[u] >> 12) + 1;
-               const uint32_t r[8] = {rows[u].x & 0xFFFFu, rows[u].x >> 16, rows[u].y & 0xFFFFu, rows[u].y >> 16,
-                                      rows[u].z & 0xFFFFu, rows[u].z >> 16, rows[u].w & 0xFFFFu, rows[u].w >> 16};
-               uint32_t n = 0;
-#pragma unroll
-               for (uint32_t q = 0; q < 8; ++q) n += q < m ? bitAt(f, r[q]) : 0u;
-               if (slot != cur) {
-                  if (acc) addCount(cur, acc);
-                  cur = slot;
-                  acc = 0;
+   const GlobalU32* light_ptr = toGlobal<GlobalU32>(col.light_ptr);
+   const GlobalU32* group_ptr = toGlobal<GlobalU32>(col.group_ptr);
+   const GlobalU32* heavy_ptr = toGlobal<GlobalU32>(col.heavy_ptr);
+   auto countTile = [&](uint32_t l0, uint32_t l1, uint32_t g0, uint32_t g1, uint32_t h0, uint32_t h1) {
+         // light entries: each lane walks a contiguous span of kUnroll / 2 16-byte groups per round;
+         // the next round's groups are loaded before this round's are used (kUnroll in flight), all filter lookups of a round
+         // are issued before any result is used, and consecutive entries of a slot are summed before
+         // one LDS atomic.  The round count is uniform over the wavefront; places past the end of the
+         // tile re-read its last group and count nothing.
+         {
+            const GlobalU32x4* e4 = toGlobal<GlobalU32x4>(col.light + l0);
+            const uint32_t n4 = (l1 - l0) >> 2;
+            constexpr uint32_t kRound = kUnroll / 2;
+            constexpr uint32_t kStride = kThreads * kRound;
+            const uint32_t rounds = (n4 + kStride - 1) / kStride;
+            uint32_t cur = kPadSlot, acc = 0;
+            u32x4 bufA[kRound], bufB[kRound];
+            auto load = [&](uint32_t round, u32x4* buf) {
+               const uint32_t base = round * kStride + me * kRound;
+   #pragma unroll
+               for (uint32_t u = 0; u < kRound; ++u) {
+                  const uint32_t j = base + u;
+                  buf[u] = e4[j < n4 ? j : n4 - 1];
                }
-               acc += n;
-            }
-         }
-         if (acc) addCount(cur, acc);
-      }
-      // bitset and run containers: one per wavefront
-      for (uint32_t d = h0; d < h1; ++d) {
-         const Desc dd = desc[d];
-         const uint32_t type = descType(dd.w0);
-         const uint32_t card = descCard(dd.w0);
-         uint32_t n = 0;
-         if (type == CT_BITSET) {
-            const ulonglong2* bits = reinterpret_cast<const ulonglong2*>(col.vert.bits + static_cast<size_t>(dd.w1) * kChunkWords);
-            ulonglong2 w[8];
-#pragma unroll
-            for (uint32_t u = 0; u < 8; ++u) w[u] = bits[lane + 64 * u];
-#pragma unroll
-            for (uint32_t u = 0; u < 8; ++u) {
-               const uint32_t j = lane + 64 * u;
-               n += __popcll(w[u].x & f[2 * j]) + __popcll(w[u].y & f[2 * j + 1]);
-            }
-         } else if (type == CT_ARRAY) {
-            const uint16_t* a = col.vert.arr + dd.w1;
-            for (uint32_t j0 = lane; j0 < card; j0 += 256) {
-               uint32_t v[4];
-#pragma unroll
-               for (uint32_t u = 0; u < 4; ++u) v[u] = j0 + 64 * u < card ? a[j0 + 64 * u] : 0xFFFFFFFFu;
-#pragma unroll
-               for (uint32_t u = 0; u < 4; ++u) {
-                  if (v[u] != 0xFFFFFFFFu) n += bitAt(f, v[u]);
+            };
+            auto use = [&](uint32_t round, const u32x4* buf) {
+               const uint32_t base = round * kStride + me * kRound;
+               uint32_t word[4 * kRound];
+   #pragma unroll
+               for (uint32_t u = 0; u < kRound; ++u) {
+   #pragma unroll
+                  for (uint32_t k = 0; k < 4; ++k) word[4 * u + k] = fw[(buf[u][k] & 0xFFFFu) >> 5];
+               }
+   #pragma unroll
+               for (uint32_t u = 0; u < kRound; ++u) {
+                  const uint32_t valid = base + u < n4 ? 1u : 0u;
+   #pragma unroll
+                  for (uint32_t k = 0; k < 4; ++k) {
+                     const uint32_t x = buf[u][k];
+                     const uint32_t slot = x >> 16;
+                     if (slot != cur) {
+                        if (acc) addCount(cur, acc);
+                        cur = slot;
+                        acc = 0;
+                     }
+                     acc += (word[4 * u + k] >> (x & 31u)) & valid;
+                  }
+               }
+            };
+            if (rounds > 0) {
+               load(0, bufA);
+               for (uint32_t round = 0; round < rounds; round += 2) {
+                  if (round + 1 < rounds) load(round + 1, bufB);
+                  use(round, bufA);
+                  if (round + 1 < rounds) {
+                     if (round + 2 < rounds) load(round + 2, bufA);
+                     use(round + 1, bufB);
+                  }
                }
             }
-         } else if (type == CT_RUN) {
-            const uint32_t* r = col.vert.runs + dd.w1;
-            for (uint32_t j = lane; j < card; j += 64) {
-               const uint32_t st = r[j] & 0xFFFFu;
-               n += rangePopcount(f, st, st + (r[j] >> 16));
-            }
-         } else if (lane == 0) {
-            for (uint32_t j = 0; j < card; ++j) n += bitAt(f, (dd.w1 >> (16 * j)) & 0xFFFFu);
+            if (acc) addCount(cur, acc);
          }
-         n = waveSum(n);
-         if (lane == 0 && n) {
-            if (kPacked) {
-               atomicAdd(&heavy[s0 + descSlot(dd.w0)], n);
-            } else {
-               atomicAdd(&cnt[descSlot(dd.w0)], n);
+         // larger arrays: groups of eight rows sharing one slot, walked like the light entries
+         {
+            const GlobalU32x4* gr = toGlobal<GlobalU32x4>(col.groups) + g0;
+            const GlobalU16* gh = toGlobal<GlobalU16>(col.group_hdr) + g0;
+            const uint32_t ng = g1 - g0;
+            constexpr uint32_t kRound = kUnroll / 2;
+            constexpr uint32_t kStride = kThreads * kRound;
+            const uint32_t rounds = (ng + kStride - 1) / kStride;
+            uint32_t cur = kPadSlot, acc = 0;
+            u32x4 rowsA[kRound], rowsB[kRound];
+            uint32_t hdrA[kRound], hdrB[kRound];
+            auto load = [&](uint32_t round, u32x4* rows, uint32_t* hdr) {
+               const uint32_t base = round * kStride + me * kRound;
+   #pragma unroll
+               for (uint32_t u = 0; u < kRound; ++u) {
+                  const uint32_t j = base + u < ng ? base + u : ng - 1;
+                  rows[u] = gr[j];
+                  hdr[u] = gh[j];
+               }
+            };
+            auto use = [&](uint32_t round, const u32x4* rows, const uint32_t* hdr) {
+               const uint32_t base = round * kStride + me * kRound;
+               // lookups four groups (32 rows) at a time, to bound the registers they hold
+   #pragma unroll
+               for (uint32_t u0 = 0; u0 < kRound; u0 += 4) {
+               uint32_t word[32];
+   #pragma unroll
+               for (uint32_t u = u0; u < u0 + 4; ++u) {
+   #pragma unroll
+                  for (uint32_t q = 0; q < 8; ++q) {
+                     const uint32_t r = (q & 1u) ? rows[u][q >> 1] >> 16 : rows[u][q >> 1] & 0xFFFFu;
+                     word[8 * (u - u0) + q] = fw[r >> 5];
+                  }
+               }
+   #pragma unroll
+               for (uint32_t u = u0; u < u0 + 4; ++u) {
+                  const uint32_t h = base + u < ng ? hdr[u] : 0xFFFFu;
+                  const uint32_t slot = h & 0xFFFu;
+                  const uint32_t m = h == 0xFFFFu ? 0u : (h >> 12) + 1;
+                  uint32_t n = 0;
+   #pragma unroll
+                  for (uint32_t q = 0; q < 8; ++q) {
+                     const uint32_t r = (q & 1u) ? rows[u][q >> 1] >> 16 : rows[u][q >> 1];
+                     n += q < m ? (word[8 * (u - u0) + q] >> (r & 31u)) & 1u : 0u;
+                  }
+                  if (slot != cur) {
+                     if (acc) addCount(cur, acc);
+                     cur = slot;
+                     acc = 0;
+                  }
+                  acc += n;
+               }
+               }
+            };
+            if (rounds > 0) {
+               load(0, rowsA, hdrA);
+               for (uint32_t round = 0; round < rounds; round += 2) {
+                  if (round + 1 < rounds) load(round + 1, rowsB, hdrB);
+                  use(round, rowsA, hdrA);
+                  if (round + 1 < rounds) {
+                     if (round + 2 < rounds) load(round + 2, rowsA, hdrA);
+                     use(round + 1, rowsB, hdrB);
+                  }
+               }
+            }
+            if (acc) addCount(cur, acc);
+         }
+         // bitset and run containers: one per wavefront
+         for (uint32_t d = h0 + (kCoop ? wave : 0); d < h1; d += kCoop ? kMutWaves : 1) {
+            const u32x2 dw = toGlobal<GlobalU32x2>(desc)[d];
+            const Desc dd{dw[0], dw[1]};
+            const uint32_t type = descType(dd.w0);
+            const uint32_t card = descCard(dd.w0);
+            uint32_t n = 0;
+            if (type == CT_BITSET) {
+               const GlobalU64x2* bits = toGlobal<GlobalU64x2>(col.vert.bits + static_cast<size_t>(dd.w1) * kChunkWords);
+               u64x2 w[8];
+   #pragma unroll
+               for (uint32_t u = 0; u < 8; ++u) w[u] = bits[lane + 64 * u];
+   #pragma unroll
+               for (uint32_t u = 0; u < 8; ++u) {
+                  const uint32_t j = lane + 64 * u;
+                  n += __popcll(w[u][0] & f[2 * j]) + __popcll(w[u][1] & f[2 * j + 1]);
+               }
+            } else if (type == CT_ARRAY) {
+               const GlobalU16* a = toGlobal<GlobalU16>(col.vert.arr + dd.w1);
+               for (uint32_t j0 = lane; j0 < card; j0 += 256) {
+                  uint32_t v[4];
+   #pragma unroll
+                  for (uint32_t u = 0; u < 4; ++u) v[u] = j0 + 64 * u < card ? a[j0 + 64 * u] : 0xFFFFFFFFu;
+   #pragma unroll
+                  for (uint32_t u = 0; u < 4; ++u) {
+                     if (v[u] != 0xFFFFFFFFu) n += bitAt(f, v[u]);
+                  }
+               }
+            } else if (type == CT_RUN) {
+               const GlobalU32* r = toGlobal<GlobalU32>(col.vert.runs + dd.w1);
+               for (uint32_t j = lane; j < card; j += 64) {
+                  const uint32_t st = r[j] & 0xFFFFu;
+                  n += rangePopcount(f, st, st + (r[j] >> 16));
+               }
+            } else if (lane == 0) {
+               for (uint32_t j = 0; j < card; ++j) n += bitAt(f, (dd.w1 >> (16 * j)) & 0xFFFFu);
+            }
+            n = waveSum(n);
+            if (lane == 0 && n) {
+               if (kPacked) {
+                  atomicAdd(&heavy[s0 + descSlot(dd.w0)], n);
+               } else {
+                  atomicAdd(&cnt[descSlot(dd.w0)], n);
+               }
             }
          }
+   };
+   if constexpr (kCoop) {
+      // chunks g, g + n_groups, ... with rows of the filter and a non-empty tile, in order
+      auto next = [&](uint32_t c) -> uint32_t {
+         for (; c < n_chunks; c += n_groups) {
+            if (filter_chunk_cards[c] == 0) continue;
+            const uint32_t tile = b * n_chunks + c;
+            if (light_ptr[tile] != light_ptr[tile + 1] || group_ptr[tile] != group_ptr[tile + 1] ||
+                heavy_ptr[tile] != heavy_ptr[tile + 1]) {
+               break;
+            }
+         }
+         return c;
+      };
+      // the next chunk's filter tile is loaded (two 16-byte loads per thread) while the current
+      // one is counted
+      u64x2 pf[2];
+      auto fetch = [&](uint32_t c) {
+         const GlobalU64x2* src = toGlobal<GlobalU64x2>(filter + static_cast<size_t>(c) * kChunkWords);
+         pf[0] = src[tid];
+         pf[1] = src[tid + kMutThreads];
+      };
+      uint32_t c = next(g);
+      if (c < n_chunks) fetch(c);
+      while (c < n_chunks) {
+         __syncthreads();  // the previous tile's readers are done
+         reinterpret_cast<u64x2*>(f)[tid] = pf[0];
+         reinterpret_cast<u64x2*>(f)[tid + kMutThreads] = pf[1];
+         const uint32_t tile = b * n_chunks + c;
+         const uint32_t l0 = light_ptr[tile], l1 = light_ptr[tile + 1];
+         const uint32_t h0 = heavy_ptr[tile], h1 = heavy_ptr[tile + 1];
+         const uint32_t g0 = group_ptr[tile], g1 = group_ptr[tile + 1];
+         const uint32_t cn = next(c + n_groups);
+         if (cn < n_chunks) fetch(cn);
+         __syncthreads();
+         countTile(l0, l1, g0, g1, h0, h1);
+         c = cn;
       }
-      waveSync();
+   } else {
+      // each wavefront takes its own chunks of the group; no block-wide barrier until the end
+      for (uint32_t k = wave;; k += kMutWaves) {
+         const uint32_t c = g + k * n_groups;
+         if (c >= n_chunks) break;
+         if (filter_chunk_cards[c] == 0) continue;
+         const uint32_t tile = b * n_chunks + c;
+         const uint32_t l0 = light_ptr[tile], l1 = light_ptr[tile + 1];
+         const uint32_t h0 = heavy_ptr[tile], h1 = heavy_ptr[tile + 1];
+         const uint32_t g0 = group_ptr[tile], g1 = group_ptr[tile + 1];
+         if (l0 == l1 && h0 == h1 && g0 == g1) continue;
+         {
+            // stage the 8 KiB filter tile: eight independent 16-byte loads per lane
+            const GlobalU64x2* src = toGlobal<GlobalU64x2>(filter + static_cast<size_t>(c) * kChunkWords);
+            u64x2 w[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) w[u] = src[lane + 64 * u];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) reinterpret_cast<u64x2*>(f)[lane + 64 * u] = w[u];
+         }
+         waveSync();
+         countTile(l0, l1, g0, g1, h0, h1);
+         waveSync();
+      }
    }
    __syncthreads();
    for (uint32_t i = tid; i < ns; i += kMutThreads) {
@@ -704,8 +826,8 @@ __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b,
 }
 
 
-template <uint32_t kUnroll, bool kPacked>
-__global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqColumnView* __restrict__ colp,
+template <uint32_t kUnroll, bool kPacked, bool kCoop>
+__global__ __launch_bounds__(kMutThreads) __attribute__((amdgpu_waves_per_eu(kPacked || kCoop ? 4 : 3))) void mutationsCountKernel(const SeqColumnView* __restrict__ colp,
                                                                     uint32_t n_chunks, uint32_t n_groups,
                                                                     const uint64_t* __restrict__ filter,
                                                                     const uint32_t* __restrict__ filter_chunk_cards,
@@ -713,19 +835,14 @@ __global__ __launch_bounds__(kMutThreads) void mutationsCountKernel(const SeqCol
                                                                     long long sparse_max,
                                                                     uint32_t* __restrict__ partial,
                                                                     uint32_t* __restrict__ heavy) {
-   __shared__ uint64_t fs[kMutWaves][kChunkWords];
+   __shared__ uint64_t fs[kCoop ? 1 : kMutWaves][kChunkWords];
    __shared__ uint32_t cnt[kPacked ? (kMaxBlockSlots + 1) / 2 : kMaxBlockSlots + 1];
    if (!denseActive(filter_card, sparse_max)) return;
    const SeqColumnView& col = *colp;
-   countBlock<kUnroll, kPacked>(col, blockIdx.x, blockIdx.y, n_chunks, n_groups, filter, filter_chunk_cards,
+   countBlock<kUnroll, kPacked, kCoop>(col, blockIdx.x, blockIdx.y, n_chunks, n_groups, filter, filter_chunk_cards,
                                 partial + static_cast<size_t>(blockIdx.y) * col.n_slots, fs, cnt, heavy);
 }
 
-__device__ inline uint32_t slotValue(const uint32_t* partial, uint32_t n_groups, uint32_t n_slots, uint32_t s) {
-   uint32_t v = 0;
-   for (uint32_t g = 0; g < n_groups; ++g) v += partial[static_cast<size_t>(g) * n_slots + s];
-   return v;
-}
 
 // ------------------------------------------------------------------------------------------------
 // Sparse Mutations counting: each filter row adds its own stored mutations (row-major CSR in the
@@ -788,43 +905,84 @@ __global__ __launch_bounds__(256) void reducePartialsKernel(uint32_t* __restrict
    partial[s] = v;
 }
 
-// Sources of the per-position quantities for the totals and finalize kernels.
-//  * SlotSource: per-slot partial sums of the dense kernel; the implicit symbol (rows covered at p
-//    with no stored entry) is the local reference.
+// Values of one block of kPosBlock positions, staged in LDS by the totals and finalize kernels:
+// stored counts per symbol and the boundary quantities
+//   a: rows of the filter whose covered range starts at p (p > 0),
+//   b: rows whose covered range ends at p,
+//   c: internal missing runs starting at p minus those ending at p.
+constexpr uint32_t kMaxSymbols = 28;
+struct BlockValues {
+   uint32_t v[kMaxSymbols][kPosBlock];
+   uint32_t a[kPosBlock];
+   uint32_t b[kPosBlock];
+   int32_t c[kPosBlock];
+};
+
+// Sources of the per-position quantities for the totals and finalize kernels; stage() fills a
+// BlockValues for positions [p0, p0 + kPosBlock) and is called by the whole work-group.
+//  * SlotSource: per-slot sums of the dense kernel (one thread per slot of the block); the implicit
+//    symbol (rows covered at p with no stored entry) is the local reference.
 //  * SparseSource: per-(symbol, position) counts of the sparse kernel in the original-reference
-//    frame; the implicit symbol is the original reference.
+//    frame (one thread per position); the implicit symbol is the original reference.  With
+//    `clear`, the counts read are zeroed for the next query.
 struct SlotSource {
    const SeqColumnView* col;
    const uint32_t* partial;
-   uint32_t n_groups;
 
-   __device__ void boundary(uint32_t p, uint32_t& A, uint32_t& B, int32_t& C) const {
-      A = 0;
-      B = 0;
-      C = 0;
-      const uint32_t e = col->pos_slot_ptr[p + 1];
-      for (uint32_t s = col->pos_slot_ptr[p]; s < e; ++s) {
-         const uint32_t sym = col->slot_info[s] >> 16;
-         if (sym < col->n_symbols) continue;
-         const uint32_t v = slotValue(partial, n_groups, col->n_slots, s);
-         switch (sym - col->n_symbols) {
-            case BND_COV_START: A += v; break;
-            case BND_COV_END: B += v; break;
-            case BND_N_START: C += static_cast<int32_t>(v); break;
-            default: C -= static_cast<int32_t>(v); break;
+   __device__ void stage(uint32_t p0, BlockValues& bv, bool) const {
+      const uint32_t t = threadIdx.x;
+      const uint32_t ns = col->n_symbols;
+      for (uint32_t k = 0; k < ns; ++k) bv.v[k][t] = 0;
+      bv.a[t] = 0;
+      bv.b[t] = 0;
+      bv.c[t] = 0;
+      __syncthreads();
+      const uint32_t L = col->genome_length;
+      const GlobalU32* pos_ptr = toGlobal<GlobalU32>(col->pos_slot_ptr);
+      const GlobalU32* info = toGlobal<GlobalU32>(col->slot_info);
+      const GlobalU32* part = toGlobal<GlobalU32>(partial);
+      const uint32_t s0 = pos_ptr[p0];
+      const uint32_t s1 = pos_ptr[p0 + kPosBlock < L ? p0 + kPosBlock : L];
+      for (uint32_t s = s0 + t; s < s1; s += kPosBlock) {
+         const uint32_t x = info[s];
+         const uint32_t v = part[s];
+         const uint32_t pl = (x & 0xFFFFu) - p0;
+         const uint32_t sym = x >> 16;
+         if (sym < ns) {
+            bv.v[sym][pl] = v;
+         } else {
+            switch (sym - ns) {
+               case BND_COV_START: bv.a[pl] = v; break;
+               case BND_COV_END: bv.b[pl] = v; break;
+               case BND_N_START: atomicAdd(&bv.c[pl], static_cast<int32_t>(v)); break;
+               default: atomicSub(&bv.c[pl], static_cast<int32_t>(v)); break;
+            }
+         }
+      }
+      __syncthreads();
+   }
+   __device__ uint32_t implicitSymbol(uint32_t p) const { return col->local_ref[p]; }
+   // this thread's share of the block sums of a, b and c (boundary slots are the last of a position)
+   __device__ void blockSums(uint32_t p0, int64_t& a, int64_t& b, int64_t& c) const {
+      const uint32_t L = col->genome_length;
+      const uint32_t ns = col->n_symbols;
+      const GlobalU32* pos_ptr = toGlobal<GlobalU32>(col->pos_slot_ptr);
+      const GlobalU32* info = toGlobal<GlobalU32>(col->slot_info);
+      const GlobalU32* part = toGlobal<GlobalU32>(partial);
+      const uint32_t s0 = pos_ptr[p0];
+      const uint32_t s1 = pos_ptr[p0 + kPosBlock < L ? p0 + kPosBlock : L];
+      for (uint32_t s = s0 + threadIdx.x; s < s1; s += kPosBlock) {
+         const uint32_t sym = info[s] >> 16;
+         if (sym < ns) continue;
+         const int64_t v = part[s];
+         switch (sym - ns) {
+            case BND_COV_START: a += v; break;
+            case BND_COV_END: b += v; break;
+            case BND_N_START: c += v; break;
+            default: c -= v; break;
          }
       }
    }
-   __device__ uint32_t cursor(uint32_t p) const { return col->pos_slot_ptr[p]; }
-   // stored count of symbol `sym` at p; symbols must be visited in increasing order
-   __device__ uint32_t stored(uint32_t p, uint32_t sym, uint32_t& cur) const {
-      if (cur < col->pos_slot_ptr[p + 1] && (col->slot_info[cur] >> 16) == sym) {
-         return slotValue(partial, n_groups, col->n_slots, cur++);
-      }
-      return 0;
-   }
-   __device__ uint32_t implicitSymbol(uint32_t p) const { return col->local_ref[p]; }
-   __device__ void clear(uint32_t) const {}
 };
 
 struct SparseSource {
@@ -832,23 +990,41 @@ struct SparseSource {
    uint32_t* counts;
    int32_t* bnd;
 
-   __device__ void boundary(uint32_t p, uint32_t& A, uint32_t& B, int32_t& C) const {
+   __device__ void stage(uint32_t p0, BlockValues& bv, bool clear) const {
+      const uint32_t t = threadIdx.x;
       const uint32_t L = col->genome_length;
-      A = static_cast<uint32_t>(bnd[p]);
-      B = static_cast<uint32_t>(bnd[L + p]);
-      C = bnd[2 * L + p];
-   }
-   __device__ uint32_t cursor(uint32_t) const { return 0; }
-   __device__ uint32_t stored(uint32_t p, uint32_t sym, uint32_t&) const {
-      return counts[static_cast<size_t>(sym) * col->genome_length + p];
+      const uint32_t p = p0 + t;
+      const uint32_t ns = col->n_symbols;
+      if (p < L) {
+         for (uint32_t k = 0; k < ns; ++k) {
+            bv.v[k][t] = counts[static_cast<size_t>(k) * L + p];
+            if (clear) counts[static_cast<size_t>(k) * L + p] = 0;
+         }
+         bv.a[t] = static_cast<uint32_t>(bnd[p]);
+         bv.b[t] = static_cast<uint32_t>(bnd[L + p]);
+         bv.c[t] = bnd[2 * L + p];
+         if (clear) {
+            bnd[p] = 0;
+            bnd[L + p] = 0;
+            bnd[2 * L + p] = 0;
+         }
+      } else {
+         for (uint32_t k = 0; k < ns; ++k) bv.v[k][t] = 0;
+         bv.a[t] = 0;
+         bv.b[t] = 0;
+         bv.c[t] = 0;
+      }
+      __syncthreads();
    }
    __device__ uint32_t implicitSymbol(uint32_t p) const { return col->reference[p]; }
-   __device__ void clear(uint32_t p) const {
+   __device__ void blockSums(uint32_t p0, int64_t& a, int64_t& b, int64_t& c) const {
       const uint32_t L = col->genome_length;
-      for (uint32_t s = 0; s < col->n_symbols; ++s) counts[static_cast<size_t>(s) * L + p] = 0;
-      bnd[p] = 0;
-      bnd[L + p] = 0;
-      bnd[2 * L + p] = 0;
+      const uint32_t p = p0 + threadIdx.x;
+      if (p < L) {
+         a += static_cast<uint32_t>(bnd[p]);
+         b += static_cast<uint32_t>(bnd[L + p]);
+         c += bnd[2 * L + p];
+      }
    }
 };
 
@@ -888,19 +1064,12 @@ __device__ inline T blockInclusiveScan(T v, T* tmp) {
    return v + prefix;
 }
 
-// Per kPosBlock-position block: totals of the three boundary quantities
-//   A(p) = rows of the filter whose covered range starts at p (p > 0),
-//   B(p) = rows whose covered range ends at p,
-//   C(p) = internal missing runs starting at p minus those ending at p.
+// Per kPosBlock-position block: totals of the three boundary quantities a, b and c.
 template <typename Src>
 __device__ __forceinline__ void totalsBody(const Src& src, uint32_t pblock, int64_t* __restrict__ totals) {
    __shared__ int64_t tmp[3][kPosWaves];
-   const SeqColumnView& col = *src.col;
-   const uint32_t p = pblock * kPosBlock + threadIdx.x;
-   uint32_t A = 0, B = 0;
-   int32_t C = 0;
-   if (p < col.genome_length) src.boundary(p, A, B, C);
-   const int64_t v[3] = {A, B, C};
+   int64_t v[3] = {0, 0, 0};
+   src.blockSums(pblock * kPosBlock, v[0], v[1], v[2]);
    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
    for (int q = 0; q < 3; ++q) {
       const int64_t x = waveTotal(v[q]);
@@ -928,9 +1097,10 @@ __device__ __forceinline__ void finalizeBody(const Src& src, uint32_t pblock, co
                                              uint32_t n_pblocks, uint64_t filter_card, double min_proportion,
                                              uint32_t* __restrict__ counts_out, MutationRowDev* __restrict__ rows,
                                              uint32_t row_capacity, uint32_t* __restrict__ n_rows) {
+   __shared__ BlockValues bv;
    __shared__ int64_t scan_tmp[kPosWaves];
    __shared__ int64_t part[4][kPosWaves];
-   __shared__ int64_t prefix[4];  // previous-block sums of A, B, C and the total of A
+   __shared__ int64_t prefix[4];  // previous-block sums of a, b, c and the total of a
    const SeqColumnView& col = *src.col;
    {
       // block totals: thread k < n_pblocks (<= kPosBlock for genome lengths below 2^16) takes block k
@@ -956,46 +1126,38 @@ __device__ __forceinline__ void finalizeBody(const Src& src, uint32_t pblock, co
          for (int w = 0; w < kPosWaves; ++w) sum += part[threadIdx.x][w];
          prefix[threadIdx.x] = sum;
       }
-      __syncthreads();
    }
+   src.stage(pblock * kPosBlock, bv, true);  // ends with a barrier, which also publishes prefix
    const uint32_t L = col.genome_length;
-   const uint32_t p = pblock * kPosBlock + threadIdx.x;
-   uint32_t A = 0, B = 0;
-   int32_t C = 0;
-   if (p < L) src.boundary(p, A, B, C);
-   const int64_t pa = blockInclusiveScan<int64_t>(A, scan_tmp) + prefix[0];
-   const int64_t pb = blockInclusiveScan<int64_t>(B, scan_tmp) + prefix[1];
-   const int64_t pc = blockInclusiveScan<int64_t>(C, scan_tmp) + prefix[2];
+   const uint32_t t = threadIdx.x;
+   const uint32_t p = pblock * kPosBlock + t;
+   const int64_t pa = blockInclusiveScan<int64_t>(bv.a[t], scan_tmp) + prefix[0];
+   const int64_t pb = blockInclusiveScan<int64_t>(bv.b[t], scan_tmp) + prefix[1];
+   const int64_t pc = blockInclusiveScan<int64_t>(bv.c[t], scan_tmp) + prefix[2];
    if (p >= L) return;
+   const uint32_t ns = col.n_symbols;
    const int64_t card = static_cast<int64_t>(filter_card);
    // rows covered at p: not starting after p, not ended at or before p, not inside a missing run
    const int64_t covered = card - (prefix[3] - pa) - pb - pc;
    int64_t stored = 0;
-   {
-      uint32_t cur = src.cursor(p);
-      for (uint32_t sym = 0; sym < col.n_symbols; ++sym) stored += src.stored(p, sym, cur);
-   }
+   for (uint32_t sym = 0; sym < ns; ++sym) stored += bv.v[sym][t];
    const uint32_t implicit_count = card == 0 ? 0u : static_cast<uint32_t>(covered - stored);
    const uint32_t implicit = src.implicitSymbol(p);
    // per-symbol counts in symbol order
    uint32_t total = 0;
-   {
-      uint32_t cur = src.cursor(p);
-      for (uint32_t sym = 0; sym < col.n_symbols; ++sym) {
-         uint32_t v = src.stored(p, sym, cur);
-         if (sym == implicit) v += implicit_count;
-         if (counts_out) counts_out[static_cast<size_t>(sym) * L + p] = v;
-         if ((col.valid_mask >> sym) & 1u) total += v;
-      }
+   for (uint32_t sym = 0; sym < ns; ++sym) {
+      uint32_t v = bv.v[sym][t];
+      if (sym == implicit) v += implicit_count;
+      if (counts_out) counts_out[static_cast<size_t>(sym) * L + p] = v;
+      if ((col.valid_mask >> sym) & 1u) total += v;
    }
    if (total != 0 && rows != nullptr) {
       const uint32_t thr = min_proportion == 0.0
                               ? 0u
                               : static_cast<uint32_t>(ceil(static_cast<double>(total) * min_proportion) - 1.0);
       const uint32_t orig = col.reference[p];
-      uint32_t cur = src.cursor(p);
-      for (uint32_t sym = 0; sym < col.n_symbols; ++sym) {
-         uint32_t v = src.stored(p, sym, cur);
+      for (uint32_t sym = 0; sym < ns; ++sym) {
+         uint32_t v = bv.v[sym][t];
          if (sym == implicit) v += implicit_count;
          if (!((col.valid_mask >> sym) & 1u) || sym == orig) continue;
          if (v > thr) {
@@ -1004,7 +1166,6 @@ __device__ __forceinline__ void finalizeBody(const Src& src, uint32_t pblock, co
          }
       }
    }
-   src.clear(p);
 }
 
 template <typename Src>
@@ -1046,9 +1207,9 @@ __global__ __launch_bounds__(1024) void batchPlanKernel(MutationsBatch m) {
 // Work-group w handles position block b of dense list entry i.  Blocks are dealt to the eight XCDs
 // (work-group w runs on XCD w % 8) so that the entries sharing a block run on one XCD, close in
 // time, and its L2 serves the block's tiles to all but the first of them.
-template <uint32_t kUnroll>
-__global__ __launch_bounds__(kMutThreads) void mutationsCountBatchKernel(MutationsBatch m) {
-   __shared__ uint64_t fs[kMutWaves][kChunkWords];
+template <uint32_t kUnroll, bool kCoop>
+__global__ __launch_bounds__(kMutThreads) __attribute__((amdgpu_waves_per_eu(kCoop ? 4 : 3))) void mutationsCountBatchKernel(MutationsBatch m) {
+   __shared__ uint64_t fs[kCoop ? 1 : kMutWaves][kChunkWords];
    __shared__ uint32_t cnt[kMaxBlockSlots + 1];
    const uint32_t n_dense = m.list_counts[0];
    if (n_dense == 0) return;
@@ -1060,55 +1221,73 @@ __global__ __launch_bounds__(kMutThreads) void mutationsCountBatchKernel(Mutatio
    if (b >= col.n_blocks) return;
    const uint32_t e = i % n_dense;
    const uint32_t q = m.lists[e];
-   countBlock<kUnroll, false>(col, b, 0, m.n_chunks, 1, m.filters + static_cast<size_t>(q) * m.n_chunks * kChunkWords,
+   countBlock<kUnroll, false, kCoop>(col, b, 0, m.n_chunks, 1, m.filters + static_cast<size_t>(q) * m.n_chunks * kChunkWords,
                        m.chunk_cards + static_cast<size_t>(q) * m.n_chunks, m.partial + static_cast<size_t>(e) * col.n_slots,
                               fs, cnt, nullptr);
 }
 
 // ---- shared scan: four dense queries per pass over the index --------------------------------------
-// Work-group (block b, quad j) streams block b's tiles of every chunk once for the dense list
-// entries 4j .. 4j+3.  Per chunk, the four filters' words are interleaved into a nibble per row
-// (bit q: the row is in entry 4j+q's filter), staged in LDS (32 KiB per chunk); each light entry
-// or group row then costs one LDS byte read, and one 64-bit counter per slot holds the four
-// queries' counts as 16-bit fields (added with one LDS atomic per run of a slot).  A field
-// gathers at most 15 chunks x 4096 rows before the work-group adds it to the query's slot sums
-// (which it owns), so it cannot overflow.  Bitset and run containers (a few thousand) are counted
-// per query from the filters' words and added with global atomics to a second sum array.
-constexpr uint32_t kQuadThreads = 512;
-constexpr uint32_t kQuadFlushChunks = 15;
+// Work-group (block b, quad j, part h) streams block b's tiles of the chunks of part h (first or
+// second half of the chunks) once for the dense list entries 4j .. 4j+3.  Per chunk, the four
+// filters are staged in LDS interleaved by 32-bit word (word i of entry 4j+k at 4 i + k), so that
+// one 16-byte LDS read gives a row's word of all four filters.  Light entries and group rows are
+// counted as in the one-query kernel, into four 16-bit counters per slot held in one u64 (one LDS
+// atomic per run of a slot).  A counter gathers at most kSharedFlushChunks chunks x 4096 rows before
+// the work-group adds it to its part's slot sums (which it owns), so it cannot overflow.  Bitset and
+// run containers are counted per query and added with global atomics to heavy_sums; a second
+// kernel then folds the second part and heavy_sums into partial.  The next chunk's filter words are
+// loaded while the current tile is counted.
+constexpr uint32_t kSharedThreads = 512;
+constexpr uint32_t kSharedFlushChunks = 15;
 
-// bit i of the low 16 bits of x moves to bit 4 i
-__device__ inline uint64_t spread4(uint64_t x) {
-   x &= 0xFFFFull;
-   x = (x | (x << 24)) & 0x000000FF000000FFull;
-   x = (x | (x << 12)) & 0x000F000F000F000Full;
-   x = (x | (x << 6)) & 0x0303030303030303ull;
-   x = (x | (x << 3)) & 0x1111111111111111ull;
-   return x;
+// popcount of bits a..b (inclusive) of query k's words in an interleaved tile
+__device__ inline uint32_t rangePopcountQ(const uint32_t* fq, uint32_t k, uint32_t a, uint32_t b) {
+   uint32_t n = 0;
+   const uint32_t wa = a >> 5, wb = b >> 5;
+   for (uint32_t w = wa; w <= wb; ++w) {
+      const uint32_t x = w == wa ? (a & 31) : 0;
+      const uint32_t y = w == wb ? (b & 31) : 31;
+      const uint32_t mask = (y == 31 ? 0xFFFFFFFFu : ((1u << (y + 1)) - 1u)) & ~((1u << x) - 1u);
+      n += __popc(fq[4 * w + k] & mask);
+   }
+   return n;
 }
 
-// a row's nibble as four 16-bit counters
-__device__ inline uint64_t nibbleCounts(uint32_t n) {
-   return static_cast<uint64_t>(n & 1u) | (static_cast<uint64_t>(n & 2u) << 15) | (static_cast<uint64_t>(n & 4u) << 30) |
-          (static_cast<uint64_t>(n & 8u) << 45);
+// the four filter bits of `row` packed as 16-bit lanes: (bit 0 | bit 1 << 16, bit 2 | bit 3 << 16)
+__device__ inline void quadBits(const u32x4 w, uint32_t row, uint32_t& a01, uint32_t& a23) {
+   const uint32_t sh = row & 31u;
+   a01 += ((w[0] >> sh) & 1u) | (((w[1] >> sh) & 1u) << 16);
+   a23 += ((w[2] >> sh) & 1u) | (((w[3] >> sh) & 1u) << 16);
 }
 
-__device__ inline uint32_t rowNibble(const uint8_t* mask, uint32_t row) {
-   return (static_cast<uint32_t>(mask[row >> 1]) >> ((row & 1u) << 2)) & 0xFu;
-}
-
-template <uint32_t kUnroll>
-__global__ __launch_bounds__(kQuadThreads) void mutationsCountQuadKernel(MutationsBatch m, uint32_t* heavy_sums) {
-   __shared__ __align__(16) uint8_t mask[kChunkRows / 2];
-   __shared__ uint64_t cnt[kMaxBlockSlots + 1];
+template <uint32_t kRound>
+__global__ __launch_bounds__(kSharedThreads) __attribute__((amdgpu_waves_per_eu(4))) void mutationsCountSharedKernel(MutationsBatch m) {
+   __shared__ __align__(16) uint32_t fq[kChunkWords * 2 * 4];
+   __shared__ unsigned long long cnt[kMaxBlockSlots + 1];
+   const u32x4* fq4 = reinterpret_cast<const u32x4*>(fq);
    const uint32_t n_dense = m.list_counts[0];
+   const uint32_t n_quads = (n_dense + 3) / 4;
+   if (n_quads == 0) return;
    const SeqColumnView& col = *m.col_dev;
-   const uint32_t b = blockIdx.x % col.n_blocks;
-   const uint32_t j = blockIdx.x / col.n_blocks;
-   if (4 * j >= n_dense) return;
+   // work-group w runs on XCD w % 8: the work-groups of one block share an XCD and its L2
+   const uint32_t xcd = blockIdx.x & 7u;
+   const uint32_t i = blockIdx.x >> 3;
+   const uint32_t b = xcd + 8u * (i / (2 * n_quads));
+   if (b >= col.n_blocks) return;
+   const uint32_t r = i % (2 * n_quads);
+   const uint32_t j = r >> 1;
+   const uint32_t part = r & 1u;
+   const uint32_t n_chunks = m.n_chunks;
+   const uint32_t n_slots = col.n_slots;
+   const uint32_t half = (n_chunks + 1) / 2;
+   const uint32_t c_end = part ? n_chunks : half;
+   uint32_t* out = part ? m.partial_b : m.partial;
    const uint32_t tid = threadIdx.x;
    const uint32_t lane = tid & 63;
-   const uint32_t n_chunks = m.n_chunks;
+   const uint32_t wave = tid >> 6;
+   const GlobalU32* light_ptr = toGlobal<GlobalU32>(col.light_ptr);
+   const GlobalU32* group_ptr = toGlobal<GlobalU32>(col.group_ptr);
+   const GlobalU32* heavy_ptr = toGlobal<GlobalU32>(col.heavy_ptr);
    const uint32_t s0 = col.block_slot_start[b];
    const uint32_t ns = col.block_slot_start[b + 1] - s0;
    uint32_t q[4];
@@ -1118,172 +1297,265 @@ __global__ __launch_bounds__(kQuadThreads) void mutationsCountQuadKernel(Mutatio
       valid[k] = 4 * j + k < n_dense;
       q[k] = valid[k] ? m.lists[4 * j + k] : 0;
    }
-   for (uint32_t i = tid; i <= kMaxBlockSlots; i += kQuadThreads) cnt[i] = 0;
+   for (uint32_t t = tid; t <= kMaxBlockSlots; t += kSharedThreads) cnt[t] = 0;
    uint32_t since_flush = 0;
    bool first_flush = true;
    auto flush = [&]() {
       __syncthreads();
-      for (uint32_t i = tid; i < ns; i += kQuadThreads) {
-         const uint64_t v = cnt[i];
-         cnt[i] = 0;
+      for (uint32_t t = tid; t < ns; t += kSharedThreads) {
+         const unsigned long long v = cnt[t];
+         cnt[t] = 0;
 #pragma unroll
          for (uint32_t k = 0; k < 4; ++k) {
             if (!valid[k]) continue;
-            uint32_t* out = m.partial + static_cast<size_t>(4 * j + k) * col.n_slots + s0 + i;
+            uint32_t* o = out + static_cast<size_t>(4 * j + k) * n_slots + s0 + t;
             const uint32_t add = static_cast<uint32_t>((v >> (16 * k)) & 0xFFFFu);
-            *out = first_flush ? add : *out + add;
+            *o = first_flush ? add : *o + add;
          }
       }
       first_flush = false;
       since_flush = 0;
       __syncthreads();
    };
-   for (uint32_t c = 0; c < n_chunks; ++c) {
-      bool any = false;
+   // the next chunk at or after c in which one of the four filters has rows and the tile is not empty
+   auto next = [&](uint32_t c) -> uint32_t {
+      for (; c < c_end; ++c) {
+         bool any = false;
 #pragma unroll
-      for (uint32_t k = 0; k < 4; ++k) any |= valid[k] && m.chunk_cards[static_cast<size_t>(q[k]) * n_chunks + c] != 0;
-      if (!any) continue;
-      const uint32_t tile = b * n_chunks + c;
-      const uint32_t l0 = col.light_ptr[tile], l1 = col.light_ptr[tile + 1];
-      const uint32_t h0 = col.heavy_ptr[tile], h1 = col.heavy_ptr[tile + 1];
-      const uint32_t g0 = col.group_ptr[tile], g1 = col.group_ptr[tile + 1];
-      if (l0 == l1 && h0 == h1 && g0 == g1) continue;
-      if (since_flush == kQuadFlushChunks) flush();
-      ++since_flush;
-      __syncthreads();  // the previous chunk's readers of the mask are done
-      // stage: 1024 words x 4 filters -> 64 nibbles (32 bytes) per word
-      for (uint32_t w = tid; w < kChunkWords; w += kQuadThreads) {
-         uint64_t x[4];
+         for (uint32_t k = 0; k < 4; ++k) any |= valid[k] && m.chunk_cards[static_cast<size_t>(q[k]) * n_chunks + c] != 0;
+         if (!any) continue;
+         const uint32_t tile = b * n_chunks + c;
+         if (light_ptr[tile] != light_ptr[tile + 1] || group_ptr[tile] != group_ptr[tile + 1] ||
+             heavy_ptr[tile] != heavy_ptr[tile + 1]) {
+            break;
+         }
+      }
+      return c;
+   };
+   // filter words of chunk c: this thread's words tid and tid + 512 of the four entries
+   uint64_t pf[2][4];
+   auto fetch = [&](uint32_t c) {
+#pragma unroll
+      for (uint32_t h = 0; h < 2; ++h) {
 #pragma unroll
          for (uint32_t k = 0; k < 4; ++k) {
-            x[k] = valid[k] ? m.filters[(static_cast<size_t>(q[k]) * n_chunks + c) * kChunkWords + w] : 0ull;
+            pf[h][k] = valid[k] ? m.filters[(static_cast<size_t>(q[k]) * n_chunks + c) * kChunkWords + tid + h * 512] : 0ull;
          }
-         uint64_t out[4];
-#pragma unroll
-         for (uint32_t h = 0; h < 4; ++h) {
-            out[h] = spread4(x[0] >> (16 * h)) | (spread4(x[1] >> (16 * h)) << 1) | (spread4(x[2] >> (16 * h)) << 2) |
-                     (spread4(x[3] >> (16 * h)) << 3);
-         }
-         uint4* dst = reinterpret_cast<uint4*>(mask + static_cast<size_t>(w) * 32);
-         dst[0] = make_uint4(static_cast<uint32_t>(out[0]), static_cast<uint32_t>(out[0] >> 32), static_cast<uint32_t>(out[1]),
-                             static_cast<uint32_t>(out[1] >> 32));
-         dst[1] = make_uint4(static_cast<uint32_t>(out[2]), static_cast<uint32_t>(out[2] >> 32), static_cast<uint32_t>(out[3]),
-                             static_cast<uint32_t>(out[3] >> 32));
       }
+   };
+   uint32_t c = next(part ? half : 0);
+   if (c < c_end) fetch(c);
+   while (c < c_end) {
+      if (since_flush == kSharedFlushChunks) flush();
+      ++since_flush;
+      __syncthreads();  // the previous tile's readers of fq are done
+#pragma unroll
+      for (uint32_t h = 0; h < 2; ++h) {
+         const uint32_t w = tid + h * 512;
+         u32x4 lo, hi;
+#pragma unroll
+         for (uint32_t k = 0; k < 4; ++k) {
+            lo[k] = static_cast<uint32_t>(pf[h][k]);
+            hi[k] = static_cast<uint32_t>(pf[h][k] >> 32);
+         }
+         reinterpret_cast<u32x4*>(fq)[2 * w] = lo;
+         reinterpret_cast<u32x4*>(fq)[2 * w + 1] = hi;
+      }
+      const uint32_t tile = b * n_chunks + c;
+      const uint32_t l0 = light_ptr[tile], l1 = light_ptr[tile + 1];
+      const uint32_t h0 = heavy_ptr[tile], h1 = heavy_ptr[tile + 1];
+      const uint32_t g0 = group_ptr[tile], g1 = group_ptr[tile + 1];
+      const uint32_t cn = next(c + 1);
+      if (cn < c_end) fetch(cn);
       __syncthreads();
-      // light entries: contiguous spans per thread, runs of a slot summed before one LDS atomic
+      // light entries
       {
-         const uint4* e4 = reinterpret_cast<const uint4*>(col.light + l0);
+         const GlobalU32x4* e4 = toGlobal<GlobalU32x4>(col.light + l0);
          const uint32_t n4 = (l1 - l0) >> 2;
-         uint32_t cur = kPadSlot;
-         uint64_t acc = 0;
-         auto take = [&](uint32_t x) {
-            const uint32_t slot = x >> 16;
-            if (slot != cur) {
-               if (acc) atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[cur]), static_cast<unsigned long long>(acc));
-               cur = slot;
-               acc = 0;
+         constexpr uint32_t kStride = kSharedThreads * kRound;
+         const uint32_t rounds = (n4 + kStride - 1) / kStride;
+         uint32_t cur = kPadSlot, a01 = 0, a23 = 0;
+         u32x4 bufA[kRound], bufB[kRound];
+         auto load = [&](uint32_t round, u32x4* buf) {
+            const uint32_t base = round * kStride + tid * kRound;
+#pragma unroll
+            for (uint32_t u = 0; u < kRound; ++u) {
+               const uint32_t jj = base + u;
+               buf[u] = e4[jj < n4 ? jj : n4 - 1];
             }
-            acc += nibbleCounts(rowNibble(mask, x & 0xFFFFu));
          };
-         for (uint32_t base = tid * kUnroll; base < n4; base += kQuadThreads * kUnroll) {
-            uint4 e[kUnroll];
+         auto use = [&](uint32_t round, const u32x4* buf) {
+            const uint32_t base = round * kStride + tid * kRound;
 #pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) {
-               e[u] = base + u < n4 ? e4[base + u] : make_uint4(kPadSlot << 16, kPadSlot << 16, kPadSlot << 16, kPadSlot << 16);
+            for (uint32_t u = 0; u < kRound; ++u) {
+               u32x4 word[4];
+#pragma unroll
+               for (uint32_t k = 0; k < 4; ++k) word[k] = fq4[(buf[u][k] & 0xFFFFu) >> 5];
+               const bool ok = base + u < n4;
+#pragma unroll
+               for (uint32_t k = 0; k < 4; ++k) {
+                  const uint32_t x = buf[u][k];
+                  const uint32_t slot = x >> 16;
+                  if (slot != cur) {
+                     if (a01 | a23) {
+                        atomicAdd(&cnt[cur], static_cast<unsigned long long>(a01) | (static_cast<unsigned long long>(a23) << 32));
+                     }
+                     cur = slot;
+                     a01 = 0;
+                     a23 = 0;
+                  }
+                  if (ok) quadBits(word[k], x, a01, a23);
+               }
             }
-#pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) {
-               take(e[u].x);
-               take(e[u].y);
-               take(e[u].z);
-               take(e[u].w);
+         };
+         if (rounds > 0) {
+            load(0, bufA);
+            for (uint32_t round = 0; round < rounds; round += 2) {
+               if (round + 1 < rounds) load(round + 1, bufB);
+               use(round, bufA);
+               if (round + 1 < rounds) {
+                  if (round + 2 < rounds) load(round + 2, bufA);
+                  use(round + 1, bufB);
+               }
             }
          }
          // slot 4095 is the padding slot: its counter is never read
-         if (acc) atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[cur]), static_cast<unsigned long long>(acc));
+         if (a01 | a23) atomicAdd(&cnt[cur], static_cast<unsigned long long>(a01) | (static_cast<unsigned long long>(a23) << 32));
       }
       // groups of eight rows of one slot
       {
-         const uint4* gr = reinterpret_cast<const uint4*>(col.groups) + g0;
-         const uint16_t* gh = col.group_hdr + g0;
+         const GlobalU32x4* gr = toGlobal<GlobalU32x4>(col.groups) + g0;
+         const GlobalU16* gh = toGlobal<GlobalU16>(col.group_hdr) + g0;
          const uint32_t ng = g1 - g0;
-         uint32_t cur = kPadSlot;
-         uint64_t acc = 0;
-         for (uint32_t base = tid * kUnroll; base < ng; base += kQuadThreads * kUnroll) {
-            uint4 rows[kUnroll];
-            uint32_t hdr[kUnroll];
+         constexpr uint32_t kStride = kSharedThreads * kRound;
+         const uint32_t rounds = (ng + kStride - 1) / kStride;
+         uint32_t cur = kPadSlot, a01 = 0, a23 = 0;
+         u32x4 rowsA[kRound], rowsB[kRound];
+         uint32_t hdrA[kRound], hdrB[kRound];
+         auto load = [&](uint32_t round, u32x4* rows, uint32_t* hdr) {
+            const uint32_t base = round * kStride + tid * kRound;
 #pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) {
-               const bool ok = base + u < ng;
-               rows[u] = ok ? gr[base + u] : make_uint4(0, 0, 0, 0);
-               hdr[u] = ok ? gh[base + u] : 0xFFFFu;
+            for (uint32_t u = 0; u < kRound; ++u) {
+               const uint32_t jj = base + u < ng ? base + u : ng - 1;
+               rows[u] = gr[jj];
+               hdr[u] = gh[jj];
             }
+         };
+         auto use = [&](uint32_t round, const u32x4* rows, const uint32_t* hdr) {
+            const uint32_t base = round * kStride + tid * kRound;
 #pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) {
-               if (hdr[u] == 0xFFFFu) continue;
-               const uint32_t slot = hdr[u] & 0xFFFu;
-               const uint32_t mm = (hdr[u] >> 12) + 1;
-               const uint32_t r[8] = {rows[u].x & 0xFFFFu, rows[u].x >> 16, rows[u].y & 0xFFFFu, rows[u].y >> 16,
-                                      rows[u].z & 0xFFFFu, rows[u].z >> 16, rows[u].w & 0xFFFFu, rows[u].w >> 16};
-               uint64_t add = 0;
+            for (uint32_t u = 0; u < kRound; ++u) {
+               u32x4 word[8];
 #pragma unroll
-               for (uint32_t t = 0; t < 8; ++t) add += t < mm ? nibbleCounts(rowNibble(mask, r[t])) : 0ull;
-               if (slot != cur) {
-                  if (acc) atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[cur]), static_cast<unsigned long long>(acc));
-                  cur = slot;
-                  acc = 0;
+               for (uint32_t t = 0; t < 8; ++t) {
+                  const uint32_t row = (t & 1u) ? rows[u][t >> 1] >> 16 : rows[u][t >> 1] & 0xFFFFu;
+                  word[t] = fq4[row >> 5];
                }
-               acc += add;
+               const uint32_t h = base + u < ng ? hdr[u] : 0xFFFFu;
+               const uint32_t slot = h & 0xFFFu;
+               const uint32_t mm = h == 0xFFFFu ? 0u : (h >> 12) + 1;
+               uint32_t n01 = 0, n23 = 0;
+#pragma unroll
+               for (uint32_t t = 0; t < 8; ++t) {
+                  const uint32_t row = (t & 1u) ? rows[u][t >> 1] >> 16 : rows[u][t >> 1];
+                  if (t < mm) quadBits(word[t], row, n01, n23);
+               }
+               if (slot != cur) {
+                  if (a01 | a23) {
+                     atomicAdd(&cnt[cur], static_cast<unsigned long long>(a01) | (static_cast<unsigned long long>(a23) << 32));
+                  }
+                  cur = slot;
+                  a01 = 0;
+                  a23 = 0;
+               }
+               a01 += n01;
+               a23 += n23;
+            }
+         };
+         if (rounds > 0) {
+            load(0, rowsA, hdrA);
+            for (uint32_t round = 0; round < rounds; round += 2) {
+               if (round + 1 < rounds) load(round + 1, rowsB, hdrB);
+               use(round, rowsA, hdrA);
+               if (round + 1 < rounds) {
+                  if (round + 2 < rounds) load(round + 2, rowsA, hdrA);
+                  use(round + 1, rowsB, hdrB);
+               }
             }
          }
-         if (acc) atomicAdd(reinterpret_cast<unsigned long long*>(&cnt[cur]), static_cast<unsigned long long>(acc));
+         if (a01 | a23) atomicAdd(&cnt[cur], static_cast<unsigned long long>(a01) | (static_cast<unsigned long long>(a23) << 32));
       }
-      // bitset and run containers: one per wavefront, per query from the filter's words
-      for (uint32_t d = h0 + (tid >> 6); d < h1; d += kQuadThreads / 64) {
-         const Desc dd = col.vert.desc[d];
-         const uint32_t type = descType(dd.w0);
-         const uint32_t card = descCard(dd.w0);
-         const uint32_t slot = descSlot(dd.w0);
+      // bitset and run containers: one per wavefront, the four queries at once
+      for (uint32_t d = h0 + wave; d < h1; d += kSharedThreads / 64) {
+         const u32x2 dw = toGlobal<GlobalU32x2>(col.vert.desc)[d];
+         const uint32_t type = descType(dw[0]);
+         const uint32_t card = descCard(dw[0]);
+         const uint32_t slot = descSlot(dw[0]);
+         uint32_t n[4] = {0, 0, 0, 0};
+         if (type == CT_BITSET) {
+            const GlobalU64x2* bits = toGlobal<GlobalU64x2>(col.vert.bits + static_cast<size_t>(dw[1]) * kChunkWords);
+            u64x2 wv[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) wv[u] = bits[lane + 64 * u];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+               const uint32_t wd = 2 * (lane + 64 * u);  // u64 word index of wv[u][0]
+#pragma unroll
+               for (uint32_t h = 0; h < 2; ++h) {
+                  const uint64_t x = wv[u][h];
+                  const u32x4 flo = fq4[2 * (wd + h)];
+                  const u32x4 fhi = fq4[2 * (wd + h) + 1];
+#pragma unroll
+                  for (uint32_t k = 0; k < 4; ++k) {
+                     n[k] += __popc(static_cast<uint32_t>(x) & flo[k]) + __popc(static_cast<uint32_t>(x >> 32) & fhi[k]);
+                  }
+               }
+            }
+         } else if (type == CT_RUN) {
+            const GlobalU32* rr = toGlobal<GlobalU32>(col.vert.runs + dw[1]);
+            for (uint32_t t = lane; t < card; t += 64) {
+               const uint32_t st = rr[t] & 0xFFFFu;
+               const uint32_t en = st + (rr[t] >> 16);
+#pragma unroll
+               for (uint32_t k = 0; k < 4; ++k) n[k] += rangePopcountQ(fq, k, st, en);
+            }
+         } else if (type == CT_ARRAY) {
+            const GlobalU16* a = toGlobal<GlobalU16>(col.vert.arr + dw[1]);
+            for (uint32_t t = lane; t < card; t += 64) {
+               const uint32_t row = a[t];
+               const u32x4 wv = fq4[row >> 5];
+#pragma unroll
+               for (uint32_t k = 0; k < 4; ++k) n[k] += (wv[k] >> (row & 31u)) & 1u;
+            }
+         } else if (lane == 0) {
+            for (uint32_t t = 0; t < card; ++t) {
+               const uint32_t row = (dw[1] >> (16 * t)) & 0xFFFFu;
+               const u32x4 wv = fq4[row >> 5];
+#pragma unroll
+               for (uint32_t k = 0; k < 4; ++k) n[k] += (wv[k] >> (row & 31u)) & 1u;
+            }
+         }
 #pragma unroll
          for (uint32_t k = 0; k < 4; ++k) {
-            if (!valid[k]) continue;
-            const uint64_t* f = m.filters + (static_cast<size_t>(q[k]) * n_chunks + c) * kChunkWords;
-            uint32_t n = 0;
-            if (type == CT_BITSET) {
-               const uint64_t* bits = col.vert.bits + static_cast<size_t>(dd.w1) * kChunkWords;
-               for (uint32_t w = lane; w < kChunkWords; w += 64) n += __popcll(bits[w] & f[w]);
-            } else if (type == CT_RUN) {
-               const uint32_t* rr = col.vert.runs + dd.w1;
-               for (uint32_t t = lane; t < card; t += 64) {
-                  const uint32_t st = rr[t] & 0xFFFFu;
-                  n += rangePopcount(f, st, st + (rr[t] >> 16));
-               }
-            } else if (type == CT_ARRAY) {
-               const uint16_t* a = col.vert.arr + dd.w1;
-               for (uint32_t t = lane; t < card; t += 64) n += bitAt(f, a[t]);
-            } else if (lane == 0) {
-               for (uint32_t t = 0; t < card; ++t) n += bitAt(f, (dd.w1 >> (16 * t)) & 0xFFFFu);
-            }
-            n = waveSum(n);
-            if (lane == 0 && n) atomicAdd(&heavy_sums[static_cast<size_t>(4 * j + k) * col.n_slots + s0 + slot], n);
+            const uint32_t v = waveSum(n[k]);
+            if (lane == 0 && valid[k] && v) atomicAdd(&m.heavy_sums[static_cast<size_t>(4 * j + k) * n_slots + s0 + slot], v);
          }
       }
+      c = cn;
    }
    flush();
 }
 
-// partial[e][s] += heavy_sums[e][s] and clears heavy_sums, for every dense list entry
-__global__ __launch_bounds__(256) void addHeavySumsKernel(MutationsBatch m, uint32_t* heavy_sums, uint32_t n_slots) {
+// partial[e][s] += partial_b[e][s] + heavy_sums[e][s] and clears heavy_sums, for every dense list
+// entry (after the shared-scan kernel)
+__global__ __launch_bounds__(256) void foldSharedPartsKernel(MutationsBatch m, uint32_t n_slots) {
    const uint32_t e = blockIdx.y;
    if (e >= m.list_counts[0]) return;
    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n_slots; s += gridDim.x * blockDim.x) {
       const size_t i = static_cast<size_t>(e) * n_slots + s;
-      const uint32_t h = heavy_sums[i];
-      if (h) {
-         m.partial[i] += h;
-         heavy_sums[i] = 0;
-      }
+      const uint32_t h = m.heavy_sums[i];
+      m.partial[i] += m.partial_b[i] + h;
+      if (h) m.heavy_sums[i] = 0;
    }
 }
 
@@ -1323,7 +1595,7 @@ __global__ __launch_bounds__(256) void sparseCountBatchKernel(MutationsBatch m) 
 }
 
 __device__ inline SlotSource batchSource(const MutationsBatch& m, uint32_t e, SlotSource*) {
-   return SlotSource{m.col_dev, m.partial + static_cast<size_t>(e) * m.col_dev->n_slots, 1};
+   return SlotSource{m.col_dev, m.partial + static_cast<size_t>(e) * m.col_dev->n_slots};
 }
 __device__ inline SparseSource batchSource(const MutationsBatch& m, uint32_t e, SparseSource*) {
    const size_t L = m.col_dev->genome_length;
@@ -1659,22 +1931,29 @@ hipError_t launchMutations(const MutationsLaunch& m, hipStream_t stream) {
    const bool packed = m.heavy_sums != nullptr && (m.n_chunks + m.n_groups - 1) / m.n_groups <= kPackedMaxChunks;
    uint32_t* heavy = packed ? m.heavy_sums : nullptr;
    const dim3 grid(h.n_blocks, m.n_groups);
-#define RDB_LAUNCH_K1(U, P)                                                                                            \
-   hipLaunchKernelGGL((mutationsCountKernel<U, P>), grid, dim3(kMutThreads), 0, stream, m.col_dev, m.n_chunks, m.n_groups, \
-                      m.filter, m.filter_chunk_cards, m.filter_card, m.sparse_max, m.partial, heavy)
+#define RDB_LAUNCH_K1(U, P, C)                                                                                         \
+   hipLaunchKernelGGL((mutationsCountKernel<U, P, C>), grid, dim3(kMutThreads), 0, stream, m.col_dev, m.n_chunks,         \
+                      m.n_groups, m.filter, m.filter_chunk_cards, m.filter_card, m.sparse_max, m.partial, heavy)
+#define RDB_LAUNCH_K1_C(U, P)       \
+   if (m.coop) {                    \
+      RDB_LAUNCH_K1(U, P, true);    \
+   } else {                         \
+      RDB_LAUNCH_K1(U, P, false);   \
+   }
    if (m.unroll == 16) {
       if (packed) {
-         RDB_LAUNCH_K1(16, true);
+         RDB_LAUNCH_K1_C(16, true);
       } else {
-         RDB_LAUNCH_K1(16, false);
+         RDB_LAUNCH_K1_C(16, false);
       }
    } else {
       if (packed) {
-         RDB_LAUNCH_K1(8, true);
+         RDB_LAUNCH_K1_C(8, true);
       } else {
-         RDB_LAUNCH_K1(8, false);
+         RDB_LAUNCH_K1_C(8, false);
       }
    }
+#undef RDB_LAUNCH_K1_C
 #undef RDB_LAUNCH_K1
    if ((err = hipGetLastError()) != hipSuccess) return err;
    if (m.count_stop && (err = hipEventRecord(m.count_stop, stream)) != hipSuccess) return err;
@@ -1684,7 +1963,7 @@ hipError_t launchMutations(const MutationsLaunch& m, hipStream_t stream) {
       if ((err = hipGetLastError()) != hipSuccess) return err;
    }
    const uint32_t n_pblocks = (h.genome_length + kPosBlock - 1) / kPosBlock;
-   const SlotSource dense{m.col_dev, m.partial, 1};
+   const SlotSource dense{m.col_dev, m.partial};
    hipLaunchKernelGGL(mutationsTotalsKernel<SlotSource>, dim3(n_pblocks), dim3(kPosBlock), 0, stream, dense,
                       m.filter_card, m.sparse_max, m.totals, m.n_rows);
    if ((err = hipGetLastError()) != hipSuccess) return err;
@@ -1716,24 +1995,33 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
    hipLaunchKernelGGL(batchPlanKernel, dim3(1), dim3(1024), 0, stream, m);
    if ((err = hipGetLastError()) != hipSuccess) return err;
    if (m.heavy_sums != nullptr) {
-      // shared scan, four dense queries per pass
-      const dim3 grid(h.n_blocks * ((m.max_queries + 3) / 4));
+      // shared scan, four dense queries per pass, chunks in two parts
+      const uint32_t n_xblocks = 8 * ((h.n_blocks + 7) / 8);
+      const dim3 grid(n_xblocks * 2 * ((m.max_queries + 3) / 4));
       if (m.unroll == 16) {
-         hipLaunchKernelGGL(mutationsCountQuadKernel<4>, grid, dim3(kQuadThreads), 0, stream, m, m.heavy_sums);
+         hipLaunchKernelGGL(mutationsCountSharedKernel<4>, grid, dim3(kSharedThreads), 0, stream, m);
       } else {
-         hipLaunchKernelGGL(mutationsCountQuadKernel<2>, grid, dim3(kQuadThreads), 0, stream, m, m.heavy_sums);
+         hipLaunchKernelGGL(mutationsCountSharedKernel<2>, grid, dim3(kSharedThreads), 0, stream, m);
       }
       if ((err = hipGetLastError()) != hipSuccess) return err;
       const uint32_t sblocks = (h.n_slots + 255) / 256;
-      hipLaunchKernelGGL(addHeavySumsKernel, dim3(sblocks == 0 ? 1 : (sblocks < 64 ? sblocks : 64), m.max_queries),
-                         dim3(256), 0, stream, m, m.heavy_sums, h.n_slots);
+      hipLaunchKernelGGL(foldSharedPartsKernel, dim3(sblocks == 0 ? 1 : (sblocks < 64 ? sblocks : 64), m.max_queries),
+                         dim3(256), 0, stream, m, h.n_slots);
    } else {
       const uint32_t n_xblocks = 8 * ((h.n_blocks + 7) / 8);
       const dim3 count_grid(n_xblocks * m.max_queries);
       if (m.unroll == 16) {
-         hipLaunchKernelGGL(mutationsCountBatchKernel<16>, count_grid, dim3(kMutThreads), 0, stream, m);
+         if (m.coop) {
+            hipLaunchKernelGGL((mutationsCountBatchKernel<16, true>), count_grid, dim3(kMutThreads), 0, stream, m);
+         } else {
+            hipLaunchKernelGGL((mutationsCountBatchKernel<16, false>), count_grid, dim3(kMutThreads), 0, stream, m);
+         }
       } else {
-         hipLaunchKernelGGL(mutationsCountBatchKernel<8>, count_grid, dim3(kMutThreads), 0, stream, m);
+         if (m.coop) {
+            hipLaunchKernelGGL((mutationsCountBatchKernel<8, true>), count_grid, dim3(kMutThreads), 0, stream, m);
+         } else {
+            hipLaunchKernelGGL((mutationsCountBatchKernel<8, false>), count_grid, dim3(kMutThreads), 0, stream, m);
+         }
       }
    }
    if ((err = hipGetLastError()) != hipSuccess) return err;

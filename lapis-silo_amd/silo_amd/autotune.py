@@ -2,9 +2,9 @@
 
 It measures, on the GPU and on the table at hand:
 
-* the dense Mutations kernel's shape: loads in flight per lane ("mutation_unroll"), chunk groups
-  ("mutation_groups") and 16-bit packed counters ("mutation_packed"), timed with HIP events on a
-  filter that touches every chunk;
+* the dense Mutations kernel's shape: wavefronts sharing each tile or not ("mutation_coop"), loads
+  in flight per lane ("mutation_unroll"), chunk groups ("mutation_groups") and 16-bit packed
+  counters ("mutation_packed"), timed with HIP events on a filter that touches every chunk;
 * the dense/sparse switch ("sparse_max_rows"): the dense method costs about the same for any filter
   that touches every chunk, the sparse one grows with the filter's rows; both are timed on filters
   of several cardinalities, the sparse time is fitted as a + b * rows, and the switch is put where
@@ -52,18 +52,21 @@ def autotune(engine, table, probes: list, column: str = "main", reps: int = 5, l
     engine.set_tuning("sparse_max_rows", -1)
     # 1. dense kernel shape: loads in flight, counter packing (work-groups per CU), chunk groups
     shapes = {}
-    for packed in (1, 0):
-        for unroll in (8, 16):
-            for groups in (0, 12, 16, 24, 32):
-                engine.set_tuning("mutation_packed", packed)
-                engine.set_tuning("mutation_unroll", unroll)
-                engine.set_tuning("mutation_groups", groups)
-                shapes[(packed, unroll, groups)] = _kernel_us(engine, table, broad, column, reps)
+    for coop in (1, 0):
+        for packed in (1, 0):
+            for unroll in (8, 16):
+                for groups in (0, 12, 16, 24, 32):
+                    engine.set_tuning("mutation_coop", coop)
+                    engine.set_tuning("mutation_packed", packed)
+                    engine.set_tuning("mutation_unroll", unroll)
+                    engine.set_tuning("mutation_groups", groups)
+                    shapes[(coop, packed, unroll, groups)] = _kernel_us(engine, table, broad, column, reps)
     best = min(shapes, key=shapes.get)
-    engine.set_tuning("mutation_packed", best[0])
-    engine.set_tuning("mutation_unroll", best[1])
-    engine.set_tuning("mutation_groups", best[2])
-    say(f"dense kernel shapes (packed, unroll, groups) -> us: {shapes}; chose {best}")
+    engine.set_tuning("mutation_coop", best[0])
+    engine.set_tuning("mutation_packed", best[1])
+    engine.set_tuning("mutation_unroll", best[2])
+    engine.set_tuning("mutation_groups", best[3])
+    say(f"dense kernel shapes (coop, packed, unroll, groups) -> us: {shapes}; chose {best}")
     # 2. dense/sparse switch from per-query wall times
     dense = []
     sparse = []
@@ -92,10 +95,39 @@ def autotune(engine, table, probes: list, column: str = "main", reps: int = 5, l
     engine.set_tuning("sparse_max_rows", threshold)
     say(f"dense/sparse per-query seconds: {[(c, round(d * 1e6, 1), round(s * 1e6, 1)) for c, d, s in zip(cards, dense, sparse)]}; "
         f"sparse fit a={a * 1e6:.1f}us b={b * 1e9:.2f}ns/row; switch at {threshold} rows")
-    return {"mutation_packed": best[0], "mutation_unroll": best[1], "mutation_groups": best[2], "sparse_max_rows": threshold,
-            "dense_shapes_us": {f"p{k[0]}u{k[1]}g{k[2]}": round(v, 2) for k, v in shapes.items()},
+    return {"mutation_coop": best[0], "mutation_packed": best[1], "mutation_unroll": best[2], "mutation_groups": best[3],
+            "sparse_max_rows": threshold,
+            "dense_shapes_us": {f"c{k[0]}p{k[1]}u{k[2]}g{k[3]}": round(v, 2) for k, v in shapes.items()},
             "probe_cards": [int(c) for c in cards],
             "dense_us": [round(d * 1e6, 1) for d in dense], "sparse_us": [round(s * 1e6, 1) for s in sparse]}
+
+
+def autotune_batch(engine, table, probes: list, column: str = "main", size: int = 64, reps: int = 3, log=None) -> dict:
+    """Chooses the batched dense kernel ("batch_shared_scan": one pass over the index per query, or
+    one pass per four queries) by timing a batch of `size` Mutations queries over the probe filters
+    both ways; the results of the two must agree.  The knob is applied."""
+    say = log or (lambda *_: None)
+    queries = [{"filter": probes[i % len(probes)], "action": "mutations", "column": column, "min_proportion": 0.05}
+               for i in range(size)]
+    saved = engine.get_tuning("batch_max_queries")
+    engine.set_tuning("batch_max_queries", size)
+    times = {}
+    results = {}
+    for shared in (0, 1):
+        engine.set_tuning("batch_shared_scan", shared)
+        results[shared] = table.batch(queries)
+        t = time.perf_counter()
+        for _ in range(reps):
+            table.batch(queries)
+        times[shared] = (time.perf_counter() - t) / reps
+    if results[0] != results[1]:
+        raise RuntimeError("shared-scan and per-query batch results differ")
+    best = min(times, key=times.get)
+    engine.set_tuning("batch_shared_scan", best)
+    engine.set_tuning("batch_max_queries", saved)
+    say(f"batch of {size} probe Mutations queries, ms: per-query kernel {times[0] * 1e3:.2f}, "
+        f"shared scan {times[1] * 1e3:.2f}; chose batch_shared_scan={best}")
+    return {"batch_shared_scan": best, "batch_ms": {"per_query": round(times[0] * 1e3, 3), "shared": round(times[1] * 1e3, 3)}}
 
 
 def probe_filters(ds, n: int = 8) -> list:

@@ -133,3 +133,32 @@ def test_batch_query_errors_are_per_query(small):
     for g in got[1:6]:
         assert isinstance(g, EngineError) and g.status == "RDB_EQUERY", g
     assert got[6] == ot.aggregate("(str_eq country null)", ["lineage"])
+
+
+def test_shared_scan_many_chunks(engine, tuning):
+    """More than 2 x 15 chunks: the shared scan's parts each gather several chunk runs in 16-bit
+    counters and flush them more than once; results equal the per-query kernel's and the oracle's."""
+    from oracle.oracle import OracleTable
+
+    ds, ts = small_spec(seed=31, n_rows=40000, L=300, chunk=1024)
+    ot = OracleTable(ts)
+    et = engine.table(ts)
+    try:
+        tuning.set_tuning("sparse_max_rows", -1)
+        tuning.set_tuning("batch_max_queries", 16)
+        queries = [{"filter": f, "action": "mutations", "column": "main", "min_proportion": mp}
+                   for f, mp in [("(true)", 0.0), ("(true)", 0.05), ("(not (has_mut main 17))", 0.01),
+                                 ("(str_eq country \"%s\")" % ds.country_names[0], 0.0),
+                                 ("(nof 1 0 (has_mut main 5) (has_mut main 40) (has_mut main 77))", 0.02),
+                                 ("(date_between date %d %d)" % (ds.spec.date_base, ds.spec.date_base + 20), 0.0)]]
+        want = [ot.mutations(q["filter"], "main", q["min_proportion"]) for q in queries]
+        per_query = []
+        for shared in (0, 1):
+            tuning.set_tuning("batch_shared_scan", shared)
+            got = et.batch(queries)
+            assert got == want, shared
+            per_query.append(got)
+        assert per_query[0] == per_query[1]
+    finally:
+        et.close()
+        ot.close()

@@ -159,3 +159,40 @@ def test_empty_covered_ranges(engine, sparse_max):
         engine.set_tuning("sparse_max_rows", -1)
         et.close()
         ot.close()
+
+
+@pytest.mark.parametrize("coop", [0, 1])
+def test_dense_kernel_shapes(engine, coop):
+    """Every shape of the dense Mutations kernel (wavefronts sharing tiles or not, packed 16-bit or
+    32-bit counters, loads in flight, chunk groups) gives the oracle's counts, on a table of 40
+    chunks so that groups hold several chunks and the packed form its limit of 15."""
+    from oracle.oracle import OracleTable
+    from silo_amd import datagen, table
+
+    spec = datagen.GenSpec(n_rows=40000, genome_length=300, seed=41, lineage_model=True, n_nodes=30, n_mean=20,
+                           null_seq_frac=0.02, null_meta_frac=0.05, n_days=50)
+    ds = datagen.generate(spec, keep_handle=True)
+    ts = table.spec_from_dataset(ds, chunk=1024)
+    ot = OracleTable(ts)
+    et = engine.table(ts)
+    keys = ("mutation_coop", "mutation_packed", "mutation_unroll", "mutation_groups", "sparse_max_rows")
+    saved = {k: engine.get_tuning(k) for k in keys}
+    exprs = ["(true)", "(has_mut main 17)", f'(str_eq country "{ds.country_names[1]}")',
+             "(not (nof 2 0 (has_mut main 5) (has_mut main 40) (sym_eq main 77 N)))"]
+    want = [ot.mutation_counts(e, "main") for e in exprs]
+    try:
+        engine.set_tuning("sparse_max_rows", -1)
+        engine.set_tuning("mutation_coop", coop)
+        for packed in (0, 1):
+            for unroll in (8, 16):
+                for groups in (0, 1, 3, 7, 32):
+                    engine.set_tuning("mutation_packed", packed)
+                    engine.set_tuning("mutation_unroll", unroll)
+                    engine.set_tuning("mutation_groups", groups)
+                    for e, w in zip(exprs, want):
+                        assert np.array_equal(et.mutation_counts(e, "main"), w), (packed, unroll, groups, e)
+    finally:
+        for k, v in saved.items():
+            engine.set_tuning(k, v)
+        et.close()
+        ot.close()
