@@ -72,11 +72,18 @@ def copy_calibration(mib):
 
 def run_plain_query(et, q):
     if q.get("action", "mutations") == "mutations":
-        return et.mutations_query(q["filter"], q.get("column", "main"), q["min_proportion"])
+        return et.mutations_query(q["filter"], q.get("column", "main"), q["min_proportion"], arrays=True)
     if q["action"] == "aggregate":
         with et.bitmap(q["filter"]) as b:
             return et.aggregate(b, q["dims"])
     return et.count(q["filter"])
+
+
+def as_rows(results: list) -> list:
+    """Mutations results delivered as columns (timed) turned into row tuples (for comparisons)."""
+    from silo_amd.engine import MutationArrays
+
+    return [r.rows() if isinstance(r, MutationArrays) else r for r in results]
 
 
 def serve(args):
@@ -102,18 +109,19 @@ def serve(args):
     reqs = serving.config4_requests(ds, genes, n=args.requests, seed=41)
     arrivals = serving.poisson_arrivals(len(reqs), rate=args.rate, seed=42)
     eng.set_tuning("sparse_max_rows", 2000)
-    srv = serving.Server(eng, et, max_batch=256, row_capacity=65536)
+    srv = serving.Server(eng, et, max_batch=256, row_capacity=65536, arrays=True)
     log(f"serving with {srv.lanes} lanes of {srv.lane_bytes / 2**20:.0f} MiB ({srv.free_bytes / 2**30:.1f} GiB free)")
     warm = serving.config4_requests(ds, genes, n=256, seed=40)
     srv.run(warm, serving.poisson_arrivals(len(warm), rate=args.rate, seed=39))
     results, latency, wall, sizes = srv.run(reqs, arrivals)
+    results = as_rows(results)
     qps = len(reqs) / wall
     log(f"served {len(reqs)} requests in {wall:.2f}s: {qps:.1f}/s, p50 {np.percentile(latency, 50) * 1e3:.2f} ms, "
         f"p99 {np.percentile(latency, 99) * 1e3:.2f} ms, {len(sizes)} batches (mean {np.mean(sizes):.1f})")
     # the same queries alone (single-query path): a bounded sample, bit for bit
     sample = list(range(0, len(reqs), max(1, len(reqs) // 200)))
     t0 = time.perf_counter()
-    identical = all(run_plain_query(et, reqs[i]) == results[i] for i in sample)
+    identical = all(as_rows([run_plain_query(et, reqs[i])])[0] == results[i] for i in sample)
     plain_qps = len(sample) / (time.perf_counter() - t0)
     log(f"alone == in the mix for {len(sample)} sampled requests: {identical}; single-query path {plain_qps:.1f}/s")
     mix = {k: sum(q["action"] == k for q in reqs) for k in ("mutations", "aggregate", "count")}
@@ -223,6 +231,7 @@ def main():
         for _ in range(args.steps):
             plain_results = plain_pass(lat)
         elapsed = time.perf_counter() - t_start
+        plain_results = as_rows(plain_results)
         plain = {"queries_per_s": round(args.steps * len(queries) / elapsed, 3),
                  "ms_per_step": round(elapsed / args.steps * 1e3, 3),
                  "p50_ms": round(percentile(lat, 50) * 1e3, 4), "p99_ms": round(percentile(lat, 99) * 1e3, 4)}
@@ -288,15 +297,16 @@ def main():
         eng.set_planner("learned", mlp)
     eng.set_tuning("batch_max_queries", min(max(len(queries), 1), 256))
     for _ in range(max(args.warmup, 1)):
-        et.batch(queries)
+        et.batch(queries, arrays=True)
     blat = []
     t_start = time.perf_counter()
     batch_results = None
     for _ in range(args.steps):
         t = time.perf_counter()
-        batch_results = et.batch(queries)
+        batch_results = et.batch(queries, arrays=True)
         blat.append(time.perf_counter() - t)
     b_elapsed = time.perf_counter() - t_start
+    batch_results = as_rows(batch_results)
     b_qps = args.steps * len(queries) / b_elapsed
     log(f"batched path: {b_qps:.1f} queries/s, {b_elapsed / args.steps * 1e3:.2f} ms per step")
     identical_paths = None

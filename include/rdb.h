@@ -173,6 +173,18 @@ int rdb_batch_count(rdb_batch* batch, uint32_t q, uint64_t* count);
 /* Rows of Mutations query q, as rdb_mutations. */
 int rdb_batch_mutations(rdb_batch* batch, uint32_t q, uint32_t* position, uint8_t* from, uint8_t* to,
                         uint32_t* count, uint32_t* coverage, double* proportion, uint64_t capacity, uint64_t* n);
+/* Every query's status (RDB_OK or the error it failed with), filter cardinality and result size
+ * (Mutations rows, Aggregated groups, 0 for Count) in one call, for the first n_queries queries:
+ * no counterpart in the reference (a batch is this engine's own); for callers that collect many
+ * results at once. */
+int rdb_batch_summary(rdb_batch* batch, uint32_t n_queries, int* status, uint64_t* cardinality, uint64_t* size);
+/* Rows of every Mutations query of the batch, concatenated in query order: query q's rows are
+ * [offsets[q], offsets[q + 1]) (empty for other actions and for failed queries); offsets has
+ * n_queries + 1 entries and the row arrays are as rdb_mutations'.  RDB_ENOMEM (nothing written but
+ * *n, the total) when capacity is too small.  No counterpart in the reference. */
+int rdb_batch_mutations_all(rdb_batch* batch, uint32_t n_queries, uint64_t* offsets, uint32_t* position,
+                            uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage, double* proportion,
+                            uint64_t capacity, uint64_t* n);
 /* Groups of Aggregated query q, as rdb_aggregate. */
 int rdb_batch_aggregate(rdb_batch* batch, uint32_t q, uint32_t* ids, uint64_t* counts, uint64_t capacity,
                         uint64_t* n);

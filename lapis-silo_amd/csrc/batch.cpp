@@ -691,6 +691,61 @@ int rdb_batch_mutations(rdb_batch* batch, uint32_t q, uint32_t* position, uint8_
    });
 }
 
+int rdb_batch_summary(rdb_batch* batch, uint32_t n_queries, int* status, uint64_t* cardinality, uint64_t* size) {
+   rdb::Context* ctx = batch && batch->b.table ? batch->b.table->ctx : nullptr;
+   return guarded(ctx, [&] {
+      if (!batch || (n_queries && (!status || !cardinality || !size))) rdb::fail(RDB_EINVAL, "null argument");
+      std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+      rdb::batchWait(batch->b);
+      if (n_queries > batch->b.queries.size()) rdb::fail(RDB_EINVAL, "more queries than the batch holds");
+      for (uint32_t q = 0; q < n_queries; ++q) {
+         const rdb::BatchQueryResult& r = batch->b.queries[q];
+         status[q] = r.status;
+         cardinality[q] = r.status == RDB_OK ? r.cardinality : 0;
+         size[q] = r.status != RDB_OK                    ? 0
+                   : r.action == RDB_ACTION_MUTATIONS    ? r.rows.size()
+                   : r.action == RDB_ACTION_AGGREGATE    ? r.agg_counts.size()
+                                                         : 0;
+      }
+   });
+}
+
+int rdb_batch_mutations_all(rdb_batch* batch, uint32_t n_queries, uint64_t* offsets, uint32_t* position, uint8_t* from,
+                            uint8_t* to, uint32_t* count, uint32_t* coverage, double* proportion, uint64_t capacity,
+                            uint64_t* n) {
+   rdb::Context* ctx = batch && batch->b.table ? batch->b.table->ctx : nullptr;
+   return guarded(ctx, [&] {
+      if (!batch || !offsets || !n) rdb::fail(RDB_EINVAL, "null argument");
+      std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+      rdb::batchWait(batch->b);
+      if (n_queries > batch->b.queries.size()) rdb::fail(RDB_EINVAL, "more queries than the batch holds");
+      auto rowsOf = [&](uint32_t q) -> const rdb::BatchQueryResult* {
+         const rdb::BatchQueryResult& r = batch->b.queries[q];
+         return r.status == RDB_OK && r.action == RDB_ACTION_MUTATIONS ? &r : nullptr;
+      };
+      uint64_t total = 0;
+      offsets[0] = 0;
+      for (uint32_t q = 0; q < n_queries; ++q) {
+         const rdb::BatchQueryResult* r = rowsOf(q);
+         total += r ? r->rows.size() : 0;
+         offsets[q + 1] = total;
+      }
+      *n = total;
+      if (total > capacity) rdb::fail(RDB_ENOMEM, "output capacity too small");
+      for (uint32_t q = 0; q < n_queries; ++q) {
+         const rdb::BatchQueryResult* r = rowsOf(q);
+         if (!r || r->rows.empty()) continue;
+         const uint64_t o = offsets[q];
+         uint64_t k = 0;
+         rdb::capi::emitMutationRows(*batch->b.table->seq[r->column], r->rows, position ? position + o : nullptr,
+                                     from ? from + o : nullptr, to ? to + o : nullptr, count ? count + o : nullptr,
+                                     coverage ? coverage + o : nullptr, proportion ? proportion + o : nullptr,
+                                     r->rows.size(), &k);
+      }
+      *n = total;
+   });
+}
+
 int rdb_batch_aggregate(rdb_batch* batch, uint32_t q, uint32_t* ids, uint64_t* counts, uint64_t capacity, uint64_t* n) {
    rdb::Context* ctx = batch && batch->b.table ? batch->b.table->ctx : nullptr;
    return guarded(ctx, [&] {

@@ -53,6 +53,8 @@ SIGNATURES = [
     ("rdb_batch_count", _int, [_vp, _u32, _vp]),
     ("rdb_batch_mutations", _int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
     ("rdb_batch_aggregate", _int, [_vp, _u32, _vp, _vp, _u64, _vp]),
+    ("rdb_batch_summary", _int, [_vp, _u32, _vp, _vp, _vp]),
+    ("rdb_batch_mutations_all", _int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
     ("rdb_batch_free", None, [_vp]),
     ("rdb_mlp_create", _int, [_vp, _u32, _u32, _u32, _u64, ctypes.POINTER(_vp)]),
     ("rdb_mlp_free", None, [_vp]),
@@ -339,15 +341,15 @@ class DeviceTable:
             self._bufs = bufs
         return cap, bufs[key]
 
-    def mutations_query(self, expr: str, column: str, min_proportion: float) -> list:
-        """Filter + Mutations in one call (one device round trip)."""
-        cap, (pos, fr, to, cnt, cov, prop) = self._row_buffers(column)
+    def mutations_query(self, expr: str, column: str, min_proportion: float, arrays: bool = False):
+        """Filter + Mutations in one call (one device round trip).  Rows as (position, from, to,
+        count, coverage, proportion) tuples, or with `arrays` as one MutationArrays of columns."""
+        cap, bufs = self._row_buffers(column)
+        pos, fr, to, cnt, cov, prop = bufs
         n = _u64()
         _check(lib().rdb_mutations_query(self._h, expr.encode(), column.encode(), float(min_proportion), _p(pos), _p(fr),
                                          _p(to), _p(cnt), _p(cov), _p(prop), cap, ctypes.byref(n)), self._ctx())
-        k = n.value
-        return list(zip(pos[:k].tolist(), fr[:k].tolist(), to[:k].tolist(), cnt[:k].tolist(), cov[:k].tolist(),
-                        prop[:k].tolist()))
+        return _mutation_rows(bufs, n.value, arrays)
 
     def mutations(self, filt, column: str, min_proportion: float) -> list:
         cap, (pos, fr, to, cnt, cov, prop) = self._row_buffers(column)
@@ -400,11 +402,30 @@ class DeviceTable:
         _check(lib().rdb_batch_submit(self._h, arr, len(queries), ctypes.byref(h)), self._ctx())
         return Batch(self, h, queries)
 
-    def batch(self, queries: list) -> list:
-        """Runs a batch and returns per query: its cardinality, Mutations rows or groups (or the
-        EngineError of a query that failed)."""
+    def batch(self, queries: list, arrays: bool = False) -> list:
+        """Runs a batch and returns per query: its cardinality, Mutations rows (columns with
+        `arrays`) or groups (or the EngineError of a query that failed)."""
         with self.batch_submit(queries) as b:
-            return b.results()
+            return b.results(arrays)
+
+
+class MutationArrays(tuple):
+    """Mutations rows as columns: position (u32), from and to symbol (u8), count and coverage (u32),
+    proportion (f64), each a numpy array of the row count."""
+
+    def __new__(cls, cols):
+        return super().__new__(cls, cols)
+
+    def rows(self) -> list:
+        return list(zip(*(c.tolist() for c in self)))
+
+
+def _mutation_rows(bufs, k: int, arrays: bool):
+    if arrays:
+        return MutationArrays(tuple(b[:k].copy() for b in bufs))
+    pos, fr, to, cnt, cov, prop = bufs
+    return list(zip(pos[:k].tolist(), fr[:k].tolist(), to[:k].tolist(), cnt[:k].tolist(), cov[:k].tolist(),
+                    prop[:k].tolist()))
 
 
 class Batch:
@@ -427,14 +448,13 @@ class Batch:
         _check(lib().rdb_batch_count(self._h, q, ctypes.byref(n)), self.table._ctx())
         return n.value
 
-    def mutations(self, q: int) -> list:
-        cap, (pos, fr, to, cnt, cov, prop) = self.table._row_buffers(self.queries[q]["column"])
+    def mutations(self, q: int, arrays: bool = False):
+        cap, bufs = self.table._row_buffers(self.queries[q]["column"])
+        pos, fr, to, cnt, cov, prop = bufs
         n = _u64()
         _check(lib().rdb_batch_mutations(self._h, q, _p(pos), _p(fr), _p(to), _p(cnt), _p(cov), _p(prop), cap,
                                          ctypes.byref(n)), self.table._ctx())
-        k = n.value
-        return list(zip(pos[:k].tolist(), fr[:k].tolist(), to[:k].tolist(), cnt[:k].tolist(), cov[:k].tolist(),
-                        prop[:k].tolist()))
+        return _mutation_rows(bufs, n.value, arrays)
 
     def aggregate(self, q: int) -> list:
         dims = self.queries[q].get("dims", [])
@@ -449,18 +469,54 @@ class Batch:
             out.append((key, int(counts[i])))
         return out
 
-    def results(self) -> list:
+    def results(self, arrays: bool = False) -> list:
+        """Every query's result in submission order (Mutations rows as in DeviceTable.mutations_query,
+        as columns with `arrays`: then every Mutations query's rows come back in one call)."""
+        if arrays:
+            return self._results_columns()
         out = []
         for i, q in enumerate(self.queries):
             try:
                 if q["action"] == "mutations":
-                    out.append(self.mutations(i))
+                    out.append(self.mutations(i, arrays))
                 elif q["action"] == "aggregate":
                     out.append(self.aggregate(i))
                 else:
                     out.append(self.count(i))
             except EngineError as e:
                 out.append(e)
+        return out
+
+    def _results_columns(self) -> list:
+        n = len(self.queries)
+        status = np.zeros(n, np.int32)
+        card = np.zeros(n, np.uint64)
+        size = np.zeros(n, np.uint64)
+        ctx = self.table._ctx()
+        _check(lib().rdb_batch_summary(self._h, n, _p(status), _p(card), _p(size)), ctx)
+        is_mut = np.array([q["action"] == "mutations" for q in self.queries], bool)
+        total = int(size[is_mut & (status == RDB_OK)].sum())
+        offsets = np.zeros(n + 1, np.uint64)
+        cols = (np.empty(total, np.uint32), np.empty(total, np.uint8), np.empty(total, np.uint8),
+                np.empty(total, np.uint32), np.empty(total, np.uint32), np.empty(total, np.float64))
+        got = _u64()
+        _check(lib().rdb_batch_mutations_all(self._h, n, _p(offsets), *[_p(c) for c in cols], total, ctypes.byref(got)),
+               ctx)
+        out = []
+        for i, q in enumerate(self.queries):
+            if status[i] != RDB_OK:
+                try:
+                    self.count(i)  # raises the query's own error
+                except EngineError as e:
+                    out.append(e)
+                continue
+            if q["action"] == "mutations":
+                a, b = int(offsets[i]), int(offsets[i + 1])
+                out.append(MutationArrays(tuple(c[a:b] for c in cols)))
+            elif q["action"] == "aggregate":
+                out.append(self.aggregate(i))
+            else:
+                out.append(int(card[i]))
         return out
 
     def free(self) -> None:

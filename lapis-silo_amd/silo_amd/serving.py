@@ -31,8 +31,9 @@ def poisson_arrivals(n: int, rate: float, seed: int) -> np.ndarray:
 
 class Server:
     def __init__(self, engine, table, max_batch: int = 256, row_capacity: int = 65536, memory_fraction: float = 0.5,
-                 max_lanes: int = 8):
+                 max_lanes: int = 8, arrays: bool = False):
         self.engine = engine
+        self.arrays = arrays  # Mutations results as columns (engine.MutationArrays) instead of row tuples
         self.table = table
         self.max_batch = max_batch
         free, total = engine.device_memory()
@@ -69,7 +70,7 @@ class Server:
             still = []
             for b, ids in inflight:
                 if b.poll():
-                    out = b.results()
+                    out = b.results(self.arrays)
                     t = time.perf_counter() - t0
                     for i, r in zip(ids, out):
                         results[i] = r

@@ -128,11 +128,12 @@ def test_batch_query_errors_are_per_query(small):
                {"filter": "(true)", "action": "mutations", "column": "main", "min_proportion": 1.5},
                {"filter": "(true)", "action": "aggregate", "dims": ["main@3"]},
                {"filter": "(str_eq country null)", "action": "aggregate", "dims": ["lineage"]}]
-    got = et.batch(queries)
-    assert got[0] == ot.count("(has_mut main 17)")
-    for g in got[1:6]:
-        assert isinstance(g, EngineError) and g.status == "RDB_EQUERY", g
-    assert got[6] == ot.aggregate("(str_eq country null)", ["lineage"])
+    for arrays in (False, True):
+        got = et.batch(queries, arrays=arrays)
+        assert got[0] == ot.count("(has_mut main 17)")
+        for g in got[1:6]:
+            assert isinstance(g, EngineError) and g.status == "RDB_EQUERY", g
+        assert got[6] == ot.aggregate("(str_eq country null)", ["lineage"])
 
 
 def test_shared_scan_many_chunks(engine, tuning):
@@ -162,3 +163,21 @@ def test_shared_scan_many_chunks(engine, tuning):
     finally:
         et.close()
         ot.close()
+
+
+def test_results_as_columns(small, tuning):
+    """Mutations results delivered as columns hold the same rows as the tuple form, on both paths."""
+    from silo_amd.engine import MutationArrays
+
+    ds, ts, ot, et = small
+    queries = make_queries(ds, seed=300, n=24)
+    rows = et.batch(queries)
+    cols = et.batch(queries, arrays=True)
+    for q, r, c in zip(queries, rows, cols):
+        if q["action"] == "mutations":
+            assert isinstance(c, MutationArrays) and len(c) == 6
+            assert c.rows() == r, q
+            single = et.mutations_query(q["filter"], q["column"], q["min_proportion"], arrays=True)
+            assert single.rows() == r, q
+        else:
+            assert c == r, q
