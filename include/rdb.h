@@ -246,8 +246,9 @@ int rdb_set_planner(rdb_ctx* ctx, int mode, rdb_mlp* mlp);
  *   "batch_max_queries" queries per lane job of the batched path (1..4096, default 256);
  *   "batch_lanes"      lanes (HIP streams with their own buffers) per table (1..8, default 2);
  *   "batch_graphs"     1: replay each lane's launch sequence from a captured HIP graph (default);
- *   "batch_shared_scan" 1: a job's dense Mutations queries share passes over the index, four per
- *                      pass; 0: one pass per query (default);
+ *   "batch_shared_scan" how a job's dense Mutations queries pass over the index: 0 one pass per
+ *                      query (default), 1 four queries per pass, 2 eight per pass (row bytes of
+ *                      eight filters; the lane then also holds 1 byte per row per 8 queries);
  *   "batch_row_capacity" Mutations rows per query kept by a lane job (more: the query runs again
  *                      alone), default 4096.
  * Results never depend on them.  RDB_EINVAL for an unknown key or an out-of-range value. */

@@ -177,6 +177,10 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column) {
       m.partial = lane.partial.as<uint32_t>();
       m.heavy_sums = ctx.batch_shared_scan ? lane.heavy_sums.as<uint32_t>() : nullptr;
       m.partial_b = lane.partial_b.as<uint32_t>();
+      m.shared_mode = ctx.batch_shared_scan;
+      m.partial_c = lane.partial_c.bytes() ? lane.partial_c.as<uint32_t>() : nullptr;
+      m.octet_masks = lane.octet_masks.bytes() ? lane.octet_masks.as<uint8_t>() : nullptr;
+      m.persistent_groups = static_cast<uint32_t>(ctx.cu_count);
       m.sparse_counts = lane.sparse_counts.as<uint32_t>();
       m.sparse_bnd = lane.sparse_bnd.as<int32_t>();
       m.totals = lane.totals.as<int64_t>();
@@ -210,14 +214,25 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column) {
             "pack rows");
 }
 
+// Buffers of the octet shared scan (outside any graph capture): the slot sums of chunk parts 2 and 3
+// and one byte per row per eight queries.
+void ensureOctetBuffers(const Table& t, BatchLane& lane) {
+   if (t.ctx->batch_shared_scan != 2 || t.seq.empty() || lane.octet_masks.bytes() != 0) return;
+   const uint32_t Q = lane.max_queries;
+   const MutScratch ms = mutScratch(t);
+   lane.partial_c = DeviceBuffer(4ULL * (kOctetParts - 2) * Q * ms.slots);
+   lane.octet_masks = DeviceBuffer(static_cast<size_t>((Q + 7) / 8) * t.n_chunks * kChunkRows);
+}
+
 void launchLane(Table& t, BatchLane& lane, uint32_t depth, int column) {
    Context& ctx = *t.ctx;
+   ensureOctetBuffers(t, lane);
    const uint32_t bucket = depthBucket(depth, maxFilterDepth(ctx));
    if (!ctx.batch_graphs) {
       enqueueKernels(t, lane, bucket, column);
       return;
    }
-   const uint32_t knobs = ctx.mutation_unroll | (ctx.batch_shared_scan ? 1u << 8 : 0u) | (ctx.mutation_coop ? 1u << 9 : 0u);
+   const uint32_t knobs = ctx.mutation_unroll | (ctx.batch_shared_scan << 8) | (ctx.mutation_coop ? 1u << 12 : 0u);
    const auto key = std::make_tuple(bucket, column, static_cast<long long>(ctx.sparse_max_rows), knobs);
    auto it = lane.graphs.find(key);
    if (it == lane.graphs.end()) {
@@ -568,6 +583,9 @@ size_t laneBytes(const Table& t, uint32_t max_queries, uint32_t row_capacity) {
    if (!t.seq.empty()) {
       const MutScratch ms = mutScratch(t);
       b += 4ULL * Q * (3 * ms.slots + ms.counts + ms.bnd) + 2ULL * Q * ms.pblocks * 3 * sizeof(int64_t);
+      if (t.ctx->batch_shared_scan == 2) {
+         b += 4ULL * (kOctetParts - 2) * Q * ms.slots + static_cast<size_t>((Q + 7) / 8) * t.n_chunks * kChunkRows;
+      }
    }
    return b;
 }

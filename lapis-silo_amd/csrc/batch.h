@@ -32,6 +32,7 @@ struct BatchLane {
    uint32_t hist_capacity = 0;
    size_t params_bytes = 0;
    size_t out_bytes = 0;
+   DeviceBuffer partial_c, octet_masks;  // octet shared scan only, allocated on first use
    DeviceBuffer params, filters, cards, lists, list_counts, partial, partial_b, heavy_sums, sparse_counts, sparse_bnd, totals, rows,
       n_rows, hist, out;
    void* host_params = nullptr;

@@ -221,7 +221,8 @@ int rdb_set_tuning(rdb_ctx* ctx, const char* key, int64_t value) {
       } else if (k == "batch_graphs") {
          ctx->c.batch_graphs = value != 0;
       } else if (k == "batch_shared_scan") {
-         ctx->c.batch_shared_scan = value != 0;
+         if (value < 0 || value > 2) fail(RDB_EINVAL, "batch_shared_scan must be 0, 1 or 2");
+         ctx->c.batch_shared_scan = static_cast<uint32_t>(value);
       } else if (k == "batch_row_capacity") {
          if (value < 16 || value > (1 << 20)) fail(RDB_EINVAL, "batch_row_capacity must be in [16, 2^20]");
          ctx->c.batch_row_capacity = static_cast<uint32_t>(value);

@@ -126,7 +126,7 @@ struct Context {
    uint32_t batch_lanes = 2;           // lanes (streams with their own buffers) per table
    uint32_t batch_graphs = 1;          // capture each lane's launch sequence into a HIP graph
    uint32_t batch_row_capacity = 4096; // Mutations rows per query a lane job keeps (more: rerun alone)
-   uint32_t batch_shared_scan = 0;     // dense Mutations of a job: four queries per pass over the index
+   uint32_t batch_shared_scan = 0;     // dense Mutations of a job: 1 four, 2 eight queries per pass over the index
    // planner of the batched path: 0 reference order, 1 independence estimates, 2 learned estimator
    int plan_mode = 0;
    struct Mlp* planner = nullptr;

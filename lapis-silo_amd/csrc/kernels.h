@@ -77,6 +77,8 @@ hipError_t launchMutations(const MutationsLaunch& m, hipStream_t stream);
 // query q's rows at rows + q * row_capacity with their number in n_rows[q] (which may exceed the
 // capacity).  Every grid is sized for max_queries and reads the actual count from the device, so
 // the launch sequence can be captured once and replayed for any batch of up to max_queries.
+constexpr uint32_t kOctetParts = 4;  // chunk parts of the octet shared scan (slot-sum arrays)
+
 struct MutationsBatch {
    const SeqColumnView* col_dev = nullptr;
    const SeqColumnView* col_host = nullptr;
@@ -97,6 +99,12 @@ struct MutationsBatch {
    uint32_t* partial = nullptr;                // [max_queries * n_slots]
    uint32_t* heavy_sums = nullptr;             // [max_queries * n_slots], zero: set = shared-scan kernel
    uint32_t* partial_b = nullptr;              // [max_queries * n_slots]: the shared scan's second part
+   // octet shared scan (shared_mode 2): parts 2 and 3 of the slot sums, [2 * max_queries * n_slots],
+   // and the row bytes of every octet, [ceil(max_queries / 8) * n_chunks * 65536]
+   uint32_t shared_mode = 0;                   // 0: one pass per query, 1: four queries, 2: eight
+   uint32_t* partial_c = nullptr;
+   uint8_t* octet_masks = nullptr;
+   uint32_t persistent_groups = 256;           // work-groups of the persistent octet kernel (one per CU)
    uint32_t* sparse_counts = nullptr;          // [max_queries * n_symbols * L], zero
    int32_t* sparse_bnd = nullptr;              // [max_queries * 3 * L], zero
    int64_t* totals = nullptr;                  // [2 * max_queries * position blocks * 3]

@@ -145,6 +145,11 @@ __device__ inline uint32_t waveSum(uint32_t v) {
    return v;
 }
 
+__device__ inline uint64_t waveSum64(uint64_t v) {
+   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+   return v;
+}
+
 // ------------------------------------------------------------------------------------------------
 // Filter VM: one wavefront evaluates one program over one (chunk, slice) item; items of several
 // programs (queries) are spread over the grid.  The operand stack lives in LDS, one 64-word row
@@ -518,6 +523,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef const u32x4 __attribute__((address_space(1))) GlobalU32x4;
 typedef const uint16_t __attribute__((address_space(1))) GlobalU16;
 typedef const uint32_t __attribute__((address_space(1))) GlobalU32;
+typedef const uint64_t __attribute__((address_space(1))) GlobalU64;
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 typedef const u64x2 __attribute__((address_space(1))) GlobalU64x2;
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
@@ -799,12 +805,13 @@ __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b,
       for (uint32_t k = wave;; k += kMutWaves) {
          const uint32_t c = g + k * n_groups;
          if (c >= n_chunks) break;
-         if (filter_chunk_cards[c] == 0) continue;
+         // the chunk's cardinality and tile offsets in one round of loads
+         const uint32_t card = toGlobal<GlobalU32>(filter_chunk_cards)[c];
          const uint32_t tile = b * n_chunks + c;
          const uint32_t l0 = light_ptr[tile], l1 = light_ptr[tile + 1];
          const uint32_t h0 = heavy_ptr[tile], h1 = heavy_ptr[tile + 1];
          const uint32_t g0 = group_ptr[tile], g1 = group_ptr[tile + 1];
-         if (l0 == l1 && h0 == h1 && g0 == g1) continue;
+         if (card == 0 || (l0 == l1 && h0 == h1 && g0 == g1)) continue;
          {
             // stage the 8 KiB filter tile: eight independent 16-byte loads per lane
             const GlobalU64x2* src = toGlobal<GlobalU64x2>(filter + static_cast<size_t>(c) * kChunkWords);
@@ -1201,6 +1208,7 @@ __global__ __launch_bounds__(1024) void batchPlanKernel(MutationsBatch m) {
    if (threadIdx.x == 0) {
       m.list_counts[0] = n_dense;
       m.list_counts[1] = n_sparse;
+      m.list_counts[2] = 0;  // work counter of the persistent octet kernel
    }
 }
 
@@ -1239,6 +1247,7 @@ __global__ __launch_bounds__(kMutThreads) __attribute__((amdgpu_waves_per_eu(kCo
 // loaded while the current tile is counted.
 constexpr uint32_t kSharedThreads = 512;
 constexpr uint32_t kSharedFlushChunks = 15;
+constexpr uint32_t kSharedMaxPartChunks = 4096;  // chunks of one half (the launch checks it)
 
 // popcount of bits a..b (inclusive) of query k's words in an interleaved tile
 __device__ inline uint32_t rangePopcountQ(const uint32_t* fq, uint32_t k, uint32_t a, uint32_t b) {
@@ -1264,6 +1273,7 @@ template <uint32_t kRound>
 __global__ __launch_bounds__(kSharedThreads) __attribute__((amdgpu_waves_per_eu(4))) void mutationsCountSharedKernel(MutationsBatch m) {
    __shared__ __align__(16) uint32_t fq[kChunkWords * 2 * 4];
    __shared__ unsigned long long cnt[kMaxBlockSlots + 1];
+   __shared__ uint8_t work[kSharedMaxPartChunks];
    const u32x4* fq4 = reinterpret_cast<const u32x4*>(fq);
    const uint32_t n_dense = m.list_counts[0];
    const uint32_t n_quads = (n_dense + 3) / 4;
@@ -1290,12 +1300,14 @@ __global__ __launch_bounds__(kSharedThreads) __attribute__((amdgpu_waves_per_eu(
    const GlobalU32* heavy_ptr = toGlobal<GlobalU32>(col.heavy_ptr);
    const uint32_t s0 = col.block_slot_start[b];
    const uint32_t ns = col.block_slot_start[b + 1] - s0;
+   // entries past the end take the last entry's query id (their loads stay unconditional, which
+   // keeps them from being serialised) and are masked out
    uint32_t q[4];
    bool valid[4];
 #pragma unroll
    for (uint32_t k = 0; k < 4; ++k) {
       valid[k] = 4 * j + k < n_dense;
-      q[k] = valid[k] ? m.lists[4 * j + k] : 0;
+      q[k] = m.lists[valid[k] ? 4 * j + k : n_dense - 1];
    }
    for (uint32_t t = tid; t <= kMaxBlockSlots; t += kSharedThreads) cnt[t] = 0;
    uint32_t since_flush = 0;
@@ -1317,19 +1329,25 @@ __global__ __launch_bounds__(kSharedThreads) __attribute__((amdgpu_waves_per_eu(
       since_flush = 0;
       __syncthreads();
    };
-   // the next chunk at or after c in which one of the four filters has rows and the tile is not empty
-   auto next = [&](uint32_t c) -> uint32_t {
-      for (; c < c_end; ++c) {
-         bool any = false;
+   // which chunks of the part have work: one of the four filters has rows there and the tile is not
+   // empty (one thread per chunk, all loads issued together)
+   const uint32_t c_begin = part ? half : 0;
+   for (uint32_t t = tid; c_begin + t < c_end; t += kSharedThreads) {
+      const uint32_t c = c_begin + t;
+      uint32_t cards[4];
 #pragma unroll
-         for (uint32_t k = 0; k < 4; ++k) any |= valid[k] && m.chunk_cards[static_cast<size_t>(q[k]) * n_chunks + c] != 0;
-         if (!any) continue;
-         const uint32_t tile = b * n_chunks + c;
-         if (light_ptr[tile] != light_ptr[tile + 1] || group_ptr[tile] != group_ptr[tile + 1] ||
-             heavy_ptr[tile] != heavy_ptr[tile + 1]) {
-            break;
-         }
-      }
+      for (uint32_t k = 0; k < 4; ++k) cards[k] = m.chunk_cards[static_cast<size_t>(q[k]) * n_chunks + c];
+      const uint32_t tile = b * n_chunks + c;
+      const bool nonempty = light_ptr[tile] != light_ptr[tile + 1] || group_ptr[tile] != group_ptr[tile + 1] ||
+                            heavy_ptr[tile] != heavy_ptr[tile + 1];
+      bool any = false;
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) any |= valid[k] & (cards[k] != 0);
+      work[t] = any && nonempty ? 1 : 0;
+   }
+   __syncthreads();
+   auto next = [&](uint32_t c) -> uint32_t {
+      while (c < c_end && !work[c - c_begin]) ++c;
       return c;
    };
    // filter words of chunk c: this thread's words tid and tid + 512 of the four entries
@@ -1338,12 +1356,12 @@ __global__ __launch_bounds__(kSharedThreads) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
       for (uint32_t h = 0; h < 2; ++h) {
 #pragma unroll
-         for (uint32_t k = 0; k < 4; ++k) {
-            pf[h][k] = valid[k] ? m.filters[(static_cast<size_t>(q[k]) * n_chunks + c) * kChunkWords + tid + h * 512] : 0ull;
-         }
+         for (uint32_t k = 0; k < 4; ++k) pf[h][k] = m.filters[(static_cast<size_t>(q[k]) * n_chunks + c) * kChunkWords + tid + h * 512];
+#pragma unroll
+         for (uint32_t k = 0; k < 4; ++k) pf[h][k] = valid[k] ? pf[h][k] : 0ull;
       }
    };
-   uint32_t c = next(part ? half : 0);
+   uint32_t c = next(c_begin);
    if (c < c_end) fetch(c);
    while (c < c_end) {
       if (since_flush == kSharedFlushChunks) flush();
@@ -1546,16 +1564,430 @@ __global__ __launch_bounds__(kSharedThreads) __attribute__((amdgpu_waves_per_eu(
    flush();
 }
 
-// partial[e][s] += partial_b[e][s] + heavy_sums[e][s] and clears heavy_sums, for every dense list
-// entry (after the shared-scan kernel)
-__global__ __launch_bounds__(256) void foldSharedPartsKernel(MutationsBatch m, uint32_t n_slots) {
+// partial[e][s] += partial_b[e][s] (+ the further parts of the octet scan) + heavy_sums[e][s] and
+// clears heavy_sums, for every dense list entry (after a shared-scan kernel)
+__global__ __launch_bounds__(256) void foldSharedPartsKernel(MutationsBatch m, uint32_t n_slots, uint32_t n_parts) {
    const uint32_t e = blockIdx.y;
    if (e >= m.list_counts[0]) return;
+   const size_t plane = static_cast<size_t>(m.max_queries) * n_slots;
    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n_slots; s += gridDim.x * blockDim.x) {
       const size_t i = static_cast<size_t>(e) * n_slots + s;
       const uint32_t h = m.heavy_sums[i];
-      m.partial[i] += m.partial_b[i] + h;
+      uint32_t v = m.partial[i] + m.partial_b[i] + h;
+      for (uint32_t p = 2; p < n_parts; ++p) v += m.partial_c[(p - 2) * plane + i];
+      m.partial[i] = v;
       if (h) m.heavy_sums[i] = 0;
+   }
+}
+
+// ---- octet shared scan: eight dense queries per pass over the index -----------------------------
+// A pre-pass turns the filters of dense list entries 8o .. 8o+7 into one byte per row (bit k: the row
+// is in entry 8o+k's filter), 64 KiB per chunk.  The counting kernel is persistent: each work-group
+// (1024 threads, one per CU) takes (position block, chunk part, octet) items from a counter.  Per
+// chunk it stages the chunk's row bytes in LDS; a light entry or group row then costs one LDS byte
+// read and one read of a 256-entry table that spreads the byte into eight 8-bit fields, added to the
+// lane's run sum with one 64-bit add, for all eight queries at once.  Runs end in two 64-bit LDS
+// atomics on counters of four 16-bit fields (flushed to the part's slot sums every 15 chunks, so
+// they cannot overflow).  Bitset and run containers AND against the eight filters' words in global
+// memory and add to heavy_sums.  The parts' sums are folded by foldSharedPartsKernel.
+constexpr uint32_t kOctetThreads = 1024;   // the container loop gives thread t word t of a bitset
+constexpr uint32_t kOctetMaskThreads = 256;
+constexpr uint32_t kOctetDescCap = 128;    // containers whose sums are gathered in LDS at a time
+constexpr uint32_t kOctetMaxPartChunks = 4096;  // chunks of one part (the launch checks it)
+static_assert(kOctetThreads == kChunkWords, "one bitset word per thread");
+
+__device__ inline uint64_t transpose8x8(uint64_t x) {
+   // bit 8r + c <-> bit 8c + r
+   uint64_t t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+   x = x ^ t ^ (t << 7);
+   t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+   x = x ^ t ^ (t << 14);
+   t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+   x = x ^ t ^ (t << 28);
+   return x;
+}
+
+// four 8-bit fields -> four 16-bit fields
+__device__ inline uint64_t widen4(uint32_t a) {
+   return static_cast<uint64_t>((a & 0xFFu) | ((a & 0xFF00u) << 8)) |
+          (static_cast<uint64_t>(((a >> 16) & 0xFFu) | ((a >> 8) & 0xFF0000u)) << 32);
+}
+
+// Grid (kChunkWords / kOctetMaskThreads, max octets * n_chunks): thread = (octet, chunk, word).
+__global__ __launch_bounds__(kOctetMaskThreads) void octetMasksKernel(MutationsBatch m) {
+   const uint32_t n_dense = m.list_counts[0];
+   const uint32_t n_chunks = m.n_chunks;
+   const uint32_t o = blockIdx.y / n_chunks;
+   const uint32_t c = blockIdx.y % n_chunks;
+   if (8 * o >= n_dense) return;
+   const uint32_t w = blockIdx.x * kOctetMaskThreads + threadIdx.x;
+   const GlobalU64* filters = toGlobal<GlobalU64>(m.filters);
+   // loads without conditions (a condition would serialise them): entries past the end read the
+   // last entry's words and are masked out
+   uint32_t qs[8];
+#pragma unroll
+   for (uint32_t k = 0; k < 8; ++k) qs[k] = m.lists[8 * o + k < n_dense ? 8 * o + k : n_dense - 1];
+   uint64_t x[8];
+#pragma unroll
+   for (uint32_t k = 0; k < 8; ++k) x[k] = filters[(static_cast<size_t>(qs[k]) * n_chunks + c) * kChunkWords + w];
+#pragma unroll
+   for (uint32_t k = 0; k < 8; ++k) x[k] = 8 * o + k < n_dense ? x[k] : 0ull;
+   uint64_t y[8];
+#pragma unroll
+   for (uint32_t j = 0; j < 8; ++j) {
+      uint64_t v = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k) v |= ((x[k] >> (8 * j)) & 0xFFull) << (8 * k);
+      y[j] = transpose8x8(v);  // byte i: row 8j + i, bit k: entry 8o + k
+   }
+   u64x2* dst = reinterpret_cast<u64x2*>(m.octet_masks + ((static_cast<size_t>(o) * n_chunks + c) << 16) + 64 * w);
+#pragma unroll
+   for (uint32_t h = 0; h < 4; ++h) dst[h] = u64x2{y[2 * h], y[2 * h + 1]};
+}
+
+// popcount of bits a..b (inclusive) of a filter tile in global memory
+__device__ inline uint32_t rangePopcountG(const GlobalU64* f, uint32_t a, uint32_t b) {
+   uint32_t n = 0;
+   const uint32_t wa = a >> 6, wb = b >> 6;
+   for (uint32_t w = wa; w <= wb; ++w) {
+      const uint32_t x = w == wa ? (a & 63) : 0;
+      const uint32_t y = w == wb ? (b & 63) : 63;
+      n += __popcll(f[w] & rangeMask(x, y));
+   }
+   return n;
+}
+
+template <uint32_t kRound>
+__global__ __launch_bounds__(kOctetThreads) void mutationsCountOctetKernel(MutationsBatch m) {
+   __shared__ __align__(16) uint8_t mk[kChunkRows];
+   __shared__ unsigned long long cnt[2][kMaxBlockSlots + 1];
+   __shared__ unsigned long long lut[256];
+   __shared__ uint32_t hacc[kOctetDescCap][8];
+   __shared__ uint8_t work[kOctetMaxPartChunks];
+   __shared__ uint32_t item_s;
+   const uint32_t n_dense = m.list_counts[0];
+   const uint32_t n_oct = (n_dense + 7) / 8;
+   if (n_oct == 0) return;
+   const SeqColumnView& col = *m.col_dev;
+   const uint32_t n_chunks = m.n_chunks;
+   const uint32_t n_slots = col.n_slots;
+   const uint32_t n_items = col.n_blocks * kOctetParts * n_oct;
+   const uint32_t per_part = (n_chunks + kOctetParts - 1) / kOctetParts;
+   const uint32_t tid = threadIdx.x;
+   const uint32_t lane = tid & 63;
+   const uint32_t wave = tid >> 6;
+   const GlobalU32* light_ptr = toGlobal<GlobalU32>(col.light_ptr);
+   const GlobalU32* group_ptr = toGlobal<GlobalU32>(col.group_ptr);
+   const GlobalU32* heavy_ptr = toGlobal<GlobalU32>(col.heavy_ptr);
+   const GlobalU64* filters = toGlobal<GlobalU64>(m.filters);
+   for (uint32_t i = tid; i < 256; i += kOctetThreads) {
+      uint64_t v = 0;
+      for (uint32_t k = 0; k < 8; ++k) v |= static_cast<uint64_t>((i >> k) & 1u) << (8 * k);
+      lut[i] = v;
+   }
+   const u32x4* mk4 = reinterpret_cast<const u32x4*>(mk);
+   for (;;) {
+      __syncthreads();  // the previous item is finished
+      if (tid == 0) item_s = atomicAdd(&m.list_counts[2], 1u);
+      __syncthreads();
+      const uint32_t item = item_s;
+      if (item >= n_items) break;
+      const uint32_t o = item % n_oct;
+      const uint32_t part = (item / n_oct) % kOctetParts;
+      const uint32_t b = item / (n_oct * kOctetParts);
+      const uint32_t c_lo = part * per_part;
+      const uint32_t c_hi = c_lo + per_part < n_chunks ? c_lo + per_part : n_chunks;
+      uint32_t* out = part == 0   ? m.partial
+                      : part == 1 ? m.partial_b
+                                  : m.partial_c + static_cast<size_t>(part - 2) * m.max_queries * n_slots;
+      const uint32_t s0 = col.block_slot_start[b];
+      const uint32_t ns = col.block_slot_start[b + 1] - s0;
+      // entries past the end take the last entry's query id (their loads stay unconditional) and
+      // are masked out
+      uint32_t q[8];
+      bool valid[8];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k) {
+         valid[k] = 8 * o + k < n_dense;
+         q[k] = m.lists[valid[k] ? 8 * o + k : n_dense - 1];
+      }
+      for (uint32_t t = tid; t <= kMaxBlockSlots; t += kOctetThreads) {
+         cnt[0][t] = 0;
+         cnt[1][t] = 0;
+      }
+      // which chunks of the part have work: one of the eight filters has rows there and the tile
+      // is not empty (one thread per chunk, all loads issued together)
+      for (uint32_t t = tid; c_lo + t < c_hi; t += kOctetThreads) {
+         const uint32_t c = c_lo + t;
+         uint32_t cards[8];
+#pragma unroll
+         for (uint32_t k = 0; k < 8; ++k) cards[k] = m.chunk_cards[static_cast<size_t>(q[k]) * n_chunks + c];
+         const uint32_t tile = b * n_chunks + c;
+         const bool nonempty = light_ptr[tile] != light_ptr[tile + 1] || group_ptr[tile] != group_ptr[tile + 1] ||
+                               heavy_ptr[tile] != heavy_ptr[tile + 1];
+         bool any = false;
+#pragma unroll
+         for (uint32_t k = 0; k < 8; ++k) any |= valid[k] & (cards[k] != 0);
+         work[t] = any && nonempty ? 1 : 0;
+      }
+      __syncthreads();
+      uint32_t since_flush = 0;
+      bool first_flush = true;
+      auto flush = [&]() {
+         __syncthreads();
+         for (uint32_t t = tid; t < ns; t += kOctetThreads) {
+            const unsigned long long v[2] = {cnt[0][t], cnt[1][t]};
+            cnt[0][t] = 0;
+            cnt[1][t] = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) {
+               if (!valid[k]) continue;
+               uint32_t* ot = out + static_cast<size_t>(8 * o + k) * n_slots + s0 + t;
+               const uint32_t add = static_cast<uint32_t>((v[k >> 2] >> (16 * (k & 3))) & 0xFFFFu);
+               *ot = first_flush ? add : *ot + add;
+            }
+         }
+         first_flush = false;
+         since_flush = 0;
+         __syncthreads();
+      };
+      auto addRun = [&](uint32_t slot, uint64_t lo, uint64_t hi) {
+         if (lo) atomicAdd(&cnt[0][slot], static_cast<unsigned long long>(lo));
+         if (hi) atomicAdd(&cnt[1][slot], static_cast<unsigned long long>(hi));
+      };
+      auto next = [&](uint32_t c) -> uint32_t {
+         while (c < c_hi && !work[c - c_lo]) ++c;
+         return c;
+      };
+      // this thread's 64 bytes of the chunk's row bytes
+      u32x4 pf[4];
+      auto fetch = [&](uint32_t c) {
+         const GlobalU32x4* src = toGlobal<GlobalU32x4>(m.octet_masks + ((static_cast<size_t>(o) * n_chunks + c) << 16));
+#pragma unroll
+         for (uint32_t h = 0; h < 4; ++h) pf[h] = src[4 * tid + h];
+      };
+      uint32_t c = next(c_lo);
+      if (c < c_hi) fetch(c);
+      while (c < c_hi) {
+         if (since_flush == kSharedFlushChunks) flush();
+         ++since_flush;
+         __syncthreads();  // the previous tile's readers of mk are done
+#pragma unroll
+         for (uint32_t h = 0; h < 4; ++h) reinterpret_cast<u32x4*>(mk)[4 * tid + h] = pf[h];
+         const uint32_t tile = b * n_chunks + c;
+         const uint32_t l0 = light_ptr[tile], l1 = light_ptr[tile + 1];
+         const uint32_t h0 = heavy_ptr[tile], h1 = heavy_ptr[tile + 1];
+         const uint32_t g0 = group_ptr[tile], g1 = group_ptr[tile + 1];
+         const uint32_t cn = next(c + 1);
+         if (cn < c_hi) fetch(cn);
+         __syncthreads();
+         // light entries: runs of a slot summed in eight 8-bit fields (a light slot has at most 32
+         // rows in a chunk)
+         {
+            const GlobalU32x4* e4 = toGlobal<GlobalU32x4>(col.light + l0);
+            const uint32_t n4 = (l1 - l0) >> 2;
+            constexpr uint32_t kStride = kOctetThreads * kRound;
+            const uint32_t rounds = (n4 + kStride - 1) / kStride;
+            uint32_t cur = kPadSlot;
+            uint64_t acc = 0;
+            u32x4 bufA[kRound], bufB[kRound];
+            auto load = [&](uint32_t round, u32x4* buf) {
+               const uint32_t base = round * kStride + tid * kRound;
+#pragma unroll
+               for (uint32_t u = 0; u < kRound; ++u) {
+                  const uint32_t jj = base + u;
+                  buf[u] = e4[jj < n4 ? jj : n4 - 1];
+               }
+            };
+            auto use = [&](uint32_t round, const u32x4* buf) {
+               const uint32_t base = round * kStride + tid * kRound;
+               uint32_t mb[4 * kRound];
+#pragma unroll
+               for (uint32_t u = 0; u < kRound; ++u) {
+#pragma unroll
+                  for (uint32_t k = 0; k < 4; ++k) mb[4 * u + k] = mk[buf[u][k] & 0xFFFFu];
+               }
+               uint64_t sp[4 * kRound];
+#pragma unroll
+               for (uint32_t i = 0; i < 4 * kRound; ++i) sp[i] = lut[mb[i]];
+#pragma unroll
+               for (uint32_t u = 0; u < kRound; ++u) {
+                  const bool ok = base + u < n4;
+#pragma unroll
+                  for (uint32_t k = 0; k < 4; ++k) {
+                     const uint32_t slot = buf[u][k] >> 16;
+                     if (slot != cur) {
+                        if (acc) addRun(cur, widen4(static_cast<uint32_t>(acc)), widen4(static_cast<uint32_t>(acc >> 32)));
+                        cur = slot;
+                        acc = 0;
+                     }
+                     if (ok) acc += sp[4 * u + k];
+                  }
+               }
+            };
+            if (rounds > 0) {
+               load(0, bufA);
+               for (uint32_t round = 0; round < rounds; round += 2) {
+                  if (round + 1 < rounds) load(round + 1, bufB);
+                  use(round, bufA);
+                  if (round + 1 < rounds) {
+                     if (round + 2 < rounds) load(round + 2, bufA);
+                     use(round + 1, bufB);
+                  }
+               }
+            }
+            // slot 4095 is the padding slot: its counters are never read
+            if (acc) addRun(cur, widen4(static_cast<uint32_t>(acc)), widen4(static_cast<uint32_t>(acc >> 32)));
+         }
+         // groups of eight rows of one slot: a group's eight rows summed in 8-bit fields, runs in
+         // 16-bit fields (at most 4096 rows of a slot in a chunk)
+         {
+            const GlobalU32x4* gr = toGlobal<GlobalU32x4>(col.groups) + g0;
+            const GlobalU16* gh = toGlobal<GlobalU16>(col.group_hdr) + g0;
+            const uint32_t ng = g1 - g0;
+            constexpr uint32_t kStride = kOctetThreads * kRound;
+            const uint32_t rounds = (ng + kStride - 1) / kStride;
+            uint32_t cur = kPadSlot;
+            uint64_t lo = 0, hi = 0;
+            u32x4 rowsA[kRound], rowsB[kRound];
+            uint32_t hdrA[kRound], hdrB[kRound];
+            auto load = [&](uint32_t round, u32x4* rows, uint32_t* hdr) {
+               const uint32_t base = round * kStride + tid * kRound;
+#pragma unroll
+               for (uint32_t u = 0; u < kRound; ++u) {
+                  const uint32_t jj = base + u < ng ? base + u : ng - 1;
+                  rows[u] = gr[jj];
+                  hdr[u] = gh[jj];
+               }
+            };
+            auto use = [&](uint32_t round, const u32x4* rows, const uint32_t* hdr) {
+               const uint32_t base = round * kStride + tid * kRound;
+#pragma unroll
+               for (uint32_t u = 0; u < kRound; ++u) {
+                  uint32_t mb[8];
+#pragma unroll
+                  for (uint32_t t = 0; t < 8; ++t) {
+                     const uint32_t row = (t & 1u) ? rows[u][t >> 1] >> 16 : rows[u][t >> 1] & 0xFFFFu;
+                     mb[t] = mk[row];
+                  }
+                  const uint32_t h = base + u < ng ? hdr[u] : 0xFFFFu;
+                  const uint32_t slot = h & 0xFFFu;
+                  const uint32_t mm = h == 0xFFFFu ? 0u : (h >> 12) + 1;
+                  uint64_t g8 = 0;
+#pragma unroll
+                  for (uint32_t t = 0; t < 8; ++t) {
+                     if (t < mm) g8 += lut[mb[t]];
+                  }
+                  if (slot != cur) {
+                     addRun(cur, lo, hi);
+                     cur = slot;
+                     lo = 0;
+                     hi = 0;
+                  }
+                  lo += widen4(static_cast<uint32_t>(g8));
+                  hi += widen4(static_cast<uint32_t>(g8 >> 32));
+               }
+            };
+            if (rounds > 0) {
+               load(0, rowsA, hdrA);
+               for (uint32_t round = 0; round < rounds; round += 2) {
+                  if (round + 1 < rounds) load(round + 1, rowsB, hdrB);
+                  use(round, rowsA, hdrA);
+                  if (round + 1 < rounds) {
+                     if (round + 2 < rounds) load(round + 2, rowsA, hdrA);
+                     use(round + 1, rowsB, hdrB);
+                  }
+               }
+            }
+            addRun(cur, lo, hi);
+         }
+         // containers.  Bitsets: thread t takes word t of every bitset of the tile, against the eight
+         // filters' word t (loaded once per tile); a wavefront's eight sums go through two 64-bit
+         // reductions of 16-bit fields into 32-bit LDS sums per container.  Runs, arrays and inline
+         // rows: one container per wavefront.  Every container's eight sums are then added to
+         // heavy_sums with global atomics.
+         for (uint32_t d0 = h0; d0 < h1; d0 += kOctetDescCap) {
+            const uint32_t d1 = h1 - d0 < kOctetDescCap ? h1 : d0 + kOctetDescCap;
+            for (uint32_t i = tid; i < (d1 - d0) * 8; i += kOctetThreads) hacc[i >> 3][i & 7] = 0;
+            __syncthreads();
+            uint64_t fwd[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) fwd[k] = filters[(static_cast<size_t>(q[k]) * n_chunks + c) * kChunkWords + tid];
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) fwd[k] = valid[k] ? fwd[k] : 0ull;
+            const GlobalU32x2* descs = toGlobal<GlobalU32x2>(col.vert.desc);
+            for (uint32_t d = d0; d < d1; ++d) {
+               const u32x2 dw = descs[d];
+               const uint32_t type = descType(dw[0]);
+               const uint32_t card = descCard(dw[0]);
+               uint64_t p01 = 0, p23 = 0;  // four 16-bit fields each: queries 0-3 and 4-7
+               if (type == CT_BITSET) {
+                  const uint64_t x = toGlobal<GlobalU64>(col.vert.bits)[static_cast<size_t>(dw[1]) * kChunkWords + tid];
+#pragma unroll
+                  for (uint32_t k = 0; k < 4; ++k) {
+                     p01 |= static_cast<uint64_t>(__popcll(x & fwd[k])) << (16 * k);
+                     p23 |= static_cast<uint64_t>(__popcll(x & fwd[k + 4])) << (16 * k);
+                  }
+               } else if (wave == (d - d0) % (kOctetThreads / 64)) {
+                  uint32_t n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                  if (type == CT_RUN) {
+                     const GlobalU32* rr = toGlobal<GlobalU32>(col.vert.runs + dw[1]);
+                     for (uint32_t t = lane; t < card; t += 64) {
+                        const uint32_t st = rr[t] & 0xFFFFu;
+                        const uint32_t en = st + (rr[t] >> 16);
+#pragma unroll
+                        for (uint32_t k = 0; k < 8; ++k) {
+                           if (valid[k]) n[k] += rangePopcountG(filters + (static_cast<size_t>(q[k]) * n_chunks + c) * kChunkWords, st, en);
+                        }
+                     }
+                  } else if (type == CT_ARRAY) {
+                     const GlobalU16* a = toGlobal<GlobalU16>(col.vert.arr + dw[1]);
+                     for (uint32_t t = lane; t < card; t += 64) {
+                        const uint64_t v = lut[mk[a[t]]];
+#pragma unroll
+                        for (uint32_t k = 0; k < 8; ++k) n[k] += static_cast<uint32_t>((v >> (8 * k)) & 0xFFu);
+                     }
+                  } else if (lane == 0) {
+                     for (uint32_t t = 0; t < card; ++t) {
+                        const uint64_t v = lut[mk[(dw[1] >> (16 * t)) & 0xFFFFu]];
+#pragma unroll
+                        for (uint32_t k = 0; k < 8; ++k) n[k] += static_cast<uint32_t>((v >> (8 * k)) & 0xFFu);
+                     }
+                  }
+                  // a lane's share of a run or array container stays far below 2^16 per query
+#pragma unroll
+                  for (uint32_t k = 0; k < 4; ++k) {
+                     p01 |= static_cast<uint64_t>(n[k]) << (16 * k);
+                     p23 |= static_cast<uint64_t>(n[k + 4]) << (16 * k);
+                  }
+               }
+               // wavefront sums: at most 64 lanes x 64 rows per field for bitsets (< 2^16)
+               const uint64_t s01 = waveSum64(p01);
+               const uint64_t s23 = waveSum64(p23);
+               if (lane == 0 && (s01 | s23)) {
+#pragma unroll
+                  for (uint32_t k = 0; k < 4; ++k) {
+                     const uint32_t v0 = static_cast<uint32_t>((s01 >> (16 * k)) & 0xFFFFu);
+                     const uint32_t v1 = static_cast<uint32_t>((s23 >> (16 * k)) & 0xFFFFu);
+                     if (v0) atomicAdd(&hacc[d - d0][k], v0);
+                     if (v1) atomicAdd(&hacc[d - d0][k + 4], v1);
+                  }
+               }
+            }
+            __syncthreads();
+            for (uint32_t i = tid; i < (d1 - d0) * 8; i += kOctetThreads) {
+               const uint32_t k = i & 7;
+               const uint32_t v = hacc[i >> 3][k];
+               if (valid[k] && v) {
+                  const uint32_t slot = descSlot(descs[d0 + (i >> 3)][0]);
+                  atomicAdd(&m.heavy_sums[static_cast<size_t>(8 * o + k) * n_slots + s0 + slot], v);
+               }
+            }
+            __syncthreads();
+         }
+         c = cn;
+      }
+      flush();
    }
 }
 
@@ -1994,8 +2426,30 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
    hipError_t err = hipSuccess;
    hipLaunchKernelGGL(batchPlanKernel, dim3(1), dim3(1024), 0, stream, m);
    if ((err = hipGetLastError()) != hipSuccess) return err;
-   if (m.heavy_sums != nullptr) {
+   if (m.heavy_sums != nullptr && m.shared_mode == 2) {
+      // shared scan, eight dense queries per pass: row bytes, then the persistent counting kernel
+      const uint32_t max_octets = (m.max_queries + 7) / 8;
+      if (static_cast<uint64_t>(max_octets) * m.n_chunks > 65535) return hipErrorInvalidValue;
+      if ((m.n_chunks + kOctetParts - 1) / kOctetParts > kOctetMaxPartChunks) return hipErrorInvalidValue;
+      if (m.octet_masks == nullptr || m.partial_c == nullptr || m.partial_b == nullptr) return hipErrorInvalidValue;
+      if (m.n_chunks > 0) {
+         hipLaunchKernelGGL(octetMasksKernel, dim3(kChunkWords / kOctetMaskThreads, max_octets * m.n_chunks),
+                            dim3(kOctetMaskThreads), 0, stream, m);
+         if ((err = hipGetLastError()) != hipSuccess) return err;
+      }
+      const uint32_t groups = m.persistent_groups == 0 ? 1 : m.persistent_groups;
+      if (m.unroll == 16) {
+         hipLaunchKernelGGL(mutationsCountOctetKernel<4>, dim3(groups), dim3(kOctetThreads), 0, stream, m);
+      } else {
+         hipLaunchKernelGGL(mutationsCountOctetKernel<2>, dim3(groups), dim3(kOctetThreads), 0, stream, m);
+      }
+      if ((err = hipGetLastError()) != hipSuccess) return err;
+      const uint32_t sblocks = (h.n_slots + 255) / 256;
+      hipLaunchKernelGGL(foldSharedPartsKernel, dim3(sblocks == 0 ? 1 : (sblocks < 64 ? sblocks : 64), m.max_queries),
+                         dim3(256), 0, stream, m, h.n_slots, kOctetParts);
+   } else if (m.heavy_sums != nullptr) {
       // shared scan, four dense queries per pass, chunks in two parts
+      if ((m.n_chunks + 1) / 2 > kSharedMaxPartChunks) return hipErrorInvalidValue;
       const uint32_t n_xblocks = 8 * ((h.n_blocks + 7) / 8);
       const dim3 grid(n_xblocks * 2 * ((m.max_queries + 3) / 4));
       if (m.unroll == 16) {
@@ -2006,7 +2460,7 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
       if ((err = hipGetLastError()) != hipSuccess) return err;
       const uint32_t sblocks = (h.n_slots + 255) / 256;
       hipLaunchKernelGGL(foldSharedPartsKernel, dim3(sblocks == 0 ? 1 : (sblocks < 64 ? sblocks : 64), m.max_queries),
-                         dim3(256), 0, stream, m, h.n_slots);
+                         dim3(256), 0, stream, m, h.n_slots, 2u);
    } else {
       const uint32_t n_xblocks = 8 * ((h.n_blocks + 7) / 8);
       const dim3 count_grid(n_xblocks * m.max_queries);

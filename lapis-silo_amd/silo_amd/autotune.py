@@ -103,9 +103,9 @@ def autotune(engine, table, probes: list, column: str = "main", reps: int = 5, l
 
 
 def autotune_batch(engine, table, probes: list, column: str = "main", size: int = 64, reps: int = 3, log=None) -> dict:
-    """Chooses the batched dense kernel ("batch_shared_scan": one pass over the index per query, or
-    one pass per four queries) by timing a batch of `size` Mutations queries over the probe filters
-    both ways; the results of the two must agree.  The knob is applied."""
+    """Chooses the batched dense kernel ("batch_shared_scan": one pass over the index per query, one
+    per four queries or one per eight) by timing a batch of `size` Mutations queries over the probe
+    filters each way; the results must agree.  The knob is applied."""
     say = log or (lambda *_: None)
     queries = [{"filter": probes[i % len(probes)], "action": "mutations", "column": column, "min_proportion": 0.05}
                for i in range(size)]
@@ -113,21 +113,23 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
     engine.set_tuning("batch_max_queries", size)
     times = {}
     results = {}
-    for shared in (0, 1):
+    for shared in (0, 1, 2):
         engine.set_tuning("batch_shared_scan", shared)
-        results[shared] = table.batch(queries)
+        results[shared] = table.batch(queries, arrays=True)
         t = time.perf_counter()
         for _ in range(reps):
-            table.batch(queries)
+            table.batch(queries, arrays=True)
         times[shared] = (time.perf_counter() - t) / reps
-    if results[0] != results[1]:
+    rows = {k: [r.rows() for r in v] for k, v in results.items()}
+    if not rows[0] == rows[1] == rows[2]:
         raise RuntimeError("shared-scan and per-query batch results differ")
     best = min(times, key=times.get)
     engine.set_tuning("batch_shared_scan", best)
     engine.set_tuning("batch_max_queries", saved)
     say(f"batch of {size} probe Mutations queries, ms: per-query kernel {times[0] * 1e3:.2f}, "
-        f"shared scan {times[1] * 1e3:.2f}; chose batch_shared_scan={best}")
-    return {"batch_shared_scan": best, "batch_ms": {"per_query": round(times[0] * 1e3, 3), "shared": round(times[1] * 1e3, 3)}}
+        f"four per pass {times[1] * 1e3:.2f}, eight per pass {times[2] * 1e3:.2f}; chose batch_shared_scan={best}")
+    return {"batch_shared_scan": best, "batch_ms": {"per_query": round(times[0] * 1e3, 3), "four": round(times[1] * 1e3, 3),
+                                                   "eight": round(times[2] * 1e3, 3)}}
 
 
 def probe_filters(ds, n: int = 8) -> list:

@@ -64,7 +64,7 @@ def check(et, ot, queries):
         assert g == expected(ot, q), q
 
 
-@pytest.mark.parametrize("shared", [0, 1])
+@pytest.mark.parametrize("shared", [0, 1, 2])
 @pytest.mark.parametrize("graphs", [0, 1])
 @pytest.mark.parametrize("sparse_max", [-1, 300, 1 << 40])
 def test_batch_matches_oracle(small, tuning, graphs, sparse_max, shared):
@@ -137,8 +137,9 @@ def test_batch_query_errors_are_per_query(small):
 
 
 def test_shared_scan_many_chunks(engine, tuning):
-    """More than 2 x 15 chunks: the shared scan's parts each gather several chunk runs in 16-bit
-    counters and flush them more than once; results equal the per-query kernel's and the oracle's."""
+    """More than 2 x 15 chunks: the four-query scan's halves and the eight-query scan's quarters each
+    gather several chunk runs in 16-bit counters and flush them more than once; results equal the
+    per-query kernel's and the oracle's."""
     from oracle.oracle import OracleTable
 
     ds, ts = small_spec(seed=31, n_rows=40000, L=300, chunk=1024)
@@ -153,13 +154,13 @@ def test_shared_scan_many_chunks(engine, tuning):
                                  ("(nof 1 0 (has_mut main 5) (has_mut main 40) (has_mut main 77))", 0.02),
                                  ("(date_between date %d %d)" % (ds.spec.date_base, ds.spec.date_base + 20), 0.0)]]
         want = [ot.mutations(q["filter"], "main", q["min_proportion"]) for q in queries]
-        per_query = []
-        for shared in (0, 1):
+        per_mode = []
+        for shared in (0, 1, 2):
             tuning.set_tuning("batch_shared_scan", shared)
             got = et.batch(queries)
             assert got == want, shared
-            per_query.append(got)
-        assert per_query[0] == per_query[1]
+            per_mode.append(got)
+        assert per_mode[0] == per_mode[1] == per_mode[2]
     finally:
         et.close()
         ot.close()
