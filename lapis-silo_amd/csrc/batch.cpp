@@ -120,7 +120,7 @@ void setupLane(Table& t, BatchLane& lane) {
    lane.filters = DeviceBuffer(words * 8 * Q);
    lane.cards = DeviceBuffer(8ULL * Q + 4ULL * Q * t.n_chunks);
    lane.lists = DeviceBuffer(8ULL * Q);
-   lane.list_counts = DeviceBuffer(16);
+   lane.list_counts = DeviceBuffer(64);  // dense / sparse list sizes, work counters
    lane.n_rows = DeviceBuffer(4ULL * Q);
    lane.hist = DeviceBuffer(4ULL * lane.hist_capacity);
    lane.rows = DeviceBuffer(sizeof(MutationRowDev) * static_cast<size_t>(Q) * lane.row_capacity);

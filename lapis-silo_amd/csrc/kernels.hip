@@ -859,6 +859,56 @@ __global__ __launch_bounds__(kMutThreads) __attribute__((amdgpu_waves_per_eu(kPa
 //   bnd[p], bnd[L + p], bnd[2L + p]  A(p), B(p), C(p) as in the dense path
 // Both arrays must be zero on entry; the sparse finalize kernel clears what it reads.
 // ------------------------------------------------------------------------------------------------
+// Rows of a filter, each counted by a whole wavefront: a wavefront takes 64 filter words (one per
+// lane) and visits their rows one at a time; lane l adds the row's mutations l, l + 64, ... and its
+// missing runs l, l + 64, ..., lane 0 its covered-range boundaries.  A row then costs about two
+// dependent loads however many mutations it has.
+__device__ inline void sparseCountRows(const SeqColumnView& col, const uint32_t* chunk_sizes, const uint64_t* filter,
+                                       uint32_t n_words, uint32_t* counts, int32_t* bnd, uint32_t wave_id,
+                                       uint32_t n_waves) {
+   const uint32_t lane = threadIdx.x & 63;
+   const uint32_t L = col.genome_length;
+   const GlobalU32* cov = toGlobal<GlobalU32>(col.cov);
+   const GlobalU32* nrun_ptr = toGlobal<GlobalU32>(col.nrun_ptr);
+   const GlobalU32* nruns = toGlobal<GlobalU32>(col.nruns);
+   const GlobalU32* mut_ptr = toGlobal<GlobalU32>(col.row_mut_ptr);
+   const GlobalU32* muts = toGlobal<GlobalU32>(col.row_muts);
+   const GlobalU64* fw = toGlobal<GlobalU64>(filter);
+   for (uint32_t base = wave_id * 64; base < n_words; base += n_waves * 64) {
+      const uint32_t wd = base + lane;
+      const uint32_t wc = wd < n_words ? wd : n_words - 1;
+      // bits outside the row layout (left by Threshold's complement of negated children) have no
+      // entries in the index: the dense path counts them covered with the local reference
+      uint64_t x = fw[wc] & layoutMask(toGlobal<GlobalU32>(chunk_sizes)[wc >> 10], wc & (kChunkWords - 1));
+      if (wd >= n_words) x = 0;
+      uint64_t pending = __ballot(x != 0);
+      while (pending) {
+         const uint32_t src = static_cast<uint32_t>(__builtin_ctzll(pending));
+         const uint64_t xs = __shfl(x, static_cast<int>(src), 64);
+         const uint32_t r = ((base + src) << 6) | static_cast<uint32_t>(__builtin_ctzll(xs));
+         if (lane == src) x &= x - 1;
+         pending = __ballot(x != 0);
+         const uint32_t cv = cov[r];
+         const uint32_t n0 = nrun_ptr[r], n1 = nrun_ptr[r + 1];
+         const uint32_t m0 = mut_ptr[r], m1 = mut_ptr[r + 1];
+         if (lane == 0) {
+            const uint32_t st = cv & 0xFFFFu, en = cv >> 16;
+            if (st > 0) atomicAdd(&bnd[st], 1);
+            if (en < L) atomicAdd(&bnd[L + en], 1);
+         }
+         for (uint32_t k = n0 + lane; k < n1; k += 64) {
+            const uint32_t run = nruns[k];
+            atomicAdd(&bnd[2 * L + (run & 0xFFFFu)], 1);
+            if ((run >> 16) < L) atomicAdd(&bnd[2 * L + (run >> 16)], -1);
+         }
+         for (uint32_t k = m0 + lane; k < m1; k += 64) {
+            const uint32_t mu = muts[k];
+            atomicAdd(&counts[(mu >> 16) * L + (mu & 0xFFFFu)], 1u);
+         }
+      }
+   }
+}
+
 __global__ __launch_bounds__(256) void sparseCountKernel(const SeqColumnView* __restrict__ colp,
                                                          const uint32_t* __restrict__ chunk_sizes, uint32_t n_words,
                                                          const uint64_t* __restrict__ filter,
@@ -866,33 +916,8 @@ __global__ __launch_bounds__(256) void sparseCountKernel(const SeqColumnView* __
                                                          long long sparse_max, uint32_t* __restrict__ counts,
                                                          int32_t* __restrict__ bnd) {
    if (!sparseActive(filter_card, sparse_max)) return;
-   const SeqColumnView& col = *colp;
-   const uint32_t L = col.genome_length;
-   for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n_words; w += gridDim.x * blockDim.x) {
-      uint64_t x = filter[w];
-      if (x == 0) continue;
-      const uint32_t size = chunk_sizes[w >> 10];
-      while (x) {
-         const uint32_t r = (w << 6) | static_cast<uint32_t>(__builtin_ctzll(x));
-         x &= x - 1;
-         // bits outside the row layout (left by Threshold's complement of negated children) have
-         // no entries in the index: the dense path counts them covered with the local reference
-         if ((r & 0xFFFFu) >= size) continue;
-         const uint32_t cv = col.cov[r];
-         const uint32_t st = cv & 0xFFFFu, en = cv >> 16;
-         if (st > 0) atomicAdd(&bnd[st], 1);
-         if (en < L) atomicAdd(&bnd[L + en], 1);
-         for (uint32_t k = col.nrun_ptr[r], e = col.nrun_ptr[r + 1]; k < e; ++k) {
-            const uint32_t run = col.nruns[k];
-            atomicAdd(&bnd[2 * L + (run & 0xFFFFu)], 1);
-            if ((run >> 16) < L) atomicAdd(&bnd[2 * L + (run >> 16)], -1);
-         }
-         for (uint32_t k = col.row_mut_ptr[r], e = col.row_mut_ptr[r + 1]; k < e; ++k) {
-            const uint32_t m = col.row_muts[k];
-            atomicAdd(&counts[(m >> 16) * L + (m & 0xFFFFu)], 1u);
-         }
-      }
-   }
+   sparseCountRows(*colp, chunk_sizes, filter, n_words, counts, bnd, blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6),
+                   gridDim.x * (blockDim.x / 64));
 }
 
 // Sums the chunk groups' partial slot counts into group 0, in place (one thread per slot).
@@ -1997,33 +2022,10 @@ __global__ __launch_bounds__(256) void sparseCountBatchKernel(MutationsBatch m) 
    const uint32_t q = m.lists[m.max_queries + e];
    const SeqColumnView& col = *m.col_dev;
    const size_t L = col.genome_length;
-   uint32_t* counts = m.sparse_counts + static_cast<size_t>(e) * col.n_symbols * L;
-   int32_t* bnd = m.sparse_bnd + static_cast<size_t>(e) * 3 * L;
-   const uint64_t* filter = m.filters + static_cast<size_t>(q) * m.n_chunks * kChunkWords;
-   const uint32_t n_words = m.n_chunks * kChunkWords;
-   for (uint32_t wd = blockIdx.x * blockDim.x + threadIdx.x; wd < n_words; wd += gridDim.x * blockDim.x) {
-      uint64_t x = filter[wd];
-      if (x == 0) continue;
-      const uint32_t size = m.chunk_sizes[wd >> 10];
-      while (x) {
-         const uint32_t r = (wd << 6) | static_cast<uint32_t>(__builtin_ctzll(x));
-         x &= x - 1;
-         if ((r & 0xFFFFu) >= size) continue;
-         const uint32_t cv = col.cov[r];
-         const uint32_t st = cv & 0xFFFFu, en = cv >> 16;
-         if (st > 0) atomicAdd(&bnd[st], 1);
-         if (en < L) atomicAdd(&bnd[L + en], 1);
-         for (uint32_t k = col.nrun_ptr[r], ke = col.nrun_ptr[r + 1]; k < ke; ++k) {
-            const uint32_t run = col.nruns[k];
-            atomicAdd(&bnd[2 * L + (run & 0xFFFFu)], 1);
-            if ((run >> 16) < L) atomicAdd(&bnd[2 * L + (run >> 16)], -1);
-         }
-         for (uint32_t k = col.row_mut_ptr[r], ke = col.row_mut_ptr[r + 1]; k < ke; ++k) {
-            const uint32_t mu = col.row_muts[k];
-            atomicAdd(&counts[(mu >> 16) * L + (mu & 0xFFFFu)], 1u);
-         }
-      }
-   }
+   sparseCountRows(col, m.chunk_sizes, m.filters + static_cast<size_t>(q) * m.n_chunks * kChunkWords,
+                   m.n_chunks * kChunkWords, m.sparse_counts + static_cast<size_t>(e) * col.n_symbols * L,
+                   m.sparse_bnd + static_cast<size_t>(e) * 3 * L, blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6),
+                   gridDim.x * (blockDim.x / 64));
 }
 
 __device__ inline SlotSource batchSource(const MutationsBatch& m, uint32_t e, SlotSource*) {
