@@ -28,6 +28,24 @@ __device__ inline void waveSync() {
 
 __device__ inline uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Index arrays are read through global-address-space pointers: the column view holds them as
+// generic pointers, which would compile to flat loads (counted against the LDS counter too, so every
+// LDS wait would also wait for them).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef const u32x4 __attribute__((address_space(1))) GlobalU32x4;
+typedef const uint16_t __attribute__((address_space(1))) GlobalU16;
+typedef const uint32_t __attribute__((address_space(1))) GlobalU32;
+typedef const uint64_t __attribute__((address_space(1))) GlobalU64;
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+typedef const u64x2 __attribute__((address_space(1))) GlobalU64x2;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef const u32x2 __attribute__((address_space(1))) GlobalU32x2;
+template <typename G>
+__device__ inline const G* toGlobal(const void* p) {
+   return (const G*)(p);
+}
+
+
 __device__ inline uint64_t layoutMask(uint32_t chunk_size, uint32_t word) {
    const uint32_t lo = word * 64;
    if (chunk_size >= lo + 64) return ~0ULL;
@@ -60,7 +78,8 @@ __device__ inline uint32_t waveLowerBound(uint32_t lo, uint32_t hi, Before befor
 }
 
 // First run whose inclusive end is >= x.
-__device__ inline uint32_t firstRunEndingAtOrAfter(const uint32_t* r, uint32_t n, uint32_t x) {
+template <typename P>
+__device__ inline uint32_t firstRunEndingAtOrAfter(P r, uint32_t n, uint32_t x) {
    uint32_t lo = 0, hi = n;  // first run with start > x
    while (lo < hi) {
       const uint32_t mid = (lo + hi) >> 1;
@@ -100,9 +119,9 @@ __device__ inline void accumulateContainer(const Desc d, const ContainerArenas& 
          if ((v >> 6) == my_word) acc |= 1ULL << (v & 63);
       }
    } else if (type == CT_BITSET) {
-      acc |= ar.bits[static_cast<size_t>(d.w1) * kChunkWords + my_word];
+      acc |= toGlobal<GlobalU64>(ar.bits)[static_cast<size_t>(d.w1) * kChunkWords + my_word];
    } else if (type == CT_ARRAY) {
-      const uint16_t* a = ar.arr + d.w1;
+      const GlobalU16* a = toGlobal<GlobalU16>(ar.arr + d.w1);
       const uint32_t b = waveLowerBound(0, card, [&](uint32_t i) { return static_cast<uint32_t>(a[i]) < lo; });
       const uint32_t e = hi >= kChunkRows ? card
                                           : waveLowerBound(b, card, [&](uint32_t i) { return static_cast<uint32_t>(a[i]) < hi; });
@@ -111,7 +130,7 @@ __device__ inline void accumulateContainer(const Desc d, const ContainerArenas& 
          atomicOr(reinterpret_cast<unsigned long long*>(&scratch[(v >> 6) & 63]), 1ULL << (v & 63));
       }
    } else {
-      const uint32_t* r = ar.runs + d.w1;
+      const GlobalU32* r = toGlobal<GlobalU32>(ar.runs + d.w1);
       const uint32_t first = firstRunEndingAtOrAfter(r, card, lo);
       for (uint32_t k = first + lane; k < card; k += 64) {
          const uint32_t s = r[k] & 0xFFFFu;
@@ -190,12 +209,12 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
             const uint32_t code = ranges[pd.pf_begin + lane];
             uint32_t* dst = pf + lane * 8;
             if (code >> 31) {
-               const Desc d = tv.store.desc[static_cast<size_t>(code & 0x7FFFFFFFu) * n_chunks + chunk];
-               dst[0] = d.w0;
-               dst[1] = d.w1;
+               const u32x2 d = toGlobal<GlobalU32x2>(tv.store.desc)[static_cast<size_t>(code & 0x7FFFFFFFu) * n_chunks + chunk];
+               dst[0] = d[0];
+               dst[1] = d[1];
             } else {
                const SeqColumnView& c = tv.seq[code >> 16];
-               const uint32_t* r = c.pos_ranges + (static_cast<size_t>(chunk) * c.genome_length + (code & 0xFFFFu)) * 6;
+               const GlobalU32* r = toGlobal<GlobalU32>(c.pos_ranges) + (static_cast<size_t>(chunk) * c.genome_length + (code & 0xFFFFu)) * 6;
                uint32_t v[6];
 #pragma unroll
                for (uint32_t k = 0; k < 6; ++k) v[k] = r[k];
@@ -224,8 +243,13 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                push(lmask);
                break;
             case OP_BITMAP: {
-               const Desc d = ins.b < kPrefetchSlots ? Desc{pf[ins.b * 8], pf[ins.b * 8 + 1]}
-                                                     : tv.store.desc[static_cast<size_t>(ins.a) * n_chunks + chunk];
+               Desc d;
+               if (ins.b < kPrefetchSlots) {
+                  d = Desc{pf[ins.b * 8], pf[ins.b * 8 + 1]};
+               } else {
+                  const u32x2 w = toGlobal<GlobalU32x2>(tv.store.desc)[static_cast<size_t>(ins.a) * n_chunks + chunk];
+                  d = Desc{w[0], w[1]};
+               }
                uint64_t acc = 0;
                scratch[lane] = 0;
                waveSync();
@@ -237,31 +261,42 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
             case OP_SYMBOLS: {
                // the position's entries in this chunk's tile, located by one record of six offsets
                const SeqColumnView& col = tv.seq[ins.a];
-               const uint32_t slot_base = col.block_slot_start[ins.b];
-               const uint32_t* pr = ins.g < kPrefetchSlots
-                                       ? pf + ins.g * 8
-                                       : col.pos_ranges + (static_cast<size_t>(chunk) * col.genome_length + ins.f) * 6;
-               const uint32_t l_lo = uniform(pr[0]), l_hi = uniform(pr[1]);
-               const uint32_t g_lo = uniform(pr[2]), g_hi = uniform(pr[3]);
-               const uint32_t h_lo = uniform(pr[4]), h_hi = uniform(pr[5]);
+               const uint32_t slot_base = toGlobal<GlobalU32>(col.block_slot_start)[ins.b];
+               uint32_t prv[6];
+               if (ins.g < kPrefetchSlots) {
+#pragma unroll
+                  for (uint32_t k = 0; k < 6; ++k) prv[k] = pf[ins.g * 8 + k];
+               } else {
+                  const GlobalU32* pr = toGlobal<GlobalU32>(col.pos_ranges) + (static_cast<size_t>(chunk) * col.genome_length + ins.f) * 6;
+#pragma unroll
+                  for (uint32_t k = 0; k < 6; ++k) prv[k] = pr[k];
+               }
+               const uint32_t l_lo = uniform(prv[0]), l_hi = uniform(prv[1]);
+               const uint32_t g_lo = uniform(prv[2]), g_hi = uniform(prv[3]);
+               const uint32_t h_lo = uniform(prv[4]), h_hi = uniform(prv[5]);
+               const GlobalU32* light = toGlobal<GlobalU32>(col.light);
+               const GlobalU32* slot_info = toGlobal<GlobalU32>(col.slot_info);
+               const GlobalU16* group_hdr = toGlobal<GlobalU16>(col.group_hdr);
+               const GlobalU32x4* groups = toGlobal<GlobalU32x4>(col.groups);
+               const GlobalU32x2* descs = toGlobal<GlobalU32x2>(col.vert.desc);
                uint64_t acc = 0;
                scratch[lane] = 0;
                waveSync();
                for (uint32_t k = l_lo + lane; k < l_hi; k += 64) {
-                  const uint32_t x = col.light[k];
+                  const uint32_t x = light[k];
                   const uint32_t row = x & 0xFFFFu;
-                  const uint32_t sym = col.slot_info[slot_base + (x >> 16)] >> 16;
+                  const uint32_t sym = slot_info[slot_base + (x >> 16)] >> 16;
                   if (((ins.e >> sym) & 1u) && (row >> 12) == slice) {
                      atomicOr(reinterpret_cast<unsigned long long*>(&scratch[(row >> 6) & 63]), 1ULL << (row & 63));
                   }
                }
                for (uint32_t k = g_lo + lane; k < g_hi; k += 64) {
-                  const uint32_t hdr = col.group_hdr[k];
-                  const uint32_t sym = col.slot_info[slot_base + (hdr & 0xFFFu)] >> 16;
+                  const uint32_t hdr = group_hdr[k];
+                  const uint32_t sym = slot_info[slot_base + (hdr & 0xFFFu)] >> 16;
                   if (!((ins.e >> sym) & 1u)) continue;
-                  const uint4 rows = reinterpret_cast<const uint4*>(col.groups)[k];
-                  const uint32_t r8[8] = {rows.x & 0xFFFFu, rows.x >> 16, rows.y & 0xFFFFu, rows.y >> 16,
-                                          rows.z & 0xFFFFu, rows.z >> 16, rows.w & 0xFFFFu, rows.w >> 16};
+                  const u32x4 rows = groups[k];
+                  const uint32_t r8[8] = {rows[0] & 0xFFFFu, rows[0] >> 16, rows[1] & 0xFFFFu, rows[1] >> 16,
+                                          rows[2] & 0xFFFFu, rows[2] >> 16, rows[3] & 0xFFFFu, rows[3] >> 16};
                   const uint32_t m = (hdr >> 12) + 1;
 #pragma unroll
                   for (uint32_t q = 0; q < 8; ++q) {
@@ -271,8 +306,9 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                   }
                }
                for (uint32_t k = h_lo; k < h_hi; ++k) {
-                  const Desc d = col.vert.desc[k];
-                  const uint32_t sym = col.slot_info[slot_base + descSlot(d.w0)] >> 16;
+                  const u32x2 dw = descs[k];
+                  const Desc d{dw[0], dw[1]};
+                  const uint32_t sym = slot_info[slot_base + descSlot(d.w0)] >> 16;
                   if ((ins.e >> sym) & 1u) accumulateContainer(d, col.vert, slice, lane, acc, scratch);
                }
                waveSync();
@@ -286,8 +322,8 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                const SeqColumnView& col = tv.seq[ins.a];
                const uint32_t p = ins.b;
                const size_t r0 = row_base + static_cast<size_t>(lane) * 64;
-               const uint4* cv = reinterpret_cast<const uint4*>(col.cov + r0);
-               uint4 c4[16];
+               const GlobalU32x4* cv = toGlobal<GlobalU32x4>(col.cov + r0);
+               u32x4 c4[16];
 #pragma unroll
                for (uint32_t u = 0; u < 16; ++u) c4[u] = cv[u];
                // candidate lists: lane j < n_lists loads list j's bounds
@@ -298,7 +334,7 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                uint32_t my_lo = 0, my_len = 0;
                if (lane < n_lists) {
                   const uint32_t k = lane + 1 < n_lists ? k_lo + lane : nb;
-                  const uint32_t* rp = col.run_index_ptr + (static_cast<size_t>(chunk) * (nb + 1) + k) * kSlicesPerChunk + slice;
+                  const GlobalU32* rp = toGlobal<GlobalU32>(col.run_index_ptr) + (static_cast<size_t>(chunk) * (nb + 1) + k) * kSlicesPerChunk + slice;
                   my_lo = rp[0];
                   my_len = rp[1] - rp[0];
                }
@@ -306,7 +342,7 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                uint64_t word = 0;
 #pragma unroll
                for (uint32_t u = 0; u < 16; ++u) {
-                  const uint32_t cs[4] = {c4[u].x, c4[u].y, c4[u].z, c4[u].w};
+                  const uint32_t cs[4] = {c4[u][0], c4[u][1], c4[u][2], c4[u][3]};
 #pragma unroll
                   for (uint32_t q = 0; q < 4; ++q) {
                      if ((cs[q] & 0xFFFFu) <= p && p < (cs[q] >> 16)) word |= 1ULL << (4 * u + q);
@@ -333,7 +369,7 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                         uint32_t j = 0;
 #pragma unroll
                         for (uint32_t t = 1; t < 8; ++t) j += idx >= start[t] ? 1u : 0u;
-                        e[u] = col.run_index[lo[j] + (idx - start[j])];
+                        e[u] = toGlobal<GlobalU64>(col.run_index)[lo[j] + (idx - start[j])];
                      }
                   }
 #pragma unroll
@@ -357,15 +393,16 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                const int32_t v = static_cast<int32_t>(ins.c);
                const int32_t v2 = static_cast<int32_t>(ins.f);
                const bool two = (ins.g & 1u) != 0;
-               const uint64_t nulls = col.null_bits[static_cast<size_t>(chunk) * kChunkWords + my_word];
-               const int4* vals = reinterpret_cast<const int4*>(col.values + row_base + static_cast<size_t>(lane) * 64);
-               int4 x[16];
+               const uint64_t nulls = toGlobal<GlobalU64>(col.null_bits)[static_cast<size_t>(chunk) * kChunkWords + my_word];
+               const GlobalU32x4* vals = toGlobal<GlobalU32x4>(col.values + row_base + static_cast<size_t>(lane) * 64);
+               u32x4 x[16];
 #pragma unroll
                for (uint32_t u = 0; u < 16; ++u) x[u] = vals[u];
                uint64_t word = 0;
 #pragma unroll
                for (uint32_t u = 0; u < 16; ++u) {
-                  const int32_t xs[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+                  const int32_t xs[4] = {static_cast<int32_t>(x[u][0]), static_cast<int32_t>(x[u][1]),
+                                         static_cast<int32_t>(x[u][2]), static_cast<int32_t>(x[u][3])};
 #pragma unroll
                   for (uint32_t q = 0; q < 4; ++q) {
                      word |= static_cast<uint64_t>(compareInt(ins.b, xs[q], v) && (!two || compareInt(ins.e, xs[q], v2)))
@@ -514,23 +551,6 @@ __device__ inline bool sparseActive(const unsigned long long* card, long long sp
 }
 __device__ inline bool denseActive(const unsigned long long* card, long long sparse_max) {
    return !sparseActive(card, sparse_max);
-}
-
-// Index arrays are read through global-address-space pointers: the column view holds them as
-// generic pointers, which would compile to flat loads (counted against the LDS counter too, so every
-// LDS wait would also wait for them).
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef const u32x4 __attribute__((address_space(1))) GlobalU32x4;
-typedef const uint16_t __attribute__((address_space(1))) GlobalU16;
-typedef const uint32_t __attribute__((address_space(1))) GlobalU32;
-typedef const uint64_t __attribute__((address_space(1))) GlobalU64;
-typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
-typedef const u64x2 __attribute__((address_space(1))) GlobalU64x2;
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-typedef const u32x2 __attribute__((address_space(1))) GlobalU32x2;
-template <typename G>
-__device__ inline const G* toGlobal(const void* p) {
-   return (const G*)(p);
 }
 
 // bit v of a filter tile: one 32-bit LDS read (a 64-bit read spans two banks)
@@ -2151,6 +2171,23 @@ constexpr uint32_t kHistLdsBins = 16384;
 // Bin ranges counted in LDS by separate work-groups; bins beyond the last range go to global atomics.
 constexpr uint32_t kHistRanges = 8;
 
+// Adds one to lh[k] for every active lane whose k is not ~0u, lanes holding the same k combined
+// first: one LDS atomic per distinct k of the wavefront instead of one per lane (group keys of
+// skewed columns repeat within 64 rows, and same-address LDS atomics serialise).
+__device__ inline void addCombined(uint32_t* lh, uint32_t k) {
+   uint64_t todo = __ballot(k != ~0u);
+   while (todo) {
+      const uint32_t lead = static_cast<uint32_t>(__builtin_ctzll(todo));
+      const uint32_t lk = __shfl(k, static_cast<int>(lead), 64);
+      const uint64_t same = __ballot(k == lk);
+      if ((threadIdx.x & 63) == lead) atomicAdd(&lh[lk], static_cast<uint32_t>(__popcll(same)));
+      todo &= ~same;
+   }
+}
+
+// Histograms of at most this many bins combine equal keys within a wavefront before counting.
+constexpr uint32_t kHistCombineBins = 4096;
+
 // Rows of one quarter chunk (16,384 rows) of a filter, each visited once: for a sparse quarter by
 // iterating the set bits of its words, for a dense one by walking the rows in order (consecutive
 // threads on consecutive rows, so the label loads coalesce).
@@ -2195,20 +2232,22 @@ __global__ __launch_bounds__(256) void groupHistogramKernel(uint32_t n_chunks, c
    for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) lh[i] = 0;
    __syncthreads();
    const uint64_t* f = filter + static_cast<size_t>(c) * kChunkWords + quarter * 256;
+   const bool combine = n_bins <= kHistCombineBins;
    forQuarterRows(f, card, (static_cast<size_t>(c) << 16) + quarter * 16384u, words, [&](size_t r) {
       uint32_t key = 0;
       bool in_table = true;
       for (uint32_t d = 0; d < dims.n; ++d) {
-         const uint32_t lab = dims.labels[d][r];
+         const uint32_t lab = toGlobal<GlobalU32>(dims.labels[d])[r];
          in_table &= lab != kInvalidId;
          key += lab * dims.stride[d];
       }
-      if (!in_table || key < lo) return;
-      if (key - lo < kHistLdsBins) {
+      const bool here = in_table && key >= lo && key - lo < kHistLdsBins;
+      if (combine) {
+         addCombined(lh, here ? key - lo : ~0u);
+      } else if (here) {
          atomicAdd(&lh[key - lo], 1u);
-      } else if (last) {
-         atomicAdd(&hist[key], 1u);
       }
+      if (in_table && last && key >= lo + kHistLdsBins) atomicAdd(&hist[key], 1u);
    });
    __syncthreads();
    for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) {
@@ -2237,6 +2276,7 @@ __global__ __launch_bounds__(256) void groupHistogramBatchKernel(AggregateBatch 
                        quarter * 256;
    uint32_t passes = (n_bins + kHistLdsBins - 1) / kHistLdsBins;
    passes = passes > kHistRanges ? kHistRanges : passes;
+   const bool combine = n_bins <= kHistCombineBins;
    for (uint32_t z = 0; z < passes; ++z) {
       const uint32_t lo = z * kHistLdsBins;
       const uint32_t span = n_bins - lo < kHistLdsBins ? n_bins - lo : kHistLdsBins;
@@ -2247,16 +2287,17 @@ __global__ __launch_bounds__(256) void groupHistogramBatchKernel(AggregateBatch 
          uint32_t key = 0;
          bool in_table = true;
          for (uint32_t d = 0; d < dims.n; ++d) {
-            const uint32_t lab = dims.labels[d][r];
+            const uint32_t lab = toGlobal<GlobalU32>(dims.labels[d])[r];
             in_table &= lab != kInvalidId;
             key += lab * dims.stride[d];
          }
-         if (!in_table || key < lo) return;
-         if (key - lo < kHistLdsBins) {
+         const bool here = in_table && key >= lo && key - lo < kHistLdsBins;
+         if (combine) {
+            addCombined(lh, here ? key - lo : ~0u);
+         } else if (here) {
             atomicAdd(&lh[key - lo], 1u);
-         } else if (last) {
-            atomicAdd(&hist[key], 1u);
          }
+         if (in_table && last && key >= lo + kHistLdsBins) atomicAdd(&hist[key], 1u);
       });
       __syncthreads();
       for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) {
