@@ -233,8 +233,12 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
          --sp;
          return stack[sp * 64 + lane];
       };
+      // the next instruction is loaded while this one runs (reloaded after a jump)
+      Instr next_ins{};
+      if (pd.n_instr > 0) next_ins = instrs[pd.instr_begin];
       for (uint32_t i = 0; i < pd.n_instr; ++i) {
-         const Instr ins = instrs[pd.instr_begin + i];
+         const Instr ins = next_ins;
+         if (i + 1 < pd.n_instr) next_ins = instrs[pd.instr_begin + i + 1];
          switch (ins.op) {
             case OP_EMPTY:
                push(0);
@@ -513,15 +517,24 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                   compare(ins.c - ins.e, gt_k, eq_k);
                   decided |= ~(gt_k | eq_k);  // count < n - left
                }
-               if (__ballot(decided != ~0ULL) == 0ULL) i = ins.a - 1;
+               if (__ballot(decided != ~0ULL) == 0ULL) {
+                  i = ins.a - 1;
+                  if (ins.a < pd.n_instr) next_ins = instrs[pd.instr_begin + ins.a];
+               }
                break;
             }
             case OP_SKIP_IF_EMPTY:
                // the slice's partial conjunction is empty: the remaining conjuncts cannot change it
-               if (__ballot(stack[(sp - 1) * 64 + lane] != 0ULL) == 0ULL) i = ins.a - 1;
+               if (__ballot(stack[(sp - 1) * 64 + lane] != 0ULL) == 0ULL) {
+                  i = ins.a - 1;
+                  if (ins.a < pd.n_instr) next_ins = instrs[pd.instr_begin + ins.a];
+               }
                break;
             case OP_SKIP_IF_FULL:
-               if (__ballot(stack[(sp - 1) * 64 + lane] != ~0ULL) == 0ULL) i = ins.a - 1;
+               if (__ballot(stack[(sp - 1) * 64 + lane] != ~0ULL) == 0ULL) {
+                  i = ins.a - 1;
+                  if (ins.a < pd.n_instr) next_ins = instrs[pd.instr_begin + ins.a];
+               }
                break;
             default:
                break;
@@ -2167,11 +2180,17 @@ __global__ void symbolScatterKernel(const SeqColumnView* __restrict__ colp, uint
    }
 }
 
-constexpr uint32_t kHistLdsBins = 16384;
+// LDS bins per pass: 16-bit counters, two per word (a work-group counts one quarter chunk, at most
+// 16,384 rows, so a counter cannot overflow)
+constexpr uint32_t kHistLdsBins = 32768;
+constexpr uint32_t kHistLdsWords = kHistLdsBins / 2;
+
+__device__ inline void histAdd(uint32_t* lh, uint32_t k, uint32_t v) { atomicAdd(&lh[k >> 1], v << ((k & 1u) << 4)); }
+__device__ inline uint32_t histGet(const uint32_t* lh, uint32_t k) { return (lh[k >> 1] >> ((k & 1u) << 4)) & 0xFFFFu; }
 // Bin ranges counted in LDS by separate work-groups; bins beyond the last range go to global atomics.
 constexpr uint32_t kHistRanges = 8;
 
-// Adds one to lh[k] for every active lane whose k is not ~0u, lanes holding the same k combined
+// Adds one to LDS counter k for every active lane whose k is not ~0u, lanes holding the same k combined
 // first: one LDS atomic per distinct k of the wavefront instead of one per lane (group keys of
 // skewed columns repeat within 64 rows, and same-address LDS atomics serialise).
 __device__ inline void addCombined(uint32_t* lh, uint32_t k) {
@@ -2180,7 +2199,7 @@ __device__ inline void addCombined(uint32_t* lh, uint32_t k) {
       const uint32_t lead = static_cast<uint32_t>(__builtin_ctzll(todo));
       const uint32_t lk = __shfl(k, static_cast<int>(lead), 64);
       const uint64_t same = __ballot(k == lk);
-      if ((threadIdx.x & 63) == lead) atomicAdd(&lh[lk], static_cast<uint32_t>(__popcll(same)));
+      if ((threadIdx.x & 63) == lead) histAdd(lh, lk, static_cast<uint32_t>(__popcll(same)));
       todo &= ~same;
    }
 }
@@ -2220,7 +2239,7 @@ __global__ __launch_bounds__(256) void groupHistogramKernel(uint32_t n_chunks, c
                                                             const uint32_t* __restrict__ filter_chunk_cards,
                                                             GroupDims dims, uint32_t n_bins,
                                                             uint32_t* __restrict__ hist) {
-   __shared__ uint32_t lh[kHistLdsBins];
+   __shared__ uint32_t lh[kHistLdsWords];
    __shared__ uint64_t words[256];
    const uint32_t c = blockIdx.x >> 2;
    const uint32_t quarter = blockIdx.x & 3;
@@ -2229,7 +2248,7 @@ __global__ __launch_bounds__(256) void groupHistogramKernel(uint32_t n_chunks, c
    if (card == 0 || lo >= n_bins) return;
    const uint32_t span = n_bins - lo < kHistLdsBins ? n_bins - lo : kHistLdsBins;
    const bool last = blockIdx.y + 1 == gridDim.y;
-   for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) lh[i] = 0;
+   for (uint32_t i = threadIdx.x; i < (span + 1) / 2; i += blockDim.x) lh[i] = 0;
    __syncthreads();
    const uint64_t* f = filter + static_cast<size_t>(c) * kChunkWords + quarter * 256;
    const bool combine = n_bins <= kHistCombineBins;
@@ -2245,13 +2264,14 @@ __global__ __launch_bounds__(256) void groupHistogramKernel(uint32_t n_chunks, c
       if (combine) {
          addCombined(lh, here ? key - lo : ~0u);
       } else if (here) {
-         atomicAdd(&lh[key - lo], 1u);
+         histAdd(lh, key - lo, 1u);
       }
       if (in_table && last && key >= lo + kHistLdsBins) atomicAdd(&hist[key], 1u);
    });
    __syncthreads();
    for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) {
-      if (lh[i]) atomicAdd(&hist[lo + i], lh[i]);
+      const uint32_t v = histGet(lh, i);
+      if (v) atomicAdd(&hist[lo + i], v);
    }
 }
 
@@ -2260,7 +2280,7 @@ __global__ __launch_bounds__(256) void groupHistogramKernel(uint32_t n_chunks, c
 // its range, with global atomics.  The passes run inside the work-group so that the grid (fixed for
 // graph replay) does not grow with the bin count.
 __global__ __launch_bounds__(256) void groupHistogramBatchKernel(AggregateBatch a) {
-   __shared__ uint32_t lh[kHistLdsBins];
+   __shared__ uint32_t lh[kHistLdsWords];
    __shared__ uint64_t words[256];
    const uint32_t e = blockIdx.y;
    if (e >= *a.n_agg) return;
@@ -2281,7 +2301,7 @@ __global__ __launch_bounds__(256) void groupHistogramBatchKernel(AggregateBatch 
       const uint32_t lo = z * kHistLdsBins;
       const uint32_t span = n_bins - lo < kHistLdsBins ? n_bins - lo : kHistLdsBins;
       const bool last = z + 1 == passes;
-      for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) lh[i] = 0;
+      for (uint32_t i = threadIdx.x; i < (span + 1) / 2; i += blockDim.x) lh[i] = 0;
       __syncthreads();
       forQuarterRows(f, card, (static_cast<size_t>(c) << 16) + quarter * 16384u, words, [&](size_t r) {
          uint32_t key = 0;
@@ -2295,13 +2315,14 @@ __global__ __launch_bounds__(256) void groupHistogramBatchKernel(AggregateBatch 
          if (combine) {
             addCombined(lh, here ? key - lo : ~0u);
          } else if (here) {
-            atomicAdd(&lh[key - lo], 1u);
+            histAdd(lh, key - lo, 1u);
          }
          if (in_table && last && key >= lo + kHistLdsBins) atomicAdd(&hist[key], 1u);
       });
       __syncthreads();
       for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) {
-         if (lh[i]) atomicAdd(&hist[lo + i], lh[i]);
+         const uint32_t v = histGet(lh, i);
+         if (v) atomicAdd(&hist[lo + i], v);
       }
       __syncthreads();
    }
