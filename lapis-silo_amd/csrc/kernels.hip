@@ -77,26 +77,6 @@ __device__ inline uint32_t waveLowerBound(uint32_t lo, uint32_t hi, Before befor
    return lo + static_cast<uint32_t>(__popcll(__ballot(b)));
 }
 
-// First run whose inclusive end is >= x.
-template <typename P>
-__device__ inline uint32_t firstRunEndingAtOrAfter(P r, uint32_t n, uint32_t x) {
-   uint32_t lo = 0, hi = n;  // first run with start > x
-   while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if ((r[mid] & 0xFFFFu) <= x) {
-         lo = mid + 1;
-      } else {
-         hi = mid;
-      }
-   }
-   if (lo > 0) {
-      const uint32_t prev = r[lo - 1];
-      const uint32_t end = (prev & 0xFFFFu) + (prev >> 16);
-      if (end >= x) return lo - 1;
-   }
-   return lo;
-}
-
 __device__ inline uint64_t rangeMask(uint32_t a, uint32_t b) {
    // bits a..b inclusive of one word (0 <= a <= b <= 63)
    const uint64_t hi = (b == 63) ? ~0ULL : ((1ULL << (b + 1)) - 1ULL);
@@ -131,7 +111,12 @@ __device__ inline void accumulateContainer(const Desc d, const ContainerArenas& 
       }
    } else {
       const GlobalU32* r = toGlobal<GlobalU32>(ar.runs + d.w1);
-      const uint32_t first = firstRunEndingAtOrAfter(r, card, lo);
+      // the first run whose inclusive end reaches the slice (runs are sorted and disjoint, so their
+      // ends are sorted too): a search by the whole wavefront, 64 probes per round
+      const uint32_t first = waveLowerBound(0, card, [&](uint32_t i) {
+         const uint32_t x = r[i];
+         return (x & 0xFFFFu) + (x >> 16) < lo;
+      });
       for (uint32_t k = first + lane; k < card; k += 64) {
          const uint32_t s = r[k] & 0xFFFFu;
          const uint32_t e = s + (r[k] >> 16);  // inclusive
@@ -325,11 +310,7 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                // rows that start in the buckets of [p - kShortRun + 1, p], and the long ones
                const SeqColumnView& col = tv.seq[ins.a];
                const uint32_t p = ins.b;
-               const size_t r0 = row_base + static_cast<size_t>(lane) * 64;
-               const GlobalU32x4* cv = toGlobal<GlobalU32x4>(col.cov + r0);
-               u32x4 c4[16];
-#pragma unroll
-               for (uint32_t u = 0; u < 16; ++u) c4[u] = cv[u];
+               const GlobalU32* cv = toGlobal<GlobalU32>(col.cov) + row_base;
                // candidate lists: lane j < n_lists loads list j's bounds
                const uint32_t nb = col.n_run_buckets;
                const uint32_t k_hi = p / kRunBucket;
@@ -343,13 +324,17 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                   my_len = rp[1] - rp[0];
                }
                scratch[lane] = 0;
+               // rows read 64 at a time, consecutive lanes on consecutive rows; the test's ballot is
+               // the word of those 64 rows, kept by the lane that owns it
                uint64_t word = 0;
+               for (uint32_t u0 = 0; u0 < 64; u0 += 32) {
+                  uint32_t cs[32];
 #pragma unroll
-               for (uint32_t u = 0; u < 16; ++u) {
-                  const uint32_t cs[4] = {c4[u][0], c4[u][1], c4[u][2], c4[u][3]};
+                  for (uint32_t u = 0; u < 32; ++u) cs[u] = cv[(u0 + u) * 64 + lane];
 #pragma unroll
-                  for (uint32_t q = 0; q < 4; ++q) {
-                     if ((cs[q] & 0xFFFFu) <= p && p < (cs[q] >> 16)) word |= 1ULL << (4 * u + q);
+                  for (uint32_t u = 0; u < 32; ++u) {
+                     const uint64_t bw = __ballot((cs[u] & 0xFFFFu) <= p && p < (cs[u] >> 16));
+                     if (lane == u0 + u) word = bw;
                   }
                }
                // flat index over the lists' entries
@@ -398,19 +383,19 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                const int32_t v2 = static_cast<int32_t>(ins.f);
                const bool two = (ins.g & 1u) != 0;
                const uint64_t nulls = toGlobal<GlobalU64>(col.null_bits)[static_cast<size_t>(chunk) * kChunkWords + my_word];
-               const GlobalU32x4* vals = toGlobal<GlobalU32x4>(col.values + row_base + static_cast<size_t>(lane) * 64);
-               u32x4 x[16];
-#pragma unroll
-               for (uint32_t u = 0; u < 16; ++u) x[u] = vals[u];
+               // rows read 64 at a time, consecutive lanes on consecutive rows; the comparison's
+               // ballot is the word of those 64 rows, kept by the lane that owns it
+               const GlobalU32* vals = toGlobal<GlobalU32>(col.values) + row_base;
                uint64_t word = 0;
+               for (uint32_t u0 = 0; u0 < 64; u0 += 32) {
+                  uint32_t x[32];
 #pragma unroll
-               for (uint32_t u = 0; u < 16; ++u) {
-                  const int32_t xs[4] = {static_cast<int32_t>(x[u][0]), static_cast<int32_t>(x[u][1]),
-                                         static_cast<int32_t>(x[u][2]), static_cast<int32_t>(x[u][3])};
+                  for (uint32_t u = 0; u < 32; ++u) x[u] = vals[(u0 + u) * 64 + lane];
 #pragma unroll
-                  for (uint32_t q = 0; q < 4; ++q) {
-                     word |= static_cast<uint64_t>(compareInt(ins.b, xs[q], v) && (!two || compareInt(ins.e, xs[q], v2)))
-                             << (4 * u + q);
+                  for (uint32_t u = 0; u < 32; ++u) {
+                     const int32_t xv = static_cast<int32_t>(x[u]);
+                     const uint64_t bw = __ballot(compareInt(ins.b, xv, v) && (!two || compareInt(ins.e, xv, v2)));
+                     if (lane == u0 + u) word = bw;
                   }
                }
                // null rows: the stored value is ignored
@@ -543,9 +528,18 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
       const uint64_t result = stack[lane];
       out[static_cast<size_t>(pd.query) * n_chunks * kChunkWords + static_cast<size_t>(chunk) * kChunkWords + my_word] = result;
       const uint32_t n = waveSum(static_cast<uint32_t>(__popcll(result)));
-      if (lane == 0 && n) {
-         atomicAdd(&cards[pd.query], static_cast<unsigned long long>(n));
-         atomicAdd(&chunk_cards[static_cast<size_t>(pd.query) * n_chunks + chunk], n);
+      if (lane == 0) {
+         // A chunk's counter gathers its slices' rows in the low 20 bits and their number above; the
+         // last slice leaves the chunk's rows alone in it and adds them to the filter's
+         // cardinality, so that the cardinality takes one atomic per chunk (atomics on one
+         // address serialise).
+         uint32_t* cc = &chunk_cards[static_cast<size_t>(pd.query) * n_chunks + chunk];
+         const uint32_t old = atomicAdd(cc, (1u << 20) | n);
+         if ((old >> 20) == kSlicesPerChunk - 1) {
+            const uint32_t total = (old & 0xFFFFFu) + n;
+            *cc = total;
+            if (total) atomicAdd(&cards[pd.query], static_cast<unsigned long long>(total));
+         }
       }
    }
 }
@@ -1743,7 +1737,6 @@ __global__ __launch_bounds__(kOctetThreads) void mutationsCountOctetKernel(Mutat
       for (uint32_t k = 0; k < 8; ++k) v |= static_cast<uint64_t>((i >> k) & 1u) << (8 * k);
       lut[i] = v;
    }
-   const u32x4* mk4 = reinterpret_cast<const u32x4*>(mk);
    for (;;) {
       __syncthreads();  // the previous item is finished
       if (tid == 0) item_s = atomicAdd(&m.list_counts[2], 1u);
