@@ -42,6 +42,9 @@ constexpr uint32_t kChunkRows = 1u << 16;
 constexpr uint32_t kChunkWords = 1024;
 constexpr uint32_t kSliceWords = 64;
 constexpr uint32_t kSlicesPerChunk = kChunkWords / kSliceWords;
+// Per-(chunk, position) record of a sequence column: [lo, hi) of the position's light entries,
+// groups and descriptors in its tile, then a mask of the 4096-row slices holding any of its rows.
+constexpr uint32_t kPosRecord = 8;
 constexpr uint32_t kMaxBlockSlots = 4095;   // slot 4095 is the padding slot of light entries
 constexpr uint32_t kPadSlot = 4095;
 constexpr uint32_t kLightMaxRows = 32;
@@ -139,8 +142,8 @@ enum OpCode : uint32_t {
    OP_EMPTY = 0,
    OP_FULL,
    OP_BITMAP,    // a = bitmap-store id, b = prefetch slot
-   OP_SYMBOLS,   // a = seq column, b = block, c = slot lo, d = slot hi, e = symbol mask, f = position,
-                 // g = prefetch slot
+   OP_SYMBOLS,   // a = seq column, b = block, c = slot lo, d = slot hi (block-relative), e = mask of
+                 // the slots c + j taken, f = position, g = prefetch slot
    OP_COVERED,   // a = seq column, b = position
    OP_CMP,       // a = int column, b = comparator, c = value (bits), d = with_nulls
    OP_RANGES,    // a = first range, b = count (global [lo, hi) pairs in the program's range table)

@@ -199,12 +199,12 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                dst[1] = d[1];
             } else {
                const SeqColumnView& c = tv.seq[code >> 16];
-               const GlobalU32* r = toGlobal<GlobalU32>(c.pos_ranges) + (static_cast<size_t>(chunk) * c.genome_length + (code & 0xFFFFu)) * 6;
-               uint32_t v[6];
+               const GlobalU32* r = toGlobal<GlobalU32>(c.pos_ranges) + (static_cast<size_t>(chunk) * c.genome_length + (code & 0xFFFFu)) * kPosRecord;
+               uint32_t v[7];
 #pragma unroll
-               for (uint32_t k = 0; k < 6; ++k) v[k] = r[k];
+               for (uint32_t k = 0; k < 7; ++k) v[k] = r[k];
 #pragma unroll
-               for (uint32_t k = 0; k < 6; ++k) dst[k] = v[k];
+               for (uint32_t k = 0; k < 7; ++k) dst[k] = v[k];
             }
          }
          waveSync();
@@ -250,40 +250,52 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
             case OP_SYMBOLS: {
                // the position's entries in this chunk's tile, located by one record of six offsets
                const SeqColumnView& col = tv.seq[ins.a];
-               const uint32_t slot_base = toGlobal<GlobalU32>(col.block_slot_start)[ins.b];
-               uint32_t prv[6];
+               uint32_t prv[7];
                if (ins.g < kPrefetchSlots) {
 #pragma unroll
-                  for (uint32_t k = 0; k < 6; ++k) prv[k] = pf[ins.g * 8 + k];
+                  for (uint32_t k = 0; k < 7; ++k) prv[k] = pf[ins.g * 8 + k];
                } else {
-                  const GlobalU32* pr = toGlobal<GlobalU32>(col.pos_ranges) + (static_cast<size_t>(chunk) * col.genome_length + ins.f) * 6;
+                  const GlobalU32* pr = toGlobal<GlobalU32>(col.pos_ranges) + (static_cast<size_t>(chunk) * col.genome_length + ins.f) * kPosRecord;
 #pragma unroll
-                  for (uint32_t k = 0; k < 6; ++k) prv[k] = pr[k];
+                  for (uint32_t k = 0; k < 7; ++k) prv[k] = pr[k];
+               }
+               if (!((uniform(prv[6]) >> slice) & 1u)) {
+                  // no row of the position in this slice
+                  push(0);
+                  break;
                }
                const uint32_t l_lo = uniform(prv[0]), l_hi = uniform(prv[1]);
                const uint32_t g_lo = uniform(prv[2]), g_hi = uniform(prv[3]);
                const uint32_t h_lo = uniform(prv[4]), h_hi = uniform(prv[5]);
                const GlobalU32* light = toGlobal<GlobalU32>(col.light);
-               const GlobalU32* slot_info = toGlobal<GlobalU32>(col.slot_info);
                const GlobalU16* group_hdr = toGlobal<GlobalU16>(col.group_hdr);
                const GlobalU32x4* groups = toGlobal<GlobalU32x4>(col.groups);
                const GlobalU32x2* descs = toGlobal<GlobalU32x2>(col.vert.desc);
                uint64_t acc = 0;
                scratch[lane] = 0;
+               // the first 64 light entries, the first 64 groups and the first container are loaded
+               // together (most positions have no more in a chunk); an entry's slot is taken when its
+               // bit (relative to the position's first slot) is set in the mask
+               const bool has_l = l_hi > l_lo, has_g = g_hi > g_lo, has_h = h_hi > h_lo;
+               uint32_t x0 = 0, hdr0 = 0;
+               u32x4 rows0 = {0, 0, 0, 0};
+               u32x2 dw0 = {0, 0};
+               if (has_l) x0 = light[l_lo + lane < l_hi ? l_lo + lane : l_hi - 1];
+               if (has_g) {
+                  const uint32_t k = g_lo + lane < g_hi ? g_lo + lane : g_hi - 1;
+                  hdr0 = group_hdr[k];
+                  rows0 = groups[k];
+               }
+               if (has_h) dw0 = descs[h_lo];
                waveSync();
-               for (uint32_t k = l_lo + lane; k < l_hi; k += 64) {
-                  const uint32_t x = light[k];
+               auto light1 = [&](uint32_t x) {
                   const uint32_t row = x & 0xFFFFu;
-                  const uint32_t sym = slot_info[slot_base + (x >> 16)] >> 16;
-                  if (((ins.e >> sym) & 1u) && (row >> 12) == slice) {
+                  if (((ins.e >> ((x >> 16) - ins.c)) & 1u) && (row >> 12) == slice) {
                      atomicOr(reinterpret_cast<unsigned long long*>(&scratch[(row >> 6) & 63]), 1ULL << (row & 63));
                   }
-               }
-               for (uint32_t k = g_lo + lane; k < g_hi; k += 64) {
-                  const uint32_t hdr = group_hdr[k];
-                  const uint32_t sym = slot_info[slot_base + (hdr & 0xFFFu)] >> 16;
-                  if (!((ins.e >> sym) & 1u)) continue;
-                  const u32x4 rows = groups[k];
+               };
+               auto group1 = [&](uint32_t hdr, const u32x4 rows) {
+                  if (!((ins.e >> ((hdr & 0xFFFu) - ins.c)) & 1u)) return;
                   const uint32_t r8[8] = {rows[0] & 0xFFFFu, rows[0] >> 16, rows[1] & 0xFFFFu, rows[1] >> 16,
                                           rows[2] & 0xFFFFu, rows[2] >> 16, rows[3] & 0xFFFFu, rows[3] >> 16};
                   const uint32_t m = (hdr >> 12) + 1;
@@ -293,12 +305,15 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                         atomicOr(reinterpret_cast<unsigned long long*>(&scratch[(r8[q] >> 6) & 63]), 1ULL << (r8[q] & 63));
                      }
                   }
-               }
+               };
+               if (has_l && l_lo + lane < l_hi) light1(x0);
+               for (uint32_t k = l_lo + 64 + lane; k < l_hi; k += 64) light1(light[k]);
+               if (has_g && g_lo + lane < g_hi) group1(hdr0, rows0);
+               for (uint32_t k = g_lo + 64 + lane; k < g_hi; k += 64) group1(group_hdr[k], groups[k]);
                for (uint32_t k = h_lo; k < h_hi; ++k) {
-                  const u32x2 dw = descs[k];
+                  const u32x2 dw = k == h_lo ? dw0 : descs[k];
                   const Desc d{dw[0], dw[1]};
-                  const uint32_t sym = slot_info[slot_base + descSlot(d.w0)] >> 16;
-                  if ((ins.e >> sym) & 1u) accumulateContainer(d, col.vert, slice, lane, acc, scratch);
+                  if ((ins.e >> (descSlot(d.w0) - ins.c)) & 1u) accumulateContainer(d, col.vert, slice, lane, acc, scratch);
                }
                waveSync();
                push(acc | scratch[lane]);

@@ -596,8 +596,35 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    std::vector<uint16_t> group_hdr;
    std::vector<Desc> desc;
    std::vector<size_t> cursor(t.n_chunks, 0);
-   // per (chunk, position): [lo, hi) of the position's light entries, groups and descriptors
-   std::vector<uint32_t> pos_ranges(static_cast<size_t>(t.n_chunks) * L * 6, 0);
+   // per (chunk, position): [lo, hi) of the position's light entries, groups and descriptors, and
+   // the mask of 4096-row slices that hold any of its rows
+   std::vector<uint32_t> pos_ranges(static_cast<size_t>(t.n_chunks) * L * kPosRecord, 0);
+   auto sliceMask = [&](const ChunkContainers& ch, const Desc& d) -> uint32_t {
+      const uint32_t type = descType(d.w0), card = descCard(d.w0);
+      uint32_t m = 0;
+      if (type == CT_INLINE) {
+         for (uint32_t j = 0; j < card; ++j) m |= 1u << (((d.w1 >> (16 * j)) & 0xFFFFu) >> 12);
+      } else if (type == CT_ARRAY) {
+         for (uint32_t j = 0; j < card; ++j) m |= 1u << (ch.arenas.arr[d.w1 + j] >> 12);
+      } else if (type == CT_RUN) {
+         for (uint32_t j = 0; j < card; ++j) {
+            const uint32_t x = ch.arenas.runs[d.w1 + j];
+            const uint32_t a = (x & 0xFFFFu) >> 12, e = ((x & 0xFFFFu) + (x >> 16)) >> 12;
+            for (uint32_t sl = a; sl <= e; ++sl) m |= 1u << sl;
+         }
+      } else {
+         const uint64_t* w = ch.arenas.bits.data() + static_cast<size_t>(d.w1) * kChunkWords;
+         for (uint32_t sl = 0; sl < kSlicesPerChunk; ++sl) {
+            for (uint32_t k = 0; k < kSliceWords; ++k) {
+               if (w[sl * kSliceWords + k]) {
+                  m |= 1u << sl;
+                  break;
+               }
+            }
+         }
+      }
+      return m;
+   };
    for (uint32_t b = 0; b < n_blocks; ++b) {
       const uint32_t pend = col.block_pos_start[b + 1];
       const uint32_t sbase = col.block_slot_start[b];
@@ -611,7 +638,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
          uint32_t range_pos = kInvalidId;
          while (k < ch.key.size() && (ch.key[k] >> 8) < pend) {
             const uint32_t p = ch.key[k] >> 8, s = ch.key[k] & 0xFF;
-            uint32_t* pr = pos_ranges.data() + (static_cast<size_t>(c) * L + p) * 6;
+            uint32_t* pr = pos_ranges.data() + (static_cast<size_t>(c) * L + p) * kPosRecord;
             if (p != range_pos) {
                range_pos = p;
                pr[0] = static_cast<uint32_t>(light.size());
@@ -620,6 +647,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
             }
             const uint32_t slot = col.pos_slot_ptr[p] + std::popcount(sym_mask[p] & ((1u << s) - 1u)) - sbase;
             Desc d = ch.arenas.desc[k];
+            pr[6] |= sliceMask(ch, d);
             const uint32_t type = descType(d.w0), card = descCard(d.w0);
             uint32_t rows = card;
             if (type == CT_RUN) {

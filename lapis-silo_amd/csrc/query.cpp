@@ -164,8 +164,16 @@ void IndexScan::emit(Emitter& e) const {
    const uint32_t p = source.position;
    const uint32_t b = col.block_of_pos[p];
    const uint32_t base = col.block_slot_start[b];
+   // the position's slots (at most 32: one per symbol and boundary kind) whose symbol is in the
+   // mask, as bits relative to the position's first slot: the kernel tests an entry's slot
+   // without looking its symbol up
+   uint32_t slot_mask = 0;
+   for (uint32_t s = col.pos_slot_ptr[p]; s < col.pos_slot_ptr[p + 1]; ++s) {
+      const uint32_t sym = col.slot_info[s] >> 16;
+      if (sym < 32 && ((source.symbol_mask >> sym) & 1u)) slot_mask |= 1u << (s - col.pos_slot_ptr[p]);
+   }
    e.leaf(Instr{OP_SYMBOLS, static_cast<uint32_t>(source.column), b, col.pos_slot_ptr[p] - base,
-                col.pos_slot_ptr[p + 1] - base, source.symbol_mask, p,
+                col.pos_slot_ptr[p + 1] - base, slot_mask, p,
                 e.prefetchSlot((static_cast<uint32_t>(source.column) << 16) | p)});
 }
 
