@@ -901,13 +901,61 @@ __global__ __launch_bounds__(kMutThreads) __attribute__((amdgpu_waves_per_eu(kPa
 //   bnd[p], bnd[L + p], bnd[2L + p]  A(p), B(p), C(p) as in the dense path
 // Both arrays must be zero on entry; the sparse finalize kernel clears what it reads.
 // ------------------------------------------------------------------------------------------------
-// Rows of a filter, each counted by a whole wavefront: a wavefront takes 64 filter words (one per
-// lane) and visits their rows one at a time; lane l adds the row's mutations l, l + 64, ... and its
-// missing runs l, l + 64, ..., lane 0 its covered-range boundaries.  A row then costs about two
-// dependent loads however many mutations it has.
+// Sparse counts go through a per-work-group LDS table before global memory: rows of one filter share
+// most of their mutations (a lineage's) and their covered-range ends, so global atomics would pile
+// onto a few addresses.  Keys: a counts index (symbol * L + position), or kSparseBndKey | a bnd index.
+constexpr uint32_t kSparseTable = 4096;
+constexpr uint32_t kSparseProbes = 16;
+constexpr uint32_t kSparseEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kSparseBndKey = 1u << 30;
+
+struct SparseTable {
+   uint32_t* keys;
+   int32_t* vals;
+   uint32_t* counts;
+   int32_t* bnd;
+
+   __device__ void clear() const {
+      for (uint32_t i = threadIdx.x; i < kSparseTable; i += blockDim.x) {
+         keys[i] = kSparseEmpty;
+         vals[i] = 0;
+      }
+   }
+   // adds v under key: in the table when the key has a place within kSparseProbes probes, else
+   // straight to global memory
+   __device__ void add(uint32_t key, int32_t v) const {
+      uint32_t h = (key * 2654435761u) >> 20;  // 12 bits
+      for (uint32_t probe = 0; probe < kSparseProbes; ++probe) {
+         uint32_t k = keys[h];
+         if (k == kSparseEmpty) k = atomicCAS(&keys[h], kSparseEmpty, key) == kSparseEmpty ? key : keys[h];
+         if (k == key) {
+            atomicAdd(&vals[h], v);
+            return;
+         }
+         h = (h + 1) & (kSparseTable - 1);
+      }
+      flushOne(key, v);
+   }
+   __device__ void flushOne(uint32_t key, int32_t v) const {
+      if (key & kSparseBndKey) {
+         atomicAdd(&bnd[key & ~kSparseBndKey], v);
+      } else {
+         atomicAdd(&counts[key], static_cast<uint32_t>(v));
+      }
+   }
+   __device__ void flush() const {
+      for (uint32_t i = threadIdx.x; i < kSparseTable; i += blockDim.x) {
+         if (keys[i] != kSparseEmpty && vals[i] != 0) flushOne(keys[i], vals[i]);
+      }
+   }
+};
+
+// Rows of a filter: a wavefront takes 64 filter words (one per lane) and visits their rows four at a
+// time, sixteen lanes per row; lane l of a row's group adds its mutations l, l + 16, ... and missing
+// runs l, l + 16, ..., the group's first lane its covered-range boundaries, all through the
+// work-group's table.  Four rows then cost about two dependent loads.
 __device__ inline void sparseCountRows(const SeqColumnView& col, const uint32_t* chunk_sizes, const uint64_t* filter,
-                                       uint32_t n_words, uint32_t* counts, int32_t* bnd, uint32_t wave_id,
-                                       uint32_t n_waves) {
+                                       uint32_t n_words, const SparseTable& tb, uint32_t wave_id, uint32_t n_waves) {
    const uint32_t lane = threadIdx.x & 63;
    const uint32_t L = col.genome_length;
    const GlobalU32* cov = toGlobal<GlobalU32>(col.cov);
@@ -925,27 +973,41 @@ __device__ inline void sparseCountRows(const SeqColumnView& col, const uint32_t*
       if (wd >= n_words) x = 0;
       uint64_t pending = __ballot(x != 0);
       while (pending) {
-         const uint32_t src = static_cast<uint32_t>(__builtin_ctzll(pending));
-         const uint64_t xs = __shfl(x, static_cast<int>(src), 64);
-         const uint32_t r = ((base + src) << 6) | static_cast<uint32_t>(__builtin_ctzll(xs));
-         if (lane == src) x &= x - 1;
-         pending = __ballot(x != 0);
+         // up to four rows at a time, sixteen lanes each
+         uint32_t rs[4] = {0, 0, 0, 0};
+         uint32_t nr = 0;
+#pragma unroll
+         for (uint32_t j = 0; j < 4; ++j) {
+            if (pending) {
+               const uint32_t src = static_cast<uint32_t>(__builtin_ctzll(pending));
+               const uint64_t xs = __shfl(x, static_cast<int>(src), 64);
+               rs[j] = ((base + src) << 6) | static_cast<uint32_t>(__builtin_ctzll(xs));
+               if (lane == src) x &= x - 1;
+               pending = __ballot(x != 0);
+               nr = j + 1;
+            }
+         }
+         const uint32_t g = lane >> 4, sub = lane & 15;
+         const bool have = g < nr;
+         const uint32_t r = g == 0 ? rs[0] : g == 1 ? rs[1] : g == 2 ? rs[2] : rs[3];
          const uint32_t cv = cov[r];
          const uint32_t n0 = nrun_ptr[r], n1 = nrun_ptr[r + 1];
          const uint32_t m0 = mut_ptr[r], m1 = mut_ptr[r + 1];
-         if (lane == 0) {
-            const uint32_t st = cv & 0xFFFFu, en = cv >> 16;
-            if (st > 0) atomicAdd(&bnd[st], 1);
-            if (en < L) atomicAdd(&bnd[L + en], 1);
-         }
-         for (uint32_t k = n0 + lane; k < n1; k += 64) {
-            const uint32_t run = nruns[k];
-            atomicAdd(&bnd[2 * L + (run & 0xFFFFu)], 1);
-            if ((run >> 16) < L) atomicAdd(&bnd[2 * L + (run >> 16)], -1);
-         }
-         for (uint32_t k = m0 + lane; k < m1; k += 64) {
-            const uint32_t mu = muts[k];
-            atomicAdd(&counts[(mu >> 16) * L + (mu & 0xFFFFu)], 1u);
+         if (have) {
+            if (sub == 0) {
+               const uint32_t st = cv & 0xFFFFu, en = cv >> 16;
+               if (st > 0) tb.add(kSparseBndKey | st, 1);
+               if (en < L) tb.add(kSparseBndKey | (L + en), 1);
+            }
+            for (uint32_t k = n0 + sub; k < n1; k += 16) {
+               const uint32_t run = nruns[k];
+               tb.add(kSparseBndKey | (2 * L + (run & 0xFFFFu)), 1);
+               if ((run >> 16) < L) tb.add(kSparseBndKey | (2 * L + (run >> 16)), -1);
+            }
+            for (uint32_t k = m0 + sub; k < m1; k += 16) {
+               const uint32_t mu = muts[k];
+               tb.add((mu >> 16) * L + (mu & 0xFFFFu), 1);
+            }
          }
       }
    }
@@ -957,9 +1019,16 @@ __global__ __launch_bounds__(256) void sparseCountKernel(const SeqColumnView* __
                                                          const unsigned long long* __restrict__ filter_card,
                                                          long long sparse_max, uint32_t* __restrict__ counts,
                                                          int32_t* __restrict__ bnd) {
+   __shared__ uint32_t keys[kSparseTable];
+   __shared__ int32_t vals[kSparseTable];
    if (!sparseActive(filter_card, sparse_max)) return;
-   sparseCountRows(*colp, chunk_sizes, filter, n_words, counts, bnd, blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6),
+   const SparseTable tb{keys, vals, counts, bnd};
+   tb.clear();
+   __syncthreads();
+   sparseCountRows(*colp, chunk_sizes, filter, n_words, tb, blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6),
                    gridDim.x * (blockDim.x / 64));
+   __syncthreads();
+   tb.flush();
 }
 
 // Sums the chunk groups' partial slot counts into group 0, in place (one thread per slot).
@@ -2058,15 +2127,22 @@ __global__ __launch_bounds__(kOctetThreads) void mutationsCountOctetKernel(Mutat
 }
 
 __global__ __launch_bounds__(256) void sparseCountBatchKernel(MutationsBatch m) {
+   __shared__ uint32_t keys[kSparseTable];
+   __shared__ int32_t vals[kSparseTable];
    const uint32_t e = blockIdx.y;
    if (e >= m.list_counts[1]) return;
    const uint32_t q = m.lists[m.max_queries + e];
    const SeqColumnView& col = *m.col_dev;
    const size_t L = col.genome_length;
+   const SparseTable tb{keys, vals, m.sparse_counts + static_cast<size_t>(e) * col.n_symbols * L,
+                        m.sparse_bnd + static_cast<size_t>(e) * 3 * L};
+   tb.clear();
+   __syncthreads();
    sparseCountRows(col, m.chunk_sizes, m.filters + static_cast<size_t>(q) * m.n_chunks * kChunkWords,
-                   m.n_chunks * kChunkWords, m.sparse_counts + static_cast<size_t>(e) * col.n_symbols * L,
-                   m.sparse_bnd + static_cast<size_t>(e) * 3 * L, blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6),
+                   m.n_chunks * kChunkWords, tb, blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6),
                    gridDim.x * (blockDim.x / 64));
+   __syncthreads();
+   tb.flush();
 }
 
 __device__ inline SlotSource batchSource(const MutationsBatch& m, uint32_t e, SlotSource*) {
