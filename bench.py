@@ -268,7 +268,7 @@ def main():
     if ds.n_rows >= 1 << 20:
         probes = autotune.probe_filters(ds)
         tuned = autotune.autotune(eng, et, probes, log=log)
-        tuned.update(autotune.autotune_batch(eng, et, probes, log=log))
+        tuned.update(autotune.autotune_batch(eng, et, probes, log=log, cards=tuned["probe_cards"]))
     log(f"autotune {time.time() - t0:.1f}s")
     traffic_tuned = 0  # Mutations bytes by the method each query takes under the tuned switch
     for q in queries:

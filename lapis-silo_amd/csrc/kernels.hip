@@ -950,10 +950,12 @@ struct SparseTable {
    }
 };
 
-// Rows of a filter: a wavefront takes 64 filter words (one per lane) and visits their rows four at a
-// time, sixteen lanes per row; lane l of a row's group adds its mutations l, l + 16, ... and missing
-// runs l, l + 16, ..., the group's first lane its covered-range boundaries, all through the
-// work-group's table.  Four rows then cost about two dependent loads.
+// Rows of a filter: a wavefront takes 64 filter words (one per lane) and visits their rows
+// kRowsPerStep at a time, 64 / kRowsPerStep lanes per row; lane l of a row's group adds its
+// mutations l, l + 64 / kRowsPerStep, ... and missing runs likewise, the group's first lane its
+// covered-range boundaries, all through the work-group's table.  A step costs about two dependent
+// loads.
+constexpr uint32_t kRowsPerStep = 8;
 __device__ inline void sparseCountRows(const SeqColumnView& col, const uint32_t* chunk_sizes, const uint64_t* filter,
                                        uint32_t n_words, const SparseTable& tb, uint32_t wave_id, uint32_t n_waves) {
    const uint32_t lane = threadIdx.x & 63;
@@ -973,11 +975,11 @@ __device__ inline void sparseCountRows(const SeqColumnView& col, const uint32_t*
       if (wd >= n_words) x = 0;
       uint64_t pending = __ballot(x != 0);
       while (pending) {
-         // up to four rows at a time, sixteen lanes each
-         uint32_t rs[4] = {0, 0, 0, 0};
+         // up to kRowsPerStep rows at a time, 64 / kRowsPerStep lanes each
+         uint32_t rs[kRowsPerStep] = {};
          uint32_t nr = 0;
 #pragma unroll
-         for (uint32_t j = 0; j < 4; ++j) {
+         for (uint32_t j = 0; j < kRowsPerStep; ++j) {
             if (pending) {
                const uint32_t src = static_cast<uint32_t>(__builtin_ctzll(pending));
                const uint64_t xs = __shfl(x, static_cast<int>(src), 64);
@@ -987,9 +989,12 @@ __device__ inline void sparseCountRows(const SeqColumnView& col, const uint32_t*
                nr = j + 1;
             }
          }
-         const uint32_t g = lane >> 4, sub = lane & 15;
+         constexpr uint32_t kLanesPerRow = 64 / kRowsPerStep;
+         const uint32_t g = lane / kLanesPerRow, sub = lane % kLanesPerRow;
          const bool have = g < nr;
-         const uint32_t r = g == 0 ? rs[0] : g == 1 ? rs[1] : g == 2 ? rs[2] : rs[3];
+         uint32_t r = rs[0];
+#pragma unroll
+         for (uint32_t j = 1; j < kRowsPerStep; ++j) r = g == j ? rs[j] : r;
          const uint32_t cv = cov[r];
          const uint32_t n0 = nrun_ptr[r], n1 = nrun_ptr[r + 1];
          const uint32_t m0 = mut_ptr[r], m1 = mut_ptr[r + 1];
@@ -999,12 +1004,12 @@ __device__ inline void sparseCountRows(const SeqColumnView& col, const uint32_t*
                if (st > 0) tb.add(kSparseBndKey | st, 1);
                if (en < L) tb.add(kSparseBndKey | (L + en), 1);
             }
-            for (uint32_t k = n0 + sub; k < n1; k += 16) {
+            for (uint32_t k = n0 + sub; k < n1; k += kLanesPerRow) {
                const uint32_t run = nruns[k];
                tb.add(kSparseBndKey | (2 * L + (run & 0xFFFFu)), 1);
                if ((run >> 16) < L) tb.add(kSparseBndKey | (2 * L + (run >> 16)), -1);
             }
-            for (uint32_t k = m0 + sub; k < m1; k += 16) {
+            for (uint32_t k = m0 + sub; k < m1; k += kLanesPerRow) {
                const uint32_t mu = muts[k];
                tb.add((mu >> 16) * L + (mu & 0xFFFFu), 1);
             }

@@ -102,10 +102,12 @@ def autotune(engine, table, probes: list, column: str = "main", reps: int = 5, l
             "dense_us": [round(d * 1e6, 1) for d in dense], "sparse_us": [round(s * 1e6, 1) for s in sparse]}
 
 
-def autotune_batch(engine, table, probes: list, column: str = "main", size: int = 64, reps: int = 3, log=None) -> dict:
+def autotune_batch(engine, table, probes: list, column: str = "main", size: int = 64, reps: int = 3, log=None,
+                   cards: list | None = None) -> dict:
     """Chooses the batched dense kernel ("batch_shared_scan": one pass over the index per query, one
     per four queries or one per eight) by timing a batch of `size` Mutations queries over the probe
-    filters each way; the results must agree.  The knob is applied."""
+    filters each way, then (given the probes' cardinalities) the dense/sparse switch for batches
+    among those cardinalities; the results must agree throughout.  The knobs are applied."""
     say = log or (lambda *_: None)
     queries = [{"filter": probes[i % len(probes)], "action": "mutations", "column": column, "min_proportion": 0.05}
                for i in range(size)]
@@ -125,11 +127,31 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
         raise RuntimeError("shared-scan and per-query batch results differ")
     best = min(times, key=times.get)
     engine.set_tuning("batch_shared_scan", best)
-    engine.set_tuning("batch_max_queries", saved)
     say(f"batch of {size} probe Mutations queries, ms: per-query kernel {times[0] * 1e3:.2f}, "
         f"four per pass {times[1] * 1e3:.2f}, eight per pass {times[2] * 1e3:.2f}; chose batch_shared_scan={best}")
-    return {"batch_shared_scan": best, "batch_ms": {"per_query": round(times[0] * 1e3, 3), "four": round(times[1] * 1e3, 3),
-                                                   "eight": round(times[2] * 1e3, 3)}}
+    # the dense/sparse switch inside a batch: a dense query there shares its pass over the index, so
+    # the switch that suits single queries is too high; candidates are the probes' own sizes
+    if cards is not None:
+        single = engine.get_tuning("sparse_max_rows")
+        sw = {}
+        for thr in sorted({-1, single, *[int(c) for c in cards]}):
+            engine.set_tuning("sparse_max_rows", thr)
+            got = table.batch(queries, arrays=True)
+            if [r.rows() for r in got] != rows[0]:
+                raise RuntimeError("batch results depend on the dense/sparse switch")
+            t = time.perf_counter()
+            for _ in range(reps):
+                table.batch(queries, arrays=True)
+            sw[thr] = (time.perf_counter() - t) / reps
+        thr = min(sw, key=sw.get)
+        engine.set_tuning("sparse_max_rows", thr)
+        say(f"batch dense/sparse switch, ms: {[(k, round(v * 1e3, 3)) for k, v in sw.items()]}; chose {thr}")
+    engine.set_tuning("batch_max_queries", saved)
+    out = {"batch_shared_scan": best, "batch_ms": {"per_query": round(times[0] * 1e3, 3), "four": round(times[1] * 1e3, 3),
+                                                  "eight": round(times[2] * 1e3, 3)}}
+    if cards is not None:
+        out["batch_sparse_max_rows"] = engine.get_tuning("sparse_max_rows")
+    return out
 
 
 def probe_filters(ds, n: int = 8) -> list:
