@@ -181,6 +181,7 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column) {
       m.partial_c = lane.partial_c.bytes() ? lane.partial_c.as<uint32_t>() : nullptr;
       m.octet_masks = lane.octet_masks.bytes() ? lane.octet_masks.as<uint8_t>() : nullptr;
       m.persistent_groups = static_cast<uint32_t>(ctx.cu_count);
+      m.octet_threads = ctx.batch_octet_threads;
       m.sparse_counts = lane.sparse_counts.as<uint32_t>();
       m.sparse_bnd = lane.sparse_bnd.as<int32_t>();
       m.totals = lane.totals.as<int64_t>();
@@ -232,7 +233,8 @@ void launchLane(Table& t, BatchLane& lane, uint32_t depth, int column) {
       enqueueKernels(t, lane, bucket, column);
       return;
    }
-   const uint32_t knobs = ctx.mutation_unroll | (ctx.batch_shared_scan << 8) | (ctx.mutation_coop ? 1u << 12 : 0u);
+   const uint32_t knobs = ctx.mutation_unroll | (ctx.batch_shared_scan << 8) | (ctx.mutation_coop ? 1u << 12 : 0u) |
+                          (ctx.batch_octet_threads == 512 ? 1u << 13 : 0u);
    const auto key = std::make_tuple(bucket, column, static_cast<long long>(ctx.sparse_max_rows), knobs);
    auto it = lane.graphs.find(key);
    if (it == lane.graphs.end()) {

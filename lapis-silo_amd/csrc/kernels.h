@@ -105,6 +105,7 @@ struct MutationsBatch {
    uint32_t* partial_c = nullptr;
    uint8_t* octet_masks = nullptr;
    uint32_t persistent_groups = 256;           // work-groups of the persistent octet kernel (one per CU)
+   uint32_t octet_threads = 1024;              // their size: 512 or 1024
    uint32_t* sparse_counts = nullptr;          // [max_queries * n_symbols * L], zero
    int32_t* sparse_bnd = nullptr;              // [max_queries * 3 * L], zero
    int64_t* totals = nullptr;                  // [2 * max_queries * position blocks * 3]

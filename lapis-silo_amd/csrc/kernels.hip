@@ -1731,11 +1731,12 @@ __global__ __launch_bounds__(256) void foldSharedPartsKernel(MutationsBatch m, u
 // atomics on counters of four 16-bit fields (flushed to the part's slot sums every 15 chunks, so
 // they cannot overflow).  Bitset and run containers AND against the eight filters' words in global
 // memory and add to heavy_sums.  The parts' sums are folded by foldSharedPartsKernel.
-constexpr uint32_t kOctetThreads = 1024;   // the container loop gives thread t word t of a bitset
+// While a tile is counted, the next tile's row bytes and the first round of its light entries and
+// groups are already being loaded into registers; the tile pointers of the item's chunks are read
+// into LDS when the item starts.
 constexpr uint32_t kOctetMaskThreads = 256;
 constexpr uint32_t kOctetDescCap = 128;    // containers whose sums are gathered in LDS at a time
-constexpr uint32_t kOctetMaxPartChunks = 4096;  // chunks of one part (the launch checks it)
-static_assert(kOctetThreads == kChunkWords, "one bitset word per thread");
+constexpr uint32_t kOctetMaxPartChunks = 1024;  // chunks of one part (the launch checks it)
 
 __device__ inline uint64_t transpose8x8(uint64_t x) {
    // bit 8r + c <-> bit 8c + r
@@ -1798,13 +1799,18 @@ __device__ inline uint32_t rangePopcountG(const GlobalU64* f, uint32_t a, uint32
    return n;
 }
 
-template <uint32_t kRound>
-__global__ __launch_bounds__(kOctetThreads) void mutationsCountOctetKernel(MutationsBatch m) {
+template <uint32_t kRound, uint32_t kThreads>
+__global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsBatch m) {
+   static_assert(kChunkWords % kThreads == 0 && kThreads % 64 == 0, "whole bitset words per thread");
+   constexpr uint32_t kWords = kChunkWords / kThreads;     // bitset words per thread
+   constexpr uint32_t kRowVecs = kChunkRows / 16 / kThreads;  // 16-byte pieces of the row bytes per thread
    __shared__ __align__(16) uint8_t mk[kChunkRows];
    __shared__ unsigned long long cnt[2][kMaxBlockSlots + 1];
    __shared__ unsigned long long lut[256];
    __shared__ uint32_t hacc[kOctetDescCap][8];
    __shared__ uint8_t work[kOctetMaxPartChunks];
+   // tile pointers of the item's chunks: entry t is chunk c_lo + t's start, entry t + 1 its end
+   __shared__ uint32_t tl[kOctetMaxPartChunks + 1], tg[kOctetMaxPartChunks + 1], th[kOctetMaxPartChunks + 1];
    __shared__ uint32_t item_s;
    const uint32_t n_dense = m.list_counts[0];
    const uint32_t n_oct = (n_dense + 7) / 8;
@@ -1821,7 +1827,7 @@ __global__ __launch_bounds__(kOctetThreads) void mutationsCountOctetKernel(Mutat
    const GlobalU32* group_ptr = toGlobal<GlobalU32>(col.group_ptr);
    const GlobalU32* heavy_ptr = toGlobal<GlobalU32>(col.heavy_ptr);
    const GlobalU64* filters = toGlobal<GlobalU64>(m.filters);
-   for (uint32_t i = tid; i < 256; i += kOctetThreads) {
+   for (uint32_t i = tid; i < 256; i += kThreads) {
       uint64_t v = 0;
       for (uint32_t k = 0; k < 8; ++k) v |= static_cast<uint64_t>((i >> k) & 1u) << (8 * k);
       lut[i] = v;
@@ -1851,20 +1857,29 @@ __global__ __launch_bounds__(kOctetThreads) void mutationsCountOctetKernel(Mutat
          valid[k] = 8 * o + k < n_dense;
          q[k] = m.lists[valid[k] ? 8 * o + k : n_dense - 1];
       }
-      for (uint32_t t = tid; t <= kMaxBlockSlots; t += kOctetThreads) {
+      for (uint32_t t = tid; t <= kMaxBlockSlots; t += kThreads) {
          cnt[0][t] = 0;
          cnt[1][t] = 0;
       }
       // which chunks of the part have work: one of the eight filters has rows there and the tile
       // is not empty (one thread per chunk, all loads issued together)
-      for (uint32_t t = tid; c_lo + t < c_hi; t += kOctetThreads) {
+      for (uint32_t t = tid; c_lo + t < c_hi; t += kThreads) {
          const uint32_t c = c_lo + t;
          uint32_t cards[8];
 #pragma unroll
          for (uint32_t k = 0; k < 8; ++k) cards[k] = m.chunk_cards[static_cast<size_t>(q[k]) * n_chunks + c];
          const uint32_t tile = b * n_chunks + c;
-         const bool nonempty = light_ptr[tile] != light_ptr[tile + 1] || group_ptr[tile] != group_ptr[tile + 1] ||
-                               heavy_ptr[tile] != heavy_ptr[tile + 1];
+         const uint32_t la = light_ptr[tile], lb = light_ptr[tile + 1];
+         const uint32_t ga = group_ptr[tile], gb = group_ptr[tile + 1];
+         const uint32_t ha = heavy_ptr[tile], hb = heavy_ptr[tile + 1];
+         // entry t + 1 is also written by the thread of chunk c + 1, with the same value
+         tl[t] = la;
+         tl[t + 1] = lb;
+         tg[t] = ga;
+         tg[t + 1] = gb;
+         th[t] = ha;
+         th[t + 1] = hb;
+         const bool nonempty = la != lb || ga != gb || ha != hb;
          bool any = false;
 #pragma unroll
          for (uint32_t k = 0; k < 8; ++k) any |= valid[k] & (cards[k] != 0);
@@ -1875,7 +1890,7 @@ __global__ __launch_bounds__(kOctetThreads) void mutationsCountOctetKernel(Mutat
       bool first_flush = true;
       auto flush = [&]() {
          __syncthreads();
-         for (uint32_t t = tid; t < ns; t += kOctetThreads) {
+         for (uint32_t t = tid; t < ns; t += kThreads) {
             const unsigned long long v[2] = {cnt[0][t], cnt[1][t]};
             cnt[0][t] = 0;
             cnt[1][t] = 0;
@@ -1899,34 +1914,69 @@ __global__ __launch_bounds__(kOctetThreads) void mutationsCountOctetKernel(Mutat
          while (c < c_hi && !work[c - c_lo]) ++c;
          return c;
       };
-      // this thread's 64 bytes of the chunk's row bytes
-      u32x4 pf[4];
-      auto fetch = [&](uint32_t c) {
+      // the next tile: this thread's 64 of the chunk's row bytes, and its share of the first round
+      // of the light entries and of the groups (left unloaded where the tile has none)
+      u32x4 pf[kRowVecs];
+      u32x4 pl[kRound], pg[kRound];
+      uint32_t ph[kRound];
+      auto fetchRows = [&](uint32_t c) {
          const GlobalU32x4* src = toGlobal<GlobalU32x4>(m.octet_masks + ((static_cast<size_t>(o) * n_chunks + c) << 16));
 #pragma unroll
-         for (uint32_t h = 0; h < 4; ++h) pf[h] = src[4 * tid + h];
+         for (uint32_t h = 0; h < kRowVecs; ++h) pf[h] = src[kRowVecs * tid + h];
+      };
+      auto fetchLight = [&](uint32_t c) {
+         const uint32_t t = c - c_lo;
+         const uint32_t la = tl[t], n4 = (tl[t + 1] - la) >> 2;
+         if (n4 > 0) {
+            const GlobalU32x4* e4 = toGlobal<GlobalU32x4>(col.light + la);
+#pragma unroll
+            for (uint32_t u = 0; u < kRound; ++u) {
+               const uint32_t jj = tid * kRound + u;
+               pl[u] = e4[jj < n4 ? jj : n4 - 1];
+            }
+         }
+      };
+      auto fetchGroups = [&](uint32_t c) {
+         const uint32_t t = c - c_lo;
+         const uint32_t ga = tg[t], ng = tg[t + 1] - ga;
+         if (ng > 0) {
+            const GlobalU32x4* gr = toGlobal<GlobalU32x4>(col.groups) + ga;
+            const GlobalU16* gh = toGlobal<GlobalU16>(col.group_hdr) + ga;
+#pragma unroll
+            for (uint32_t u = 0; u < kRound; ++u) {
+               const uint32_t jj = tid * kRound + u < ng ? tid * kRound + u : ng - 1;
+               pg[u] = gr[jj];
+               ph[u] = gh[jj];
+            }
+         }
       };
       uint32_t c = next(c_lo);
-      if (c < c_hi) fetch(c);
+      if (c < c_hi) {
+         fetchRows(c);
+         fetchLight(c);
+         fetchGroups(c);
+      }
+      // the next tile's loads are issued as soon as the registers they go to are free: row bytes
+      // once this tile's are in LDS, light entries and groups once this tile's first rounds are used
       while (c < c_hi) {
          if (since_flush == kSharedFlushChunks) flush();
          ++since_flush;
          __syncthreads();  // the previous tile's readers of mk are done
 #pragma unroll
-         for (uint32_t h = 0; h < 4; ++h) reinterpret_cast<u32x4*>(mk)[4 * tid + h] = pf[h];
-         const uint32_t tile = b * n_chunks + c;
-         const uint32_t l0 = light_ptr[tile], l1 = light_ptr[tile + 1];
-         const uint32_t h0 = heavy_ptr[tile], h1 = heavy_ptr[tile + 1];
-         const uint32_t g0 = group_ptr[tile], g1 = group_ptr[tile + 1];
+         for (uint32_t h = 0; h < kRowVecs; ++h) reinterpret_cast<u32x4*>(mk)[kRowVecs * tid + h] = pf[h];
+         const uint32_t ct = c - c_lo;
+         const uint32_t l0 = tl[ct], l1 = tl[ct + 1];
+         const uint32_t h0 = th[ct], h1 = th[ct + 1];
+         const uint32_t g0 = tg[ct], g1 = tg[ct + 1];
          const uint32_t cn = next(c + 1);
-         if (cn < c_hi) fetch(cn);
+         if (cn < c_hi) fetchRows(cn);
          __syncthreads();
          // light entries: runs of a slot summed in eight 8-bit fields (a light slot has at most 32
          // rows in a chunk)
          {
             const GlobalU32x4* e4 = toGlobal<GlobalU32x4>(col.light + l0);
             const uint32_t n4 = (l1 - l0) >> 2;
-            constexpr uint32_t kStride = kOctetThreads * kRound;
+            constexpr uint32_t kStride = kThreads * kRound;
             const uint32_t rounds = (n4 + kStride - 1) / kStride;
             uint32_t cur = kPadSlot;
             uint64_t acc = 0;
@@ -1966,26 +2016,29 @@ __global__ __launch_bounds__(kOctetThreads) void mutationsCountOctetKernel(Mutat
                }
             };
             if (rounds > 0) {
-               load(0, bufA);
-               for (uint32_t round = 0; round < rounds; round += 2) {
-                  if (round + 1 < rounds) load(round + 1, bufB);
-                  use(round, bufA);
+               // round 0 is in pl; later rounds alternate between bufB (odd) and bufA (even)
+               if (rounds > 1) load(1, bufB);
+               use(0, pl);
+               for (uint32_t round = 1; round < rounds; round += 2) {
+                  if (round + 1 < rounds) load(round + 1, bufA);
+                  use(round, bufB);
                   if (round + 1 < rounds) {
-                     if (round + 2 < rounds) load(round + 2, bufA);
-                     use(round + 1, bufB);
+                     if (round + 2 < rounds) load(round + 2, bufB);
+                     use(round + 1, bufA);
                   }
                }
             }
             // slot 4095 is the padding slot: its counters are never read
             if (acc) addRun(cur, widen4(static_cast<uint32_t>(acc)), widen4(static_cast<uint32_t>(acc >> 32)));
          }
+         if (cn < c_hi) fetchLight(cn);
          // groups of eight rows of one slot: a group's eight rows summed in 8-bit fields, runs in
          // 16-bit fields (at most 4096 rows of a slot in a chunk)
          {
             const GlobalU32x4* gr = toGlobal<GlobalU32x4>(col.groups) + g0;
             const GlobalU16* gh = toGlobal<GlobalU16>(col.group_hdr) + g0;
             const uint32_t ng = g1 - g0;
-            constexpr uint32_t kStride = kOctetThreads * kRound;
+            constexpr uint32_t kStride = kThreads * kRound;
             const uint32_t rounds = (ng + kStride - 1) / kStride;
             uint32_t cur = kPadSlot;
             uint64_t lo = 0, hi = 0;
@@ -2029,93 +2082,92 @@ __global__ __launch_bounds__(kOctetThreads) void mutationsCountOctetKernel(Mutat
                }
             };
             if (rounds > 0) {
-               load(0, rowsA, hdrA);
-               for (uint32_t round = 0; round < rounds; round += 2) {
-                  if (round + 1 < rounds) load(round + 1, rowsB, hdrB);
-                  use(round, rowsA, hdrA);
+               // round 0 is in pg / ph; later rounds alternate between the B (odd) and A (even) buffers
+               if (rounds > 1) load(1, rowsB, hdrB);
+               use(0, pg, ph);
+               for (uint32_t round = 1; round < rounds; round += 2) {
+                  if (round + 1 < rounds) load(round + 1, rowsA, hdrA);
+                  use(round, rowsB, hdrB);
                   if (round + 1 < rounds) {
-                     if (round + 2 < rounds) load(round + 2, rowsA, hdrA);
-                     use(round + 1, rowsB, hdrB);
+                     if (round + 2 < rounds) load(round + 2, rowsB, hdrB);
+                     use(round + 1, rowsA, hdrA);
                   }
                }
             }
             addRun(cur, lo, hi);
          }
-         // containers.  Bitsets: thread t takes word t of every bitset of the tile, against the eight
-         // filters' word t (loaded once per tile); a wavefront's eight sums go through two 64-bit
-         // reductions of 16-bit fields into 32-bit LDS sums per container.  Runs, arrays and inline
-         // rows: one container per wavefront.  Every container's eight sums are then added to
-         // heavy_sums with global atomics.
+         if (cn < c_hi) fetchGroups(cn);
+         // containers (a tile holds only run and bitset containers: smaller ones are light entries
+         // or groups).  Bitsets: each thread takes kWords words of every bitset of the tile, against
+         // the eight filters' words (loaded once per tile); a wavefront's eight sums go through two
+         // 64-bit reductions of 16-bit fields (at most 64 lanes x kWords x 64 rows < 2^16) into
+         // 32-bit LDS sums per container.  Runs: one container per wavefront, a run per lane, 32-bit
+         // sums (a run container can hold all 65,536 rows of a chunk).  Every container's eight sums
+         // are then added to heavy_sums with global atomics.
          for (uint32_t d0 = h0; d0 < h1; d0 += kOctetDescCap) {
             const uint32_t d1 = h1 - d0 < kOctetDescCap ? h1 : d0 + kOctetDescCap;
-            for (uint32_t i = tid; i < (d1 - d0) * 8; i += kOctetThreads) hacc[i >> 3][i & 7] = 0;
+            for (uint32_t i = tid; i < (d1 - d0) * 8; i += kThreads) hacc[i >> 3][i & 7] = 0;
             __syncthreads();
-            uint64_t fwd[8];
+            uint64_t fwd[kWords][8];
 #pragma unroll
-            for (uint32_t k = 0; k < 8; ++k) fwd[k] = filters[(static_cast<size_t>(q[k]) * n_chunks + c) * kChunkWords + tid];
+            for (uint32_t w = 0; w < kWords; ++w) {
 #pragma unroll
-            for (uint32_t k = 0; k < 8; ++k) fwd[k] = valid[k] ? fwd[k] : 0ull;
+               for (uint32_t k = 0; k < 8; ++k) {
+                  fwd[w][k] = filters[(static_cast<size_t>(q[k]) * n_chunks + c) * kChunkWords + w * kThreads + tid];
+               }
+            }
+#pragma unroll
+            for (uint32_t w = 0; w < kWords; ++w) {
+#pragma unroll
+               for (uint32_t k = 0; k < 8; ++k) fwd[w][k] = valid[k] ? fwd[w][k] : 0ull;
+            }
             const GlobalU32x2* descs = toGlobal<GlobalU32x2>(col.vert.desc);
             for (uint32_t d = d0; d < d1; ++d) {
                const u32x2 dw = descs[d];
                const uint32_t type = descType(dw[0]);
-               const uint32_t card = descCard(dw[0]);
-               uint64_t p01 = 0, p23 = 0;  // four 16-bit fields each: queries 0-3 and 4-7
                if (type == CT_BITSET) {
-                  const uint64_t x = toGlobal<GlobalU64>(col.vert.bits)[static_cast<size_t>(dw[1]) * kChunkWords + tid];
+                  uint64_t p01 = 0, p23 = 0;  // four 16-bit fields each: queries 0-3 and 4-7
+                  const GlobalU64* bits = toGlobal<GlobalU64>(col.vert.bits) + static_cast<size_t>(dw[1]) * kChunkWords;
+                  uint64_t x[kWords];
 #pragma unroll
-                  for (uint32_t k = 0; k < 4; ++k) {
-                     p01 |= static_cast<uint64_t>(__popcll(x & fwd[k])) << (16 * k);
-                     p23 |= static_cast<uint64_t>(__popcll(x & fwd[k + 4])) << (16 * k);
-                  }
-               } else if (wave == (d - d0) % (kOctetThreads / 64)) {
-                  uint32_t n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-                  if (type == CT_RUN) {
-                     const GlobalU32* rr = toGlobal<GlobalU32>(col.vert.runs + dw[1]);
-                     for (uint32_t t = lane; t < card; t += 64) {
-                        const uint32_t st = rr[t] & 0xFFFFu;
-                        const uint32_t en = st + (rr[t] >> 16);
+                  for (uint32_t w = 0; w < kWords; ++w) x[w] = bits[w * kThreads + tid];
 #pragma unroll
-                        for (uint32_t k = 0; k < 8; ++k) {
-                           if (valid[k]) n[k] += rangePopcountG(filters + (static_cast<size_t>(q[k]) * n_chunks + c) * kChunkWords, st, en);
-                        }
-                     }
-                  } else if (type == CT_ARRAY) {
-                     const GlobalU16* a = toGlobal<GlobalU16>(col.vert.arr + dw[1]);
-                     for (uint32_t t = lane; t < card; t += 64) {
-                        const uint64_t v = lut[mk[a[t]]];
+                  for (uint32_t w = 0; w < kWords; ++w) {
 #pragma unroll
-                        for (uint32_t k = 0; k < 8; ++k) n[k] += static_cast<uint32_t>((v >> (8 * k)) & 0xFFu);
-                     }
-                  } else if (lane == 0) {
-                     for (uint32_t t = 0; t < card; ++t) {
-                        const uint64_t v = lut[mk[(dw[1] >> (16 * t)) & 0xFFFFu]];
-#pragma unroll
-                        for (uint32_t k = 0; k < 8; ++k) n[k] += static_cast<uint32_t>((v >> (8 * k)) & 0xFFu);
+                     for (uint32_t k = 0; k < 4; ++k) {
+                        p01 += static_cast<uint64_t>(__popcll(x[w] & fwd[w][k])) << (16 * k);
+                        p23 += static_cast<uint64_t>(__popcll(x[w] & fwd[w][k + 4])) << (16 * k);
                      }
                   }
-                  // a lane's share of a run or array container stays far below 2^16 per query
+                  const uint64_t s01 = waveSum64(p01);
+                  const uint64_t s23 = waveSum64(p23);
+                  if (lane == 0 && (s01 | s23)) {
 #pragma unroll
-                  for (uint32_t k = 0; k < 4; ++k) {
-                     p01 |= static_cast<uint64_t>(n[k]) << (16 * k);
-                     p23 |= static_cast<uint64_t>(n[k + 4]) << (16 * k);
+                     for (uint32_t k = 0; k < 4; ++k) {
+                        const uint32_t v0 = static_cast<uint32_t>((s01 >> (16 * k)) & 0xFFFFu);
+                        const uint32_t v1 = static_cast<uint32_t>((s23 >> (16 * k)) & 0xFFFFu);
+                        if (v0) atomicAdd(&hacc[d - d0][k], v0);
+                        if (v1) atomicAdd(&hacc[d - d0][k + 4], v1);
+                     }
                   }
-               }
-               // wavefront sums: at most 64 lanes x 64 rows per field for bitsets (< 2^16)
-               const uint64_t s01 = waveSum64(p01);
-               const uint64_t s23 = waveSum64(p23);
-               if (lane == 0 && (s01 | s23)) {
-#pragma unroll
-                  for (uint32_t k = 0; k < 4; ++k) {
-                     const uint32_t v0 = static_cast<uint32_t>((s01 >> (16 * k)) & 0xFFFFu);
-                     const uint32_t v1 = static_cast<uint32_t>((s23 >> (16 * k)) & 0xFFFFu);
-                     if (v0) atomicAdd(&hacc[d - d0][k], v0);
-                     if (v1) atomicAdd(&hacc[d - d0][k + 4], v1);
+               } else if (type == CT_RUN && wave == (d - d0) % (kThreads / 64)) {
+                  const uint32_t card = descCard(dw[0]);
+                  const GlobalU32* rr = toGlobal<GlobalU32>(col.vert.runs + dw[1]);
+                  for (uint32_t t = lane; t < card; t += 64) {
+                     const uint32_t st = rr[t] & 0xFFFFu;
+                     const uint32_t en = st + (rr[t] >> 16);
+#pragma unroll 1
+                     for (uint32_t k = 0; k < 8; ++k) {
+                        if (8 * o + k >= n_dense) break;
+                        const uint32_t qk = m.lists[8 * o + k];
+                        const uint32_t v = rangePopcountG(filters + (static_cast<size_t>(qk) * n_chunks + c) * kChunkWords, st, en);
+                        if (v) atomicAdd(&hacc[d - d0][k], v);
+                     }
                   }
                }
             }
             __syncthreads();
-            for (uint32_t i = tid; i < (d1 - d0) * 8; i += kOctetThreads) {
+            for (uint32_t i = tid; i < (d1 - d0) * 8; i += kThreads) {
                const uint32_t k = i & 7;
                const uint32_t v = hacc[i >> 3][k];
                if (valid[k] && v) {
@@ -2591,10 +2643,18 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
          if ((err = hipGetLastError()) != hipSuccess) return err;
       }
       const uint32_t groups = m.persistent_groups == 0 ? 1 : m.persistent_groups;
-      if (m.unroll == 16) {
-         hipLaunchKernelGGL(mutationsCountOctetKernel<4>, dim3(groups), dim3(kOctetThreads), 0, stream, m);
+      if (m.octet_threads == 512) {
+         if (m.unroll == 16) {
+            hipLaunchKernelGGL((mutationsCountOctetKernel<4, 512>), dim3(groups), dim3(512), 0, stream, m);
+         } else {
+            hipLaunchKernelGGL((mutationsCountOctetKernel<2, 512>), dim3(groups), dim3(512), 0, stream, m);
+         }
       } else {
-         hipLaunchKernelGGL(mutationsCountOctetKernel<2>, dim3(groups), dim3(kOctetThreads), 0, stream, m);
+         if (m.unroll == 16) {
+            hipLaunchKernelGGL((mutationsCountOctetKernel<4, 1024>), dim3(groups), dim3(1024), 0, stream, m);
+         } else {
+            hipLaunchKernelGGL((mutationsCountOctetKernel<2, 1024>), dim3(groups), dim3(1024), 0, stream, m);
+         }
       }
       if ((err = hipGetLastError()) != hipSuccess) return err;
       const uint32_t sblocks = (h.n_slots + 255) / 256;

@@ -105,9 +105,10 @@ def autotune(engine, table, probes: list, column: str = "main", reps: int = 5, l
 def autotune_batch(engine, table, probes: list, column: str = "main", size: int = 64, reps: int = 3, log=None,
                    cards: list | None = None) -> dict:
     """Chooses the batched dense kernel ("batch_shared_scan": one pass over the index per query, one
-    per four queries or one per eight) by timing a batch of `size` Mutations queries over the probe
-    filters each way, then (given the probes' cardinalities) the dense/sparse switch for batches
-    among those cardinalities; the results must agree throughout.  The knobs are applied."""
+    per four queries or one per eight, the last with work-groups of 512 or 1024 threads,
+    "batch_octet_threads") by timing a batch of `size` Mutations queries over the probe filters each
+    way, then (given the probes' cardinalities) the dense/sparse switch for batches among those
+    cardinalities; the results must agree throughout.  The knobs are applied."""
     say = log or (lambda *_: None)
     queries = [{"filter": probes[i % len(probes)], "action": "mutations", "column": column, "min_proportion": 0.05}
                for i in range(size)]
@@ -115,20 +116,23 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
     engine.set_tuning("batch_max_queries", size)
     times = {}
     results = {}
-    for shared in (0, 1, 2):
-        engine.set_tuning("batch_shared_scan", shared)
-        results[shared] = table.batch(queries, arrays=True)
+    for mode in ((0, 1024), (1, 1024), (2, 1024), (2, 512)):
+        engine.set_tuning("batch_shared_scan", mode[0])
+        engine.set_tuning("batch_octet_threads", mode[1])
+        results[mode] = [r.rows() for r in table.batch(queries, arrays=True)]
         t = time.perf_counter()
         for _ in range(reps):
             table.batch(queries, arrays=True)
-        times[shared] = (time.perf_counter() - t) / reps
-    rows = {k: [r.rows() for r in v] for k, v in results.items()}
-    if not rows[0] == rows[1] == rows[2]:
+        times[mode] = (time.perf_counter() - t) / reps
+    ref_rows = results[(0, 1024)]
+    if any(r != ref_rows for r in results.values()):
         raise RuntimeError("shared-scan and per-query batch results differ")
     best = min(times, key=times.get)
-    engine.set_tuning("batch_shared_scan", best)
-    say(f"batch of {size} probe Mutations queries, ms: per-query kernel {times[0] * 1e3:.2f}, "
-        f"four per pass {times[1] * 1e3:.2f}, eight per pass {times[2] * 1e3:.2f}; chose batch_shared_scan={best}")
+    engine.set_tuning("batch_shared_scan", best[0])
+    engine.set_tuning("batch_octet_threads", best[1])
+    say(f"batch of {size} probe Mutations queries, ms: per-query kernel {times[(0, 1024)] * 1e3:.2f}, "
+        f"four per pass {times[(1, 1024)] * 1e3:.2f}, eight per pass {times[(2, 1024)] * 1e3:.2f} "
+        f"(512 threads {times[(2, 512)] * 1e3:.2f}); chose batch_shared_scan={best[0]}, threads {best[1]}")
     # the dense/sparse switch inside a batch: a dense query there shares its pass over the index, so
     # the switch that suits single queries is too high; candidates are the probes' own sizes
     if cards is not None:
@@ -137,7 +141,7 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
         for thr in sorted({-1, single, *[int(c) for c in cards]}):
             engine.set_tuning("sparse_max_rows", thr)
             got = table.batch(queries, arrays=True)
-            if [r.rows() for r in got] != rows[0]:
+            if [r.rows() for r in got] != ref_rows:
                 raise RuntimeError("batch results depend on the dense/sparse switch")
             t = time.perf_counter()
             for _ in range(reps):
@@ -147,8 +151,9 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
         engine.set_tuning("sparse_max_rows", thr)
         say(f"batch dense/sparse switch, ms: {[(k, round(v * 1e3, 3)) for k, v in sw.items()]}; chose {thr}")
     engine.set_tuning("batch_max_queries", saved)
-    out = {"batch_shared_scan": best, "batch_ms": {"per_query": round(times[0] * 1e3, 3), "four": round(times[1] * 1e3, 3),
-                                                  "eight": round(times[2] * 1e3, 3)}}
+    out = {"batch_shared_scan": best[0], "batch_octet_threads": best[1],
+           "batch_ms": {"per_query": round(times[(0, 1024)] * 1e3, 3), "four": round(times[(1, 1024)] * 1e3, 3),
+                        "eight": round(times[(2, 1024)] * 1e3, 3), "eight_512": round(times[(2, 512)] * 1e3, 3)}}
     if cards is not None:
         out["batch_sparse_max_rows"] = engine.get_tuning("sparse_max_rows")
     return out

@@ -7,7 +7,10 @@ from helpers import ExprGen, random_bitmaps, small_spec
 
 pytestmark = pytest.mark.gpu
 
-KNOBS = ("sparse_max_rows", "batch_max_queries", "batch_lanes", "batch_graphs", "batch_row_capacity", "batch_shared_scan")
+KNOBS = ("sparse_max_rows", "batch_max_queries", "batch_lanes", "batch_graphs", "batch_row_capacity", "batch_shared_scan",
+         "batch_octet_threads")
+# (batch_shared_scan, batch_octet_threads): every dense counting kernel of a batch
+SCANS = [(0, 1024), (1, 1024), (2, 1024), (2, 512)]
 
 
 @pytest.fixture(scope="module")
@@ -64,13 +67,14 @@ def check(et, ot, queries):
         assert g == expected(ot, q), q
 
 
-@pytest.mark.parametrize("shared", [0, 1, 2])
+@pytest.mark.parametrize("shared,threads", SCANS)
 @pytest.mark.parametrize("graphs", [0, 1])
 @pytest.mark.parametrize("sparse_max", [-1, 300, 1 << 40])
-def test_batch_matches_oracle(small, tuning, graphs, sparse_max, shared):
+def test_batch_matches_oracle(small, tuning, graphs, sparse_max, shared, threads):
     ds, ts, ot, et = small
     tuning.set_tuning("batch_graphs", graphs)
     tuning.set_tuning("batch_shared_scan", shared)
+    tuning.set_tuning("batch_octet_threads", threads)
     tuning.set_tuning("sparse_max_rows", sparse_max)
     tuning.set_tuning("batch_max_queries", 16)  # several lane jobs per batch
     check(et, ot, make_queries(ds, seed=100 + graphs, n=60, bitmaps=len(ts.bitmaps)))
@@ -154,15 +158,42 @@ def test_shared_scan_many_chunks(engine, tuning):
                                  ("(nof 1 0 (has_mut main 5) (has_mut main 40) (has_mut main 77))", 0.02),
                                  ("(date_between date %d %d)" % (ds.spec.date_base, ds.spec.date_base + 20), 0.0)]]
         want = [ot.mutations(q["filter"], "main", q["min_proportion"]) for q in queries]
-        per_mode = []
-        for shared in (0, 1, 2):
+        for shared, threads in SCANS:
             tuning.set_tuning("batch_shared_scan", shared)
-            got = et.batch(queries)
-            assert got == want, shared
-            per_mode.append(got)
-        assert per_mode[0] == per_mode[1] == per_mode[2]
+            tuning.set_tuning("batch_octet_threads", threads)
+            assert et.batch(queries) == want, (shared, threads)
     finally:
         et.close()
+
+
+def test_full_chunk_run(engine, tuning):
+    """A non-reference symbol that holds every row of a full 65,536-row chunk (one run container of
+    65,536 rows; the symbol is still a minority of the column): its count under a filter of all rows
+    is 65,536 in every dense kernel, the single-query ones and each batched scan."""
+    from oracle.oracle import OracleTable
+    from silo_amd import table
+
+    ref = "ACGTACGT"
+    rows = ["ACGTTCGT"] * 65536 + ["ACGTACGT"] * 65536 + ["ACGTACGA"] * 65536
+    spec = table.TableSpec(chunk_sizes=np.asarray([65536, 65536, 65536], np.uint32))
+    spec.seq_columns.append(table.seq_column_from_strings("main", ref, rows, 0))
+    ot = OracleTable(spec)
+    et = engine.table(spec)
+    try:
+        tuning.set_tuning("sparse_max_rows", -1)
+        queries = [{"filter": f, "action": "mutations", "column": "main", "min_proportion": 0.0}
+                   for f in ("(true)", "(sym_eq main 4 T)", "(not (sym_eq main 7 A))")]
+        want = [ot.mutations(q["filter"], "main", 0.0) for q in queries]
+        assert any(r[3] == 65536 for r in want[0]), want[0]
+        for q, w in zip(queries, want):
+            assert et.mutations(q["filter"], "main", 0.0) == w, q
+        for shared, threads in SCANS:
+            tuning.set_tuning("batch_shared_scan", shared)
+            tuning.set_tuning("batch_octet_threads", threads)
+            assert et.batch(queries) == want, (shared, threads)
+    finally:
+        et.close()
+        ot.close()
         ot.close()
 
 
