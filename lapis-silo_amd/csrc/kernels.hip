@@ -16,6 +16,10 @@ namespace {
 
 constexpr int kVmWaves = 4;
 constexpr int kVmThreads = kVmWaves * 64;
+constexpr uint32_t kVmUnitSlices = 4;      // most slices of one (program, chunk) a wavefront takes in a row
+constexpr uint32_t kVmUnitMinUnits = 8192;  // units of a launch for more than one slice a unit (two per
+                                            // wavefront resident on 256 CUs)
+constexpr uint32_t kProgCache = 64;    // instructions of a program a wavefront keeps in LDS
 constexpr int kMutWaves = 4;
 constexpr int kMutThreads = kMutWaves * 64;
 constexpr int kPosBlock = static_cast<int>(kMutPosBlock);
@@ -165,7 +169,8 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                                                              uint32_t depth, uint64_t* __restrict__ out,
                                                              unsigned long long* __restrict__ cards,
                                                              uint32_t* __restrict__ chunk_cards,
-                                                             const uint32_t* __restrict__ n_queries_dev) {
+                                                             const uint32_t* __restrict__ n_queries_dev,
+                                                             uint32_t unit_slices) {
    extern __shared__ uint64_t lds[];
    const uint32_t wave = threadIdx.x >> 6;
    const uint32_t lane = threadIdx.x & 63;
@@ -173,22 +178,33 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
    uint64_t* scratch = stack + static_cast<size_t>(depth) * 64;
    uint32_t* pf = reinterpret_cast<uint32_t*>(lds + static_cast<size_t>(kVmWaves) * (depth + 1) * 64) +
                   static_cast<size_t>(wave) * kPrefetchSlots * 8;
+   u32x4* prog = reinterpret_cast<u32x4*>(lds + static_cast<size_t>(kVmWaves) * (depth + 1) * 64 +
+                                          static_cast<size_t>(kVmWaves) * kPrefetchSlots * 4) +
+                 static_cast<size_t>(wave) * kProgCache * 2;
    const uint32_t n_chunks = tv.n_chunks;
    const uint32_t per_query = n_chunks * kSlicesPerChunk;
    if (n_queries_dev) n_items = uniform(*n_queries_dev * per_query);
-   for (uint32_t item = uniform(blockIdx.x * kVmWaves + wave); item < n_items; item += gridDim.x * kVmWaves) {
-      const uint32_t qi = item / per_query;
-      const uint32_t rem = item - qi * per_query;
-      const uint32_t chunk = rem / kSlicesPerChunk;
-      const uint32_t slice = rem - chunk * kSlicesPerChunk;
-      const ProgramDesc pd = progs[qi];
-      const uint32_t chunk_size = tv.chunk_sizes[chunk];
-      const uint32_t my_word = slice * 64 + lane;
-      const uint64_t lmask = layoutMask(chunk_size, my_word);
-      const size_t row_base = (static_cast<size_t>(chunk) << 16) + slice * 4096u;
+   // A wavefront takes units of unit_slices (1, 2 or 4) consecutive slices of one (program, chunk):
+   // the program (its first kProgCache instructions) and the leaf records are loaded into LDS once
+   // per unit, so that an instruction costs an LDS read instead of a memory round trip.
+   const uint32_t n_units = n_items / unit_slices;
+   for (uint32_t unit = uniform(blockIdx.x * kVmWaves + wave); unit < n_units; unit += gridDim.x * kVmWaves) {
+   const uint32_t unit_item = unit * unit_slices;
+   const uint32_t qi = unit_item / per_query;
+   const uint32_t chunk = (unit_item - qi * per_query) / kSlicesPerChunk;
+   const ProgramDesc pd = progs[qi];
+   {
+      waveSync();  // the previous unit's reads of the records and the program are done
+      // the program's first instructions, one per lane
+      const uint32_t n_cached = pd.n_instr < kProgCache ? pd.n_instr : kProgCache;
+      if (lane < n_cached) {
+         const GlobalU32x4* src = toGlobal<GlobalU32x4>(instrs + pd.instr_begin + lane);
+         const u32x4 a = src[0], b = src[1];
+         prog[2 * lane] = a;
+         prog[2 * lane + 1] = b;
+      }
+      // the program's leaf records for this chunk, one per lane, in one round of loads
       {
-         // the program's leaf records for this chunk, one per lane, in one round of loads
-         waveSync();  // the previous item's reads of the records are done
          const uint32_t n_pf = pd.n_pf < kPrefetchSlots ? pd.n_pf : kPrefetchSlots;
          if (lane < n_pf) {
             const uint32_t code = ranges[pd.pf_begin + lane];
@@ -207,8 +223,15 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                for (uint32_t k = 0; k < 7; ++k) dst[k] = v[k];
             }
          }
-         waveSync();
       }
+      waveSync();
+   }
+   for (uint32_t slice = (unit_item - qi * per_query) % kSlicesPerChunk, slice_end = slice + unit_slices;
+        slice < slice_end; ++slice) {
+      const uint32_t chunk_size = tv.chunk_sizes[chunk];
+      const uint32_t my_word = slice * 64 + lane;
+      const uint64_t lmask = layoutMask(chunk_size, my_word);
+      const size_t row_base = (static_cast<size_t>(chunk) << 16) + slice * 4096u;
       uint32_t sp = 0;
       auto push = [&](uint64_t v) {
          stack[sp * 64 + lane] = v;
@@ -218,12 +241,17 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
          --sp;
          return stack[sp * 64 + lane];
       };
-      // the next instruction is loaded while this one runs (reloaded after a jump)
-      Instr next_ins{};
-      if (pd.n_instr > 0) next_ins = instrs[pd.instr_begin];
+      // instruction i: from the LDS copy, or (past kProgCache) from memory
+      auto fetch = [&](uint32_t i) -> Instr {
+         if (i < kProgCache) {
+            const u32x4 a = prog[2 * i], b = prog[2 * i + 1];
+            return Instr{uniform(a[0]), uniform(a[1]), uniform(a[2]), uniform(a[3]),
+                         uniform(b[0]), uniform(b[1]), uniform(b[2]), uniform(b[3])};
+         }
+         return instrs[pd.instr_begin + i];
+      };
       for (uint32_t i = 0; i < pd.n_instr; ++i) {
-         const Instr ins = next_ins;
-         if (i + 1 < pd.n_instr) next_ins = instrs[pd.instr_begin + i + 1];
+         const Instr ins = fetch(i);
          switch (ins.op) {
             case OP_EMPTY:
                push(0);
@@ -519,7 +547,6 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                }
                if (__ballot(decided != ~0ULL) == 0ULL) {
                   i = ins.a - 1;
-                  if (ins.a < pd.n_instr) next_ins = instrs[pd.instr_begin + ins.a];
                }
                break;
             }
@@ -527,13 +554,11 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                // the slice's partial conjunction is empty: the remaining conjuncts cannot change it
                if (__ballot(stack[(sp - 1) * 64 + lane] != 0ULL) == 0ULL) {
                   i = ins.a - 1;
-                  if (ins.a < pd.n_instr) next_ins = instrs[pd.instr_begin + ins.a];
                }
                break;
             case OP_SKIP_IF_FULL:
                if (__ballot(stack[(sp - 1) * 64 + lane] != ~0ULL) == 0ULL) {
                   i = ins.a - 1;
-                  if (ins.a < pd.n_instr) next_ins = instrs[pd.instr_begin + ins.a];
                }
                break;
             default:
@@ -556,6 +581,7 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
             if (total) atomicAdd(&cards[pd.query], static_cast<unsigned long long>(total));
          }
       }
+   }
    }
 }
 
@@ -1726,14 +1752,14 @@ __global__ __launch_bounds__(256) void foldSharedPartsKernel(MutationsBatch m, u
 // is in entry 8o+k's filter), 64 KiB per chunk.  The counting kernel is persistent: each work-group
 // (1024 threads, one per CU) takes (position block, chunk part, octet) items from a counter.  Per
 // chunk it stages the chunk's row bytes in LDS; a light entry or group row then costs one LDS byte
-// read and one read of a 256-entry table that spreads the byte into eight 8-bit fields, added to the
-// lane's run sum with one 64-bit add, for all eight queries at once.  Runs end in two 64-bit LDS
+// read, spread into eight 8-bit fields by two 24-bit multiplies (spread8) and added to the lane's
+// run sum with one 64-bit add, for all eight queries at once.  Runs end in two 64-bit LDS
 // atomics on counters of four 16-bit fields (flushed to the part's slot sums every 15 chunks, so
 // they cannot overflow).  Bitset and run containers AND against the eight filters' words in global
 // memory and add to heavy_sums.  The parts' sums are folded by foldSharedPartsKernel.
 // While a tile is counted, the next tile's row bytes and the first round of its light entries and
-// groups are already being loaded into registers; the tile pointers of the item's chunks are read
-// into LDS when the item starts.
+// groups are loaded into registers; the tile pointers of the item's chunks are read into LDS when
+// the item starts.
 constexpr uint32_t kOctetMaskThreads = 256;
 constexpr uint32_t kOctetDescCap = 128;    // containers whose sums are gathered in LDS at a time
 constexpr uint32_t kOctetMaxPartChunks = 1024;  // chunks of one part (the launch checks it)
@@ -1747,6 +1773,14 @@ __device__ inline uint64_t transpose8x8(uint64_t x) {
    t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
    x = x ^ t ^ (t << 28);
    return x;
+}
+
+// byte k of the result = bit k of b (b < 256): each nibble times 1 + 2^7 + 2^14 + 2^21 puts its bit k
+// at bit 8k with no overlapping partial products (so no carries), then the mask keeps those bits
+__device__ inline uint64_t spread8(uint32_t b) {
+   const uint32_t lo = ((b & 0xFu) * 0x00204081u) & 0x01010101u;
+   const uint32_t hi = (((b >> 4) & 0xFu) * 0x00204081u) & 0x01010101u;
+   return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
 // four 8-bit fields -> four 16-bit fields
@@ -1806,7 +1840,6 @@ __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsB
    constexpr uint32_t kRowVecs = kChunkRows / 16 / kThreads;  // 16-byte pieces of the row bytes per thread
    __shared__ __align__(16) uint8_t mk[kChunkRows];
    __shared__ unsigned long long cnt[2][kMaxBlockSlots + 1];
-   __shared__ unsigned long long lut[256];
    __shared__ uint32_t hacc[kOctetDescCap][8];
    __shared__ uint8_t work[kOctetMaxPartChunks];
    // tile pointers of the item's chunks: entry t is chunk c_lo + t's start, entry t + 1 its end
@@ -1827,11 +1860,6 @@ __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsB
    const GlobalU32* group_ptr = toGlobal<GlobalU32>(col.group_ptr);
    const GlobalU32* heavy_ptr = toGlobal<GlobalU32>(col.heavy_ptr);
    const GlobalU64* filters = toGlobal<GlobalU64>(m.filters);
-   for (uint32_t i = tid; i < 256; i += kThreads) {
-      uint64_t v = 0;
-      for (uint32_t k = 0; k < 8; ++k) v |= static_cast<uint64_t>((i >> k) & 1u) << (8 * k);
-      lut[i] = v;
-   }
    for (;;) {
       __syncthreads();  // the previous item is finished
       if (tid == 0) item_s = atomicAdd(&m.list_counts[2], 1u);
@@ -1914,8 +1942,8 @@ __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsB
          while (c < c_hi && !work[c - c_lo]) ++c;
          return c;
       };
-      // the next tile: this thread's 64 of the chunk's row bytes, and its share of the first round
-      // of the light entries and of the groups (left unloaded where the tile has none)
+      // the next tile: this thread's share of the chunk's row bytes and of the first round of the
+      // light entries and of the groups (left unloaded where the tile has none)
       u32x4 pf[kRowVecs];
       u32x4 pl[kRound], pg[kRound];
       uint32_t ph[kRound];
@@ -1999,7 +2027,7 @@ __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsB
                }
                uint64_t sp[4 * kRound];
 #pragma unroll
-               for (uint32_t i = 0; i < 4 * kRound; ++i) sp[i] = lut[mb[i]];
+               for (uint32_t i = 0; i < 4 * kRound; ++i) sp[i] = spread8(mb[i]);
 #pragma unroll
                for (uint32_t u = 0; u < kRound; ++u) {
                   const bool ok = base + u < n4;
@@ -2069,7 +2097,7 @@ __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsB
                   uint64_t g8 = 0;
 #pragma unroll
                   for (uint32_t t = 0; t < 8; ++t) {
-                     if (t < mm) g8 += lut[mb[t]];
+                     if (t < mm) g8 += spread8(mb[t]);
                   }
                   if (slot != cur) {
                      addRun(cur, lo, hi);
@@ -2101,7 +2129,7 @@ __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsB
          // or groups).  Bitsets: each thread takes kWords words of every bitset of the tile, against
          // the eight filters' words (loaded once per tile); a wavefront's eight sums go through two
          // 64-bit reductions of 16-bit fields (at most 64 lanes x kWords x 64 rows < 2^16) into
-         // 32-bit LDS sums per container.  Runs: one container per wavefront, a run per lane, 32-bit
+         // 32-bit LDS sums per container.  Runs: the container's runs spread over all threads, 32-bit
          // sums (a run container can hold all 65,536 rows of a chunk).  Every container's eight sums
          // are then added to heavy_sums with global atomics.
          for (uint32_t d0 = h0; d0 < h1; d0 += kOctetDescCap) {
@@ -2150,19 +2178,28 @@ __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsB
                         if (v1) atomicAdd(&hacc[d - d0][k + 4], v1);
                      }
                   }
-               } else if (type == CT_RUN && wave == (d - d0) % (kThreads / 64)) {
+               } else if (type == CT_RUN) {
+                  // a run container can hold a thousand runs: they are spread over all threads, with
+                  // 32-bit sums per query reduced per wavefront
                   const uint32_t card = descCard(dw[0]);
                   const GlobalU32* rr = toGlobal<GlobalU32>(col.vert.runs + dw[1]);
-                  for (uint32_t t = lane; t < card; t += 64) {
-                     const uint32_t st = rr[t] & 0xFFFFu;
-                     const uint32_t en = st + (rr[t] >> 16);
-#pragma unroll 1
-                     for (uint32_t k = 0; k < 8; ++k) {
-                        if (8 * o + k >= n_dense) break;
-                        const uint32_t qk = m.lists[8 * o + k];
-                        const uint32_t v = rangePopcountG(filters + (static_cast<size_t>(qk) * n_chunks + c) * kChunkWords, st, en);
-                        if (v) atomicAdd(&hacc[d - d0][k], v);
+                  uint32_t n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                  for (uint32_t t = tid; t < card; t += kThreads) {
+                     const uint32_t x = rr[t];
+                     const uint32_t st = x & 0xFFFFu;
+                     const uint32_t en = st + (x >> 16);  // inclusive, inside the chunk
+                     for (uint32_t w = st >> 6; w <= (en >> 6); ++w) {
+                        const uint64_t rm = rangeMask(w == (st >> 6) ? (st & 63) : 0, w == (en >> 6) ? (en & 63) : 63);
+#pragma unroll
+                        for (uint32_t k = 0; k < 8; ++k) {
+                           n[k] += static_cast<uint32_t>(__popcll(filters[(static_cast<size_t>(q[k]) * n_chunks + c) * kChunkWords + w] & rm));
+                        }
                      }
+                  }
+#pragma unroll
+                  for (uint32_t k = 0; k < 8; ++k) {
+                     const uint32_t v = waveSum(n[k]);
+                     if (lane == 0 && valid[k] && v) atomicAdd(&hacc[d - d0][k], v);
                   }
                }
             }
@@ -2541,8 +2578,10 @@ hipError_t configureFilterKernel(size_t max_lds_bytes) {
 }
 
 size_t filterLdsBytes(uint32_t depth) {
-   // per wave: depth stack entries and a scratch entry of 64 words, and the prefetched leaf records
-   return static_cast<size_t>(kVmWaves) * ((depth + 1) * 64 * sizeof(uint64_t) + kPrefetchSlots * 8 * sizeof(uint32_t));
+   // per wave: depth stack entries and a scratch entry of 64 words, the prefetched leaf records and
+   // the cached instructions
+   return static_cast<size_t>(kVmWaves) *
+          ((depth + 1) * 64 * sizeof(uint64_t) + kPrefetchSlots * 8 * sizeof(uint32_t) + kProgCache * sizeof(Instr));
 }
 
 hipError_t launchFilter(const TableView& tv, const ProgramDesc* progs, const Instr* instrs, const uint32_t* ranges,
@@ -2551,10 +2590,16 @@ hipError_t launchFilter(const TableView& tv, const ProgramDesc* progs, const Ins
                         const uint32_t* n_queries_dev) {
    const uint32_t n_items = n_queries * tv.n_chunks * kSlicesPerChunk;
    if (n_items == 0) return hipSuccess;
-   uint32_t blocks = (n_items + kVmWaves - 1) / kVmWaves;
+   // more slices a unit only where the units still outnumber the wavefronts well (a single query
+   // needs every slice in parallel); unit_slices divides kSlicesPerChunk
+   const uint32_t unit = n_items >= kVmUnitSlices * kVmUnitMinUnits ? kVmUnitSlices
+                         : n_items >= 2 * kVmUnitMinUnits           ? 2u
+                                                                    : 1u;
+   static_assert(kSlicesPerChunk % kVmUnitSlices == 0, "units stay inside a chunk");
+   uint32_t blocks = (n_items / unit + kVmWaves - 1) / kVmWaves;
    if (max_blocks && blocks > max_blocks) blocks = max_blocks;
    hipLaunchKernelGGL(filterVmKernel, dim3(blocks), dim3(kVmThreads), filterLdsBytes(depth), stream, tv, progs,
-                      instrs, ranges, n_items, depth, out, cards, chunk_cards, n_queries_dev);
+                      instrs, ranges, n_items, depth, out, cards, chunk_cards, n_queries_dev, unit);
    return hipGetLastError();
 }
 
