@@ -2388,27 +2388,78 @@ constexpr uint32_t kHistCombineBins = 4096;
 // Rows of one quarter chunk (16,384 rows) of a filter, each visited once: for a sparse quarter by
 // iterating the set bits of its words, for a dense one by walking the rows in order (consecutive
 // threads on consecutive rows, so the label loads coalesce).
-template <typename F>
-__device__ inline void forQuarterRows(const uint64_t* __restrict__ f, uint32_t chunk_card, size_t row_base,
-                                      uint64_t* words_lds, F visit) {
-   if (chunk_card > 8192) {
-      for (uint32_t w = threadIdx.x; w < 256; w += blockDim.x) words_lds[w] = f[w];
-      __syncthreads();
-      for (uint32_t i = threadIdx.x; i < 16384; i += blockDim.x) {
-         if ((words_lds[i >> 6] >> (i & 63)) & 1ULL) visit(row_base + i);
+// Bin keys of U rows: the labels of every dimension are loaded together (a load behind a
+// per-row condition would wait for the loads before it); ~0u for a row with a null label.
+constexpr uint32_t kHistBatch = 8;
+template <uint32_t U>
+__device__ inline void binKeys(const GroupDims& dims, const size_t (&r)[U], uint32_t (&key)[U]) {
+   const uint32_t n = dims.n;
+   uint32_t lab[4][U];
+#pragma unroll
+   for (uint32_t d = 0; d < 4; ++d) {
+      if (d < n) {
+         const GlobalU32* l = toGlobal<GlobalU32>(dims.labels[d]);
+#pragma unroll
+         for (uint32_t u = 0; u < U; ++u) lab[d][u] = l[r[u]];
       }
-   } else {
-      for (uint32_t w = threadIdx.x; w < 256; w += blockDim.x) {
-         uint64_t x = f[w];
-         while (x) {
-            const uint32_t bit = __builtin_ctzll(x);
-            x &= x - 1;
-            visit(row_base + w * 64 + bit);
+   }
+#pragma unroll
+   for (uint32_t u = 0; u < U; ++u) {
+      uint32_t k = 0;
+      bool ok = true;
+#pragma unroll
+      for (uint32_t d = 0; d < 4; ++d) {
+         if (d < n) {
+            ok &= lab[d][u] != kInvalidId;
+            k += lab[d][u] * dims.stride[d];
          }
       }
+      key[u] = ok ? k : ~0u;
    }
 }
 
+// Calls count(key) for the rows of a quarter chunk (16,384 rows from row_base) in the filter words f,
+// kHistBatch rows per thread at a time, every lane of a wavefront in every call (key ~0u: no row).
+// Dense quarters (chunk_card > 8192) read consecutive rows on consecutive lanes; sparse ones take
+// the set bits of one filter word per thread.  blockDim.x is 256.
+template <typename Count>
+__device__ inline void histQuarter(const uint64_t* __restrict__ f, uint32_t chunk_card, size_t row_base,
+                                   uint64_t* words_lds, const GroupDims& dims, Count count) {
+   const uint32_t tid = threadIdx.x;
+   if (chunk_card > 8192) {
+      words_lds[tid] = f[tid];
+      __syncthreads();
+      for (uint32_t i0 = 0; i0 < 16384; i0 += 256 * kHistBatch) {
+         size_t r[kHistBatch];
+         uint32_t key[kHistBatch];
+#pragma unroll
+         for (uint32_t u = 0; u < kHistBatch; ++u) r[u] = row_base + i0 + u * 256 + tid;
+         binKeys(dims, r, key);
+#pragma unroll
+         for (uint32_t u = 0; u < kHistBatch; ++u) {
+            const uint32_t i = i0 + u * 256 + tid;
+            count(((words_lds[i >> 6] >> (i & 63)) & 1ULL) ? key[u] : ~0u);
+         }
+      }
+   } else {
+      uint64_t x = f[tid];
+      while (__ballot(x != 0)) {
+         size_t r[kHistBatch];
+         bool have[kHistBatch];
+         uint32_t key[kHistBatch];
+#pragma unroll
+         for (uint32_t u = 0; u < kHistBatch; ++u) {
+            have[u] = x != 0;
+            const uint32_t bit = have[u] ? static_cast<uint32_t>(__builtin_ctzll(x)) : 0u;
+            x &= x - 1;
+            r[u] = row_base + tid * 64 + bit;
+         }
+         binKeys(dims, r, key);
+#pragma unroll
+         for (uint32_t u = 0; u < kHistBatch; ++u) count(have[u] ? key[u] : ~0u);
+      }
+   }
+}
 
 // Work-group (chunk quarter, bin range y): labels of up to four dimensions combined with strides
 // into one bin; the rows whose bin falls in [y * kHistLdsBins, (y + 1) * kHistLdsBins) are counted
@@ -2430,14 +2481,8 @@ __global__ __launch_bounds__(256) void groupHistogramKernel(uint32_t n_chunks, c
    __syncthreads();
    const uint64_t* f = filter + static_cast<size_t>(c) * kChunkWords + quarter * 256;
    const bool combine = n_bins <= kHistCombineBins;
-   forQuarterRows(f, card, (static_cast<size_t>(c) << 16) + quarter * 16384u, words, [&](size_t r) {
-      uint32_t key = 0;
-      bool in_table = true;
-      for (uint32_t d = 0; d < dims.n; ++d) {
-         const uint32_t lab = toGlobal<GlobalU32>(dims.labels[d])[r];
-         in_table &= lab != kInvalidId;
-         key += lab * dims.stride[d];
-      }
+   histQuarter(f, card, (static_cast<size_t>(c) << 16) + quarter * 16384u, words, dims, [&](uint32_t key) {
+      const bool in_table = key != ~0u;
       const bool here = in_table && key >= lo && key - lo < kHistLdsBins;
       if (combine) {
          addCombined(lh, here ? key - lo : ~0u);
@@ -2481,14 +2526,8 @@ __global__ __launch_bounds__(256) void groupHistogramBatchKernel(AggregateBatch 
       const bool last = z + 1 == passes;
       for (uint32_t i = threadIdx.x; i < (span + 1) / 2; i += blockDim.x) lh[i] = 0;
       __syncthreads();
-      forQuarterRows(f, card, (static_cast<size_t>(c) << 16) + quarter * 16384u, words, [&](size_t r) {
-         uint32_t key = 0;
-         bool in_table = true;
-         for (uint32_t d = 0; d < dims.n; ++d) {
-            const uint32_t lab = toGlobal<GlobalU32>(dims.labels[d])[r];
-            in_table &= lab != kInvalidId;
-            key += lab * dims.stride[d];
-         }
+      histQuarter(f, card, (static_cast<size_t>(c) << 16) + quarter * 16384u, words, dims, [&](uint32_t key) {
+         const bool in_table = key != ~0u;
          const bool here = in_table && key >= lo && key - lo < kHistLdsBins;
          if (combine) {
             addCombined(lh, here ? key - lo : ~0u);
