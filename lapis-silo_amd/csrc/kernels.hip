@@ -181,6 +181,10 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
    u32x4* prog = reinterpret_cast<u32x4*>(lds + static_cast<size_t>(kVmWaves) * (depth + 1) * 64 +
                                           static_cast<size_t>(kVmWaves) * kPrefetchSlots * 4) +
                  static_cast<size_t>(wave) * kProgCache * 2;
+   // the unit's result words, stored together after its last slice (a store before a later load
+   // would make that load's wait include the store)
+   uint64_t* res = lds + static_cast<size_t>(kVmWaves) * ((depth + 1) * 64 + kPrefetchSlots * 4 + kProgCache * 4) +
+                   static_cast<size_t>(wave) * kVmUnitSlices * 64;
    const uint32_t n_chunks = tv.n_chunks;
    const uint32_t per_query = n_chunks * kSlicesPerChunk;
    if (n_queries_dev) n_items = uniform(*n_queries_dev * per_query);
@@ -226,8 +230,9 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
       }
       waveSync();
    }
-   for (uint32_t slice = (unit_item - qi * per_query) % kSlicesPerChunk, slice_end = slice + unit_slices;
-        slice < slice_end; ++slice) {
+   uint32_t unit_rows = 0;  // this lane's rows of the unit's slices
+   const uint32_t slice0 = (unit_item - qi * per_query) % kSlicesPerChunk;
+   for (uint32_t slice = slice0; slice < slice0 + unit_slices; ++slice) {
       const uint32_t chunk_size = tv.chunk_sizes[chunk];
       const uint32_t my_word = slice * 64 + lane;
       const uint64_t lmask = layoutMask(chunk_size, my_word);
@@ -566,20 +571,24 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
          }
       }
       const uint64_t result = stack[lane];
-      out[static_cast<size_t>(pd.query) * n_chunks * kChunkWords + static_cast<size_t>(chunk) * kChunkWords + my_word] = result;
-      const uint32_t n = waveSum(static_cast<uint32_t>(__popcll(result)));
-      if (lane == 0) {
-         // A chunk's counter gathers its slices' rows in the low 20 bits and their number above; the
-         // last slice leaves the chunk's rows alone in it and adds them to the filter's
-         // cardinality, so that the cardinality takes one atomic per chunk (atomics on one
-         // address serialise).
-         uint32_t* cc = &chunk_cards[static_cast<size_t>(pd.query) * n_chunks + chunk];
-         const uint32_t old = atomicAdd(cc, (1u << 20) | n);
-         if ((old >> 20) == kSlicesPerChunk - 1) {
-            const uint32_t total = (old & 0xFFFFFu) + n;
-            *cc = total;
-            if (total) atomicAdd(&cards[pd.query], static_cast<unsigned long long>(total));
-         }
+      res[(slice - slice0) * 64 + lane] = result;
+      unit_rows += static_cast<uint32_t>(__popcll(result));
+   }
+   uint64_t* dst = out + static_cast<size_t>(pd.query) * n_chunks * kChunkWords + static_cast<size_t>(chunk) * kChunkWords +
+                   slice0 * 64 + lane;
+   for (uint32_t k = 0; k < unit_slices; ++k) dst[k * 64] = res[k * 64 + lane];
+   const uint32_t n = waveSum(unit_rows);
+   if (lane == 0) {
+      // A chunk's counter gathers its units' rows in the low 20 bits and their slices above; the
+      // unit that completes the chunk leaves the chunk's rows alone in it and adds them to the
+      // filter's cardinality, so that the cardinality takes one atomic per chunk (atomics on one
+      // address serialise), and a unit waits for one atomic, not one per slice.
+      uint32_t* cc = &chunk_cards[static_cast<size_t>(pd.query) * n_chunks + chunk];
+      const uint32_t old = atomicAdd(cc, (unit_slices << 20) | n);
+      if ((old >> 20) + unit_slices == kSlicesPerChunk) {
+         const uint32_t total = (old & 0xFFFFFu) + n;
+         *cc = total;
+         if (total) atomicAdd(&cards[pd.query], static_cast<unsigned long long>(total));
       }
    }
    }
@@ -2617,10 +2626,10 @@ hipError_t configureFilterKernel(size_t max_lds_bytes) {
 }
 
 size_t filterLdsBytes(uint32_t depth) {
-   // per wave: depth stack entries and a scratch entry of 64 words, the prefetched leaf records and
-   // the cached instructions
-   return static_cast<size_t>(kVmWaves) *
-          ((depth + 1) * 64 * sizeof(uint64_t) + kPrefetchSlots * 8 * sizeof(uint32_t) + kProgCache * sizeof(Instr));
+   // per wave: depth stack entries and a scratch entry of 64 words, the prefetched leaf records, the
+   // cached instructions and a unit's result words
+   return static_cast<size_t>(kVmWaves) * ((depth + 1) * 64 * sizeof(uint64_t) + kPrefetchSlots * 8 * sizeof(uint32_t) +
+                                           kProgCache * sizeof(Instr) + kVmUnitSlices * 64 * sizeof(uint64_t));
 }
 
 hipError_t launchFilter(const TableView& tv, const ProgramDesc* progs, const Instr* instrs, const uint32_t* ranges,
