@@ -66,16 +66,20 @@ def evaluate(et, mlp, filters: list, chunk: int = 2000) -> dict:
 
 
 def speedup(engine, et, queries: list, mlp, steps: int = 3) -> dict:
-    """Batched workload under the reference plan and under the learned plan; results compared."""
+    """Batched workload under the reference plan and under the learned plan (results as columns, so
+    that the device work dominates the clock); results compared."""
+    from .engine import MutationArrays
+
     res = {}
     outs = {}
     for mode in ("reference", "independence", "learned"):
         engine.set_planner(mode, mlp if mode == "learned" else None)
-        et.batch(queries)  # warm-up (graph capture, lanes)
+        et.batch(queries, arrays=True)  # warm-up (graph capture, lanes)
         t = time.perf_counter()
         for _ in range(steps):
-            outs[mode] = et.batch(queries)
+            got = et.batch(queries, arrays=True)
         res[mode] = len(queries) * steps / (time.perf_counter() - t)
+        outs[mode] = [r.rows() if isinstance(r, MutationArrays) else r for r in got]
     engine.set_planner("reference")
     return {"queries_per_s": {k: round(v, 2) for k, v in res.items()},
             "learned_vs_reference": round(res["learned"] / res["reference"], 4),
