@@ -1898,6 +1898,7 @@ __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsB
          cnt[0][t] = 0;
          cnt[1][t] = 0;
       }
+      for (uint32_t i = tid; i < kOctetDescCap * 8; i += kThreads) hacc[i >> 3][i & 7] = 0;
       // which chunks of the part have work: one of the eight filters has rows there and the tile
       // is not empty (one thread per chunk, all loads issued together)
       for (uint32_t t = tid; c_lo + t < c_hi; t += kThreads) {
@@ -2141,10 +2142,15 @@ __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsB
          // 32-bit LDS sums per container.  Runs: the container's runs spread over all threads, 32-bit
          // sums (a run container can hold all 65,536 rows of a chunk).  Every container's eight sums
          // are then added to heavy_sums with global atomics.
+         // hacc is zero here: cleared when the item starts and by each flush below, and the barriers
+         // at a tile's start order the flush's clearing before the next tile's sums
+         static_assert(kOctetDescCap == 128, "two descriptors per lane");
          for (uint32_t d0 = h0; d0 < h1; d0 += kOctetDescCap) {
             const uint32_t d1 = h1 - d0 < kOctetDescCap ? h1 : d0 + kOctetDescCap;
-            for (uint32_t i = tid; i < (d1 - d0) * 8; i += kThreads) hacc[i >> 3][i & 7] = 0;
-            __syncthreads();
+            const GlobalU32x2* descs = toGlobal<GlobalU32x2>(col.vert.desc);
+            // the group's descriptors, two per lane, loaded together with the filter words
+            const u32x2 da = descs[d0 + lane < d1 ? d0 + lane : d1 - 1];
+            const u32x2 db = descs[d0 + 64 + lane < d1 ? d0 + 64 + lane : d1 - 1];
             uint64_t fwd[kWords][8];
 #pragma unroll
             for (uint32_t w = 0; w < kWords; ++w) {
@@ -2158,9 +2164,11 @@ __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsB
 #pragma unroll
                for (uint32_t k = 0; k < 8; ++k) fwd[w][k] = valid[k] ? fwd[w][k] : 0ull;
             }
-            const GlobalU32x2* descs = toGlobal<GlobalU32x2>(col.vert.desc);
             for (uint32_t d = d0; d < d1; ++d) {
-               const u32x2 dw = descs[d];
+               const uint32_t j = d - d0;
+               const u32x2 dsel = j < 64 ? da : db;
+               const u32x2 dw = {uniform(__shfl(dsel[0], static_cast<int>(j & 63), 64)),
+                                 uniform(__shfl(dsel[1], static_cast<int>(j & 63), 64))};
                const uint32_t type = descType(dw[0]);
                if (type == CT_BITSET) {
                   uint64_t p01 = 0, p23 = 0;  // four 16-bit fields each: queries 0-3 and 4-7
@@ -2216,12 +2224,13 @@ __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsB
             for (uint32_t i = tid; i < (d1 - d0) * 8; i += kThreads) {
                const uint32_t k = i & 7;
                const uint32_t v = hacc[i >> 3][k];
+               hacc[i >> 3][k] = 0;
                if (valid[k] && v) {
                   const uint32_t slot = descSlot(descs[d0 + (i >> 3)][0]);
                   atomicAdd(&m.heavy_sums[static_cast<size_t>(8 * o + k) * n_slots + s0 + slot], v);
                }
             }
-            __syncthreads();
+            if (d1 < h1) __syncthreads();  // the next group's sums go to the cleared hacc
          }
          c = cn;
       }
