@@ -127,7 +127,14 @@ struct MutationRowDev {
 struct IntColumnView {
    const int32_t* values;     // [n_chunks * 2^16]
    const uint64_t* null_bits; // [n_chunks * 1024]
+   // Bit-sliced copy when the non-null values span fewer than 2^16: plane p, word w of chunk c at
+   // planes[(p * n_chunks + c) * 1024 + w] holds bit p of (value - plane_min) of the word's rows
+   // (0 for null rows).  n_planes == 0: no copy (comparisons read `values`).
+   const uint64_t* planes;
+   int32_t plane_min;
+   uint32_t n_planes;
 };
+constexpr uint32_t kMaxIntPlanes = 16;
 
 struct TableView {
    uint32_t n_chunks;

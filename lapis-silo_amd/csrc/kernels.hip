@@ -197,6 +197,7 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
    const uint32_t qi = unit_item / per_query;
    const uint32_t chunk = (unit_item - qi * per_query) / kSlicesPerChunk;
    const ProgramDesc pd = progs[qi];
+   uint32_t prog_base = 0;  // the LDS copy holds instructions prog_base .. prog_base + kProgCache - 1
    {
       waveSync();  // the previous unit's reads of the records and the program are done
       // the program's first instructions, one per lane
@@ -247,13 +248,25 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
          return stack[sp * 64 + lane];
       };
       // instruction i: from the LDS copy, or (past kProgCache) from memory
+      // Instruction i from the LDS copy, a window of kProgCache instructions: a longer program
+      // moves the window forward (jumps only go forward) and back to its start for the next slice,
+      // one round of loads per window instead of one load per instruction.
       auto fetch = [&](uint32_t i) -> Instr {
-         if (i < kProgCache) {
-            const u32x4 a = prog[2 * i], b = prog[2 * i + 1];
-            return Instr{uniform(a[0]), uniform(a[1]), uniform(a[2]), uniform(a[3]),
-                         uniform(b[0]), uniform(b[1]), uniform(b[2]), uniform(b[3])};
+         if (i - prog_base >= kProgCache) {
+            waveSync();  // this window's readers are done
+            prog_base = i;
+            if (i + lane < pd.n_instr) {
+               const GlobalU32x4* src = toGlobal<GlobalU32x4>(instrs + pd.instr_begin + i + lane);
+               const u32x4 a = src[0], b = src[1];
+               prog[2 * lane] = a;
+               prog[2 * lane + 1] = b;
+            }
+            waveSync();
          }
-         return instrs[pd.instr_begin + i];
+         const uint32_t k = i - prog_base;
+         const u32x4 a = prog[2 * k], b = prog[2 * k + 1];
+         return Instr{uniform(a[0]), uniform(a[1]), uniform(a[2]), uniform(a[3]),
+                      uniform(b[0]), uniform(b[1]), uniform(b[2]), uniform(b[3])};
       };
       for (uint32_t i = 0; i < pd.n_instr; ++i) {
          const Instr ins = fetch(i);
@@ -431,10 +444,49 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                const int32_t v2 = static_cast<int32_t>(ins.f);
                const bool two = (ins.g & 1u) != 0;
                const uint64_t nulls = toGlobal<GlobalU64>(col.null_bits)[static_cast<size_t>(chunk) * kChunkWords + my_word];
+               uint64_t word = 0;
+               const uint32_t np = col.n_planes;
+               if (np > 0) {
+                  // bit-sliced: the lane's word of every plane, then a most-significant-first
+                  // comparison of (value - plane_min) with (v - plane_min), rows below and equal
+                  const GlobalU64* pl = toGlobal<GlobalU64>(col.planes) + static_cast<size_t>(chunk) * kChunkWords + my_word;
+                  const size_t plane_stride = static_cast<size_t>(n_chunks) * kChunkWords;
+                  uint64_t pw[kMaxIntPlanes];
+#pragma unroll
+                  for (uint32_t p = 0; p < kMaxIntPlanes; ++p) pw[p] = p < np ? pl[p * plane_stride] : 0ull;
+                  auto compareWord = [&](uint32_t cmp, int32_t value) -> uint64_t {
+                     const int64_t t = static_cast<int64_t>(value) - col.plane_min;
+                     uint64_t lt = 0, eq = 0;
+                     if (t >= (int64_t{1} << np)) {
+                        lt = ~0ull;  // every value is below
+                     } else if (t >= 0) {
+                        eq = ~0ull;
+#pragma unroll
+                        for (int p = static_cast<int>(kMaxIntPlanes) - 1; p >= 0; --p) {
+                           if (static_cast<uint32_t>(p) < np) {
+                              if ((t >> p) & 1) {
+                                 lt |= eq & ~pw[p];
+                                 eq &= pw[p];
+                              } else {
+                                 eq &= ~pw[p];
+                              }
+                           }
+                        }
+                     }  // t < 0: every value is above
+                     switch (cmp) {
+                        case CMP_EQ: return eq;
+                        case CMP_LT: return lt;
+                        case CMP_GT: return ~(lt | eq);
+                        case CMP_LE: return lt | eq;
+                        case CMP_GE: return ~lt;
+                        default: return ~eq;
+                     }
+                  };
+                  word = compareWord(ins.b, v) & (two ? compareWord(ins.e, v2) : ~0ull);
+               } else {
                // rows read 64 at a time, consecutive lanes on consecutive rows; the comparison's
                // ballot is the word of those 64 rows, kept by the lane that owns it
                const GlobalU32* vals = toGlobal<GlobalU32>(col.values) + row_base;
-               uint64_t word = 0;
                for (uint32_t u0 = 0; u0 < 64; u0 += 32) {
                   uint32_t x[32];
 #pragma unroll
@@ -445,6 +497,7 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                      const uint64_t bw = __ballot(compareInt(ins.b, xv, v) && (!two || compareInt(ins.e, xv, v2)));
                      if (lane == u0 + u) word = bw;
                   }
+               }
                }
                // null rows: the stored value is ignored
                const uint64_t null_match = (ins.d != 0) && (!two || (ins.g & 2u) != 0) ? nulls : 0ULL;

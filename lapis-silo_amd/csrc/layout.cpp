@@ -920,6 +920,35 @@ void buildIntColumn(Table& t, IntColumn& col, const int32_t* values, const uint8
    col.view.values = col.buffers.back().as<int32_t>();
    col.buffers.emplace_back(null_bits.data(), null_bits.size() * 8);
    col.view.null_bits = col.buffers.back().as<uint64_t>();
+   // bit-sliced copy for comparisons: a date range then reads ceil(log2(span)) bits per row instead
+   // of 32
+   col.view.planes = nullptr;
+   col.view.plane_min = 0;
+   col.view.n_planes = 0;
+   if (!col.sorted_values.empty()) {
+      const int64_t lo = col.sorted_values.front(), hi = col.sorted_values.back();
+      const uint64_t span = static_cast<uint64_t>(hi - lo);
+      if (span < (uint64_t{1} << kMaxIntPlanes)) {
+         const uint32_t np = std::max<uint32_t>(1, static_cast<uint32_t>(std::bit_width(span)));
+         const size_t plane_words = static_cast<size_t>(t.n_chunks) * kChunkWords;
+         std::vector<uint64_t> planes(np * plane_words, 0);
+         for (uint32_t c = 0; c < t.n_chunks; ++c) {
+            for (uint32_t r = 0; r < t.chunk_sizes[c]; ++r) {
+               const uint64_t i = t.chunk_offset[c] + r;
+               if (is_null[i]) continue;
+               const size_t g = (static_cast<size_t>(c) << 16) | r;
+               const uint64_t x = static_cast<uint64_t>(static_cast<int64_t>(values[i]) - lo);
+               for (uint32_t p = 0; p < np; ++p) {
+                  if ((x >> p) & 1u) planes[p * plane_words + (g >> 6)] |= 1ULL << (g & 63);
+               }
+            }
+         }
+         col.buffers.emplace_back(planes.data(), planes.size() * 8);
+         col.view.planes = col.buffers.back().as<uint64_t>();
+         col.view.plane_min = static_cast<int32_t>(lo);
+         col.view.n_planes = np;
+      }
+   }
 }
 
 void buildDictColumn(Table& t, DictColumn& col, const uint32_t* ids, const uint8_t* is_null,

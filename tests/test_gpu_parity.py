@@ -196,3 +196,45 @@ def test_dense_kernel_shapes(engine, coop):
             engine.set_tuning(k, v)
         et.close()
         ot.close()
+
+
+def test_int_columns_bit_sliced(engine):
+    """Comparisons on int and date columns: bit-sliced copies (spans 0, 2,000 and 65,535, negative
+    values, nulls) and the plain value scan (span 65,536) give the oracle's rows for thresholds
+    inside, at the edges of and outside each column's range, with open bounds and negations."""
+    from oracle.oracle import OracleTable
+    from silo_amd import table
+
+    rng = np.random.default_rng(17)
+    n = 70000
+    spec = table.TableSpec(chunk_sizes=np.asarray([40000, 30000], np.uint32))
+    cols = {
+        "a": (False, rng.integers(-1000, 1001, n)),
+        "b": (False, np.concatenate([[7, 7 + 65535], rng.integers(7, 7 + 65536, n - 2)])),
+        "c": (False, np.concatenate([[-5, -5 + 65536], rng.integers(-5, -5 + 65537, n - 2)])),
+        "d": (False, np.full(n, 42)),
+        "e": (True, rng.integers(18000, 19001, n)),
+    }
+    for name, (is_date, vals) in cols.items():
+        nulls = (rng.random(n) < 0.05).astype(np.uint8)
+        nulls[:2] = 0  # keep the range ends
+        spec.int_columns.append(table.IntColumnData(name, is_date, vals.astype(np.int32), nulls))
+    ot = OracleTable(spec)
+    et = engine.table(spec)
+    try:
+        exprs = []
+        for name, (is_date, vals) in cols.items():
+            lo, hi = int(vals.min()), int(vals.max())
+            mid = int(vals[rng.integers(0, n)])
+            points = [lo - 1, lo, lo + 1, mid, hi - 1, hi, hi + 1, -2**31, 2**31 - 1]
+            op = "date_between" if is_date else "int_between"
+            for v in points:
+                if not is_date:
+                    exprs.append(f"(int_eq {name} {v})")
+                exprs += [f"({op} {name} {v} _)", f"({op} {name} _ {v})", f"(not ({op} {name} {v} {mid}))"]
+            exprs += [f"({op} {name} {lo} {hi})", f"({op} {name} {mid} {hi + 5})", f"({op} {name} _ _)"]
+        for e in exprs:
+            assert np.array_equal(et.filter(e), ot.filter(e)), e
+    finally:
+        et.close()
+        ot.close()
