@@ -99,6 +99,11 @@ struct SeqColumnView {
    const uint8_t* local_ref;          // [genome_length]
    const uint8_t* reference;          // [genome_length]
    const uint32_t* cov;               // [n_chunks * 2^16] start | end << 16
+   // per (chunk, slice): the largest start and smallest end over the slice's rows that have a
+   // covered range (start | end << 16; 0xFFFF | 0 when none), and the rows without one (null or
+   // empty range): a position inside the bounds is covered by every other row of the slice
+   const uint32_t* cov_bounds;        // [n_chunks * 16]
+   const uint64_t* cov_never;         // [n_chunks * 1024]
    const uint32_t* nrun_ptr;          // [n_chunks * 2^16 + 1]
    const uint32_t* nruns;             // start | end << 16, end exclusive
    // missing-run index by position: list (chunk c, bucket k, slice s) holds the runs of slice s's

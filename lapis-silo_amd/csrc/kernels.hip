@@ -385,9 +385,15 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                   my_len = rp[1] - rp[0];
                }
                scratch[lane] = 0;
-               // rows read 64 at a time, consecutive lanes on consecutive rows; the test's ballot is
-               // the word of those 64 rows, kept by the lane that owns it
+               // inside the slice's coverage bounds every row with a covered range covers p; else
+               // rows are read 64 at a time, consecutive lanes on consecutive rows, and the test's
+               // ballot is the word of those 64 rows, kept by the lane that owns it
+               const uint32_t cb = uniform(toGlobal<GlobalU32>(col.cov_bounds)[static_cast<size_t>(chunk) * kSlicesPerChunk + slice]);
+               const uint64_t never = toGlobal<GlobalU64>(col.cov_never)[static_cast<size_t>(chunk) * kChunkWords + my_word];
                uint64_t word = 0;
+               if ((cb & 0xFFFFu) <= p && p < (cb >> 16)) {
+                  word = ~never;
+               } else {
                for (uint32_t u0 = 0; u0 < 64; u0 += 32) {
                   uint32_t cs[32];
 #pragma unroll
@@ -397,6 +403,7 @@ __global__ __launch_bounds__(kVmThreads) void filterVmKernel(TableView tv, const
                      const uint64_t bw = __ballot((cs[u] & 0xFFFFu) <= p && p < (cs[u] >> 16));
                      if (lane == u0 + u) word = bw;
                   }
+               }
                }
                // flat index over the lists' entries
                uint32_t lo[8], start[9];

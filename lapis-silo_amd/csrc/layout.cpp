@@ -867,6 +867,30 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    v.local_ref = static_cast<const uint8_t*>(up(col.local_ref.data(), L));
    v.reference = static_cast<const uint8_t*>(up(col.reference.data(), L));
    v.cov = static_cast<const uint32_t*>(up(cov.data(), cov.size() * 4));
+   {
+      std::vector<uint32_t> bounds(static_cast<size_t>(t.n_chunks) * kSlicesPerChunk, 0xFFFFu);
+      std::vector<uint64_t> never(static_cast<size_t>(t.n_chunks) * kChunkWords, 0);
+      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+         for (uint32_t sl = 0; sl < kSlicesPerChunk; ++sl) {
+            uint32_t lo = 0, hi = 0xFFFFu;
+            bool any = false;
+            for (uint32_t r = sl * 4096u; r < std::min<uint32_t>((sl + 1) * 4096u, t.chunk_sizes[c]); ++r) {
+               const size_t g = (static_cast<size_t>(c) << 16) | r;
+               const uint32_t x = cov[g];
+               if (x == 0) {
+                  never[g >> 6] |= 1ULL << (g & 63);  // null row or empty range
+               } else {
+                  lo = std::max(lo, x & 0xFFFFu);
+                  hi = std::min(hi, x >> 16);
+                  any = true;
+               }
+            }
+            bounds[static_cast<size_t>(c) * kSlicesPerChunk + sl] = any ? (lo | (hi << 16)) : 0xFFFFu;
+         }
+      }
+      v.cov_bounds = static_cast<const uint32_t*>(up(bounds.data(), bounds.size() * 4));
+      v.cov_never = static_cast<const uint64_t*>(up(never.data(), never.size() * 8));
+   }
    v.nrun_ptr = static_cast<const uint32_t*>(up(pptr.data(), pptr.size() * 4));
    v.nruns = static_cast<const uint32_t*>(up(runs_all.data(), runs_all.size() * 4));
    v.n_run_buckets = n_rb;
