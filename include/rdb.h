@@ -244,7 +244,7 @@ int rdb_set_planner(rdb_ctx* ctx, int mode, rdb_mlp* mlp);
  *   "mutation_coop"    1: the four wavefronts of a dense-kernel work-group count each chunk's tile
  *                      together, 0: each wavefront takes its own chunks (default);
  *   "batch_max_queries" queries per lane job of the batched path (1..4096, default 256);
- *   "batch_lanes"      lanes (HIP streams with their own buffers) per table (1..8, default 2);
+ *   "batch_lanes"      lanes (HIP streams with their own buffers) per table (1..16, default 2);
  *   "batch_graphs"     1: replay each lane's launch sequence from a captured HIP graph (default);
  *   "batch_shared_scan" how a job's dense Mutations queries pass over the index: 0 one pass per
  *                      query (default), 1 four queries per pass, 2 eight per pass (row bytes of

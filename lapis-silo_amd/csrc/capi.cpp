@@ -216,7 +216,7 @@ int rdb_set_tuning(rdb_ctx* ctx, const char* key, int64_t value) {
          if (value < 1 || value > 4096) fail(RDB_EINVAL, "batch_max_queries must be in [1, 4096]");
          ctx->c.batch_max_queries = static_cast<uint32_t>(value);
       } else if (k == "batch_lanes") {
-         if (value < 1 || value > 8) fail(RDB_EINVAL, "batch_lanes must be in [1, 8]");
+         if (value < 1 || value > 16) fail(RDB_EINVAL, "batch_lanes must be in [1, 16]");
          ctx->c.batch_lanes = static_cast<uint32_t>(value);
       } else if (k == "batch_graphs") {
          ctx->c.batch_graphs = value != 0;

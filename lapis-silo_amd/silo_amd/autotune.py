@@ -108,7 +108,8 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
     per four queries or one per eight, the last with work-groups of 512 or 1024 threads,
     "batch_octet_threads") by timing a batch of `size` Mutations queries over the probe filters each
     way, then (given the probes' cardinalities) the dense/sparse switch for batches among those
-    cardinalities; the results must agree throughout.  The knobs are applied."""
+    cardinalities, then the number of batches in flight ("batch_lanes") and the dense kernel again at
+    that concurrency; the results must agree throughout.  The knobs are applied."""
     say = log or (lambda *_: None)
     queries = [{"filter": probes[i % len(probes)], "action": "mutations", "column": column, "min_proportion": 0.05}
                for i in range(size)]
@@ -154,13 +155,57 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
         thr = min(sw, key=sw.get)
         engine.set_tuning("sparse_max_rows", thr)
         say(f"batch dense/sparse switch, ms: {[(k, round(v * 1e3, 3)) for k, v in sw.items()]}; chose {thr}")
+    # lanes in flight: concurrent batches overlap one batch's latency-bound kernels with another's
+    # (candidates whose buffers fit half the free device memory), then the dense scan again at
+    # that concurrency
+    free, _ = engine.device_memory()
+    lane = table.lane_bytes(size, engine.get_tuning("batch_row_capacity"))
+    # at most 8: a query's latency grows with the batches ahead of it
+    fit = int(max(1, min(8, (free // 2) // max(lane, 1))))
+    lanes_qps = {}
+    for lanes in sorted({min(n, fit) for n in (1, 2, 4, 6, 8)}):
+        engine.set_tuning("batch_lanes", lanes)
+        lanes_qps[lanes] = pipelined(table, queries, lanes, steps=max(32, 8 * lanes))
+    best_lanes = max(lanes_qps, key=lanes_qps.get)
+    engine.set_tuning("batch_lanes", best_lanes)
+    mode_qps = {}
+    for mode in ((0, 1024), (1, 1024), (2, 1024), (2, 512)):
+        engine.set_tuning("batch_shared_scan", mode[0])
+        engine.set_tuning("batch_octet_threads", mode[1])
+        mode_qps[mode] = pipelined(table, queries, best_lanes, steps=max(32, 8 * best_lanes))
+    best = max(mode_qps, key=mode_qps.get)
+    engine.set_tuning("batch_shared_scan", best[0])
+    engine.set_tuning("batch_octet_threads", best[1])
+    say(f"batches in flight, queries/s: {[(k, round(v)) for k, v in lanes_qps.items()]}; chose {best_lanes}; "
+        f"dense scans at that concurrency: {[(k, round(v)) for k, v in mode_qps.items()]}; chose {best}")
     engine.set_tuning("batch_max_queries", saved)
-    out = {"batch_shared_scan": best[0], "batch_octet_threads": best[1],
+    out = {"batch_shared_scan": best[0], "batch_octet_threads": best[1], "batch_lanes": best_lanes,
            "batch_ms": {"per_query": round(times[(0, 1024)] * 1e3, 3), "four": round(times[(1, 1024)] * 1e3, 3),
                         "eight": round(times[(2, 1024)] * 1e3, 3), "eight_512": round(times[(2, 512)] * 1e3, 3)}}
     if cards is not None:
         out["batch_sparse_max_rows"] = engine.get_tuning("sparse_max_rows")
     return out
+
+
+def pipelined(table, queries: list, in_flight: int, steps: int) -> float:
+    """Queries/s of `steps` batches of `queries` with up to `in_flight` of them submitted ahead of
+    the collection of their results (columnar), after one warm-up batch."""
+    from collections import deque
+
+    table.batch(queries, arrays=True)
+    pend = deque()
+    t = time.perf_counter()
+    for _ in range(steps):
+        pend.append(table.batch_submit(queries))
+        if len(pend) == in_flight:
+            b = pend.popleft()
+            b.results(arrays=True)
+            b.free()
+    while pend:
+        b = pend.popleft()
+        b.results(arrays=True)
+        b.free()
+    return steps * len(queries) / (time.perf_counter() - t)
 
 
 def probe_filters(ds, n: int = 8) -> list:
