@@ -139,8 +139,9 @@ void setupLane(Table& t, BatchLane& lane) {
 }
 
 // The lane's fixed kernel sequence for programs of at most `depth` stack entries whose Mutations
-// queries read sequence column `column` (-1: none).
-void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column) {
+// queries read sequence column `column` (-1: none); the histogram kernels only when the job has
+// Aggregated queries.
+void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column, bool aggregates) {
    Context& ctx = *t.ctx;
    const uint32_t Q = lane.max_queries;
    const ParamsLayout pl(Q, lane.instr_capacity, lane.range_capacity);
@@ -192,23 +193,25 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column) {
    } else {
       hipCheck(launchFill(lane.n_rows.as<uint32_t>(), 0, Q, lane.stream), "clear row counts");
    }
-   AggregateBatch a;
-   a.n_chunks = t.n_chunks;
-   a.max_entries = Q;
-   a.n_agg = n_queries + 1;
-   a.total_bins = n_queries + 2;
-   a.max_total_bins = lane.hist_capacity;
-   a.agg_query = reinterpret_cast<const uint32_t*>(pd + pl.agg_query);
-   a.bins = reinterpret_cast<const uint32_t*>(pd + pl.agg_bins);
-   a.hist_offset = reinterpret_cast<const uint32_t*>(pd + pl.agg_offset);
-   a.dims = reinterpret_cast<const GroupDims*>(pd + pl.dims);
-   a.filters = lane.filters.as<uint64_t>();
-   a.chunk_cards = chunk_cards;
-   a.hist = lane.hist.as<uint32_t>();
-   a.rows = lane.rows.as<MutationRowDev>();
-   a.row_capacity = lane.row_capacity;
-   a.n_rows = lane.n_rows.as<uint32_t>();
-   hipCheck(launchAggregateBatch(a, lane.stream), "batched group histogram");
+   if (aggregates) {
+      AggregateBatch a;
+      a.n_chunks = t.n_chunks;
+      a.max_entries = Q;
+      a.n_agg = n_queries + 1;
+      a.total_bins = n_queries + 2;
+      a.max_total_bins = lane.hist_capacity;
+      a.agg_query = reinterpret_cast<const uint32_t*>(pd + pl.agg_query);
+      a.bins = reinterpret_cast<const uint32_t*>(pd + pl.agg_bins);
+      a.hist_offset = reinterpret_cast<const uint32_t*>(pd + pl.agg_offset);
+      a.dims = reinterpret_cast<const GroupDims*>(pd + pl.dims);
+      a.filters = lane.filters.as<uint64_t>();
+      a.chunk_cards = chunk_cards;
+      a.hist = lane.hist.as<uint32_t>();
+      a.rows = lane.rows.as<MutationRowDev>();
+      a.row_capacity = lane.row_capacity;
+      a.n_rows = lane.n_rows.as<uint32_t>();
+      hipCheck(launchAggregateBatch(a, lane.stream), "batched group histogram");
+   }
    hipCheck(launchPackRows(n_queries, Q, lane.n_rows.as<uint32_t>(), cards, lane.rows.as<MutationRowDev>(), lane.row_capacity,
                            reinterpret_cast<uint32_t*>(od + ol.header), reinterpret_cast<MutationRowDev*>(od + ol.rows),
                            lane.stream),
@@ -225,23 +228,23 @@ void ensureOctetBuffers(const Table& t, BatchLane& lane) {
    lane.octet_masks = DeviceBuffer(static_cast<size_t>((Q + 7) / 8) * t.n_chunks * kChunkRows);
 }
 
-void launchLane(Table& t, BatchLane& lane, uint32_t depth, int column) {
+void launchLane(Table& t, BatchLane& lane, uint32_t depth, int column, bool aggregates) {
    Context& ctx = *t.ctx;
    ensureOctetBuffers(t, lane);
    const uint32_t bucket = depthBucket(depth, maxFilterDepth(ctx));
    if (!ctx.batch_graphs) {
-      enqueueKernels(t, lane, bucket, column);
+      enqueueKernels(t, lane, bucket, column, aggregates);
       return;
    }
    const uint32_t knobs = ctx.mutation_unroll | (ctx.batch_shared_scan << 8) | (ctx.mutation_coop ? 1u << 12 : 0u) |
-                          (ctx.batch_octet_threads == 512 ? 1u << 13 : 0u);
+                          (ctx.batch_octet_threads == 512 ? 1u << 13 : 0u) | (aggregates ? 1u << 14 : 0u);
    const auto key = std::make_tuple(bucket, column, static_cast<long long>(ctx.sparse_max_rows), knobs);
    auto it = lane.graphs.find(key);
    if (it == lane.graphs.end()) {
       hipGraph_t graph = nullptr;
       hipCheck(hipStreamBeginCapture(lane.stream, hipStreamCaptureModeThreadLocal), "begin capture");
       try {
-         enqueueKernels(t, lane, bucket, column);
+         enqueueKernels(t, lane, bucket, column, aggregates);
       } catch (...) {
          (void)hipStreamEndCapture(lane.stream, &graph);
          if (graph) (void)hipGraphDestroy(graph);
@@ -322,7 +325,7 @@ void enqueueJob(Table& t, BatchResult& b, const std::vector<CompiledQuery>& cq, 
    lane.job_first_bytes = std::min(lane.out_bytes, ol.rows + (size_t{1} << 20));
    b.pending.push_back(&lane);
    hipCheck(hipMemcpyAsync(lane.params.as<void>(), hp, pl.bytes, hipMemcpyHostToDevice, lane.stream), "stage batch");
-   launchLane(t, lane, depth, job.column);
+   launchLane(t, lane, depth, job.column, n_agg > 0);
    hipCheck(hipMemcpyAsync(lane.host_out, lane.out.as<void>(), lane.job_first_bytes, hipMemcpyDeviceToHost, lane.stream),
             "copy batch results");
    hipCheck(hipEventRecord(lane.done, lane.stream), "hipEventRecord");
