@@ -295,7 +295,11 @@ def main():
                    "train_s": round(t_train, 1), "workload": sp}
         log(f"planner: {planner}")
         eng.set_planner("learned", mlp)
-    eng.set_tuning("batch_max_queries", min(max(len(queries), 1), 256))
+    job = min(max(len(queries), 1), 256)
+    if tuned and "batch_job_queries" in tuned and len(queries) <= tuned.get("probe_batch", 0):
+        # a step the size of the autotuner's probe batch is cut into jobs as the probe's was
+        job = max(1, len(queries) * tuned["batch_job_queries"] // tuned["probe_batch"])
+    eng.set_tuning("batch_max_queries", job)
     for _ in range(max(args.warmup, 1)):
         et.batch(queries, arrays=True)
     # up to one batch per lane in flight: a step's host work (compiling and staging the next batch,

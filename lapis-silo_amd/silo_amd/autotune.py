@@ -178,8 +178,26 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
     engine.set_tuning("batch_octet_threads", best[1])
     say(f"batches in flight, queries/s: {[(k, round(v)) for k, v in lanes_qps.items()]}; chose {best_lanes}; "
         f"dense scans at that concurrency: {[(k, round(v)) for k, v in mode_qps.items()]}; chose {best}")
+    steps = max(32, 8 * best_lanes)
+    # the dense/sparse switch and the job size (queries per lane job; the batch of `size` is cut into
+    # size / job jobs) at that concurrency
+    if cards is not None:
+        sw_qps = {}
+        for thr in sorted({-1, engine.get_tuning("sparse_max_rows"), *[int(c) for c in cards if c < 1 << 20]}):
+            engine.set_tuning("sparse_max_rows", thr)
+            sw_qps[thr] = pipelined(table, queries, best_lanes, steps=steps)
+        thr = max(sw_qps, key=sw_qps.get)
+        engine.set_tuning("sparse_max_rows", thr)
+        say(f"dense/sparse switch at that concurrency, queries/s: {[(k, round(v)) for k, v in sw_qps.items()]}; chose {thr}")
+    job_qps = {}
+    for job in sorted({max(1, size // 4), max(1, size // 2), size}):
+        engine.set_tuning("batch_max_queries", job)
+        job_qps[job] = pipelined(table, queries, best_lanes, steps=steps)
+    best_job = max(job_qps, key=job_qps.get)
+    say(f"queries per lane job, queries/s: {[(k, round(v)) for k, v in job_qps.items()]}; chose {best_job}")
     engine.set_tuning("batch_max_queries", saved)
     out = {"batch_shared_scan": best[0], "batch_octet_threads": best[1], "batch_lanes": best_lanes,
+           "batch_job_queries": best_job, "probe_batch": size,
            "batch_ms": {"per_query": round(times[(0, 1024)] * 1e3, 3), "four": round(times[(1, 1024)] * 1e3, 3),
                         "eight": round(times[(2, 1024)] * 1e3, 3), "eight_512": round(times[(2, 512)] * 1e3, 3)}}
     if cards is not None:
