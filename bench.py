@@ -22,7 +22,9 @@ Two paths run the same workload:
 * plain: one C-ABI call per query (filter kernel + action kernels + one host round trip);
 * batched: the whole step as one batch (rdb_batch_*): filters of all queries in one launch,
   dense or sparse Mutations chosen per query on the device, HIP-graph replay, with the kernel
-  knobs picked by the autotuner beforehand (outside the timed region).
+  knobs picked by the autotuner beforehand (outside the timed region); the next step's batch is
+  submitted before the previous one's results are collected, up to one batch per lane (the
+  autotuner picks 1-8 lanes), so that concurrent batches share the GPU.
 
 `value` is the batched path's queries/s; the plain path's figures sit beside it.  Every query's
 batched result is compared with its plain result.  The CPU baseline is the oracle (a
