@@ -250,6 +250,8 @@ int rdb_set_planner(rdb_ctx* ctx, int mode, rdb_mlp* mlp);
  *                      query (default), 1 four queries per pass, 2 eight per pass (row bytes of
  *                      eight filters; the lane then also holds 1 byte per row per 8 queries);
  *   "batch_octet_threads" work-group size of the eight-per-pass scan, 512 or 1024 (default);
+ *   "batch_octet_groups" its persistent work-groups, 0 (default: one per CU) .. 4096; fewer leave
+ *                      CUs to concurrent batches;
  *   "batch_row_capacity" Mutations rows per query kept by a lane job (more: the query runs again
  *                      alone), default 4096.
  * Results never depend on them.  RDB_EINVAL for an unknown key or an out-of-range value. */

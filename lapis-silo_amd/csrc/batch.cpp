@@ -181,7 +181,7 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column, bool 
       m.shared_mode = ctx.batch_shared_scan;
       m.partial_c = lane.partial_c.bytes() ? lane.partial_c.as<uint32_t>() : nullptr;
       m.octet_masks = lane.octet_masks.bytes() ? lane.octet_masks.as<uint8_t>() : nullptr;
-      m.persistent_groups = static_cast<uint32_t>(ctx.cu_count);
+      m.persistent_groups = ctx.batch_octet_groups ? ctx.batch_octet_groups : static_cast<uint32_t>(ctx.cu_count);
       m.octet_threads = ctx.batch_octet_threads;
       m.sparse_counts = lane.sparse_counts.as<uint32_t>();
       m.sparse_bnd = lane.sparse_bnd.as<int32_t>();
@@ -237,7 +237,8 @@ void launchLane(Table& t, BatchLane& lane, uint32_t depth, int column, bool aggr
       return;
    }
    const uint32_t knobs = ctx.mutation_unroll | (ctx.batch_shared_scan << 8) | (ctx.mutation_coop ? 1u << 12 : 0u) |
-                          (ctx.batch_octet_threads == 512 ? 1u << 13 : 0u) | (aggregates ? 1u << 14 : 0u);
+                          (ctx.batch_octet_threads == 512 ? 1u << 13 : 0u) | (aggregates ? 1u << 14 : 0u) |
+                          (ctx.batch_octet_groups << 16);
    const auto key = std::make_tuple(bucket, column, static_cast<long long>(ctx.sparse_max_rows), knobs);
    auto it = lane.graphs.find(key);
    if (it == lane.graphs.end()) {

@@ -226,6 +226,9 @@ int rdb_set_tuning(rdb_ctx* ctx, const char* key, int64_t value) {
       } else if (k == "batch_octet_threads") {
          if (value != 512 && value != 1024) fail(RDB_EINVAL, "batch_octet_threads must be 512 or 1024");
          ctx->c.batch_octet_threads = static_cast<uint32_t>(value);
+      } else if (k == "batch_octet_groups") {
+         if (value < 0 || value > 4096) fail(RDB_EINVAL, "batch_octet_groups must be in [0, 4096]");
+         ctx->c.batch_octet_groups = static_cast<uint32_t>(value);
       } else if (k == "batch_row_capacity") {
          if (value < 16 || value > (1 << 20)) fail(RDB_EINVAL, "batch_row_capacity must be in [16, 2^20]");
          ctx->c.batch_row_capacity = static_cast<uint32_t>(value);
@@ -260,6 +263,8 @@ int rdb_get_tuning(rdb_ctx* ctx, const char* key, int64_t* value) {
          *value = ctx->c.batch_shared_scan;
       } else if (k == "batch_octet_threads") {
          *value = ctx->c.batch_octet_threads;
+      } else if (k == "batch_octet_groups") {
+         *value = ctx->c.batch_octet_groups;
       } else if (k == "batch_row_capacity") {
          *value = ctx->c.batch_row_capacity;
       } else {

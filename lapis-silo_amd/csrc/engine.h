@@ -128,6 +128,7 @@ struct Context {
    uint32_t batch_row_capacity = 4096; // Mutations rows per query a lane job keeps (more: rerun alone)
    uint32_t batch_shared_scan = 0;     // dense Mutations of a job: 1 four, 2 eight queries per pass over the index
    uint32_t batch_octet_threads = 1024;  // work-group size of the eight-query scan (512 or 1024)
+   uint32_t batch_octet_groups = 0;      // its persistent work-groups (0: one per CU)
    // planner of the batched path: 0 reference order, 1 independence estimates, 2 learned estimator
    int plan_mode = 0;
    struct Mlp* planner = nullptr;

@@ -117,6 +117,7 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
     engine.set_tuning("batch_max_queries", size)
     times = {}
     results = {}
+    engine.set_tuning("batch_octet_groups", 0)
     for mode in ((0, 1024), (1, 1024), (2, 1024), (2, 512)):
         engine.set_tuning("batch_shared_scan", mode[0])
         engine.set_tuning("batch_octet_threads", mode[1])
@@ -168,14 +169,18 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
         lanes_qps[lanes] = pipelined(table, queries, lanes, steps=max(32, 8 * lanes))
     best_lanes = max(lanes_qps, key=lanes_qps.get)
     engine.set_tuning("batch_lanes", best_lanes)
+    # (scan, work-group size, persistent work-groups of the eight-query scan: 0 = one per CU; fewer
+    # leave CUs to the other batches in flight)
     mode_qps = {}
-    for mode in ((0, 1024), (1, 1024), (2, 1024), (2, 512)):
+    for mode in ((0, 1024, 0), (1, 1024, 0), (2, 1024, 0), (2, 512, 0), (2, 512, 128), (2, 512, 64)):
         engine.set_tuning("batch_shared_scan", mode[0])
         engine.set_tuning("batch_octet_threads", mode[1])
+        engine.set_tuning("batch_octet_groups", mode[2])
         mode_qps[mode] = pipelined(table, queries, best_lanes, steps=max(32, 8 * best_lanes))
     best = max(mode_qps, key=mode_qps.get)
     engine.set_tuning("batch_shared_scan", best[0])
     engine.set_tuning("batch_octet_threads", best[1])
+    engine.set_tuning("batch_octet_groups", best[2])
     say(f"batches in flight, queries/s: {[(k, round(v)) for k, v in lanes_qps.items()]}; chose {best_lanes}; "
         f"dense scans at that concurrency: {[(k, round(v)) for k, v in mode_qps.items()]}; chose {best}")
     steps = max(32, 8 * best_lanes)
@@ -196,7 +201,8 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
     best_job = max(job_qps, key=job_qps.get)
     say(f"queries per lane job, queries/s: {[(k, round(v)) for k, v in job_qps.items()]}; chose {best_job}")
     engine.set_tuning("batch_max_queries", saved)
-    out = {"batch_shared_scan": best[0], "batch_octet_threads": best[1], "batch_lanes": best_lanes,
+    out = {"batch_shared_scan": best[0], "batch_octet_threads": best[1], "batch_octet_groups": best[2],
+           "batch_lanes": best_lanes,
            "batch_job_queries": best_job, "probe_batch": size,
            "batch_ms": {"per_query": round(times[(0, 1024)] * 1e3, 3), "four": round(times[(1, 1024)] * 1e3, 3),
                         "eight": round(times[(2, 1024)] * 1e3, 3), "eight_512": round(times[(2, 512)] * 1e3, 3)}}

@@ -8,7 +8,7 @@ from helpers import ExprGen, random_bitmaps, small_spec
 pytestmark = pytest.mark.gpu
 
 KNOBS = ("sparse_max_rows", "batch_max_queries", "batch_lanes", "batch_graphs", "batch_row_capacity", "batch_shared_scan",
-         "batch_octet_threads")
+         "batch_octet_threads", "batch_octet_groups")
 # (batch_shared_scan, batch_octet_threads): every dense counting kernel of a batch
 SCANS = [(0, 1024), (1, 1024), (2, 1024), (2, 512)]
 
@@ -162,6 +162,12 @@ def test_shared_scan_many_chunks(engine, tuning):
             tuning.set_tuning("batch_shared_scan", shared)
             tuning.set_tuning("batch_octet_threads", threads)
             assert et.batch(queries) == want, (shared, threads)
+        # the persistent eight-query scan with fewer work-groups than CUs (each takes more items)
+        tuning.set_tuning("batch_shared_scan", 2)
+        tuning.set_tuning("batch_octet_threads", 512)
+        for groups in (1, 7, 64):
+            tuning.set_tuning("batch_octet_groups", groups)
+            assert et.batch(queries) == want, groups
     finally:
         et.close()
 
