@@ -2793,12 +2793,18 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
    hipError_t err = hipSuccess;
    hipLaunchKernelGGL(batchPlanKernel, dim3(1), dim3(1024), 0, stream, m);
    if ((err = hipGetLastError()) != hipSuccess) return err;
-   if (m.heavy_sums != nullptr && m.shared_mode == 2) {
+   // the dense scan asked for, or the next smaller one where a table is too large for it (the
+   // knob never changes results and never fails a batch)
+   const uint32_t max_octets = (m.max_queries + 7) / 8;
+   uint32_t mode = m.heavy_sums == nullptr ? 0u : m.shared_mode;
+   if (mode == 2 && (static_cast<uint64_t>(max_octets) * m.n_chunks > 65535 ||
+                     (m.n_chunks + kOctetParts - 1) / kOctetParts > kOctetMaxPartChunks || m.octet_masks == nullptr ||
+                     m.partial_c == nullptr || m.partial_b == nullptr)) {
+      mode = 1;
+   }
+   if (mode == 1 && ((m.n_chunks + 1) / 2 > kSharedMaxPartChunks || m.partial_b == nullptr)) mode = 0;
+   if (mode == 2) {
       // shared scan, eight dense queries per pass: row bytes, then the persistent counting kernel
-      const uint32_t max_octets = (m.max_queries + 7) / 8;
-      if (static_cast<uint64_t>(max_octets) * m.n_chunks > 65535) return hipErrorInvalidValue;
-      if ((m.n_chunks + kOctetParts - 1) / kOctetParts > kOctetMaxPartChunks) return hipErrorInvalidValue;
-      if (m.octet_masks == nullptr || m.partial_c == nullptr || m.partial_b == nullptr) return hipErrorInvalidValue;
       if (m.n_chunks > 0) {
          hipLaunchKernelGGL(octetMasksKernel, dim3(kChunkWords / kOctetMaskThreads, max_octets * m.n_chunks),
                             dim3(kOctetMaskThreads), 0, stream, m);
@@ -2822,9 +2828,8 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
       const uint32_t sblocks = (h.n_slots + 255) / 256;
       hipLaunchKernelGGL(foldSharedPartsKernel, dim3(sblocks == 0 ? 1 : (sblocks < 64 ? sblocks : 64), m.max_queries),
                          dim3(256), 0, stream, m, h.n_slots, kOctetParts);
-   } else if (m.heavy_sums != nullptr) {
+   } else if (mode == 1) {
       // shared scan, four dense queries per pass, chunks in two parts
-      if ((m.n_chunks + 1) / 2 > kSharedMaxPartChunks) return hipErrorInvalidValue;
       const uint32_t n_xblocks = 8 * ((h.n_blocks + 7) / 8);
       const dim3 grid(n_xblocks * 2 * ((m.max_queries + 3) / 4));
       if (m.unroll == 16) {
