@@ -1890,18 +1890,6 @@ __global__ __launch_bounds__(kOctetMaskThreads) void octetMasksKernel(MutationsB
    for (uint32_t h = 0; h < 4; ++h) dst[h] = u64x2{y[2 * h], y[2 * h + 1]};
 }
 
-// popcount of bits a..b (inclusive) of a filter tile in global memory
-__device__ inline uint32_t rangePopcountG(const GlobalU64* f, uint32_t a, uint32_t b) {
-   uint32_t n = 0;
-   const uint32_t wa = a >> 6, wb = b >> 6;
-   for (uint32_t w = wa; w <= wb; ++w) {
-      const uint32_t x = w == wa ? (a & 63) : 0;
-      const uint32_t y = w == wb ? (b & 63) : 63;
-      n += __popcll(f[w] & rangeMask(x, y));
-   }
-   return n;
-}
-
 template <uint32_t kRound, uint32_t kThreads>
 __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsBatch m) {
    static_assert(kChunkWords % kThreads == 0 && kThreads % 64 == 0, "whole bitset words per thread");
@@ -1924,7 +1912,6 @@ __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsB
    const uint32_t per_part = (n_chunks + kOctetParts - 1) / kOctetParts;
    const uint32_t tid = threadIdx.x;
    const uint32_t lane = tid & 63;
-   const uint32_t wave = tid >> 6;
    const GlobalU32* light_ptr = toGlobal<GlobalU32>(col.light_ptr);
    const GlobalU32* group_ptr = toGlobal<GlobalU32>(col.group_ptr);
    const GlobalU32* heavy_ptr = toGlobal<GlobalU32>(col.heavy_ptr);
