@@ -150,8 +150,8 @@ def serve(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--rows", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -305,7 +305,8 @@ def main():
     for _ in range(max(args.warmup, 1)):
         et.batch(queries, arrays=True)
     # up to one batch per lane in flight: a step's host work (compiling and staging the next batch,
-    # collecting the previous one's results) overlaps the GPU work of the other
+    # collecting an earlier one's results) overlaps the GPU work, and the kernels of batches in
+    # flight share the GPU
     from collections import deque
 
     in_flight = max(1, eng.get_tuning("batch_lanes"))
