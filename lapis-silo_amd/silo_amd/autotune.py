@@ -192,8 +192,16 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
             engine.set_tuning("sparse_max_rows", thr)
             sw_qps[thr] = pipelined(table, queries, best_lanes, steps=steps)
         thr = max(sw_qps, key=sw_qps.get)
+        if thr > 0:
+            # refine between the probes' sizes around the best one
+            for f in (0.6, 0.8, 1.25, 1.6, 2.0):
+                cand = int(thr * f)
+                if cand not in sw_qps:
+                    engine.set_tuning("sparse_max_rows", cand)
+                    sw_qps[cand] = pipelined(table, queries, best_lanes, steps=steps)
+            thr = max(sw_qps, key=sw_qps.get)
         engine.set_tuning("sparse_max_rows", thr)
-        say(f"dense/sparse switch at that concurrency, queries/s: {[(k, round(v)) for k, v in sw_qps.items()]}; chose {thr}")
+        say(f"dense/sparse switch at that concurrency, queries/s: {[(k, round(v)) for k, v in sorted(sw_qps.items())]}; chose {thr}")
     job_qps = {}
     for job in sorted({max(1, size // 4), max(1, size // 2), size}):
         engine.set_tuning("batch_max_queries", job)
