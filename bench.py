@@ -272,6 +272,23 @@ def main():
         tuned = autotune.autotune(eng, et, probes, log=log)
         tuned.update(autotune.autotune_batch(eng, et, probes, log=log, cards=tuned["probe_cards"]))
     log(f"autotune {time.time() - t0:.1f}s")
+    # the plain path again with the autotuned single-query knobs (the sparse method below the
+    # single-query switch): the batched path is also compared with this
+    plain_tuned = None
+    if tuned is not None and plain is not None:
+        batch_switch = eng.get_tuning("sparse_max_rows")
+        eng.set_tuning("sparse_max_rows", tuned["sparse_max_rows"])
+        for q in queries:
+            run_plain_query(et, q)
+        t_pt = time.perf_counter()
+        for _ in range(args.steps):
+            pt_results = as_rows([run_plain_query(et, q) for q in queries])
+        pt_elapsed = time.perf_counter() - t_pt
+        eng.set_tuning("sparse_max_rows", batch_switch)
+        plain_tuned = {"queries_per_s": round(args.steps * len(queries) / pt_elapsed, 3),
+                       "sparse_max_rows": tuned["sparse_max_rows"],
+                       "identical_to_plain": pt_results == plain_results}
+        log(f"plain path with the tuned switch: {plain_tuned}")
     traffic_tuned = 0  # Mutations bytes by the method each query takes under the tuned switch
     for q in queries:
         if q["action"] == "mutations":
@@ -411,6 +428,8 @@ def main():
         "latency_note": "batched path: a query's latency is its batch's submit-to-results time",
         "plain": plain,
         "batched_vs_plain": round(b_qps / plain["queries_per_s"], 3) if plain else None,
+        "plain_tuned": plain_tuned,
+        "batched_vs_plain_tuned": round(b_qps / plain_tuned["queries_per_s"], 3) if plain_tuned else None,
         "batched_identical_to_plain": identical_paths,
         "tuning": tuned,
         "planner": planner,
