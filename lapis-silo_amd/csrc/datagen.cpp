@@ -543,31 +543,41 @@ int dg_translate(void* handle, uint32_t n_genes, const uint32_t* gene_start, con
       for (uint32_t i = 0; i < gl; ++i) {
          col.reference[i] = translate(ds->reference[gs + 3 * i], ds->reference[gs + 3 * i + 1], ds->reference[gs + 3 * i + 2]);
       }
-      struct RowAa {
-         uint32_t start = 0, end = 0;
-         std::vector<std::pair<uint32_t, uint32_t>> runs;
+      // Row r's residues in the gene, in time proportional to its mutations and missing runs.  A
+      // residue is missing when its codon is not fully inside the covered range [cs, ce), touches
+      // an internal N run, or translates to the missing symbol (partial deletion, ambiguous base).
+      // Leading and trailing missing residues bound the amino-acid covered range; the missing
+      // residues inside it form its runs; the other residues that differ from the gene's
+      // reference are its mutations.  Returns false for a row without a covered residue.
+      struct Scratch {
+         std::vector<std::pair<uint32_t, uint32_t>> miss;  // missing residue intervals
          std::vector<std::pair<uint32_t, uint8_t>> muts;
       };
-      std::vector<RowAa> rows(n);
-#pragma omp parallel for schedule(dynamic, 4096)
-      for (int64_t r = 0; r < static_cast<int64_t>(n); ++r) {
-         RowAa& out = rows[r];
-         if (ds->is_null[r] || ds->cov_start[r] >= ds->cov_end[r]) continue;
+      auto rowAa = [&](uint32_t r, Scratch& w, uint32_t& start, uint32_t& end) -> bool {
+         w.miss.clear();
+         w.muts.clear();
+         start = end = 0;
+         if (ds->is_null[r] || ds->cov_start[r] >= ds->cov_end[r]) return false;
          const uint32_t cs = ds->cov_start[r], ce = ds->cov_end[r];
-         // missing residues: codons not fully inside [cs, ce) or touching an internal N run
-         std::vector<uint8_t> missing(gl, 0);
-         for (uint32_t i = 0; i < gl; ++i) {
-            const uint32_t a = gs + 3 * i;
-            if (a < cs || a + 3 > ce) missing[i] = 1;
-         }
+         // residues whose codon lies inside [cs, ce): [lo, hi)
+         const uint32_t lo = cs <= gs ? 0u : std::min<uint32_t>(gl, (cs - gs + 2) / 3);
+         const uint32_t hi = ce < gs ? 0u : std::min<uint32_t>(gl, (ce - gs) / 3);
+         if (lo >= hi) return false;
          for (uint64_t k = ds->nrun_ptr[r]; k < ds->nrun_ptr[r + 1]; ++k) {
             const uint32_t a = ds->nrun_start[k], b = ds->nrun_end[k];
             if (b <= gs || a >= gs + 3 * gl) continue;
             const uint32_t i0 = a > gs ? (a - gs) / 3 : 0;
             const uint32_t i1 = std::min<uint32_t>(gl, (b - gs + 2) / 3);
-            for (uint32_t i = i0; i < i1; ++i) missing[i] = 1;
+            w.miss.emplace_back(i0, i1);
          }
-         // residues whose codon carries a nucleotide mutation
+         // the runs are disjoint and ascending in nucleotides, so both ends of their residue
+         // intervals are non-decreasing: the last interval starting at or before i is the one
+         // that can cover i
+         const size_t n_run_iv = w.miss.size();
+         auto inRun = [&](uint32_t i) {
+            auto it = std::upper_bound(w.miss.begin(), w.miss.begin() + n_run_iv, std::make_pair(i, UINT32_MAX));
+            return it != w.miss.begin() && std::prev(it)->second > i;
+         };
          const uint32_t* mp = ds->mut_pos.data() + ds->mut_ptr[r];
          const uint32_t* me = ds->mut_pos.data() + ds->mut_ptr[r + 1];
          const uint8_t* ms = ds->mut_sym.data() + ds->mut_ptr[r];
@@ -575,68 +585,93 @@ int dg_translate(void* handle, uint32_t n_genes, const uint32_t* gene_start, con
             const uint32_t* it = std::lower_bound(mp, me, pos);
             return (it != me && *it == pos) ? ms[it - mp] : ds->reference[pos];
          };
-         const uint32_t* it = std::lower_bound(mp, me, gs);
          uint32_t last = UINT32_MAX;
-         for (; it != me && *it < gs + 3 * gl; ++it) {
+         for (const uint32_t* it = std::lower_bound(mp, me, gs); it != me && *it < gs + 3 * gl; ++it) {
             const uint32_t i = (*it - gs) / 3;
-            if (i == last || missing[i]) continue;
+            if (i == last) continue;
             last = i;
+            if (i < lo || i >= hi || inRun(i)) continue;
             const uint32_t a = gs + 3 * i;
             const uint8_t aa = translate(nucAt(a), nucAt(a + 1), nucAt(a + 2));
             if (aa == kAaMissing) {
-               missing[i] = 1;
+               w.miss.emplace_back(i, i + 1);
             } else if (aa != col.reference[i]) {
-               out.muts.emplace_back(i, aa);
+               w.muts.emplace_back(i, aa);
             }
          }
-         // coverage: leading and trailing missing residues
-         uint32_t s0 = 0, e0 = gl;
-         while (s0 < gl && missing[s0]) ++s0;
-         while (e0 > s0 && missing[e0 - 1]) --e0;
-         if (s0 >= e0) continue;  // nothing covered
-         out.start = s0;
-         out.end = e0;
-         for (uint32_t i = s0; i < e0;) {
-            if (!missing[i]) {
-               ++i;
-               continue;
+         // clip to [lo, hi), merge overlapping or adjacent intervals
+         std::sort(w.miss.begin(), w.miss.end());
+         size_t m = 0;
+         for (size_t k = 0; k < w.miss.size(); ++k) {
+            const uint32_t a = std::max(w.miss[k].first, lo), b = std::min(w.miss[k].second, hi);
+            if (a >= b) continue;
+            if (m > 0 && a <= w.miss[m - 1].second) {
+               w.miss[m - 1].second = std::max(w.miss[m - 1].second, b);
+            } else {
+               w.miss[m++] = {a, b};
             }
-            uint32_t j = i;
-            while (j < e0 && missing[j]) ++j;
-            out.runs.emplace_back(i, j);
-            i = j;
          }
-         out.muts.erase(std::remove_if(out.muts.begin(), out.muts.end(),
-                                       [&](const auto& m) { return missing[m.first] != 0; }),
-                        out.muts.end());
-      }
-      col.cov_start.resize(n);
-      col.cov_end.resize(n);
+         w.miss.resize(m);
+         uint32_t s0 = lo, e0 = hi;
+         size_t first = 0, stop = m;
+         if (m > 0 && w.miss[0].first <= s0) s0 = w.miss[first++].second;
+         if (stop > first && w.miss[stop - 1].second >= e0) e0 = w.miss[--stop].first;
+         if (s0 >= e0) {
+            w.miss.clear();
+            w.muts.clear();
+            return false;
+         }
+         start = s0;
+         end = e0;
+         w.miss.erase(w.miss.begin() + stop, w.miss.end());
+         w.miss.erase(w.miss.begin(), w.miss.begin() + first);
+         return true;
+      };
+      col.cov_start.assign(n, 0);
+      col.cov_end.assign(n, 0);
       col.mut_ptr.assign(n + 1, 0);
       col.nrun_ptr.assign(n + 1, 0);
+#pragma omp parallel
+      {
+         Scratch w;
+#pragma omp for schedule(dynamic, 4096)
+         for (int64_t r = 0; r < static_cast<int64_t>(n); ++r) {
+            uint32_t st, en;
+            if (rowAa(static_cast<uint32_t>(r), w, st, en)) {
+               col.cov_start[r] = st;
+               col.cov_end[r] = en;
+               col.mut_ptr[r + 1] = w.muts.size();
+               col.nrun_ptr[r + 1] = w.miss.size();
+            }
+         }
+      }
       for (uint32_t r = 0; r < n; ++r) {
-         col.mut_ptr[r + 1] = col.mut_ptr[r] + rows[r].muts.size();
-         col.nrun_ptr[r + 1] = col.nrun_ptr[r] + rows[r].runs.size();
+         col.mut_ptr[r + 1] += col.mut_ptr[r];
+         col.nrun_ptr[r + 1] += col.nrun_ptr[r];
       }
       col.mut_pos.resize(col.mut_ptr[n]);
       col.mut_sym.resize(col.mut_ptr[n]);
       col.nrun_start.resize(col.nrun_ptr[n]);
       col.nrun_end.resize(col.nrun_ptr[n]);
-#pragma omp parallel for schedule(static)
-      for (int64_t r = 0; r < static_cast<int64_t>(n); ++r) {
-         col.cov_start[r] = rows[r].start;
-         col.cov_end[r] = rows[r].end;
-         uint64_t o = col.mut_ptr[r];
-         for (const auto& [i, a] : rows[r].muts) {
-            col.mut_pos[o] = i;
-            col.mut_sym[o] = a;
-            ++o;
-         }
-         o = col.nrun_ptr[r];
-         for (const auto& [a, b] : rows[r].runs) {
-            col.nrun_start[o] = a;
-            col.nrun_end[o] = b;
-            ++o;
+#pragma omp parallel
+      {
+         Scratch w;
+#pragma omp for schedule(dynamic, 4096)
+         for (int64_t r = 0; r < static_cast<int64_t>(n); ++r) {
+            uint32_t st, en;
+            if (!rowAa(static_cast<uint32_t>(r), w, st, en)) continue;
+            uint64_t o = col.mut_ptr[r];
+            for (const auto& [i, a] : w.muts) {
+               col.mut_pos[o] = i;
+               col.mut_sym[o] = a;
+               ++o;
+            }
+            o = col.nrun_ptr[r];
+            for (const auto& [a, b] : w.miss) {
+               col.nrun_start[o] = a;
+               col.nrun_end[o] = b;
+               ++o;
+            }
          }
       }
    }

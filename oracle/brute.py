@@ -49,7 +49,8 @@ def to_symbol(alphabet: int, ch: str) -> int:
 
 class BruteTable:
     def __init__(self, chunk_sizes, seq_columns=None, int_columns=None, dict_columns=None, bitmaps=None):
-        """seq_columns: {name: (alphabet, reference_str, [str | None per row])}
+        """seq_columns: {name: (alphabet, reference_str, rows)} with rows either [str | None per row]
+        or (symbol matrix [n_rows, L], null flags [n_rows]);
         int_columns: {name: (values, is_null)}; dict_columns: {name: (ids, is_null, names, parent)}"""
         self.chunk_sizes = [int(x) for x in chunk_sizes]
         self.n = sum(self.chunk_sizes)
@@ -59,6 +60,11 @@ class BruteTable:
         for name, (alphabet, ref, rows) in (seq_columns or {}).items():
             L = len(ref)
             missing = (15 if alphabet == 0 else 27)
+            if isinstance(rows, tuple):
+                null = np.asarray(rows[1]).astype(bool)
+                M = np.where(null[:, None], missing, np.asarray(rows[0], np.uint8)).astype(np.uint8)
+                self.seq[name] = (alphabet, np.array([to_symbol(alphabet, c) for c in ref], np.uint8), M, null)
+                continue
             M = np.full((self.n, L), missing, dtype=np.uint8)
             null = np.zeros(self.n, bool)
             for i, s in enumerate(rows):

@@ -103,3 +103,101 @@ def random_bitmaps(ts, seed: int, count: int = 3) -> list:
         k = int(rng.integers(0, max(len(ids) // 3, 1)))
         out.append(np.sort(rng.choice(ids, size=k, replace=False)).astype(np.uint32))
     return out
+
+
+def column_from_matrix(name: str, reference: np.ndarray, M: np.ndarray, null: np.ndarray, alphabet: int = 0):
+    """The diff form (aligned_sequence.cpp:18-122 with offset 0) of aligned rows given as a symbol
+    matrix M[row, position]: leading and trailing missing symbols bound the covered range, missing
+    symbols inside it become runs, other symbols that differ from the reference are mutations.
+    Null rows (null[i] != 0) get the empty range [0, 0).  Vectorised, for tables of 10^5 rows."""
+    missing = 15 if alphabet == 0 else 27
+    n, L = M.shape
+    isn = (M == missing) | null.astype(bool)[:, None]
+    covered_any = ~isn.all(axis=1)
+    start = np.where(covered_any, np.argmax(~isn, axis=1), 0).astype(np.uint32)
+    end = np.where(covered_any, L - np.argmax(~isn[:, ::-1], axis=1), 0).astype(np.uint32)
+    col = np.arange(L, dtype=np.uint32)[None, :]
+    inside = (col >= start[:, None]) & (col < end[:, None])
+    nmask = isn & inside
+    prev = np.zeros_like(nmask)
+    prev[:, 1:] = nmask[:, :-1]
+    nxt = np.zeros_like(nmask)
+    nxt[:, :-1] = nmask[:, 1:]
+    rs, cs = np.nonzero(nmask & ~prev)
+    re_, ce = np.nonzero(nmask & ~nxt)
+    assert np.array_equal(rs, re_)
+    nrun_ptr = np.zeros(n + 1, np.uint64)
+    nrun_ptr[1:] = np.cumsum(np.bincount(rs, minlength=n))
+    mm = inside & ~isn & (M != reference[None, :])
+    mr, mc = np.nonzero(mm)
+    mut_ptr = np.zeros(n + 1, np.uint64)
+    mut_ptr[1:] = np.cumsum(np.bincount(mr, minlength=n))
+    return table.SeqColumnData(name, alphabet, reference.astype(np.uint8), null.astype(np.uint8), start, end, mut_ptr,
+                               mc.astype(np.uint32), M[mr, mc].astype(np.uint8), nrun_ptr, cs.astype(np.uint32),
+                               (ce + 1).astype(np.uint32))
+
+
+def bitset_table(seed: int = 5):
+    """Three chunks (65,536 + 65,536 + 40,000 rows) x 96 positions whose vertical index holds bitset
+    containers (more than 4,096 scattered rows of a non-local-reference symbol in one chunk), arrays
+    just under 4,096 rows, run containers, a position whose local reference flips, internal missing
+    runs, ambiguity codes and null rows; a 3-value dictionary column whose value bitmaps are
+    bitsets; a date column.  Returns (spec, M, null) with M the symbol matrix."""
+    rng = np.random.default_rng(seed)
+    sizes = np.asarray([65536, 65536, 40000], np.uint32)
+    n, L = int(sizes.sum()), 96
+    ref = np.tile(np.asarray([1, 2, 3, 4], np.uint8), L // 4)  # ACGT...
+    M = np.repeat(ref[None, :], n, axis=0)
+    r = np.arange(n)
+    local = np.concatenate([np.arange(s) for s in sizes.tolist()])
+    A, C, G, T, DEL, R, N = 1, 2, 3, 4, 0, 5, 15
+    M[r % 8 == 3, 10] = T                               # 8,192 rows per full chunk, 8,192 runs: bitset
+    M[rng.random(n) < 0.3, 20] = C                      # ~19.7 k random rows per chunk: bitset
+    M[r % 16 == 0, 30] = A                              # 4,096 rows per full chunk less nulls: array
+    M[r % 16 == 1, 30] = T
+    M[(r % 16 == 2) | (r % 128 == 7), 31] = DEL         # 4,608 rows per full chunk less nulls: bitset
+    M[((local >= 1000) & (local < 9000)) | ((local >= 30000) & (local < 31000)), 40] = G  # runs
+    u = rng.random(n)
+    M[u < 0.42, 50] = A                                 # A is the majority: G (the reference) is stored
+    M[(u >= 0.42) & (u < 0.8), 50] = C
+    M[r % 50 == 1, 5] = R                               # ambiguity code
+    M[(r % 3 == 0) & (r % 13 != 0), 62] = C             # bitset next to missing runs
+    M[r % 13 == 0, 60:64] = N                           # internal missing runs
+    M[r % 97 == 5, 0:3] = N                             # leading missing symbols
+    M[r % 89 == 7, 94:96] = N                           # trailing missing symbols
+    for lo, hi in ((70, 82), (82, 94)):                 # two private substitutions per row
+        p = rng.integers(lo, hi, n)
+        M[r, p] = np.asarray([1, 2, 3, 4, 0], np.uint8)[(ref[p].astype(np.int64) + rng.integers(0, 4, n)) % 5]
+    null = (r % 211 == 17).astype(np.uint8)
+    M[null.astype(bool)] = N
+    spec = table.TableSpec(chunk_sizes=sizes)
+    spec.seq_columns.append(column_from_matrix("main", ref, M, null))
+    ids = rng.integers(0, 3, n).astype(np.uint32)
+    cnull = (rng.random(n) < 0.02).astype(np.uint8)
+    spec.dict_columns.append(table.DictColumnData("country", ids, cnull, ["c0", "c1", "c2"]))
+    dnull = (rng.random(n) < 0.03).astype(np.uint8)
+    spec.int_columns.append(table.IntColumnData("date", True, (18262 + rng.integers(0, 100, n)).astype(np.int32),
+                                                dnull))
+    return spec, M, null
+
+
+BITSET_FILTERS = ["(true)", "(sym_eq main 10 T)", "(sym_eq main 10 G)", "(sym_eq main 10 .)", "(has_mut main 20)",
+                  "(sym_eq main 50 C)", "(sym_eq main 50 A)", '(sym_in main 50 "CG")', "(not (sym_eq main 31 -))",
+                  "(sym_eq main 30 A)", "(or (sym_eq main 40 G) (sym_eq main 30 T))", "(has_mut main 5)",
+                  "(sym_eq main 61 N)", "(covered main 62)", "(sym_eq main 62 C)", '(str_eq country "c1")',
+                  '(and (str_eq country "c0") (has_mut main 62))', "(str_eq country null)",
+                  "(nof 2 0 (has_mut main 10) (has_mut main 20) (has_mut main 50) (sym_eq main 62 C))",
+                  "(nof 2 1 (sym_eq main 10 T) (not (sym_eq main 20 C)) (sym_eq main 31 -))",
+                  "(date_between date 18270 18300)", "(and (has_mut main 31) (not (has_mut main 10)))"]
+
+
+def brute_from_matrix(spec, M, null):
+    from oracle import brute
+
+    c = spec.seq_columns[0]
+    ref = "".join(NUC[x] for x in c.reference)
+    d = spec.dict_columns[0]
+    i = spec.int_columns[0]
+    return brute.BruteTable(spec.chunk_sizes, seq_columns={c.name: (0, ref, (M, null))},
+                            int_columns={i.name: (i.values, i.is_null)},
+                            dict_columns={d.name: (d.ids, d.is_null, d.names, None)})
