@@ -100,6 +100,11 @@ int rdb_bitmap_cardinality(const rdb_bitmap* bitmap, uint64_t* cardinality);
 /* Global row ids (chunk << 16 | row), ascending (CopyOnWriteBitmap::toRoaring, copy_on_write_bitmap.cpp:352-364). */
 int rdb_bitmap_rows(rdb_bitmap* bitmap, uint32_t* rows, uint64_t capacity, uint64_t* n);
 void rdb_bitmap_free(rdb_bitmap* bitmap);
+/* Operator::toString (filter/operators/operator.h:33) of the operator tree the filter compiles to
+ * (rewrite + compile, compute_filter.cpp:14-20, without evaluating it), e.g. "RangeSelection(...)"
+ * for a range on a sorted date column.  *length receives the text's length; the text (with a
+ * terminating zero) is written when capacity > length, else RDB_ENOMEM. */
+int rdb_filter_explain(rdb_table* table, const char* filter, char* out, uint64_t capacity, uint64_t* length);
 /* CountFilterNode: |filter| (count_filter_node.cpp:35-71). */
 int rdb_count(rdb_table* table, const char* filter, uint64_t* count);
 
@@ -131,6 +136,10 @@ int rdb_aggregate(rdb_table* table, const rdb_bitmap* filter, const char* dims, 
 int rdb_profile(rdb_ctx* ctx, int enable);
 int rdb_profile_read(rdb_ctx* ctx, double* mutations_ms, uint64_t* mutations_launches, double* filter_ms,
                      uint64_t* filter_launches);
+/* The batched path's dense counting kernel while profiling: its total time, launches (one per lane
+ * job with Mutations queries) and the dense Mutations queries those launches counted.  Profiled
+ * jobs run without graph replay, with HIP events on their lane's stream around the kernel. */
+int rdb_profile_read_scan(rdb_ctx* ctx, double* scan_ms, uint64_t* scan_launches, uint64_t* dense_queries);
 /* Algorithmic bytes of one Mutations evaluation under `filter` by the method the context's tuning
  * selects.  Dense: stored container bytes (array 2*card, bitset 8192, run 4*runs, plus an 8-byte
  * descriptor each) of the chunks the filter touches, 8192 per such chunk (filter tile), 8 per row
@@ -253,7 +262,10 @@ int rdb_set_planner(rdb_ctx* ctx, int mode, rdb_mlp* mlp);
  *   "batch_octet_groups" its persistent work-groups, 0 (default: one per CU) .. 4096; fewer leave
  *                      CUs to concurrent batches;
  *   "batch_row_capacity" Mutations rows per query kept by a lane job (more: the query runs again
- *                      alone), default 4096.
+ *                      alone), default 4096;
+ *   "batch_fail_job"   fault injection for tests: the n-th lane job enqueued from now on fails with
+ *                      RDB_ENOMEM after its inputs were staged, failing its rdb_batch_submit (0, the
+ *                      default: never).
  * Results never depend on them.  RDB_EINVAL for an unknown key or an out-of-range value. */
 int rdb_set_tuning(rdb_ctx* ctx, const char* key, int64_t value);
 int rdb_get_tuning(rdb_ctx* ctx, const char* key, int64_t* value);

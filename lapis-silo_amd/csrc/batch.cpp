@@ -189,6 +189,14 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column, bool 
       m.rows = lane.rows.as<MutationRowDev>();
       m.row_capacity = lane.row_capacity;
       m.n_rows = lane.n_rows.as<uint32_t>();
+      if (ctx.profiling) {
+         for (auto& e : lane.scan_events) {
+            if (e == nullptr) hipCheck(hipEventCreate(&e), "hipEventCreate");
+         }
+         m.scan_start = lane.scan_events[0];
+         m.scan_stop = lane.scan_events[1];
+         lane.job_timed = true;
+      }
       hipCheck(launchMutationsBatch(m, lane.stream), "batched Mutations kernels");
    } else {
       hipCheck(launchFill(lane.n_rows.as<uint32_t>(), 0, Q, lane.stream), "clear row counts");
@@ -221,7 +229,10 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column, bool 
 // Buffers of the octet shared scan (outside any graph capture): the slot sums of chunk parts 2 and 3
 // and one byte per row per eight queries.
 void ensureOctetBuffers(const Table& t, BatchLane& lane) {
-   if (t.ctx->batch_shared_scan != 2 || t.seq.empty() || lane.octet_masks.bytes() != 0) return;
+   if (t.seq.empty() || lane.octet_masks.bytes() != 0 ||
+       effectiveSharedScan(t.ctx->batch_shared_scan, lane.max_queries, t.n_chunks) != 2) {
+      return;
+   }
    const uint32_t Q = lane.max_queries;
    const MutScratch ms = mutScratch(t);
    lane.partial_c = DeviceBuffer(4ULL * (kOctetParts - 2) * Q * ms.slots);
@@ -232,7 +243,8 @@ void launchLane(Table& t, BatchLane& lane, uint32_t depth, int column, bool aggr
    Context& ctx = *t.ctx;
    ensureOctetBuffers(t, lane);
    const uint32_t bucket = depthBucket(depth, maxFilterDepth(ctx));
-   if (!ctx.batch_graphs) {
+   lane.job_timed = false;
+   if (!ctx.batch_graphs || ctx.profiling) {  // timed jobs run eagerly: events around their kernels
       enqueueKernels(t, lane, bucket, column, aggregates);
       return;
    }
@@ -317,19 +329,27 @@ void enqueueJob(Table& t, BatchResult& b, const std::vector<CompiledQuery>& cq, 
    header[1] = n_agg;
    header[2] = total_bins;
    header[3] = 0;
+   // the header and up to 1 MiB of packed records (the rest, if any, on retirement)
+   lane.job_first_bytes = std::min(lane.out_bytes, ol.rows + (size_t{1} << 20));
+   try {
+      hipCheck(hipMemcpyAsync(lane.params.as<void>(), hp, pl.bytes, hipMemcpyHostToDevice, lane.stream), "stage batch");
+      if (t.ctx->batch_fail_job > 0 && --t.ctx->batch_fail_job == 0) fail(RDB_ENOMEM, "injected lane-job failure");
+      launchLane(t, lane, depth, job.column, n_agg > 0);
+      hipCheck(hipMemcpyAsync(lane.host_out, lane.out.as<void>(), lane.job_first_bytes, hipMemcpyDeviceToHost, lane.stream),
+               "copy batch results");
+      hipCheck(hipEventRecord(lane.done, lane.stream), "hipEventRecord");
+   } catch (...) {
+      // whatever was enqueued finishes before the lane takes another job; the lane stays free
+      (void)hipStreamSynchronize(lane.stream);
+      throw;
+   }
+   // the lane holds the job only once all of its work is enqueued
    lane.busy = true;
    lane.owner = &b;
    lane.job_seq = ++t.job_counter;
    lane.job_queries = job.queries;
    lane.job_total_bins = total_bins;
-   // the header and up to 1 MiB of packed records (the rest, if any, on retirement)
-   lane.job_first_bytes = std::min(lane.out_bytes, ol.rows + (size_t{1} << 20));
    b.pending.push_back(&lane);
-   hipCheck(hipMemcpyAsync(lane.params.as<void>(), hp, pl.bytes, hipMemcpyHostToDevice, lane.stream), "stage batch");
-   launchLane(t, lane, depth, job.column, n_agg > 0);
-   hipCheck(hipMemcpyAsync(lane.host_out, lane.out.as<void>(), lane.job_first_bytes, hipMemcpyDeviceToHost, lane.stream),
-            "copy batch results");
-   hipCheck(hipEventRecord(lane.done, lane.stream), "hipEventRecord");
 }
 
 BatchLane& acquireLane(Table& t) {
@@ -360,9 +380,15 @@ BatchLane& acquireLane(Table& t) {
    return acquireLane(t);
 }
 
-// Copies a finished job's results into its batch and frees the lane.
+// Copies a finished job's results into its batch and frees the lane.  A job detached from its
+// batch (owner null) is dropped.
 void retire(BatchLane& lane) {
    if (!lane.busy) return;
+   if (lane.owner == nullptr) {
+      lane.busy = false;
+      lane.job_queries.clear();
+      return;
+   }
    Table& t = *lane.table;
    BatchResult& b = *lane.owner;
    const uint32_t Q = lane.max_queries;
@@ -370,6 +396,24 @@ void retire(BatchLane& lane) {
    auto* ho = static_cast<uint8_t*>(lane.host_out);
    const auto* header = reinterpret_cast<const uint32_t*>(ho + ol.header);
    const uint32_t n = static_cast<uint32_t>(lane.job_queries.size());
+   if (lane.job_timed) {
+      // the dense counting kernel's time and the number of dense queries it counted (Mutations
+      // queries with a non-empty filter above the dense/sparse switch)
+      float ms = 0;
+      hipCheck(hipEventElapsedTime(&ms, lane.scan_events[0], lane.scan_events[1]), "profile time");
+      Context& ctx = *t.ctx;
+      ctx.prof_ms[2] += ms;
+      ctx.prof_launches[2] += 1;
+      for (uint32_t s = 0; s < n; ++s) {
+         const BatchQueryResult& r = b.queries[lane.job_queries[s]];
+         const uint64_t card = static_cast<uint64_t>(header[4 * s + 2]) | (static_cast<uint64_t>(header[4 * s + 3]) << 32);
+         if (r.status == RDB_OK && r.action == RDB_ACTION_MUTATIONS && card > 0 &&
+             (ctx.sparse_max_rows < 0 || card > static_cast<uint64_t>(ctx.sparse_max_rows))) {
+            ++ctx.prof_dense_queries;
+         }
+      }
+      lane.job_timed = false;
+   }
    // rows beyond the first copy
    size_t kept = 0;
    for (uint32_t s = 0; s < n; ++s) kept += std::min(header[4 * s], lane.row_capacity);
@@ -438,6 +482,9 @@ void retire(BatchLane& lane) {
 BatchLane::~BatchLane() {
    if (stream) (void)hipStreamSynchronize(stream);
    for (auto& [k, g] : graphs) (void)hipGraphExecDestroy(g);
+   for (auto& e : scan_events) {
+      if (e) (void)hipEventDestroy(e);
+   }
    if (done) (void)hipEventDestroy(done);
    if (host_params) (void)hipHostFree(host_params);
    if (host_out) (void)hipHostFree(host_out);
@@ -498,7 +545,22 @@ void batchSubmit(Table& t, BatchResult& b) {
          cq[i].depth = 1;
       }
    }
-   submitCompiled(t, b, cq);
+   try {
+      submitCompiled(t, b, cq);
+   } catch (...) {
+      batchDetach(b);
+      throw;
+   }
+}
+
+void batchDetach(BatchResult& b) {
+   for (BatchLane* l : b.pending) {
+      // the lane's job finishes before the lane takes another one; its results are dropped
+      (void)hipEventSynchronize(l->done);
+      l->owner = nullptr;
+      retire(*l);
+   }
+   b.pending.clear();
 }
 
 void submitCompiled(Table& t, BatchResult& b, std::vector<CompiledQuery>& cq) {
@@ -554,8 +616,13 @@ std::vector<uint64_t> countPrograms(Table& t, const std::vector<query::Emitter>&
       cq[i].depth = std::max<uint32_t>(programs[i].max_sp, 1);
       if (cq[i].depth > maxFilterDepth(*t.ctx)) fail(RDB_EQUERY, "filter expression nests too deeply");
    }
-   submitCompiled(t, b, cq);
-   batchWait(b);
+   try {
+      submitCompiled(t, b, cq);
+      batchWait(b);
+   } catch (...) {
+      batchDetach(b);
+      throw;
+   }
    std::vector<uint64_t> out(programs.size());
    for (size_t i = 0; i < programs.size(); ++i) {
       if (b.queries[i].status != RDB_OK) fail(b.queries[i].status, b.queries[i].message);
@@ -589,7 +656,7 @@ size_t laneBytes(const Table& t, uint32_t max_queries, uint32_t row_capacity) {
    if (!t.seq.empty()) {
       const MutScratch ms = mutScratch(t);
       b += 4ULL * Q * (3 * ms.slots + ms.counts + ms.bnd) + 2ULL * Q * ms.pblocks * 3 * sizeof(int64_t);
-      if (t.ctx->batch_shared_scan == 2) {
+      if (effectiveSharedScan(t.ctx->batch_shared_scan, Q, t.n_chunks) == 2) {
          b += 4ULL * (kOctetParts - 2) * Q * ms.slots + static_cast<size_t>((Q + 7) / 8) * t.n_chunks * kChunkRows;
       }
    }
@@ -605,7 +672,11 @@ void batchWait(BatchResult& b) {
       const hipError_t err = hipEventSynchronize(ev);
       ctx.mu.lock();
       hipCheck(err, "batch wait");
-      if (lane->busy && lane->owner == &b) retire(*lane);
+      if (lane->busy && lane->owner == &b) {
+         retire(*lane);
+      } else {
+         b.pending.erase(std::remove(b.pending.begin(), b.pending.end(), lane), b.pending.end());
+      }
    }
 }
 
@@ -792,11 +863,8 @@ void rdb_batch_free(rdb_batch* batch) {
       try {
          rdb::batchWait(batch->b);
       } catch (...) {
-         // the lanes stay valid; their results are dropped with the batch
-         for (rdb::BatchLane* l : batch->b.pending) {
-            l->busy = false;
-            l->owner = nullptr;
-         }
+         // the lanes stay valid; their jobs finish and their results are dropped with the batch
+         rdb::batchDetach(batch->b);
       }
    }
    delete batch;

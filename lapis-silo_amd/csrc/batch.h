@@ -47,6 +47,9 @@ struct BatchLane {
    std::vector<uint32_t> job_queries;  // batch query index of each lane slot
    uint32_t job_total_bins = 0;
    size_t job_first_bytes = 0;
+   // timing of the job's dense counting kernel (profiling only)
+   hipEvent_t scan_events[2] = {nullptr, nullptr};
+   bool job_timed = false;
 
    BatchLane() = default;
    BatchLane(const BatchLane&) = delete;
@@ -80,6 +83,9 @@ struct BatchResult {
 void batchSubmit(Table& t, BatchResult& out);
 // Waits for and retires every pending job of `b`.
 void batchWait(BatchResult& b);
+// Waits for the pending jobs of `b` and frees their lanes without collecting results (after an
+// error, or before `b` goes away with jobs still pending).
+void batchDetach(BatchResult& b);
 // Retires the jobs of `b` that have finished, without waiting; true when none is left.
 bool batchPoll(BatchResult& b);
 // Device bytes of one lane (its Mutations scratch serves every sequence column).

@@ -232,6 +232,9 @@ int rdb_set_tuning(rdb_ctx* ctx, const char* key, int64_t value) {
       } else if (k == "batch_row_capacity") {
          if (value < 16 || value > (1 << 20)) fail(RDB_EINVAL, "batch_row_capacity must be in [16, 2^20]");
          ctx->c.batch_row_capacity = static_cast<uint32_t>(value);
+      } else if (k == "batch_fail_job") {
+         if (value < 0 || value > 1000000) fail(RDB_EINVAL, "batch_fail_job must be in [0, 10^6]");
+         ctx->c.batch_fail_job = static_cast<uint32_t>(value);
       } else {
          fail(RDB_EINVAL, "unknown tuning key '" + k + "'");
       }
@@ -267,6 +270,8 @@ int rdb_get_tuning(rdb_ctx* ctx, const char* key, int64_t* value) {
          *value = ctx->c.batch_octet_groups;
       } else if (k == "batch_row_capacity") {
          *value = ctx->c.batch_row_capacity;
+      } else if (k == "batch_fail_job") {
+         *value = ctx->c.batch_fail_job;
       } else {
          fail(RDB_EINVAL, "unknown tuning key '" + k + "'");
       }
@@ -449,6 +454,20 @@ void rdb_bitmap_free(rdb_bitmap* bitmap) {
    } else {
       delete bitmap;
    }
+}
+
+int rdb_filter_explain(rdb_table* table, const char* filter, char* out, uint64_t capacity, uint64_t* length) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireFinalized(t);
+      if (!filter || !length) fail(RDB_EINVAL, "null argument");
+      const std::string text = query::compileFilter(t, filter)->toString();
+      *length = text.size();
+      if (out == nullptr || capacity <= text.size()) fail(RDB_ENOMEM, "explain: output capacity too small");
+      std::memcpy(out, text.c_str(), text.size() + 1);
+   });
 }
 
 int rdb_count(rdb_table* table, const char* filter, uint64_t* count) {
@@ -689,8 +708,21 @@ int rdb_profile(rdb_ctx* ctx, int enable) {
       if (!ctx) fail(RDB_EINVAL, "null context");
       std::lock_guard<std::recursive_mutex> lock(ctx->c.mu);
       ctx->c.profiling = enable != 0;
-      ctx->c.prof_ms[0] = ctx->c.prof_ms[1] = 0;
-      ctx->c.prof_launches[0] = ctx->c.prof_launches[1] = 0;
+      for (int k = 0; k < 3; ++k) {
+         ctx->c.prof_ms[k] = 0;
+         ctx->c.prof_launches[k] = 0;
+      }
+      ctx->c.prof_dense_queries = 0;
+   });
+}
+
+int rdb_profile_read_scan(rdb_ctx* ctx, double* scan_ms, uint64_t* scan_launches, uint64_t* dense_queries) {
+   return guarded(ctx ? &ctx->c : nullptr, [&] {
+      if (!ctx) fail(RDB_EINVAL, "null context");
+      std::lock_guard<std::recursive_mutex> lock(ctx->c.mu);
+      if (scan_ms) *scan_ms = ctx->c.prof_ms[2];
+      if (scan_launches) *scan_launches = ctx->c.prof_launches[2];
+      if (dense_queries) *dense_queries = ctx->c.prof_dense_queries;
    });
 }
 

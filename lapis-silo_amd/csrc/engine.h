@@ -113,8 +113,11 @@ struct Context {
    // kernel timing with HIP events on `stream` (enabled by rdb_profile)
    bool profiling = false;
    hipEvent_t prof_events[4] = {};
-   double prof_ms[2] = {0, 0};     // [0] Mutations counting kernel, [1] filter kernel
-   uint64_t prof_launches[2] = {0, 0};
+   // [0] Mutations counting kernel, [1] filter kernel, [2] dense counting kernel of batch jobs (the
+   // jobs then run without graphs, with events on their lane's stream)
+   double prof_ms[3] = {0, 0, 0};
+   uint64_t prof_launches[3] = {0, 0, 0};
+   uint64_t prof_dense_queries = 0;  // dense Mutations queries of the timed batch jobs
    void profileAccumulate(int which);  // waits for the pair's stop event and adds its time
    // tuning knobs (rdb_set_tuning)
    long long sparse_max_rows = -1;     // filters with at most this many rows take the sparse path
@@ -129,6 +132,9 @@ struct Context {
    uint32_t batch_shared_scan = 0;     // dense Mutations of a job: 1 four, 2 eight queries per pass over the index
    uint32_t batch_octet_threads = 1024;  // work-group size of the eight-query scan (512 or 1024)
    uint32_t batch_octet_groups = 0;      // its persistent work-groups (0: one per CU)
+   // fault injection for tests: the n-th lane job enqueued from now on fails with RDB_ENOMEM after
+   // its inputs were staged (0: never)
+   uint32_t batch_fail_job = 0;
    // planner of the batched path: 0 reference order, 1 independence estimates, 2 learned estimator
    int plan_mode = 0;
    struct Mlp* planner = nullptr;

@@ -18,6 +18,7 @@ struct GroupDims {
 
 hipError_t launchFill(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
 
+
 size_t filterLdsBytes(uint32_t depth);
 // Lets the filter kernel allocate up to max_lds_bytes of dynamic LDS (deep operator trees).
 hipError_t configureFilterKernel(size_t max_lds_bytes);
@@ -78,6 +79,21 @@ hipError_t launchMutations(const MutationsLaunch& m, hipStream_t stream);
 // capacity).  Every grid is sized for max_queries and reads the actual count from the device, so
 // the launch sequence can be captured once and replayed for any batch of up to max_queries.
 constexpr uint32_t kOctetParts = 4;  // chunk parts of the octet shared scan (slot-sum arrays)
+constexpr uint32_t kOctetMaxPartChunks = 1024;   // chunks of one part of the octet scan
+constexpr uint32_t kSharedMaxPartChunks = 4096;  // chunks of one half of the four-query scan
+
+// The dense scan a batch job runs: the one asked for (0 one pass per query, 1 four queries per
+// pass, 2 eight per pass), or the next smaller one where the table is too large for it.  The lane
+// allocates the eight-query buffers, and its footprint counts them, only when this returns 2.
+inline uint32_t effectiveSharedScan(uint32_t requested, uint32_t max_queries, uint32_t n_chunks) {
+   uint32_t mode = requested > 2 ? 2 : requested;
+   const uint64_t max_octets = (static_cast<uint64_t>(max_queries) + 7) / 8;
+   if (mode == 2 && (max_octets * n_chunks > 65535 || (n_chunks + kOctetParts - 1) / kOctetParts > kOctetMaxPartChunks)) {
+      mode = 1;
+   }
+   if (mode == 1 && (n_chunks + 1) / 2 > kSharedMaxPartChunks) mode = 0;
+   return mode;
+}
 
 struct MutationsBatch {
    const SeqColumnView* col_dev = nullptr;
@@ -112,6 +128,8 @@ struct MutationsBatch {
    MutationRowDev* rows = nullptr;
    uint32_t row_capacity = 0;
    uint32_t* n_rows = nullptr;                 // [max_queries]
+   hipEvent_t scan_start = nullptr;            // recorded around the dense counting kernel when set
+   hipEvent_t scan_stop = nullptr;
 };
 
 hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream);

@@ -1483,7 +1483,6 @@ __global__ __launch_bounds__(kMutThreads) __attribute__((amdgpu_waves_per_eu(kCo
 // loaded while the current tile is counted.
 constexpr uint32_t kSharedThreads = 512;
 constexpr uint32_t kSharedFlushChunks = 15;
-constexpr uint32_t kSharedMaxPartChunks = 4096;  // chunks of one half (the launch checks it)
 
 // popcount of bits a..b (inclusive) of query k's words in an interleaved tile
 __device__ inline uint32_t rangePopcountQ(const uint32_t* fq, uint32_t k, uint32_t a, uint32_t b) {
@@ -1831,7 +1830,6 @@ __global__ __launch_bounds__(256) void foldSharedPartsKernel(MutationsBatch m, u
 // the item starts.
 constexpr uint32_t kOctetMaskThreads = 256;
 constexpr uint32_t kOctetDescCap = 128;    // containers whose sums are gathered in LDS at a time
-constexpr uint32_t kOctetMaxPartChunks = 1024;  // chunks of one part (the launch checks it)
 
 __device__ inline uint64_t transpose8x8(uint64_t x) {
    // bit 8r + c <-> bit 8c + r
@@ -2781,15 +2779,14 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
    hipLaunchKernelGGL(batchPlanKernel, dim3(1), dim3(1024), 0, stream, m);
    if ((err = hipGetLastError()) != hipSuccess) return err;
    // the dense scan asked for, or the next smaller one where a table is too large for it (the
-   // knob never changes results and never fails a batch)
+   // knob never changes results); the lane holds the buffers of the scan this chooses
    const uint32_t max_octets = (m.max_queries + 7) / 8;
-   uint32_t mode = m.heavy_sums == nullptr ? 0u : m.shared_mode;
-   if (mode == 2 && (static_cast<uint64_t>(max_octets) * m.n_chunks > 65535 ||
-                     (m.n_chunks + kOctetParts - 1) / kOctetParts > kOctetMaxPartChunks || m.octet_masks == nullptr ||
-                     m.partial_c == nullptr || m.partial_b == nullptr)) {
-      mode = 1;
+   uint32_t mode = m.heavy_sums == nullptr ? 0u : effectiveSharedScan(m.shared_mode, m.max_queries, m.n_chunks);
+   if (mode == 2 && (m.octet_masks == nullptr || m.partial_c == nullptr || m.partial_b == nullptr)) {
+      return hipErrorInvalidValue;
    }
-   if (mode == 1 && ((m.n_chunks + 1) / 2 > kSharedMaxPartChunks || m.partial_b == nullptr)) mode = 0;
+   if (mode == 1 && m.partial_b == nullptr) return hipErrorInvalidValue;
+   if (m.scan_start && (err = hipEventRecord(m.scan_start, stream)) != hipSuccess) return err;
    if (mode == 2) {
       // shared scan, eight dense queries per pass: row bytes, then the persistent counting kernel
       if (m.n_chunks > 0) {
@@ -2812,6 +2809,7 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
          }
       }
       if ((err = hipGetLastError()) != hipSuccess) return err;
+      if (m.scan_stop && (err = hipEventRecord(m.scan_stop, stream)) != hipSuccess) return err;
       const uint32_t sblocks = (h.n_slots + 255) / 256;
       hipLaunchKernelGGL(foldSharedPartsKernel, dim3(sblocks == 0 ? 1 : (sblocks < 64 ? sblocks : 64), m.max_queries),
                          dim3(256), 0, stream, m, h.n_slots, kOctetParts);
@@ -2825,6 +2823,7 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
          hipLaunchKernelGGL(mutationsCountSharedKernel<2>, grid, dim3(kSharedThreads), 0, stream, m);
       }
       if ((err = hipGetLastError()) != hipSuccess) return err;
+      if (m.scan_stop && (err = hipEventRecord(m.scan_stop, stream)) != hipSuccess) return err;
       const uint32_t sblocks = (h.n_slots + 255) / 256;
       hipLaunchKernelGGL(foldSharedPartsKernel, dim3(sblocks == 0 ? 1 : (sblocks < 64 ? sblocks : 64), m.max_queries),
                          dim3(256), 0, stream, m, h.n_slots, 2u);
@@ -2844,6 +2843,8 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
             hipLaunchKernelGGL((mutationsCountBatchKernel<8, false>), count_grid, dim3(kMutThreads), 0, stream, m);
          }
       }
+      if ((err = hipGetLastError()) != hipSuccess) return err;
+      if (m.scan_stop && (err = hipEventRecord(m.scan_stop, stream)) != hipSuccess) return err;
    }
    if ((err = hipGetLastError()) != hipSuccess) return err;
    const uint32_t n_pblocks = (h.genome_length + kPosBlock - 1) / kPosBlock;

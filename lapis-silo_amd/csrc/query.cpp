@@ -43,14 +43,7 @@ const char* cmpText(Comparator c) {
 }  // namespace
 
 // ---- program execution -------------------------------------------------------------------------
-DeviceBitmap runProgram(Table& t, const Emitter& program) {
-   Context& ctx = *t.ctx;
-   if (program.sp != 1) fail(RDB_ECOMPILE, "internal: filter program leaves " + std::to_string(program.sp) + " results");
-   const uint32_t depth = std::max<uint32_t>(program.max_sp, 1);
-   if (depth > maxFilterDepth(ctx)) fail(RDB_EQUERY, "filter expression nests too deeply");
-   DeviceBitmap out(&t);
-   hipCheck(launchFill(reinterpret_cast<uint32_t*>(out.card_dev), 0, (64 + 4 * static_cast<size_t>(t.n_chunks)) / 4, ctx.stream),
-            "clear cardinalities");
+std::vector<uint8_t> programBlob(const Emitter& program) {
    std::vector<uint8_t> blob(sizeof(ProgramDesc) + program.code.size() * sizeof(Instr) +
                              (program.ranges.size() + program.prefetch.size()) * 4 + 16);
    ProgramDesc pd{0, static_cast<uint32_t>(program.code.size()), 0, 0, static_cast<uint32_t>(program.ranges.size()),
@@ -60,16 +53,47 @@ DeviceBitmap runProgram(Table& t, const Emitter& program) {
    std::memcpy(blob.data() + sizeof(pd) + program.code.size() * sizeof(Instr), program.ranges.data(), program.ranges.size() * 4);
    std::memcpy(blob.data() + sizeof(pd) + program.code.size() * sizeof(Instr) + program.ranges.size() * 4,
                program.prefetch.data(), program.prefetch.size() * 4);
-   auto* dev = static_cast<uint8_t*>(ctx.stage(blob.data(), blob.size()));
+   return blob;
+}
+
+uint32_t programDepth(const Context& ctx, const Emitter& program) {
+   if (program.sp != 1) fail(RDB_ECOMPILE, "internal: filter program leaves " + std::to_string(program.sp) + " results");
+   const uint32_t depth = std::max<uint32_t>(program.max_sp, 1);
+   if (depth > maxFilterDepth(ctx)) fail(RDB_EQUERY, "filter expression nests too deeply");
+   return depth;
+}
+
+void launchProgram(Table& t, const Emitter& program, const uint8_t* dev_blob, uint64_t* words, unsigned long long* card,
+                   uint32_t* chunk_cards) {
+   Context& ctx = *t.ctx;
+   const uint32_t depth = programDepth(ctx, program);
+   if (reinterpret_cast<uint8_t*>(chunk_cards) == reinterpret_cast<uint8_t*>(card) + 64) {
+      // the cardinality (64 bytes) directly followed by the per-chunk cardinalities: one clear
+      hipCheck(launchFill(reinterpret_cast<uint32_t*>(card), 0, (64 + 4 * static_cast<size_t>(t.n_chunks)) / 4, ctx.stream),
+               "clear cardinalities");
+   } else {
+      hipCheck(launchFill(reinterpret_cast<uint32_t*>(card), 0, 64 / 4, ctx.stream), "clear cardinality");
+      hipCheck(launchFill(chunk_cards, 0, t.n_chunks, ctx.stream), "clear chunk cardinalities");
+   }
    if (ctx.profiling) hipCheck(hipEventRecord(ctx.prof_events[2], ctx.stream), "hipEventRecord");
-   hipCheck(launchFilter(t.view, reinterpret_cast<const ProgramDesc*>(dev), reinterpret_cast<const Instr*>(dev + sizeof(pd)),
-                         reinterpret_cast<const uint32_t*>(dev + sizeof(pd) + program.code.size() * sizeof(Instr)), 1, depth,
-                         out.words, out.card_dev, out.chunk_cards, 0, ctx.stream),
+   hipCheck(launchFilter(t.view, reinterpret_cast<const ProgramDesc*>(dev_blob),
+                         reinterpret_cast<const Instr*>(dev_blob + sizeof(ProgramDesc)),
+                         reinterpret_cast<const uint32_t*>(dev_blob + sizeof(ProgramDesc) + program.code.size() * sizeof(Instr)),
+                         1, depth, words, card, chunk_cards, 0, ctx.stream),
             "filter kernel");
    if (ctx.profiling) {
       hipCheck(hipEventRecord(ctx.prof_events[3], ctx.stream), "hipEventRecord");
       ctx.profileAccumulate(1);
    }
+}
+
+DeviceBitmap runProgram(Table& t, const Emitter& program) {
+   Context& ctx = *t.ctx;
+   programDepth(ctx, program);
+   DeviceBitmap out(&t);
+   const std::vector<uint8_t> blob = programBlob(program);
+   auto* dev = static_cast<uint8_t*>(ctx.stage(blob.data(), blob.size()));
+   launchProgram(t, program, dev, out.words, out.card_dev, out.chunk_cards);
    return out;
 }
 
@@ -421,6 +445,8 @@ ExpressionPtr Expression::shallowCopy() const {
    e->to = to;
    e->int_value = int_value;
    e->bitmap_id = bitmap_id;
+   e->row_ranges = row_ranges;
+   e->predicates = predicates;
    return e;
 }
 
@@ -583,6 +609,38 @@ class Parser {
       } else if (op == "op_complement") {
          e = std::make_unique<Expression>(K::OP_COMPLEMENT);
          e->children.push_back(expr());
+      } else if (op == "op_range") {
+         // RangeSelection over global row-id ranges: (op_range start end start end ...)
+         e = std::make_unique<Expression>(K::OP_RANGES);
+         while (!at(')')) e->row_ranges.push_back(position());
+         if (e->row_ranges.size() % 2 != 0) bad("op_range takes start / end pairs");
+      } else if (op == "op_select") {
+         // Selection with an optional child and (cmp COLUMN OP VALUE) predicates, OP one of
+         // eq lt gt le ge ne
+         e = std::make_unique<Expression>(K::OP_SELECT);
+         while (!at(')')) {
+            const size_t save = i_;
+            expect('(');
+            const auto [quoted, head] = atom();
+            if (!quoted && head == "cmp") {
+               Expression::ValuePredicate p;
+               p.column = atom().second;
+               const std::string c = atom().second;
+               static const char* const kCmp[] = {"eq", "lt", "gt", "le", "ge", "ne"};
+               p.comparator = -1;
+               for (int k = 0; k < 6; ++k) {
+                  if (c == kCmp[k]) p.comparator = k;
+               }
+               if (p.comparator < 0) bad("unknown comparator '" + c + "'");
+               p.value = integer();
+               expect(')');
+               e->predicates.push_back(std::move(p));
+            } else {
+               i_ = save;
+               if (!e->children.empty()) bad("op_select takes at most one child");
+               e->children.push_back(expr());
+            }
+         }
       } else {
          bad("unknown expression '" + op + "'");
       }
@@ -605,7 +663,9 @@ ExpressionPtr rewrite(const Table& t, const Expression& e) {
       case K::OP_THRESHOLD:
       case K::OP_INTERSECTION:
       case K::OP_UNION:
-      case K::OP_COMPLEMENT: {
+      case K::OP_COMPLEMENT:
+      case K::OP_RANGES:
+      case K::OP_SELECT: {
          // and.cpp:139-147, nof.cpp:249-257 (mode NONE keeps an exact NOf), negation.cpp
          auto r = e.shallowCopy();
          for (const auto& c : e.children) r->children.push_back(rewrite(t, *c));
@@ -1038,6 +1098,38 @@ OperatorPtr compile(Table& t, const Expression& e) {
       }
       case K::OP_COMPLEMENT:
          return std::make_unique<Complement>(tp, compile(t, *e.children[0]));
+      case K::OP_RANGES: {
+         // range_selection.cpp:54-60: every end is a row of the layout or the end of the last chunk
+         std::vector<RangeSelection::Range> ranges;
+         const uint64_t layout_end = static_cast<uint64_t>(t.n_chunks) << 16;
+         auto valid = [&](uint32_t id) {
+            const uint32_t c = id >> 16;
+            return (c < t.n_chunks && (id & 0xFFFFu) < t.chunk_sizes[c]) || id == layout_end;
+         };
+         for (size_t k = 0; k < e.row_ranges.size(); k += 2) {
+            const uint32_t a = e.row_ranges[k], b = e.row_ranges[k + 1];
+            if (!valid(a) || !valid(b) || a > b) fail(RDB_ECOMPILE, "RangeSelection: range outside the row layout");
+            ranges.push_back({a, b});
+         }
+         return std::make_unique<RangeSelection>(tp, std::move(ranges));
+      }
+      case K::OP_SELECT: {
+         // selection.h:24-166: Selection(child?, predicates); at least one predicate
+         if (e.predicates.empty()) fail(RDB_ECOMPILE, "Selection needs at least one predicate");
+         PredicateVector p;
+         for (const auto& vp : e.predicates) {
+            const int idx = t.intIndex(vp.column);
+            if (idx < 0) fail(RDB_EQUERY, "The column '" + vp.column + "' is not of type int or date");
+            if (vp.value < std::numeric_limits<int32_t>::min() || vp.value > std::numeric_limits<int32_t>::max()) {
+               fail(RDB_EQUERY, "Value out of int32 range");
+            }
+            p.push_back(std::make_unique<CompareToValueSelection>(tp, idx, static_cast<Comparator>(vp.comparator),
+                                                                  static_cast<int32_t>(vp.value), false));
+         }
+         std::optional<OperatorPtr> child;
+         if (!e.children.empty()) child = compile(t, *e.children[0]);
+         return std::make_unique<Selection>(tp, std::move(child), std::move(p));
+      }
       default:
          fail(RDB_ECOMPILE, "expression must be rewritten before compilation");
    }

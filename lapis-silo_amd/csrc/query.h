@@ -72,6 +72,14 @@ class Emitter {
 
 // Runs one emitted program over the table.
 DeviceBitmap runProgram(Table& table, const Emitter& program);
+// The staged form of one program: its ProgramDesc, instructions, ranges and prefetch codes.
+std::vector<uint8_t> programBlob(const Emitter& program);
+// The operand-stack depth the program needs (errors when it leaves a wrong stack or nests too deep).
+uint32_t programDepth(const Context& ctx, const Emitter& program);
+// Enqueues the filter kernel for a program staged on the device in programBlob's layout; clears
+// and writes the row words, the cardinality (64 bytes) and the per-chunk cardinalities.
+void launchProgram(Table& table, const Emitter& program, const uint8_t* dev_blob, uint64_t* words,
+                   unsigned long long* card, uint32_t* chunk_cards);
 
 // ---- predicates (filter/operators/selection.h:24-166, is_in_covered_region.h) -------------------
 enum class Comparator { EQUALS, LESS, HIGHER, LESS_OR_EQUALS, HIGHER_OR_EQUALS, NOT_EQUALS };
@@ -271,7 +279,13 @@ struct Expression {
       DATE_BETWEEN, INT_BETWEEN, INT_EQUALS, BITMAP, COVERED,
       // operator-level forms (the reference's operator tests build operators directly with their
       // constructor rules)
-      OP_THRESHOLD, OP_INTERSECTION, OP_UNION, OP_COMPLEMENT
+      OP_THRESHOLD, OP_INTERSECTION, OP_UNION, OP_COMPLEMENT, OP_RANGES, OP_SELECT
+   };
+   // OP_SELECT: a CompareToValueSelection on an int/date column
+   struct ValuePredicate {
+      std::string column;
+      int comparator;  // Comparator
+      int64_t value;
    };
    Kind kind;
    std::vector<ExpressionPtr> children;
@@ -287,6 +301,8 @@ struct Expression {
    std::optional<int64_t> from, to;
    int64_t int_value = 0;
    uint32_t bitmap_id = 0;
+   std::vector<uint32_t> row_ranges;  // OP_RANGES: global [start, end) pairs
+   std::vector<ValuePredicate> predicates;
 
    explicit Expression(Kind k) : kind(k) {}
    ExpressionPtr shallowCopy() const;

@@ -103,13 +103,16 @@ def autotune(engine, table, probes: list, column: str = "main", reps: int = 5, l
 
 
 def autotune_batch(engine, table, probes: list, column: str = "main", size: int = 64, reps: int = 5, log=None,
-                   cards: list | None = None) -> dict:
+                   cards: list | None = None, margin: float = 0.02) -> dict:
     """Chooses the batched dense kernel ("batch_shared_scan": one pass over the index per query, one
     per four queries or one per eight, the last with work-groups of 512 or 1024 threads,
     "batch_octet_threads") by timing a batch of `size` Mutations queries over the probe filters each
     way, then (given the probes' cardinalities) the dense/sparse switch for batches among those
     cardinalities, then the number of batches in flight ("batch_lanes") and the dense kernel again at
-    that concurrency; the results must agree throughout.  The knobs are applied."""
+    that concurrency, the switch and the job size.  Every knob value tried returns the per-query
+    kernel's results (checked on the last batch of each timing, RuntimeError otherwise); a
+    candidate replaces the current choice only when it is faster by more than `margin` (the
+    run-to-run spread of a pipelined timing).  The knobs are applied."""
     say = log or (lambda *_: None)
     queries = [{"filter": probes[i % len(probes)], "action": "mutations", "column": column, "min_proportion": 0.05}
                for i in range(size)]
@@ -156,6 +159,19 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
         thr = min(sw, key=sw.get)
         engine.set_tuning("sparse_max_rows", thr)
         say(f"batch dense/sparse switch, ms: {[(k, round(v * 1e3, 3)) for k, v in sw.items()]}; chose {thr}")
+    def timed(lanes, steps):
+        qps, rows = pipelined(table, queries, lanes, steps=steps)
+        if rows != ref_rows:
+            raise RuntimeError("pipelined batch results differ from the per-query kernel's")
+        return qps
+
+    def pick(scores: dict, current):
+        """The fastest candidate, unless it beats `current` by no more than the margin."""
+        best_key = max(scores, key=scores.get)
+        if current in scores and scores[best_key] <= scores[current] * (1.0 + margin):
+            return current
+        return best_key
+
     # lanes in flight: concurrent batches overlap one batch's latency-bound kernels with another's
     # (candidates whose buffers fit half the free device memory), then the dense scan again at
     # that concurrency
@@ -166,8 +182,12 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
     lanes_qps = {}
     for lanes in sorted({min(n, fit) for n in (1, 2, 4, 6, 8)}):
         engine.set_tuning("batch_lanes", lanes)
-        lanes_qps[lanes] = pipelined(table, queries, lanes, steps=max(32, 8 * lanes))
-    best_lanes = max(lanes_qps, key=lanes_qps.get)
+        lanes_qps[lanes] = timed(lanes, max(32, 8 * lanes))
+    # fewer lanes (less memory, shorter queues) unless more are faster beyond the noise
+    best_lanes = min(lanes_qps)
+    for lanes in sorted(lanes_qps):
+        if lanes_qps[lanes] > lanes_qps[best_lanes] * (1.0 + margin):
+            best_lanes = lanes
     engine.set_tuning("batch_lanes", best_lanes)
     # (scan, work-group size, persistent work-groups of the eight-query scan: 0 = one per CU; fewer
     # leave CUs to the other batches in flight)
@@ -176,8 +196,8 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
         engine.set_tuning("batch_shared_scan", mode[0])
         engine.set_tuning("batch_octet_threads", mode[1])
         engine.set_tuning("batch_octet_groups", mode[2])
-        mode_qps[mode] = pipelined(table, queries, best_lanes, steps=max(32, 8 * best_lanes))
-    best = max(mode_qps, key=mode_qps.get)
+        mode_qps[mode] = timed(best_lanes, max(32, 8 * best_lanes))
+    best = pick(mode_qps, (best[0], best[1], 0))
     engine.set_tuning("batch_shared_scan", best[0])
     engine.set_tuning("batch_octet_threads", best[1])
     engine.set_tuning("batch_octet_groups", best[2])
@@ -188,25 +208,27 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
     # size / job jobs) at that concurrency
     if cards is not None:
         sw_qps = {}
-        for thr in sorted({-1, engine.get_tuning("sparse_max_rows"), *[int(c) for c in cards if c < 1 << 20]}):
+        current = engine.get_tuning("sparse_max_rows")
+        for thr in sorted({-1, current, *[int(c) for c in cards if c < 1 << 20]}):
             engine.set_tuning("sparse_max_rows", thr)
-            sw_qps[thr] = pipelined(table, queries, best_lanes, steps=steps)
-        thr = max(sw_qps, key=sw_qps.get)
+            sw_qps[thr] = timed(best_lanes, steps)
+        thr = pick(sw_qps, current)
         if thr > 0:
             # refine between the probes' sizes around the best one
+            probe_best = thr
             for f in (0.6, 0.8, 1.25, 1.6, 2.0):
                 cand = int(thr * f)
                 if cand not in sw_qps:
                     engine.set_tuning("sparse_max_rows", cand)
-                    sw_qps[cand] = pipelined(table, queries, best_lanes, steps=steps)
-            thr = max(sw_qps, key=sw_qps.get)
+                    sw_qps[cand] = timed(best_lanes, steps)
+            thr = pick(sw_qps, probe_best)
         engine.set_tuning("sparse_max_rows", thr)
         say(f"dense/sparse switch at that concurrency, queries/s: {[(k, round(v)) for k, v in sorted(sw_qps.items())]}; chose {thr}")
     job_qps = {}
     for job in sorted({max(1, size // 4), max(1, size // 2), size}):
         engine.set_tuning("batch_max_queries", job)
-        job_qps[job] = pipelined(table, queries, best_lanes, steps=steps)
-    best_job = max(job_qps, key=job_qps.get)
+        job_qps[job] = timed(best_lanes, steps)
+    best_job = pick(job_qps, size)
     say(f"queries per lane job, queries/s: {[(k, round(v)) for k, v in job_qps.items()]}; chose {best_job}")
     engine.set_tuning("batch_max_queries", saved)
     out = {"batch_shared_scan": best[0], "batch_octet_threads": best[1], "batch_octet_groups": best[2],
@@ -219,14 +241,15 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
     return out
 
 
-def pipelined(table, queries: list, in_flight: int, steps: int, reps: int = 2) -> float:
+def pipelined(table, queries: list, in_flight: int, steps: int, reps: int = 2) -> tuple:
     """Queries/s of `steps` batches of `queries` with up to `in_flight` of them submitted ahead of
     the collection of their results (columnar), after one warm-up batch; the best of `reps` runs
-    (host noise only ever slows a run down)."""
+    (host noise only ever slows a run down).  Returns (queries/s, the last batch's results as rows)."""
     from collections import deque
 
     table.batch(queries, arrays=True)
     best = 0.0
+    last = None
     for _ in range(reps):
         pend = deque()
         t = time.perf_counter()
@@ -234,14 +257,14 @@ def pipelined(table, queries: list, in_flight: int, steps: int, reps: int = 2) -
             pend.append(table.batch_submit(queries))
             if len(pend) == in_flight:
                 b = pend.popleft()
-                b.results(arrays=True)
+                last = b.results(arrays=True)
                 b.free()
         while pend:
             b = pend.popleft()
-            b.results(arrays=True)
+            last = b.results(arrays=True)
             b.free()
         best = max(best, steps * len(queries) / (time.perf_counter() - t))
-    return best
+    return best, [r.rows() if hasattr(r, "rows") else r for r in last]
 
 
 def probe_filters(ds, n: int = 8) -> list:

@@ -44,6 +44,7 @@ SIGNATURES = [
     ("rdb_aggregate", _int, [_vp, _vp, _cp, _vp, _vp, _u64, _vp]),
     ("rdb_profile", _int, [_vp, _int]),
     ("rdb_profile_read", _int, [_vp, _vp, _vp, _vp, _vp]),
+    ("rdb_profile_read_scan", _int, [_vp, _vp, _vp, _vp]),
     ("rdb_mutations_traffic", _int, [_vp, _vp, _cp, _vp]),
     ("rdb_batch_submit", _int, [_vp, _vp, _u32, ctypes.POINTER(_vp)]),
     ("rdb_batch_wait", _int, [_vp]),
@@ -67,6 +68,7 @@ SIGNATURES = [
     ("rdb_plan_samples", _int, [_vp, _vp, _u32, _vp, _vp, _vp, _u64, _vp]),
     ("rdb_plan_estimates", _int, [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp, _u64, _vp]),
     ("rdb_set_planner", _int, [_vp, _int, _vp]),
+    ("rdb_filter_explain", _int, [_vp, _cp, _vp, _u64, _vp]),
     ("rdb_set_tuning", _int, [_vp, _cp, ctypes.c_int64]),
     ("rdb_get_tuning", _int, [_vp, _cp, _vp]),
     ("rdb_diff_sequence", _int, [_int, _cp, _u32, _cp, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _u64, _vp]),
@@ -151,8 +153,11 @@ class Engine:
         mm, mn, fm, fn = _f64(), _u64(), _f64(), _u64()
         _check(lib().rdb_profile_read(self._h, ctypes.byref(mm), ctypes.byref(mn), ctypes.byref(fm), ctypes.byref(fn)),
                self._h)
+        sm, sn, dq = _f64(), _u64(), _u64()
+        _check(lib().rdb_profile_read_scan(self._h, ctypes.byref(sm), ctypes.byref(sn), ctypes.byref(dq)), self._h)
         return {"mutations_ms": mm.value, "mutations_launches": mn.value, "filter_ms": fm.value,
-                "filter_launches": fn.value}
+                "filter_launches": fn.value, "scan_ms": sm.value, "scan_launches": sn.value,
+                "scan_dense_queries": dq.value}
 
     def device_memory(self) -> tuple:
         """(free, total) bytes of the GPU."""
@@ -295,6 +300,14 @@ class DeviceTable:
         n = _u64()
         _check(lib().rdb_count(self._h, expr.encode(), ctypes.byref(n)), self._ctx())
         return n.value
+
+    def explain(self, expr: str) -> str:
+        """The compiled operator tree of the filter as text (Operator::toString)."""
+        n = _u64()
+        lib().rdb_filter_explain(self._h, expr.encode(), None, 0, ctypes.byref(n))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        _check(lib().rdb_filter_explain(self._h, expr.encode(), buf, n.value + 1, ctypes.byref(n)), self._ctx())
+        return buf.value.decode()
 
     def local_reference(self, column: str) -> np.ndarray:
         out = np.zeros(self.seq_lengths[column], np.uint8)

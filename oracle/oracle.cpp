@@ -617,6 +617,7 @@ struct RangeSelection : Operator {
          }
       }
       std::sort(rows.begin(), rows.end());
+      rows.erase(std::unique(rows.begin(), rows.end()), rows.end());  // addRange is idempotent
       return Bitmap::fromSorted(rows.data(), rows.size());
    }
 };
@@ -776,8 +777,13 @@ struct Expr {
       DATE_BETWEEN, INT_BETWEEN, INT_EQ, BITMAP, COVERED,
       // operator-level forms used by the reference's operator tests: they build the operator
       // directly with its constructor rules instead of through an expression's compile()
-      OP_THRESHOLD, OP_INTERSECTION, OP_UNION, OP_COMPLEMENT
+      OP_THRESHOLD, OP_INTERSECTION, OP_UNION, OP_COMPLEMENT, OP_RANGES, OP_SELECT
    } kind;
+   struct ValuePredicate {  // OP_SELECT: CompareToValueSelection (selection.h:105-131)
+      std::string column;
+      int comparator;
+      int64_t value;
+   };
    std::vector<ExprPtr> children;
    std::string column;
    uint32_t position = 0;
@@ -791,6 +797,8 @@ struct Expr {
    std::optional<int64_t> from, to;
    int64_t value = 0;
    uint32_t bitmap_id = 0;
+   std::vector<uint32_t> row_ranges;  // OP_RANGES: global [start, end) pairs
+   std::vector<ValuePredicate> predicates;
 
    ExprPtr clone() const {
       auto e = std::make_unique<Expr>(*this, 0);
@@ -801,7 +809,8 @@ struct Expr {
    Expr(const Expr& o, int)
        : kind(o.kind), column(o.column), position(o.position), symbol_char(o.symbol_char),
          symbols(o.symbols), symbol_chars(o.symbol_chars), number(o.number), exact(o.exact), str(o.str),
-         sublineage(o.sublineage), from(o.from), to(o.to), value(o.value), bitmap_id(o.bitmap_id) {}
+         sublineage(o.sublineage), from(o.from), to(o.to), value(o.value), bitmap_id(o.bitmap_id),
+         row_ranges(o.row_ranges), predicates(o.predicates) {}
 };
 
 // --- S-expression reader -------------------------------------------------------------------------
@@ -952,6 +961,41 @@ struct Reader {
       } else if (op == "op_complement") {
          e = std::make_unique<Expr>(Expr::OP_COMPLEMENT);
          e->children.push_back(expr());
+      } else if (op == "op_range") {
+         // RangeSelection(ranges, row_layout) (range_selection.h) over global row-id pairs
+         e = std::make_unique<Expr>(Expr::OP_RANGES);
+         while (!peekClose()) {
+            const int64_t v = integer();
+            if (v < 0 || v > UINT32_MAX) throw QueryError("malformed filter: row id out of range");
+            e->row_ranges.push_back(static_cast<uint32_t>(v));
+         }
+         if (e->row_ranges.size() % 2 != 0) throw QueryError("malformed filter: op_range takes start/end pairs");
+      } else if (op == "op_select") {
+         // Selection(child?, predicates) (selection.h) with (cmp COLUMN OP VALUE) predicates
+         e = std::make_unique<Expr>(Expr::OP_SELECT);
+         while (!peekClose()) {
+            const size_t save = i;
+            expect('(');
+            auto [q, head] = atom();
+            if (!q && head == "cmp") {
+               Expr::ValuePredicate p;
+               p.column = name();
+               const std::string c = name();
+               const char* const names[] = {"eq", "lt", "gt", "le", "ge", "ne"};
+               p.comparator = -1;
+               for (int k = 0; k < 6; ++k) {
+                  if (c == names[k]) p.comparator = k;
+               }
+               if (p.comparator < 0) throw QueryError("malformed filter: unknown comparator " + c);
+               p.value = integer();
+               expect(')');
+               e->predicates.push_back(std::move(p));
+            } else {
+               i = save;
+               if (!e->children.empty()) throw QueryError("malformed filter: op_select takes at most one child");
+               e->children.push_back(expr());
+            }
+         }
       } else {
          throw QueryError("malformed filter: unknown expression '" + op + "'");
       }
@@ -983,7 +1027,9 @@ ExprPtr rewrite(const Table& t, const Expr& e) {
       case Expr::OP_THRESHOLD:
       case Expr::OP_INTERSECTION:
       case Expr::OP_UNION:
-      case Expr::OP_COMPLEMENT: {
+      case Expr::OP_COMPLEMENT:
+      case Expr::OP_RANGES:
+      case Expr::OP_SELECT: {
          // and.cpp:139-147, nof.cpp:249-257 (NONE keeps exact NOf), negation.cpp:18-21
          auto r = std::make_unique<Expr>(e, 0);
          for (const auto& c : e.children) r->children.push_back(rewrite(t, *c));
@@ -1413,6 +1459,36 @@ OperatorPtr compile(const Table& t, const Expr& e) {
       }
       case Expr::OP_COMPLEMENT:
          return std::make_unique<Complement>(tp, compile(t, *e.children[0]));
+      case Expr::OP_RANGES: {
+         // range_selection.cpp:54-60 asserts every end is a row of the layout or the layout's end
+         const auto& sizes = t.layout.sizes;
+         const uint64_t layout_end = static_cast<uint64_t>(sizes.size()) << 16;
+         auto valid = [&](uint32_t id) {
+            const uint32_t c = id >> 16;
+            return (c < sizes.size() && (id & 0xFFFFu) < sizes[c]) || id == layout_end;
+         };
+         std::vector<std::pair<uint32_t, uint32_t>> ranges;
+         for (size_t k = 0; k < e.row_ranges.size(); k += 2) {
+            const uint32_t a = e.row_ranges[k], b = e.row_ranges[k + 1];
+            if (!valid(a) || !valid(b) || a > b) throw QueryError("Compilation Error: RangeSelection range outside the row layout");
+            ranges.emplace_back(a, b);
+         }
+         return std::make_unique<RangeSelection>(tp, std::move(ranges));
+      }
+      case Expr::OP_SELECT: {
+         if (e.predicates.empty()) throw QueryError("Compilation Error: Selection needs at least one predicate");
+         PredicateVector p;
+         for (const auto& vp : e.predicates) {
+            auto it = t.ints.find(vp.column);
+            if (it == t.ints.end()) throw QueryError("The column '" + vp.column + "' is not of type int or date");
+            if (vp.value < INT32_MIN || vp.value > INT32_MAX) throw QueryError("Value out of int32 range");
+            p.push_back(std::make_unique<CompareToValue>(tp, &it->second, static_cast<Comparator>(vp.comparator),
+                                                         static_cast<int32_t>(vp.value), false));
+         }
+         std::optional<OperatorPtr> child;
+         if (!e.children.empty()) child = compile(t, *e.children[0]);
+         return std::make_unique<Selection>(tp, std::move(child), std::move(p));
+      }
       default:
          throw QueryError("expression must be rewritten before compilation");
    }
@@ -1766,6 +1842,20 @@ int or_filter(void* tv, const char* expr, uint32_t* out_rows, uint64_t cap, uint
          uint64_t k = 0;
          b.forEach([&](uint32_t g) { out_rows[k++] = g; });
       }
+      return 0;
+   } catch (const std::exception& e) {
+      return fail(t, e.what());
+   }
+}
+
+// The type of the compiled filter's root operator (OpType order of filter/operators/operator.h:11-22):
+// tests use it to check which operator a filter compiles to.
+int or_filter_root_type(void* tv, const char* expr, int* type) {
+   auto* t = static_cast<Table*>(tv);
+   try {
+      auto e = parseFilter(expr);
+      auto r = rewrite(*t, *e);
+      *type = static_cast<int>(compile(*t, *r)->type());
       return 0;
    } catch (const std::exception& e) {
       return fail(t, e.what());

@@ -48,6 +48,8 @@ def lib() -> ctypes.CDLL:
         L.or_index_stats.argtypes = [vp, cp, vp, vp]
         L.or_filter.restype = ip
         L.or_filter.argtypes = [vp, cp, vp, u64, vp]
+        L.or_filter_root_type.restype = ip
+        L.or_filter_root_type.argtypes = [vp, cp, vp]
         L.or_mutation_counts.restype = ip
         L.or_mutation_counts.argtypes = [vp, cp, cp, vp]
         L.or_mutations.restype = ip
@@ -140,6 +142,15 @@ class OracleTable:
         out = np.zeros(max(card.value, 1), dtype=np.uint32)
         self._check(lib().or_filter(self._h, expr.encode(), _p(out), card.value, ctypes.byref(card)))
         return out[: card.value]
+
+    OP_TYPES = ("Empty", "Full", "IndexScan", "Intersection", "Complement", "RangeSelection", "Selection",
+                "Threshold", "Union")
+
+    def root_operator(self, expr: str) -> str:
+        """Name of the operator the filter compiles to (its root)."""
+        k = ctypes.c_int()
+        self._check(lib().or_filter_root_type(self._h, expr.encode(), ctypes.byref(k)))
+        return self.OP_TYPES[k.value]
 
     def count(self, expr: str) -> int:
         card = ctypes.c_uint64()

@@ -201,3 +201,48 @@ def brute_from_matrix(spec, M, null):
     return brute.BruteTable(spec.chunk_sizes, seq_columns={c.name: (0, ref, (M, null))},
                             int_columns={i.name: (i.values, i.is_null)},
                             dict_columns={d.name: (d.ids, d.is_null, d.names, None)})
+
+
+def sorted_date_table(seed: int = 3):
+    """A table whose `date` column is null-free and non-decreasing (the reference then compiles
+    date ranges to RangeSelection, date_between.cpp:75-79): chunks of 3,000, 0, 2,500, 4,096 and
+    1,500 rows, the values of chunk 0 all below the first value of chunk 2, repeated values across
+    chunk boundaries; `date_u` holds the same values with one null (Selection path); `country` a
+    small dictionary.  Returns (spec, date values)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.asarray([3000, 0, 2500, 4096, 1500], np.uint32)
+    n = int(sizes.sum())
+    first = np.sort(rng.integers(18000, 18100, 3000))
+    rest = np.sort(rng.integers(18100, 18400, n - 3000))
+    rest[:40] = 18100                                    # a run of equal values at the start of chunk 2
+    rest[2500 - 30:2500 + 30] = 18250                    # equal values across the chunk 2 / 3 boundary
+    dates = np.sort(np.concatenate([first, rest])).astype(np.int32)
+    spec = table.TableSpec(chunk_sizes=sizes)
+    spec.int_columns.append(table.IntColumnData("date", True, dates, np.zeros(n, np.uint8)))
+    u_null = np.zeros(n, np.uint8)
+    u_null[4321] = 1
+    spec.int_columns.append(table.IntColumnData("date_u", True, dates.copy(), u_null))
+    spec.dict_columns.append(table.DictColumnData("country", rng.integers(0, 3, n).astype(np.uint32),
+                                                  np.zeros(n, np.uint8), ["c0", "c1", "c2"]))
+    return spec, dates
+
+
+def sorted_date_filters(dates) -> list:
+    """date_between filters whose ends fall below, at and above the column's values and exactly at
+    chunk boundaries, open bounds, negations and combinations."""
+    lo, hi = int(dates.min()), int(dates.max())
+    edge = int(dates[2999])  # last value of chunk 0 (ranges ending exactly at its end)
+    points = sorted({lo - 1, lo, lo + 1, edge, edge + 1, 18100, 18250, 18251, hi - 1, hi, hi + 1})
+    out = []
+    for a in points:
+        out += [f"(date_between date {a} _)", f"(date_between date _ {a})", f"(not (date_between date {a} _))"]
+        for b in points:
+            if b >= a:
+                out.append(f"(date_between date {a} {b})")
+    out += ["(date_between date _ _)", f"(date_between date {hi + 5} {hi + 9})", f"(date_between date {lo} {lo - 3})",
+            f'(and (date_between date 18100 18250) (str_eq country "c1"))',
+            f"(or (date_between date _ {edge}) (date_between date 18300 _))",
+            f"(not (or (date_between date _ 18050) (date_between date 18250 18260)))",
+            f"(nof 2 0 (date_between date 18050 18300) (date_between date_u 18100 _) (str_eq country \"c0\"))",
+            f"(date_between date_u 18100 18250)", f"(not (date_between date_u 18100 18250))"]
+    return out

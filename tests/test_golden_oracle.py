@@ -1,7 +1,7 @@
 """The oracle reproduces the reference's own known-answer tests (CPU only)."""
 import pytest
 
-from kat_support import KATS, dataset_spec, operator_spec, row_ids, symbol_chars
+from kat_support import KATS, aggregate_names, dataset_spec, operator_spec, row_ids, symbol_chars
 
 
 @pytest.mark.parametrize("case", KATS["operator_cases"], ids=lambda c: c["name"])
@@ -42,6 +42,19 @@ def test_mutation_kats(case):
     rows = [[p, chars[f], chars[t], c, cov, prop] for p, f, t, c, cov, prop in
             ot.mutations(case["filter"], case["column"], case["min_proportion"])]
     assert rows == case["expect"]
+
+
+@pytest.mark.parametrize("case", KATS["aggregate_cases"], ids=lambda c: c["name"])
+def test_aggregate_kats(case):
+    from oracle.oracle import OracleError, OracleTable
+
+    spec = dataset_spec(case["dataset"])
+    ot = OracleTable(spec)
+    if "expect_error" in case:
+        with pytest.raises(OracleError):
+            ot.aggregate(case["filter"], case["dims"])
+        return
+    assert aggregate_names(spec, case["dims"], ot.aggregate(case["filter"], case["dims"])) == case["expect"]
 
 
 @pytest.mark.parametrize("case", KATS["local_reference_cases"], ids=lambda c: c["dataset"])

@@ -8,7 +8,7 @@ from helpers import ExprGen, random_bitmaps, small_spec
 pytestmark = pytest.mark.gpu
 
 KNOBS = ("sparse_max_rows", "batch_max_queries", "batch_lanes", "batch_graphs", "batch_row_capacity", "batch_shared_scan",
-         "batch_octet_threads", "batch_octet_groups")
+         "batch_octet_threads", "batch_octet_groups", "batch_fail_job")
 # (batch_shared_scan, batch_octet_threads): every dense counting kernel of a batch
 SCANS = [(0, 1024), (1, 1024), (2, 1024), (2, 512)]
 
@@ -219,3 +219,67 @@ def test_results_as_columns(small, tuning):
             assert single.rows() == r, q
         else:
             assert c == r, q
+
+
+def test_failed_submit_leaves_lanes_usable(small, tuning):
+    """A lane job that fails after others of its batch were enqueued fails that submission only:
+    a batch already in flight keeps its results, and every lane serves later batches."""
+    from silo_amd.engine import EngineError
+
+    ds, ts, ot, et = small
+    tuning.set_tuning("batch_lanes", 3)
+    tuning.set_tuning("batch_max_queries", 8)
+    tuning.set_tuning("sparse_max_rows", 200)
+    first = make_queries(ds, seed=400, n=16)
+    pending = et.batch_submit(first)  # two jobs in flight
+    try:
+        tuning.set_tuning("batch_fail_job", 2)  # the second job of the next submission fails
+        with pytest.raises(EngineError) as err:
+            et.batch_submit(make_queries(ds, seed=401, n=40))
+        assert err.value.status == "RDB_ENOMEM"
+        assert tuning.get_tuning("batch_fail_job") == 0
+        assert pending.results() == [expected(ot, q) for q in first]
+    finally:
+        pending.free()
+        tuning.set_tuning("batch_fail_job", 0)
+    # all three lanes in use at once, results exact
+    qsets = [make_queries(ds, seed=410 + k, n=8) for k in range(3)]
+    handles = [et.batch_submit(qs) for qs in qsets]
+    try:
+        for qs, h in zip(qsets, handles):
+            assert h.results() == [expected(ot, q) for q in qs]
+    finally:
+        for h in handles:
+            h.free()
+
+
+@pytest.mark.parametrize("n_chunks,effective", [(4100, 1), (8200, 0)])
+def test_shared_scan_falls_back_on_many_chunks(engine, tuning, n_chunks, effective):
+    """The eight-query scan holds at most 1,024 chunks per part and the four-query scan 4,096 per
+    half: on larger tables a job runs the next smaller scan, the lane holds (and its footprint
+    counts) only that scan's buffers, and results equal the oracle's."""
+    from oracle.oracle import OracleTable
+    from silo_amd import datagen, table
+
+    ds = datagen.generate(datagen.GenSpec(n_rows=n_chunks * 3, genome_length=60, seed=57, lineage_model=True,
+                                          n_nodes=20, n_mean=5, null_seq_frac=0.02, n_days=30), keep_handle=True)
+    ts = table.spec_from_dataset(ds, chunk=3)
+    assert len(ts.chunk_sizes) == n_chunks
+    ot = OracleTable(ts)
+    et = engine.table(ts)
+    try:
+        tuning.set_tuning("sparse_max_rows", -1)
+        tuning.set_tuning("batch_max_queries", 8)
+        tuning.set_tuning("batch_shared_scan", effective)
+        reference_bytes = et.lane_bytes(8, 4096)
+        tuning.set_tuning("batch_shared_scan", 2)
+        assert et.lane_bytes(8, 4096) == reference_bytes
+        queries = [{"filter": f, "action": "mutations", "column": "main", "min_proportion": mp}
+                   for f, mp in [("(true)", 0.0), ("(has_mut main 7)", 0.05),
+                                 (f'(str_eq country "{ds.country_names[0]}")', 0.0),
+                                 ("(not (sym_eq main 30 N))", 0.02)]]
+        want = [ot.mutations(q["filter"], "main", q["min_proportion"]) for q in queries]
+        assert et.batch(queries) == want
+    finally:
+        et.close()
+        ot.close()

@@ -1,7 +1,7 @@
 """The engine (HIP, through the C ABI) reproduces the reference's own known-answer tests."""
 import pytest
 
-from kat_support import KATS, dataset_spec, operator_spec, row_ids, symbol_chars
+from kat_support import KATS, aggregate_names, dataset_spec, operator_spec, row_ids, symbol_chars
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +53,23 @@ def test_mutation_kats(engine, case):
         rows = [[p, chars[f], chars[t], c, cov, prop] for p, f, t, c, cov, prop in
                 et.mutations(case["filter"], case["column"], case["min_proportion"])]
         assert rows == case["expect"]
+    finally:
+        et.close()
+
+
+@pytest.mark.parametrize("case", KATS["aggregate_cases"], ids=lambda c: c["name"])
+def test_aggregate_kats(engine, case):
+    from silo_amd.engine import EngineError
+
+    spec = dataset_spec(case["dataset"])
+    et = engine.table(spec)
+    try:
+        if "expect_error" in case:
+            with pytest.raises(EngineError) as err:
+                et.aggregate(case["filter"], case["dims"])
+            assert err.value.status == STATUS[case["expect_error"]]
+            return
+        assert aggregate_names(spec, case["dims"], et.aggregate(case["filter"], case["dims"])) == case["expect"]
     finally:
         et.close()
 
