@@ -122,6 +122,17 @@ int rdb_mutations(rdb_table* table, const rdb_bitmap* filter, const char* column
 int rdb_mutations_query(rdb_table* table, const char* filter, const char* column, double min_proportion,
                         uint32_t* position, uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage,
                         double* proportion, uint64_t capacity, uint64_t* n);
+/* No counterpart in the reference.  rdb_mutations_query for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same results, but every device buffer the call needs beyond the table (staged filter program,
+ * filter words, partial sums, accumulators, rows) is carved from one file-scope device array of
+ * 128 MiB per GPU, held until the process ends; the call makes no hipMalloc, hipFree or pinned
+ * allocation.  RDB_EINVAL when the table needs more scratch than that (about 8 KiB per chunk for
+ * the filter plus 4 bytes per (position, symbol) slot per chunk group).  All calls on one GPU share
+ * the array: calls that overlap on one GPU (from different contexts or threads) race. */
+int rdb_mutations_query_noalloc(rdb_table* table, const char* filter, const char* column, double min_proportion,
+                                uint32_t* position, uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage,
+                                double* proportion, uint64_t capacity, uint64_t* n);
 /* Aggregated count group-by (bitmap_aggregation_node.cpp:91-130, 189-253).  dims: comma-separated
  * dimensions; "name" = indexed dictionary column, "name@pos" = symbol at 0-based position of a
  * sequence column.  Output: combinations in the reference's depth-first order, n_dims ids each

@@ -30,6 +30,7 @@ void Context::init(int dev) {
    stage_dev = DeviceBuffer(kStageSlots * kStageSlotBytes);
    for (auto& e : stage_events) hipCheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
    for (auto& e : prof_events) hipCheck(hipEventCreate(&e), "hipEventCreate");
+   hipCheck(noallocScratch(dev, &noalloc_scratch), "allocation-free scratch address");
 }
 
 void Context::profileAccumulate(int which) {
@@ -626,6 +627,112 @@ int rdb_mutations_query(rdb_table* table, const char* filter, const char* column
       if (!filter || !column || !n) fail(RDB_EINVAL, "null argument");
       DeviceBitmap f = query::compileFilter(t, filter)->evaluate();
       runMutations(t, f, column, min_proportion, position, from, to, count, coverage, proportion, capacity, n);
+   });
+}
+
+// Mutations query whose kernels use only the device's file-scope scratch array: the staged filter
+// program, the filter words, partial sums, heavy sums, totals, sparse accumulators and rows are
+// carved from it (256-byte aligned); nothing is allocated on the device or pinned on the host.
+int rdb_mutations_query_noalloc(rdb_table* table, const char* filter, const char* column, double min_proportion,
+                                uint32_t* position, uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage,
+                                double* proportion, uint64_t capacity, uint64_t* n) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireFinalized(t);
+      if (!filter || !column || !n) fail(RDB_EINVAL, "null argument");
+      if (t.ctx->noalloc_scratch == nullptr) fail(RDB_EDEVICE, "allocation-free scratch not resolved");
+      const int idx = capi::mutationColumn(t, column, min_proportion);
+      SeqColumn& col = *t.seq[idx];
+      query::Emitter program;
+      query::compileFilter(t, filter)->emit(program);
+      query::programDepth(*t.ctx, program);
+      const std::vector<uint8_t> blob = query::programBlob(program);
+      const uint32_t L = col.genome_length;
+      const uint32_t n_slots = col.view.n_slots;
+      const uint32_t n_pblocks = (L + kMutPosBlock - 1) / kMutPosBlock;
+      const size_t row_cap = std::max<size_t>(static_cast<size_t>(L) * 32, 2048);
+      // carve the scratch; fewer chunk groups (partial sums) when the table needs it
+      struct Carve {
+         size_t blob, words, card, partial, heavy, totals, sparse_counts, sparse_bnd, rows, end;
+      } c{};
+      auto layout = [&](uint32_t groups) {
+         size_t o = 0;
+         auto take = [&](size_t bytes) {
+            const size_t at = o;
+            o = (o + bytes + 255) / 256 * 256;
+            return at;
+         };
+         c.blob = take(blob.size());
+         c.words = take(static_cast<size_t>(t.n_chunks) * kChunkWords * 8);
+         c.card = take(64 + 4ULL * t.n_chunks);  // cardinality, then the per-chunk cardinalities
+         c.partial = take(4ULL * groups * n_slots);
+         c.heavy = take(4ULL * n_slots);
+         c.totals = take(static_cast<size_t>(n_pblocks) * 3 * sizeof(int64_t));
+         c.sparse_counts = take(4ULL * L * col.abc->count);
+         c.sparse_bnd = take(4ULL * 3 * L);
+         c.rows = take(16 + row_cap * sizeof(MutationRowDev));
+         c.end = o;
+      };
+      uint32_t groups = mutationGroups(t, col);
+      layout(groups);
+      while (c.end > kNoAllocScratchBytes && groups > 1) {
+         groups = (groups + 1) / 2;
+         layout(groups);
+      }
+      if (c.end > kNoAllocScratchBytes) {
+         fail(RDB_EINVAL, "table too large for rdb_mutations_query_noalloc: needs " + std::to_string(c.end) +
+                             " bytes of scratch, the limit is " + std::to_string(kNoAllocScratchBytes));
+      }
+      auto* base = static_cast<uint8_t*>(t.ctx->noalloc_scratch);
+      hipStream_t stream = t.ctx->stream;
+      hipCheck(hipMemcpyAsync(base + c.blob, blob.data(), blob.size(), hipMemcpyHostToDevice, stream), "stage program");
+      auto* card = reinterpret_cast<unsigned long long*>(base + c.card);
+      auto* chunk_cards = reinterpret_cast<uint32_t*>(base + c.card + 64);
+      query::launchProgram(t, program, base + c.blob, reinterpret_cast<uint64_t*>(base + c.words), card, chunk_cards);
+      // the accumulators that must start at zero: heavy sums and the sparse arrays (earlier calls
+      // leave whatever they left)
+      hipCheck(launchFill(reinterpret_cast<uint32_t*>(base + c.heavy), 0, n_slots, stream), "clear heavy sums");
+      hipCheck(launchFill(reinterpret_cast<uint32_t*>(base + c.sparse_counts), 0, static_cast<size_t>(L) * col.abc->count,
+                          stream),
+               "clear sparse counts");
+      hipCheck(launchFill(reinterpret_cast<uint32_t*>(base + c.sparse_bnd), 0, 3ULL * L, stream), "clear sparse boundaries");
+      MutationsLaunch m;
+      m.col_dev = t.seq_views_dev.as<SeqColumnView>() + idx;
+      m.col_host = &col.view;
+      m.chunk_sizes = t.chunk_sizes_dev.as<uint32_t>();
+      m.n_chunks = t.n_chunks;
+      m.n_groups = groups;
+      m.unroll = t.ctx->mutation_unroll;
+      m.coop = t.ctx->mutation_coop != 0;
+      m.filter = reinterpret_cast<const uint64_t*>(base + c.words);
+      m.filter_chunk_cards = chunk_cards;
+      m.filter_card = card;
+      m.sparse_max = t.ctx->sparse_max_rows;
+      m.partial = reinterpret_cast<uint32_t*>(base + c.partial);
+      m.heavy_sums = t.ctx->mutation_packed ? reinterpret_cast<uint32_t*>(base + c.heavy) : nullptr;
+      m.sparse_counts = reinterpret_cast<uint32_t*>(base + c.sparse_counts);
+      m.sparse_bnd = reinterpret_cast<int32_t*>(base + c.sparse_bnd);
+      m.totals = reinterpret_cast<int64_t*>(base + c.totals);
+      m.min_proportion = min_proportion;
+      m.n_rows = reinterpret_cast<uint32_t*>(base + c.rows);
+      m.rows = reinterpret_cast<MutationRowDev*>(base + c.rows + 16);
+      m.row_capacity = static_cast<uint32_t>(row_cap);
+      hipCheck(launchMutations(m, stream), "mutations kernels");
+      uint32_t rows_n = 0;
+      hipCheck(hipMemcpyAsync(&rows_n, base + c.rows, 4, hipMemcpyDeviceToHost, stream), "copy row count");
+      hipCheck(hipStreamSynchronize(stream), "synchronize");
+      if (rows_n > row_cap) fail(RDB_EDEVICE, "internal: mutation row buffer overflow");
+      std::vector<MutationRowDev> rows(rows_n);
+      if (rows_n) {
+         hipCheck(hipMemcpyAsync(rows.data(), base + c.rows + 16, rows_n * sizeof(MutationRowDev), hipMemcpyDeviceToHost,
+                                 stream),
+                  "copy rows");
+         hipCheck(hipStreamSynchronize(stream), "synchronize");
+      }
+      capi::sortMutationRows(rows);
+      capi::emitMutationRows(col, rows, position, from, to, count, coverage, proportion, capacity, n);
    });
 }
 

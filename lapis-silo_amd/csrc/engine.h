@@ -95,6 +95,8 @@ struct Context {
    std::recursive_mutex mu;
    std::string error;
    int cu_count = 256;
+   // device address of this GPU's allocation-free scratch (resolved by init)
+   void* noalloc_scratch = nullptr;
    size_t max_lds_bytes = 64 * 1024;  // LDS a work-group may allocate
    // pinned host staging
    void* pinned = nullptr;

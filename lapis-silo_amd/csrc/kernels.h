@@ -18,6 +18,10 @@ struct GroupDims {
 
 hipError_t launchFill(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
 
+// The allocation-free Mutations entry point's device scratch: a file-scope device array of this
+// many bytes per GPU; noallocScratch resolves its device address on `device` (once, then cached).
+constexpr size_t kNoAllocScratchBytes = size_t{128} << 20;
+hipError_t noallocScratch(int device, void** ptr);
 
 size_t filterLdsBytes(uint32_t depth);
 // Lets the filter kernel allocate up to max_lds_bytes of dynamic LDS (deep operator trees).

@@ -41,6 +41,7 @@ SIGNATURES = [
     ("rdb_mutation_counts", _int, [_vp, _vp, _cp, _vp]),
     ("rdb_mutations", _int, [_vp, _vp, _cp, _f64, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
     ("rdb_mutations_query", _int, [_vp, _cp, _cp, _f64, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
+    ("rdb_mutations_query_noalloc", _int, [_vp, _cp, _cp, _f64, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
     ("rdb_aggregate", _int, [_vp, _vp, _cp, _vp, _vp, _u64, _vp]),
     ("rdb_profile", _int, [_vp, _int]),
     ("rdb_profile_read", _int, [_vp, _vp, _vp, _vp, _vp]),
