@@ -72,6 +72,83 @@ def copy_calibration(mib):
     log(f"copied {mib} MiB five times")
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_query(ot, q):
+    if q["action"] == "mutations":
+        return ot.mutations(q["filter"], q.get("column", "main"), q["min_proportion"])
+    if q["action"] == "aggregate":
+        return ot.aggregate(q["filter"], q["dims"])
+    return ot.count(q["filter"])
+
+
+def cpu_legs(spec, queries, want, seconds_one: float, seconds_all: float, threads: int) -> tuple:
+    """The oracle (a single-threaded restatement of the reference's CPU operators) timed on a
+    bounded sample of the workload: one thread, one query at a time (the reference evaluates a
+    query on one worker thread), with per-query latencies; then `threads` threads running
+    independent queries of the same workload concurrently on the same table (the reference's worker
+    pool of hardware_concurrency threads).  Every oracle result is compared with the engine's
+    result for that query (`want`).  Returns (baseline dict, parity dict)."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle.oracle import OracleTable
+
+    t0 = time.time()
+    ot = OracleTable(spec)
+    build_s = time.time() - t0
+    log(f"oracle built in {build_s:.1f}s")
+    n = len(queries)
+    lat = []
+    identical = True
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds_one and len(lat) < n:
+        i = (len(lat) * 7) % n
+        t = time.perf_counter()
+        identical &= oracle_query(ot, queries[i]) == want[i]
+        lat.append(time.perf_counter() - t)
+    one_s = time.perf_counter() - t0
+    one = {"queries_per_s": round(len(lat) / one_s, 4), "queries": len(lat),
+           "p50_ms": round(percentile(lat, 50) * 1e3, 2), "p99_ms": round(percentile(lat, 99) * 1e3, 2)}
+    log(f"cpu baseline, one thread: {one}")
+    lock = threading.Lock()
+    stop = time.perf_counter() + seconds_all
+    results = {"done": 0, "identical": True}
+
+    def worker(k):
+        done, same, i = 0, True, k
+        while time.perf_counter() < stop:
+            j = (i * 7) % n
+            same &= oracle_query(ot, queries[j]) == want[j]
+            done += 1
+            i += threads
+        with lock:
+            results["done"] += done
+            results["identical"] &= same
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as pool:
+        list(pool.map(worker, range(threads)))
+    all_s = time.perf_counter() - t0
+    ot.close()
+    identical &= results["identical"]
+    cpu = {"value": round(results["done"] / all_s, 4), "unit": "queries/s", "cores": threads, "kind": "port",
+           "sample": (f"{results['done']} queries of the {n}-query workload on {threads} threads in {all_s:.1f}s "
+                      f"(independent queries, round-robin), oracle build {build_s:.1f}s excluded"),
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count(), "single_thread": one}
+    parity = {"queries_compared": len(lat) + results["done"], "identical": bool(identical)}
+    log(f"cpu baseline, {threads} threads: {cpu['value']} queries/s; parity {parity}")
+    return cpu, parity
+
+
 def run_plain_query(et, q):
     if q.get("action", "mutations") == "mutations":
         return et.mutations_query(q["filter"], q.get("column", "main"), q["min_proportion"], arrays=True)
@@ -86,6 +163,47 @@ def as_rows(results: list) -> list:
     from silo_amd.engine import MutationArrays
 
     return [r.rows() if isinstance(r, MutationArrays) else r for r in results]
+
+
+def scan_roofline(eng, et, queries, prof, n_chunks, log):
+    """Roofline object of the batched path's dense counting kernel: algorithmic bytes of its passes
+    over the index (the index bytes of the chunks the pass's filters touch, counted once per pass,
+    plus one 8 KiB filter tile per query and chunk) over its average launch time.  Every dense
+    filter of this workload touches every chunk (checked), so a pass of k queries reads the bytes of
+    the dense method under the all-rows filter plus (k - 1) more filter tiles per chunk."""
+    launches = prof["scan_launches"]
+    if not launches:
+        return None
+    dense = prof["scan_dense_queries"] / launches
+    switch = eng.get_tuning("sparse_max_rows")
+    for q in queries:
+        if q["action"] != "mutations":
+            continue
+        with et.bitmap(q["filter"]) as b:
+            if b.cardinality > 0 and (switch < 0 or b.cardinality > switch):
+                if len(set((b.rows() >> 16).tolist())) != n_chunks:
+                    log("scan roofline skipped: a dense filter misses a chunk")
+                    return None
+    mode = eng.get_tuning("batch_shared_scan")
+    per_pass = {0: 1, 1: 4, 2: 8}[mode]
+    saved = eng.get_tuning("sparse_max_rows")
+    eng.set_tuning("sparse_max_rows", -1)
+    with et.bitmap("(true)") as b:
+        all_rows = et.mutations_traffic(b, "main")
+    eng.set_tuning("sparse_max_rows", saved)
+    full_passes, rest = divmod(int(round(dense)), per_pass)
+    bytes_per_launch = full_passes * (all_rows + (per_pass - 1) * 8192 * n_chunks)
+    if rest:
+        bytes_per_launch += all_rows + (rest - 1) * 8192 * n_chunks
+    avg_ms = prof["scan_ms"] / launches
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    kernel = {0: "mutationsCountBatchKernel", 1: "mutationsCountSharedKernel", 2: "mutationsCountOctetKernel"}[mode]
+    out = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+           "algorithmic_bytes_per_launch": int(bytes_per_launch), "dense_queries_per_launch": round(dense, 2),
+           "queries_per_pass": per_pass, "launches": launches, "avg_launch_ms": round(avg_ms, 5)}
+    log(f"batched dense scan roofline: {out}")
+    return out
 
 
 def serve(args):
@@ -154,7 +272,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--rows", type=int, default=None)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="one-thread CPU baseline leg (seconds)")
+    ap.add_argument("--cpu-all-seconds", type=float, default=12.0, help="all-thread CPU baseline leg (seconds)")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="threads of the all-thread CPU leg (at most the host's CPUs)")
+    ap.add_argument("--steady-steps", type=int, default=200,
+                    help="pipelined steps of the steady-state figure reported beside the headline (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-query path (the batched one still runs)")
     ap.add_argument("--no-batched", action="store_true", help="only the per-query path (for profiling its kernels)")
@@ -354,37 +477,45 @@ def main():
         identical_paths = all(a == b for a, b in zip(plain_results, batch_results))
         log(f"batched results identical to plain: {identical_paths}")
 
+    # steady state: the same pipelined loop over many more steps than the headline's K (its fill and
+    # drain with several batches in flight weigh little here); reported beside `value`
+    steady = None
+    if args.steady_steps > 0:
+        n_steady = max(args.steady_steps, 16 * in_flight)
+        pend = deque()
+        t_st = time.perf_counter()
+        for _ in range(n_steady):
+            pend.append(et.batch_submit(queries))
+            if len(pend) == in_flight:
+                b = pend.popleft()
+                b.results(arrays=True)
+                b.free()
+        while pend:
+            b = pend.popleft()
+            b.results(arrays=True)
+            b.free()
+        st_s = time.perf_counter() - t_st
+        steady = {"queries_per_s": round(n_steady * len(queries) / st_s, 3), "steps": n_steady,
+                  "ms_per_step": round(st_s / n_steady * 1e3, 3)}
+        log(f"steady state: {steady}")
+
+    # the batched path's dense counting kernel, timed with HIP events on its lane stream (jobs run
+    # without graph replay while profiling), in a pass of its own outside the timed regions
+    scan_roof = None
+    if args.config == 1 and n_mut:
+        eng.profile(True)
+        for _ in range(3):
+            et.batch(queries, arrays=True)
+        sp = eng.profile_read()
+        eng.profile(False)
+        scan_roof = scan_roofline(eng, et, queries, sp, len(spec.chunk_sizes), log)
+
     # ---- CPU baseline ------------------------------------------------------------------------------
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
-        from oracle.oracle import OracleTable
-
-        t0 = time.time()
-        ot = OracleTable(spec)
-        build_s = time.time() - t0
-        log(f"oracle built in {build_s:.1f}s")
-        done = 0
-        identical = True
-        t0 = time.perf_counter()
-        while time.perf_counter() - t0 < args.cpu_seconds and done < len(queries):
-            i = (done * 7) % len(queries)
-            q = queries[i]
-            if q["action"] == "mutations":
-                want = ot.mutations(q["filter"], "main", q["min_proportion"])
-            elif q["action"] == "aggregate":
-                want = ot.aggregate(q["filter"], q["dims"])
-            else:
-                want = ot.count(q["filter"])
-            identical &= want == batch_results[i]
-            done += 1
-        cpu_s = time.perf_counter() - t0
-        cpu = {"value": round(done / cpu_s, 4), "unit": "queries/s", "cores": 1, "kind": "port",
-               "sample": f"{done} of the {len(queries)} workload queries, single thread, oracle build "
-                         f"{build_s:.1f}s excluded"}
-        parity = {"queries_compared": done, "identical": bool(identical)}
-        log(f"cpu baseline {cpu}; parity {parity}")
-        ot.close()
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        cpu, parity = cpu_legs(spec, queries, batch_results, args.cpu_seconds, args.cpu_all_seconds, threads)
 
     pmc = None
     pmc_path = os.path.join(ROOT, "profiles", "mutations_count_pmc.json")
@@ -434,7 +565,9 @@ def main():
         "tuning": tuned,
         "planner": planner,
         "bitmap_GBps": round(traffic_tuned * args.steps / b_elapsed / 1e9, 2),
+        "steady_state": steady,
         "roofline": roofline,
+        "roofline_batched_scan": scan_roof,
         "cpu_baseline": cpu,
         "parity": parity,
         "upload_s": round(upload_s, 2),
