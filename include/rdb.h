@@ -68,6 +68,34 @@ int rdb_table_add_sequence_column(rdb_table* table, const char* name, int alphab
                                   const uint32_t* cov_end, const uint64_t* mut_ptr, const uint32_t* mut_pos,
                                   const uint8_t* mut_sym, const uint64_t* nrun_ptr, const uint32_t* nrun_start,
                                   const uint32_t* nrun_end);
+/* A sequence column in the state the reference's SequenceColumn holds after finalize
+ * (storage/column/sequence_column.h:86-97), as a maintainer reads it out of a finalized or loaded
+ * column:
+ *   - reference: SequenceColumnMetadata::reference_sequence; local_reference: the symbols of
+ *     local_reference_sequence_string (sequence_column.h:118 getLocalReference);
+ *   - the vertical index (vertical_sequence_index.h:22-44): n_containers entries of
+ *     vertical_bitmaps, keyed (position, chunk = v_index, symbol), each container as
+ *     RoaringContainer serializes it (roaring_container.h:105-116): its cardinality, its CRoaring
+ *     typecode (1 bitset, 2 array, 3 run) and the bytes container_write writes for it, at
+ *     payload_offset in `payload` (bitset 8,192 bytes; array 2 per value; run a u16 run count and
+ *     (start, length - 1) u16 pairs);
+ *   - the horizontal coverage index (horizontal_coverage_index.h:21-35): every row's [start, end)
+ *     from start_end, and the missing positions horizontal_bitmaps holds for it (CSR by row:
+ *     missing_ptr[n_rows + 1], missing_pos ascending per row);
+ *   - null_bitmap, as a flag per row.
+ * Rows are in row-layout order (row r of chunk c at index chunk offset + r).  The engine keeps the
+ * given local reference (the reference adapts it again after each append, relative to the one
+ * before) and answers every query as for the same column uploaded in diff form; RDB_EINVAL for
+ * containers that do not fit the row layout, hold the local reference symbol, or rows whose
+ * mutations lie outside their covered range. */
+int rdb_table_add_sequence_column_finalized(rdb_table* table, const char* name, int alphabet, const uint8_t* reference,
+                                            const uint8_t* local_reference, uint32_t genome_length, uint64_t n_containers,
+                                            const uint32_t* key_position, const uint16_t* key_chunk,
+                                            const uint8_t* key_symbol, const uint8_t* typecode,
+                                            const uint32_t* cardinality, const uint64_t* payload_offset,
+                                            const uint8_t* payload, uint64_t payload_bytes, const uint8_t* is_null,
+                                            const uint32_t* cov_start, const uint32_t* cov_end,
+                                            const uint64_t* missing_ptr, const uint32_t* missing_pos);
 /* int32 or date32 column (storage/column/int_column.h:140, date32_column.h:20-80). */
 int rdb_table_add_int_column(rdb_table* table, const char* name, int is_date, const int32_t* values,
                              const uint8_t* is_null);

@@ -311,6 +311,49 @@ int rdb_table_add_sequence_column(rdb_table* table, const char* name, int alphab
    });
 }
 
+int rdb_table_add_sequence_column_finalized(rdb_table* table, const char* name, int alphabet, const uint8_t* reference,
+                                            const uint8_t* local_reference, uint32_t genome_length, uint64_t n_containers,
+                                            const uint32_t* key_position, const uint16_t* key_chunk,
+                                            const uint8_t* key_symbol, const uint8_t* typecode,
+                                            const uint32_t* cardinality, const uint64_t* payload_offset,
+                                            const uint8_t* payload, uint64_t payload_bytes, const uint8_t* is_null,
+                                            const uint32_t* cov_start, const uint32_t* cov_end,
+                                            const uint64_t* missing_ptr, const uint32_t* missing_pos) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireOpen(t);
+      if (!name || !reference) fail(RDB_EINVAL, "null argument");
+      if (nameTaken(t, name)) fail(RDB_EINVAL, std::string("duplicate column '") + name + "'");
+      if (alphabet != RDB_NUCLEOTIDE && alphabet != RDB_AMINO_ACID) fail(RDB_EINVAL, "unknown alphabet");
+      auto col = std::make_unique<SeqColumn>();
+      col->name = name;
+      col->alphabet = alphabet;
+      col->abc = &alphabetFor(alphabet);
+      col->genome_length = genome_length;
+      col->reference.assign(reference, reference + genome_length);
+      FinalizedSequence f;
+      f.local_reference = local_reference;
+      f.n_containers = n_containers;
+      f.key_position = key_position;
+      f.key_chunk = key_chunk;
+      f.key_symbol = key_symbol;
+      f.typecode = typecode;
+      f.cardinality = cardinality;
+      f.payload_offset = payload_offset;
+      f.payload = payload;
+      f.payload_bytes = payload_bytes;
+      f.is_null = is_null;
+      f.cov_start = cov_start;
+      f.cov_end = cov_end;
+      f.missing_ptr = missing_ptr;
+      f.missing_pos = missing_pos;
+      buildSequenceColumnFinalized(t, *col, f);
+      t.seq.push_back(std::move(col));
+   });
+}
+
 int rdb_table_add_int_column(rdb_table* table, const char* name, int is_date, const int32_t* values, const uint8_t* is_null) {
    Context* ctx = table ? table->t.ctx : nullptr;
    return guarded(ctx, [&] {

@@ -165,6 +165,8 @@ struct SeqColumn {
    uint32_t genome_length = 0;
    std::vector<uint8_t> reference;
    std::vector<uint8_t> local_ref;
+   // a local reference given with the column (a finalized upload): used as is instead of adapting
+   std::vector<uint8_t> local_ref_given;
    // host copies used when compiling filter leaves
    std::vector<uint32_t> block_pos_start, block_slot_start, pos_slot_ptr, slot_info;
    std::vector<uint32_t> block_of_pos;
@@ -257,6 +259,28 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
                          const uint8_t* mut_sym, const uint64_t* nrun_ptr, const uint32_t* nrun_start,
                          const uint32_t* nrun_end);
 void buildIntColumn(Table& t, IntColumn& col, const int32_t* values, const uint8_t* is_null);
+
+// A sequence column in the state the reference's SequenceColumn holds after finalize, brought to
+// the diff form buildSequenceColumn takes (mutations against the column's reference, missing runs,
+// covered ranges).  Containers are CRoaring containers as container_write writes them.
+struct FinalizedSequence {
+   const uint8_t* local_reference = nullptr;
+   uint64_t n_containers = 0;
+   const uint32_t* key_position = nullptr;
+   const uint16_t* key_chunk = nullptr;
+   const uint8_t* key_symbol = nullptr;
+   const uint8_t* typecode = nullptr;  // 1 bitset, 2 array, 3 run (CRoaring's container types)
+   const uint32_t* cardinality = nullptr;
+   const uint64_t* payload_offset = nullptr;
+   const uint8_t* payload = nullptr;
+   uint64_t payload_bytes = 0;
+   const uint8_t* is_null = nullptr;
+   const uint32_t* cov_start = nullptr;
+   const uint32_t* cov_end = nullptr;
+   const uint64_t* missing_ptr = nullptr;  // [n_rows + 1]
+   const uint32_t* missing_pos = nullptr;
+};
+void buildSequenceColumnFinalized(Table& t, SeqColumn& col, const FinalizedSequence& f);
 void buildDictColumn(Table& t, DictColumn& col, const uint32_t* ids, const uint8_t* is_null,
                      const int32_t* lineage_parent);
 void finalizeTable(Table& t);

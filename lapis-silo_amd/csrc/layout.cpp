@@ -443,6 +443,9 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
             best_count = counts[s];
          }
       }
+      // a column uploaded finalized keeps its own local reference: the reference adapts it again
+      // after every append relative to the previous one, which ties can leave elsewhere
+      if (!col.local_ref_given.empty()) best = col.local_ref_given[p];
       if (best != current) {
          flip_to[p] = best;
          flipped.push_back(p);
@@ -1156,6 +1159,175 @@ std::vector<uint32_t> DeviceBitmap::rows() {
       }
    }
    return out;
+}
+
+}  // namespace rdb
+
+namespace rdb {
+
+// The reference's finalized SequenceColumn (sequence_column.h:86-97) back in diff form: a stored
+// container (position p, chunk, symbol s) gives its rows the mutation (p, s) unless s is the
+// column's reference symbol (a position whose local reference was adapted stores the reference
+// symbol's rows, vertical_sequence_index.cpp:118-164); at such a position the rows left implicit
+// (covered at p, p not missing, in no stored container) hold the local reference symbol, which is
+// a mutation against the reference (symbol_in_set.cpp:185-212 reads them the same way).  Missing
+// positions inside a row's covered range (horizontal_coverage_index.h:25-27) become runs.
+void buildSequenceColumnFinalized(Table& t, SeqColumn& col, const FinalizedSequence& f) {
+   const uint32_t L = col.genome_length;
+   const uint32_t nsym = col.abc->count;
+   const uint64_t n = t.num_rows;
+   if (L == 0 || L > 65535) fail(RDB_EINVAL, "genome length must be in [1, 65535]");
+   if (!f.local_reference || !f.is_null || !f.cov_start || !f.cov_end || !f.missing_ptr) fail(RDB_EINVAL, "null argument");
+   if (f.n_containers && (!f.key_position || !f.key_chunk || !f.key_symbol || !f.typecode || !f.cardinality ||
+                          !f.payload_offset || !f.payload)) {
+      fail(RDB_EINVAL, "null container argument");
+   }
+   for (uint32_t p = 0; p < L; ++p) {
+      if (f.local_reference[p] >= nsym || col.reference[p] >= nsym) fail(RDB_EINVAL, "invalid reference symbol");
+   }
+   if (f.missing_ptr[0] != 0) fail(RDB_EINVAL, "missing_ptr must start at 0");
+   for (uint64_t i = 0; i < n; ++i) {
+      if (f.missing_ptr[i + 1] < f.missing_ptr[i]) fail(RDB_EINVAL, "missing_ptr must be non-decreasing");
+   }
+   // missing positions -> runs [start, end)
+   std::vector<uint64_t> nrun_ptr(n + 1, 0);
+   std::vector<uint32_t> nrun_start, nrun_end;
+   for (uint64_t i = 0; i < n; ++i) {
+      const uint32_t* a = f.missing_pos + f.missing_ptr[i];
+      const uint32_t* e = f.missing_pos + f.missing_ptr[i + 1];
+      for (const uint32_t* q = a; q < e;) {
+         const uint32_t s = *q;
+         uint32_t last = s;
+         ++q;
+         while (q < e && *q == last + 1) last = *q++;
+         if (q < e && *q <= last) fail(RDB_EINVAL, "missing positions of a row must ascend");
+         nrun_start.push_back(s);
+         nrun_end.push_back(last + 1);
+      }
+      nrun_ptr[i + 1] = nrun_start.size();
+   }
+   auto isMissing = [&](uint64_t i, uint32_t p) {
+      const uint32_t* a = f.missing_pos + f.missing_ptr[i];
+      const uint32_t* e = f.missing_pos + f.missing_ptr[i + 1];
+      return std::binary_search(a, e, p);
+   };
+   // containers by chunk
+   std::vector<std::vector<uint64_t>> by_chunk(t.n_chunks);
+   for (uint64_t k = 0; k < f.n_containers; ++k) {
+      const uint32_t c = f.key_chunk[k];
+      if (c >= t.n_chunks) fail(RDB_EINVAL, "container chunk outside the row layout");
+      const uint32_t p = f.key_position[k], s = f.key_symbol[k];
+      if (p >= L) fail(RDB_EINVAL, "container position outside the genome");
+      if (s >= nsym || s == col.abc->missing) fail(RDB_EINVAL, "invalid container symbol");
+      if (s == f.local_reference[p]) fail(RDB_EINVAL, "a stored container holds the local reference symbol");
+      by_chunk[c].push_back(k);
+   }
+   std::vector<uint32_t> adapted;
+   for (uint32_t p = 0; p < L; ++p) {
+      if (f.local_reference[p] != col.reference[p]) adapted.push_back(p);
+   }
+   std::vector<std::vector<uint64_t>> muts(t.n_chunks);  // row << 24 | position << 8 | symbol
+   std::vector<std::string> errors(t.n_chunks);
+   parallelFor(t.n_chunks, [&](size_t c) {
+      const uint32_t size = t.chunk_sizes[c];
+      const uint64_t off = t.chunk_offset[c];
+      auto& out = muts[c];
+      // rows stored at each adapted position of this chunk
+      std::vector<uint64_t> stored(adapted.size() * kChunkWords, 0);
+      std::vector<uint32_t> rows;
+      auto decode = [&](uint64_t k) -> bool {
+         rows.clear();
+         const uint8_t type = f.typecode[k];
+         const uint64_t o = f.payload_offset[k];
+         const uint32_t card = f.cardinality[k];
+         const uint8_t* b = f.payload + o;
+         if (type == 1) {  // bitset: 1024 little-endian u64 words
+            if (o + 8192 > f.payload_bytes) return false;
+            for (uint32_t w = 0; w < kChunkWords; ++w) {
+               uint64_t x;
+               std::memcpy(&x, b + 8 * w, 8);
+               while (x) {
+                  rows.push_back(w * 64 + static_cast<uint32_t>(std::countr_zero(x)));
+                  x &= x - 1;
+               }
+            }
+         } else if (type == 2) {  // array: card sorted u16
+            if (o + 2ULL * card > f.payload_bytes) return false;
+            for (uint32_t j = 0; j < card; ++j) {
+               uint16_t v;
+               std::memcpy(&v, b + 2 * j, 2);
+               if (j && v <= rows.back()) return false;
+               rows.push_back(v);
+            }
+         } else if (type == 3) {  // run: u16 run count, then (start, length - 1) pairs
+            if (o + 2 > f.payload_bytes) return false;
+            uint16_t nr;
+            std::memcpy(&nr, b, 2);
+            if (o + 2 + 4ULL * nr > f.payload_bytes) return false;
+            for (uint32_t j = 0; j < nr; ++j) {
+               uint16_t v, len;
+               std::memcpy(&v, b + 2 + 4 * j, 2);
+               std::memcpy(&len, b + 4 + 4 * j, 2);
+               if (static_cast<uint32_t>(v) + len > 0xFFFFu) return false;
+               if (!rows.empty() && v <= rows.back()) return false;
+               for (uint32_t r = v; r <= static_cast<uint32_t>(v) + len; ++r) rows.push_back(r);
+            }
+         } else {
+            return false;
+         }
+         return rows.size() == card;
+      };
+      for (uint64_t k : by_chunk[c]) {
+         if (!decode(k)) {
+            errors[c] = "malformed container payload";
+            return;
+         }
+         const uint32_t p = f.key_position[k], s = f.key_symbol[k];
+         const auto ai = std::lower_bound(adapted.begin(), adapted.end(), p);
+         const bool is_adapted = ai != adapted.end() && *ai == p;
+         for (uint32_t r : rows) {
+            if (r >= size) {
+               errors[c] = "container row outside its chunk";
+               return;
+            }
+            if (is_adapted) stored[static_cast<size_t>(ai - adapted.begin()) * kChunkWords + r / 64] |= 1ULL << (r % 64);
+            if (s != col.reference[p]) out.push_back((static_cast<uint64_t>(r) << 24) | (static_cast<uint64_t>(p) << 8) | s);
+         }
+      }
+      for (size_t a = 0; a < adapted.size(); ++a) {
+         const uint32_t p = adapted[a];
+         for (uint32_t r = 0; r < size; ++r) {
+            const uint64_t i = off + r;
+            if (f.is_null[i] || p < f.cov_start[i] || p >= f.cov_end[i]) continue;
+            if ((stored[a * kChunkWords + r / 64] >> (r % 64)) & 1ULL) continue;
+            if (isMissing(i, p)) continue;
+            out.push_back((static_cast<uint64_t>(r) << 24) | (static_cast<uint64_t>(p) << 8) | f.local_reference[p]);
+         }
+      }
+      std::sort(out.begin(), out.end());
+   });
+   for (const auto& e : errors) {
+      if (!e.empty()) fail(RDB_EINVAL, e);
+   }
+   std::vector<uint64_t> mut_ptr(n + 1, 0);
+   for (uint32_t c = 0; c < t.n_chunks; ++c) {
+      for (uint64_t m : muts[c]) mut_ptr[t.chunk_offset[c] + (m >> 24) + 1] += 1;
+   }
+   for (uint64_t i = 0; i < n; ++i) mut_ptr[i + 1] += mut_ptr[i];
+   std::vector<uint32_t> mut_pos(mut_ptr[n]);
+   std::vector<uint8_t> mut_sym(mut_ptr[n]);
+   for (uint32_t c = 0; c < t.n_chunks; ++c) {
+      uint64_t w = mut_ptr[t.chunk_offset[c]];
+      for (uint64_t m : muts[c]) {  // sorted by row, then position
+         mut_pos[w] = static_cast<uint32_t>((m >> 8) & 0xFFFF);
+         mut_sym[w] = static_cast<uint8_t>(m & 0xFF);
+         ++w;
+      }
+      std::vector<uint64_t>().swap(muts[c]);
+   }
+   col.local_ref_given.assign(f.local_reference, f.local_reference + L);
+   buildSequenceColumn(t, col, f.is_null, f.cov_start, f.cov_end, mut_ptr.data(), mut_pos.data(), mut_sym.data(),
+                       nrun_ptr.data(), nrun_start.data(), nrun_end.data());
 }
 
 }  // namespace rdb

@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _native
-from .table import TableSpec, alphabet_chars
+from .table import FinalizedSeqColumnData, TableSpec, alphabet_chars
 
 RDB_OK = 0
 _STATUS = {1: "RDB_EINVAL", 2: "RDB_ENOMEM", 3: "RDB_EDEVICE", 4: "RDB_EQUERY", 5: "RDB_ECOMPILE"}
@@ -27,6 +27,8 @@ SIGNATURES = [
     ("rdb_table_create", _int, [_vp, _vp, _u32, ctypes.POINTER(_vp)]),
     ("rdb_table_free", None, [_vp]),
     ("rdb_table_add_sequence_column", _int, [_vp, _cp, _int, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("rdb_table_add_sequence_column_finalized", _int, [_vp, _cp, _int, _vp, _vp, _u32, _u64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                        _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
     ("rdb_table_add_int_column", _int, [_vp, _cp, _int, _vp, _vp]),
     ("rdb_table_add_dict_column", _int, [_vp, _cp, _vp, _vp, _u32, _cp, _vp, _vp]),
     ("rdb_table_add_bitmap", _int, [_vp, _vp, _u64, _vp]),
@@ -242,6 +244,9 @@ class DeviceTable:
         self.dict_names = {}
         try:
             for c in spec.seq_columns:
+                if isinstance(c, FinalizedSeqColumnData):
+                    self._add_finalized(c)
+                    continue
                 arrs = [np.ascontiguousarray(x) for x in (
                     np.asarray(c.reference, np.uint8), np.asarray(c.is_null, np.uint8), np.asarray(c.cov_start, np.uint32),
                     np.asarray(c.cov_end, np.uint32), np.asarray(c.mut_ptr, np.uint64), np.asarray(c.mut_pos, np.uint32),
@@ -274,6 +279,19 @@ class DeviceTable:
             self.close()
             raise
         self.n_rows = spec.n_rows
+
+    def _add_finalized(self, c: "FinalizedSeqColumnData") -> None:
+        a = [np.ascontiguousarray(np.asarray(x, dt)) for x, dt in (
+            (c.reference, np.uint8), (c.local_reference, np.uint8), (c.key_position, np.uint32),
+            (c.key_chunk, np.uint16), (c.key_symbol, np.uint8), (c.typecode, np.uint8), (c.cardinality, np.uint32),
+            (c.payload_offset, np.uint64), (c.payload, np.uint8), (c.is_null, np.uint8), (c.cov_start, np.uint32),
+            (c.cov_end, np.uint32), (c.missing_ptr, np.uint64), (c.missing_pos, np.uint32))]
+        _check(lib().rdb_table_add_sequence_column_finalized(
+            self._h, c.name.encode(), c.alphabet, _p(a[0]), _p(a[1]), len(c.reference), len(a[2]), _p(a[2]), _p(a[3]),
+            _p(a[4]), _p(a[5]), _p(a[6]), _p(a[7]), _p(a[8]), len(a[8]), _p(a[9]), _p(a[10]), _p(a[11]), _p(a[12]),
+            _p(a[13])), self.engine.handle)
+        self.seq_lengths[c.name] = len(c.reference)
+        self.alphabets[c.name] = c.alphabet
 
     @property
     def handle(self):

@@ -47,6 +47,31 @@ class SeqColumnData:
 
 
 @dataclass
+class FinalizedSeqColumnData:
+    """A sequence column as the reference's SequenceColumn holds it after finalize
+    (`storage/column/sequence_column.h:86-97`): the local reference, the vertical index's containers
+    keyed (position, chunk, symbol) with cardinality, CRoaring typecode (1 bitset, 2 array, 3 run)
+    and `container_write` bytes, and per row the null flag, the covered range and the missing
+    positions inside it (CSR)."""
+    name: str
+    alphabet: int
+    reference: np.ndarray        # u8 symbols (the column's reference)
+    local_reference: np.ndarray  # u8 symbols
+    key_position: np.ndarray     # u32
+    key_chunk: np.ndarray        # u16
+    key_symbol: np.ndarray       # u8
+    typecode: np.ndarray         # u8
+    cardinality: np.ndarray      # u32
+    payload_offset: np.ndarray   # u64 byte offsets into payload
+    payload: np.ndarray          # u8
+    is_null: np.ndarray
+    cov_start: np.ndarray
+    cov_end: np.ndarray
+    missing_ptr: np.ndarray      # u64 [n_rows + 1]
+    missing_pos: np.ndarray      # u32
+
+
+@dataclass
 class IntColumnData:
     name: str
     is_date: bool

@@ -1816,6 +1816,68 @@ int or_local_reference(void* tv, const char* col, uint8_t* out) {
    return 0;
 }
 
+/// The column's finalized storage in the form a finalized reference SequenceColumn holds
+/// (sequence_column.h:86-97): vertical containers keyed (position, chunk, symbol) in map order with
+/// their cardinality, CRoaring typecode (1 bitset, 2 array, 3 run) and container_write bytes
+/// (roaring_container.h:105-116); per row (row-layout order) the null flag, [start, end) and the
+/// internal missing positions (CSR).  With null outputs it reports the sizes only.
+int or_export_sequence_column(void* tv, const char* col, uint64_t* n_containers, uint64_t* payload_bytes,
+                              uint64_t* n_missing, uint8_t* local_ref, uint32_t* key_position, uint16_t* key_chunk,
+                              uint8_t* key_symbol, uint8_t* typecode, uint32_t* cardinality, uint64_t* payload_offset,
+                              uint8_t* payload, uint8_t* is_null, uint32_t* cov_start, uint32_t* cov_end,
+                              uint64_t* missing_ptr, uint32_t* missing_pos) {
+   auto* t = static_cast<Table*>(tv);
+   auto it = t->seq.find(col);
+   if (it == t->seq.end()) return fail(t, "unknown sequence column");
+   const SequenceColumn& c = it->second;
+   uint64_t k = 0, bytes = 0;
+   for (const auto& [key, cont] : c.vertical) {
+      const uint64_t b = cont.type == Container::BITSET ? 8192 : cont.type == Container::ARRAY ? 2ULL * cont.arr.size()
+                                                                                             : 2 + 2ULL * cont.runs.size();
+      if (key_position) {
+         key_position[k] = key.pos;
+         key_chunk[k] = key.chunk;
+         key_symbol[k] = key.sym;
+         typecode[k] = cont.type == Container::BITSET ? 1 : cont.type == Container::ARRAY ? 2 : 3;
+         cardinality[k] = cont.card;
+         payload_offset[k] = bytes;
+         uint8_t* out = payload + bytes;
+         if (cont.type == Container::BITSET) {
+            std::memcpy(out, cont.bits.data(), 8192);
+         } else if (cont.type == Container::ARRAY) {
+            std::memcpy(out, cont.arr.data(), 2 * cont.arr.size());
+         } else {
+            const uint16_t nr = static_cast<uint16_t>(cont.runs.size() / 2);
+            std::memcpy(out, &nr, 2);
+            std::memcpy(out + 2, cont.runs.data(), 2 * cont.runs.size());
+         }
+      }
+      bytes += b;
+      ++k;
+   }
+   *n_containers = k;
+   *payload_bytes = bytes;
+   uint64_t miss = 0;
+   for (const auto& [row, bm] : c.horizontal) miss += bm.cardinality();
+   *n_missing = miss;
+   if (!key_position && !is_null) return 0;
+   std::copy(c.local_ref.begin(), c.local_ref.end(), local_ref);
+   uint64_t i = 0, m = 0;
+   missing_ptr[0] = 0;
+   for (size_t ch = 0; ch < c.start_end.size(); ++ch) {
+      for (size_t r = 0; r < c.start_end[ch].size(); ++r) {
+         const uint32_t g = (static_cast<uint32_t>(ch) << 16) | static_cast<uint32_t>(r);
+         is_null[i] = c.null_bitmap.contains(g) ? 1 : 0;
+         cov_start[i] = c.start_end[ch][r].first;
+         cov_end[i] = c.start_end[ch][r].second;
+         auto h = c.horizontal.find(g);
+         if (h != c.horizontal.end()) h->second.forEach([&](uint32_t p) { missing_pos[m++] = p; });
+         missing_ptr[++i] = m;
+      }
+   }
+   return 0;
+}
+
 /// Number of stored vertical containers and their total payload bytes (array 2*card, bitset 8192,
 /// run 4*runs) for the sequence column.
 int or_index_stats(void* tv, const char* col, uint64_t* n_containers, uint64_t* payload_bytes) {

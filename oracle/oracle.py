@@ -44,6 +44,8 @@ def lib() -> ctypes.CDLL:
         L.or_add_bitmap.argtypes = [vp, vp, u64]
         L.or_local_reference.restype = ip
         L.or_local_reference.argtypes = [vp, cp, vp]
+        L.or_export_sequence_column.restype = ip
+        L.or_export_sequence_column.argtypes = [vp, cp, vp, vp, vp] + [vp] * 13
         L.or_index_stats.restype = ip
         L.or_index_stats.argtypes = [vp, cp, vp, vp]
         L.or_filter.restype = ip
@@ -81,6 +83,7 @@ class OracleTable:
             raise OracleError("invalid row layout")
         self.seq_lengths = {}
         self.alphabets = {}
+        self.references = {}
         self.dict_names = {}
         for c in spec.seq_columns:
             arrs = [np.ascontiguousarray(x) for x in (c.reference, c.is_null, c.cov_start, c.cov_end, c.mut_ptr,
@@ -90,6 +93,7 @@ class OracleTable:
             self._check(rc)
             self.seq_lengths[c.name] = len(c.reference)
             self.alphabets[c.name] = c.alphabet
+            self.references[c.name] = np.asarray(c.reference, np.uint8)
         for c in spec.int_columns:
             v = np.ascontiguousarray(c.values, dtype=np.int32)
             n = np.ascontiguousarray(c.is_null, dtype=np.uint8)
@@ -129,6 +133,31 @@ class OracleTable:
         out = np.zeros(self.seq_lengths[column], dtype=np.uint8)
         self._check(lib().or_local_reference(self._h, column.encode(), _p(out)))
         return out
+
+    def export_finalized(self, column: str):
+        """The column's finalized storage (the state a finalized reference SequenceColumn holds) as
+        silo_amd.table.FinalizedSeqColumnData."""
+        from silo_amd.table import FinalizedSeqColumnData
+
+        nc, nb, nm = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        c = column.encode()
+        self._check(lib().or_export_sequence_column(self._h, c, ctypes.byref(nc), ctypes.byref(nb), ctypes.byref(nm),
+                                                    *([None] * 13)))
+        k, b, m, n = nc.value, nb.value, nm.value, self.n_rows
+        out = dict(local_reference=np.zeros(self.seq_lengths[column], np.uint8), key_position=np.zeros(k, np.uint32),
+                   key_chunk=np.zeros(k, np.uint16), key_symbol=np.zeros(k, np.uint8), typecode=np.zeros(k, np.uint8),
+                   cardinality=np.zeros(k, np.uint32), payload_offset=np.zeros(k, np.uint64),
+                   payload=np.zeros(max(b, 1), np.uint8), is_null=np.zeros(n, np.uint8),
+                   cov_start=np.zeros(n, np.uint32), cov_end=np.zeros(n, np.uint32),
+                   missing_ptr=np.zeros(n + 1, np.uint64), missing_pos=np.zeros(max(m, 1), np.uint32))
+        order = ("local_reference", "key_position", "key_chunk", "key_symbol", "typecode", "cardinality",
+                 "payload_offset", "payload", "is_null", "cov_start", "cov_end", "missing_ptr", "missing_pos")
+        self._check(lib().or_export_sequence_column(self._h, c, ctypes.byref(nc), ctypes.byref(nb), ctypes.byref(nm),
+                                                    *[_p(out[key]) for key in order]))
+        out["payload"] = out["payload"][:b]
+        out["missing_pos"] = out["missing_pos"][:m]
+        return FinalizedSeqColumnData(name=column, alphabet=self.alphabets[column],
+                                      reference=self.references[column].copy(), **out)
 
     def index_stats(self, column: str) -> tuple[int, int]:
         n = ctypes.c_uint64()
