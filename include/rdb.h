@@ -161,6 +161,41 @@ int rdb_mutations_query(rdb_table* table, const char* filter, const char* column
 int rdb_mutations_query_noalloc(rdb_table* table, const char* filter, const char* column, double min_proportion,
                                 uint32_t* position, uint8_t* from, uint8_t* to, uint32_t* count, uint32_t* coverage,
                                 double* proportion, uint64_t capacity, uint64_t* n);
+/* ---- result sinks ------------------------------------------------------------------------------ */
+/* Output columns of the Mutations action (MutationsNode::getOutputSchema, operators/
+ * mutations_node.h:60-84), written in that order whatever the order of the bits. */
+enum rdb_mutation_field {
+   RDB_FIELD_MUTATION_FROM = 1,  /* utf8: the reference symbol */
+   RDB_FIELD_MUTATION_TO = 2,    /* utf8 */
+   RDB_FIELD_SEQUENCE_NAME = 4,  /* utf8 */
+   RDB_FIELD_POSITION = 8,       /* int32, 1-based */
+   RDB_FIELD_PROPORTION = 16,    /* float64 */
+   RDB_FIELD_COVERAGE = 32,      /* int32 */
+   RDB_FIELD_COUNT = 64,         /* int32 */
+   RDB_FIELDS_ALL = 127
+};
+/* Mutations rows (as rdb_mutations returns them, symbols of `alphabet`) written as the reference's
+ * NdjsonSink writes a result batch (query_engine/exec_node/ndjson_sink.cpp: one JSON object per
+ * line, fields in schema order, doubles in shortest round-trip form).  *length receives the byte
+ * count; RDB_ENOMEM (nothing written) when capacity is smaller.  Host only: no device work. */
+int rdb_mutation_rows_ndjson(int alphabet, const char* sequence_name, uint32_t fields, const uint32_t* position,
+                             const uint8_t* from, const uint8_t* to, const uint32_t* count, const uint32_t* coverage,
+                             const double* proportion, uint64_t n, char* out, uint64_t capacity, uint64_t* length);
+/* The same rows as an Arrow IPC stream (schema message, one record batch, end-of-stream marker),
+ * the format the reference's ArrowIpcSink writes through arrow::ipc::MakeStreamWriter
+ * (exec_node/arrow_ipc_sink.cpp:49-60); string columns utf8, position / coverage / count int32,
+ * proportion float64, all nullable.  Host only. */
+int rdb_mutation_rows_arrow_ipc(int alphabet, const char* sequence_name, uint32_t fields, const uint32_t* position,
+                                const uint8_t* from, const uint8_t* to, const uint32_t* count, const uint32_t* coverage,
+                                const double* proportion, uint64_t n, uint8_t* out, uint64_t capacity, uint64_t* length);
+/* The Mutations action under `filter` on `column` (as rdb_mutations), its rows written to NDJSON
+ * (addMutationsToOutput -> NdjsonSink, mutations_node.cpp:292-366) or to an Arrow IPC stream
+ * (-> ArrowIpcSink).  *length and RDB_ENOMEM as above. */
+int rdb_mutations_ndjson(rdb_table* table, const rdb_bitmap* filter, const char* column, double min_proportion,
+                         uint32_t fields, char* out, uint64_t capacity, uint64_t* length);
+int rdb_mutations_arrow_ipc(rdb_table* table, const rdb_bitmap* filter, const char* column, double min_proportion,
+                            uint32_t fields, uint8_t* out, uint64_t capacity, uint64_t* length);
+
 /* Aggregated count group-by (bitmap_aggregation_node.cpp:91-130, 189-253).  dims: comma-separated
  * dimensions; "name" = indexed dictionary column, "name@pos" = symbol at 0-based position of a
  * sequence column.  Output: combinations in the reference's depth-first order, n_dims ids each

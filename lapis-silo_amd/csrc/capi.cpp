@@ -779,6 +779,107 @@ int rdb_mutations_query_noalloc(rdb_table* table, const char* filter, const char
    });
 }
 
+int rdb_mutation_rows_ndjson(int alphabet, const char* sequence_name, uint32_t fields, const uint32_t* position,
+                             const uint8_t* from, const uint8_t* to, const uint32_t* count, const uint32_t* coverage,
+                             const double* proportion, uint64_t n, char* out, uint64_t capacity, uint64_t* length) {
+   return guarded(nullptr, [&] {
+      if (!length || !sequence_name) fail(RDB_EINVAL, "null argument");
+      if (n && (!position || !from || !to || !count || !coverage || !proportion)) fail(RDB_EINVAL, "null row array");
+      if (alphabet != RDB_NUCLEOTIDE && alphabet != RDB_AMINO_ACID) fail(RDB_EINVAL, "unknown alphabet");
+      capi::MutationRowsView v{&alphabetFor(alphabet), sequence_name, capi::validFields(fields), position, from, to, count,
+                               coverage, proportion, n};
+      capi::checkSymbols(v);
+      const std::string text = capi::mutationRowsNdjson(v);
+      *length = text.size();
+      if (text.size() > capacity || (!out && !text.empty())) fail(RDB_ENOMEM, "output capacity too small");
+      if (!text.empty()) std::memcpy(out, text.data(), text.size());
+   });
+}
+
+int rdb_mutation_rows_arrow_ipc(int alphabet, const char* sequence_name, uint32_t fields, const uint32_t* position,
+                                const uint8_t* from, const uint8_t* to, const uint32_t* count, const uint32_t* coverage,
+                                const double* proportion, uint64_t n, uint8_t* out, uint64_t capacity, uint64_t* length) {
+   return guarded(nullptr, [&] {
+      if (!length || !sequence_name) fail(RDB_EINVAL, "null argument");
+      if (n && (!position || !from || !to || !count || !coverage || !proportion)) fail(RDB_EINVAL, "null row array");
+      if (alphabet != RDB_NUCLEOTIDE && alphabet != RDB_AMINO_ACID) fail(RDB_EINVAL, "unknown alphabet");
+      capi::MutationRowsView v{&alphabetFor(alphabet), sequence_name, capi::validFields(fields), position, from, to, count,
+                               coverage, proportion, n};
+      capi::checkSymbols(v);
+      const std::vector<uint8_t> bytes = capi::mutationRowsArrowIpc(v);
+      *length = bytes.size();
+      if (bytes.size() > capacity || !out) fail(RDB_ENOMEM, "output capacity too small");
+      std::memcpy(out, bytes.data(), bytes.size());
+   });
+}
+
+namespace {
+
+// The Mutations rows of `filter` on `column` as row arrays for the sinks.
+struct RowArrays {
+   std::vector<uint32_t> position, count, coverage;
+   std::vector<uint8_t> from, to;
+   std::vector<double> proportion;
+   capi::MutationRowsView view(const SeqColumn& col, uint32_t fields) const {
+      return {col.abc, col.name.c_str(), capi::validFields(fields), position.data(), from.data(), to.data(),
+              count.data(), coverage.data(), proportion.data(), position.size()};
+   }
+};
+
+RowArrays mutationRowArrays(Table& t, const DeviceBitmap& f, int idx, double min_proportion) {
+   const auto rows = capi::mutationRows(t, f, idx, min_proportion);
+   RowArrays a;
+   const size_t n = rows.size();
+   a.position.resize(n);
+   a.count.resize(n);
+   a.coverage.resize(n);
+   a.from.resize(n);
+   a.to.resize(n);
+   a.proportion.resize(n);
+   uint64_t k = 0;
+   capi::emitMutationRows(*t.seq[idx], rows, a.position.data(), a.from.data(), a.to.data(), a.count.data(),
+                          a.coverage.data(), a.proportion.data(), n, &k);
+   return a;
+}
+
+}  // namespace
+
+int rdb_mutations_ndjson(rdb_table* table, const rdb_bitmap* filter, const char* column, double min_proportion,
+                         uint32_t fields, char* out, uint64_t capacity, uint64_t* length) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireFinalized(t);
+      if (!column || !length) fail(RDB_EINVAL, "null argument");
+      capi::validFields(fields);
+      const int idx = capi::mutationColumn(t, column, min_proportion);
+      const RowArrays a = mutationRowArrays(t, filterOf(t, filter), idx, min_proportion);
+      const std::string text = capi::mutationRowsNdjson(a.view(*t.seq[idx], fields));
+      *length = text.size();
+      if (text.size() > capacity || (!out && !text.empty())) fail(RDB_ENOMEM, "output capacity too small");
+      if (!text.empty()) std::memcpy(out, text.data(), text.size());
+   });
+}
+
+int rdb_mutations_arrow_ipc(rdb_table* table, const rdb_bitmap* filter, const char* column, double min_proportion,
+                            uint32_t fields, uint8_t* out, uint64_t capacity, uint64_t* length) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireFinalized(t);
+      if (!column || !length) fail(RDB_EINVAL, "null argument");
+      capi::validFields(fields);
+      const int idx = capi::mutationColumn(t, column, min_proportion);
+      const RowArrays a = mutationRowArrays(t, filterOf(t, filter), idx, min_proportion);
+      const std::vector<uint8_t> bytes = capi::mutationRowsArrowIpc(a.view(*t.seq[idx], fields));
+      *length = bytes.size();
+      if (bytes.size() > capacity || !out) fail(RDB_ENOMEM, "output capacity too small");
+      std::memcpy(out, bytes.data(), bytes.size());
+   });
+}
+
 }  // extern "C"
 
 namespace rdb::capi {

@@ -77,4 +77,26 @@ void emitMutationRows(const SeqColumn& col, const std::vector<MutationRowDev>& r
 // Column index of a Mutations query; RDB_EQUERY for an unknown column or a bad minProportion.
 int mutationColumn(const Table& t, const char* column, double min_proportion);
 
+// Mutations rows (1-based positions, symbols of the column's alphabet) for the result sinks.
+struct MutationRowsView {
+   const Alphabet* alphabet = nullptr;
+   const char* sequence_name = "";
+   uint32_t fields = 0;  // rdb_mutation_field bits
+   const uint32_t* position = nullptr;
+   const uint8_t* from = nullptr;
+   const uint8_t* to = nullptr;
+   const uint32_t* count = nullptr;
+   const uint32_t* coverage = nullptr;
+   const double* proportion = nullptr;
+   uint64_t n = 0;
+};
+uint32_t validFields(uint32_t fields);
+// RDB_EINVAL when a row's from / to symbol lies outside the alphabet.
+void checkSymbols(const MutationRowsView& rows);
+// NDJSON lines as the reference's NdjsonSink writes them (exec_node/ndjson_sink.cpp).
+std::string mutationRowsNdjson(const MutationRowsView& rows);
+// An Arrow IPC stream (schema, one record batch, end of stream) as arrow::ipc::MakeStreamWriter
+// writes the reference's ArrowIpcSink output (exec_node/arrow_ipc_sink.cpp).
+std::vector<uint8_t> mutationRowsArrowIpc(const MutationRowsView& rows);
+
 }  // namespace rdb::capi

@@ -44,6 +44,10 @@ SIGNATURES = [
     ("rdb_mutations", _int, [_vp, _vp, _cp, _f64, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
     ("rdb_mutations_query", _int, [_vp, _cp, _cp, _f64, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
     ("rdb_mutations_query_noalloc", _int, [_vp, _cp, _cp, _f64, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp]),
+    ("rdb_mutation_rows_ndjson", _int, [_int, _cp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _u64, _vp]),
+    ("rdb_mutation_rows_arrow_ipc", _int, [_int, _cp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _u64, _vp]),
+    ("rdb_mutations_ndjson", _int, [_vp, _vp, _cp, _f64, _u32, _vp, _u64, _vp]),
+    ("rdb_mutations_arrow_ipc", _int, [_vp, _vp, _cp, _f64, _u32, _vp, _u64, _vp]),
     ("rdb_aggregate", _int, [_vp, _vp, _cp, _vp, _vp, _u64, _vp]),
     ("rdb_profile", _int, [_vp, _int]),
     ("rdb_profile_read", _int, [_vp, _vp, _vp, _vp, _vp]),
@@ -130,6 +134,46 @@ def diff_sequence(sequence: str, reference: str, alphabet: int = 0):
     _check(rc, None)
     return (start.value, end.value, miss[: nm.value].tolist(),
             list(zip(mpos[: nmut.value].tolist(), msym[: nmut.value].tolist())))
+
+
+FIELDS_ALL = 127  # rdb_mutation_field: every output column of the Mutations action
+
+
+def _sink_call(fn, head: list, ctx=None) -> bytes:
+    """Calls a sink entry point twice: for the length, then into a buffer of that size."""
+    n = _u64()
+    rc = fn(*head, None, 0, ctypes.byref(n))
+    if rc not in (0, 2):  # RDB_ENOMEM reports the length
+        _check(rc, ctx)
+    buf = ctypes.create_string_buffer(max(n.value, 1))
+    _check(fn(*head, buf, n.value, ctypes.byref(n)), ctx)
+    return buf.raw[: n.value]
+
+
+def _row_arrays(rows: list):
+    n = len(rows)
+    pos = np.asarray([r[0] for r in rows], np.uint32)
+    fr = np.asarray([r[1] for r in rows], np.uint8)
+    to = np.asarray([r[2] for r in rows], np.uint8)
+    cnt = np.asarray([r[3] for r in rows], np.uint32)
+    cov = np.asarray([r[4] for r in rows], np.uint32)
+    prop = np.asarray([r[5] for r in rows], np.float64)
+    return n, (pos, fr, to, cnt, cov, prop)
+
+
+def mutation_rows_ndjson(rows: list, sequence_name: str, alphabet: int = 0, fields: int = FIELDS_ALL) -> str:
+    """Mutations rows (position, from, to, count, coverage, proportion) as the reference's NDJSON."""
+    n, arrs = _row_arrays(rows)
+    out = _sink_call(lib().rdb_mutation_rows_ndjson, [alphabet, sequence_name.encode(), fields] +
+                     [_p(a) for a in arrs] + [n])
+    return out.decode()
+
+
+def mutation_rows_arrow_ipc(rows: list, sequence_name: str, alphabet: int = 0, fields: int = FIELDS_ALL) -> bytes:
+    """Mutations rows as an Arrow IPC stream (schema + one record batch)."""
+    n, arrs = _row_arrays(rows)
+    return _sink_call(lib().rdb_mutation_rows_arrow_ipc, [alphabet, sequence_name.encode(), fields] +
+                      [_p(a) for a in arrs] + [n])
 
 
 class Engine:
@@ -382,6 +426,18 @@ class DeviceTable:
         _check(lib().rdb_mutations_query(self._h, expr.encode(), column.encode(), float(min_proportion), _p(pos), _p(fr),
                                          _p(to), _p(cnt), _p(cov), _p(prop), cap, ctypes.byref(n)), self._ctx())
         return _mutation_rows(bufs, n.value, arrays)
+
+    def mutations_ndjson(self, filt, column: str, min_proportion: float, fields: int = FIELDS_ALL) -> str:
+        def run(b):
+            return _sink_call(lib().rdb_mutations_ndjson, [self._h, b.handle, column.encode(), float(min_proportion),
+                                                           fields], self._ctx()).decode()
+        return self._with_bitmap(filt, run)
+
+    def mutations_arrow_ipc(self, filt, column: str, min_proportion: float, fields: int = FIELDS_ALL) -> bytes:
+        def run(b):
+            return _sink_call(lib().rdb_mutations_arrow_ipc, [self._h, b.handle, column.encode(), float(min_proportion),
+                                                              fields], self._ctx())
+        return self._with_bitmap(filt, run)
 
     def mutations(self, filt, column: str, min_proportion: float) -> list:
         cap, (pos, fr, to, cnt, cov, prop) = self._row_buffers(column)
