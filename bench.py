@@ -265,6 +265,74 @@ def serve(args):
     eng.close()
 
 
+def ingest(args):
+    """--ingest: the index build of the config's main column (4 M sequences x 29,903 nt for config
+    1), timed three ways: from the diff form with the host build; from the diff form with the
+    local-reference adaptation and the sorted container entries on the GPU (the default); and from
+    aligned sequence text through the column builder (diff on the GPU; the text is rendered on the
+    host batch by batch, which is timed apart and subtracted).  All three layouts must be equal."""
+    from silo_amd import datagen, table
+    from silo_amd.engine import Engine
+
+    t0 = time.time()
+    ds = datagen.generate_config(args.config, n_rows=args.rows)
+    spec = table.spec_from_dataset(ds)
+    log(f"generated {ds.n_rows} rows x {ds.genome_length} nt in {time.time() - t0:.1f}s")
+    main = spec.seq_columns[0]
+    eng = Engine(0)
+
+    def timed_build(col, device):
+        eng.set_tuning("index_build_device", device)
+        try:
+            t = time.perf_counter()
+            et = eng.table(table.TableSpec(chunk_sizes=spec.chunk_sizes, seq_columns=[col]))
+            wall = time.perf_counter() - t
+        finally:
+            eng.set_tuning("index_build_device", 1)
+        out = {"wall_s": round(wall, 3), "phases_ms": {k: round(v, 1) for k, v in et.build_profile("main").items()},
+               "digest": f"{et.digest('main'):016x}"}
+        et.close()
+        return out
+
+    host = timed_build(main, 0)
+    log(f"host build: {host}")
+    dev = timed_build(main, 1)
+    log(f"device build: {dev}")
+    render = [0.0]
+
+    def batches():
+        it = table.aligned_batches(main, 65536)
+        while True:
+            t = time.perf_counter()
+            b = next(it, None)
+            render[0] += time.perf_counter() - t
+            if b is None:
+                return
+            yield b
+
+    text_bytes = ds.n_rows * ds.genome_length
+    aligned = timed_build(table.AlignedSeqColumnData("main", main.alphabet, main.reference, batches), 1)
+    aligned["render_s"] = round(render[0], 3)
+    aligned["builder_s"] = round(aligned["wall_s"] - render[0], 3)
+    aligned["text_GB"] = round(text_bytes / 1e9, 2)
+    aligned["text_GB_per_s"] = round(text_bytes / 1e9 / aligned["builder_s"], 2)
+    log(f"aligned-text build: {aligned}")
+    equal = host["digest"] == dev["digest"] == aligned["digest"]
+    line = {
+        "metric": "sequence index build time", "value": dev["wall_s"], "unit": "s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": round(dev["wall_s"] * 1e3, 1), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"index build of {ds.n_rows} sequences x {ds.genome_length} nt (config {args.config})",
+                   "rows": ds.n_rows},
+        "host_build": host, "device_build": dev, "aligned_text_build": aligned, "layouts_equal": equal,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    eng.close()
+    if not equal:
+        sys.exit(1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +354,7 @@ def main():
     ap.add_argument("--planner-epochs", type=int, default=30)
     ap.add_argument("--requests", type=int, default=4000, help="config 4: requests served")
     ap.add_argument("--rate", type=float, default=3000.0, help="config 4: Poisson arrival rate (requests/s)")
+    ap.add_argument("--ingest", action="store_true", help="time the index build of the main column instead")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (scales PMC byte counters)")
     args = ap.parse_args()
@@ -295,6 +364,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.ingest:
+        ingest(args)
         return
     if args.config == 4:
         serve(args)

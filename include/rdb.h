@@ -96,6 +96,29 @@ int rdb_table_add_sequence_column_finalized(rdb_table* table, const char* name, 
                                             const uint8_t* payload, uint64_t payload_bytes, const uint8_t* is_null,
                                             const uint32_t* cov_start, const uint32_t* cov_end,
                                             const uint64_t* missing_ptr, const uint32_t* missing_pos);
+/* A sequence column built from aligned sequences, as the reference's SequenceColumnBuilder
+ * (storage/column/sequence_column.h:178-233) buffers rows and SequenceColumn::appendChunk /
+ * finalize (sequence_column.cpp:99-212) index them.  Each append diffs its rows on the GPU against
+ * `reference` (symbols; extractCoverageAndMutationsFromSequence, common/aligned_sequence.cpp:18-122):
+ * row i's characters are data[row_ptr[i] .. row_ptr[i + 1]), placed from position offsets[i]
+ * (SequenceColumnBuilder::insert's offset, sequence_column.cpp:312-341; offsets may be null: 0),
+ * is_null may be null.  Errors as the reference's insert: RDB_EINVAL "illegal character 'c' at
+ * position p in the input sequence", or "the sequence '...' which was inserted with an offset o is
+ * larger than the length of the reference genome: L"; a failed append adds none of its rows.
+ * finish (every row of the table appended, in row order) builds the column into the table: the
+ * same layout rdb_table_add_sequence_column builds from the same rows in diff form, the local-
+ * reference adaptation and the sorted container entries computed on the GPU.  Free a builder
+ * before its table. */
+typedef struct rdb_seq_builder rdb_seq_builder;
+int rdb_seq_builder_create(rdb_table* table, const char* name, int alphabet, const uint8_t* reference,
+                           uint32_t genome_length, rdb_seq_builder** out);
+int rdb_seq_builder_append(rdb_seq_builder* builder, const char* data, const uint64_t* row_ptr, const uint32_t* offsets,
+                           const uint8_t* is_null, uint64_t n_rows);
+int rdb_seq_builder_finish(rdb_seq_builder* builder);
+void rdb_seq_builder_free(rdb_seq_builder* builder);
+/* No reference counterpart: a 64-bit digest of a sequence column's device buffers and host-side
+ * index state, equal for equal layouts (build checks). */
+int rdb_table_column_digest(rdb_table* table, const char* column, uint64_t* digest);
 /* int32 or date32 column (storage/column/int_column.h:140, date32_column.h:20-80). */
 int rdb_table_add_int_column(rdb_table* table, const char* name, int is_date, const int32_t* values,
                              const uint8_t* is_null);
@@ -116,6 +139,12 @@ int rdb_table_local_reference(const rdb_table* table, const char* column, uint8_
 /* Device bytes held by the table and the vertical-index payload statistics of a sequence column. */
 int rdb_table_stats(const rdb_table* table, const char* column, uint64_t* device_bytes, uint64_t* n_containers,
                     uint64_t* payload_bytes);
+
+/* No reference counterpart: wall time (ms) of the phases of a sequence column's index build, in
+ * order: checks of the rows, local-reference adaptation, containers (entries generated, sorted and
+ * packed), slots / position blocks / tiles, row-level arrays, layout checks, upload.  Phases past
+ * the seventh read 0. */
+int rdb_table_build_profile(const rdb_table* table, const char* column, double* phase_ms, uint32_t n_phases);
 
 /* ---- filters ---------------------------------------------------------------------------------- */
 /* computeFilter (operators/compute_filter.cpp:14-21).  `filter` is the expression tree in the
@@ -337,6 +366,8 @@ int rdb_set_planner(rdb_ctx* ctx, int mode, rdb_mlp* mlp);
  *                      CUs to concurrent batches;
  *   "batch_row_capacity" Mutations rows per query kept by a lane job (more: the query runs again
  *                      alone), default 4096;
+ *   "index_build_device" 1: the local-reference adaptation and container entries of a sequence
+ *                      column's index build run on the GPU (default), 0: on the host;
  *   "batch_fail_job"   fault injection for tests: the n-th lane job enqueued from now on fails with
  *                      RDB_ENOMEM after its inputs were staged, failing its rdb_batch_submit (0, the
  *                      default: never).

@@ -6,6 +6,7 @@
 
 #include "capi_internal.h"
 #include "engine.h"
+#include "ingest.h"
 #include "query.h"
 
 using namespace rdb;
@@ -236,6 +237,9 @@ int rdb_set_tuning(rdb_ctx* ctx, const char* key, int64_t value) {
       } else if (k == "batch_fail_job") {
          if (value < 0 || value > 1000000) fail(RDB_EINVAL, "batch_fail_job must be in [0, 10^6]");
          ctx->c.batch_fail_job = static_cast<uint32_t>(value);
+      } else if (k == "index_build_device") {
+         if (value != 0 && value != 1) fail(RDB_EINVAL, "index_build_device must be 0 or 1");
+         ctx->c.index_build_device = static_cast<uint32_t>(value);
       } else {
          fail(RDB_EINVAL, "unknown tuning key '" + k + "'");
       }
@@ -273,6 +277,8 @@ int rdb_get_tuning(rdb_ctx* ctx, const char* key, int64_t* value) {
          *value = ctx->c.batch_row_capacity;
       } else if (k == "batch_fail_job") {
          *value = ctx->c.batch_fail_job;
+      } else if (k == "index_build_device") {
+         *value = ctx->c.index_build_device;
       } else {
          fail(RDB_EINVAL, "unknown tuning key '" + k + "'");
       }
@@ -354,6 +360,81 @@ int rdb_table_add_sequence_column_finalized(rdb_table* table, const char* name, 
    });
 }
 
+struct rdb_seq_builder {
+   rdb::SeqBuilder b;
+};
+
+int rdb_seq_builder_create(rdb_table* table, const char* name, int alphabet, const uint8_t* reference,
+                           uint32_t genome_length, rdb_seq_builder** out) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      requireOpen(t);
+      if (!name || !reference || !out) fail(RDB_EINVAL, "null argument");
+      *out = nullptr;
+      if (nameTaken(t, name)) fail(RDB_EINVAL, std::string("duplicate column '") + name + "'");
+      if (alphabet != RDB_NUCLEOTIDE && alphabet != RDB_AMINO_ACID) fail(RDB_EINVAL, "unknown alphabet");
+      if (genome_length == 0 || genome_length > 65535) fail(RDB_EINVAL, "genome length must be in [1, 65535]");
+      auto h = std::make_unique<rdb_seq_builder>();
+      SeqBuilder& b = h->b;
+      b.table = &t;
+      b.col = std::make_unique<SeqColumn>();
+      b.col->name = name;
+      b.col->alphabet = alphabet;
+      b.col->abc = &alphabetFor(alphabet);
+      b.col->genome_length = genome_length;
+      b.col->reference.assign(reference, reference + genome_length);
+      for (uint32_t p = 0; p < genome_length; ++p) {
+         if (reference[p] >= b.col->abc->count) fail(RDB_EINVAL, "invalid reference symbol");
+      }
+      uint8_t lut[256];
+      for (int c = 0; c < 256; ++c) {
+         const int sym = b.col->abc->fromChar(static_cast<char>(c));
+         lut[c] = sym < 0 ? 0xFF : static_cast<uint8_t>(sym);
+      }
+      b.lut = DeviceBuffer(lut, sizeof(lut));
+      b.ref = DeviceBuffer(reference, genome_length);
+      *out = h.release();
+   });
+}
+
+int rdb_seq_builder_append(rdb_seq_builder* builder, const char* data, const uint64_t* row_ptr, const uint32_t* offsets,
+                           const uint8_t* is_null, uint64_t n_rows) {
+   Context* ctx = builder ? builder->b.table->ctx : nullptr;
+   return guarded(ctx, [&] {
+      if (!builder) fail(RDB_EINVAL, "null builder");
+      std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+      seqBuilderAppend(builder->b, data, row_ptr, offsets, is_null, n_rows);
+   });
+}
+
+int rdb_seq_builder_finish(rdb_seq_builder* builder) {
+   Context* ctx = builder ? builder->b.table->ctx : nullptr;
+   return guarded(ctx, [&] {
+      if (!builder) fail(RDB_EINVAL, "null builder");
+      std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+      Table& t = *builder->b.table;
+      requireOpen(t);
+      if (!builder->b.finished && nameTaken(t, builder->b.col->name)) {
+         fail(RDB_EINVAL, std::string("duplicate column '") + builder->b.col->name + "'");
+      }
+      seqBuilderFinish(builder->b);
+   });
+}
+
+void rdb_seq_builder_free(rdb_seq_builder* builder) { delete builder; }
+
+int rdb_table_column_digest(rdb_table* table, const char* column, uint64_t* digest) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      Table& t = tableOf(table);
+      std::lock_guard<std::recursive_mutex> lock(t.ctx->mu);
+      if (!column || !digest) fail(RDB_EINVAL, "null argument");
+      *digest = seqColumnDigest(t, t.seqColumn(column));
+   });
+}
+
 int rdb_table_add_int_column(rdb_table* table, const char* name, int is_date, const int32_t* values, const uint8_t* is_null) {
    Context* ctx = table ? table->t.ctx : nullptr;
    return guarded(ctx, [&] {
@@ -421,6 +502,15 @@ int rdb_table_local_reference(const rdb_table* table, const char* column, uint8_
       if (!table || !column || !out) fail(RDB_EINVAL, "null argument");
       const SeqColumn& col = table->t.seqColumn(column);
       std::copy(col.local_ref.begin(), col.local_ref.end(), out);
+   });
+}
+
+int rdb_table_build_profile(const rdb_table* table, const char* column, double* phase_ms, uint32_t n_phases) {
+   Context* ctx = table ? table->t.ctx : nullptr;
+   return guarded(ctx, [&] {
+      if (!table || !column || (!phase_ms && n_phases)) fail(RDB_EINVAL, "null argument");
+      const SeqColumn& col = table->t.seqColumn(column);
+      for (uint32_t k = 0; k < n_phases; ++k) phase_ms[k] = k < kBuildPhases ? col.build_ms[k] : 0.0;
    });
 }
 
@@ -813,6 +903,7 @@ int rdb_mutation_rows_arrow_ipc(int alphabet, const char* sequence_name, uint32_
    });
 }
 
+extern "C++" {
 namespace {
 
 // The Mutations rows of `filter` on `column` as row arrays for the sinks.
@@ -843,6 +934,7 @@ RowArrays mutationRowArrays(Table& t, const DeviceBitmap& f, int idx, double min
 }
 
 }  // namespace
+}  // extern "C++"
 
 int rdb_mutations_ndjson(rdb_table* table, const rdb_bitmap* filter, const char* column, double min_proportion,
                          uint32_t fields, char* out, uint64_t capacity, uint64_t* length) {

@@ -489,6 +489,35 @@ int dg_materialize(void* h, uint32_t row, char* out) {
    return 0;
 }
 
+/// Renders rows [row0, row0 + n) of a column in diff form as aligned strings of L characters each
+/// (out: n * L): the reference's characters, the mutations, and `missing_char` outside the covered
+/// range and inside the missing runs (a null row: all missing).  `chars` maps symbols to
+/// characters; pointers are the column's whole arrays (mut_ptr and nrun_ptr index them globally).
+void dg_render(uint32_t L, const uint8_t* reference, const char* chars, char missing_char, uint64_t row0, uint64_t n,
+               const uint8_t* is_null, const uint32_t* cov_start, const uint32_t* cov_end, const uint64_t* mut_ptr,
+               const uint32_t* mut_pos, const uint8_t* mut_sym, const uint64_t* nrun_ptr, const uint32_t* nrun_start,
+               const uint32_t* nrun_end, char* out) {
+   std::string ref(L, ' ');
+   for (uint32_t i = 0; i < L; ++i) ref[i] = chars[reference[i]];
+#pragma omp parallel for schedule(dynamic, 256)
+   for (int64_t k = 0; k < static_cast<int64_t>(n); ++k) {
+      const uint64_t r = row0 + static_cast<uint64_t>(k);
+      char* o = out + static_cast<uint64_t>(k) * L;
+      if (is_null[r]) {
+         std::memset(o, missing_char, L);
+         continue;
+      }
+      const uint32_t s = std::min(cov_start[r], L), e = std::min(std::max(cov_end[r], s), L);
+      std::memset(o, missing_char, s);
+      std::memcpy(o + s, ref.data() + s, e - s);
+      std::memset(o + e, missing_char, L - e);
+      for (uint64_t m = mut_ptr[r]; m < mut_ptr[r + 1]; ++m) o[mut_pos[m]] = chars[mut_sym[m]];
+      for (uint64_t m = nrun_ptr[r]; m < nrun_ptr[r + 1]; ++m) {
+         for (uint32_t p = nrun_start[m]; p < nrun_end[m] && p < L; ++p) o[p] = missing_char;
+      }
+   }
+}
+
 }  // extern "C"
 
 namespace {

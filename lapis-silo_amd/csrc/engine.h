@@ -137,6 +137,8 @@ struct Context {
    // fault injection for tests: the n-th lane job enqueued from now on fails with RDB_ENOMEM after
    // its inputs were staged (0: never)
    uint32_t batch_fail_job = 0;
+   // sequence index build: 1 local-reference adaptation and container entries on the device, 0 host
+   uint32_t index_build_device = 1;
    // planner of the batched path: 0 reference order, 1 independence estimates, 2 learned estimator
    int plan_mode = 0;
    struct Mlp* planner = nullptr;
@@ -158,6 +160,8 @@ struct ArenaBuilder {
 uint64_t containerPayloadBytes(const Desc& d);
 
 // ---- columns --------------------------------------------------------------------------------------
+constexpr uint32_t kBuildPhases = 7;
+
 struct SeqColumn {
    std::string name;
    int alphabet = 0;
@@ -183,6 +187,9 @@ struct SeqColumn {
    uint64_t row_major_bytes = 0;             // row-major mutation CSR (sparse path)
    std::vector<uint32_t> row_mut_ptr, row_nrun_ptr;  // host copies over padded rows (traffic accounting)
    DeviceBuffer sparse_counts, sparse_bnd;   // zero between calls; cleared by the sparse finalize
+   // wall time (ms) of the index build's phases: checks, adaptation, containers, slots and tiles,
+   // row-level arrays, layout checks, upload
+   double build_ms[kBuildPhases] = {};
 };
 
 struct IntColumn {
@@ -254,10 +261,13 @@ struct Table {
    int intIndex(const std::string& name) const;
 };
 
+// Builds a sequence column's device layout from its rows in diff form.  `trusted`: the rows come
+// from the device diff (mutations ascending and valid, runs internal, sorted and disjoint), so they
+// are not checked or normalized again.
 void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const uint32_t* cov_start,
                          const uint32_t* cov_end, const uint64_t* mut_ptr, const uint32_t* mut_pos,
                          const uint8_t* mut_sym, const uint64_t* nrun_ptr, const uint32_t* nrun_start,
-                         const uint32_t* nrun_end);
+                         const uint32_t* nrun_end, bool trusted = false);
 void buildIntColumn(Table& t, IntColumn& col, const int32_t* values, const uint8_t* is_null);
 
 // A sequence column in the state the reference's SequenceColumn holds after finalize, brought to

@@ -176,4 +176,84 @@ hipError_t launchSymbolLabels(const SeqColumnView* col_dev, const uint32_t* chun
 hipError_t launchGroupHistogram(uint32_t n_chunks, const uint64_t* filter, const uint32_t* filter_chunk_cards,
                                 const GroupDims& dims, uint32_t n_bins, uint32_t* hist, hipStream_t stream);
 
+// ---- ingestion (ingest.hip) -------------------------------------------------------------------------
+// Diff of aligned sequences against the reference.  Row i's characters are data[row_ptr[i] ..
+// row_ptr[i + 1]), placed from position offsets[i] (0 when offsets is null).  The count pass writes
+// per row the covered range [start, end) ([0, 0) when nothing is covered), the number of mutations
+// and of internal missing runs, and the first illegal character's position (0xFFFFFFFF: none); the
+// write pass writes the mutations at mut_off[i] and the runs at run_off[i].
+struct DiffBatch {
+   const char* data = nullptr;
+   const uint64_t* row_ptr = nullptr;
+   const uint32_t* offsets = nullptr;
+   const uint8_t* is_null = nullptr;
+   const uint8_t* lut = nullptr;  // 256 entries: the symbol of a character, 0xFF for an illegal one
+   const uint8_t* ref = nullptr;  // reference symbols
+   uint32_t n_rows = 0;
+   uint32_t length = 0;
+   uint32_t missing = 0;  // the missing symbol
+   uint32_t* start = nullptr;
+   uint32_t* end = nullptr;
+   uint32_t* n_mut = nullptr;
+   uint32_t* n_run = nullptr;
+   uint32_t* err = nullptr;
+   const uint64_t* mut_off = nullptr;
+   const uint64_t* run_off = nullptr;
+   uint32_t* mut_pos = nullptr;
+   uint8_t* mut_sym = nullptr;
+   uint32_t* run_start = nullptr;
+   uint32_t* run_end = nullptr;
+};
+hipError_t launchDiff(const DiffBatch& d, bool write, hipStream_t stream);
+
+// A sequence column's rows in diff form on the device: internal missing runs normalized (inside the
+// covered range, sorted, disjoint), packed start | end << 16.
+struct DiffRowsDev {
+   uint64_t n_rows = 0;
+   uint32_t length = 0;
+   uint32_t n_symbols = 0;
+   const uint8_t* is_null = nullptr;
+   const uint32_t* cov_start = nullptr;
+   const uint32_t* cov_end = nullptr;
+   const uint64_t* mut_ptr = nullptr;
+   const uint32_t* mut_pos = nullptr;
+   const uint8_t* mut_sym = nullptr;
+   const uint64_t* nrun_ptr = nullptr;
+   const uint32_t* runs = nullptr;
+   const uint8_t* reference = nullptr;
+};
+
+// Coverage and symbol counts per position, then the local reference of every position.
+// `changes` (length + 1 int64) and `sym_counts` (length * n_symbols) must be zero.
+struct AdaptCount {
+   DiffRowsDev rows;
+   long long* changes = nullptr;
+   uint32_t* sym_counts = nullptr;
+   uint32_t* coverage = nullptr;
+   uint8_t* local_ref = nullptr;
+};
+hipError_t launchAdapt(const AdaptCount& a, hipStream_t stream);
+
+// The container entries of every row (chunk << 40 | position << 24 | symbol << 16 | row in chunk):
+// the count pass writes their number per row, the write pass writes them at offsets[i].
+struct EntryBatch {
+   DiffRowsDev rows;
+   const uint64_t* chunk_offset = nullptr;  // [n_chunks + 1]: first row of each chunk
+   uint32_t n_chunks = 0;
+   const uint8_t* flip_to = nullptr;  // per position: the local reference symbol, 0xFF if unflipped
+   const uint32_t* flipped = nullptr;  // flipped positions, ascending
+   uint32_t n_flipped = 0;
+   uint32_t* counts = nullptr;
+   const uint64_t* offsets = nullptr;
+   uint64_t* keys = nullptr;
+};
+hipError_t launchEntries(const EntryBatch& e, bool write, hipStream_t stream);
+// Exclusive prefix sum of n u32 counts into u64 offsets (temp == null: sets *temp_bytes).
+hipError_t scanEntryCounts(const uint32_t* counts, uint64_t* offsets, uint64_t n, void* temp, size_t* temp_bytes,
+                           hipStream_t stream);
+// Radix sort of n (< 2^31) keys over bits [0, 56); the sorted keys end in *sorted (keys or
+// keys_alt).  temp == null: sets *temp_bytes.
+hipError_t sortEntryKeys(uint64_t* keys, uint64_t* keys_alt, uint64_t n, void* temp, size_t* temp_bytes,
+                         hipStream_t stream, uint64_t** sorted);
+
 }  // namespace rdb

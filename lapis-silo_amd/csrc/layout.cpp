@@ -8,11 +8,13 @@
 // chunk) tiles for the Mutations kernel and uploaded once.
 #include <algorithm>
 #include <bit>
+#include <chrono>
 #include <cstring>
 #include <numeric>
 #include <thread>
 
 #include "engine.h"
+#include "ingest.h"
 
 namespace rdb {
 
@@ -110,7 +112,9 @@ DeviceBuffer::DeviceBuffer(size_t bytes) : bytes_(bytes) {
 }
 
 DeviceBuffer::DeviceBuffer(const void* host, size_t bytes) : DeviceBuffer(std::max<size_t>(bytes, 16)) {
-   // empty arrays still get a (small) allocation so that every device pointer is valid
+   // empty arrays still get a (small) allocation so that every device pointer is valid; the bytes
+   // past a short array are zero, so equal arrays make equal buffers
+   if (bytes < 16) hipCheck(hipMemset(ptr_, 0, 16), "clear");
    if (bytes > 0 && host != nullptr) hipCheck(hipMemcpy(ptr_, host, bytes, hipMemcpyHostToDevice), "upload");
 }
 
@@ -329,7 +333,14 @@ bool inRuns(const uint32_t* runs, uint64_t n, uint32_t p) {
 void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const uint32_t* cov_start,
                          const uint32_t* cov_end, const uint64_t* mut_ptr, const uint32_t* mut_pos,
                          const uint8_t* mut_sym, const uint64_t* nrun_ptr, const uint32_t* nrun_start,
-                         const uint32_t* nrun_end) {
+                         const uint32_t* nrun_end, bool trusted) {
+   // wall time of the build's phases (SeqColumn::build_ms)
+   auto clock0 = std::chrono::steady_clock::now();
+   auto phase = [&](int k) {
+      const auto now = std::chrono::steady_clock::now();
+      col.build_ms[k] = std::chrono::duration<double, std::milli>(now - clock0).count();
+      clock0 = now;
+   };
    const uint32_t L = col.genome_length;
    const Alphabet& A = *col.abc;
    const uint32_t nsym = A.count;
@@ -342,8 +353,8 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    // normalized internal missing runs: clipped to [start, end) as insertCoverage does
    // (horizontal_coverage_index.cpp:34-37), sorted and merged
    std::vector<uint64_t> nptr(n + 1, 0);
-   std::vector<std::vector<uint32_t>> row_runs_tmp(t.n_chunks);
-   parallelFor(t.n_chunks, [&](size_t c) {
+   std::vector<std::vector<uint32_t>> row_runs_tmp(trusted ? 0 : t.n_chunks);
+   if (!trusted) parallelFor(t.n_chunks, [&](size_t c) {
       auto& out = row_runs_tmp[c];
       std::vector<std::pair<uint32_t, uint32_t>> runs;
       for (uint64_t i = t.chunk_offset[c]; i < t.chunk_offset[c + 1]; ++i) {
@@ -394,9 +405,15 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
          }
       }
    });
-   for (uint64_t i = 0; i < n; ++i) nptr[i + 1] += nptr[i];
-   std::vector<uint32_t> runs_all(nptr[n]);
-   {
+   std::vector<uint32_t> runs_all;
+   if (trusted) {
+      // rows of the device diff: runs already internal, sorted and disjoint
+      std::copy(nrun_ptr, nrun_ptr + n + 1, nptr.begin());
+      runs_all.resize(nptr[n]);
+      for (uint64_t k = 0; k < nptr[n]; ++k) runs_all[k] = nrun_start[k] | (nrun_end[k] << 16);
+   } else {
+      for (uint64_t i = 0; i < n; ++i) nptr[i + 1] += nptr[i];
+      runs_all.resize(nptr[n]);
       size_t o = 0;
       for (auto& v : row_runs_tmp) {
          std::copy(v.begin(), v.end(), runs_all.begin() + static_cast<ptrdiff_t>(o));
@@ -405,18 +422,32 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
       }
    }
 
+   phase(0);
    // --- local reference adaptation (sequence_column.cpp:157-212) ---------------------------------
-   std::vector<uint32_t> sym_counts(static_cast<size_t>(L) * nsym, 0);
-   std::vector<int64_t> changes(L + 1, 0);
-   for (uint64_t i = 0; i < n; ++i) {
-      if (is_null[i]) continue;
-      changes[cov_start[i]] += 1;
-      changes[cov_end[i]] -= 1;
-      for (uint64_t k = nptr[i]; k < nptr[i + 1]; ++k) {
-         changes[runs_all[k] & 0xFFFFu] -= 1;
-         changes[runs_all[k] >> 16] += 1;
+   // On the device (the default) the coverage and symbol counts and the rule below run as kernels
+   // over the uploaded rows, which then also generate and sort the container entries.
+   std::unique_ptr<DeviceIndexBuild> dev;
+   std::vector<uint32_t> dev_coverage;
+   std::vector<uint8_t> dev_best;
+   std::vector<uint32_t> sym_counts;
+   std::vector<int64_t> changes;
+   if (t.ctx->index_build_device) {
+      dev = std::make_unique<DeviceIndexBuild>(t, col, is_null, cov_start, cov_end, mut_ptr, mut_pos, mut_sym,
+                                               nptr.data(), runs_all.data());
+      dev->adapt(dev_coverage, dev_best);
+   } else {
+      sym_counts.assign(static_cast<size_t>(L) * nsym, 0);
+      changes.assign(L + 1, 0);
+      for (uint64_t i = 0; i < n; ++i) {
+         if (is_null[i]) continue;
+         changes[cov_start[i]] += 1;
+         changes[cov_end[i]] -= 1;
+         for (uint64_t k = nptr[i]; k < nptr[i + 1]; ++k) {
+            changes[runs_all[k] & 0xFFFFu] -= 1;
+            changes[runs_all[k] >> 16] += 1;
+         }
+         for (uint64_t k = mut_ptr[i]; k < mut_ptr[i + 1]; ++k) sym_counts[static_cast<size_t>(mut_pos[k]) * nsym + mut_sym[k]] += 1;
       }
-      for (uint64_t k = mut_ptr[i]; k < mut_ptr[i + 1]; ++k) sym_counts[static_cast<size_t>(mut_pos[k]) * nsym + mut_sym[k]] += 1;
    }
    col.local_ref = col.reference;
    std::vector<uint8_t> flip_to(L, 0xFF);
@@ -425,22 +456,27 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    std::vector<uint32_t> counts(nsym);
    col.coverage.assign(L, 0);
    for (uint32_t p = 0; p < L; ++p) {
-      coverage += changes[p];
-      col.coverage[p] = static_cast<uint32_t>(coverage);
       const uint8_t current = col.reference[p];
-      for (uint32_t s = 0; s < nsym; ++s) counts[s] = sym_counts[static_cast<size_t>(p) * nsym + s];
-      uint32_t ref_count = static_cast<uint32_t>(coverage);
-      for (uint32_t s = 0; s < nsym; ++s) {
-         if (s != current) ref_count -= counts[s];
-      }
-      counts[current] = ref_count;
       uint8_t best = current;
-      uint32_t best_count = counts[current];
-      for (uint32_t s = 0; s < nsym; ++s) {
-         if (s == current) continue;
-         if (counts[s] > best_count) {
-            best = static_cast<uint8_t>(s);
-            best_count = counts[s];
+      if (dev) {
+         col.coverage[p] = dev_coverage[p];
+         best = dev_best[p];
+      } else {
+         coverage += changes[p];
+         col.coverage[p] = static_cast<uint32_t>(coverage);
+         for (uint32_t s = 0; s < nsym; ++s) counts[s] = sym_counts[static_cast<size_t>(p) * nsym + s];
+         uint32_t ref_count = static_cast<uint32_t>(coverage);
+         for (uint32_t s = 0; s < nsym; ++s) {
+            if (s != current) ref_count -= counts[s];
+         }
+         counts[current] = ref_count;
+         uint32_t best_count = counts[current];
+         for (uint32_t s = 0; s < nsym; ++s) {
+            if (s == current) continue;
+            if (counts[s] > best_count) {
+               best = static_cast<uint8_t>(s);
+               best_count = counts[s];
+            }
          }
       }
       // a column uploaded finalized keeps its own local reference: the reference adapts it again
@@ -453,17 +489,33 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
       }
    }
    col.n_flipped = static_cast<uint32_t>(flipped.size());
+   std::vector<int64_t>().swap(changes);
+   std::vector<uint32_t>().swap(sym_counts);
 
+   phase(1);
    // --- per-chunk containers ----------------------------------------------------------------------
+   // entries position << 24 | symbol << 16 | row, sorted per chunk (from the device: with the chunk
+   // in bits 40 and up, all chunks in one array)
+   std::vector<uint64_t> dev_keys, dev_chunk_ptr;
+   if (dev) {
+      dev->entries(flip_to, flipped, dev_keys, dev_chunk_ptr);
+      dev.reset();
+   }
    std::vector<ChunkContainers> chunks(t.n_chunks);
    parallelFor(t.n_chunks, [&](size_t c) {
       std::vector<uint64_t> entries;
+      const uint64_t* ebegin = nullptr;
+      size_t ecount = 0;
+      if (!dev_chunk_ptr.empty()) {
+         ebegin = dev_keys.data() + dev_chunk_ptr[c];
+         ecount = dev_chunk_ptr[c + 1] - dev_chunk_ptr[c];
+      }
       const uint64_t off = t.chunk_offset[c];
       const uint32_t size = t.chunk_sizes[c];
       auto push = [&](uint32_t p, uint32_t sym, uint32_t r) {
          entries.push_back((static_cast<uint64_t>(p) << 24) | (static_cast<uint64_t>(sym) << 16) | r);
       };
-      for (uint32_t r = 0; r < size; ++r) {
+      for (uint32_t r = 0; r < (ebegin ? 0 : size); ++r) {
          const uint64_t i = off + r;
          if (is_null[i]) {
             push(0, nsym + BND_COV_END, r);  // no covered range: uncovered everywhere
@@ -494,15 +546,20 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
             if (b < L) push(b, nsym + BND_N_END, r);
          }
       }
-      std::sort(entries.begin(), entries.end());
+      if (!ebegin) {
+         std::sort(entries.begin(), entries.end());
+         ebegin = entries.data();
+         ecount = entries.size();
+      }
       auto& out = chunks[c];
       std::vector<uint16_t> rows;
       size_t k = 0;
-      while (k < entries.size()) {
-         const uint64_t key = entries[k] >> 16;
+      constexpr uint64_t kInChunk = (uint64_t{1} << 40) - 1;
+      while (k < ecount) {
+         const uint64_t key = (ebegin[k] & kInChunk) >> 16;
          rows.clear();
-         while (k < entries.size() && (entries[k] >> 16) == key) {
-            rows.push_back(static_cast<uint16_t>(entries[k] & 0xFFFF));
+         while (k < ecount && ((ebegin[k] & kInChunk) >> 16) == key) {
+            rows.push_back(static_cast<uint16_t>(ebegin[k] & 0xFFFF));
             ++k;
          }
          out.key.push_back(static_cast<uint32_t>(key));
@@ -511,6 +568,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
       }
    });
 
+   phase(2);
    // --- slots: distinct (position, symbol) pairs, sorted ------------------------------------------
    const uint32_t total_syms = nsym + kBoundaryCount;
    if (total_syms > 32) fail(RDB_EINVAL, "alphabet too large");
@@ -716,6 +774,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    chunks.clear();
    col.desc_bytes = desc.size() * sizeof(Desc) + light.size() * 4 + group_hdr.size() * 18;
 
+   phase(3);
    // --- row-level arrays (padded to 2^16 rows per chunk) ------------------------------------------
    const size_t padded = static_cast<size_t>(t.n_chunks) << 16;
    std::vector<uint32_t> cov(padded, 0), pptr(padded + 1, 0);
@@ -809,6 +868,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
       rmptr[padded] = acc;
    }
 
+   phase(4);
    // --- checks ------------------------------------------------------------------------------------
    validateDescriptors(desc.data(), desc.size(), arr_all.size(), run_all.size(), bit_all.size() / kChunkWords, kMaxBlockSlots);
    for (uint32_t b = 0; b < n_blocks; ++b) {
@@ -840,6 +900,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
       if ((runs_all[k] & 0xFFFFu) >= (runs_all[k] >> 16) || (runs_all[k] >> 16) > L) fail(RDB_EDEVICE, "internal: bad missing run");
    }
 
+   phase(5);
    // --- upload ------------------------------------------------------------------------------------
    auto up = [&](const void* p, size_t bytes) -> const void* {
       col.buffers.emplace_back(p, bytes);
@@ -906,6 +967,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    col.row_major_bytes = (rmptr.size() + rmuts.size()) * 4;
    col.row_mut_ptr = std::move(rmptr);
    col.row_nrun_ptr = std::move(pptr);
+   phase(6);
 }
 
 // ---- metadata columns --------------------------------------------------------------------------

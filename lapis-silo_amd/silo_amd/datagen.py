@@ -92,6 +92,8 @@ def _lib() -> ctypes.CDLL:
             getattr(lib, name).argtypes = [vp]
         lib.dg_materialize.restype = ctypes.c_int
         lib.dg_materialize.argtypes = [vp, u32, ctypes.c_char_p]
+        lib.dg_render.restype = None
+        lib.dg_render.argtypes = [u32, vp, ctypes.c_char_p, ctypes.c_char, u64, u64] + [vp] * 10
         lib.dg_translate.restype = ctypes.c_int
         lib.dg_translate.argtypes = [vp, u32, vp, vp]
         for name in ("dg_aa_reference", "dg_aa_cov_start", "dg_aa_cov_end", "dg_aa_mut_ptr", "dg_aa_mut_pos",

@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _native
-from .table import FinalizedSeqColumnData, TableSpec, alphabet_chars
+from .table import AlignedSeqColumnData, FinalizedSeqColumnData, TableSpec, alphabet_chars
 
 RDB_OK = 0
 _STATUS = {1: "RDB_EINVAL", 2: "RDB_ENOMEM", 3: "RDB_EDEVICE", 4: "RDB_EQUERY", 5: "RDB_ECOMPILE"}
@@ -29,12 +29,18 @@ SIGNATURES = [
     ("rdb_table_add_sequence_column", _int, [_vp, _cp, _int, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("rdb_table_add_sequence_column_finalized", _int, [_vp, _cp, _int, _vp, _vp, _u32, _u64, _vp, _vp, _vp, _vp, _vp, _vp,
                                                         _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
+    ("rdb_seq_builder_create", _int, [_vp, _cp, _int, _vp, _u32, ctypes.POINTER(_vp)]),
+    ("rdb_seq_builder_append", _int, [_vp, _vp, _vp, _vp, _vp, _u64]),
+    ("rdb_seq_builder_finish", _int, [_vp]),
+    ("rdb_seq_builder_free", None, [_vp]),
+    ("rdb_table_column_digest", _int, [_vp, _cp, _vp]),
     ("rdb_table_add_int_column", _int, [_vp, _cp, _int, _vp, _vp]),
     ("rdb_table_add_dict_column", _int, [_vp, _cp, _vp, _vp, _u32, _cp, _vp, _vp]),
     ("rdb_table_add_bitmap", _int, [_vp, _vp, _u64, _vp]),
     ("rdb_table_finalize", _int, [_vp]),
     ("rdb_table_local_reference", _int, [_vp, _cp, _vp]),
     ("rdb_table_stats", _int, [_vp, _cp, _vp, _vp, _vp]),
+    ("rdb_table_build_profile", _int, [_vp, _cp, _vp, _u32]),
     ("rdb_filter", _int, [_vp, _cp, ctypes.POINTER(_vp), _vp]),
     ("rdb_bitmap_cardinality", _int, [_vp, _vp]),
     ("rdb_bitmap_rows", _int, [_vp, _vp, _u64, _vp]),
@@ -291,6 +297,9 @@ class DeviceTable:
                 if isinstance(c, FinalizedSeqColumnData):
                     self._add_finalized(c)
                     continue
+                if isinstance(c, AlignedSeqColumnData):
+                    self._add_aligned(c)
+                    continue
                 arrs = [np.ascontiguousarray(x) for x in (
                     np.asarray(c.reference, np.uint8), np.asarray(c.is_null, np.uint8), np.asarray(c.cov_start, np.uint32),
                     np.asarray(c.cov_end, np.uint32), np.asarray(c.mut_ptr, np.uint64), np.asarray(c.mut_pos, np.uint32),
@@ -337,6 +346,32 @@ class DeviceTable:
         self.seq_lengths[c.name] = len(c.reference)
         self.alphabets[c.name] = c.alphabet
 
+    def _add_aligned(self, c: "AlignedSeqColumnData") -> None:
+        L, ctx = lib(), self.engine.handle
+        ref = np.ascontiguousarray(c.reference, np.uint8)
+        b = _vp()
+        _check(L.rdb_seq_builder_create(self._h, c.name.encode(), c.alphabet, _p(ref), len(ref), ctypes.byref(b)), ctx)
+        try:
+            batches = c.batches() if callable(c.batches) else c.batches
+            for chars, row_ptr, offsets, is_null in batches:
+                chars = np.ascontiguousarray(chars, np.uint8)
+                row_ptr = np.ascontiguousarray(row_ptr, np.uint64)
+                offs = None if offsets is None else np.ascontiguousarray(offsets, np.uint32)
+                nl = None if is_null is None else np.ascontiguousarray(is_null, np.uint8)
+                _check(L.rdb_seq_builder_append(b, _p(chars), _p(row_ptr), None if offs is None else _p(offs),
+                                                None if nl is None else _p(nl), len(row_ptr) - 1), ctx)
+            _check(L.rdb_seq_builder_finish(b), ctx)
+        finally:
+            L.rdb_seq_builder_free(b)
+        self.seq_lengths[c.name] = len(ref)
+        self.alphabets[c.name] = c.alphabet
+
+    def digest(self, column: str) -> int:
+        """64-bit digest of a sequence column's device layout (equal layouts, equal digests)."""
+        d = _u64()
+        _check(lib().rdb_table_column_digest(self._h, column.encode(), ctypes.byref(d)), self._ctx())
+        return int(d.value)
+
     @property
     def handle(self):
         return self._h
@@ -382,6 +417,14 @@ class DeviceTable:
         _check(lib().rdb_table_stats(self._h, None if column is None else column.encode(), ctypes.byref(db),
                                      ctypes.byref(nc), ctypes.byref(pb)), self._ctx())
         return {"device_bytes": db.value, "containers": nc.value, "payload_bytes": pb.value}
+
+    BUILD_PHASES = ("checks", "adaptation", "containers", "tiles", "row_arrays", "layout_checks", "upload")
+
+    def build_profile(self, column: str) -> dict:
+        """Wall time (ms) of the phases of a sequence column's index build."""
+        ms = np.zeros(len(self.BUILD_PHASES), np.float64)
+        _check(lib().rdb_table_build_profile(self._h, column.encode(), _p(ms), len(ms)), self._ctx())
+        return dict(zip(self.BUILD_PHASES, ms.tolist()))
 
     def _with_bitmap(self, filt, fn):
         if isinstance(filt, Bitmap):

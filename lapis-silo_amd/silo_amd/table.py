@@ -72,6 +72,50 @@ class FinalizedSeqColumnData:
 
 
 @dataclass
+class AlignedSeqColumnData:
+    """A sequence column given as aligned sequences, diffed and indexed on the GPU by the column
+    builder (the reference's SequenceColumnBuilder insert / insertNull, storage/column/
+    sequence_column.h:178-233).  `batches` yields, in row order, tuples (chars u8, row_ptr u64
+    [n + 1], offsets u32 or None, is_null u8 or None): row i of a batch is chars[row_ptr[i] ..
+    row_ptr[i + 1]) placed from position offsets[i]."""
+    name: str
+    alphabet: int
+    reference: np.ndarray  # u8 symbols
+    batches: object        # an iterable of batches, or a callable returning one
+
+
+def render_rows(col: SeqColumnData, row0: int, n: int) -> np.ndarray:
+    """Rows [row0, row0 + n) of a column in diff form as aligned characters, shape (n, L) u8: the
+    reference's characters with the mutations, and the missing symbol's character outside the
+    covered range, inside missing runs and on null rows."""
+    from . import datagen
+
+    chars = alphabet_chars(col.alphabet)
+    missing = "N" if col.alphabet == ALPHABET_NUC else "X"
+    L = len(col.reference)
+    out = np.empty((n, L), np.uint8)
+    arrs = [np.ascontiguousarray(np.asarray(a, dt)) for a, dt in (
+        (col.reference, np.uint8), (col.is_null, np.uint8), (col.cov_start, np.uint32), (col.cov_end, np.uint32),
+        (col.mut_ptr, np.uint64), (col.mut_pos, np.uint32), (col.mut_sym, np.uint8), (col.nrun_ptr, np.uint64),
+        (col.nrun_start, np.uint32), (col.nrun_end, np.uint32))]
+    if n:
+        datagen._lib().dg_render(L, arrs[0].ctypes.data, chars.encode(), missing.encode(), row0, n,
+                                 *[a.ctypes.data for a in arrs[1:]], out.ctypes.data)
+    return out
+
+
+def aligned_batches(col: SeqColumnData, batch_rows: int = 65536):
+    """The rows of a column in diff form as AlignedSeqColumnData batches of full-length rows."""
+    L = len(col.reference)
+    n = len(col.is_null)
+    for a in range(0, n, batch_rows):
+        m = min(batch_rows, n - a)
+        chars = render_rows(col, a, m)
+        row_ptr = np.arange(m + 1, dtype=np.uint64) * np.uint64(L)
+        yield chars.reshape(-1), row_ptr, None, np.ascontiguousarray(col.is_null[a:a + m], np.uint8)
+
+
+@dataclass
 class IntColumnData:
     name: str
     is_date: bool
