@@ -1738,60 +1738,51 @@ __global__ __launch_bounds__(kSharedThreads) __attribute__((amdgpu_waves_per_eu(
          }
          if (a01 | a23) atomicAdd(&cnt[cur], static_cast<unsigned long long>(a01) | (static_cast<unsigned long long>(a23) << 32));
       }
-      // bitset and run containers: one per wavefront, the four queries at once
-      for (uint32_t d = h0 + wave; d < h1; d += kSharedThreads / 64) {
+      // bitset and run containers: two wavefronts per container, each a half.  A bitset half's
+      // 1,024 u32 words are read lane-consecutive (16 loads of 256 bytes per wavefront, all in
+      // flight at once), so the matching filter words come out of LDS without bank conflicts; sums
+      // in 32-bit fields (a container and a filter may share all 65,536 rows).
+      for (uint32_t t = 2 * h0 + wave; t < 2 * h1; t += kSharedThreads / 64) {
+         const uint32_t d = t >> 1, half = t & 1u;
          const u32x2 dw = toGlobal<GlobalU32x2>(col.vert.desc)[d];
          const uint32_t type = descType(dw[0]);
-         const uint32_t card = descCard(dw[0]);
          const uint32_t slot = descSlot(dw[0]);
          uint32_t n[4] = {0, 0, 0, 0};
          if (type == CT_BITSET) {
-            const GlobalU64x2* bits = toGlobal<GlobalU64x2>(col.vert.bits + static_cast<size_t>(dw[1]) * kChunkWords);
-            u64x2 wv[8];
+            constexpr uint32_t kHalfWords = kChunkWords / 64;  // u32 words of a half per lane
+            const GlobalU32* b32 = toGlobal<GlobalU32>(reinterpret_cast<const uint32_t*>(
+                                      col.vert.bits + static_cast<size_t>(dw[1]) * kChunkWords)) +
+                                   half * kChunkWords;
+            const u32x4* f4 = fq4 + half * kChunkWords;
+            uint32_t x[kHalfWords];
 #pragma unroll
-            for (uint32_t u = 0; u < 8; ++u) wv[u] = bits[lane + 64 * u];
+            for (uint32_t u = 0; u < kHalfWords; ++u) x[u] = b32[64 * u + lane];
 #pragma unroll
-            for (uint32_t u = 0; u < 8; ++u) {
-               const uint32_t wd = 2 * (lane + 64 * u);  // u64 word index of wv[u][0]
+            for (uint32_t u = 0; u < kHalfWords; ++u) {
+               const u32x4 f = f4[64 * u + lane];
 #pragma unroll
-               for (uint32_t h = 0; h < 2; ++h) {
-                  const uint64_t x = wv[u][h];
-                  const u32x4 flo = fq4[2 * (wd + h)];
-                  const u32x4 fhi = fq4[2 * (wd + h) + 1];
-#pragma unroll
-                  for (uint32_t k = 0; k < 4; ++k) {
-                     n[k] += __popc(static_cast<uint32_t>(x) & flo[k]) + __popc(static_cast<uint32_t>(x >> 32) & fhi[k]);
-                  }
-               }
+               for (uint32_t q4 = 0; q4 < 4; ++q4) n[q4] += __popc(x[u] & f[q4]);
             }
-         } else if (type == CT_RUN) {
+         } else if (type == CT_RUN && half == 0) {
+            const uint32_t card = descCard(dw[0]);
             const GlobalU32* rr = toGlobal<GlobalU32>(col.vert.runs + dw[1]);
-            for (uint32_t t = lane; t < card; t += 64) {
-               const uint32_t st = rr[t] & 0xFFFFu;
-               const uint32_t en = st + (rr[t] >> 16);
+            for (uint32_t r = lane; r < card; r += 64) {
+               const uint32_t st = rr[r] & 0xFFFFu;
+               const uint32_t en = st + (rr[r] >> 16);
 #pragma unroll
-               for (uint32_t k = 0; k < 4; ++k) n[k] += rangePopcountQ(fq, k, st, en);
-            }
-         } else if (type == CT_ARRAY) {
-            const GlobalU16* a = toGlobal<GlobalU16>(col.vert.arr + dw[1]);
-            for (uint32_t t = lane; t < card; t += 64) {
-               const uint32_t row = a[t];
-               const u32x4 wv = fq4[row >> 5];
-#pragma unroll
-               for (uint32_t k = 0; k < 4; ++k) n[k] += (wv[k] >> (row & 31u)) & 1u;
-            }
-         } else if (lane == 0) {
-            for (uint32_t t = 0; t < card; ++t) {
-               const uint32_t row = (dw[1] >> (16 * t)) & 0xFFFFu;
-               const u32x4 wv = fq4[row >> 5];
-#pragma unroll
-               for (uint32_t k = 0; k < 4; ++k) n[k] += (wv[k] >> (row & 31u)) & 1u;
+               for (uint32_t q4 = 0; q4 < 4; ++q4) n[q4] += rangePopcountQ(fq, q4, st, en);
             }
          }
+         // per lane at most 1,024 a query: two queries per 64-bit sum in 32-bit fields
+         const uint64_t s01 = waveSum64(static_cast<uint64_t>(n[0]) | (static_cast<uint64_t>(n[1]) << 32));
+         const uint64_t s23 = waveSum64(static_cast<uint64_t>(n[2]) | (static_cast<uint64_t>(n[3]) << 32));
+         if (lane == 0) {
+            const uint32_t v[4] = {static_cast<uint32_t>(s01), static_cast<uint32_t>(s01 >> 32),
+                                   static_cast<uint32_t>(s23), static_cast<uint32_t>(s23 >> 32)};
 #pragma unroll
-         for (uint32_t k = 0; k < 4; ++k) {
-            const uint32_t v = waveSum(n[k]);
-            if (lane == 0 && valid[k] && v) atomicAdd(&m.heavy_sums[static_cast<size_t>(4 * j + k) * n_slots + s0 + slot], v);
+            for (uint32_t q4 = 0; q4 < 4; ++q4) {
+               if (valid[q4] && v[q4]) atomicAdd(&m.heavy_sums[static_cast<size_t>(4 * j + q4) * n_slots + s0 + slot], v[q4]);
+            }
          }
       }
       c = cn;
