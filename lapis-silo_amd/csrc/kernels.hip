@@ -1061,13 +1061,19 @@ __device__ inline void sparseCountRows(const SeqColumnView& col, const uint32_t*
    const GlobalU32* mut_ptr = toGlobal<GlobalU32>(col.row_mut_ptr);
    const GlobalU32* muts = toGlobal<GlobalU32>(col.row_muts);
    const GlobalU64* fw = toGlobal<GlobalU64>(filter);
-   for (uint32_t base = wave_id * 64; base < n_words; base += n_waves * 64) {
+   // the next 64 words are loaded while a step's rows are counted
+   auto words = [&](uint32_t base) -> uint64_t {
       const uint32_t wd = base + lane;
       const uint32_t wc = wd < n_words ? wd : n_words - 1;
       // bits outside the row layout (left by Threshold's complement of negated children) have no
       // entries in the index: the dense path counts them covered with the local reference
-      uint64_t x = fw[wc] & layoutMask(toGlobal<GlobalU32>(chunk_sizes)[wc >> 10], wc & (kChunkWords - 1));
-      if (wd >= n_words) x = 0;
+      const uint64_t x = fw[wc] & layoutMask(toGlobal<GlobalU32>(chunk_sizes)[wc >> 10], wc & (kChunkWords - 1));
+      return wd < n_words ? x : 0ull;
+   };
+   uint64_t next = wave_id * 64 < n_words ? words(wave_id * 64) : 0ull;
+   for (uint32_t base = wave_id * 64; base < n_words; base += n_waves * 64) {
+      uint64_t x = next;
+      if (base + n_waves * 64 < n_words) next = words(base + n_waves * 64);
       uint64_t pending = __ballot(x != 0);
       while (pending) {
          // up to kRowsPerStep rows at a time, 64 / kRowsPerStep lanes each
@@ -2274,7 +2280,10 @@ __global__ __launch_bounds__(kThreads) void mutationsCountOctetKernel(MutationsB
    }
 }
 
-__global__ __launch_bounds__(256) void sparseCountBatchKernel(MutationsBatch m) {
+// Work-groups of eight wavefronts: one LDS table sums a key over the rows of 512 filter words
+// before its single global atomic, and a CU holds 32 wavefronts of it.
+constexpr uint32_t kSparseBatchThreads = 512;
+__global__ __launch_bounds__(kSparseBatchThreads) void sparseCountBatchKernel(MutationsBatch m) {
    __shared__ uint32_t keys[kSparseTable];
    __shared__ int32_t vals[kSparseTable];
    const uint32_t e = blockIdx.y;
@@ -2848,7 +2857,8 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
    if (m.sparse_max < 0) return hipSuccess;
    const uint32_t n_words = m.n_chunks * kChunkWords;
    if (n_words == 0) return hipSuccess;  // no chunks: no rows
-   hipLaunchKernelGGL(sparseCountBatchKernel, dim3((n_words + 255) / 256, m.max_queries), dim3(256), 0, stream, m);
+   hipLaunchKernelGGL(sparseCountBatchKernel, dim3((n_words + kSparseBatchThreads - 1) / kSparseBatchThreads, m.max_queries),
+                      dim3(kSparseBatchThreads), 0, stream, m);
    if ((err = hipGetLastError()) != hipSuccess) return err;
    hipLaunchKernelGGL(mutationsTotalsBatchKernel<SparseSource>, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0,
                       stream, m, n_pblocks);
