@@ -1177,8 +1177,28 @@ struct BlockValues {
 struct SlotSource {
    const SeqColumnView* col;
    const uint32_t* partial;
+   // further parts of the batched shared scans, summed here: part two, parts three and four
+   // (`n_more` planes `plane` words apart) and the heavy-container sums (zeroed when staged with
+   // `clear`, for the next batch)
+   const uint32_t* partial_b = nullptr;
+   const uint32_t* partial_c = nullptr;
+   size_t plane = 0;
+   uint32_t n_more = 0;
+   uint32_t* heavy = nullptr;
 
-   __device__ void stage(uint32_t p0, BlockValues& bv, bool) const {
+   __device__ uint32_t value(uint32_t s, bool clear) const {
+      uint32_t v = partial[s];
+      if (partial_b) v += partial_b[s];
+      for (uint32_t k = 0; k < n_more; ++k) v += partial_c[k * plane + s];
+      if (heavy) {
+         const uint32_t h = heavy[s];
+         v += h;
+         if (clear && h) heavy[s] = 0;
+      }
+      return v;
+   }
+
+   __device__ void stage(uint32_t p0, BlockValues& bv, bool clear) const {
       const uint32_t t = threadIdx.x;
       const uint32_t ns = col->n_symbols;
       for (uint32_t k = 0; k < ns; ++k) bv.v[k][t] = 0;
@@ -1189,12 +1209,11 @@ struct SlotSource {
       const uint32_t L = col->genome_length;
       const GlobalU32* pos_ptr = toGlobal<GlobalU32>(col->pos_slot_ptr);
       const GlobalU32* info = toGlobal<GlobalU32>(col->slot_info);
-      const GlobalU32* part = toGlobal<GlobalU32>(partial);
       const uint32_t s0 = pos_ptr[p0];
       const uint32_t s1 = pos_ptr[p0 + kPosBlock < L ? p0 + kPosBlock : L];
       for (uint32_t s = s0 + t; s < s1; s += kPosBlock) {
          const uint32_t x = info[s];
-         const uint32_t v = part[s];
+         const uint32_t v = value(s, clear);
          const uint32_t pl = (x & 0xFFFFu) - p0;
          const uint32_t sym = x >> 16;
          if (sym < ns) {
@@ -1217,13 +1236,12 @@ struct SlotSource {
       const uint32_t ns = col->n_symbols;
       const GlobalU32* pos_ptr = toGlobal<GlobalU32>(col->pos_slot_ptr);
       const GlobalU32* info = toGlobal<GlobalU32>(col->slot_info);
-      const GlobalU32* part = toGlobal<GlobalU32>(partial);
       const uint32_t s0 = pos_ptr[p0];
       const uint32_t s1 = pos_ptr[p0 + kPosBlock < L ? p0 + kPosBlock : L];
       for (uint32_t s = s0 + threadIdx.x; s < s1; s += kPosBlock) {
          const uint32_t sym = info[s] >> 16;
          if (sym < ns) continue;
-         const int64_t v = part[s];
+         const int64_t v = value(s, false);
          switch (sym - ns) {
             case BND_COV_START: a += v; break;
             case BND_COV_END: b += v; break;
@@ -1341,15 +1359,28 @@ __global__ __launch_bounds__(kPosBlock) void mutationsTotalsKernel(Src src, cons
    totalsBody(src, blockIdx.x, totals);
 }
 
+// LDS of the finalize step, one instance per kernel whichever sources it finalizes
+struct FinalizeShared {
+   BlockValues bv;
+   int64_t scan_tmp[kPosWaves];
+   int64_t part[4][kPosWaves];
+   int64_t prefix[4];  // previous-block sums of a, b, c and the total of a
+};
+__device__ inline FinalizeShared& finalizeShared() {
+   __shared__ FinalizeShared sh;
+   return sh;
+}
+
 template <typename Src>
 __device__ __forceinline__ void finalizeBody(const Src& src, uint32_t pblock, const int64_t* __restrict__ totals,
                                              uint32_t n_pblocks, uint64_t filter_card, double min_proportion,
                                              uint32_t* __restrict__ counts_out, MutationRowDev* __restrict__ rows,
                                              uint32_t row_capacity, uint32_t* __restrict__ n_rows) {
-   __shared__ BlockValues bv;
-   __shared__ int64_t scan_tmp[kPosWaves];
-   __shared__ int64_t part[4][kPosWaves];
-   __shared__ int64_t prefix[4];  // previous-block sums of a, b, c and the total of a
+   FinalizeShared& sh = finalizeShared();
+   BlockValues& bv = sh.bv;
+   int64_t* scan_tmp = sh.scan_tmp;
+   int64_t (*part)[kPosWaves] = sh.part;
+   int64_t* prefix = sh.prefix;
    const SeqColumnView& col = *src.col;
    {
       // block totals: thread k < n_pblocks (<= kPosBlock for genome lengths below 2^16) takes block k
@@ -1484,9 +1515,10 @@ __global__ __launch_bounds__(kMutThreads) __attribute__((amdgpu_waves_per_eu(kCo
 // counted as in the one-query kernel, into four 16-bit counters per slot held in one u64 (one LDS
 // atomic per run of a slot).  A counter gathers at most kSharedFlushChunks chunks x 4096 rows before
 // the work-group adds it to its part's slot sums (which it owns), so it cannot overflow.  Bitset and
-// run containers are counted per query and added with global atomics to heavy_sums; a second
-// kernel then folds the second part and heavy_sums into partial.  The next chunk's filter words are
-// loaded while the current tile is counted.
+// run containers are counted per query (two wavefronts per bitset, a half each) and added with
+// global atomics to heavy_sums; the totals and finalize kernels add the second part and heavy_sums
+// to partial as they read it.  The next chunk's filter words are loaded while the current tile is
+// counted.
 constexpr uint32_t kSharedThreads = 512;
 constexpr uint32_t kSharedFlushChunks = 15;
 
@@ -1796,21 +1828,6 @@ __global__ __launch_bounds__(kSharedThreads) __attribute__((amdgpu_waves_per_eu(
    flush();
 }
 
-// partial[e][s] += partial_b[e][s] (+ the further parts of the octet scan) + heavy_sums[e][s] and
-// clears heavy_sums, for every dense list entry (after a shared-scan kernel)
-__global__ __launch_bounds__(256) void foldSharedPartsKernel(MutationsBatch m, uint32_t n_slots, uint32_t n_parts) {
-   const uint32_t e = blockIdx.y;
-   if (e >= m.list_counts[0]) return;
-   const size_t plane = static_cast<size_t>(m.max_queries) * n_slots;
-   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n_slots; s += gridDim.x * blockDim.x) {
-      const size_t i = static_cast<size_t>(e) * n_slots + s;
-      const uint32_t h = m.heavy_sums[i];
-      uint32_t v = m.partial[i] + m.partial_b[i] + h;
-      for (uint32_t p = 2; p < n_parts; ++p) v += m.partial_c[(p - 2) * plane + i];
-      m.partial[i] = v;
-      if (h) m.heavy_sums[i] = 0;
-   }
-}
 
 // ---- octet shared scan: eight dense queries per pass over the index -----------------------------
 // A pre-pass turns the filters of dense list entries 8o .. 8o+7 into one byte per row (bit k: the row
@@ -1821,7 +1838,7 @@ __global__ __launch_bounds__(256) void foldSharedPartsKernel(MutationsBatch m, u
 // run sum with one 64-bit add, for all eight queries at once.  Runs end in two 64-bit LDS
 // atomics on counters of four 16-bit fields (flushed to the part's slot sums every 15 chunks, so
 // they cannot overflow).  Bitset and run containers AND against the eight filters' words in global
-// memory and add to heavy_sums.  The parts' sums are folded by foldSharedPartsKernel.
+// memory and add to heavy_sums.  The finalize step adds the parts' sums as it reads them.
 // While a tile is counted, the next tile's row bytes and the first round of its light entries and
 // groups are loaded into registers; the tile pointers of the item's chunks are read into LDS when
 // the item starts.
@@ -2302,36 +2319,58 @@ __global__ __launch_bounds__(kSparseBatchThreads) void sparseCountBatchKernel(Mu
    tb.flush();
 }
 
-__device__ inline SlotSource batchSource(const MutationsBatch& m, uint32_t e, SlotSource*) {
-   return SlotSource{m.col_dev, m.partial + static_cast<size_t>(e) * m.col_dev->n_slots};
+// Dense list entry e's slot sums: the parts of the shared scans (`dense_parts`: 1 for the one-pass-
+// per-query kernel, 2 for the four-query scan, 4 for the eight-query scan) plus the heavy sums.
+__device__ inline SlotSource denseSource(const MutationsBatch& m, uint32_t e, uint32_t dense_parts) {
+   const size_t n_slots = m.col_dev->n_slots;
+   SlotSource src{m.col_dev, m.partial + e * n_slots};
+   if (dense_parts > 1) {
+      src.partial_b = m.partial_b + e * n_slots;
+      src.heavy = m.heavy_sums + e * n_slots;
+   }
+   if (dense_parts > 2) {
+      src.partial_c = m.partial_c + e * n_slots;
+      src.plane = static_cast<size_t>(m.max_queries) * n_slots;
+      src.n_more = dense_parts - 2;
+   }
+   return src;
 }
-__device__ inline SparseSource batchSource(const MutationsBatch& m, uint32_t e, SparseSource*) {
+__device__ inline SparseSource sparseSource(const MutationsBatch& m, uint32_t e) {
    const size_t L = m.col_dev->genome_length;
    return SparseSource{m.col_dev, m.sparse_counts + static_cast<size_t>(e) * m.col_dev->n_symbols * L,
                        m.sparse_bnd + static_cast<size_t>(e) * 3 * L};
 }
-__device__ inline uint32_t batchList(const SlotSource*) { return 0; }
-__device__ inline uint32_t batchList(const SparseSource*) { return 1; }
 
-template <typename Src>
-__global__ __launch_bounds__(kPosBlock) void mutationsTotalsBatchKernel(MutationsBatch m, uint32_t n_pblocks) {
-   const uint32_t list = batchList(static_cast<Src*>(nullptr));
-   const uint32_t e = blockIdx.y;
-   if (e >= m.list_counts[list]) return;
-   const Src src = batchSource(m, e, static_cast<Src*>(nullptr));
-   totalsBody(src, blockIdx.x, m.totals + (static_cast<size_t>(list) * m.max_queries + e) * n_pblocks * 3);
+// Both lists in one launch: work-group row y below the dense count is dense entry y, the rows after
+// it the sparse entries.
+__global__ __launch_bounds__(kPosBlock) void mutationsTotalsBatchKernel(MutationsBatch m, uint32_t n_pblocks,
+                                                                        uint32_t dense_parts) {
+   const uint32_t n_dense = m.list_counts[0];
+   const uint32_t y = blockIdx.y;
+   if (y < n_dense) {
+      totalsBody(denseSource(m, y, dense_parts), blockIdx.x, m.totals + static_cast<size_t>(y) * n_pblocks * 3);
+   } else if (y - n_dense < m.list_counts[1]) {
+      const uint32_t e = y - n_dense;
+      totalsBody(sparseSource(m, e), blockIdx.x, m.totals + (static_cast<size_t>(m.max_queries) + e) * n_pblocks * 3);
+   }
 }
 
-template <typename Src>
-__global__ __launch_bounds__(kPosBlock) void mutationsFinalizeBatchKernel(MutationsBatch m, uint32_t n_pblocks) {
-   const uint32_t list = batchList(static_cast<Src*>(nullptr));
-   const uint32_t e = blockIdx.y;
-   if (e >= m.list_counts[list]) return;
-   const uint32_t q = m.lists[list * m.max_queries + e];
-   const Src src = batchSource(m, e, static_cast<Src*>(nullptr));
-   finalizeBody(src, blockIdx.x, m.totals + (static_cast<size_t>(list) * m.max_queries + e) * n_pblocks * 3, n_pblocks,
-                m.cards[q], m.min_proportion[q], nullptr, m.rows + static_cast<size_t>(q) * m.row_capacity,
-                m.row_capacity, m.n_rows + q);
+__global__ __launch_bounds__(kPosBlock) void mutationsFinalizeBatchKernel(MutationsBatch m, uint32_t n_pblocks,
+                                                                          uint32_t dense_parts) {
+   const uint32_t n_dense = m.list_counts[0];
+   const uint32_t y = blockIdx.y;
+   if (y < n_dense) {
+      const uint32_t q = m.lists[y];
+      finalizeBody(denseSource(m, y, dense_parts), blockIdx.x, m.totals + static_cast<size_t>(y) * n_pblocks * 3,
+                   n_pblocks, m.cards[q], m.min_proportion[q], nullptr, m.rows + static_cast<size_t>(q) * m.row_capacity,
+                   m.row_capacity, m.n_rows + q);
+   } else if (y - n_dense < m.list_counts[1]) {
+      const uint32_t e = y - n_dense;
+      const uint32_t q = m.lists[m.max_queries + e];
+      finalizeBody(sparseSource(m, e), blockIdx.x, m.totals + (static_cast<size_t>(m.max_queries) + e) * n_pblocks * 3,
+                   n_pblocks, m.cards[q], m.min_proportion[q], nullptr, m.rows + static_cast<size_t>(q) * m.row_capacity,
+                   m.row_capacity, m.n_rows + q);
+   }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -2810,9 +2849,6 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
       }
       if ((err = hipGetLastError()) != hipSuccess) return err;
       if (m.scan_stop && (err = hipEventRecord(m.scan_stop, stream)) != hipSuccess) return err;
-      const uint32_t sblocks = (h.n_slots + 255) / 256;
-      hipLaunchKernelGGL(foldSharedPartsKernel, dim3(sblocks == 0 ? 1 : (sblocks < 64 ? sblocks : 64), m.max_queries),
-                         dim3(256), 0, stream, m, h.n_slots, kOctetParts);
    } else if (mode == 1) {
       // shared scan, four dense queries per pass, chunks in two parts
       const uint32_t n_xblocks = 8 * ((h.n_blocks + 7) / 8);
@@ -2824,9 +2860,6 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
       }
       if ((err = hipGetLastError()) != hipSuccess) return err;
       if (m.scan_stop && (err = hipEventRecord(m.scan_stop, stream)) != hipSuccess) return err;
-      const uint32_t sblocks = (h.n_slots + 255) / 256;
-      hipLaunchKernelGGL(foldSharedPartsKernel, dim3(sblocks == 0 ? 1 : (sblocks < 64 ? sblocks : 64), m.max_queries),
-                         dim3(256), 0, stream, m, h.n_slots, 2u);
    } else {
       const uint32_t n_xblocks = 8 * ((h.n_blocks + 7) / 8);
       const dim3 count_grid(n_xblocks * m.max_queries);
@@ -2847,24 +2880,21 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
       if (m.scan_stop && (err = hipEventRecord(m.scan_stop, stream)) != hipSuccess) return err;
    }
    if ((err = hipGetLastError()) != hipSuccess) return err;
-   const uint32_t n_pblocks = (h.genome_length + kPosBlock - 1) / kPosBlock;
-   hipLaunchKernelGGL(mutationsTotalsBatchKernel<SlotSource>, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0, stream,
-                      m, n_pblocks);
-   if ((err = hipGetLastError()) != hipSuccess) return err;
-   hipLaunchKernelGGL(mutationsFinalizeBatchKernel<SlotSource>, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0,
-                      stream, m, n_pblocks);
-   if ((err = hipGetLastError()) != hipSuccess) return err;
-   if (m.sparse_max < 0) return hipSuccess;
+   // the sparse entries' counts, then totals and finalize of both lists (the dense slot sums add the
+   // scan's parts and heavy sums as they are read)
    const uint32_t n_words = m.n_chunks * kChunkWords;
-   if (n_words == 0) return hipSuccess;  // no chunks: no rows
-   hipLaunchKernelGGL(sparseCountBatchKernel, dim3((n_words + kSparseBatchThreads - 1) / kSparseBatchThreads, m.max_queries),
-                      dim3(kSparseBatchThreads), 0, stream, m);
+   if (m.sparse_max >= 0 && n_words > 0) {
+      hipLaunchKernelGGL(sparseCountBatchKernel, dim3((n_words + kSparseBatchThreads - 1) / kSparseBatchThreads, m.max_queries),
+                         dim3(kSparseBatchThreads), 0, stream, m);
+      if ((err = hipGetLastError()) != hipSuccess) return err;
+   }
+   const uint32_t dense_parts = mode == 2 ? kOctetParts : (mode == 1 ? 2u : 1u);
+   const uint32_t n_pblocks = (h.genome_length + kPosBlock - 1) / kPosBlock;
+   hipLaunchKernelGGL(mutationsTotalsBatchKernel, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0, stream, m,
+                      n_pblocks, dense_parts);
    if ((err = hipGetLastError()) != hipSuccess) return err;
-   hipLaunchKernelGGL(mutationsTotalsBatchKernel<SparseSource>, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0,
-                      stream, m, n_pblocks);
-   if ((err = hipGetLastError()) != hipSuccess) return err;
-   hipLaunchKernelGGL(mutationsFinalizeBatchKernel<SparseSource>, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0,
-                      stream, m, n_pblocks);
+   hipLaunchKernelGGL(mutationsFinalizeBatchKernel, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0, stream, m,
+                      n_pblocks, dense_parts);
    return hipGetLastError();
 }
 
