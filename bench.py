@@ -336,7 +336,7 @@ def ingest(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--rows", type=int, default=None)
@@ -344,7 +344,7 @@ def main():
     ap.add_argument("--cpu-all-seconds", type=float, default=12.0, help="all-thread CPU baseline leg (seconds)")
     ap.add_argument("--cpu-threads", type=int, default=16,
                     help="threads of the all-thread CPU leg (at most the host's CPUs)")
-    ap.add_argument("--steady-steps", type=int, default=200,
+    ap.add_argument("--steady-steps", type=int, default=1000,
                     help="pipelined steps of the steady-state figure reported beside the headline (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-query path (the batched one still runs)")

@@ -160,7 +160,7 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
         engine.set_tuning("sparse_max_rows", thr)
         say(f"batch dense/sparse switch, ms: {[(k, round(v * 1e3, 3)) for k, v in sw.items()]}; chose {thr}")
     def timed(lanes, steps):
-        qps, rows = pipelined(table, queries, lanes, steps=steps)
+        qps, rows = pipelined(table, queries, lanes, steps=steps, reps=3)
         if rows != ref_rows:
             raise RuntimeError("pipelined batch results differ from the per-query kernel's")
         return qps
@@ -182,7 +182,7 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
     lanes_qps = {}
     for lanes in sorted({min(n, fit) for n in (1, 2, 4, 6, 8)}):
         engine.set_tuning("batch_lanes", lanes)
-        lanes_qps[lanes] = timed(lanes, max(32, 8 * lanes))
+        lanes_qps[lanes] = timed(lanes, max(64, 16 * lanes))
     # fewer lanes (less memory, shorter queues) unless more are faster beyond the noise
     best_lanes = min(lanes_qps)
     for lanes in sorted(lanes_qps):
@@ -196,14 +196,14 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
         engine.set_tuning("batch_shared_scan", mode[0])
         engine.set_tuning("batch_octet_threads", mode[1])
         engine.set_tuning("batch_octet_groups", mode[2])
-        mode_qps[mode] = timed(best_lanes, max(32, 8 * best_lanes))
+        mode_qps[mode] = timed(best_lanes, max(64, 16 * best_lanes))
     best = pick(mode_qps, (best[0], best[1], 0))
     engine.set_tuning("batch_shared_scan", best[0])
     engine.set_tuning("batch_octet_threads", best[1])
     engine.set_tuning("batch_octet_groups", best[2])
     say(f"batches in flight, queries/s: {[(k, round(v)) for k, v in lanes_qps.items()]}; chose {best_lanes}; "
         f"dense scans at that concurrency: {[(k, round(v)) for k, v in mode_qps.items()]}; chose {best}")
-    steps = max(32, 8 * best_lanes)
+    steps = max(64, 16 * best_lanes)
     # the dense/sparse switch and the job size (queries per lane job; the batch of `size` is cut into
     # size / job jobs) at that concurrency
     if cards is not None:
