@@ -73,13 +73,25 @@ def copy_calibration(mib):
 
 
 def cpu_model() -> str:
+    """The host CPU's model name with its family/model/stepping, clock and vector extensions (a
+    virtualised host may report only a brand name)."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                info.setdefault(k.strip(), v.strip())
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    flags = set(info.get("flags", "").split())
+    vec = [f for f in ("avx512f", "avx512vpopcntdq", "avx2") if f in flags]
+    detail = f"family {info.get('cpu family', '?')} model {info.get('model', '?')} stepping {info.get('stepping', '?')}"
+    if info.get("cpu MHz"):
+        detail += f", {float(info['cpu MHz']) / 1000:.2f} GHz"
+    if vec:
+        detail += ", " + " ".join(vec)
+    return f"{name} ({detail})"
 
 
 def oracle_query(ot, q):
