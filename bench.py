@@ -256,6 +256,23 @@ def serve(args):
     identical = all(as_rows([run_plain_query(et, reqs[i])])[0] == results[i] for i in sample)
     plain_qps = len(sample) / (time.perf_counter() - t0)
     log(f"alone == in the mix for {len(sample)} sampled requests: {identical}; single-query path {plain_qps:.1f}/s")
+    # capacity: the offered rate raised until p99 passes the target or throughput falls behind
+    sweep = []
+    capacity = None
+    for rate in [float(r) for r in args.sweep_rates.split(",") if r.strip()]:
+        n_req = int(max(2000, rate * args.sweep_seconds))
+        sreqs = serving.config4_requests(ds, genes, n=n_req, seed=int(rate) + 7)
+        sarr = serving.poisson_arrivals(n_req, rate=rate, seed=int(rate) + 8)
+        _, slat, swall, ssizes = srv.run(sreqs, sarr)
+        point = {"offered_rps": rate, "served_rps": round(n_req / swall, 1),
+                 "p50_ms": round(float(np.percentile(slat, 50)) * 1e3, 3),
+                 "p99_ms": round(float(np.percentile(slat, 99)) * 1e3, 3),
+                 "mean_batch": round(float(np.mean(ssizes)), 1), "requests": n_req}
+        sweep.append(point)
+        log(f"capacity sweep: {point}")
+        if point["p99_ms"] > args.p99_target_ms or point["served_rps"] < 0.95 * rate:
+            break
+        capacity = point
     mix = {k: sum(q["action"] == k for q in reqs) for k in ("mutations", "aggregate", "count")}
     line = {
         "metric": METRIC, "value": round(qps, 3), "unit": "queries/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -270,6 +287,8 @@ def serve(args):
         "batches": len(sizes), "mean_batch": round(float(np.mean(sizes)), 2),
         "alone_equals_mix": {"sampled": len(sample), "identical": bool(identical)},
         "single_query_path_qps": round(plain_qps, 2), "upload_s": round(upload_s, 2),
+        "capacity": {"p99_target_ms": args.p99_target_ms, "max_sustained_rps": capacity["offered_rps"] if capacity else None,
+                     "at_capacity": capacity, "sweep": sweep},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -348,7 +367,8 @@ def ingest(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 200 for 64-query steps, 20 for the 1,024-tree steps of configs 2-3)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--rows", type=int, default=None)
@@ -361,15 +381,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the per-query path (the batched one still runs)")
     ap.add_argument("--no-batched", action="store_true", help="only the per-query path (for profiling its kernels)")
-    ap.add_argument("--planner-trees", type=int, default=20000, help="config 3: training trees (200000 in full)")
-    ap.add_argument("--planner-eval", type=int, default=2000, help="config 3: held-out trees (10000 in full)")
+    ap.add_argument("--planner-trees", type=int, default=200000, help="config 3: training trees")
+    ap.add_argument("--planner-eval", type=int, default=10000, help="config 3: held-out trees")
     ap.add_argument("--planner-epochs", type=int, default=30)
     ap.add_argument("--requests", type=int, default=4000, help="config 4: requests served")
     ap.add_argument("--rate", type=float, default=3000.0, help="config 4: Poisson arrival rate (requests/s)")
+    ap.add_argument("--sweep-rates", default="2000,4000,6000,8000,10000,12000,16000,20000,24000,32000",
+                    help="config 4: offered rates of the capacity sweep, ascending (empty: no sweep)")
+    ap.add_argument("--sweep-seconds", type=float, default=1.5, help="config 4: arrivals per sweep point, in seconds")
+    ap.add_argument("--p99-target-ms", type=float, default=50.0, help="config 4: p99 latency bound of the sweep")
     ap.add_argument("--ingest", action="store_true", help="time the index build of the main column instead")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (scales PMC byte counters)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 20 if args.config in (2, 3) else 200
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine runs on a single MI355X (no multi-GPU path); "
               "use --gpus 1", file=sys.stderr)
@@ -512,7 +538,10 @@ def main():
         t_samples = time.time() - t0
         mlp = eng.mlp(n_in=32, n_hidden=128, n_hidden_layers=3, seed=7)
         t0 = time.time()
-        loss = mlp.train(x, y, epochs=args.planner_epochs, batch=4096, lr=1e-3, seed=8)
+        loss = None
+        for ep in range(args.planner_epochs):  # one call per epoch (Adam state persists): a line each
+            loss = mlp.train(x, y, epochs=1, batch=4096, lr=1e-3, seed=8 + ep)
+            log(f"planner training: epoch {ep + 1}/{args.planner_epochs}, mse {loss:.5f}, {time.time() - t0:.1f}s")
         t_train = time.time() - t0
         ev = planner_train.evaluate(et, mlp, held_out)
         sp = planner_train.speedup(eng, et, queries, mlp)
@@ -565,7 +594,9 @@ def main():
     # drain with several batches in flight weigh little here); reported beside `value`
     steady = None
     if args.steady_steps > 0:
-        n_steady = max(args.steady_steps, 16 * in_flight)
+        # at most about 10 s of steps (a 1,024-tree step takes 100 ms)
+        per_step = b_elapsed / args.steps
+        n_steady = max(16 * in_flight, min(args.steady_steps, int(10.0 / max(per_step, 1e-6))))
         pend = deque()
         t_st = time.perf_counter()
         for _ in range(n_steady):

@@ -498,9 +498,12 @@ void batchSubmit(Table& t, BatchResult& b) {
    const uint32_t n = static_cast<uint32_t>(b.queries.size());
    std::vector<CompiledQuery> cq(n);
    std::vector<query::OperatorPtr> ops(n);
+   // host threads for the per-query work of large batches (a small batch's work is shorter than
+   // starting the threads)
+   const unsigned threads = n >= kParallelCompileQueries ? 0u : 1u;
    // host side: parse, rewrite and compile every filter; a query that fails keeps its status and
    // runs as the empty program
-   for (uint32_t i = 0; i < n; ++i) {
+   parallelFor(n, [&](size_t i) {
       BatchQueryResult& r = b.queries[i];
       try {
          if (r.action == RDB_ACTION_MUTATIONS) {
@@ -517,20 +520,41 @@ void batchSubmit(Table& t, BatchResult& b) {
          r.status = e.code;
          r.message = e.what();
       }
+   }, threads);
+   // the planner orders conjuncts and disjuncts and turns on short-circuit (results unchanged); the
+   // compiled order's stack depth is kept to see whether a plan needs a deeper stack
+   const bool planned = ctx.plan_mode != plan::REFERENCE;
+   std::vector<uint32_t> ref_depth(planned ? n : 0, 1);
+   if (planned) {
+      parallelFor(n, [&](size_t i) {
+         if (!ops[i]) return;
+         query::Emitter e;
+         ops[i]->emit(e);
+         ref_depth[i] = std::max<uint32_t>(e.max_sp, 1);
+      }, threads);
    }
-   // the planner orders conjuncts and disjuncts and turns on short-circuit (results unchanged)
-   if (ctx.plan_mode != plan::REFERENCE) {
+   if (planned) {
       std::vector<query::Operator*> roots;
       for (auto& op : ops) {
          if (op) roots.push_back(op.get());
       }
-      plan::planAll(t, roots, static_cast<plan::Mode>(ctx.plan_mode), ctx.planner);
+      plan::planAll(t, roots, static_cast<plan::Mode>(ctx.plan_mode), ctx.planner, threads);
    }
-   for (uint32_t i = 0; i < n; ++i) {
+   parallelFor(n, [&](size_t i) {
       BatchQueryResult& r = b.queries[i];
       if (r.status == RDB_OK) {
          try {
             ops[i]->emit(cq[i].program);
+            const uint32_t max_depth = maxFilterDepth(ctx);
+            if (planned && depthBucket(std::max<uint32_t>(cq[i].program.max_sp, 1), max_depth) >
+                              depthBucket(ref_depth[i], max_depth)) {
+               // the plan's order needs a deeper stack, i.e. more LDS per filter work-group and so
+               // fewer of them per CU, than the compiled order: keep that order, with short-circuit
+               auto op = query::compileFilter(t, r.filter);
+               plan::shortCircuitOnly(*op);
+               cq[i].program = query::Emitter();
+               op->emit(cq[i].program);
+            }
             if (cq[i].program.sp != 1) fail(RDB_ECOMPILE, "internal: filter program leaves a wrong stack");
             cq[i].depth = std::max<uint32_t>(cq[i].program.max_sp, 1);
             if (cq[i].depth > maxFilterDepth(ctx)) fail(RDB_EQUERY, "filter expression nests too deeply");
@@ -544,7 +568,7 @@ void batchSubmit(Table& t, BatchResult& b) {
          cq[i].program.leaf(Instr{OP_EMPTY, 0, 0, 0, 0, 0, 0, 0});
          cq[i].depth = 1;
       }
-   }
+   }, threads);
    try {
       submitCompiled(t, b, cq);
    } catch (...) {

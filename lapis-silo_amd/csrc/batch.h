@@ -21,6 +21,9 @@ namespace rdb {
 
 struct BatchResult;
 
+// Batches of at least this many queries compile, plan and emit their filters on host threads.
+constexpr uint32_t kParallelCompileQueries = 256;
+
 struct BatchLane {
    Table* table = nullptr;
    hipStream_t stream = nullptr;

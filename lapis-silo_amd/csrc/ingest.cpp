@@ -187,22 +187,33 @@ DeviceIndexBuild::DeviceIndexBuild(Table& t, const SeqColumn& col, const uint8_t
    rows_.mut_sym = upload(bufs_, mut_sym, mut_ptr[n]);
    rows_.nrun_ptr = upload(bufs_, nrun_ptr, n + 1);
    rows_.runs = upload(bufs_, runs, nrun_ptr[n]);
+   mut_total_ = mut_ptr[n];
+   run_total_ = nrun_ptr[n];
    rows_.reference = upload(bufs_, col.reference.data(), col.genome_length);
 }
 
 void DeviceIndexBuild::adapt(std::vector<uint32_t>& coverage, std::vector<uint8_t>& local_ref) {
    const hipStream_t stream = t_.ctx->stream;
    const uint32_t L = rows_.length;
+   const uint64_t n = rows_.n_rows;
    std::vector<DeviceBuffer> tmp;
    AdaptCount a;
    a.rows = rows_;
-   a.changes = alloc<long long>(tmp, L + 1);
-   a.sym_counts = alloc<uint32_t>(tmp, static_cast<size_t>(L) * rows_.n_symbols);
+   a.key_count = mut_total_ + 2 * run_total_ + 2 * n;
+   if (a.key_count > static_cast<uint64_t>(INT_MAX)) fail(RDB_ENOMEM, "too many rows for the device adaptation sort");
+   a.keys = alloc<uint32_t>(tmp, a.key_count);
+   a.keys_alt = alloc<uint32_t>(tmp, a.key_count);
+   const size_t n_keys = (static_cast<size_t>(L) + 1) * 32;
+   a.run_start = alloc<uint32_t>(tmp, n_keys);
+   a.run_end = alloc<uint32_t>(tmp, n_keys);
    a.coverage = alloc<uint32_t>(tmp, L);
    a.local_ref = alloc<uint8_t>(tmp, L);
-   hipCheck(hipMemsetAsync(a.changes, 0, (L + 1) * sizeof(long long), stream), "clear");
-   hipCheck(hipMemsetAsync(a.sym_counts, 0, static_cast<size_t>(L) * rows_.n_symbols * 4, stream), "clear");
-   hipCheck(launchAdapt(a, stream), "adaptation launch");
+   hipCheck(hipMemsetAsync(a.run_start, 0, n_keys * 4, stream), "clear");
+   hipCheck(hipMemsetAsync(a.run_end, 0, n_keys * 4, stream), "clear");
+   size_t temp_bytes = 0;
+   hipCheck(launchAdapt(a, nullptr, &temp_bytes, stream), "adaptation sort size");
+   DeviceBuffer temp(std::max<size_t>(temp_bytes, 16));
+   hipCheck(launchAdapt(a, temp.as<void>(), &temp_bytes, stream), "adaptation launch");
    download(coverage, a.coverage, L, stream);
    download(local_ref, a.local_ref, L, stream);
    hipCheck(hipStreamSynchronize(stream), "adaptation");

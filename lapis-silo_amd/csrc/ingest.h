@@ -29,6 +29,7 @@ class DeviceIndexBuild {
   private:
    Table& t_;
    DiffRowsDev rows_{};
+   uint64_t mut_total_ = 0, run_total_ = 0;
    std::vector<DeviceBuffer> bufs_;
 };
 

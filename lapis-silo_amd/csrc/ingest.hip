@@ -107,37 +107,44 @@ __global__ __launch_bounds__(64 * kDiffWaves) void diffKernel(DiffBatch d) {
    }
 }
 
-// Per row: +1 at the covered range's start, -1 at its end, -1 / +1 at each internal run's start /
-// end (the coverage at p is the prefix sum), and one count per mutation (position, symbol).  The
-// adds at position 0 and at the genome's end, which most rows make, are summed per wavefront first.
-__global__ void adaptCountKernel(AdaptCount a) {
+// Row i's adaptation keys (see AdaptCount) at mut_ptr[i] + 2 nrun_ptr[i] + 2 i.
+__global__ void adaptKeysKernel(AdaptCount a) {
    const DiffRowsDev& R = a.rows;
    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-   const bool live = i < R.n_rows && !R.is_null[i];
-   const uint32_t s = live ? R.cov_start[i] : 0, e = live ? R.cov_end[i] : 0;
+   if (i >= R.n_rows) return;
+   const uint32_t past = (R.length + 1) << 5;  // sorts after every real key
+   uint32_t* k = a.keys + R.mut_ptr[i] + 2 * R.nrun_ptr[i] + 2 * i;
+   const bool live = !R.is_null[i];
+   const uint32_t s = R.cov_start[i], e = R.cov_end[i];
    const bool cov = live && s < e;
-   const uint64_t at0 = __ballot(cov && s == 0), atL = __ballot(cov && e == R.length);
-   if ((threadIdx.x & 63) == 0) {
-      if (at0) atomicAdd(reinterpret_cast<unsigned long long*>(&a.changes[0]), static_cast<unsigned long long>(__popcll(at0)));
-      if (atL) atomicAdd(reinterpret_cast<unsigned long long*>(&a.changes[R.length]), 0ull - static_cast<unsigned long long>(__popcll(atL)));
+   k[0] = cov ? (s << 5 | kAdaptCodeUp) : past;
+   k[1] = cov ? (e << 5 | kAdaptCodeDown) : past;
+   k += 2;
+   // a null row counts nothing: whatever it lists fills its keys with `past`
+   for (uint64_t r = R.nrun_ptr[i]; r < R.nrun_ptr[i + 1]; ++r) {
+      *k++ = live ? (R.runs[r] & 0xFFFFu) << 5 | kAdaptCodeDown : past;
+      *k++ = live ? (R.runs[r] >> 16) << 5 | kAdaptCodeUp : past;
    }
-   if (!live) return;
-   if (cov) {
-      if (s != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&a.changes[s]), 1ull);
-      if (e != R.length) atomicAdd(reinterpret_cast<unsigned long long*>(&a.changes[e]), ~0ull);
-   }
-   for (uint64_t k = R.nrun_ptr[i]; k < R.nrun_ptr[i + 1]; ++k) {
-      atomicAdd(reinterpret_cast<unsigned long long*>(&a.changes[R.runs[k] & 0xFFFFu]), ~0ull);
-      atomicAdd(reinterpret_cast<unsigned long long*>(&a.changes[R.runs[k] >> 16]), 1ull);
-   }
-   for (uint64_t k = R.mut_ptr[i]; k < R.mut_ptr[i + 1]; ++k) {
-      atomicAdd(&a.sym_counts[static_cast<size_t>(R.mut_pos[k]) * R.n_symbols + R.mut_sym[k]], 1u);
-   }
+   for (uint64_t m = R.mut_ptr[i]; m < R.mut_ptr[i + 1]; ++m) *k++ = live ? R.mut_pos[m] << 5 | R.mut_sym[m] : past;
 }
 
-// One work-group: coverage[p] = prefix sum of changes; per position the reference symbol's count is
-// the coverage minus the other symbols' counts, and the local reference is the reference symbol
-// unless another symbol's count is strictly larger (the first such symbol with the largest count).
+// Boundaries of the runs of equal sorted keys: run_start[key] and run_end[key] (count = end - start).
+__global__ void adaptRunsKernel(const uint32_t* __restrict__ keys, uint64_t n, uint32_t past, uint32_t* run_start,
+                                uint32_t* run_end) {
+   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+   if (i >= n) return;
+   const uint32_t k = keys[i];
+   if (k >= past) return;
+   if (i == 0 || keys[i - 1] != k) run_start[k] = static_cast<uint32_t>(i);
+   if (i + 1 == n || keys[i + 1] != k) run_end[k] = static_cast<uint32_t>(i + 1);
+}
+
+__device__ inline uint32_t adaptCount(const AdaptCount& a, uint32_t key) { return a.run_end[key] - a.run_start[key]; }
+
+// One work-group: coverage[p] = prefix sum of the coverage events; per position the reference
+// symbol's count is the coverage minus the other symbols' counts, and the local reference is the
+// reference symbol unless another symbol's count is strictly larger (the first such symbol with
+// the largest count).
 constexpr uint32_t kAdaptThreads = 1024;
 __global__ __launch_bounds__(kAdaptThreads) void adaptRuleKernel(AdaptCount a) {
    __shared__ long long part[kAdaptThreads];
@@ -145,8 +152,12 @@ __global__ __launch_bounds__(kAdaptThreads) void adaptRuleKernel(AdaptCount a) {
    const uint32_t L = R.length, nsym = R.n_symbols;
    const uint32_t per = (L + kAdaptThreads - 1) / kAdaptThreads;
    const uint32_t lo = min(L, threadIdx.x * per), hi = min(L, lo + per);
+   auto change = [&](uint32_t p) -> long long {
+      return static_cast<long long>(adaptCount(a, p << 5 | kAdaptCodeUp)) -
+             static_cast<long long>(adaptCount(a, p << 5 | kAdaptCodeDown));
+   };
    long long sum = 0;
-   for (uint32_t p = lo; p < hi; ++p) sum += a.changes[p];
+   for (uint32_t p = lo; p < hi; ++p) sum += change(p);
    part[threadIdx.x] = sum;
    __syncthreads();
    for (uint32_t step = 1; step < kAdaptThreads; step <<= 1) {
@@ -157,19 +168,19 @@ __global__ __launch_bounds__(kAdaptThreads) void adaptRuleKernel(AdaptCount a) {
    }
    long long coverage = part[threadIdx.x] - sum;
    for (uint32_t p = lo; p < hi; ++p) {
-      coverage += a.changes[p];
+      coverage += change(p);
       a.coverage[p] = static_cast<uint32_t>(coverage);
-      const uint32_t* c = a.sym_counts + static_cast<size_t>(p) * nsym;
       const uint32_t current = R.reference[p];
       uint32_t ref_count = static_cast<uint32_t>(coverage);
       for (uint32_t s = 0; s < nsym; ++s) {
-         if (s != current) ref_count -= c[s];
+         if (s != current) ref_count -= adaptCount(a, p << 5 | s);
       }
       uint32_t best = current, best_count = ref_count;
       for (uint32_t s = 0; s < nsym; ++s) {
-         if (s != current && c[s] > best_count) {
+         const uint32_t c = adaptCount(a, p << 5 | s);
+         if (s != current && c > best_count) {
             best = s;
-            best_count = c[s];
+            best_count = c;
          }
       }
       a.local_ref[p] = static_cast<uint8_t>(best);
@@ -248,11 +259,26 @@ hipError_t launchDiff(const DiffBatch& d, bool write, hipStream_t stream) {
    return hipGetLastError();
 }
 
-hipError_t launchAdapt(const AdaptCount& a, hipStream_t stream) {
+hipError_t launchAdapt(const AdaptCount& a, void* temp, size_t* temp_bytes, hipStream_t stream) {
+   const uint32_t past = (a.rows.length + 1) << 5;
+   int end_bit = 1;
+   while ((1u << end_bit) <= past) ++end_bit;
+   hipcub::DoubleBuffer<uint32_t> buf(a.keys, a.keys_alt);
+   if (temp == nullptr) {
+      return hipcub::DeviceRadixSort::SortKeys(nullptr, *temp_bytes, buf, static_cast<int>(a.key_count), 0, end_bit,
+                                               stream);
+   }
+   hipError_t err = hipSuccess;
    if (a.rows.n_rows > 0) {
-      hipLaunchKernelGGL(adaptCountKernel, dim3(static_cast<uint32_t>((a.rows.n_rows + 255) / 256)), dim3(256), 0, stream, a);
-      const hipError_t err = hipGetLastError();
+      hipLaunchKernelGGL(adaptKeysKernel, dim3(static_cast<uint32_t>((a.rows.n_rows + 255) / 256)), dim3(256), 0, stream, a);
+      if ((err = hipGetLastError()) != hipSuccess) return err;
+   }
+   if (a.key_count > 0) {
+      err = hipcub::DeviceRadixSort::SortKeys(temp, *temp_bytes, buf, static_cast<int>(a.key_count), 0, end_bit, stream);
       if (err != hipSuccess) return err;
+      hipLaunchKernelGGL(adaptRunsKernel, dim3(static_cast<uint32_t>((a.key_count + 255) / 256)), dim3(256), 0, stream,
+                         buf.Current(), a.key_count, past, a.run_start, a.run_end);
+      if ((err = hipGetLastError()) != hipSuccess) return err;
    }
    hipLaunchKernelGGL(adaptRuleKernel, dim3(1), dim3(kAdaptThreads), 0, stream, a);
    return hipGetLastError();

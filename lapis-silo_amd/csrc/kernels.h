@@ -223,16 +223,25 @@ struct DiffRowsDev {
    const uint8_t* reference = nullptr;
 };
 
-// Coverage and symbol counts per position, then the local reference of every position.
-// `changes` (length + 1 int64) and `sym_counts` (length * n_symbols) must be zero.
+// Coverage and symbol counts per position, then the local reference of every position, without
+// atomics: every row writes one key `position << 5 | code` per mutation (code = symbol) and per
+// coverage event (code 28: coverage +1 at a range start or run end, 29: -1 at a range end or run
+// start; a row without a covered range writes two keys past the end), the keys are radix-sorted,
+// and a run of equal keys [start, end) is its count.  keys / keys_alt: key_count u32 each (row i's
+// keys at mut_ptr[i] + 2 nrun_ptr[i] + 2 i); run_start / run_end: (length + 1) * 32 u32, zero.
+constexpr uint32_t kAdaptCodeUp = 28, kAdaptCodeDown = 29;
 struct AdaptCount {
    DiffRowsDev rows;
-   long long* changes = nullptr;
-   uint32_t* sym_counts = nullptr;
+   uint32_t* keys = nullptr;
+   uint32_t* keys_alt = nullptr;
+   uint64_t key_count = 0;
+   uint32_t* run_start = nullptr;
+   uint32_t* run_end = nullptr;
    uint32_t* coverage = nullptr;
    uint8_t* local_ref = nullptr;
 };
-hipError_t launchAdapt(const AdaptCount& a, hipStream_t stream);
+// Key writing, sort (temp == null: sets *temp_bytes only), run boundaries and the rule.
+hipError_t launchAdapt(const AdaptCount& a, void* temp, size_t* temp_bytes, hipStream_t stream);
 
 // The container entries of every row (chunk << 40 | position << 24 | symbol << 16 | row in chunk):
 // the count pass writes their number per row, the write pass writes them at offsets[i].

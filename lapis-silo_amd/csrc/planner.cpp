@@ -307,13 +307,14 @@ void apply(std::vector<Node>& nodes, const float* estimates) {
    }
 }
 
-std::vector<std::vector<float>> estimate(const Table& t, std::vector<std::vector<Node>>& trees, Mode mode, Mlp* mlp) {
+std::vector<std::vector<float>> estimate(const Table& t, std::vector<std::vector<Node>>& trees, Mode mode, Mlp* mlp,
+                                         unsigned threads) {
    std::vector<std::vector<float>> est(trees.size());
    for (size_t q = 0; q < trees.size(); ++q) est[q].assign(trees[q].size(), 0.0f);
    if (mode != LEARNED || mlp == nullptr) {
-      for (size_t q = 0; q < trees.size(); ++q) {
+      parallelFor(trees.size(), [&](size_t q) {
          for (size_t k = 0; k < trees[q].size(); ++k) est[q][k] = independenceEstimate(t, trees[q], k, est[q].data());
-      }
+      }, threads);
       return est;
    }
    // heights: leaves 0; a node's height is one more than its highest child
@@ -334,10 +335,16 @@ std::vector<std::vector<float>> estimate(const Table& t, std::vector<std::vector
    for (uint32_t h = 0; h <= max_h; ++h) {
       x.clear();
       who.clear();
+      // the nodes of height h compose their features from their children's estimates (per tree on
+      // the host threads), then one estimator launch takes them all
+      parallelFor(trees.size(), [&](size_t q) {
+         for (size_t k = 0; k < trees[q].size(); ++k) {
+            if (height[q][k] == h) compose(t, trees[q], k, est[q].data());
+         }
+      }, threads);
       for (size_t q = 0; q < trees.size(); ++q) {
          for (size_t k = 0; k < trees[q].size(); ++k) {
             if (height[q][k] != h) continue;
-            compose(t, trees[q], k, est[q].data());
             x.insert(x.end(), trees[q][k].features, trees[q][k].features + kFeatures);
             who.emplace_back(static_cast<uint32_t>(q), static_cast<uint32_t>(k));
          }
@@ -349,13 +356,48 @@ std::vector<std::vector<float>> estimate(const Table& t, std::vector<std::vector
    return est;
 }
 
-void planAll(const Table& t, std::vector<query::Operator*>& roots, Mode mode, Mlp* mlp) {
+void shortCircuitOnly(query::Operator& op) {
+   using namespace query;
+   switch (op.type()) {
+      case INTERSECTION: {
+         auto& o = static_cast<Intersection&>(op);
+         o.short_circuit = true;
+         for (auto& c : o.children) shortCircuitOnly(*c);
+         for (auto& c : o.negated_children) shortCircuitOnly(*c);
+         break;
+      }
+      case UNION: {
+         auto& o = static_cast<Union&>(op);
+         o.short_circuit = true;
+         for (auto& c : o.children) shortCircuitOnly(*c);
+         break;
+      }
+      case COMPLEMENT:
+         shortCircuitOnly(*static_cast<Complement&>(op).child);
+         break;
+      case SELECTION: {
+         auto& o = static_cast<Selection&>(op);
+         if (o.child_operator.has_value()) shortCircuitOnly(**o.child_operator);
+         break;
+      }
+      case THRESHOLD: {
+         auto& o = static_cast<Threshold&>(op);
+         o.short_circuit = true;
+         for (auto& c : o.non_negated_children) shortCircuitOnly(*c);
+         for (auto& c : o.negated_children) shortCircuitOnly(*c);
+         break;
+      }
+      default:
+         break;
+   }
+}
+
+void planAll(const Table& t, std::vector<query::Operator*>& roots, Mode mode, Mlp* mlp, unsigned threads) {
    if (mode == REFERENCE) return;
-   std::vector<std::vector<Node>> trees;
-   trees.reserve(roots.size());
-   for (auto* r : roots) trees.push_back(collect(t, *r));
-   const auto est = estimate(t, trees, mode, mlp);
-   for (size_t q = 0; q < trees.size(); ++q) apply(trees[q], est[q].data());
+   std::vector<std::vector<Node>> trees(roots.size());
+   parallelFor(roots.size(), [&](size_t q) { trees[q] = collect(t, *roots[q]); }, threads);
+   const auto est = estimate(t, trees, mode, mlp, threads);
+   parallelFor(trees.size(), [&](size_t q) { apply(trees[q], est[q].data()); }, threads);
 }
 
 }  // namespace rdb::plan

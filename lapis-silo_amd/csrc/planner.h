@@ -55,9 +55,15 @@ enum Mode { REFERENCE = 0, INDEPENDENCE = 1, LEARNED = 2 };
 // Estimates (normalised log-cardinalities) of every node of every tree, bottom up: the learned
 // estimator composes each node's features from its children's estimates and runs level by level
 // (all trees' nodes of one height per launch); INDEPENDENCE uses independenceEstimate alone.
-std::vector<std::vector<float>> estimate(const Table& t, std::vector<std::vector<Node>>& trees, Mode mode, Mlp* mlp);
+std::vector<std::vector<float>> estimate(const Table& t, std::vector<std::vector<Node>>& trees, Mode mode, Mlp* mlp,
+                                         unsigned threads = 1);
 
 // Plans compiled filters in place (no effect in REFERENCE mode).
-void planAll(const Table& t, std::vector<query::Operator*>& roots, Mode mode, Mlp* mlp);
+// Turns on short-circuit everywhere in the tree, keeping the compiled order of every operator's
+// children.
+void shortCircuitOnly(query::Operator& root);
+
+// `threads`: host threads for the per-tree work (0: as many as useful, 1: this thread only).
+void planAll(const Table& t, std::vector<query::Operator*>& roots, Mode mode, Mlp* mlp, unsigned threads = 1);
 
 }  // namespace rdb::plan

@@ -33,7 +33,7 @@ def samples(et, filters: list, chunk: int = 2000, log=None):
         f, y, _ = plan_samples(et, filters[i:i + chunk])
         xs.append(f)
         ys.append(y)
-        if log and (i // chunk) % 10 == 0:
+        if log:
             log(f"samples: {i + len(filters[i:i + chunk])} trees, {sum(len(a) for a in ys)} nodes, {time.time() - t0:.1f}s")
     return np.concatenate(xs), np.concatenate(ys)
 
@@ -65,23 +65,53 @@ def evaluate(et, mlp, filters: list, chunk: int = 2000) -> dict:
     return out
 
 
-def speedup(engine, et, queries: list, mlp, steps: int = 3) -> dict:
-    """Batched workload under the reference plan and under the learned plan (results as columns, so
-    that the device work dominates the clock); results compared."""
+def speedup(engine, et, queries: list, mlp, steps: int = 3, rounds: int = 5) -> dict:
+    """Batched workload under the reference plan, the independence plan and the learned plan
+    (results as columns, so that the device work dominates the clock), the modes taking turns for
+    `rounds` rounds: `steps` batches one at a time (the host's planning in series with the device)
+    and `steps` batches pipelined over the lanes (one batch's planning beside the others' device
+    work); the median round per mode; results compared."""
+    from collections import deque
+
     from .engine import MutationArrays
 
-    res = {}
+    modes = ("reference", "independence", "learned")
+    in_flight = max(1, engine.get_tuning("batch_lanes"))
+    sync = {m: [] for m in modes}
+    pipe = {m: [] for m in modes}
     outs = {}
-    for mode in ("reference", "independence", "learned"):
-        engine.set_planner(mode, mlp if mode == "learned" else None)
-        et.batch(queries, arrays=True)  # warm-up (graph capture, lanes)
-        t = time.perf_counter()
-        for _ in range(steps):
-            got = et.batch(queries, arrays=True)
-        res[mode] = len(queries) * steps / (time.perf_counter() - t)
-        outs[mode] = [r.rows() if isinstance(r, MutationArrays) else r for r in got]
+    for m in modes:  # warm-up (graph capture, lanes)
+        engine.set_planner(m, mlp if m == "learned" else None)
+        et.batch(queries, arrays=True)
+    for _ in range(rounds):
+        for m in modes:
+            engine.set_planner(m, mlp if m == "learned" else None)
+            t = time.perf_counter()
+            for _ in range(steps):
+                got = et.batch(queries, arrays=True)
+            sync[m].append(time.perf_counter() - t)
+            outs[m] = [r.rows() if isinstance(r, MutationArrays) else r for r in got]
+            pend = deque()
+            t = time.perf_counter()
+            for _ in range(steps):
+                pend.append(et.batch_submit(queries))
+                if len(pend) == in_flight:
+                    b = pend.popleft()
+                    b.results(arrays=True)
+                    b.free()
+            while pend:
+                b = pend.popleft()
+                b.results(arrays=True)
+                b.free()
+            pipe[m].append(time.perf_counter() - t)
     engine.set_planner("reference")
-    return {"queries_per_s": {k: round(v, 2) for k, v in res.items()},
-            "learned_vs_reference": round(res["learned"] / res["reference"], 4),
-            "independence_vs_reference": round(res["independence"] / res["reference"], 4),
+    qs = {m: len(queries) * steps / float(np.median(sync[m])) for m in modes}
+    qp = {m: len(queries) * steps / float(np.median(pipe[m])) for m in modes}
+    return {"queries_per_s": {k: round(v, 2) for k, v in qs.items()},
+            "learned_vs_reference": round(qs["learned"] / qs["reference"], 4),
+            "independence_vs_reference": round(qs["independence"] / qs["reference"], 4),
+            "pipelined_queries_per_s": {k: round(v, 2) for k, v in qp.items()},
+            "pipelined_learned_vs_reference": round(qp["learned"] / qp["reference"], 4),
+            "pipelined_independence_vs_reference": round(qp["independence"] / qp["reference"], 4),
+            "in_flight": in_flight,
             "results_identical": outs["learned"] == outs["reference"] == outs["independence"]}
