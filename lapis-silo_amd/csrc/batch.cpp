@@ -141,7 +141,7 @@ void setupLane(Table& t, BatchLane& lane) {
 // The lane's fixed kernel sequence for programs of at most `depth` stack entries whose Mutations
 // queries read sequence column `column` (-1: none); the histogram kernels only when the job has
 // Aggregated queries.
-void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column, bool aggregates) {
+void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column, bool aggregates, uint32_t grid_q) {
    Context& ctx = *t.ctx;
    const uint32_t Q = lane.max_queries;
    const ParamsLayout pl(Q, lane.instr_capacity, lane.range_capacity);
@@ -153,7 +153,7 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column, bool 
    auto* chunk_cards = reinterpret_cast<uint32_t*>(lane.cards.as<uint8_t>() + 8ULL * Q);
    hipCheck(launchFill(reinterpret_cast<uint32_t*>(cards), 0, lane.cards.bytes() / 4, lane.stream), "clear cardinalities");
    hipCheck(launchFilter(t.view, reinterpret_cast<const ProgramDesc*>(pd + pl.progs), reinterpret_cast<const Instr*>(pd + pl.instrs),
-                         reinterpret_cast<const uint32_t*>(pd + pl.ranges), Q, depth, lane.filters.as<uint64_t>(), cards,
+                         reinterpret_cast<const uint32_t*>(pd + pl.ranges), grid_q, depth, lane.filters.as<uint64_t>(), cards,
                          chunk_cards, static_cast<uint32_t>(ctx.cu_count) * 8, lane.stream, n_queries),
             "filter kernel");
    if (column >= 0) {
@@ -164,6 +164,7 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column, bool 
       m.chunk_sizes = t.chunk_sizes_dev.as<uint32_t>();
       m.n_chunks = t.n_chunks;
       m.max_queries = Q;
+      m.grid_queries = grid_q;
       m.n_queries = n_queries;
       m.filters = lane.filters.as<uint64_t>();
       m.chunk_cards = chunk_cards;
@@ -204,7 +205,7 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column, bool 
    if (aggregates) {
       AggregateBatch a;
       a.n_chunks = t.n_chunks;
-      a.max_entries = Q;
+      a.max_entries = grid_q;
       a.n_agg = n_queries + 1;
       a.total_bins = n_queries + 2;
       a.max_total_bins = lane.hist_capacity;
@@ -220,7 +221,7 @@ void enqueueKernels(Table& t, BatchLane& lane, uint32_t depth, int column, bool 
       a.n_rows = lane.n_rows.as<uint32_t>();
       hipCheck(launchAggregateBatch(a, lane.stream), "batched group histogram");
    }
-   hipCheck(launchPackRows(n_queries, Q, lane.n_rows.as<uint32_t>(), cards, lane.rows.as<MutationRowDev>(), lane.row_capacity,
+   hipCheck(launchPackRows(n_queries, grid_q, lane.n_rows.as<uint32_t>(), cards, lane.rows.as<MutationRowDev>(), lane.row_capacity,
                            reinterpret_cast<uint32_t*>(od + ol.header), reinterpret_cast<MutationRowDev*>(od + ol.rows),
                            lane.stream),
             "pack rows");
@@ -239,25 +240,31 @@ void ensureOctetBuffers(const Table& t, BatchLane& lane) {
    lane.octet_masks = DeviceBuffer(static_cast<size_t>((Q + 7) / 8) * t.n_chunks * kChunkRows);
 }
 
-void launchLane(Table& t, BatchLane& lane, uint32_t depth, int column, bool aggregates) {
+void launchLane(Table& t, BatchLane& lane, uint32_t depth, int column, bool aggregates, uint32_t n_queries) {
    Context& ctx = *t.ctx;
    ensureOctetBuffers(t, lane);
    const uint32_t bucket = depthBucket(depth, maxFilterDepth(ctx));
+   // the per-query grids cover the job's size rounded up to a power of two, not the lane's
+   // capacity: a small job (a serving loop's few queries) then does not dispatch the work-groups of
+   // the capacity's empty query slots
+   uint32_t grid_q = 1;
+   while (grid_q < n_queries) grid_q <<= 1;
+   grid_q = std::min(grid_q, lane.max_queries);
    lane.job_timed = false;
    if (!ctx.batch_graphs || ctx.profiling) {  // timed jobs run eagerly: events around their kernels
-      enqueueKernels(t, lane, bucket, column, aggregates);
+      enqueueKernels(t, lane, bucket, column, aggregates, grid_q);
       return;
    }
    const uint32_t knobs = ctx.mutation_unroll | (ctx.batch_shared_scan << 8) | (ctx.mutation_coop ? 1u << 12 : 0u) |
                           (ctx.batch_octet_threads == 512 ? 1u << 13 : 0u) | (aggregates ? 1u << 14 : 0u) |
                           (ctx.batch_octet_groups << 16);
-   const auto key = std::make_tuple(bucket, column, static_cast<long long>(ctx.sparse_max_rows), knobs);
+   const auto key = std::make_tuple(bucket | (grid_q << 16), column, static_cast<long long>(ctx.sparse_max_rows), knobs);
    auto it = lane.graphs.find(key);
    if (it == lane.graphs.end()) {
       hipGraph_t graph = nullptr;
       hipCheck(hipStreamBeginCapture(lane.stream, hipStreamCaptureModeThreadLocal), "begin capture");
       try {
-         enqueueKernels(t, lane, bucket, column, aggregates);
+         enqueueKernels(t, lane, bucket, column, aggregates, grid_q);
       } catch (...) {
          (void)hipStreamEndCapture(lane.stream, &graph);
          if (graph) (void)hipGraphDestroy(graph);
@@ -334,7 +341,7 @@ void enqueueJob(Table& t, BatchResult& b, const std::vector<CompiledQuery>& cq, 
    try {
       hipCheck(hipMemcpyAsync(lane.params.as<void>(), hp, pl.bytes, hipMemcpyHostToDevice, lane.stream), "stage batch");
       if (t.ctx->batch_fail_job > 0 && --t.ctx->batch_fail_job == 0) fail(RDB_ENOMEM, "injected lane-job failure");
-      launchLane(t, lane, depth, job.column, n_agg > 0);
+      launchLane(t, lane, depth, job.column, n_agg > 0, n);
       hipCheck(hipMemcpyAsync(lane.host_out, lane.out.as<void>(), lane.job_first_bytes, hipMemcpyDeviceToHost, lane.stream),
                "copy batch results");
       hipCheck(hipEventRecord(lane.done, lane.stream), "hipEventRecord");

@@ -105,6 +105,9 @@ struct MutationsBatch {
    const uint32_t* chunk_sizes = nullptr;
    uint32_t n_chunks = 0;
    uint32_t max_queries = 0;
+   // query slots the grids cover (the job's size rounded up; 0: max_queries); the buffers' strides
+   // stay max_queries
+   uint32_t grid_queries = 0;
    const uint32_t* n_queries = nullptr;        // device
    const uint64_t* filters = nullptr;
    const uint32_t* chunk_cards = nullptr;      // [q * n_chunks + c]

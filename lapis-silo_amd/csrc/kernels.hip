@@ -2812,14 +2812,15 @@ hipError_t launchMutations(const MutationsLaunch& m, hipStream_t stream) {
 
 hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
    if (m.max_queries == 0) return hipSuccess;
-   if (m.max_queries > 65535) return hipErrorInvalidValue;
+   if (m.max_queries > 65535 || m.grid_queries > m.max_queries) return hipErrorInvalidValue;
+   const uint32_t gq = m.grid_queries ? m.grid_queries : m.max_queries;  // grid rows of per-query kernels
    const SeqColumnView& h = *m.col_host;
    hipError_t err = hipSuccess;
    hipLaunchKernelGGL(batchPlanKernel, dim3(1), dim3(1024), 0, stream, m);
    if ((err = hipGetLastError()) != hipSuccess) return err;
    // the dense scan asked for, or the next smaller one where a table is too large for it (the
    // knob never changes results); the lane holds the buffers of the scan this chooses
-   const uint32_t max_octets = (m.max_queries + 7) / 8;
+   const uint32_t max_octets = (gq + 7) / 8;
    uint32_t mode = m.heavy_sums == nullptr ? 0u : effectiveSharedScan(m.shared_mode, m.max_queries, m.n_chunks);
    if (mode == 2 && (m.octet_masks == nullptr || m.partial_c == nullptr || m.partial_b == nullptr)) {
       return hipErrorInvalidValue;
@@ -2852,7 +2853,7 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
    } else if (mode == 1) {
       // shared scan, four dense queries per pass, chunks in two parts
       const uint32_t n_xblocks = 8 * ((h.n_blocks + 7) / 8);
-      const dim3 grid(n_xblocks * 2 * ((m.max_queries + 3) / 4));
+      const dim3 grid(n_xblocks * 2 * ((gq + 3) / 4));
       if (m.unroll == 16) {
          hipLaunchKernelGGL(mutationsCountSharedKernel<4>, grid, dim3(kSharedThreads), 0, stream, m);
       } else {
@@ -2862,7 +2863,7 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
       if (m.scan_stop && (err = hipEventRecord(m.scan_stop, stream)) != hipSuccess) return err;
    } else {
       const uint32_t n_xblocks = 8 * ((h.n_blocks + 7) / 8);
-      const dim3 count_grid(n_xblocks * m.max_queries);
+      const dim3 count_grid(n_xblocks * gq);
       if (m.unroll == 16) {
          if (m.coop) {
             hipLaunchKernelGGL((mutationsCountBatchKernel<16, true>), count_grid, dim3(kMutThreads), 0, stream, m);
@@ -2884,16 +2885,16 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
    // scan's parts and heavy sums as they are read)
    const uint32_t n_words = m.n_chunks * kChunkWords;
    if (m.sparse_max >= 0 && n_words > 0) {
-      hipLaunchKernelGGL(sparseCountBatchKernel, dim3((n_words + kSparseBatchThreads - 1) / kSparseBatchThreads, m.max_queries),
+      hipLaunchKernelGGL(sparseCountBatchKernel, dim3((n_words + kSparseBatchThreads - 1) / kSparseBatchThreads, gq),
                          dim3(kSparseBatchThreads), 0, stream, m);
       if ((err = hipGetLastError()) != hipSuccess) return err;
    }
    const uint32_t dense_parts = mode == 2 ? kOctetParts : (mode == 1 ? 2u : 1u);
    const uint32_t n_pblocks = (h.genome_length + kPosBlock - 1) / kPosBlock;
-   hipLaunchKernelGGL(mutationsTotalsBatchKernel, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0, stream, m,
+   hipLaunchKernelGGL(mutationsTotalsBatchKernel, dim3(n_pblocks, gq), dim3(kPosBlock), 0, stream, m,
                       n_pblocks, dense_parts);
    if ((err = hipGetLastError()) != hipSuccess) return err;
-   hipLaunchKernelGGL(mutationsFinalizeBatchKernel, dim3(n_pblocks, m.max_queries), dim3(kPosBlock), 0, stream, m,
+   hipLaunchKernelGGL(mutationsFinalizeBatchKernel, dim3(n_pblocks, gq), dim3(kPosBlock), 0, stream, m,
                       n_pblocks, dense_parts);
    return hipGetLastError();
 }
