@@ -11,6 +11,10 @@
 #include <chrono>
 #include <cstring>
 #include <numeric>
+#include <atomic>
+#include <condition_variable>
+#include <memory>
+#include <mutex>
 #include <thread>
 
 #include "engine.h"
@@ -18,11 +22,110 @@
 
 namespace rdb {
 
+namespace {
+
+// Host worker threads started once and reused by every parallelFor that asks for all of them, so
+// that the short parallel sections of a batch submission (compiling, planning and emitting a
+// thousand filters) do not pay for starting threads each time.
+thread_local bool t_pool_worker = false;
+
+class WorkerPool {
+  public:
+   explicit WorkerPool(unsigned workers) {
+      for (unsigned k = 0; k < workers; ++k) threads_.emplace_back([this] { loop(); });
+   }
+   // Runs f(0 .. n-1) on the workers and the calling thread; false (nothing run) when another job
+   // holds the pool.
+   bool run(size_t n, const std::function<void(size_t)>& f) {
+      std::unique_lock<std::mutex> job_lock(job_mu_, std::try_to_lock);
+      if (!job_lock.owns_lock()) return false;
+      // the calling thread works as one of the pool's threads: a parallelFor inside f runs serially
+      t_pool_worker = true;
+      struct Reset {
+         ~Reset() { t_pool_worker = false; }
+      } reset;
+      auto job = std::make_shared<Job>();
+      job->f = &f;
+      job->n = n;
+      {
+         std::lock_guard<std::mutex> lock(mu_);
+         current_ = job;
+      }
+      cv_.notify_all();
+      work(*job);
+      {
+         // every item has finished (a worker may still be inside f until then)
+         std::unique_lock<std::mutex> lock(mu_);
+         done_cv_.wait(lock, [&] { return job->done.load() == n; });
+         current_.reset();
+      }
+      if (job->error) std::rethrow_exception(job->error);
+      return true;
+   }
+
+  private:
+   struct Job {
+      const std::function<void(size_t)>* f = nullptr;
+      size_t n = 0;
+      std::atomic<size_t> next{0};
+      std::atomic<size_t> done{0};
+      std::exception_ptr error;
+      std::mutex error_mu;
+   };
+
+   void work(Job& j) {
+      for (;;) {
+         const size_t i = j.next.fetch_add(1);
+         if (i >= j.n) return;
+         try {
+            (*j.f)(i);
+         } catch (...) {
+            std::lock_guard<std::mutex> lock(j.error_mu);
+            if (!j.error) j.error = std::current_exception();
+         }
+         if (j.done.fetch_add(1) + 1 == j.n) {
+            std::lock_guard<std::mutex> lock(mu_);
+            done_cv_.notify_all();
+         }
+      }
+   }
+
+   void loop() {
+      t_pool_worker = true;
+      std::shared_ptr<Job> last;
+      for (;;) {
+         std::shared_ptr<Job> job;
+         {
+            std::unique_lock<std::mutex> lock(mu_);
+            cv_.wait(lock, [&] { return current_ != nullptr && current_ != last; });
+            job = current_;
+         }
+         last = job;
+         work(*job);
+      }
+   }
+
+   std::vector<std::thread> threads_;
+   std::mutex job_mu_, mu_;
+   std::condition_variable cv_, done_cv_;
+   std::shared_ptr<Job> current_;
+};
+
+unsigned defaultThreads() { return std::max(1u, std::min(16u, std::thread::hardware_concurrency())); }
+
+}  // namespace
+
 void parallelFor(size_t n, const std::function<void(size_t)>& f, unsigned threads) {
-   if (threads == 0) threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-   if (n <= 1 || threads <= 1) {
+   const unsigned all = defaultThreads();
+   if (threads == 0) threads = all;
+   if (n <= 1 || threads <= 1 || t_pool_worker) {
       for (size_t i = 0; i < n; ++i) f(i);
       return;
+   }
+   if (threads >= all) {
+      // created on first use and kept for the life of the process (its workers wait on a condition)
+      static WorkerPool* pool = new WorkerPool(all - 1);
+      if (pool->run(n, f)) return;
    }
    std::atomic<size_t> next{0};
    std::vector<std::thread> pool;
