@@ -128,6 +128,13 @@ int rdb_table_add_int_column(rdb_table* table, const char* name, int is_date, co
 int rdb_table_add_dict_column(rdb_table* table, const char* name, const uint32_t* ids, const uint8_t* is_null,
                               uint32_t n_values, const char* names, const uint32_t* name_offsets,
                               const int32_t* lineage_parent);
+/* rdb_table_add_dict_column with lineage aliases: lineage_alias[v] is the value v resolves to (-1 =
+ * none; LineageTree::resolveAlias, common/lineage_tree.cpp:164-169).  Rows are indexed under their
+ * value's resolved lineage and a lineage filter on an alias gives its lineage's rows
+ * (storage/column/lineage_index.cpp:17-45); string equality keeps the stored values. */
+int rdb_table_add_lineage_column(rdb_table* table, const char* name, const uint32_t* ids, const uint8_t* is_null,
+                                 uint32_t n_values, const char* names, const uint32_t* name_offsets,
+                                 const int32_t* lineage_parent, const int32_t* lineage_alias);
 /* A literal row set usable in filters as (bitmap ID) — the IndexScan over a literal bitmap that the
  * reference's operator tests use (filter/operators/index_scan.h).  Row ids must lie in existing
  * chunks. */

@@ -454,6 +454,13 @@ int rdb_table_add_int_column(rdb_table* table, const char* name, int is_date, co
 int rdb_table_add_dict_column(rdb_table* table, const char* name, const uint32_t* ids, const uint8_t* is_null,
                               uint32_t n_values, const char* names, const uint32_t* name_offsets,
                               const int32_t* lineage_parent) {
+   return rdb_table_add_lineage_column(table, name, ids, is_null, n_values, names, name_offsets, lineage_parent,
+                                       nullptr);
+}
+
+int rdb_table_add_lineage_column(rdb_table* table, const char* name, const uint32_t* ids, const uint8_t* is_null,
+                                 uint32_t n_values, const char* names, const uint32_t* name_offsets,
+                                 const int32_t* lineage_parent, const int32_t* lineage_alias) {
    Context* ctx = table ? table->t.ctx : nullptr;
    return guarded(ctx, [&] {
       Table& t = tableOf(table);
@@ -468,7 +475,8 @@ int rdb_table_add_dict_column(rdb_table* table, const char* name, const uint32_t
          col->dictionary.emplace_back(names + name_offsets[v], names + name_offsets[v + 1]);
          if (!col->ids.emplace(col->dictionary.back(), v).second) fail(RDB_EINVAL, "duplicate dictionary value");
       }
-      buildDictColumn(t, *col, ids, is_null, lineage_parent);
+      if (lineage_alias && !lineage_parent) fail(RDB_EINVAL, "lineage aliases need the lineage parents");
+      buildDictColumn(t, *col, ids, is_null, lineage_parent, lineage_alias);
       t.dicts.push_back(std::move(col));
    });
 }

@@ -210,6 +210,8 @@ struct DictColumn {
    std::unordered_map<std::string, uint32_t> ids;
    std::vector<uint32_t> value_bitmap;      // value id -> store id (kInvalidId when no row has it)
    std::vector<uint32_t> sublineage_bitmap; // value id -> store id (lineage index)
+   std::vector<uint32_t> lineage_bitmap;    // value id -> store id of the rows whose value resolves to
+                                            // the same lineage (aliases resolved; lineage index)
    uint32_t null_bitmap_id = kInvalidId;
    bool has_nulls = false;
    bool lineage = false;
@@ -291,8 +293,10 @@ struct FinalizedSequence {
    const uint32_t* missing_pos = nullptr;
 };
 void buildSequenceColumnFinalized(Table& t, SeqColumn& col, const FinalizedSequence& f);
+// lineage_parent (-1 = root) builds the lineage index; lineage_alias (-1 = none) names the value
+// an alias resolves to (lineage_tree.cpp:164-169), only with lineage_parent
 void buildDictColumn(Table& t, DictColumn& col, const uint32_t* ids, const uint8_t* is_null,
-                     const int32_t* lineage_parent);
+                     const int32_t* lineage_parent, const int32_t* lineage_alias = nullptr);
 void finalizeTable(Table& t);
 
 // ---- device row sets ---------------------------------------------------------------------------

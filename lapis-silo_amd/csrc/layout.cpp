@@ -1144,7 +1144,7 @@ void buildIntColumn(Table& t, IntColumn& col, const int32_t* values, const uint8
 }
 
 void buildDictColumn(Table& t, DictColumn& col, const uint32_t* ids, const uint8_t* is_null,
-                     const int32_t* lineage_parent) {
+                     const int32_t* lineage_parent, const int32_t* lineage_alias) {
    const uint32_t nv = static_cast<uint32_t>(col.dictionary.size());
    std::vector<std::vector<uint32_t>> rows(nv);
    std::vector<uint32_t> null_rows;
@@ -1171,30 +1171,54 @@ void buildDictColumn(Table& t, DictColumn& col, const uint32_t* ids, const uint8
    col.has_nulls = !null_rows.empty();
    col.null_bitmap_id = t.addStoreBitmap(null_rows);
    if (lineage_parent) {
-      // lineage_index.h:17-42: a value's rows together with those of all its descendants
+      // lineage_index.cpp:17-45: rows are indexed under their value's resolved alias, a value's
+      // sublineage rows are its own and those of all its descendants, and a query value resolves
+      // its alias before the lookup
       col.lineage = true;
+      std::vector<uint32_t> resolved(nv);
+      for (uint32_t v = 0; v < nv; ++v) {
+         const int32_t a = lineage_alias ? lineage_alias[v] : -1;
+         if (a >= 0 && static_cast<uint32_t>(a) >= nv) fail(RDB_EINVAL, "lineage alias out of range");
+         resolved[v] = a >= 0 ? static_cast<uint32_t>(a) : v;
+      }
+      std::vector<std::vector<uint32_t>> own(nv);
+      for (uint32_t v = 0; v < nv; ++v) own[resolved[v]].insert(own[resolved[v]].end(), rows[v].begin(), rows[v].end());
+      col.lineage_bitmap.assign(nv, kInvalidId);
+      for (uint32_t v = 0; v < nv; ++v) {
+         if (resolved[v] != v || own[v].empty()) continue;
+         std::sort(own[v].begin(), own[v].end());
+         col.lineage_bitmap[v] = t.addStoreBitmap(own[v]);
+      }
       col.sublineage_bitmap.assign(nv, kInvalidId);
       std::vector<std::vector<uint32_t>> children(nv);
       for (uint32_t v = 0; v < nv; ++v) {
          const int32_t p = lineage_parent[v];
-         if (p >= 0 && static_cast<uint32_t>(p) < nv && static_cast<uint32_t>(p) != v) children[p].push_back(v);
+         if (resolved[v] == v && p >= 0 && static_cast<uint32_t>(p) < nv && static_cast<uint32_t>(p) != v) {
+            children[p].push_back(v);
+         }
       }
+      std::vector<uint32_t> seen(nv, kInvalidId);
       for (uint32_t v = 0; v < nv; ++v) {
+         if (resolved[v] != v) continue;
          std::vector<uint32_t> acc;
          std::vector<uint32_t> stack{v};
-         std::vector<uint8_t> seen(nv, 0);
          while (!stack.empty()) {
             const uint32_t w = stack.back();
             stack.pop_back();
-            if (seen[w]) continue;
-            seen[w] = 1;
-            acc.insert(acc.end(), rows[w].begin(), rows[w].end());
+            if (seen[w] == v) continue;
+            seen[w] = v;
+            acc.insert(acc.end(), own[w].begin(), own[w].end());
             for (uint32_t ch : children[w]) stack.push_back(ch);
          }
          if (!acc.empty()) {
             std::sort(acc.begin(), acc.end());
             col.sublineage_bitmap[v] = t.addStoreBitmap(acc);
          }
+      }
+      for (uint32_t v = 0; v < nv; ++v) {
+         if (resolved[v] == v) continue;
+         col.lineage_bitmap[v] = col.lineage_bitmap[resolved[v]];
+         col.sublineage_bitmap[v] = col.sublineage_bitmap[resolved[v]];
       }
    }
    std::vector<uint32_t> order(nv);

@@ -998,7 +998,7 @@ OperatorPtr compile(Table& t, const Expression& e) {
             if (it == col->ids.end()) {
                fail(RDB_EQUERY, "The lineage '" + *e.value + "' is not a valid lineage for column '" + e.column + "'.");
             }
-            id = e.sublineage ? col->sublineage_bitmap[it->second] : col->value_bitmap[it->second];
+            id = e.sublineage ? col->sublineage_bitmap[it->second] : col->lineage_bitmap[it->second];
          }
          if (id == kInvalidId) return std::make_unique<Empty>(tp);
          return std::make_unique<IndexScan>(tp, BitmapSource{BitmapSource::STORE, id});

@@ -36,6 +36,7 @@ SIGNATURES = [
     ("rdb_table_column_digest", _int, [_vp, _cp, _vp]),
     ("rdb_table_add_int_column", _int, [_vp, _cp, _int, _vp, _vp]),
     ("rdb_table_add_dict_column", _int, [_vp, _cp, _vp, _vp, _u32, _cp, _vp, _vp]),
+    ("rdb_table_add_lineage_column", _int, [_vp, _cp, _vp, _vp, _u32, _cp, _vp, _vp, _vp]),
     ("rdb_table_add_bitmap", _int, [_vp, _vp, _u64, _vp]),
     ("rdb_table_finalize", _int, [_vp]),
     ("rdb_table_local_reference", _int, [_vp, _cp, _vp]),
@@ -320,8 +321,11 @@ class DeviceTable:
                 offs = np.zeros(len(blobs) + 1, dtype=np.uint32)
                 offs[1:] = np.cumsum([len(b) for b in blobs])
                 parent = None if c.lineage_parent is None else np.ascontiguousarray(c.lineage_parent, dtype=np.int32)
-                _check(L.rdb_table_add_dict_column(h, c.name.encode(), _p(ids), _p(nl), len(c.names), b"".join(blobs),
-                                                   _p(offs), None if parent is None else _p(parent)), ctx)
+                alias = None if c.lineage_alias is None else np.ascontiguousarray(c.lineage_alias, dtype=np.int32)
+                _check(L.rdb_table_add_lineage_column(h, c.name.encode(), _p(ids), _p(nl), len(c.names),
+                                                      b"".join(blobs), _p(offs),
+                                                      None if parent is None else _p(parent),
+                                                      None if alias is None else _p(alias)), ctx)
                 self.dict_names[c.name] = list(c.names)
             for b in spec.bitmaps:
                 arr = np.ascontiguousarray(b, dtype=np.uint32)

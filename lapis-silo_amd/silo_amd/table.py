@@ -130,6 +130,7 @@ class DictColumnData:
     is_null: np.ndarray  # u8
     names: list
     lineage_parent: np.ndarray | None = None  # i32 per value, -1 = root
+    lineage_alias: np.ndarray | None = None   # i32 per value: the value its alias resolves to, -1 = none
 
 
 @dataclass

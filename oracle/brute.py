@@ -75,9 +75,14 @@ class BruteTable:
             self.seq[name] = (alphabet, np.array([to_symbol(alphabet, c) for c in ref], np.uint8), M, null)
         self.ints = {k: (np.asarray(v[0]), np.asarray(v[1]).astype(bool)) for k, v in (int_columns or {}).items()}
         self.dicts = {}
-        for k, (ids, nulls, names, parent) in (dict_columns or {}).items():
+        self.aliases = {}
+        for k, col in (dict_columns or {}).items():
+            ids, nulls, names, parent = col[:4]
             self.dicts[k] = (np.asarray(ids), np.asarray(nulls).astype(bool), list(names),
                              None if parent is None else [int(x) for x in parent])
+            # optional fifth entry: per value the value its lineage alias resolves to (-1 = none)
+            if len(col) > 4 and col[4] is not None:
+                self.aliases[k] = [int(x) for x in col[4]]
         self.bitmaps = [set(int(x) for x in b) for b in (bitmaps or [])]
 
     # ---- expressions -----------------------------------------------------------------------
@@ -170,7 +175,8 @@ class BruteTable:
                 target = {to_symbol(alphabet, c) for c in self._atom(args[2])}
             return (~null) & np.isin(sym, list(target))
         if op == "str_eq" or op == "lineage":
-            ids, nulls, names, parent = self.dicts[self._atom(args[0])]
+            column = self._atom(args[0])
+            ids, nulls, names, parent = self.dicts[column]
             v = args[1]
             if not isinstance(v, tuple) and v == "null":
                 return nulls.copy()
@@ -178,17 +184,23 @@ class BruteTable:
             if name not in names:
                 return np.zeros(n, bool)
             vid = names.index(name)
-            if op == "lineage" and int(args[2]) != 0:
-                def is_desc(w):
-                    seen = 0
-                    while w >= 0 and seen <= len(parent):
-                        if w == vid:
-                            return True
-                        w = parent[w]
-                        seen += 1
-                    return False
-                ok = np.array([is_desc(w) for w in range(len(names))], bool)
-                return (~nulls) & ok[ids]
+            if op == "lineage":
+                # lineage_index.cpp:17-45: values and the query resolve their alias first
+                alias = self.aliases.get(column)
+                res = np.array([alias[w] if alias and alias[w] >= 0 else w for w in range(len(names))], np.int64)
+                vid = int(res[vid])
+                if int(args[2]) != 0:
+                    def is_desc(w):
+                        seen = 0
+                        while w >= 0 and seen <= len(parent):
+                            if w == vid:
+                                return True
+                            w = parent[w]
+                            seen += 1
+                        return False
+                    ok = np.array([is_desc(int(res[w])) for w in range(len(names))], bool)
+                    return (~nulls) & ok[ids]
+                return (~nulls) & (res[ids] == vid)
             return (~nulls) & (ids == vid)
         if op in ("date_between", "int_between"):
             vals, nulls = self.ints[self._atom(args[0])]

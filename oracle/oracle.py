@@ -104,6 +104,8 @@ class OracleTable:
             blobs = [s.encode() for s in c.names]
             offs = np.zeros(len(blobs) + 1, dtype=np.uint32)
             offs[1:] = np.cumsum([len(b) for b in blobs])
+            if getattr(c, "lineage_alias", None) is not None and np.any(np.asarray(c.lineage_alias) >= 0):
+                raise OracleError("lineage aliases are checked against oracle/brute.py, not this oracle")
             parent = None if c.lineage_parent is None else np.ascontiguousarray(c.lineage_parent, dtype=np.int32)
             rc = L.or_add_dict_column(self._h, c.name.encode(), _p(ids), _p(n), len(c.names), b"".join(blobs),
                                       _p(offs), None if parent is None else _p(parent))

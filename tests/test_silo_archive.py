@@ -1,0 +1,218 @@
+"""Reading the reference's saved database state (CPU only).
+
+The fixture is the state database.test.cpp:71-96 saves from testBaseData/unitTestDummyDataset.
+The loader must read the whole of each archive, give the counts that test checks (5 sequences,
+a non-empty vertical index, 9 bytes of horizontal bitmaps), and hold exactly what the dataset's
+input.ndjson says: each sequence column equal to the oracle's finalized column of the same rows,
+the int, date, bool and dictionary values row by row, and the lineage tree of the lineage
+definition text the schema carries.  Damaged or foreign archives are refused."""
+import gzip
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import silo_support as ss
+
+
+@pytest.fixture(scope="module")
+def db(tmp_path_factory):
+    from silo_amd import silo_archive as sa
+
+    root = os.path.dirname(ss.saved_state(tmp_path_factory.mktemp("state")))
+    path = sa.most_recent_data_directory(root)
+    assert path is not None and os.path.basename(path) == ss.TIMESTAMP
+    return sa.load_database(path)
+
+
+def test_database_info_matches_the_reference_test(db):
+    # database.test.cpp:97-112: sequence_count 5, vertical_bitmaps_size > 0, horizontal_bitmaps_size 9
+    from silo_amd import silo_archive as sa
+
+    t = db.tables["default"]
+    assert t.sequence_count == 5 and t.chunk_sizes.tolist() == [5]
+    cols = [c for c in t.columns.values() if isinstance(c, sa.SequenceColumnState)]
+    assert sorted(c.name for c in cols) == ["E", "M", "main", "testSecondSequence"]
+    assert all(c.info[0] == 5 and c.sequence_count == 5 for c in cols)
+    assert sum(c.info[1] for c in cols) > 0
+    assert sum(c.info[2] for c in cols) == 9
+    assert db.timestamp == int(ss.TIMESTAMP) and db.serialization_version == sa.SERIALIZATION_VERSION
+
+
+def test_schema_lists_the_dataset_columns(db):
+    from silo_amd import silo_archive as sa
+
+    s = db.tables["default"].schema
+    assert s.primary_key == "primaryKey"
+    types = {c.name: c.type for c in s.columns}
+    assert types["main"] == sa.NUC_SEQUENCE and types["E"] == sa.AA_SEQUENCE
+    assert types["date"] == sa.DATE32 and types["age"] == sa.INT32 and types["qc_value"] == sa.FLOAT
+    assert types["pango_lineage"] == sa.DICTIONARY_ENCODED and types["test_boolean_column"] == sa.BOOL
+    assert types["unaligned_main"] == sa.ZSTD_STRING and types["primaryKey"] == sa.STRING
+    refs = ss.references()
+    for name, (alphabet, ref) in refs.items():
+        from silo_amd import table
+
+        want = [table.symbol_of(alphabet, ch) for ch in ref]
+        assert s.column(name).reference.tolist() == want
+
+
+def container_rows(f, k):
+    from silo_amd import silo_archive as sa
+
+    o = int(f.payload_offset[k])
+    return sa.container_values(int(f.typecode[k]), int(f.cardinality[k]), bytes(f.payload[o:o + 8192 + 4]))
+
+
+def index_sets(f) -> dict:
+    return {(int(f.key_position[k]), int(f.key_chunk[k]), int(f.key_symbol[k])): container_rows(f, k).tolist()
+            for k in range(len(f.key_position))}
+
+
+def test_sequence_columns_equal_the_oracle_finalize_of_the_ndjson_rows(db):
+    from oracle.oracle import OracleTable
+
+    spec = db.table_spec()
+    lineage = db.tables["default"].schema.column("pango_lineage").lineage_tree
+    ot = OracleTable(ss.ndjson_spec(lineage.file, aliases=False))
+    try:
+        for got in spec.seq_columns:
+            want = ot.export_finalized(got.name)
+            assert got.local_reference.tolist() == want.local_reference.tolist(), got.name
+            assert index_sets(got) == index_sets(want), got.name
+            assert all(int(got.cardinality[k]) == len(container_rows(got, k)) for k in range(len(got.key_position)))
+            for a in ("is_null", "cov_start", "cov_end", "missing_ptr", "missing_pos"):
+                assert getattr(got, a).tolist() == getattr(want, a).tolist(), (got.name, a)
+    finally:
+        ot.close()
+    main = next(c for c in spec.seq_columns if c.name == "main")
+    # row key2 = AAGNAAGN over ACGTACGT: C>A at 1 and 5, N at 3 inside the covered range, N at 7 ends it
+    assert index_sets(main) == {(1, 0, 1): [1], (5, 0, 1): [1]}
+    assert main.cov_end.tolist() == [8, 7, 8, 8, 8] and main.missing_pos.tolist() == [3]
+
+
+def test_metadata_columns_hold_the_ndjson_values(db):
+    t = db.tables["default"]
+    spec = db.table_spec()
+    rs = ss.rows()
+    ints = {c.name: c for c in spec.int_columns}
+    for name in ("date", "unsorted_date"):
+        vals = [r.get(name) for r in rs]
+        assert ints[name].is_date
+        assert ints[name].is_null.tolist() == [int(v is None) for v in vals]
+        assert [x for x, v in zip(ints[name].values.tolist(), vals) if v is not None] == \
+               [ss.days(v) for v in vals if v is not None]
+    assert ints["age"].values.tolist() == [r["age"] for r in rs] and not ints["age"].is_date
+    dicts = {c.name: c for c in spec.dict_columns}
+    for name in ("country", "division", "region", "pango_lineage"):
+        c = dicts[name]
+        assert [c.names[i] for i in c.ids.tolist()] == [r[name] for r in rs], name
+    b = t.columns["test_boolean_column"]
+    flags = [r["test_boolean_column"] for r in rs]
+    assert b["true_rows"].tolist() == [i for i, v in enumerate(flags) if v is True]
+    assert b["false_rows"].tolist() == [i for i, v in enumerate(flags) if v is False]
+    assert b["null_rows"].tolist() == [i for i, v in enumerate(flags) if v is None]
+    q = t.columns["qc_value"]
+    assert np.concatenate(q["chunks"]).tolist() == [r["qc_value"] for r in rs]
+    assert not any(d["null_rows"].size for n, d in t.columns.items() if n in ("age", "qc_value", "country"))
+
+
+def test_lineage_tree_is_the_definition_it_was_built_from(db):
+    from silo_amd import silo_archive as sa
+
+    col = db.tables["default"].schema.column("pango_lineage")
+    tree = col.lineage_tree
+    assert col.dictionary == tree.names
+    names, parent, alias = ss.lineage_definition(tree.file)
+    got_parent, got_alias = sa.lineage_arrays(tree, len(col.dictionary))
+    assert sorted(names) == sorted(col.dictionary)
+    index = {n: i for i, n in enumerate(names)}
+    perm = np.array([index[n] for n in col.dictionary])
+    # the same tree up to the order of ids: parents and alias targets by name
+    by_name = lambda arr, ns: [None if x < 0 else ns[x] for x in arr.tolist()]
+    assert by_name(got_parent, col.dictionary) == [by_name(parent, names)[i] for i in perm]
+    assert by_name(got_alias, col.dictionary) == [by_name(alias, names)[i] for i in perm]
+    # the saved lineage index: rows under each of B.1.1.7's ancestors, including itself
+    li = db.tables["default"].columns["pango_lineage"]["lineage_index"]
+    b117 = col.dictionary.index("B.1.1.7")
+    chain = []
+    v = b117
+    while v >= 0:
+        chain.append(v)
+        v = int(got_parent[v])
+    for mode, index_including in li["including_sublineages"].items():
+        assert sorted(k for k, rows in index_including.items() if rows.tolist() == [0, 1, 2]) == sorted(chain)
+
+
+def encode_portable(values: np.ndarray, runs: bool) -> bytes:
+    """CRoaring's portable format, written from its published specification (RoaringFormatSpec):
+    cookie, container keys and cardinalities, offsets, then array / bitset / run containers."""
+    values = np.unique(np.asarray(values, np.uint32))
+    keys = sorted(set((values >> 16).tolist()))
+    conts = []
+    for k in keys:
+        low = (values[(values >> 16) == k] & 0xFFFF).astype(np.uint32)
+        breaks = np.nonzero(np.diff(low) != 1)[0]
+        starts = np.concatenate([[low[0]], low[breaks + 1]])
+        ends = np.concatenate([low[breaks], [low[-1]]])
+        if runs and len(starts) * 4 + 2 < min(2 * len(low), 8192):
+            body = struct.pack("<H", len(starts)) + b"".join(struct.pack("<HH", s, e - s) for s, e in zip(starts, ends))
+            conts.append((k, len(low), True, body))
+        elif len(low) > 4096:
+            bits = np.zeros(65536, np.uint8)
+            bits[low] = 1
+            conts.append((k, len(low), False, np.packbits(bits, bitorder="little").tobytes()))
+        else:
+            conts.append((k, len(low), False, low.astype("<u2").tobytes()))
+    n = len(conts)
+    has_run = any(c[2] for c in conts)
+    if has_run:
+        flags = bytearray((n + 7) // 8)
+        for i, c in enumerate(conts):
+            if c[2]:
+                flags[i // 8] |= 1 << (i % 8)
+        head = struct.pack("<I", 12347 | ((n - 1) << 16)) + bytes(flags)
+    else:
+        head = struct.pack("<II", 12346, n)
+    head += b"".join(struct.pack("<HH", k, card - 1) for k, card, _, _ in conts)
+    if not has_run or n >= 4:
+        off = len(head) + 4 * n
+        for c in conts:
+            head += struct.pack("<I", off)
+            off += len(c[3])
+    return head + b"".join(c[3] for c in conts)
+
+
+@pytest.mark.parametrize("runs", [False, True])
+def test_portable_roaring_decode_round_trip(runs):
+    from silo_amd import silo_archive as sa
+
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        parts = [rng.integers(0, 1 << 16, size=int(rng.integers(1, 50)), dtype=np.uint32) + (7 << 16),
+                 np.arange(3 << 16, (3 << 16) + int(rng.integers(1, 65536)), dtype=np.uint32),
+                 rng.choice(1 << 16, size=int(rng.integers(4097, 30000)), replace=False).astype(np.uint32) + (9 << 16)]
+        vals = np.concatenate(parts[: 1 + trial % 3])
+        got = sa.roaring_values(encode_portable(vals, runs))
+        assert got.tolist() == np.unique(vals).tolist()
+    assert sa.roaring_values(struct.pack("<II", 12346, 0)).size == 0
+
+
+def test_damaged_or_foreign_archives_are_refused(tmp_path):
+    from silo_amd import silo_archive as sa
+
+    d = ss.saved_state(tmp_path)
+    with open(os.path.join(d, "default.silo"), "rb") as f:
+        table_bytes = f.read()
+    with open(os.path.join(d, "database_schema.silo"), "rb") as f:
+        schema = sa.read_schema(f.read())["default"]
+    for bad, what in ((table_bytes[:-3], "truncated"), (table_bytes + b"\0", "trailing"),
+                      (b"\x05" + table_bytes[1:], "not a Boost"), (table_bytes[:30] + b"\x09\x00" + table_bytes[32:], "library version")):
+        with pytest.raises(sa.SiloFormatError, match=what):
+            sa.read_table(bad, schema)
+    with open(os.path.join(d, "data_version.silo"), "w") as f:
+        f.write("timestamp: 1785915539\nserializationVersion: 1\n")
+    with pytest.raises(sa.SiloFormatError, match="serialization version"):
+        sa.load_database(d)
+    assert sa.most_recent_data_directory(os.path.dirname(d)) is None
