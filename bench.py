@@ -377,6 +377,79 @@ def ingest(args):
         sys.exit(1)
 
 
+def silo_load(args):
+    """--silo: loading a saved database state (the reference's .silo archives) of the config's
+    table.  The table (main column, date, country, lineage) is finalized on the host and written as
+    a saved state first (untimed); the timed part reads the archives (schema + table), builds the
+    upload form and uploads it (finalized-column build on the GPU).  The loaded table must answer
+    the config's queries exactly as the same table uploaded in diff form."""
+    import tempfile
+
+    from silo_amd import datagen, silo_archive, table, workloads
+    from silo_amd.engine import Engine
+    from silo_amd.finalize import finalize_column
+
+    t0 = time.time()
+    ds = datagen.generate_config(args.config, n_rows=args.rows)
+    spec = table.spec_from_dataset(ds)
+    log(f"generated {ds.n_rows} rows x {ds.genome_length} nt in {time.time() - t0:.1f}s")
+    t0 = time.perf_counter()
+    fin = table.TableSpec(chunk_sizes=spec.chunk_sizes, int_columns=spec.int_columns, dict_columns=spec.dict_columns)
+    fin.seq_columns = [finalize_column(c, spec.chunk_sizes) for c in spec.seq_columns]
+    finalize_s = time.perf_counter() - t0
+    root = tempfile.mkdtemp(prefix="silo_state_")
+    t0 = time.perf_counter()
+    path = silo_archive.save_database(root, {"default": fin}, int(time.time()))
+    write_s = time.perf_counter() - t0
+    del fin
+    file_bytes = sum(os.path.getsize(os.path.join(path, f)) for f in os.listdir(path))
+    log(f"finalized on the host in {finalize_s:.1f}s, wrote {file_bytes / 1e6:.1f} MB in {write_s:.1f}s")
+
+    eng = Engine(0)
+    t0 = time.perf_counter()
+    db = silo_archive.load_database(path)
+    read_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    loaded_spec = db.table_spec()
+    form_s = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    loaded = eng.table(loaded_spec)
+    upload_s = time.perf_counter() - t2
+    total_s = time.perf_counter() - t0
+    del db, loaded_spec
+    log(f"read {read_s:.2f}s, upload form {form_s:.2f}s, upload {upload_s:.2f}s")
+    t0 = time.perf_counter()
+    direct = eng.table(spec)
+    direct_s = time.perf_counter() - t0
+    queries = workloads.config0_queries(ds) if args.config == 0 else workloads.config1_queries(ds)
+    identical = True
+    for q in queries + [{"filter": "(true)", "min_proportion": 0.05}]:
+        f, p = q["filter"], q["min_proportion"]
+        identical &= loaded.count(f) == direct.count(f)
+        identical &= loaded.mutations_query(f, "main", p) == direct.mutations_query(f, "main", p)
+    identical &= np.array_equal(loaded.local_reference("main"), direct.local_reference("main"))
+    line = {
+        "metric": "saved-state load time", "value": round(total_s, 3), "unit": "s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": round(total_s * 1e3, 1), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"load of a saved state of {ds.n_rows} sequences x {ds.genome_length} nt "
+                               f"(config {args.config}: main, date, country, lineage)", "rows": ds.n_rows},
+        "archive_MB": round(file_bytes / 1e6, 1), "read_s": round(read_s, 3), "upload_form_s": round(form_s, 3),
+        "upload_s": round(upload_s, 3), "read_MB_per_s": round(file_bytes / 1e6 / read_s, 1),
+        "diff_form_upload_s": round(direct_s, 3), "host_finalize_s": round(finalize_s, 2),
+        "write_s": round(write_s, 2), "queries_identical": bool(identical), "queries_checked": len(queries) + 1,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    loaded.close()
+    direct.close()
+    eng.close()
+    import shutil
+    shutil.rmtree(root, ignore_errors=True)
+    if not identical:
+        sys.exit(1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -406,6 +479,7 @@ def main():
     ap.add_argument("--serving-graphs", type=int, default=1,
                     help="config 4: replay captured HIP graphs per lane job (0: launch the kernels directly)")
     ap.add_argument("--ingest", action="store_true", help="time the index build of the main column instead")
+    ap.add_argument("--silo", action="store_true", help="time loading the config's table from a saved state instead")
     ap.add_argument("--knobs", default="",
                     help="key=value,... tuning knobs set instead of running the autotuner (e.g. for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -422,6 +496,9 @@ def main():
         return
     if args.ingest:
         ingest(args)
+        return
+    if args.silo:
+        silo_load(args)
         return
     if args.config == 4:
         serve(args)

@@ -96,6 +96,32 @@ int rdb_table_add_sequence_column_finalized(rdb_table* table, const char* name, 
                                             const uint8_t* payload, uint64_t payload_bytes, const uint8_t* is_null,
                                             const uint32_t* cov_start, const uint32_t* cov_end,
                                             const uint64_t* missing_ptr, const uint32_t* missing_pos);
+/* Scanners for saved sequence columns (the reference's Boost binary archives; no counterpart in the
+ * reference, which reads them through Boost.Serialization).  rdb_archive_scan_containers reads n
+ * vertical-index records starting at buf[pos] (after the first record's class information: u32
+ * position, u16 chunk, i32 symbol, u32 cardinality, u8 typecode, u64 length, the container_write
+ * bytes; vertical_sequence_index.h:31-44, roaring_container.h:105-116), fills the keys and each
+ * container's offset in the packed payload, copies the bytes into payload when it is non-null
+ * (payload_capacity bytes), and returns the payload's size and the position after the records.
+ * rdb_archive_scan_row_bitmaps reads n horizontal-index records (u32 row id, u64 length, a
+ * portable roaring bitmap; horizontal_coverage_index.h:21-35) and returns where each bitmap lies. */
+int rdb_archive_scan_containers(const uint8_t* buf, uint64_t len, uint64_t pos, uint64_t n, uint32_t* position,
+                                uint16_t* chunk, int32_t* symbol, uint32_t* cardinality, uint8_t* typecode,
+                                uint64_t* payload_offset, uint8_t* payload, uint64_t payload_capacity,
+                                uint64_t* payload_bytes, uint64_t* end);
+int rdb_archive_scan_row_bitmaps(const uint8_t* buf, uint64_t len, uint64_t pos, uint64_t n, uint32_t* row,
+                                 uint64_t* bitmap_start, uint64_t* bitmap_length, uint64_t* end);
+/* rdb_table_add_sequence_column_finalized with each row's missing positions given as the runs
+ * [start, end) of its horizontal bitmap (CSR: missing_run_ptr[n_rows + 1], runs ascending and
+ * disjoint, ends at most the genome length): the form a saved state's run-optimized roaring
+ * bitmaps hold (horizontal_coverage_index.cpp:35-45), without expanding every position. */
+int rdb_table_add_sequence_column_finalized_runs(
+   rdb_table* table, const char* name, int alphabet, const uint8_t* reference, const uint8_t* local_reference,
+   uint32_t genome_length, uint64_t n_containers, const uint32_t* key_position, const uint16_t* key_chunk,
+   const uint8_t* key_symbol, const uint8_t* typecode, const uint32_t* cardinality, const uint64_t* payload_offset,
+   const uint8_t* payload, uint64_t payload_bytes, const uint8_t* is_null, const uint32_t* cov_start,
+   const uint32_t* cov_end, const uint64_t* missing_run_ptr, const uint32_t* missing_run_start,
+   const uint32_t* missing_run_end);
 /* A sequence column built from aligned sequences, as the reference's SequenceColumnBuilder
  * (storage/column/sequence_column.h:178-233) buffers rows and SequenceColumn::appendChunk /
  * finalize (sequence_column.cpp:99-212) index them.  Each append diffs its rows on the GPU against

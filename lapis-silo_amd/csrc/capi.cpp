@@ -317,14 +317,10 @@ int rdb_table_add_sequence_column(rdb_table* table, const char* name, int alphab
    });
 }
 
-int rdb_table_add_sequence_column_finalized(rdb_table* table, const char* name, int alphabet, const uint8_t* reference,
-                                            const uint8_t* local_reference, uint32_t genome_length, uint64_t n_containers,
-                                            const uint32_t* key_position, const uint16_t* key_chunk,
-                                            const uint8_t* key_symbol, const uint8_t* typecode,
-                                            const uint32_t* cardinality, const uint64_t* payload_offset,
-                                            const uint8_t* payload, uint64_t payload_bytes, const uint8_t* is_null,
-                                            const uint32_t* cov_start, const uint32_t* cov_end,
-                                            const uint64_t* missing_ptr, const uint32_t* missing_pos) {
+extern "C++" {
+namespace {
+int addFinalized(rdb_table* table, const char* name, int alphabet, const uint8_t* reference, uint32_t genome_length,
+                 const FinalizedSequence& f) {
    Context* ctx = table ? table->t.ctx : nullptr;
    return guarded(ctx, [&] {
       Table& t = tableOf(table);
@@ -339,25 +335,67 @@ int rdb_table_add_sequence_column_finalized(rdb_table* table, const char* name, 
       col->abc = &alphabetFor(alphabet);
       col->genome_length = genome_length;
       col->reference.assign(reference, reference + genome_length);
-      FinalizedSequence f;
-      f.local_reference = local_reference;
-      f.n_containers = n_containers;
-      f.key_position = key_position;
-      f.key_chunk = key_chunk;
-      f.key_symbol = key_symbol;
-      f.typecode = typecode;
-      f.cardinality = cardinality;
-      f.payload_offset = payload_offset;
-      f.payload = payload;
-      f.payload_bytes = payload_bytes;
-      f.is_null = is_null;
-      f.cov_start = cov_start;
-      f.cov_end = cov_end;
-      f.missing_ptr = missing_ptr;
-      f.missing_pos = missing_pos;
       buildSequenceColumnFinalized(t, *col, f);
       t.seq.push_back(std::move(col));
    });
+}
+
+FinalizedSequence finalizedIndex(const uint8_t* local_reference, uint64_t n_containers, const uint32_t* key_position,
+                                 const uint16_t* key_chunk, const uint8_t* key_symbol, const uint8_t* typecode,
+                                 const uint32_t* cardinality, const uint64_t* payload_offset, const uint8_t* payload,
+                                 uint64_t payload_bytes, const uint8_t* is_null, const uint32_t* cov_start,
+                                 const uint32_t* cov_end) {
+   FinalizedSequence f;
+   f.local_reference = local_reference;
+   f.n_containers = n_containers;
+   f.key_position = key_position;
+   f.key_chunk = key_chunk;
+   f.key_symbol = key_symbol;
+   f.typecode = typecode;
+   f.cardinality = cardinality;
+   f.payload_offset = payload_offset;
+   f.payload = payload;
+   f.payload_bytes = payload_bytes;
+   f.is_null = is_null;
+   f.cov_start = cov_start;
+   f.cov_end = cov_end;
+   return f;
+}
+}  // namespace
+}  // extern "C++"
+
+int rdb_table_add_sequence_column_finalized(rdb_table* table, const char* name, int alphabet, const uint8_t* reference,
+                                            const uint8_t* local_reference, uint32_t genome_length, uint64_t n_containers,
+                                            const uint32_t* key_position, const uint16_t* key_chunk,
+                                            const uint8_t* key_symbol, const uint8_t* typecode,
+                                            const uint32_t* cardinality, const uint64_t* payload_offset,
+                                            const uint8_t* payload, uint64_t payload_bytes, const uint8_t* is_null,
+                                            const uint32_t* cov_start, const uint32_t* cov_end,
+                                            const uint64_t* missing_ptr, const uint32_t* missing_pos) {
+   FinalizedSequence f = finalizedIndex(local_reference, n_containers, key_position, key_chunk, key_symbol, typecode,
+                                        cardinality, payload_offset, payload, payload_bytes, is_null, cov_start, cov_end);
+   f.missing_ptr = missing_ptr;
+   f.missing_pos = missing_pos;
+   return addFinalized(table, name, alphabet, reference, genome_length, f);
+}
+
+int rdb_table_add_sequence_column_finalized_runs(
+   rdb_table* table, const char* name, int alphabet, const uint8_t* reference, const uint8_t* local_reference,
+   uint32_t genome_length, uint64_t n_containers, const uint32_t* key_position, const uint16_t* key_chunk,
+   const uint8_t* key_symbol, const uint8_t* typecode, const uint32_t* cardinality, const uint64_t* payload_offset,
+   const uint8_t* payload, uint64_t payload_bytes, const uint8_t* is_null, const uint32_t* cov_start,
+   const uint32_t* cov_end, const uint64_t* missing_run_ptr, const uint32_t* missing_run_start,
+   const uint32_t* missing_run_end) {
+   FinalizedSequence f = finalizedIndex(local_reference, n_containers, key_position, key_chunk, key_symbol, typecode,
+                                        cardinality, payload_offset, payload, payload_bytes, is_null, cov_start, cov_end);
+   if (!missing_run_ptr) {
+      Context* ctx = table ? table->t.ctx : nullptr;
+      return guarded(ctx, [] { fail(RDB_EINVAL, "null argument"); });
+   }
+   f.run_ptr = missing_run_ptr;
+   f.run_start = missing_run_start;
+   f.run_end = missing_run_end;
+   return addFinalized(table, name, alphabet, reference, genome_length, f);
 }
 
 struct rdb_seq_builder {

@@ -291,6 +291,10 @@ struct FinalizedSequence {
    const uint32_t* cov_end = nullptr;
    const uint64_t* missing_ptr = nullptr;  // [n_rows + 1]
    const uint32_t* missing_pos = nullptr;
+   // or the missing positions as runs [start, end) per row (CSR, run_ptr[n_rows + 1])
+   const uint64_t* run_ptr = nullptr;
+   const uint32_t* run_start = nullptr;
+   const uint32_t* run_end = nullptr;
 };
 void buildSequenceColumnFinalized(Table& t, SeqColumn& col, const FinalizedSequence& f);
 // lineage_parent (-1 = root) builds the lineage index; lineage_alias (-1 = none) names the value

@@ -1366,7 +1366,10 @@ void buildSequenceColumnFinalized(Table& t, SeqColumn& col, const FinalizedSeque
    const uint32_t nsym = col.abc->count;
    const uint64_t n = t.num_rows;
    if (L == 0 || L > 65535) fail(RDB_EINVAL, "genome length must be in [1, 65535]");
-   if (!f.local_reference || !f.is_null || !f.cov_start || !f.cov_end || !f.missing_ptr) fail(RDB_EINVAL, "null argument");
+   const bool given_runs = f.run_ptr != nullptr;
+   if (!f.local_reference || !f.is_null || !f.cov_start || !f.cov_end || (!f.missing_ptr && !given_runs)) {
+      fail(RDB_EINVAL, "null argument");
+   }
    if (f.n_containers && (!f.key_position || !f.key_chunk || !f.key_symbol || !f.typecode || !f.cardinality ||
                           !f.payload_offset || !f.payload)) {
       fail(RDB_EINVAL, "null container argument");
@@ -1374,31 +1377,48 @@ void buildSequenceColumnFinalized(Table& t, SeqColumn& col, const FinalizedSeque
    for (uint32_t p = 0; p < L; ++p) {
       if (f.local_reference[p] >= nsym || col.reference[p] >= nsym) fail(RDB_EINVAL, "invalid reference symbol");
    }
-   if (f.missing_ptr[0] != 0) fail(RDB_EINVAL, "missing_ptr must start at 0");
-   for (uint64_t i = 0; i < n; ++i) {
-      if (f.missing_ptr[i + 1] < f.missing_ptr[i]) fail(RDB_EINVAL, "missing_ptr must be non-decreasing");
-   }
-   // missing positions -> runs [start, end)
    std::vector<uint64_t> nrun_ptr(n + 1, 0);
    std::vector<uint32_t> nrun_start, nrun_end;
-   for (uint64_t i = 0; i < n; ++i) {
-      const uint32_t* a = f.missing_pos + f.missing_ptr[i];
-      const uint32_t* e = f.missing_pos + f.missing_ptr[i + 1];
-      for (const uint32_t* q = a; q < e;) {
-         const uint32_t s = *q;
-         uint32_t last = s;
-         ++q;
-         while (q < e && *q == last + 1) last = *q++;
-         if (q < e && *q <= last) fail(RDB_EINVAL, "missing positions of a row must ascend");
-         nrun_start.push_back(s);
-         nrun_end.push_back(last + 1);
+   if (given_runs) {
+      if (!f.run_start || !f.run_end) fail(RDB_EINVAL, "null argument");
+      if (f.run_ptr[0] != 0) fail(RDB_EINVAL, "missing run offsets must start at 0");
+      for (uint64_t i = 0; i < n; ++i) {
+         if (f.run_ptr[i + 1] < f.run_ptr[i]) fail(RDB_EINVAL, "missing run offsets must be non-decreasing");
+         for (uint64_t k = f.run_ptr[i]; k < f.run_ptr[i + 1]; ++k) {
+            if (f.run_start[k] >= f.run_end[k] || f.run_end[k] > L) fail(RDB_EINVAL, "empty missing run or one past the genome");
+            if (k > f.run_ptr[i] && f.run_start[k] < f.run_end[k - 1]) fail(RDB_EINVAL, "missing runs of a row must ascend");
+         }
+         nrun_ptr[i + 1] = f.run_ptr[i + 1];
       }
-      nrun_ptr[i + 1] = nrun_start.size();
+      nrun_start.assign(f.run_start, f.run_start + f.run_ptr[n]);
+      nrun_end.assign(f.run_end, f.run_end + f.run_ptr[n]);
+   } else {
+      if (f.missing_ptr[0] != 0) fail(RDB_EINVAL, "missing_ptr must start at 0");
+      for (uint64_t i = 0; i < n; ++i) {
+         if (f.missing_ptr[i + 1] < f.missing_ptr[i]) fail(RDB_EINVAL, "missing_ptr must be non-decreasing");
+      }
+      // missing positions -> runs [start, end)
+      for (uint64_t i = 0; i < n; ++i) {
+         const uint32_t* a = f.missing_pos + f.missing_ptr[i];
+         const uint32_t* e = f.missing_pos + f.missing_ptr[i + 1];
+         for (const uint32_t* q = a; q < e;) {
+            const uint32_t s = *q;
+            uint32_t last = s;
+            ++q;
+            while (q < e && *q == last + 1) last = *q++;
+            if (q < e && *q <= last) fail(RDB_EINVAL, "missing positions of a row must ascend");
+            nrun_start.push_back(s);
+            nrun_end.push_back(last + 1);
+         }
+         nrun_ptr[i + 1] = nrun_start.size();
+      }
    }
+   // p inside one of row i's missing runs
    auto isMissing = [&](uint64_t i, uint32_t p) {
-      const uint32_t* a = f.missing_pos + f.missing_ptr[i];
-      const uint32_t* e = f.missing_pos + f.missing_ptr[i + 1];
-      return std::binary_search(a, e, p);
+      const uint32_t* a = nrun_start.data() + nrun_ptr[i];
+      const uint32_t* e = nrun_start.data() + nrun_ptr[i + 1];
+      const uint32_t* q = std::upper_bound(a, e, p);
+      return q != a && p < nrun_end[static_cast<size_t>(q - nrun_start.data()) - 1];
    };
    // containers by chunk
    std::vector<std::vector<uint64_t>> by_chunk(t.n_chunks);

@@ -29,6 +29,10 @@ SIGNATURES = [
     ("rdb_table_add_sequence_column", _int, [_vp, _cp, _int, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("rdb_table_add_sequence_column_finalized", _int, [_vp, _cp, _int, _vp, _vp, _u32, _u64, _vp, _vp, _vp, _vp, _vp, _vp,
                                                         _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
+    ("rdb_archive_scan_containers", _int, [_vp, _u64, _u64, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _vp]),
+    ("rdb_archive_scan_row_bitmaps", _int, [_vp, _u64, _u64, _u64, _vp, _vp, _vp, _vp]),
+    ("rdb_table_add_sequence_column_finalized_runs", _int, [_vp, _cp, _int, _vp, _vp, _u32, _u64, _vp, _vp, _vp, _vp, _vp,
+                                                             _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("rdb_seq_builder_create", _int, [_vp, _cp, _int, _vp, _u32, ctypes.POINTER(_vp)]),
     ("rdb_seq_builder_append", _int, [_vp, _vp, _vp, _vp, _vp, _u64]),
     ("rdb_seq_builder_finish", _int, [_vp]),
@@ -338,15 +342,18 @@ class DeviceTable:
         self.n_rows = spec.n_rows
 
     def _add_finalized(self, c: "FinalizedSeqColumnData") -> None:
+        runs = c.missing_run_ptr is not None
+        tail = ((c.missing_run_ptr, np.uint64), (c.missing_run_start, np.uint32), (c.missing_run_end, np.uint32)) if runs \
+            else ((c.missing_ptr, np.uint64), (c.missing_pos, np.uint32))
         a = [np.ascontiguousarray(np.asarray(x, dt)) for x, dt in (
             (c.reference, np.uint8), (c.local_reference, np.uint8), (c.key_position, np.uint32),
             (c.key_chunk, np.uint16), (c.key_symbol, np.uint8), (c.typecode, np.uint8), (c.cardinality, np.uint32),
             (c.payload_offset, np.uint64), (c.payload, np.uint8), (c.is_null, np.uint8), (c.cov_start, np.uint32),
-            (c.cov_end, np.uint32), (c.missing_ptr, np.uint64), (c.missing_pos, np.uint32))]
-        _check(lib().rdb_table_add_sequence_column_finalized(
-            self._h, c.name.encode(), c.alphabet, _p(a[0]), _p(a[1]), len(c.reference), len(a[2]), _p(a[2]), _p(a[3]),
-            _p(a[4]), _p(a[5]), _p(a[6]), _p(a[7]), _p(a[8]), len(a[8]), _p(a[9]), _p(a[10]), _p(a[11]), _p(a[12]),
-            _p(a[13])), self.engine.handle)
+            (c.cov_end, np.uint32)) + tail]
+        fn = lib().rdb_table_add_sequence_column_finalized_runs if runs else lib().rdb_table_add_sequence_column_finalized
+        _check(fn(self._h, c.name.encode(), c.alphabet, _p(a[0]), _p(a[1]), len(c.reference), len(a[2]), _p(a[2]),
+                  _p(a[3]), _p(a[4]), _p(a[5]), _p(a[6]), _p(a[7]), _p(a[8]), len(a[8]), _p(a[9]), _p(a[10]), _p(a[11]),
+                  *[_p(x) for x in a[12:]]), self.engine.handle)
         self.seq_lengths[c.name] = len(c.reference)
         self.alphabets[c.name] = c.alphabet
 

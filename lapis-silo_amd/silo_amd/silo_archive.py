@@ -183,9 +183,21 @@ class ArchiveReader:
 # ------------------------------------------------------------------------------------------------
 # CRoaring's portable serialization (`roaring_bitmap_portable_serialize`) and container_write forms
 # ------------------------------------------------------------------------------------------------
+_ENTRY = struct.Struct("<IHiIBQ")
+_ROW_BITMAP = struct.Struct("<IQ")
 _SERIAL_COOKIE_NO_RUN = 12346
 _SERIAL_COOKIE = 12347
 _NO_OFFSET_THRESHOLD = 4
+
+
+def _expand_runs(runs: np.ndarray) -> np.ndarray:
+    """(start, length - 1) u16 pairs to the u32 values they cover."""
+    start = runs[:, 0].astype(np.int64)
+    length = runs[:, 1].astype(np.int64) + 1
+    if not len(start):
+        return np.zeros(0, np.uint32)
+    base = np.repeat(start - np.concatenate([[0], np.cumsum(length)[:-1]]), length)
+    return (base + np.arange(int(length.sum()), dtype=np.int64)).astype(np.uint32)
 
 
 def container_values(typecode: int, cardinality: int, payload: bytes) -> np.ndarray:
@@ -201,8 +213,161 @@ def container_values(typecode: int, cardinality: int, payload: bytes) -> np.ndar
         runs = np.frombuffer(payload[2:2 + 4 * n_runs], dtype="<u2").reshape(-1, 2)
         if n_runs == 0:
             return np.zeros(0, np.uint16)
-        return np.concatenate([np.arange(s, int(s) + int(l) + 1, dtype=np.uint32) for s, l in runs]).astype(np.uint16)
+        return _expand_runs(runs).astype(np.uint16)
     raise SiloFormatError(f"unknown container typecode {typecode}")
+
+
+def _gather_u16(data: np.ndarray, at: np.ndarray) -> np.ndarray:
+    return data[at].astype(np.uint32) | (data[at + 1].astype(np.uint32) << np.uint32(8))
+
+
+def _ranges(starts: np.ndarray, counts: np.ndarray) -> np.ndarray:
+    """Concatenated arange(starts[i], starts[i] + counts[i])."""
+    counts = counts.astype(np.int64)
+    total = int(counts.sum())
+    if total == 0:
+        return np.zeros(0, np.int64)
+    base = np.repeat(starts.astype(np.int64) - np.concatenate([[0], np.cumsum(counts)[:-1]]), counts)
+    return base + np.arange(total, dtype=np.int64)
+
+
+def decode_bitmaps(data: np.ndarray, starts: np.ndarray, lens: np.ndarray, name: str = "roaring",
+                   block: int = 2048):
+    """Many portable roaring bitmaps in one buffer (bitmap i at data[starts[i]:starts[i] + lens[i]])
+    to a CSR (ptr u64 [n + 1], values u32), `block` bitmaps at a time (the temporaries stay in
+    cache)."""
+    n = len(starts)
+    ptrs, vals = [np.zeros(1, np.uint64)], []
+    total = 0
+    for b in range(0, n, block):
+        p, v = _decode_block(data, starts[b:b + block], lens[b:b + block], name)
+        ptrs.append(p[1:] + np.uint64(total))
+        vals.append(v)
+        total += len(v)
+    return np.concatenate(ptrs), (np.concatenate(vals) if vals else np.zeros(0, np.uint32))
+
+
+def decode_bitmap_runs(data: np.ndarray, starts: np.ndarray, lens: np.ndarray, name: str = "roaring",
+                       block: int = 4096):
+    """Many portable roaring bitmaps (as decode_bitmaps) to their runs: CSR ptr u64 [n + 1] and the
+    runs [start, end) u32.  Run containers give their runs as written; other containers' values are
+    joined into runs."""
+    n = len(starts)
+    ptrs, rs, re_ = [np.zeros(1, np.uint64)], [], []
+    total = 0
+    for b in range(0, n, block):
+        st = starts[b:b + block].astype(np.int64)
+        ln = lens[b:b + block].astype(np.int64)
+        m = len(st)
+        if np.any(ln < 8):
+            raise SiloFormatError(f"{name}: roaring bitmap too short")
+        u32 = lambda at: _gather_u16(data, at) | (_gather_u16(data, at + 2) << np.uint32(16))
+        cookie = u32(st)
+        # one run container: cookie (size 1), flag byte with the run bit, key, cardinality - 1,
+        # run count, (start, length - 1) pairs
+        run = (cookie == _SERIAL_COOKIE) & (ln >= 11)
+        run &= np.where(run, data[np.where(run, st + 4, 0)] & 1, 0) == 1
+        n_runs = np.where(run, _gather_u16(data, np.where(run, st + 9, 0)).astype(np.int64), 0)
+        run &= ln == 11 + 4 * n_runs
+        counts = np.zeros(m, np.int64)
+        r_idx = np.nonzero(run)[0]
+        pair_at = _ranges(st[r_idx] + 11, n_runs[r_idx] * 4)[::4]
+        key = np.repeat(_gather_u16(data, st[r_idx] + 5).astype(np.int64) << 16, n_runs[r_idx])
+        r_start = _gather_u16(data, pair_at).astype(np.int64) + key
+        r_end = r_start + _gather_u16(data, pair_at + 2).astype(np.int64) + 1
+        counts[r_idx] = n_runs[r_idx]
+        o_idx = np.nonzero(~run)[0]
+        o_start = o_end = np.zeros(0, np.int64)
+        if len(o_idx):
+            optr, ovals = _decode_block(data, st[o_idx], ln[o_idx], name)
+            v = ovals.astype(np.int64)
+            owner = np.repeat(np.arange(len(o_idx)), np.diff(optr.astype(np.int64)))
+            first = np.ones(len(v), bool)
+            first[1:] = (owner[1:] != owner[:-1]) | (v[1:] != v[:-1] + 1)
+            heads = np.nonzero(first)[0]
+            tails = np.concatenate([heads[1:], [len(v)]]) - 1
+            o_start, o_end = v[heads], v[tails] + 1
+            counts[o_idx] = np.bincount(owner[heads], minlength=len(o_idx))
+        ptr = np.zeros(m + 1, np.int64)
+        np.cumsum(counts, out=ptr[1:])
+        out_s = np.zeros(int(ptr[-1]), np.int64)
+        out_e = np.zeros(int(ptr[-1]), np.int64)
+        dest = _ranges(ptr[:-1][r_idx], n_runs[r_idx])
+        out_s[dest], out_e[dest] = r_start, r_end
+        dest = _ranges(ptr[:-1][o_idx], counts[o_idx])
+        out_s[dest], out_e[dest] = o_start, o_end
+        ptrs.append((ptr[1:] + total).astype(np.uint64))
+        rs.append(out_s.astype(np.uint32))
+        re_.append(out_e.astype(np.uint32))
+        total += int(ptr[-1])
+    empty = np.zeros(0, np.uint32)
+    return np.concatenate(ptrs), (np.concatenate(rs) if rs else empty), (np.concatenate(re_) if re_ else empty)
+
+
+def _decode_block(data: np.ndarray, starts: np.ndarray, lens: np.ndarray, name: str):
+    """decode_bitmaps for one block: bitmaps of one array or one run container (key and
+    cardinality, then the values or the runs) are decoded together; the rest one by one."""
+    n = len(starts)
+    ptr = np.zeros(n + 1, np.uint64)
+    if n == 0:
+        return ptr, np.zeros(0, np.uint32)
+    starts = starts.astype(np.int64)
+    lens = lens.astype(np.int64)
+    if np.any(lens < 8):
+        raise SiloFormatError(f"{name}: roaring bitmap too short")
+    u32 = lambda at: _gather_u16(data, at) | (_gather_u16(data, at + 2) << np.uint32(16))
+    cookie = u32(starts)
+    # one array container without runs: cookie, count 1, key, cardinality - 1, offset 16, values
+    plain = (cookie == _SERIAL_COOKIE_NO_RUN) & (lens >= 18)
+    plain &= np.where(plain, u32(np.where(plain, starts + 4, 0)), 0) == 1
+    p_card = np.where(plain, _gather_u16(data, np.where(plain, starts + 10, 0)).astype(np.int64) + 1, 0)
+    plain &= (p_card <= 4096) & (lens == 16 + 2 * p_card)
+    # one container with the run cookie: cookie (size 1), flag byte, key, cardinality - 1, then
+    # runs (count, (start, length - 1) pairs) or array values
+    single = (cookie == _SERIAL_COOKIE) & (lens >= 9)
+    flag = np.where(single, data[np.where(single, starts + 4, 0)], 0)
+    s_card = np.where(single, _gather_u16(data, np.where(single, starts + 7, 0)).astype(np.int64) + 1, 0)
+    run = single & (flag & 1 == 1) & (lens >= 11)
+    n_runs = np.where(run, _gather_u16(data, np.where(run, starts + 9, 0)).astype(np.int64), 0)
+    run &= lens == 11 + 4 * n_runs
+    arr1 = single & (flag & 1 == 0) & (s_card <= 4096) & (lens == 9 + 2 * s_card)
+    key = np.zeros(n, np.uint32)
+    key[plain] = _gather_u16(data, starts[plain] + 8)
+    key[run | arr1] = _gather_u16(data, starts[run | arr1] + 5)
+    counts = np.zeros(n, np.int64)
+    counts[plain] = p_card[plain]
+    counts[arr1] = s_card[arr1]
+    # runs: lengths from the pairs
+    r_idx = np.nonzero(run)[0]
+    pair_at = _ranges(starts[r_idx] + 11, n_runs[r_idx] * 4)[::4]
+    r_start = _gather_u16(data, pair_at).astype(np.int64)
+    r_len = _gather_u16(data, pair_at + 2).astype(np.int64) + 1
+    run_owner = np.repeat(r_idx, n_runs[r_idx])
+    np.add.at(counts, run_owner, r_len)
+    other = np.nonzero(~(plain | run | arr1))[0]
+    decoded = {}
+    for i in other.tolist():
+        v = roaring_values(data[starts[i]:starts[i] + lens[i]].tobytes(), name)
+        decoded[i] = v
+        counts[i] = len(v)
+    np.cumsum(counts, out=ptr[1:])
+    values = np.zeros(int(ptr[-1]), np.uint32)
+    for sel, off in ((plain, 16), (arr1, 9)):
+        idx = np.nonzero(sel)[0]
+        at = _ranges(starts[idx] + off, counts[idx] * 2)[::2]
+        dest = _ranges(ptr[:-1][idx].astype(np.int64), counts[idx])
+        values[dest] = _gather_u16(data, at) | (np.repeat(key[idx], counts[idx]) << np.uint32(16))
+    if len(r_idx):
+        run_first = np.zeros(len(r_idx), np.int64)
+        run_first[1:] = np.cumsum(n_runs[r_idx])[:-1]
+        # each run's place in the output: its bitmap's start plus the lengths of the runs before it
+        within = np.cumsum(r_len) - r_len - np.repeat(np.cumsum(r_len)[run_first] - r_len[run_first], n_runs[r_idx])
+        dest = _ranges(ptr[:-1][run_owner].astype(np.int64) + within, r_len)
+        values[dest] = (_ranges(r_start, r_len) | (np.repeat(key[run_owner], r_len).astype(np.int64) << 16)).astype(
+            np.uint32)
+    for i, v in decoded.items():
+        values[int(ptr[i]):int(ptr[i + 1])] = v
+    return ptr, values
 
 
 def roaring_values(buf: bytes, name: str = "roaring") -> np.ndarray:
@@ -210,6 +375,12 @@ def roaring_values(buf: bytes, name: str = "roaring") -> np.ndarray:
     if len(buf) < 4:
         raise SiloFormatError(f"{name}: roaring bitmap too short")
     cookie = struct.unpack_from("<I", buf)[0]
+    # one array container, the common case of small bitmaps: cookie, count 1, key, cardinality - 1,
+    # offset, values
+    if cookie == _SERIAL_COOKIE_NO_RUN and len(buf) >= 16 and struct.unpack_from("<I", buf, 4)[0] == 1:
+        key, c1 = struct.unpack_from("<HH", buf, 8)
+        if c1 < 4096 and len(buf) == 16 + 2 * (c1 + 1):
+            return np.frombuffer(buf, "<u2", c1 + 1, 16).astype(np.uint32) | np.uint32(key << 16)
     pos = 4
     run_flags = None
     if cookie & 0xFFFF == _SERIAL_COOKIE:
@@ -234,7 +405,7 @@ def roaring_values(buf: bytes, name: str = "roaring") -> np.ndarray:
             n_runs = struct.unpack_from("<H", buf, pos)[0]
             runs = np.frombuffer(buf[pos + 2:pos + 2 + 4 * n_runs], dtype="<u2").reshape(-1, 2)
             pos += 2 + 4 * n_runs
-            vals = np.concatenate([np.arange(s, int(s) + int(l) + 1, dtype=np.uint32) for s, l in runs])
+            vals = _expand_runs(runs)
         elif card > 4096:
             words = np.frombuffer(buf[pos:pos + 8192], dtype="<u8")
             pos += 8192
@@ -440,13 +611,18 @@ class SequenceColumnState:
     cardinality: np.ndarray
     payload_offset: np.ndarray
     payload: np.ndarray
-    missing: dict                   # row -> sorted u32 positions of its internal missing symbols
+    missing_rows: np.ndarray        # u32 global ids of the rows with internal missing symbols
+    missing_ptr: np.ndarray         # u64 [len(missing_rows) + 1]: their runs in missing_start/end
+    missing_start: np.ndarray       # u32 runs [start, end) of missing positions, ascending per row
+    missing_end: np.ndarray
     start_end: np.ndarray           # (rows, 2) u32 covered ranges, in row order
     null_rows: np.ndarray           # u32 row ids
     sequence_count: int
     num_chunks: int
     info: tuple                     # SequenceColumnInfo: (sequence_count, vertical, horizontal sizes)
     insertion_count: int
+    span: tuple = (0, 0)            # its bytes in the table archive
+    classes_before: frozenset = frozenset()  # class information written before it
 
 
 @dataclass
@@ -494,12 +670,26 @@ def _string_column(r: ArchiveReader) -> dict:
     return {"null_rows": nulls, "chunks": chunks}
 
 
+class SavedBitmap:
+    """A roaring bitmap of the archive kept in its portable form; `values()` decodes it."""
+    __slots__ = ("raw",)
+
+    def __init__(self, raw: bytes):
+        self.raw = raw
+
+    def values(self) -> np.ndarray:
+        return roaring_values(self.raw)
+
+
 def _unordered_bitmaps(r: ArchiveReader) -> dict:
+    """unordered_map<u32, Roaring>: value id -> SavedBitmap (the engine rebuilds these row sets from
+    the value ids, so they are not decoded here)."""
     out = {}
     for _ in range(r.unordered_items("unordered_map<u32,Roaring>")):
         r.cls("pair<const u32,Roaring>")
         k = r.u32()
-        out[k] = r.roaring()
+        r.cls("roaring::Roaring")
+        out[k] = SavedBitmap(bytes(r.take(r.count())))
     return out
 
 
@@ -558,6 +748,52 @@ def _insertion_index(r: ArchiveReader, tag: str) -> int:
     return count
 
 
+def _buffer(r: ArchiveReader):
+    import ctypes
+
+    arr = np.frombuffer(r.data, np.uint8)
+    return arr, ctypes.c_void_p(arr.ctypes.data)
+
+
+def _scan_containers(r: ArchiveReader, n: int, name: str):
+    """n vertical-index records from r.pos (rdb_archive_scan_containers): position, chunk, symbol,
+    cardinality, typecode, payload offsets and the packed payload."""
+    import ctypes
+
+    from .engine import _p, lib
+
+    arr, base = _buffer(r)
+    out = [np.empty(n, dt) for dt in (np.uint32, np.uint16, np.int32, np.uint32, np.uint8, np.uint64)]
+    total, end = ctypes.c_uint64(), ctypes.c_uint64()
+    for payload in (None, True):
+        buf = None if payload is None else np.empty(max(total.value, 1), np.uint8)
+        rc = lib().rdb_archive_scan_containers(base, len(arr), r.pos, n, *[_p(a) for a in out],
+                                               None if buf is None else _p(buf), 0 if buf is None else len(buf),
+                                               ctypes.byref(total), ctypes.byref(end))
+        if rc != 0:
+            r.fail(f"column {name!r}: {lib().rdb_last_error(None).decode()}")
+    r.pos = end.value
+    return out + [buf[:total.value]]
+
+
+def _scan_row_bitmaps(r: ArchiveReader, n: int, name: str):
+    """n horizontal-index records from r.pos (rdb_archive_scan_row_bitmaps): row ids and where each
+    bitmap lies."""
+    import ctypes
+
+    from .engine import _p, lib
+
+    arr, base = _buffer(r)
+    rows, starts, lens = np.empty(n, np.uint32), np.empty(n, np.uint64), np.empty(n, np.uint64)
+    end = ctypes.c_uint64()
+    rc = lib().rdb_archive_scan_row_bitmaps(base, len(arr), r.pos, n, _p(rows), _p(starts), _p(lens),
+                                            ctypes.byref(end))
+    if rc != 0:
+        r.fail(f"column {name!r}: {lib().rdb_last_error(None).decode()}")
+    r.pos = end.value
+    return rows, starts.astype(np.int64), lens.astype(np.int64)
+
+
 def _sequence_column(r: ArchiveReader, col: ColumnSchema) -> SequenceColumnState:
     alphabet = ALPHABET_NUC if col.type == NUC_SEQUENCE else ALPHABET_AA
     tag = "Nucleotide" if alphabet == ALPHABET_NUC else "AminoAcid"
@@ -571,36 +807,65 @@ def _sequence_column(r: ArchiveReader, col: ColumnSchema) -> SequenceColumnState
         r.fail(f"column {col.name!r}: local reference holds a character outside the alphabet")
     r.cls(f"VerticalSequenceIndex<{tag}>")
     n = r.items(f"map<SequenceDiffKey<{tag}>,RoaringContainer>")
-    pos = np.empty(n, np.uint32)
-    chunk = np.empty(n, np.uint16)
-    sym = np.empty(n, np.uint8)
-    tc = np.empty(n, np.uint8)
-    card = np.empty(n, np.uint32)
+    keys = np.empty((n, 5), np.int64)  # position, chunk, symbol, cardinality, typecode
     offs = np.zeros(n, np.uint64)
     payload = []
     total = 0
-    for i in range(n):
-        r.cls(f"pair<const SequenceDiffKey<{tag}>,RoaringContainer>")
-        r.cls(f"SequenceDiffKey<{tag}>")
-        pos[i] = r.u32()
-        chunk[i] = r.u16()
-        s = r.i32()
-        if not 0 <= s < n_symbols:
-            r.fail(f"column {col.name!r}: symbol {s} out of range")
-        sym[i] = s
-        r.cls("RoaringContainer")
-        card[i] = r.u32()
-        tc[i] = r.u8()
+    classes = (f"pair<const SequenceDiffKey<{tag}>,RoaringContainer>", f"SequenceDiffKey<{tag}>", "RoaringContainer")
+    i = 0
+    while i < n:
+        if all(k in r.classes and not r.classes[k][0] for k in classes):
+            # every class of an entry is known and untracked: the rest are fixed 23-byte headers
+            # (u32 position, u16 chunk, i32 symbol, u32 cardinality, u8 typecode, u64 length) and
+            # the container bytes, scanned natively
+            rest = _scan_containers(r, n - i, col.name)
+            keys[i:, 0], keys[i:, 1], keys[i:, 2], keys[i:, 3], keys[i:, 4] = rest[:5]
+            offs[i:] = rest[5] + np.uint64(total)
+            payload.append(rest[6])
+            total += len(rest[6])
+            break
+        for k in classes[:2]:
+            r.cls(k)
+        keys[i, 0] = r.u32()
+        keys[i, 1] = r.u16()
+        keys[i, 2] = r.i32()
+        r.cls(classes[2])
+        keys[i, 3] = r.u32()
+        keys[i, 4] = r.u8()
         buf = r.string_bytes()
         offs[i] = total
         payload.append(buf)
         total += len(buf)
+        i += 1
+    if n and (keys[:, 2].min() < 0 or keys[:, 2].max() >= n_symbols):
+        r.fail(f"column {col.name!r}: symbol out of range")
+    pos = keys[:, 0].astype(np.uint32)
+    chunk = keys[:, 1].astype(np.uint16)
+    sym = keys[:, 2].astype(np.uint8)
+    card = keys[:, 3].astype(np.uint32)
+    tc = keys[:, 4].astype(np.uint8)
     r.cls("HorizontalCoverageIndex")
-    missing = {}
-    for _ in range(r.items("map<u32,Roaring>")):
-        r.cls("pair<const u32,Roaring>")
-        row = r.u32()
-        missing[row] = r.roaring()
+    n_h = r.items("map<u32,Roaring>")
+    h_rows = np.empty(n_h, np.uint32)
+    h_start = np.empty(n_h, np.int64)
+    h_len = np.empty(n_h, np.int64)
+    i = 0
+    entry_classes = ("pair<const u32,Roaring>", "roaring::Roaring")
+    while i < n_h:
+        if all(k in r.classes and not r.classes[k][0] for k in entry_classes):
+            # the rest are u32 row, u64 length, the bitmap: scanned natively
+            h_rows[i:], h_start[i:], h_len[i:] = _scan_row_bitmaps(r, n_h - i, col.name)
+            break
+        r.cls(entry_classes[0])
+        h_rows[i] = r.u32()
+        r.cls(entry_classes[1])
+        h_len[i] = r.count()
+        h_start[i] = r.pos
+        r.take(int(h_len[i]))
+        i += 1
+    if n_h > 1 and np.any(np.diff(h_rows.astype(np.int64)) <= 0):
+        r.fail(f"column {col.name!r}: coverage index rows out of order")
+    h_ptr, h_run_start, h_run_end = decode_bitmap_runs(np.frombuffer(r.data, np.uint8), h_start, h_len, r.name)
     chunks = [_pair_array(r) for _ in range(r.items("vector<vector<pair<u32,u32>>>"))]
     _pair_array(r)  # batch_start_ends
     insertions = _insertion_index(r, tag)
@@ -612,7 +877,9 @@ def _sequence_column(r: ArchiveReader, col: ColumnSchema) -> SequenceColumnState
     start_end = np.concatenate(chunks) if chunks else np.zeros((0, 2), np.uint32)
     return SequenceColumnState(
         col.name, alphabet, col.reference, local_ref, pos, chunk, sym, tc, card, offs,
-        np.frombuffer(b"".join(payload), np.uint8).copy(), missing, start_end, nulls, seq_count, num_chunks, info,
+        np.frombuffer(b"".join(payload), np.uint8).copy(), h_rows, h_ptr, h_run_start, h_run_end, start_end, nulls,
+        seq_count,
+        num_chunks, info,
         insertions)
 
 
@@ -645,7 +912,10 @@ def read_table(data: bytes, schema: TableSchema, name: str | None = None) -> Tab
                 cols[col.name] = {"null_rows": nulls, "chunks": _chunked(r, "i32", np.int32),
                                   "is_sorted": r.boolean()}
             elif t in (NUC_SEQUENCE, AA_SEQUENCE):
+                start, before = r.pos, frozenset(r.classes)
                 cols[col.name] = _sequence_column(r, col)
+                cols[col.name].span = (start, r.pos)
+                cols[col.name].classes_before = before
             elif t == ZSTD_STRING:
                 r.cls("ZstdCompressedStringColumn")
                 r.cls("ChunkedValueBuffer<string>")
@@ -693,20 +963,18 @@ def finalized_column(col: SequenceColumnState, chunk_sizes: np.ndarray) -> Final
         raise SiloFormatError(f"column {col.name!r}: {col.start_end.shape[0]} covered ranges for {n} rows")
     if col.reference is None or len(col.local_reference) != len(col.reference):
         raise SiloFormatError(f"column {col.name!r}: local reference and reference differ in length")
-    missing_rows = np.array(sorted(col.missing), np.uint32)
-    dense = _dense_rows(missing_rows, chunk_sizes, col.name)
+    # rows come in ascending global id, which is row-layout order
+    dense = _dense_rows(col.missing_rows, chunk_sizes, col.name)
     counts = np.zeros(n, np.uint64)
-    counts[dense] = [len(col.missing[int(g)]) for g in missing_rows]
+    counts[dense] = np.diff(col.missing_ptr.astype(np.int64))
     ptr = np.zeros(n + 1, np.uint64)
     np.cumsum(counts, out=ptr[1:])
-    pos = np.zeros(int(ptr[-1]), np.uint32)
-    for g, d in zip(missing_rows.tolist(), dense.tolist()):
-        pos[int(ptr[d]):int(ptr[d + 1])] = col.missing[g]
     is_null = _flags(_dense_rows(col.null_rows, chunk_sizes, col.name), n)
     return FinalizedSeqColumnData(
         col.name, col.alphabet, col.reference, col.local_reference, col.key_position, col.key_chunk, col.key_symbol,
         col.typecode, col.cardinality, col.payload_offset, col.payload, is_null,
-        np.ascontiguousarray(col.start_end[:, 0]), np.ascontiguousarray(col.start_end[:, 1]), ptr, pos)
+        np.ascontiguousarray(col.start_end[:, 0]), np.ascontiguousarray(col.start_end[:, 1]), None, None,
+        ptr, col.missing_start, col.missing_end)
 
 
 def lineage_arrays(tree: LineageTree, n_values: int) -> tuple[np.ndarray, np.ndarray]:
@@ -830,3 +1098,548 @@ def most_recent_data_directory(root: str) -> str | None:
         if version == SERIALIZATION_VERSION and timestamp == int(entry):
             found.append((timestamp, d))
     return max(found)[1] if found else None
+
+
+# ------------------------------------------------------------------------------------------------
+# Writing a saved state (`Database::saveDatabaseState`, `database.cpp:380-415`) for the column types
+# the engine holds: sequence, int32, date32 and dictionary-encoded columns
+# ------------------------------------------------------------------------------------------------
+class ArchiveWriter:
+    """A Boost binary archive being written, with the class information already written."""
+
+    def __init__(self):
+        self.buf = bytearray()
+        self.classes: set = set()
+        self.string(b"serialization::archive")
+        self.buf += struct.pack("<H", 20) + b"\x04\x08\x04\x08" + struct.pack("<i", 1)
+
+    def cls(self, key: str, version: int = 0):
+        if key not in self.classes:
+            self.classes.add(key)
+            self.buf += struct.pack("<BI", 0, version)
+
+    def u8(self, v: int):
+        self.buf += struct.pack("<B", v)
+
+    def u16(self, v: int):
+        self.buf += struct.pack("<H", v)
+
+    def u32(self, v: int):
+        self.buf += struct.pack("<I", v)
+
+    def i32(self, v: int):
+        self.buf += struct.pack("<i", v)
+
+    def u64(self, v: int):
+        self.buf += struct.pack("<Q", v)
+
+    def string(self, s):
+        b = s.encode("utf-8") if isinstance(s, str) else bytes(s)
+        self.u64(len(b))
+        self.buf += b
+
+    def items(self, key: str, n: int):
+        self.cls(key)
+        self.u64(n)
+        self.u32(0)
+
+    def unordered_items(self, key: str, n: int, buckets: int | None = None):
+        self.cls(key)
+        self.u64(n)
+        self.u64(_bucket_count(n) if buckets is None else buckets)
+        self.u32(0)
+
+    def array(self, values: np.ndarray, dtype, key: str | None = None):
+        if key is not None:
+            self.cls(key)
+        a = np.ascontiguousarray(values, dtype=np.dtype(dtype).newbyteorder("<"))
+        self.u64(a.size)
+        self.buf += a.tobytes()
+
+    def roaring(self, values: np.ndarray):
+        self.cls("roaring::Roaring")
+        b = portable_bytes(values)
+        self.u64(len(b))
+        self.buf += b
+
+
+def _bucket_count(n: int) -> int:
+    """The bucket count written for an unordered_map of n elements (libstdc++'s prime list, 1 for a
+    default-constructed empty map); readers ignore it."""
+    if n == 0:
+        return 1
+    for p in (13, 29, 59, 127, 257, 541, 1109, 2357, 5087, 10273, 20753, 42043, 85229, 172933, 351061, 712697,
+              1447153, 2938679, 5967347, 12117689, 24607243, 49969847, 101473717):
+        if p >= n:
+            return p
+    return n | 1
+
+
+def _runs_of(low: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    breaks = np.nonzero(np.diff(low.astype(np.int64)) != 1)[0]
+    starts = np.concatenate([low[:1], low[breaks + 1]]).astype(np.int64)
+    ends = np.concatenate([low[breaks], low[-1:]]).astype(np.int64)
+    return starts, ends
+
+
+def _containers(values: np.ndarray, run_optimize: bool) -> list:
+    """(key, cardinality, is_run, container bytes) per 2^16 block: arrays up to 4,096 values, else
+    bitsets; with run_optimize a container becomes runs where that is smaller (CRoaring's
+    `runOptimize`)."""
+    v = np.unique(np.asarray(values, np.uint32))
+    out = []
+    if not v.size:
+        return out
+    hi = v >> np.uint32(16)
+    cuts = np.nonzero(np.diff(hi))[0] + 1
+    for part in np.split(v, cuts):
+        key = int(part[0] >> 16)
+        low = (part & np.uint32(0xFFFF)).astype(np.uint16)
+        card = len(low)
+        size = 2 * card if card <= 4096 else 8192
+        starts, ends = _runs_of(low)
+        if run_optimize and 2 + 4 * len(starts) < size:
+            body = struct.pack("<H", len(starts)) + np.stack([starts, ends - starts], 1).astype("<u2").tobytes()
+            out.append((key, card, True, body))
+        elif card <= 4096:
+            out.append((key, card, False, low.astype("<u2").tobytes()))
+        else:
+            bits = np.zeros(65536, np.uint8)
+            bits[low] = 1
+            out.append((key, card, False, np.packbits(bits, bitorder="little").tobytes()))
+    return out
+
+
+def portable_bytes(values: np.ndarray, run_optimize: bool = True) -> bytes:
+    """CRoaring's portable serialization of a set of u32 (RoaringFormatSpec): cookie (with run flags
+    when a container is a run), keys and cardinalities, offsets (without run containers, or from 4
+    containers on), the containers."""
+    conts = _containers(values, run_optimize)
+    n = len(conts)
+    has_run = any(c[2] for c in conts)
+    if has_run:
+        flags = bytearray((n + 7) // 8)
+        for i, c in enumerate(conts):
+            if c[2]:
+                flags[i // 8] |= 1 << (i % 8)
+        head = bytearray(struct.pack("<I", _SERIAL_COOKIE | ((n - 1) << 16)) + bytes(flags))
+    else:
+        head = bytearray(struct.pack("<II", _SERIAL_COOKIE_NO_RUN, n))
+    for key, card, _, _ in conts:
+        head += struct.pack("<HH", key, card - 1)
+    if not has_run or n >= _NO_OFFSET_THRESHOLD:
+        off = len(head) + 4 * n
+        for c in conts:
+            head += struct.pack("<I", off)
+            off += len(c[3])
+    return bytes(head) + b"".join(c[3] for c in conts)
+
+
+def _one_container_bitmap(starts: list, ends: list) -> bytes:
+    """The run-optimized portable form of a set of runs inside one 2^16 block (a row's missing
+    positions: the genome is shorter than 2^16)."""
+    key = starts[0] >> 16
+    card = sum(e - s for s, e in zip(starts, ends))
+    size = 2 * card if card <= 4096 else 8192
+    if 2 + 4 * len(starts) < size:
+        body = b"".join(struct.pack("<HH", s & 0xFFFF, e - s - 1) for s, e in zip(starts, ends))
+        return struct.pack("<IBHHH", _SERIAL_COOKIE, 1, key, card - 1, len(starts)) + body
+    values = np.concatenate([np.arange(s, e, dtype=np.uint32) for s, e in zip(starts, ends)])
+    return portable_bytes(values, run_optimize=False)
+
+
+def _frozen_size(values: np.ndarray) -> int:
+    """`Roaring::getSizeInBytes(false)`: the smaller of the portable form + 1 and a plain u32 list."""
+    card = int(np.unique(np.asarray(values, np.uint32)).size)
+    return min(len(portable_bytes(values)) + 1, 1 + 4 + 4 * card)
+
+
+def _global_ids(rows: np.ndarray, chunk_sizes: np.ndarray) -> np.ndarray:
+    offsets = np.zeros(len(chunk_sizes) + 1, np.int64)
+    np.cumsum(chunk_sizes, out=offsets[1:])
+    rows = np.asarray(rows, np.int64)
+    chunk = np.searchsorted(offsets, rows, side="right") - 1
+    return ((chunk << 16) | (rows - offsets[chunk])).astype(np.uint32)
+
+
+_CONTAINER_HEADER = np.dtype([("position", "<u4"), ("chunk", "<u2"), ("symbol", "<i4"), ("cardinality", "<u4"),
+                              ("typecode", "u1"), ("length", "<u8")])
+_RUN_BITMAP_HEADER = np.dtype([("row", "<u4"), ("length", "<u8"), ("cookie", "<u4"), ("flags", "u1"), ("key", "<u2"),
+                               ("card", "<u2"), ("runs", "<u2")])
+
+
+def _interleave(headers: np.ndarray, body_src: np.ndarray, body_at: np.ndarray, body_len: np.ndarray) -> np.ndarray:
+    """Records of a fixed header each followed by body_len[i] bytes of body_src from body_at[i]."""
+    h = headers.dtype.itemsize
+    size = h + body_len
+    start = np.zeros(len(size), np.int64)
+    start[1:] = np.cumsum(size)[:-1]
+    out = np.empty(int(size.sum()), np.uint8)
+    out[(start[:, None] + np.arange(h)).ravel()] = headers.view(np.uint8).reshape(-1, h).ravel()
+    out[_ranges(start + h, body_len)] = body_src[_ranges(body_at, body_len)]
+    return out
+
+
+def _records(w: ArchiveWriter, order: np.ndarray, c: FinalizedSeqColumnData, payload: np.ndarray, offs: np.ndarray,
+             sizes: np.ndarray, block: int = 1 << 16):
+    """Vertical-index records after the first: header + container bytes, block by block."""
+    for b in range(0, len(order), block):
+        sel = order[b:b + block]
+        hdr = np.empty(len(sel), _CONTAINER_HEADER)
+        hdr["position"] = np.asarray(c.key_position)[sel]
+        hdr["chunk"] = np.asarray(c.key_chunk)[sel]
+        hdr["symbol"] = np.asarray(c.key_symbol)[sel]
+        hdr["cardinality"] = np.asarray(c.cardinality)[sel]
+        hdr["typecode"] = np.asarray(c.typecode)[sel]
+        hdr["length"] = sizes[b:b + block]
+        w.buf += _interleave(hdr, payload, offs[b:b + block], sizes[b:b + block]).tobytes()
+
+
+def _row_bitmap_records(w: ArchiveWriter, rows: np.ndarray, gids: np.ndarray, rptr: np.ndarray, rstart: np.ndarray,
+                        rend: np.ndarray, block: int = 1 << 15) -> int:
+    """Horizontal-index records after the first: row id, then the row's missing positions as a
+    run-optimized portable bitmap of one container (the genome is shorter than 2^16).  Rows whose
+    runs would not be the smaller form are written one by one.  Returns their frozen sizes' sum."""
+    total = 0
+    rstart = rstart.astype(np.int64)
+    rend = rend.astype(np.int64)
+    for b in range(0, len(rows), block):
+        sel = rows[b:b + block].astype(np.int64)
+        g = gids[b:b + block]
+        nr = (rptr[sel + 1] - rptr[sel]).astype(np.int64)
+        idx = _ranges(rptr[sel].astype(np.int64), nr)
+        lens = rend[idx] - rstart[idx]
+        owner = np.repeat(np.arange(len(sel)), nr)
+        card = np.bincount(owner, weights=lens, minlength=len(sel)).astype(np.int64)
+        as_runs = 2 + 4 * nr < np.where(card <= 4096, 2 * card, 8192)
+        if as_runs.all():
+            hdr = np.empty(len(sel), _RUN_BITMAP_HEADER)
+            hdr["row"] = g
+            hdr["length"] = 11 + 4 * nr
+            hdr["cookie"] = _SERIAL_COOKIE
+            hdr["flags"] = 1
+            hdr["key"] = 0
+            hdr["card"] = card - 1
+            hdr["runs"] = nr
+            pairs = np.stack([rstart[idx], lens - 1], 1).astype("<u2").view(np.uint8).ravel()
+            body_at = np.zeros(len(sel), np.int64)
+            body_at[1:] = np.cumsum(4 * nr)[:-1]
+            w.buf += _interleave(hdr, pairs, body_at, 4 * nr).tobytes()
+            total += int(np.minimum(11 + 4 * nr + 1, 5 + 4 * card).sum())
+            continue
+        for j in range(len(sel)):
+            a, e = int(rptr[sel[j]]), int(rptr[sel[j] + 1])
+            bm = _one_container_bitmap(rstart[a:e].tolist(), rend[a:e].tolist())
+            w.u32(int(g[j]))
+            w.u64(len(bm))
+            w.buf += bm
+            total += min(len(bm) + 1, 5 + 4 * int(card[j]))
+    return total
+
+
+def _write_sequence_column(w: ArchiveWriter, c: FinalizedSeqColumnData, chunk_sizes: np.ndarray):
+    tag = "Nucleotide" if c.alphabet == ALPHABET_NUC else "AminoAcid"
+    chars = _symbol_chars(c.alphabet)
+    w.cls(f"SequenceColumn<{tag}>")
+    w.string("".join(chars[s] for s in np.asarray(c.local_reference).tolist()))
+    w.cls(f"VerticalSequenceIndex<{tag}>")
+    k = len(c.key_position)
+    order = np.lexsort((np.asarray(c.key_symbol), np.asarray(c.key_chunk), np.asarray(c.key_position)))
+    w.items(f"map<SequenceDiffKey<{tag}>,RoaringContainer>", k)
+    payload = np.asarray(c.payload, np.uint8)
+    tcs = np.asarray(c.typecode, np.int64)[order]
+    cards = np.asarray(c.cardinality, np.int64)[order]
+    offs = np.asarray(c.payload_offset, np.int64)[order]
+    n_runs = np.zeros(k, np.int64)
+    runs = np.nonzero(tcs == 3)[0]
+    n_runs[runs] = payload[offs[runs]].astype(np.int64) | (payload[offs[runs] + 1].astype(np.int64) << 8)
+    sizes = np.where(tcs == 1, 8192, np.where(tcs == 2, 2 * cards, 2 + 4 * n_runs))
+    vertical_size = int(sizes.sum())
+    if k:
+        # the first entry carries the class information; the rest are fixed headers + bytes
+        i = int(order[0])
+        w.cls(f"pair<const SequenceDiffKey<{tag}>,RoaringContainer>")
+        w.cls(f"SequenceDiffKey<{tag}>")
+        w.u32(int(c.key_position[i]))
+        w.u16(int(c.key_chunk[i]))
+        w.i32(int(c.key_symbol[i]))
+        w.cls("RoaringContainer")
+        w.u32(int(c.cardinality[i]))
+        w.u8(int(c.typecode[i]))
+        w.string(payload[offs[0]:offs[0] + sizes[0]].tobytes())
+        _records(w, order[1:], c, payload, offs[1:], sizes[1:])
+    w.cls("HorizontalCoverageIndex")
+    n = len(c.cov_start)
+    rptr, rstart, rend = c.missing_runs()
+    rptr = rptr.astype(np.int64)
+    with_missing = np.nonzero(np.diff(rptr) > 0)[0]
+    gids = _global_ids(with_missing, chunk_sizes)
+    w.items("map<u32,Roaring>", len(with_missing))
+    horizontal_size = 0
+    if len(with_missing):
+        row = int(with_missing[0])
+        w.cls("pair<const u32,Roaring>")
+        w.u32(int(gids[0]))
+        w.cls("roaring::Roaring")
+        a, b = int(rptr[row]), int(rptr[row + 1])
+        bm = _one_container_bitmap(rstart[a:b].tolist(), rend[a:b].tolist())
+        w.u64(len(bm))
+        w.buf += bm
+        card = int((rend[a:b].astype(np.int64) - rstart[a:b]).sum())
+        horizontal_size = min(len(bm) + 1, 5 + 4 * card)
+        horizontal_size += _row_bitmap_records(w, with_missing[1:], gids[1:], rptr, rstart, rend)
+    start_end = np.stack([np.asarray(c.cov_start, np.uint32), np.asarray(c.cov_end, np.uint32)], 1)
+    offsets = np.concatenate([[0], np.cumsum(chunk_sizes)]).astype(np.int64)
+    w.items("vector<vector<pair<u32,u32>>>", len(chunk_sizes))
+    batch = []
+    for ci in range(len(chunk_sizes)):
+        part = start_end[offsets[ci]:offsets[ci + 1]]
+        w.cls("vector<pair<u32,u32>>")
+        w.u64(len(part))
+        w.buf += part.astype("<u4").tobytes()
+        batch.append((int(part[:, 0].min()) if len(part) else 0xFFFFFFFF, int(part[:, 1].max()) if len(part) else 0))
+    w.cls("vector<pair<u32,u32>>")
+    w.u64(len(batch))
+    w.buf += np.asarray(batch, "<u4").reshape(-1, 2).tobytes()
+    w.cls(f"InsertionIndex<{tag}>")
+    # an empty insertion index as the reference saves it (its position map shows 2 buckets)
+    w.unordered_items(f"unordered_map<u32,InsertionPosition<{tag}>>", 0, 2)
+    w.unordered_items("unordered_map<u32,unordered_map<string,Roaring>>", 0)
+    w.cls("SequenceColumnInfo")
+    w.u32(n)
+    w.u64(vertical_size)
+    w.u64(horizontal_size)
+    w.u32(n)
+    w.roaring(_global_ids(np.nonzero(np.asarray(c.is_null))[0], chunk_sizes))
+    w.u16(len(chunk_sizes))
+
+
+def _chunks(values: np.ndarray, chunk_sizes: np.ndarray) -> list:
+    offsets = np.concatenate([[0], np.cumsum(chunk_sizes)]).astype(np.int64)
+    return [values[offsets[i]:offsets[i + 1]] for i in range(len(chunk_sizes))]
+
+
+def _write_chunked(w: ArchiveWriter, tag: str, dtype, values: np.ndarray, chunk_sizes: np.ndarray):
+    w.cls(f"ChunkedValueBuffer<{tag}>")
+    parts = _chunks(np.asarray(values), chunk_sizes)
+    w.items(f"vector<vector<{tag}>>", len(parts))
+    for part in parts:
+        w.array(part, dtype)
+
+
+def _resolved(c: DictColumnData) -> np.ndarray:
+    n = len(c.names)
+    alias = np.full(n, -1, np.int64) if c.lineage_alias is None else np.asarray(c.lineage_alias, np.int64)
+    return np.where(alias >= 0, alias, np.arange(n))
+
+
+def _write_bitmap_map(w: ArchiveWriter, groups: dict, encoded: dict | None = None):
+    """unordered_map<u32, Roaring>; `encoded` caches the portable bytes of maps written again."""
+    w.unordered_items("unordered_map<u32,Roaring>", len(groups))
+    for k, rows in groups.items():
+        w.cls("pair<const u32,Roaring>")
+        w.u32(k)
+        w.cls("roaring::Roaring")
+        b = encoded.get(k) if encoded is not None else None
+        if b is None:
+            b = portable_bytes(rows)
+            if encoded is not None:
+                encoded[k] = b
+        w.u64(len(b))
+        w.buf += b
+
+
+def _write_dictionary_column(w: ArchiveWriter, c: DictColumnData, chunk_sizes: np.ndarray):
+    w.cls("DictionaryEncodedColumn")
+    ids = np.asarray(c.ids, np.uint32)
+    nulls = np.asarray(c.is_null).astype(bool)
+    _write_chunked(w, "u32", np.uint32, ids, chunk_sizes)
+    gids = _global_ids(np.arange(len(ids)), chunk_sizes)
+    valid = np.nonzero(~nulls)[0]
+    by_value = {}
+    for v in np.unique(ids[valid]).tolist():
+        by_value[v] = gids[valid[ids[valid] == v]]
+    _write_bitmap_map(w, by_value)
+    w.roaring(gids[nulls])
+    if c.lineage_parent is not None:
+        # lineage_index.cpp:17-25: rows under their value's resolved lineage, and under every
+        # ancestor of it for each recombinant-edge mode (one parent per lineage: the modes agree)
+        res = _resolved(c)
+        parent = np.asarray(c.lineage_parent, np.int64)
+        own = {}
+        for v, rows in by_value.items():
+            r_ = int(res[v])
+            own[r_] = np.concatenate([own[r_], rows]) if r_ in own else rows
+        own = {k: np.sort(v) for k, v in own.items()}
+        including = {}
+        for v, rows in own.items():
+            a, seen = v, 0
+            while a >= 0 and seen <= len(parent):
+                including.setdefault(a, []).append(rows)
+                a = int(parent[a])
+                seen += 1
+        including = {k: np.sort(np.concatenate(v)) for k, v in including.items()}
+        w.cls("LineageIndex")
+        w.unordered_items("unordered_map<RecombinantEdgeFollowingMode,unordered_map<u32,Roaring>>", 3)
+        encoded = {}
+        for mode in range(3):
+            w.cls("pair<const RecombinantEdgeFollowingMode,unordered_map<u32,Roaring>>")
+            w.i32(mode)
+            _write_bitmap_map(w, including, encoded)
+        _write_bitmap_map(w, own)
+
+
+def _lineage_yaml(c: DictColumnData) -> str:
+    """A lineage definition file (`preprocessing/lineage_definition_file.cpp` format) of the
+    column's lineages: per lineage its parent and aliases."""
+    res = _resolved(c)
+    parent = np.asarray(c.lineage_parent, np.int64)
+    aliases = {}
+    for v in range(len(c.names)):
+        if res[v] != v:
+            aliases.setdefault(int(res[v]), []).append(c.names[v])
+    lines = []
+    for v, name in enumerate(c.names):
+        if res[v] != v:
+            continue
+        body = []
+        if v in aliases:
+            body += ["  aliases:"] + [f"  - {json_quote(a)}" for a in aliases[v]]
+        if parent[v] >= 0:
+            body += ["  parents:", f"  - {json_quote(c.names[parent[v]])}"]
+        lines.append(f"{json_quote(name)}:" + ("" if body else " {}"))
+        lines += body
+    return "\n".join(lines) + "\n"
+
+
+def json_quote(s: str) -> str:
+    import json
+    return json.dumps(s)
+
+
+def _write_column_metadata(w: ArchiveWriter, kind: int, name: str, col):
+    if kind in (INT32, DATE32):
+        w.cls("ColumnMetadata")
+        w.string(name)
+    elif kind == DICTIONARY_ENCODED:
+        w.cls("DictionaryEncodedColumnMetadata")
+        w.string(name)
+        w.cls("BidirectionalStringMap")
+        w.items("vector<string>", len(col.names))
+        for s in col.names:
+            w.string(s)
+        w.cls("optional<LineageTreeAndIdMap>", 1)
+        w.u8(1 if col.lineage_parent is not None else 0)
+        if col.lineage_parent is not None:
+            res = _resolved(col)
+            parent = np.asarray(col.lineage_parent, np.int64)
+            w.cls("LineageTreeAndIdMap")
+            w.cls("LineageTree")
+            w.items("vector<vector<u32>>", len(col.names))
+            for v in range(len(col.names)):
+                w.array(np.asarray([parent[v]] if parent[v] >= 0 and res[v] == v else [], np.uint32), np.uint32)
+            w.unordered_items("unordered_map<u32,optional<u32>>", 0)
+            alias = [(v, int(res[v])) for v in range(len(col.names)) if res[v] != v]
+            w.unordered_items("unordered_map<u32,u32>", len(alias))
+            for a, t in alias:
+                w.cls("pair<const u32,u32>")
+                w.u32(a)
+                w.u32(t)
+            w.cls("BidirectionalStringMap")
+            w.items("vector<string>", len(col.names))
+            for s in col.names:
+                w.string(s)
+            w.string(_lineage_yaml(col))
+        w.u8(0)  # treat_unknown_lineages_as_null
+    else:
+        tag = "Nucleotide" if kind == NUC_SEQUENCE else "AminoAcid"
+        w.cls(f"SequenceColumnMetadata<{tag}>")
+        w.string(name)
+        ref = np.asarray(col.reference, np.int64)
+        w.items(f"vector<{tag}::Symbol>", len(ref))
+        w.buf += ref.astype("<i4").tobytes()
+
+
+def _spec_columns(spec: TableSpec) -> list:
+    """(name, type, data) of every column of the spec, in the schema's (name, type) order."""
+    cols = []
+    for c in spec.seq_columns:
+        if not isinstance(c, FinalizedSeqColumnData):
+            raise TypeError(f"column {c.name!r}: only finalized sequence columns are written")
+        cols.append((c.name, NUC_SEQUENCE if c.alphabet == ALPHABET_NUC else AA_SEQUENCE, c))
+    cols += [(c.name, DATE32 if c.is_date else INT32, c) for c in spec.int_columns]
+    cols += [(c.name, DICTIONARY_ENCODED, c) for c in spec.dict_columns]
+    names = [n for n, _, _ in cols]
+    if len(set(names)) != len(names):
+        raise ValueError("duplicate column names")
+    return sorted(cols, key=lambda x: (x[0].encode("utf-8"), x[1]))
+
+
+def write_schema(tables: dict) -> bytes:
+    """database_schema.silo for {table name: TableSpec}.  The primary key names the table's first
+    column in schema order (the reference's string primary-key column is not part of the engine's
+    tables)."""
+    w = ArchiveWriter()
+    w.items("map<TableName,shared_ptr<TableSchema>>", len(tables))
+    for tname in sorted(tables, key=lambda s: s.encode("utf-8")):
+        cols = _spec_columns(tables[tname])
+        if not cols:
+            raise ValueError(f"table {tname!r} has no columns")
+        w.cls("pair<const TableName,shared_ptr<TableSchema>>")
+        w.cls("TableName")
+        w.string(tname)
+        w.cls("shared_ptr<TableSchema>")
+        w.cls("TableSchema")
+        w.cls("ColumnIdentifier")
+        w.string(cols[0][0])
+        w.i32(cols[0][1])
+        w.items("vector<ColumnIdentifier>", len(cols))
+        for name, kind, _ in cols:
+            w.cls("ColumnIdentifier")
+            w.string(name)
+            w.i32(kind)
+        for name, kind, col in cols:
+            _write_column_metadata(w, kind, name, col)
+    return bytes(w.buf)
+
+
+def write_table(spec: TableSpec) -> bytes:
+    """<table>.silo for a TableSpec of finalized sequence, int32, date32 and dictionary columns."""
+    chunk_sizes = np.asarray(spec.chunk_sizes, np.uint32)
+    cols = _spec_columns(spec)
+    w = ArchiveWriter()
+    w.cls("ColumnGroup")
+    for t in TABLE_COLUMN_ORDER:
+        for name, kind, col in sorted((c for c in cols if c[1] == t), key=lambda c: c[0].encode("utf-8")):
+            if kind == DICTIONARY_ENCODED:
+                _write_dictionary_column(w, col, chunk_sizes)
+            elif kind == INT32:
+                w.cls("IntColumn<i32>")
+                _write_chunked(w, "i32", np.int32, col.values, chunk_sizes)
+                w.roaring(_global_ids(np.nonzero(np.asarray(col.is_null))[0], chunk_sizes))
+            elif kind == DATE32:
+                w.cls("Date32Column")
+                w.roaring(_global_ids(np.nonzero(np.asarray(col.is_null))[0], chunk_sizes))
+                _write_chunked(w, "i32", np.int32, col.values, chunk_sizes)
+                w.u8(0)  # is_sorted
+            else:
+                _write_sequence_column(w, col, chunk_sizes)
+    w.u32(int(np.sum(chunk_sizes, dtype=np.int64)))
+    w.cls("RowLayout")
+    w.array(chunk_sizes, np.uint32)
+    return bytes(w.buf)
+
+
+def save_database(root: str, tables: dict, timestamp: int) -> str:
+    """Writes {table name: TableSpec} as a saved state under root/<timestamp>/ (the directory layout
+    `Database::saveDatabaseState` writes, `database.cpp:380-415`) and returns that directory."""
+    d = os.path.join(root, str(int(timestamp)))
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, SCHEMA_FILENAME), "wb") as f:
+        f.write(write_schema(tables))
+    for name, spec in tables.items():
+        with open(os.path.join(d, name + ".silo"), "wb") as f:
+            f.write(write_table(spec))
+    with open(os.path.join(d, DATA_VERSION_FILENAME), "w", encoding="utf-8") as f:
+        f.write(f"timestamp: {int(timestamp)}\nserializationVersion: {SERIALIZATION_VERSION}")
+    return d

@@ -67,8 +67,43 @@ class FinalizedSeqColumnData:
     is_null: np.ndarray
     cov_start: np.ndarray
     cov_end: np.ndarray
-    missing_ptr: np.ndarray      # u64 [n_rows + 1]
-    missing_pos: np.ndarray      # u32
+    missing_ptr: np.ndarray | None  # u64 [n_rows + 1]
+    missing_pos: np.ndarray | None  # u32
+    # or the missing positions as runs [start, end) per row, as a saved state holds them
+    missing_run_ptr: np.ndarray | None = None    # u64 [n_rows + 1]
+    missing_run_start: np.ndarray | None = None  # u32
+    missing_run_end: np.ndarray | None = None    # u32
+
+    def missing_runs(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(ptr, start, end) of every row's missing runs, from whichever form the column holds."""
+        if self.missing_run_ptr is not None:
+            return (np.asarray(self.missing_run_ptr, np.uint64), np.asarray(self.missing_run_start, np.uint32),
+                    np.asarray(self.missing_run_end, np.uint32))
+        ptr = np.asarray(self.missing_ptr, np.int64)
+        pos = np.asarray(self.missing_pos, np.int64)[: int(ptr[-1])]
+        row = np.repeat(np.arange(len(ptr) - 1, dtype=np.int64), np.diff(ptr))
+        first = np.ones(len(pos), bool)
+        first[1:] = (row[1:] != row[:-1]) | (pos[1:] != pos[:-1] + 1)
+        starts = np.nonzero(first)[0]
+        ends = np.concatenate([starts[1:], [len(pos)]])
+        run_ptr = np.zeros(len(ptr), np.uint64)
+        np.cumsum(np.bincount(row[starts], minlength=len(ptr) - 1), out=run_ptr[1:])
+        return run_ptr, pos[starts].astype(np.uint32), (pos[ends - 1] + 1).astype(np.uint32)
+
+    def missing_positions(self) -> tuple[np.ndarray, np.ndarray]:
+        """(ptr, positions) of every row's missing positions, from whichever form the column holds."""
+        if self.missing_run_ptr is None:
+            return np.asarray(self.missing_ptr, np.uint64), np.asarray(self.missing_pos, np.uint32)
+        rp = np.asarray(self.missing_run_ptr, np.int64)
+        s = np.asarray(self.missing_run_start, np.int64)
+        e = np.asarray(self.missing_run_end, np.int64)
+        length = e - s
+        row = np.repeat(np.arange(len(rp) - 1, dtype=np.int64), np.diff(rp))
+        ptr = np.zeros(len(rp), np.uint64)
+        np.cumsum(np.bincount(row, weights=length, minlength=len(rp) - 1).astype(np.int64), out=ptr[1:])
+        total = int(length.sum())
+        base = np.repeat(s - np.concatenate([[0], np.cumsum(length)[:-1]]), length)
+        return ptr, (base + np.arange(total, dtype=np.int64)).astype(np.uint32)
 
 
 @dataclass

@@ -48,3 +48,41 @@ def test_finalized_form_reads_back_into_the_diff_form(name, spec):
                 assert (f.local_reference != f.reference).any()
     finally:
         ot.close()
+
+
+def decoded_index(f) -> dict:
+    from silo_amd import silo_archive as sa
+
+    out = {}
+    for k in range(len(f.key_position)):
+        o = int(f.payload_offset[k])
+        rows = sa.container_values(int(f.typecode[k]), int(f.cardinality[k]), bytes(f.payload[o:o + 8194]))
+        assert len(rows) == int(f.cardinality[k])
+        out[(int(f.key_position[k]), int(f.key_chunk[k]), int(f.key_symbol[k]))] = rows.tolist()
+    return out
+
+
+@pytest.mark.parametrize("name,spec", list(tables()), ids=lambda x: x if isinstance(x, str) else "")
+def test_host_finalize_equals_the_oracle_export(name, spec):
+    """silo_amd.finalize (numpy, used to write saved states) gives the oracle's finalized column:
+    the same local reference, containers with the same rows (and typecodes, both run-optimized),
+    covered ranges, missing positions and nulls."""
+    from oracle.oracle import OracleTable
+    from silo_amd.finalize import finalize_column
+
+    ot = OracleTable(spec)
+    try:
+        for c in spec.seq_columns:
+            want = ot.export_finalized(c.name)
+            got = finalize_column(c, spec.chunk_sizes)
+            assert got.local_reference.tolist() == want.local_reference.tolist()
+            assert decoded_index(got) == decoded_index(want)
+            assert got.typecode.tolist() == want.typecode.tolist()
+            for a in ("is_null", "cov_start", "cov_end"):
+                assert np.asarray(getattr(got, a)).tolist() == np.asarray(getattr(want, a)).tolist(), a
+            for x, y in zip(got.missing_positions(), want.missing_positions()):
+                assert x.tolist() == y.tolist()
+            for x, y in zip(got.missing_runs(), want.missing_runs()):
+                assert x.tolist() == y.tolist()
+    finally:
+        ot.close()

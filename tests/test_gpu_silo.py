@@ -129,3 +129,38 @@ def test_lineage_aliases_resolve_like_the_lineage_index(engine):
             assert et.filter(e).tolist() == rows, e
     finally:
         et.close()
+
+
+def test_saved_generated_table_answers_like_its_diff_form(engine, tmp_path):
+    """A generated table written as a saved state (host finalize + writer) and loaded answers every
+    filter, Mutations and Aggregated query as the same table uploaded in diff form, and as the
+    oracle."""
+    from helpers import ExprGen, small_spec
+    from oracle.oracle import OracleTable
+    from silo_amd import silo_archive as sa, table
+    from silo_amd.finalize import finalize_column
+
+    ds, ts = small_spec(seed=23, n_rows=7000, L=700, chunk=2048)
+    spec = table.TableSpec(chunk_sizes=ts.chunk_sizes, int_columns=ts.int_columns, dict_columns=ts.dict_columns)
+    spec.seq_columns = [finalize_column(c, ts.chunk_sizes) for c in ts.seq_columns]
+    loaded_spec = sa.load_database(sa.save_database(str(tmp_path), {"default": spec}, 7)).table_spec()
+    gen = ExprGen(ds, seed=9)
+    filters = ["(true)", "(has_mut main 17)"] + [gen.expr(3) for _ in range(30)]
+    ot = OracleTable(ts)
+    a = engine.table(ts)
+    b = engine.table(loaded_spec)
+    try:
+        assert np.array_equal(b.local_reference("main"), ot.local_reference("main"))
+        for e in filters:
+            rows = ot.filter(e)
+            assert np.array_equal(b.filter(e), rows), e
+        for e in filters[:10]:
+            assert b.mutations_query(e, "main", 0.05) == ot.mutations(e, "main", 0.05), e
+            assert a.mutations_query(e, "main", 0.05) == ot.mutations(e, "main", 0.05), e
+            assert b.aggregate(e, ["country", "main@3"]) == a.aggregate(e, ["country", "main@3"]), e
+        queries = [{"filter": e, "action": "mutations", "column": "main", "min_proportion": 0.02} for e in filters]
+        assert b.batch(queries) == a.batch(queries)
+    finally:
+        a.close()
+        b.close()
+        ot.close()
