@@ -214,6 +214,9 @@ def scan_roofline(eng, et, queries, prof, n_chunks, log):
            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
            "algorithmic_bytes_per_launch": int(bytes_per_launch), "dense_queries_per_launch": round(dense, 2),
            "queries_per_pass": per_pass, "launches": launches, "avg_launch_ms": round(avg_ms, 5)}
+    stream = eng.read_bandwidth(bytes_per_launch, reps=10)
+    out["stream_read_GBps_same_bytes"] = round(stream, 1)
+    out["frac_of_stream_read_same_bytes"] = round(achieved / stream, 4)
     log(f"batched dense scan roofline: {out}")
     return out
 
@@ -603,7 +606,10 @@ def main():
     elif ds.n_rows >= 1 << 20:
         probes = autotune.probe_filters(ds)
         tuned = autotune.autotune(eng, et, probes, log=log)
-        tuned.update(autotune.autotune_batch(eng, et, probes, log=log, cards=tuned["probe_cards"]))
+        # the batched knobs on batches of the workload's own queries (the queries a server would see)
+        mut = [q for q in queries if q["action"] == "mutations"]
+        tuned.update(autotune.autotune_batch(eng, et, probes, log=log, cards=tuned["probe_cards"],
+                                             workload=mut if len(mut) == len(queries) else None))
     log(f"autotune {time.time() - t0:.1f}s")
     # the plain path again with the autotuned single-query knobs (the sparse method below the
     # single-query switch): the batched path is also compared with this
@@ -742,10 +748,15 @@ def main():
         n_dense = prof["mutations_launches"]
         per_launch = traffic / max(n_mut, 1)
         achieved = per_launch * n_dense / (prof["mutations_ms"] * 1e-3) / 1e9
+        # the device-side reference point: a plain streaming read of the same bytes, and of 4 GiB
+        stream_same = eng.read_bandwidth(per_launch)
+        stream_large = eng.read_bandwidth(4 << 30, reps=5)
         roofline = {"bound": "hbm", "kernel": "mutationsCountKernel", "achieved": round(achieved, 2),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc,
                     "algorithmic_bytes_per_launch_mean": int(per_launch), "launches": n_dense,
-                    "avg_launch_ms": round(prof["mutations_ms"] / n_dense, 5)}
+                    "avg_launch_ms": round(prof["mutations_ms"] / n_dense, 5),
+                    "stream_read_GBps_same_bytes": round(stream_same, 1), "stream_read_GBps_4GiB": round(stream_large, 1),
+                    "frac_of_stream_read_same_bytes": round(achieved / stream_same, 4)}
 
     line = {
         "metric": METRIC,

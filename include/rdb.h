@@ -269,6 +269,10 @@ int rdb_aggregate(rdb_table* table, const rdb_bitmap* filter, const char* dims, 
 /* No reference counterpart (the reference's evobench probes time CPU scopes, src/evobench/
  * evobench.hpp:144-230).  Enables (and resets) HIP-event timing of the Mutations counting kernel
  * and the filter kernel on the context's stream; timing adds a wait after each timed launch. */
+/* No reference counterpart: the bandwidth a plain streaming read of `bytes` (a multiple of 16)
+ * reaches on this device, averaged over `reps` back-to-back launches timed with HIP events (the
+ * device-side reference point for the counting kernels' roofline fractions). */
+int rdb_measure_read_bandwidth(rdb_ctx* ctx, uint64_t bytes, int reps, double* gb_per_s);
 int rdb_profile(rdb_ctx* ctx, int enable);
 int rdb_profile_read(rdb_ctx* ctx, double* mutations_ms, uint64_t* mutations_launches, double* filter_ms,
                      uint64_t* filter_launches);

@@ -197,6 +197,37 @@ int rdb_device_memory(rdb_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes)
    });
 }
 
+int rdb_measure_read_bandwidth(rdb_ctx* ctx, uint64_t bytes, int reps, double* gb_per_s) {
+   return guarded(ctx ? &ctx->c : nullptr, [&] {
+      if (!ctx || !gb_per_s) fail(RDB_EINVAL, "null argument");
+      if (bytes < 16 || bytes % 16 || reps < 1) fail(RDB_EINVAL, "bytes must be a positive multiple of 16, reps >= 1");
+      std::lock_guard<std::recursive_mutex> lock(ctx->c.mu);
+      hipCheck(hipSetDevice(ctx->c.device), "hipSetDevice");
+      // enough work-groups to fill the device, none without work
+      const uint64_t words = bytes / 16;
+      const uint32_t blocks = static_cast<uint32_t>(std::max<uint64_t>(
+         1, std::min<uint64_t>(static_cast<uint64_t>(ctx->c.cu_count) * 8, (words + 1023) / 1024)));
+      DeviceBuffer src(bytes), out(static_cast<size_t>(blocks) * 256 * 4);
+      hipStream_t s = ctx->c.stream;
+      hipCheck(hipMemsetAsync(src.as<void>(), 0, bytes, s), "hipMemsetAsync");
+      hipEvent_t a, b;
+      hipCheck(hipEventCreate(&a), "hipEventCreate");
+      hipCheck(hipEventCreate(&b), "hipEventCreate");
+      hipCheck(launchStreamRead(src.as<void>(), bytes, out.as<uint32_t>(), blocks, s), "stream read");
+      hipCheck(hipEventRecord(a, s), "hipEventRecord");
+      for (int r = 0; r < reps; ++r) {
+         hipCheck(launchStreamRead(src.as<void>(), bytes, out.as<uint32_t>(), blocks, s), "stream read");
+      }
+      hipCheck(hipEventRecord(b, s), "hipEventRecord");
+      hipCheck(hipEventSynchronize(b), "hipEventSynchronize");
+      float ms = 0;
+      hipCheck(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+      *gb_per_s = static_cast<double>(bytes) * reps / (ms * 1e-3) / 1e9;
+   });
+}
+
 int rdb_set_tuning(rdb_ctx* ctx, const char* key, int64_t value) {
    return guarded(ctx ? &ctx->c : nullptr, [&] {
       if (!ctx || !key) fail(RDB_EINVAL, "null argument");

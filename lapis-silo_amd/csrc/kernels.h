@@ -17,6 +17,9 @@ struct GroupDims {
 };
 
 hipError_t launchFill(uint32_t* p, uint32_t value, size_t n, hipStream_t stream);
+// Streaming read of `bytes` (a multiple of 16) with `blocks` work-groups of 256; `out` holds
+// blocks * 256 u32 (written only for a data value zero-filled memory never gives).
+hipError_t launchStreamRead(const void* src, size_t bytes, uint32_t* out, uint32_t blocks, hipStream_t stream);
 
 // The allocation-free Mutations entry point's device scratch: a file-scope device array of this
 // many bytes per GPU; noallocScratch resolves its device address on `device` (once, then cached).

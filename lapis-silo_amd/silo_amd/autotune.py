@@ -103,7 +103,7 @@ def autotune(engine, table, probes: list, column: str = "main", reps: int = 5, l
 
 
 def autotune_batch(engine, table, probes: list, column: str = "main", size: int = 64, reps: int = 5, log=None,
-                   cards: list | None = None, margin: float = 0.02) -> dict:
+                   cards: list | None = None, margin: float = 0.02, workload: list | None = None) -> dict:
     """Chooses the batched dense kernel ("batch_shared_scan": one pass over the index per query, one
     per four queries or one per eight, the last with work-groups of 512 or 1024 threads,
     "batch_octet_threads") by timing a batch of `size` Mutations queries over the probe filters each
@@ -112,10 +112,23 @@ def autotune_batch(engine, table, probes: list, column: str = "main", size: int 
     that concurrency, the switch and the job size.  Every knob value tried returns the per-query
     kernel's results (checked on the last batch of each timing, RuntimeError otherwise); a
     candidate replaces the current choice only when it is faster by more than `margin` (the
-    run-to-run spread of a pipelined timing).  The knobs are applied."""
+    run-to-run spread of a pipelined timing).  With `workload` (the Mutations queries the table will
+    serve, as batches of `size`) the batches timed are those queries and the switch candidates are
+    their filters' cardinalities (each candidate then splits the workload into dense and sparse
+    queries differently); otherwise the batches cycle through the probe filters.  The knobs are
+    applied."""
     say = log or (lambda *_: None)
-    queries = [{"filter": probes[i % len(probes)], "action": "mutations", "column": column, "min_proportion": 0.05}
-               for i in range(size)]
+    if workload:
+        queries = [dict(q) for q in workload[:size]]
+        size = len(queries)
+        served = []
+        for q in queries:
+            with table.bitmap(q["filter"]) as b:
+                served.append(b.cardinality)
+        cards = sorted(set(int(c) for c in served if c > 0) | set(int(c) for c in (cards or [])))
+    else:
+        queries = [{"filter": probes[i % len(probes)], "action": "mutations", "column": column,
+                    "min_proportion": 0.05} for i in range(size)]
     saved = engine.get_tuning("batch_max_queries")
     engine.set_tuning("batch_max_queries", size)
     times = {}

@@ -40,6 +40,7 @@ SIGNATURES = [
     ("rdb_table_column_digest", _int, [_vp, _cp, _vp]),
     ("rdb_table_add_int_column", _int, [_vp, _cp, _int, _vp, _vp]),
     ("rdb_table_add_dict_column", _int, [_vp, _cp, _vp, _vp, _u32, _cp, _vp, _vp]),
+    ("rdb_measure_read_bandwidth", _int, [_vp, _u64, _int, ctypes.POINTER(ctypes.c_double)]),
     ("rdb_table_add_lineage_column", _int, [_vp, _cp, _vp, _vp, _u32, _cp, _vp, _vp, _vp]),
     ("rdb_table_add_bitmap", _int, [_vp, _vp, _u64, _vp]),
     ("rdb_table_finalize", _int, [_vp]),
@@ -216,6 +217,12 @@ class Engine:
         return {"mutations_ms": mm.value, "mutations_launches": mn.value, "filter_ms": fm.value,
                 "filter_launches": fn.value, "scan_ms": sm.value, "scan_launches": sn.value,
                 "scan_dense_queries": dq.value}
+
+    def read_bandwidth(self, n_bytes: int, reps: int = 20) -> float:
+        """GB/s of a plain streaming read of n_bytes (rounded down to 16) on this GPU."""
+        g = ctypes.c_double()
+        _check(lib().rdb_measure_read_bandwidth(self._h, int(n_bytes) // 16 * 16, reps, ctypes.byref(g)), self._h)
+        return g.value
 
     def device_memory(self) -> tuple:
         """(free, total) bytes of the GPU."""
