@@ -307,6 +307,18 @@ void apply(std::vector<Node>& nodes, const float* estimates) {
    }
 }
 
+// A leaf whose statistic is its exact row count: Empty, Full, an index scan (store cardinality or
+// the stored symbols' rows), row ranges, a selection of one predicate (its table count); a
+// selection of several predicates multiplies their selectivities and is estimated like an
+// operator.
+bool exactLeaf(const Node& n) {
+   if (!n.leaf_hint) return false;
+   if (n.op->type() == query::SELECTION) {
+      return static_cast<const query::Selection&>(*n.op).predicates.size() <= 1;
+   }
+   return true;
+}
+
 std::vector<std::vector<float>> estimate(const Table& t, std::vector<std::vector<Node>>& trees, Mode mode, Mlp* mlp,
                                          unsigned threads) {
    std::vector<std::vector<float>> est(trees.size());
@@ -345,10 +357,17 @@ std::vector<std::vector<float>> estimate(const Table& t, std::vector<std::vector
       for (size_t q = 0; q < trees.size(); ++q) {
          for (size_t k = 0; k < trees[q].size(); ++k) {
             if (height[q][k] != h) continue;
-            x.insert(x.end(), trees[q][k].features, trees[q][k].features + kFeatures);
+            const Node& n = trees[q][k];
+            if (exactLeaf(n)) {
+               // the host statistic is the leaf's row count: no estimate can improve on it
+               est[q][k] = n.features[F_LEAF_ROWS];
+               continue;
+            }
+            x.insert(x.end(), n.features, n.features + kFeatures);
             who.emplace_back(static_cast<uint32_t>(q), static_cast<uint32_t>(k));
          }
       }
+      if (who.empty()) continue;
       y.assign(who.size(), 0.0f);
       mlp->predict(x.data(), who.size(), y.data());
       for (size_t i = 0; i < who.size(); ++i) est[who[i].first][who[i].second] = std::clamp(y[i], 0.0f, 1.0f);

@@ -164,3 +164,35 @@ def test_saved_generated_table_answers_like_its_diff_form(engine, tmp_path):
         a.close()
         b.close()
         ot.close()
+
+
+def test_saved_amino_acid_and_bitset_tables(engine, tmp_path):
+    """Amino-acid columns and 65,536-row chunks with bitset and run containers survive a saved state:
+    the loaded tables answer Mutations on every column like the oracle."""
+    from helpers import BITSET_FILTERS, bitset_table
+    from oracle.oracle import OracleTable
+    from silo_amd import datagen, silo_archive as sa, table
+    from silo_amd.finalize import finalize_column
+
+    ds = datagen.generate(datagen.GenSpec(n_rows=6000, genome_length=900, seed=78, lineage_model=True, n_nodes=40,
+                                          n_mean=40, null_seq_frac=0.02), keep_handle=True)
+    aa = datagen.translate(ds, datagen.gene_layout(ds.genome_length, [("g1", 120), ("g2", 150)], seed=5))
+    cases = [(table.spec_from_dataset(ds, chunk=2048, aa_columns=aa), ["(true)", "(has_mut g1 30)", "(sym_eq g2 10 X)",
+                                                                       "(has_mut main 200)"], ["main", "g1", "g2"]),
+             (bitset_table()[0], BITSET_FILTERS, ["main"])]
+    for k, (ts, filters, columns) in enumerate(cases):
+        spec = table.TableSpec(chunk_sizes=ts.chunk_sizes, int_columns=ts.int_columns, dict_columns=ts.dict_columns)
+        spec.seq_columns = [finalize_column(c, ts.chunk_sizes) for c in ts.seq_columns]
+        loaded = sa.load_database(sa.save_database(str(tmp_path / str(k)), {"default": spec}, 11)).table_spec()
+        ot = OracleTable(ts)
+        et = engine.table(loaded)
+        try:
+            for col in columns:
+                assert np.array_equal(et.local_reference(col), ot.local_reference(col)), col
+            for e in filters:
+                assert np.array_equal(et.filter(e), ot.filter(e)), e
+                for col in columns:
+                    assert et.mutations_query(e, col, 0.02) == ot.mutations(e, col, 0.02), (e, col)
+        finally:
+            et.close()
+            ot.close()

@@ -276,21 +276,36 @@ def test_saved_state_round_trip(db, tmp_path):
         assert flat(li["including_sublineages"][mode]) == flat(li2["including_sublineages"][mode])
 
 
-def test_generated_table_round_trip(tmp_path):
-    """A generated table (partial chunks, nulls, adapted local references, missing runs) finalized on
-    the host, saved and loaded gives back the same finalized columns and metadata."""
-    from helpers import small_spec
+def generated_tables():
+    from helpers import bitset_table, small_spec
+    from silo_amd import datagen, table
+
+    yield "lineage", small_spec(seed=21, n_rows=5000, L=600, chunk=1500)[1]
+    yield "bitset", bitset_table()[0]
+    ds = datagen.generate(datagen.GenSpec(n_rows=6000, genome_length=900, seed=78, lineage_model=True, n_nodes=40,
+                                          n_mean=40, null_seq_frac=0.02), keep_handle=True)
+    aa = datagen.translate(ds, datagen.gene_layout(ds.genome_length, [("g1", 120), ("g2", 150)], seed=5))
+    yield "amino_acid", table.spec_from_dataset(ds, chunk=2048, aa_columns=aa)
+
+
+@pytest.mark.parametrize("name,ts", list(generated_tables()), ids=lambda x: x if isinstance(x, str) else "")
+def test_generated_table_round_trip(tmp_path, name, ts):
+    """Generated tables (partial chunks, nulls, adapted local references, missing runs; 65,536-row
+    chunks with bitset and run containers; amino-acid columns) finalized on the host, saved and
+    loaded give back the same finalized columns and metadata."""
     from silo_amd import silo_archive as sa, table
     from silo_amd.finalize import finalize_column
 
-    ds, ts = small_spec(seed=21, n_rows=5000, L=600, chunk=1500)
     spec = table.TableSpec(chunk_sizes=ts.chunk_sizes, int_columns=ts.int_columns, dict_columns=ts.dict_columns)
     spec.seq_columns = [finalize_column(c, ts.chunk_sizes) for c in ts.seq_columns]
-    assert (spec.seq_columns[0].local_reference != spec.seq_columns[0].reference).any()
+    if name == "lineage":
+        assert (spec.seq_columns[0].local_reference != spec.seq_columns[0].reference).any()
+    if name == "bitset":
+        assert {1, 2, 3} <= set(np.unique(spec.seq_columns[0].typecode).tolist())
     d = sa.save_database(str(tmp_path), {"default": spec}, 99)
     back = sa.load_database(d).table_spec()
     assert back.chunk_sizes.tolist() == spec.chunk_sizes.tolist()
-    for a, b in zip(spec.seq_columns, back.seq_columns):
+    for a, b in zip(sorted(spec.seq_columns, key=lambda c: c.name), sorted(back.seq_columns, key=lambda c: c.name)):
         assert same(a, b)
     for a, b in zip(sorted(spec.int_columns, key=lambda c: c.name), back.int_columns):
         assert a.name == b.name and np.array_equal(a.values, b.values) and np.array_equal(a.is_null, b.is_null)
