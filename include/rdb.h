@@ -388,10 +388,11 @@ int rdb_set_planner(rdb_ctx* ctx, int mode, rdb_mlp* mlp);
  *   "mutation_groups"  chunk groups (partial sums) of the dense Mutations kernel, 0 = automatic
  *                      (at most 32);
  *   "mutation_packed"  1: the dense Mutations kernel keeps two 16-bit slot counters per LDS word
- *                      (default; used when a group holds at most 15 chunks), 0: one 32-bit;
- *   "mutation_unroll"  16-byte loads in flight per lane in the dense Mutations kernel, 8 or 16;
+ *                      (used when a group holds at most 15 chunks), 0: one 32-bit (default);
+ *   "mutation_unroll"  16-byte loads per lane and round in the dense Mutations kernel (two rounds
+ *                      in flight): 2, 4 (default), 8 or 16;
  *   "mutation_coop"    1: the four wavefronts of a dense-kernel work-group count each chunk's tile
- *                      together, 0: each wavefront takes its own chunks (default);
+ *                      together (default), 0: each wavefront takes its own chunks;
  *   "batch_max_queries" queries per lane job of the batched path (1..4096, default 256);
  *   "batch_lanes"      lanes (HIP streams with their own buffers) per table (1..16, default 2);
  *   "batch_graphs"     1: replay each lane's launch sequence from a captured HIP graph (default);

@@ -124,9 +124,9 @@ struct Context {
    // tuning knobs (rdb_set_tuning)
    long long sparse_max_rows = -1;     // filters with at most this many rows take the sparse path
    uint32_t mutation_groups = 0;       // chunk groups of the dense Mutations kernel (0 = automatic)
-   uint32_t mutation_unroll = 8;       // 16-byte loads in flight per lane in the dense Mutations kernel
-   uint32_t mutation_packed = 1;       // dense Mutations kernel with 16-bit packed LDS counters
-   uint32_t mutation_coop = 0;         // dense Mutations kernel: a work-group's wavefronts share each tile
+   uint32_t mutation_unroll = 4;       // 16-byte loads in flight per lane in the dense Mutations kernel
+   uint32_t mutation_packed = 0;       // dense Mutations kernel with 16-bit packed LDS counters
+   uint32_t mutation_coop = 1;         // dense Mutations kernel: a work-group's wavefronts share each tile
    uint32_t batch_max_queries = 256;   // queries per lane job of the batched path
    uint32_t batch_lanes = 2;           // lanes (streams with their own buffers) per table
    uint32_t batch_graphs = 1;          // capture each lane's launch sequence into a HIP graph

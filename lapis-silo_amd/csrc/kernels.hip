@@ -811,9 +811,10 @@ __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b,
                // lookups four groups (32 rows) at a time, to bound the registers they hold
    #pragma unroll
                for (uint32_t u0 = 0; u0 < kRound; u0 += 4) {
-               uint32_t word[32];
+               constexpr uint32_t kQuad = kRound < 4 ? kRound : 4;
+               uint32_t word[8 * kQuad];
    #pragma unroll
-               for (uint32_t u = u0; u < u0 + 4; ++u) {
+               for (uint32_t u = u0; u < u0 + kQuad; ++u) {
    #pragma unroll
                   for (uint32_t q = 0; q < 8; ++q) {
                      const uint32_t r = (q & 1u) ? rows[u][q >> 1] >> 16 : rows[u][q >> 1] & 0xFFFFu;
@@ -821,7 +822,7 @@ __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b,
                   }
                }
    #pragma unroll
-               for (uint32_t u = u0; u < u0 + 4; ++u) {
+               for (uint32_t u = u0; u < u0 + kQuad; ++u) {
                   const uint32_t h = base + u < ng ? hdr[u] : 0xFFFFu;
                   const uint32_t slot = h & 0xFFFu;
                   const uint32_t m = h == 0xFFFFu ? 0u : (h >> 12) + 1;
@@ -971,7 +972,10 @@ __device__ __forceinline__ void countBlock(const SeqColumnView& col, uint32_t b,
 
 
 template <uint32_t kUnroll, bool kPacked, bool kCoop>
-__global__ __launch_bounds__(kMutThreads) __attribute__((amdgpu_waves_per_eu(kPacked || kCoop ? 4 : 3))) void mutationsCountKernel(const SeqColumnView* __restrict__ colp,
+// Occupancy target: with the work-group sharing each tile and two 16-byte loads in flight per lane,
+// the kernel fits 73 VGPRs and six wavefronts per SIMD (the 24 KiB of LDS a work-group takes allow
+// six per CU); larger unrolls keep four, where they do not spill.
+__global__ __launch_bounds__(kMutThreads) __attribute__((amdgpu_waves_per_eu(kCoop && kUnroll <= 4 ? 6 : (kPacked || kCoop ? 4 : 3)))) void mutationsCountKernel(const SeqColumnView* __restrict__ colp,
                                                                     uint32_t n_chunks, uint32_t n_groups,
                                                                     const uint64_t* __restrict__ filter,
                                                                     const uint32_t* __restrict__ filter_chunk_cards,
@@ -2793,6 +2797,18 @@ hipError_t launchMutations(const MutationsLaunch& m, hipStream_t stream) {
       } else {
          RDB_LAUNCH_K1_C(16, false);
       }
+   } else if (m.unroll == 4) {
+      if (packed) {
+         RDB_LAUNCH_K1_C(4, true);
+      } else {
+         RDB_LAUNCH_K1_C(4, false);
+      }
+   } else if (m.unroll == 2) {
+      if (packed) {
+         RDB_LAUNCH_K1_C(2, true);
+      } else {
+         RDB_LAUNCH_K1_C(2, false);
+      }
    } else {
       if (packed) {
          RDB_LAUNCH_K1_C(8, true);
@@ -2880,6 +2896,8 @@ hipError_t launchMutationsBatch(const MutationsBatch& m, hipStream_t stream) {
       const dim3 grid(n_xblocks * 2 * ((gq + 3) / 4));
       if (m.unroll == 16) {
          hipLaunchKernelGGL(mutationsCountSharedKernel<4>, grid, dim3(kSharedThreads), 0, stream, m);
+      } else if (m.unroll == 2) {
+         hipLaunchKernelGGL(mutationsCountSharedKernel<1>, grid, dim3(kSharedThreads), 0, stream, m);
       } else {
          hipLaunchKernelGGL(mutationsCountSharedKernel<2>, grid, dim3(kSharedThreads), 0, stream, m);
       }

@@ -5,6 +5,7 @@ It measures, on the GPU and on the table at hand:
 * the dense Mutations kernel's shape: wavefronts sharing each tile or not ("mutation_coop"), loads
   in flight per lane ("mutation_unroll"), chunk groups ("mutation_groups") and 16-bit packed
   counters ("mutation_packed"), timed with HIP events on a filter that touches every chunk;
+  fewer loads in flight per lane (4) can win: they leave registers for more wavefronts;
 * the dense/sparse switch ("sparse_max_rows"): the dense method costs about the same for any filter
   that touches every chunk, the sparse one grows with the filter's rows; both are timed on filters
   of several cardinalities, the sparse time is fitted as a + b * rows, and the switch is put where
@@ -54,8 +55,8 @@ def autotune(engine, table, probes: list, column: str = "main", reps: int = 5, l
     shapes = {}
     for coop in (1, 0):
         for packed in (1, 0):
-            for unroll in (8, 16):
-                for groups in (0, 12, 16, 24, 32):
+            for unroll in (2, 4, 8):
+                for groups in (0, 12, 16, 18, 21, 24, 32):
                     engine.set_tuning("mutation_coop", coop)
                     engine.set_tuning("mutation_packed", packed)
                     engine.set_tuning("mutation_unroll", unroll)
