@@ -110,7 +110,11 @@ class ArchiveReader:
         return bytes(self.take(self.count()))
 
     def string(self) -> str:
-        return self.string_bytes().decode("utf-8")
+        b = self.string_bytes()
+        try:
+            return b.decode("utf-8")
+        except UnicodeDecodeError:
+            self.fail("a string that is not UTF-8")
 
     def _header(self):
         magic = self.string_bytes()
@@ -203,16 +207,26 @@ def _expand_runs(runs: np.ndarray) -> np.ndarray:
 def container_values(typecode: int, cardinality: int, payload: bytes) -> np.ndarray:
     """The u16 values of one container in its `container_write` form (1 bitset, 2 array, 3 run)."""
     if typecode == 1:
+        if len(payload) < 8192:
+            raise SiloFormatError("truncated bitset container")
         words = np.frombuffer(payload[:8192], dtype="<u8")
         bits = np.unpackbits(words.view(np.uint8), bitorder="little")
         return np.nonzero(bits)[0].astype(np.uint16)
     if typecode == 2:
+        if len(payload) < 2 * cardinality:
+            raise SiloFormatError("truncated array container")
         return np.frombuffer(payload[:2 * cardinality], dtype="<u2").copy()
     if typecode == 3:
+        if len(payload) < 2:
+            raise SiloFormatError("truncated run container")
         n_runs = struct.unpack_from("<H", payload)[0]
+        if len(payload) < 2 + 4 * n_runs:
+            raise SiloFormatError("truncated run container")
         runs = np.frombuffer(payload[2:2 + 4 * n_runs], dtype="<u2").reshape(-1, 2)
         if n_runs == 0:
             return np.zeros(0, np.uint16)
+        if np.any(runs[:, 0].astype(np.int64) + runs[:, 1] > 0xFFFF):
+            raise SiloFormatError("run past the end of its block")
         return _expand_runs(runs).astype(np.uint16)
     raise SiloFormatError(f"unknown container typecode {typecode}")
 
@@ -371,29 +385,46 @@ def _decode_block(data: np.ndarray, starts: np.ndarray, lens: np.ndarray, name: 
 
 
 def roaring_values(buf: bytes, name: str = "roaring") -> np.ndarray:
-    """Decodes CRoaring's portable format (the format `Roaring::write` produces) to sorted u32."""
-    if len(buf) < 4:
-        raise SiloFormatError(f"{name}: roaring bitmap too short")
+    """Decodes CRoaring's portable format (the format `Roaring::write` produces) to sorted u32.
+    Malformed input (a short buffer, keys out of order, a container whose values do not match its
+    cardinality) raises SiloFormatError."""
+    def bad(what: str):
+        raise SiloFormatError(f"{name}: {what}")
+
+    n_buf = len(buf)
+    if n_buf < 4:
+        bad("roaring bitmap too short")
     cookie = struct.unpack_from("<I", buf)[0]
     # one array container, the common case of small bitmaps: cookie, count 1, key, cardinality - 1,
     # offset, values
-    if cookie == _SERIAL_COOKIE_NO_RUN and len(buf) >= 16 and struct.unpack_from("<I", buf, 4)[0] == 1:
+    if cookie == _SERIAL_COOKIE_NO_RUN and n_buf >= 16 and struct.unpack_from("<I", buf, 4)[0] == 1:
         key, c1 = struct.unpack_from("<HH", buf, 8)
-        if c1 < 4096 and len(buf) == 16 + 2 * (c1 + 1):
-            return np.frombuffer(buf, "<u2", c1 + 1, 16).astype(np.uint32) | np.uint32(key << 16)
+        if c1 < 4096 and n_buf == 16 + 2 * (c1 + 1):
+            vals = np.frombuffer(buf, "<u2", c1 + 1, 16)
+            if c1 and np.any(np.diff(vals.astype(np.int32)) <= 0):
+                bad("array container values out of order")
+            return vals.astype(np.uint32) | np.uint32(key << 16)
     pos = 4
     run_flags = None
     if cookie & 0xFFFF == _SERIAL_COOKIE:
         size = (cookie >> 16) + 1
         nbytes = (size + 7) // 8
+        if pos + nbytes > n_buf:
+            bad("truncated roaring bitmap")
         run_flags = np.unpackbits(np.frombuffer(buf[pos:pos + nbytes], np.uint8), bitorder="little")[:size]
         pos += nbytes
     elif cookie == _SERIAL_COOKIE_NO_RUN:
+        if n_buf < 8:
+            bad("truncated roaring bitmap")
         size = struct.unpack_from("<I", buf, pos)[0]
         pos += 4
     else:
-        raise SiloFormatError(f"{name}: bad roaring cookie {cookie:#x}")
+        bad(f"bad roaring cookie {cookie:#x}")
+    if size > 65536 or pos + 4 * size > n_buf:
+        bad("truncated roaring bitmap")
     hdr = np.frombuffer(buf[pos:pos + 4 * size], dtype="<u2").reshape(-1, 2) if size else np.zeros((0, 2), np.uint16)
+    if size > 1 and np.any(np.diff(hdr[:, 0].astype(np.int32)) <= 0):
+        bad("container keys out of order")
     pos += 4 * size
     if run_flags is None or size >= _NO_OFFSET_THRESHOLD:
         pos += 4 * size  # container offsets
@@ -402,20 +433,30 @@ def roaring_values(buf: bytes, name: str = "roaring") -> np.ndarray:
         key = int(hdr[i, 0])
         card = int(hdr[i, 1]) + 1
         if run_flags is not None and run_flags[i]:
+            if pos + 2 > n_buf:
+                bad("truncated run container")
             n_runs = struct.unpack_from("<H", buf, pos)[0]
+            if pos + 2 + 4 * n_runs > n_buf:
+                bad("truncated run container")
             runs = np.frombuffer(buf[pos + 2:pos + 2 + 4 * n_runs], dtype="<u2").reshape(-1, 2)
             pos += 2 + 4 * n_runs
+            if np.any(runs[:, 0].astype(np.int64) + runs[:, 1] > 0xFFFF):
+                bad("run past the end of its block")
             vals = _expand_runs(runs)
         elif card > 4096:
+            if pos + 8192 > n_buf:
+                bad("truncated bitset container")
             words = np.frombuffer(buf[pos:pos + 8192], dtype="<u8")
             pos += 8192
             vals = np.nonzero(np.unpackbits(words.view(np.uint8), bitorder="little"))[0].astype(np.uint32)
         else:
+            if pos + 2 * card > n_buf:
+                bad("truncated array container")
             vals = np.frombuffer(buf[pos:pos + 2 * card], dtype="<u2").astype(np.uint32)
             pos += 2 * card
+        if len(vals) != card or (len(vals) > 1 and np.any(np.diff(vals.astype(np.int64)) <= 0)):
+            bad("container values do not match its cardinality")
         out.append((np.uint32(key) << np.uint32(16)) | vals.astype(np.uint32))
-    if pos > len(buf):
-        raise SiloFormatError(f"{name}: truncated roaring bitmap")
     return np.concatenate(out).astype(np.uint32) if out else np.zeros(0, np.uint32)
 
 

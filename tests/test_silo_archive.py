@@ -315,3 +315,33 @@ def test_generated_table_round_trip(tmp_path, name, ts):
         assert np.array_equal(np.asarray(a.ids)[valid], np.asarray(b.ids)[valid])
         if a.lineage_parent is not None:
             assert np.array_equal(a.lineage_parent, b.lineage_parent)
+
+
+def test_corrupted_archives_fail_cleanly(tmp_path):
+    """Random byte changes and truncations of the saved table either still read (a changed value) or
+    raise SiloFormatError: no other exception, no hang."""
+    import random
+
+    from silo_amd import silo_archive as sa
+
+    d = ss.saved_state(tmp_path)
+    with open(os.path.join(d, "default.silo"), "rb") as f:
+        raw = f.read()
+    with open(os.path.join(d, "database_schema.silo"), "rb") as f:
+        schema = sa.read_schema(f.read())["default"]
+    rng = random.Random(7)
+    outcomes = {"read": 0, "refused": 0}
+    for trial in range(300):
+        data = bytearray(raw)
+        if trial % 3 == 0:
+            data = data[:rng.randrange(len(data))]
+        else:
+            for _ in range(rng.randrange(1, 4)):
+                data[rng.randrange(len(data))] = rng.randrange(256)
+        try:
+            state = sa.read_table(bytes(data), schema)
+            sa.table_spec(state)
+            outcomes["read"] += 1
+        except sa.SiloFormatError:
+            outcomes["refused"] += 1
+    assert outcomes["refused"] > 100
