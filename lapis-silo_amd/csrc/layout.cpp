@@ -1361,6 +1361,31 @@ namespace rdb {
 // (covered at p, p not missing, in no stored container) hold the local reference symbol, which is
 // a mutation against the reference (symbol_in_set.cpp:185-212 reads them the same way).  Missing
 // positions inside a row's covered range (horizontal_coverage_index.h:25-27) become runs.
+// Sorts keys row << 24 | position << 8 | symbol (row < 2^16, position < 2^16) in ascending order
+// with three counting passes (symbol, position, row): linear in the keys, where a comparison sort
+// of a chunk's millions of entries dominated the finalized upload.
+static void sortRowKeys(std::vector<uint64_t>& keys) {
+   if (keys.size() < 2) return;
+   std::vector<uint64_t> tmp(keys.size());
+   std::vector<uint32_t> count(1u << 16);
+   auto pass = [&](const std::vector<uint64_t>& src, std::vector<uint64_t>& dst, uint32_t shift, uint32_t bits) {
+      const uint64_t mask = (1ull << bits) - 1;
+      std::fill(count.begin(), count.begin() + (1u << bits), 0u);
+      for (uint64_t k : src) ++count[(k >> shift) & mask];
+      uint32_t sum = 0;
+      for (uint32_t b = 0; b < (1u << bits); ++b) {
+         const uint32_t c = count[b];
+         count[b] = sum;
+         sum += c;
+      }
+      for (uint64_t k : src) dst[count[(k >> shift) & mask]++] = k;
+   };
+   pass(keys, tmp, 0, 8);
+   pass(tmp, keys, 8, 16);
+   pass(keys, tmp, 24, 16);
+   keys.swap(tmp);
+}
+
 void buildSequenceColumnFinalized(Table& t, SeqColumn& col, const FinalizedSequence& f) {
    const uint32_t L = col.genome_length;
    const uint32_t nsym = col.abc->count;
@@ -1513,19 +1538,20 @@ void buildSequenceColumnFinalized(Table& t, SeqColumn& col, const FinalizedSeque
             out.push_back((static_cast<uint64_t>(r) << 24) | (static_cast<uint64_t>(p) << 8) | f.local_reference[p]);
          }
       }
-      std::sort(out.begin(), out.end());
+      sortRowKeys(out);
    });
    for (const auto& e : errors) {
       if (!e.empty()) fail(RDB_EINVAL, e);
    }
+   // each chunk counts and writes its own rows' range of the diff form
    std::vector<uint64_t> mut_ptr(n + 1, 0);
-   for (uint32_t c = 0; c < t.n_chunks; ++c) {
+   parallelFor(t.n_chunks, [&](size_t c) {
       for (uint64_t m : muts[c]) mut_ptr[t.chunk_offset[c] + (m >> 24) + 1] += 1;
-   }
+   });
    for (uint64_t i = 0; i < n; ++i) mut_ptr[i + 1] += mut_ptr[i];
    std::vector<uint32_t> mut_pos(mut_ptr[n]);
    std::vector<uint8_t> mut_sym(mut_ptr[n]);
-   for (uint32_t c = 0; c < t.n_chunks; ++c) {
+   parallelFor(t.n_chunks, [&](size_t c) {
       uint64_t w = mut_ptr[t.chunk_offset[c]];
       for (uint64_t m : muts[c]) {  // sorted by row, then position
          mut_pos[w] = static_cast<uint32_t>((m >> 8) & 0xFFFF);
@@ -1533,7 +1559,7 @@ void buildSequenceColumnFinalized(Table& t, SeqColumn& col, const FinalizedSeque
          ++w;
       }
       std::vector<uint64_t>().swap(muts[c]);
-   }
+   });
    col.local_ref_given.assign(f.local_reference, f.local_reference + L);
    buildSequenceColumn(t, col, f.is_null, f.cov_start, f.cov_end, mut_ptr.data(), mut_pos.data(), mut_sym.data(),
                        nrun_ptr.data(), nrun_start.data(), nrun_end.data());
