@@ -110,7 +110,8 @@ def finalize_column(col: SeqColumnData, chunk_sizes: np.ndarray) -> FinalizedSeq
     e_chunk = (key >> 8) & 0xFFFF
     e_sym = key & 0xFF
     cuts = np.nonzero(np.diff(key))[0] + 1
-    bounds = np.concatenate([[0], cuts, [len(key)]]).astype(np.int64)
+    # container boundaries in the sorted entries (no entries: no containers)
+    bounds = np.concatenate([[0], cuts, [len(key)]]).astype(np.int64) if len(key) else np.zeros(1, np.int64)
     k = len(bounds) - 1
     first = bounds[:-1]
     card = np.diff(bounds)

@@ -345,3 +345,23 @@ def test_corrupted_archives_fail_cleanly(tmp_path):
         except sa.SiloFormatError:
             outcomes["refused"] += 1
     assert outcomes["refused"] > 100
+
+
+def test_round_trip_of_tables_without_containers_or_rows(tmp_path):
+    """A column with no stored container (a null row, an all-missing row, a reference row), and a
+    table with no rows, write and read back unchanged."""
+    from silo_amd import silo_archive as sa, table
+    from silo_amd.finalize import finalize_column
+
+    for k, (rows, sizes) in enumerate(((["ACGT", None, "NNNN"], [3]), ([], []))):
+        n = len(rows)
+        spec = table.TableSpec(chunk_sizes=np.asarray(sizes, np.uint32))
+        spec.seq_columns.append(finalize_column(ss.seq_column("main", 0, "ACGT", rows), spec.chunk_sizes))
+        spec.int_columns.append(table.IntColumnData("d", True, np.arange(n, dtype=np.int32), np.zeros(n, np.uint8)))
+        assert len(spec.seq_columns[0].key_position) == 0
+        back = sa.load_database(sa.save_database(str(tmp_path / str(k)), {"default": spec}, 3)).table_spec()
+        assert back.n_rows == n
+        assert same(spec.seq_columns[0], back.seq_columns[0])
+        if n:
+            assert back.seq_columns[0].cov_end.tolist() == [4, 0, 0]
+            assert back.seq_columns[0].is_null.tolist() == [0, 1, 0]
