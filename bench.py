@@ -475,9 +475,10 @@ def main():
     ap.add_argument("--planner-epochs", type=int, default=30)
     ap.add_argument("--requests", type=int, default=4000, help="config 4: requests served")
     ap.add_argument("--rate", type=float, default=3000.0, help="config 4: Poisson arrival rate (requests/s)")
-    ap.add_argument("--sweep-rates", default="2000,3000,4000,5000,6000,8000,10000,12000,16000,20000,24000,32000",
+    ap.add_argument("--sweep-rates", default="2000,2500,3000,3500,4000,5000,6000,8000,10000,12000,16000,20000,24000,32000",
                     help="config 4: offered rates of the capacity sweep, ascending (empty: no sweep)")
-    ap.add_argument("--sweep-seconds", type=float, default=1.5, help="config 4: arrivals per sweep point, in seconds")
+    ap.add_argument("--sweep-seconds", type=float, default=4.0,
+                    help="config 4: arrivals per sweep point, in seconds (the p99 of a heavy-tailed mix needs many)")
     ap.add_argument("--p99-target-ms", type=float, default=50.0, help="config 4: p99 latency bound of the sweep")
     ap.add_argument("--serving-graphs", type=int, default=1,
                     help="config 4: replay captured HIP graphs per lane job (0: launch the kernels directly)")
