@@ -309,9 +309,10 @@ void enqueueJob(Table& t, BatchResult& b, const std::vector<CompiledQuery>& cq, 
       const query::Emitter& e = cq[qi].program;
       progs[slot] = ProgramDesc{n_instr, static_cast<uint32_t>(e.code.size()), n_range, slot,
                                 n_range + static_cast<uint32_t>(e.ranges.size()), static_cast<uint32_t>(e.prefetch.size())};
-      std::memcpy(instrs + n_instr, e.code.data(), e.code.size() * sizeof(Instr));
-      std::memcpy(ranges + n_range, e.ranges.data(), e.ranges.size() * 4);
-      std::memcpy(ranges + n_range + e.ranges.size(), e.prefetch.data(), e.prefetch.size() * 4);
+      // (empty vectors may hold null data(): memcpy is only called for non-empty ones)
+      if (!e.code.empty()) std::memcpy(instrs + n_instr, e.code.data(), e.code.size() * sizeof(Instr));
+      if (!e.ranges.empty()) std::memcpy(ranges + n_range, e.ranges.data(), e.ranges.size() * 4);
+      if (!e.prefetch.empty()) std::memcpy(ranges + n_range + e.ranges.size(), e.prefetch.data(), e.prefetch.size() * 4);
       n_instr += static_cast<uint32_t>(e.code.size());
       n_range += static_cast<uint32_t>(e.ranges.size() + e.prefetch.size());
       depth = std::max(depth, cq[qi].depth);

@@ -761,7 +761,7 @@ std::vector<MutationRowDev> mutationRows(Table& t, const DeviceBitmap& f, int id
    if (rows_n > row_cap) fail(RDB_EDEVICE, "internal: mutation row buffer overflow");
    std::vector<MutationRowDev> rows(rows_n);
    const size_t in_first = std::min<size_t>(rows_n, (first_bytes - 16) / sizeof(MutationRowDev));
-   std::memcpy(rows.data(), host + 16, in_first * sizeof(MutationRowDev));
+   if (in_first) std::memcpy(rows.data(), host + 16, in_first * sizeof(MutationRowDev));
    if (rows_n > in_first) {
       hipCheck(hipMemcpyAsync(rows.data() + in_first, rows_dev + in_first, (rows_n - in_first) * sizeof(MutationRowDev),
                               hipMemcpyDeviceToHost, t.ctx->stream),

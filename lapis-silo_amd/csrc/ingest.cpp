@@ -304,7 +304,7 @@ struct Hasher {
          mix(w);
       }
       uint64_t tail = 0;
-      std::memcpy(&tail, c + i, bytes - i);
+      if (bytes > i) std::memcpy(&tail, c + i, bytes - i);  // an empty vector's data() may be null
       mix(tail ^ (static_cast<uint64_t>(bytes) << 56));
    }
    void mix(uint64_t w) {

@@ -1474,9 +1474,12 @@ void buildSequenceColumnFinalized(Table& t, SeqColumn& col, const FinalizedSeque
          const uint8_t type = f.typecode[k];
          const uint64_t o = f.payload_offset[k];
          const uint32_t card = f.cardinality[k];
+         // [o, o + n) inside the payload, without wrapping for offsets near 2^64
+         auto fits = [&](uint64_t n) { return o <= f.payload_bytes && n <= f.payload_bytes - o; };
+         if (!fits(0)) return false;
          const uint8_t* b = f.payload + o;
          if (type == 1) {  // bitset: 1024 little-endian u64 words
-            if (o + 8192 > f.payload_bytes) return false;
+            if (!fits(8192)) return false;
             for (uint32_t w = 0; w < kChunkWords; ++w) {
                uint64_t x;
                std::memcpy(&x, b + 8 * w, 8);
@@ -1486,7 +1489,7 @@ void buildSequenceColumnFinalized(Table& t, SeqColumn& col, const FinalizedSeque
                }
             }
          } else if (type == 2) {  // array: card sorted u16
-            if (o + 2ULL * card > f.payload_bytes) return false;
+            if (!fits(2ULL * card)) return false;
             for (uint32_t j = 0; j < card; ++j) {
                uint16_t v;
                std::memcpy(&v, b + 2 * j, 2);
@@ -1494,10 +1497,10 @@ void buildSequenceColumnFinalized(Table& t, SeqColumn& col, const FinalizedSeque
                rows.push_back(v);
             }
          } else if (type == 3) {  // run: u16 run count, then (start, length - 1) pairs
-            if (o + 2 > f.payload_bytes) return false;
+            if (!fits(2)) return false;
             uint16_t nr;
             std::memcpy(&nr, b, 2);
-            if (o + 2 + 4ULL * nr > f.payload_bytes) return false;
+            if (!fits(2 + 4ULL * nr)) return false;
             for (uint32_t j = 0; j < nr; ++j) {
                uint16_t v, len;
                std::memcpy(&v, b + 2 + 4 * j, 2);

@@ -49,10 +49,15 @@ std::vector<uint8_t> programBlob(const Emitter& program) {
    ProgramDesc pd{0, static_cast<uint32_t>(program.code.size()), 0, 0, static_cast<uint32_t>(program.ranges.size()),
                   static_cast<uint32_t>(program.prefetch.size())};
    std::memcpy(blob.data(), &pd, sizeof(pd));
-   std::memcpy(blob.data() + sizeof(pd), program.code.data(), program.code.size() * sizeof(Instr));
-   std::memcpy(blob.data() + sizeof(pd) + program.code.size() * sizeof(Instr), program.ranges.data(), program.ranges.size() * 4);
-   std::memcpy(blob.data() + sizeof(pd) + program.code.size() * sizeof(Instr) + program.ranges.size() * 4,
-               program.prefetch.data(), program.prefetch.size() * 4);
+   uint8_t* at = blob.data() + sizeof(pd);
+   // (empty vectors may hold null data(): memcpy is only called for non-empty ones)
+   auto append = [&](const void* src, size_t bytes) {
+      if (bytes) std::memcpy(at, src, bytes);
+      at += bytes;
+   };
+   append(program.code.data(), program.code.size() * sizeof(Instr));
+   append(program.ranges.data(), program.ranges.size() * 4);
+   append(program.prefetch.data(), program.prefetch.size() * 4);
    return blob;
 }
 
