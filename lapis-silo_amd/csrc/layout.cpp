@@ -409,6 +409,30 @@ const DictColumn* Table::dictColumn(const std::string& name) const {
 // ---- sequence column ---------------------------------------------------------------------------
 namespace {
 
+// An allocator whose value-less construct leaves elements uninitialised: the build's large arrays
+// are written in full by parallel passes, and zeroing them first (on one thread, taking every page
+// fault there) cost more than the passes.
+template <typename T>
+struct NoInit : std::allocator<T> {
+   template <typename U>
+   struct rebind {
+      using other = NoInit<U>;
+   };
+   NoInit() = default;
+   template <typename U>
+   NoInit(const NoInit<U>&) {}
+   template <typename U, typename... A>
+   void construct(U* p, A&&... a) {
+      if constexpr (sizeof...(A) == 0) {
+         ::new (static_cast<void*>(p)) U;
+      } else {
+         ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+      }
+   }
+};
+template <typename T>
+using RawVector = std::vector<T, NoInit<T>>;
+
 struct ChunkContainers {
    std::vector<uint32_t> key;  // position << 8 | symbol, sorted
    std::vector<uint32_t> rows; // rows of each container
@@ -678,19 +702,39 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    std::vector<uint32_t> sym_mask(L, 0);
    std::vector<uint64_t> pos_bytes(L, 0);
    col.chunk_index_bytes.assign(t.n_chunks, 0);
-   for (uint32_t c = 0; c < t.n_chunks; ++c) {
-      const auto& ch = chunks[c];
-      for (size_t k = 0; k < ch.key.size(); ++k) {
-         const uint32_t p = ch.key[k] >> 8, s = ch.key[k] & 0xFF;
-         sym_mask[p] |= 1u << s;
-         const uint64_t b = containerPayloadBytes(ch.arenas.desc[k]) + sizeof(Desc);
-         pos_bytes[p] += b;
-         if (s < nsym) {
-            col.chunk_index_bytes[c] += b;
-            col.n_containers += 1;
-            col.payload_bytes += containerPayloadBytes(ch.arenas.desc[k]);
+   // Per-position statistics in parallel over position ranges: every chunk's keys are sorted by
+   // position, so a range's keys of a chunk are found by binary search and each position is written
+   // by one range only.  Per-chunk sums are kept per range and added after.
+   const uint32_t n_ranges = std::min<uint32_t>(64, L);
+   auto rangeKeys = [&](const ChunkContainers& ch, uint32_t r) {
+      const uint32_t p0 = static_cast<uint32_t>(uint64_t{L} * r / n_ranges), p1 = static_cast<uint32_t>(uint64_t{L} * (r + 1) / n_ranges);
+      const auto a = std::lower_bound(ch.key.begin(), ch.key.end(), p0 << 8);
+      const auto e = std::lower_bound(a, ch.key.end(), p1 << 8);
+      return std::pair<size_t, size_t>(static_cast<size_t>(a - ch.key.begin()), static_cast<size_t>(e - ch.key.begin()));
+   };
+   std::vector<uint64_t> range_chunk_bytes(static_cast<size_t>(n_ranges) * t.n_chunks, 0), range_containers(n_ranges, 0),
+      range_payload(n_ranges, 0);
+   parallelFor(n_ranges, [&](size_t r) {
+      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+         const auto& ch = chunks[c];
+         const auto [k0, k1] = rangeKeys(ch, static_cast<uint32_t>(r));
+         for (size_t k = k0; k < k1; ++k) {
+            const uint32_t p = ch.key[k] >> 8, s = ch.key[k] & 0xFF;
+            sym_mask[p] |= 1u << s;
+            const uint64_t b = containerPayloadBytes(ch.arenas.desc[k]) + sizeof(Desc);
+            pos_bytes[p] += b;
+            if (s < nsym) {
+               range_chunk_bytes[r * t.n_chunks + c] += b;
+               range_containers[r] += 1;
+               range_payload[r] += containerPayloadBytes(ch.arenas.desc[k]);
+            }
          }
       }
+   });
+   for (uint32_t r = 0; r < n_ranges; ++r) {
+      for (uint32_t c = 0; c < t.n_chunks; ++c) col.chunk_index_bytes[c] += range_chunk_bytes[static_cast<size_t>(r) * t.n_chunks + c];
+      col.n_containers += range_containers[r];
+      col.payload_bytes += range_payload[r];
    }
    col.pos_slot_ptr.assign(L + 1, 0);
    for (uint32_t p = 0; p < L; ++p) col.pos_slot_ptr[p + 1] = col.pos_slot_ptr[p] + std::popcount(sym_mask[p]);
@@ -707,14 +751,17 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
 
    // rows per slot over all chunks (planner statistics)
    col.slot_rows.assign(n_slots, 0);
-   for (uint32_t c = 0; c < t.n_chunks; ++c) {
-      const auto& ch = chunks[c];
-      for (size_t k = 0; k < ch.key.size(); ++k) {
-         const uint32_t p = ch.key[k] >> 8, sym = ch.key[k] & 0xFF;
-         const uint32_t slot = col.pos_slot_ptr[p] + static_cast<uint32_t>(std::popcount(sym_mask[p] & ((1u << sym) - 1u)));
-         col.slot_rows[slot] += ch.rows[k];
+   parallelFor(n_ranges, [&](size_t r) {
+      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+         const auto& ch = chunks[c];
+         const auto [k0, k1] = rangeKeys(ch, static_cast<uint32_t>(r));
+         for (size_t k = k0; k < k1; ++k) {
+            const uint32_t p = ch.key[k] >> 8, sym = ch.key[k] & 0xFF;
+            const uint32_t slot = col.pos_slot_ptr[p] + static_cast<uint32_t>(std::popcount(sym_mask[p] & ((1u << sym) - 1u)));
+            col.slot_rows[slot] += ch.rows[k];
+         }
       }
-   }
+   });
 
    // --- position blocks: <= 4096 slots and about equal bytes --------------------------------------
    const uint64_t total_bytes = std::accumulate(pos_bytes.begin(), pos_bytes.end(), uint64_t{0});
@@ -755,11 +802,6 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    // descriptors into the concatenated arenas.
    const size_t n_tiles = static_cast<size_t>(n_blocks) * t.n_chunks;
    std::vector<uint32_t> light_ptr(n_tiles + 1, 0), heavy_ptr(n_tiles + 1, 0), group_ptr(n_tiles + 1, 0);
-   std::vector<uint32_t> light;
-   std::vector<uint16_t> group_rows;  // 8 per group
-   std::vector<uint16_t> group_hdr;
-   std::vector<Desc> desc;
-   std::vector<size_t> cursor(t.n_chunks, 0);
    // per (chunk, position): [lo, hi) of the position's light entries, groups and descriptors, and
    // the mask of 4096-row slices that hold any of its rows
    std::vector<uint32_t> pos_ranges(static_cast<size_t>(t.n_chunks) * L * kPosRecord, 0);
@@ -789,81 +831,122 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
       }
       return m;
    };
-   for (uint32_t b = 0; b < n_blocks; ++b) {
-      const uint32_t pend = col.block_pos_start[b + 1];
-      const uint32_t sbase = col.block_slot_start[b];
-      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+   // rows a container holds (a bitset counts as a full chunk): at most kLightMaxRows make light entries
+   auto containerRows = [&](const ChunkContainers& ch, const Desc& d) -> uint32_t {
+      const uint32_t type = descType(d.w0), card = descCard(d.w0);
+      if (type == CT_BITSET) return kChunkRows;
+      if (type != CT_RUN) return card;
+      uint32_t rows = 0;
+      for (uint32_t j = 0; j < card; ++j) rows += (ch.arenas.runs[d.w1 + j] >> 16) + 1;
+      return rows;
+   };
+   // Tiles are laid out block-major ((block, chunk) order) but built per chunk in parallel: a first
+   // pass counts each tile's light entries (padded to a multiple of 4), groups and descriptors, a
+   // prefix sum over the tiles places them, and a second pass writes them.
+   std::vector<uint64_t> n_light(n_tiles), n_group(n_tiles), n_desc(n_tiles);
+   parallelFor(t.n_chunks, [&](size_t c) {
+      const auto& ch = chunks[c];
+      size_t k = 0;
+      for (uint32_t b = 0; b < n_blocks; ++b) {
+         const uint32_t pend = col.block_pos_start[b + 1];
+         uint64_t nl = 0, ng = 0, nd = 0;
+         for (; k < ch.key.size() && (ch.key[k] >> 8) < pend; ++k) {
+            const Desc& d = ch.arenas.desc[k];
+            const uint32_t rows = containerRows(ch, d);
+            if (rows <= kLightMaxRows) {
+               nl += rows;
+            } else if (descType(d.w0) == CT_ARRAY) {
+               ng += (descCard(d.w0) + 7) / 8;
+            } else {
+               nd += 1;
+            }
+         }
          const size_t tile = static_cast<size_t>(b) * t.n_chunks + c;
-         light_ptr[tile] = static_cast<uint32_t>(light.size());
-         heavy_ptr[tile] = static_cast<uint32_t>(desc.size());
-         group_ptr[tile] = static_cast<uint32_t>(group_hdr.size());
-         auto& ch = chunks[c];
-         size_t& k = cursor[c];
+         n_light[tile] = (nl + 3) & ~uint64_t{3};
+         n_group[tile] = ng;
+         n_desc[tile] = nd;
+      }
+   });
+   {
+      uint64_t l = 0, g = 0, d = 0;
+      for (size_t tile = 0; tile < n_tiles; ++tile) {
+         light_ptr[tile] = static_cast<uint32_t>(l);
+         group_ptr[tile] = static_cast<uint32_t>(g);
+         heavy_ptr[tile] = static_cast<uint32_t>(d);
+         l += n_light[tile];
+         g += n_group[tile];
+         d += n_desc[tile];
+         if (l > 0xFFFFFFF0ULL || d > 0xFFFFFFF0ULL || g > 0xFFFFFFF0ULL) fail(RDB_ENOMEM, "too many containers");
+      }
+      light_ptr[n_tiles] = static_cast<uint32_t>(l);
+      group_ptr[n_tiles] = static_cast<uint32_t>(g);
+      heavy_ptr[n_tiles] = static_cast<uint32_t>(d);
+   }
+   RawVector<uint32_t> light(light_ptr[n_tiles]);
+   RawVector<uint16_t> group_rows(static_cast<size_t>(group_ptr[n_tiles]) * 8);  // 8 per group
+   RawVector<uint16_t> group_hdr(group_ptr[n_tiles]);
+   RawVector<Desc> desc(heavy_ptr[n_tiles]);
+   parallelFor(t.n_chunks, [&](size_t c) {
+      const auto& ch = chunks[c];
+      size_t k = 0;
+      for (uint32_t b = 0; b < n_blocks; ++b) {
+         const uint32_t pend = col.block_pos_start[b + 1];
+         const uint32_t sbase = col.block_slot_start[b];
+         const size_t tile = static_cast<size_t>(b) * t.n_chunks + c;
+         uint32_t li = light_ptr[tile], gi = group_ptr[tile], di = heavy_ptr[tile];
          uint32_t range_pos = kInvalidId;
-         while (k < ch.key.size() && (ch.key[k] >> 8) < pend) {
+         for (; k < ch.key.size() && (ch.key[k] >> 8) < pend; ++k) {
             const uint32_t p = ch.key[k] >> 8, s = ch.key[k] & 0xFF;
             uint32_t* pr = pos_ranges.data() + (static_cast<size_t>(c) * L + p) * kPosRecord;
             if (p != range_pos) {
                range_pos = p;
-               pr[0] = static_cast<uint32_t>(light.size());
-               pr[2] = static_cast<uint32_t>(group_hdr.size());
-               pr[4] = static_cast<uint32_t>(desc.size());
+               pr[0] = li;
+               pr[2] = gi;
+               pr[4] = di;
             }
             const uint32_t slot = col.pos_slot_ptr[p] + std::popcount(sym_mask[p] & ((1u << s) - 1u)) - sbase;
             Desc d = ch.arenas.desc[k];
             pr[6] |= sliceMask(ch, d);
             const uint32_t type = descType(d.w0), card = descCard(d.w0);
-            uint32_t rows = card;
-            if (type == CT_RUN) {
-               rows = 0;
-               for (uint32_t j = 0; j < card; ++j) rows += (ch.arenas.runs[d.w1 + j] >> 16) + 1;
-            } else if (type == CT_BITSET) {
-               rows = kChunkRows;
-            }
-            if (rows <= kLightMaxRows) {
+            if (containerRows(ch, d) <= kLightMaxRows) {
                const uint32_t hi = slot << 16;
                if (type == CT_INLINE) {
-                  for (uint32_t j = 0; j < card; ++j) light.push_back(hi | ((d.w1 >> (16 * j)) & 0xFFFFu));
+                  for (uint32_t j = 0; j < card; ++j) light[li++] = hi | ((d.w1 >> (16 * j)) & 0xFFFFu);
                } else if (type == CT_ARRAY) {
-                  for (uint32_t j = 0; j < card; ++j) light.push_back(hi | ch.arenas.arr[d.w1 + j]);
+                  for (uint32_t j = 0; j < card; ++j) light[li++] = hi | ch.arenas.arr[d.w1 + j];
                } else {
                   for (uint32_t j = 0; j < card; ++j) {
                      const uint32_t x = ch.arenas.runs[d.w1 + j];
-                     for (uint32_t r = (x & 0xFFFFu); r <= (x & 0xFFFFu) + (x >> 16); ++r) light.push_back(hi | r);
+                     for (uint32_t r = (x & 0xFFFFu); r <= (x & 0xFFFFu) + (x >> 16); ++r) light[li++] = hi | r;
                   }
                }
             } else if (type == CT_ARRAY) {
                const uint16_t* a = ch.arenas.arr.data() + d.w1;
                for (uint32_t j = 0; j < card; j += 8) {
                   const uint32_t m = std::min<uint32_t>(8, card - j);
-                  for (uint32_t u = 0; u < 8; ++u) group_rows.push_back(u < m ? a[j + u] : 0);
-                  group_hdr.push_back(static_cast<uint16_t>(slot | ((m - 1) << 12)));
+                  for (uint32_t u = 0; u < 8; ++u) group_rows[static_cast<size_t>(gi) * 8 + u] = u < m ? a[j + u] : 0;
+                  group_hdr[gi++] = static_cast<uint16_t>(slot | ((m - 1) << 12));
                }
             } else {
                d.w0 = makeW0(slot, type, card);
-               if (type == CT_ARRAY) d.w1 += static_cast<uint32_t>(arr_base[c]);
                if (type == CT_RUN) d.w1 += static_cast<uint32_t>(run_base[c]);
                if (type == CT_BITSET) d.w1 += static_cast<uint32_t>(bit_base[c]);
-               desc.push_back(d);
+               desc[di++] = d;
             }
-            pr[1] = static_cast<uint32_t>(light.size());
-            pr[3] = static_cast<uint32_t>(group_hdr.size());
-            pr[5] = static_cast<uint32_t>(desc.size());
-            ++k;
+            pr[1] = li;
+            pr[3] = gi;
+            pr[5] = di;
          }
-         while (light.size() % 4 != 0) light.push_back(kPadSlot << 16);
-         if (light.size() > 0xFFFFFFF0ULL || desc.size() > 0xFFFFFFF0ULL || group_hdr.size() > 0xFFFFFFF0ULL) {
-            fail(RDB_ENOMEM, "too many containers");
+         while (li % 4 != 0) light[li++] = kPadSlot << 16;
+         if (li != light_ptr[tile + 1] || gi != group_ptr[tile + 1] || di != heavy_ptr[tile + 1]) {
+            fail(RDB_EDEVICE, "internal: tile sizes");
          }
       }
-   }
-   light_ptr[n_tiles] = static_cast<uint32_t>(light.size());
-   heavy_ptr[n_tiles] = static_cast<uint32_t>(desc.size());
-   group_ptr[n_tiles] = static_cast<uint32_t>(group_hdr.size());
+   });
 
-   std::vector<uint16_t> arr_all(arr_base[t.n_chunks]);
-   std::vector<uint32_t> run_all(run_base[t.n_chunks]);
-   std::vector<uint64_t> bit_all(bit_base[t.n_chunks] * kChunkWords);
+   RawVector<uint16_t> arr_all(arr_base[t.n_chunks]);
+   RawVector<uint32_t> run_all(run_base[t.n_chunks]);
+   RawVector<uint64_t> bit_all(bit_base[t.n_chunks] * kChunkWords);
    parallelFor(t.n_chunks, [&](size_t c) {
       auto& a = chunks[c].arenas;
       std::copy(a.arr.begin(), a.arr.end(), arr_all.begin() + static_cast<ptrdiff_t>(arr_base[c]));
@@ -882,94 +965,88 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    const size_t padded = static_cast<size_t>(t.n_chunks) << 16;
    std::vector<uint32_t> cov(padded, 0), pptr(padded + 1, 0);
    std::vector<uint64_t> null_bits(static_cast<size_t>(t.n_chunks) * kChunkWords, 0);
-   std::vector<uint32_t> null_rows;
    col.chunk_nruns.assign(t.n_chunks, 0);
-   for (uint32_t c = 0; c < t.n_chunks; ++c) {
-      for (uint32_t r = 0; r < t.chunk_sizes[c]; ++r) {
-         const uint64_t i = t.chunk_offset[c] + r;
-         const size_t g = (static_cast<size_t>(c) << 16) | r;
+   if (nptr[n] > 0xFFFFFFFFULL) fail(RDB_ENOMEM, "too many missing runs");
+   // Every array below is chunk-local except for running offsets, which come from per-chunk counts:
+   // the chunks are filled in parallel.  Null rows per chunk, concatenated in chunk order after.
+   std::vector<std::vector<uint32_t>> chunk_null_rows(t.n_chunks);
+   std::vector<uint64_t> chunk_muts(t.n_chunks + 1, 0);
+   parallelFor(t.n_chunks, [&](size_t c) {
+      const uint64_t off = t.chunk_offset[c];
+      const uint32_t size = t.chunk_sizes[c];
+      uint64_t muts = 0;
+      for (uint32_t r = 0; r < size; ++r) {
+         const uint64_t i = off + r;
+         const size_t g = (c << 16) | r;
          if (is_null[i]) {
             null_bits[g >> 6] |= 1ULL << (g & 63);
-            null_rows.push_back(static_cast<uint32_t>(g));
+            chunk_null_rows[c].push_back(static_cast<uint32_t>(g));
          } else {
             if (cov_start[i] < cov_end[i]) cov[g] = cov_start[i] | (cov_end[i] << 16);  // empty range: [0, 0)
-         }
-         col.chunk_nruns[c] += nptr[i + 1] - nptr[i];
-      }
-   }
-   {
-      // pptr over padded rows
-      uint32_t acc = 0;
-      if (nptr[n] > 0xFFFFFFFFULL) fail(RDB_ENOMEM, "too many missing runs");
-      for (uint32_t c = 0; c < t.n_chunks; ++c) {
-         for (uint32_t r = 0; r < kChunkRows; ++r) {
-            const size_t g = (static_cast<size_t>(c) << 16) | r;
-            pptr[g] = acc;
-            if (r < t.chunk_sizes[c]) {
-               const uint64_t i = t.chunk_offset[c] + r;
-               acc += static_cast<uint32_t>(nptr[i + 1] - nptr[i]);
-            }
+            muts += mut_ptr[i + 1] - mut_ptr[i];
          }
       }
-      pptr[padded] = acc;
-   }
-   // missing-run index by (chunk, start bucket or "long", slice), rows in order
+      col.chunk_nruns[c] = nptr[off + size] - nptr[off];
+      chunk_muts[c + 1] = muts;
+      // pptr over padded rows: the runs of every row before, null rows included
+      for (uint32_t r = 0; r < kChunkRows; ++r) pptr[(c << 16) | r] = static_cast<uint32_t>(nptr[off + std::min(r, size)] - nptr[0]);
+   });
+   pptr[padded] = static_cast<uint32_t>(nptr[n] - nptr[0]);
+   std::vector<uint32_t> null_rows;
+   for (auto& v : chunk_null_rows) null_rows.insert(null_rows.end(), v.begin(), v.end());
+   // missing-run index by (chunk, start bucket or "long", slice), rows in order: each chunk owns its
+   // lists, so counting and filling run per chunk
    const uint32_t n_rb = (L + kRunBucket - 1) / kRunBucket;
    const size_t n_lists = static_cast<size_t>(t.n_chunks) * (n_rb + 1) * kSlicesPerChunk;
    std::vector<uint32_t> ri_ptr(n_lists + 1, 0);
-   std::vector<uint64_t> ri;
+   RawVector<uint64_t> ri;
    {
       auto listOf = [&](uint32_t c, uint32_t r, uint32_t run) {
          const uint32_t a = run & 0xFFFFu, b = run >> 16;
          const uint32_t k = b - a > kShortRun ? n_rb : a / kRunBucket;
          return (static_cast<size_t>(c) * (n_rb + 1) + k) * kSlicesPerChunk + (r >> 12);
       };
-      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+      parallelFor(t.n_chunks, [&](size_t c) {
          for (uint32_t r = 0; r < t.chunk_sizes[c]; ++r) {
             const uint64_t i = t.chunk_offset[c] + r;
-            for (uint64_t k = nptr[i]; k < nptr[i + 1]; ++k) ri_ptr[listOf(c, r, runs_all[k]) + 1] += 1;
+            for (uint64_t k = nptr[i]; k < nptr[i + 1]; ++k) ri_ptr[listOf(static_cast<uint32_t>(c), r, runs_all[k]) + 1] += 1;
          }
-      }
+      });
       for (size_t l = 0; l < n_lists; ++l) ri_ptr[l + 1] += ri_ptr[l];
       if (ri_ptr[n_lists] != nptr[n]) fail(RDB_EDEVICE, "internal: missing-run index size");
       ri.resize(ri_ptr[n_lists]);
       std::vector<uint32_t> fill(ri_ptr.begin(), ri_ptr.end() - 1);
-      for (uint32_t c = 0; c < t.n_chunks; ++c) {
+      parallelFor(t.n_chunks, [&](size_t c) {
          for (uint32_t r = 0; r < t.chunk_sizes[c]; ++r) {
             const uint64_t i = t.chunk_offset[c] + r;
             for (uint64_t k = nptr[i]; k < nptr[i + 1]; ++k) {
                const uint32_t run = runs_all[k];
-               ri[fill[listOf(c, r, run)]++] = r | (static_cast<uint64_t>(run & 0xFFFFu) << 16) |
-                                              (static_cast<uint64_t>(run >> 16) << 32);
+               ri[fill[listOf(static_cast<uint32_t>(c), r, run)]++] = r | (static_cast<uint64_t>(run & 0xFFFFu) << 16) |
+                                                                     (static_cast<uint64_t>(run >> 16) << 32);
             }
          }
-      }
+      });
    }
    col.has_nulls = !null_rows.empty();
    col.null_bitmap_id = t.addStoreBitmap(null_rows);
 
    // row-major mutations over padded rows (null rows have none)
-   std::vector<uint32_t> rmptr(padded + 1, 0), rmuts;
-   {
-      uint64_t total = 0;
-      for (uint64_t i = 0; i < n; ++i) {
-         if (!is_null[i]) total += mut_ptr[i + 1] - mut_ptr[i];
+   for (uint32_t c = 0; c < t.n_chunks; ++c) chunk_muts[c + 1] += chunk_muts[c];
+   if (chunk_muts[t.n_chunks] > 0xFFFFFFFFULL) fail(RDB_ENOMEM, "too many mutations for 32-bit offsets");
+   std::vector<uint32_t> rmptr(padded + 1, 0);
+   RawVector<uint32_t> rmuts(chunk_muts[t.n_chunks]);
+   parallelFor(t.n_chunks, [&](size_t c) {
+      uint32_t acc = static_cast<uint32_t>(chunk_muts[c]);
+      for (uint32_t r = 0; r < kChunkRows; ++r) {
+         const size_t g = (c << 16) | r;
+         rmptr[g] = acc;
+         if (r >= t.chunk_sizes[c]) continue;
+         const uint64_t i = t.chunk_offset[c] + r;
+         if (is_null[i]) continue;
+         for (uint64_t k = mut_ptr[i]; k < mut_ptr[i + 1]; ++k) rmuts[acc++] = mut_pos[k] | (static_cast<uint32_t>(mut_sym[k]) << 16);
       }
-      if (total > 0xFFFFFFFFULL) fail(RDB_ENOMEM, "too many mutations for 32-bit offsets");
-      rmuts.resize(total);
-      uint32_t acc = 0;
-      for (uint32_t c = 0; c < t.n_chunks; ++c) {
-         for (uint32_t r = 0; r < kChunkRows; ++r) {
-            const size_t g = (static_cast<size_t>(c) << 16) | r;
-            rmptr[g] = acc;
-            if (r >= t.chunk_sizes[c]) continue;
-            const uint64_t i = t.chunk_offset[c] + r;
-            if (is_null[i]) continue;
-            for (uint64_t k = mut_ptr[i]; k < mut_ptr[i + 1]; ++k) rmuts[acc++] = mut_pos[k] | (static_cast<uint32_t>(mut_sym[k]) << 16);
-         }
-      }
-      rmptr[padded] = acc;
-   }
+   });
+   rmptr[padded] = static_cast<uint32_t>(chunk_muts[t.n_chunks]);
 
    phase(4);
    // --- checks ------------------------------------------------------------------------------------
