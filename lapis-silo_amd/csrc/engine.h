@@ -254,6 +254,9 @@ struct Table {
    uint64_t job_counter = 0;
 
    uint32_t addStoreBitmap(const std::vector<uint32_t>& sorted_global_rows);
+   // a store bitmap's rows split by chunk (sorted, unique; thread-safe), and its registration
+   std::vector<std::vector<uint16_t>> storeChunks(const std::vector<uint32_t>& global_rows) const;
+   uint32_t addStoreChunks(std::vector<std::vector<uint16_t>>&& per_chunk);
    uint32_t globalRow(uint64_t dense) const;
 
    const SeqColumn& seqColumn(const std::string& name) const;

@@ -356,7 +356,7 @@ uint32_t Table::globalRow(uint64_t dense) const {
    return static_cast<uint32_t>((c << 16) | (dense - *it));
 }
 
-uint32_t Table::addStoreBitmap(const std::vector<uint32_t>& rows) {
+std::vector<std::vector<uint16_t>> Table::storeChunks(const std::vector<uint32_t>& rows) const {
    std::vector<std::vector<uint16_t>> per_chunk(n_chunks);
    for (uint32_t g : rows) {
       const uint32_t c = g >> 16;
@@ -364,12 +364,18 @@ uint32_t Table::addStoreBitmap(const std::vector<uint32_t>& rows) {
       per_chunk[c].push_back(static_cast<uint16_t>(g & 0xFFFF));
    }
    for (auto& v : per_chunk) {
-      std::sort(v.begin(), v.end());
+      if (!std::is_sorted(v.begin(), v.end())) std::sort(v.begin(), v.end());
       v.erase(std::unique(v.begin(), v.end()), v.end());
    }
+   return per_chunk;
+}
+
+uint32_t Table::addStoreChunks(std::vector<std::vector<uint16_t>>&& per_chunk) {
    store_rows.push_back(std::move(per_chunk));
    return static_cast<uint32_t>(store_rows.size() - 1);
 }
+
+uint32_t Table::addStoreBitmap(const std::vector<uint32_t>& rows) { return addStoreChunks(storeChunks(rows)); }
 
 const SeqColumn& Table::seqColumn(const std::string& name) const {
    for (const auto& c : seq) {
@@ -1181,10 +1187,25 @@ void buildIntColumn(Table& t, IntColumn& col, const int32_t* values, const uint8
    col.null_count = null_rows.size();
    col.sorted_values.clear();
    col.sorted_values.reserve(t.num_rows - null_rows.size());
+   int64_t vmin = INT64_MAX, vmax = INT64_MIN;
    for (uint64_t i = 0; i < t.num_rows; ++i) {
-      if (!is_null[i]) col.sorted_values.push_back(values[i]);
+      if (is_null[i]) continue;
+      vmin = std::min<int64_t>(vmin, values[i]);
+      vmax = std::max<int64_t>(vmax, values[i]);
    }
-   std::sort(col.sorted_values.begin(), col.sorted_values.end());
+   if (vmin <= vmax && static_cast<uint64_t>(vmax - vmin) < (uint64_t{1} << 24)) {
+      // counting sort over a narrow range (dates span a few thousand days)
+      std::vector<uint64_t> count(static_cast<size_t>(vmax - vmin) + 1, 0);
+      for (uint64_t i = 0; i < t.num_rows; ++i) {
+         if (!is_null[i]) ++count[static_cast<size_t>(values[i] - vmin)];
+      }
+      for (size_t k = 0; k < count.size(); ++k) col.sorted_values.insert(col.sorted_values.end(), count[k], static_cast<int32_t>(vmin + static_cast<int64_t>(k)));
+   } else {
+      for (uint64_t i = 0; i < t.num_rows; ++i) {
+         if (!is_null[i]) col.sorted_values.push_back(values[i]);
+      }
+      std::sort(col.sorted_values.begin(), col.sorted_values.end());
+   }
    col.buffers.emplace_back(vals.data(), vals.size() * 4);
    col.view.values = col.buffers.back().as<int32_t>();
    col.buffers.emplace_back(null_bits.data(), null_bits.size() * 8);
@@ -1201,7 +1222,8 @@ void buildIntColumn(Table& t, IntColumn& col, const int32_t* values, const uint8
          const uint32_t np = std::max<uint32_t>(1, static_cast<uint32_t>(std::bit_width(span)));
          const size_t plane_words = static_cast<size_t>(t.n_chunks) * kChunkWords;
          std::vector<uint64_t> planes(np * plane_words, 0);
-         for (uint32_t c = 0; c < t.n_chunks; ++c) {
+         // each chunk's plane words are its own
+         parallelFor(t.n_chunks, [&](size_t c) {
             for (uint32_t r = 0; r < t.chunk_sizes[c]; ++r) {
                const uint64_t i = t.chunk_offset[c] + r;
                if (is_null[i]) continue;
@@ -1211,7 +1233,7 @@ void buildIntColumn(Table& t, IntColumn& col, const int32_t* values, const uint8
                   if ((x >> p) & 1u) planes[p * plane_words + (g >> 6)] |= 1ULL << (g & 63);
                }
             }
-         }
+         });
          col.buffers.emplace_back(planes.data(), planes.size() * 8);
          col.view.planes = col.buffers.back().as<uint64_t>();
          col.view.plane_min = static_cast<int32_t>(lo);
@@ -1241,9 +1263,23 @@ void buildDictColumn(Table& t, DictColumn& col, const uint32_t* ids, const uint8
          }
       }
    }
+   // Store bitmaps are split into chunks in parallel, then registered in value order (the ids a
+   // serial build gives).
+   auto addAll = [&](const std::vector<std::vector<uint32_t>>& lists, const std::vector<uint8_t>& use,
+                     std::vector<uint32_t>& ids) {
+      std::vector<std::vector<std::vector<uint16_t>>> split(lists.size());
+      parallelFor(lists.size(), [&](size_t v) {
+         if (use[v]) split[v] = t.storeChunks(lists[v]);
+      });
+      for (size_t v = 0; v < lists.size(); ++v) {
+         if (use[v]) ids[v] = t.addStoreChunks(std::move(split[v]));
+      }
+   };
    col.value_bitmap.assign(nv, kInvalidId);
-   for (uint32_t v = 0; v < nv; ++v) {
-      if (!rows[v].empty()) col.value_bitmap[v] = t.addStoreBitmap(rows[v]);
+   {
+      std::vector<uint8_t> use(nv);
+      for (uint32_t v = 0; v < nv; ++v) use[v] = !rows[v].empty();
+      addAll(rows, use, col.value_bitmap);
    }
    col.has_nulls = !null_rows.empty();
    col.null_bitmap_id = t.addStoreBitmap(null_rows);
@@ -1261,10 +1297,13 @@ void buildDictColumn(Table& t, DictColumn& col, const uint32_t* ids, const uint8
       std::vector<std::vector<uint32_t>> own(nv);
       for (uint32_t v = 0; v < nv; ++v) own[resolved[v]].insert(own[resolved[v]].end(), rows[v].begin(), rows[v].end());
       col.lineage_bitmap.assign(nv, kInvalidId);
-      for (uint32_t v = 0; v < nv; ++v) {
-         if (resolved[v] != v || own[v].empty()) continue;
-         std::sort(own[v].begin(), own[v].end());
-         col.lineage_bitmap[v] = t.addStoreBitmap(own[v]);
+      {
+         std::vector<uint8_t> use(nv);
+         for (uint32_t v = 0; v < nv; ++v) use[v] = resolved[v] == v && !own[v].empty();
+         parallelFor(nv, [&](size_t v) {
+            if (use[v]) std::sort(own[v].begin(), own[v].end());
+         });
+         addAll(own, use, col.lineage_bitmap);
       }
       col.sublineage_bitmap.assign(nv, kInvalidId);
       std::vector<std::vector<uint32_t>> children(nv);
@@ -1274,23 +1313,34 @@ void buildDictColumn(Table& t, DictColumn& col, const uint32_t* ids, const uint8
             children[p].push_back(v);
          }
       }
+      // the lineages under each lineage (a walk over lineages, each visited once per start even
+      // if the parent links form a cycle), then their rows gathered and sorted in parallel
+      std::vector<std::vector<uint32_t>> under(nv);
       std::vector<uint32_t> seen(nv, kInvalidId);
       for (uint32_t v = 0; v < nv; ++v) {
          if (resolved[v] != v) continue;
-         std::vector<uint32_t> acc;
          std::vector<uint32_t> stack{v};
          while (!stack.empty()) {
             const uint32_t w = stack.back();
             stack.pop_back();
             if (seen[w] == v) continue;
             seen[w] = v;
-            acc.insert(acc.end(), own[w].begin(), own[w].end());
+            under[v].push_back(w);
             for (uint32_t ch : children[w]) stack.push_back(ch);
          }
-         if (!acc.empty()) {
-            std::sort(acc.begin(), acc.end());
-            col.sublineage_bitmap[v] = t.addStoreBitmap(acc);
-         }
+      }
+      std::vector<std::vector<std::vector<uint16_t>>> split(nv);
+      std::vector<uint8_t> use(nv, 0);
+      parallelFor(nv, [&](size_t v) {
+         std::vector<uint32_t> acc;
+         for (uint32_t w : under[v]) acc.insert(acc.end(), own[w].begin(), own[w].end());
+         if (acc.empty()) return;
+         std::sort(acc.begin(), acc.end());
+         split[v] = t.storeChunks(acc);
+         use[v] = 1;
+      });
+      for (uint32_t v = 0; v < nv; ++v) {
+         if (use[v]) col.sublineage_bitmap[v] = t.addStoreChunks(std::move(split[v]));
       }
       for (uint32_t v = 0; v < nv; ++v) {
          if (resolved[v] == v) continue;
