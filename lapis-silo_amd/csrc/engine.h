@@ -35,6 +35,30 @@ inline void hipCheck(hipError_t e, const char* what) {
 // Runs f(i) for i in [0, n) on up to `threads` host threads.
 void parallelFor(size_t n, const std::function<void(size_t)>& f, unsigned threads = 0);
 
+// An allocator whose value-less construct leaves elements uninitialised: the build's large arrays
+// are written in full by parallel passes, and zeroing them first (on one thread, taking every page
+// fault there) cost more than the passes.
+template <typename T>
+struct NoInit : std::allocator<T> {
+   template <typename U>
+   struct rebind {
+      using other = NoInit<U>;
+   };
+   NoInit() = default;
+   template <typename U>
+   NoInit(const NoInit<U>&) {}
+   template <typename U, typename... A>
+   void construct(U* p, A&&... a) {
+      if constexpr (sizeof...(A) == 0) {
+         ::new (static_cast<void*>(p)) U;
+      } else {
+         ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+      }
+   }
+};
+template <typename T>
+using RawVector = std::vector<T, NoInit<T>>;
+
 // ---- alphabets (common/nucleotide_symbols.h:23-106, common/aa_symbols.h:20-118) ----------------
 struct Alphabet {
    uint32_t count;

@@ -220,7 +220,7 @@ void DeviceIndexBuild::adapt(std::vector<uint32_t>& coverage, std::vector<uint8_
 }
 
 void DeviceIndexBuild::entries(const std::vector<uint8_t>& flip_to, const std::vector<uint32_t>& flipped,
-                               std::vector<uint64_t>& keys, std::vector<uint64_t>& chunk_ptr) {
+                               RawVector<uint64_t>& keys, std::vector<uint64_t>& chunk_ptr) {
    const hipStream_t stream = t_.ctx->stream;
    const uint64_t n = rows_.n_rows;
    std::vector<DeviceBuffer> tmp;
@@ -258,7 +258,16 @@ void DeviceIndexBuild::entries(const std::vector<uint8_t>& flip_to, const std::v
    e.keys = k0;
    hipCheck(launchEntries(e, true, stream), "entry write launch");
    // sort groups of whole chunks, each under 2^30 keys (chunk ids order the groups)
+   // the host copy of the sorted keys: allocated without zeroing and first touched by the worker
+   // pool, so that the page faults of a gigabyte are not taken by the copy's one thread
    keys.resize(total);
+   {
+      constexpr uint64_t kPiece = uint64_t{1} << 21;
+      parallelFor((total + kPiece - 1) / kPiece, [&](size_t k) {
+         const uint64_t a = k * kPiece, b = std::min(total, a + kPiece);
+         std::memset(keys.data() + a, 0, (b - a) * 8);
+      });
+   }
    constexpr uint64_t kGroupKeys = uint64_t{1} << 30;
    std::vector<std::pair<uint64_t, uint64_t>> groups;
    {

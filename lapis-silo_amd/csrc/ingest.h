@@ -23,7 +23,7 @@ class DeviceIndexBuild {
    void adapt(std::vector<uint32_t>& coverage, std::vector<uint8_t>& local_ref);
    // Every chunk's container entries, sorted: keys[chunk_ptr[c] .. chunk_ptr[c + 1]) are chunk c's,
    // each c << 40 | position << 24 | symbol << 16 | row in chunk.
-   void entries(const std::vector<uint8_t>& flip_to, const std::vector<uint32_t>& flipped, std::vector<uint64_t>& keys,
+   void entries(const std::vector<uint8_t>& flip_to, const std::vector<uint32_t>& flipped, RawVector<uint64_t>& keys,
                 std::vector<uint64_t>& chunk_ptr);
 
   private:

@@ -415,29 +415,6 @@ const DictColumn* Table::dictColumn(const std::string& name) const {
 // ---- sequence column ---------------------------------------------------------------------------
 namespace {
 
-// An allocator whose value-less construct leaves elements uninitialised: the build's large arrays
-// are written in full by parallel passes, and zeroing them first (on one thread, taking every page
-// fault there) cost more than the passes.
-template <typename T>
-struct NoInit : std::allocator<T> {
-   template <typename U>
-   struct rebind {
-      using other = NoInit<U>;
-   };
-   NoInit() = default;
-   template <typename U>
-   NoInit(const NoInit<U>&) {}
-   template <typename U, typename... A>
-   void construct(U* p, A&&... a) {
-      if constexpr (sizeof...(A) == 0) {
-         ::new (static_cast<void*>(p)) U;
-      } else {
-         ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
-      }
-   }
-};
-template <typename T>
-using RawVector = std::vector<T, NoInit<T>>;
 
 struct ChunkContainers {
    std::vector<uint32_t> key;  // position << 8 | symbol, sorted
@@ -629,7 +606,8 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    // --- per-chunk containers ----------------------------------------------------------------------
    // entries position << 24 | symbol << 16 | row, sorted per chunk (from the device: with the chunk
    // in bits 40 and up, all chunks in one array)
-   std::vector<uint64_t> dev_keys, dev_chunk_ptr;
+   RawVector<uint64_t> dev_keys;
+   std::vector<uint64_t> dev_chunk_ptr;
    if (dev) {
       dev->entries(flip_to, flipped, dev_keys, dev_chunk_ptr);
       dev.reset();
