@@ -916,9 +916,16 @@ def _sequence_column(r: ArchiveReader, col: ColumnSchema) -> SequenceColumnState
     nulls = r.roaring()
     num_chunks = r.u16()
     start_end = np.concatenate(chunks) if chunks else np.zeros((0, 2), np.uint32)
+    # the packed payload: the native scan's own array when it read every record (no copy), else the
+    # records' bytes concatenated
+    parts = [p if isinstance(p, np.ndarray) else np.frombuffer(p, np.uint8) for p in payload]
+    if len(parts) == 1 and isinstance(payload[0], np.ndarray):
+        packed = parts[0]
+    else:
+        packed = np.concatenate(parts) if parts else np.zeros(0, np.uint8)
     return SequenceColumnState(
         col.name, alphabet, col.reference, local_ref, pos, chunk, sym, tc, card, offs,
-        np.frombuffer(b"".join(payload), np.uint8).copy(), h_rows, h_ptr, h_run_start, h_run_end, start_end, nulls,
+        packed, h_rows, h_ptr, h_run_start, h_run_end, start_end, nulls,
         seq_count,
         num_chunks, info,
         insertions)
