@@ -211,7 +211,7 @@ def scan_roofline(eng, et, queries, prof, n_chunks, log):
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     kernel = {0: "mutationsCountBatchKernel", 1: "mutationsCountSharedKernel", 2: "mutationsCountOctetKernel"}[mode]
     out = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": scan_pmc_bytes(kernel),
            "algorithmic_bytes_per_launch": int(bytes_per_launch), "dense_queries_per_launch": round(dense, 2),
            "queries_per_pass": per_pass, "launches": launches, "avg_launch_ms": round(avg_ms, 5)}
     stream = eng.read_bandwidth(bytes_per_launch, reps=10)
@@ -378,6 +378,17 @@ def ingest(args):
     eng.close()
     if not equal:
         sys.exit(1)
+
+
+def scan_pmc_bytes(kernel: str):
+    """HBM bytes per launch of the batched dense scan from its FETCH_SIZE / WRITE_SIZE counter runs
+    (profiles/batch_scan_pmc.json), when they cover this kernel; else None."""
+    path = os.path.join(ROOT, "profiles", "batch_scan_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch") if d.get("kernel", "").startswith(kernel) else None
 
 
 def silo_load(args):
