@@ -109,6 +109,14 @@ int rdb_archive_scan_containers(const uint8_t* buf, uint64_t len, uint64_t pos, 
                                 uint16_t* chunk, int32_t* symbol, uint32_t* cardinality, uint8_t* typecode,
                                 uint64_t* payload_offset, uint8_t* payload, uint64_t payload_capacity,
                                 uint64_t* payload_bytes, uint64_t* end);
+/* The maximal runs [start, end) of n portable roaring bitmaps (bitmap i at buf[start[i]], length[i]
+ * bytes; CRoaring's portable format, roaring_serialize.h:16-48): the missing positions of a saved
+ * horizontal coverage index as runs (horizontal_coverage_index.cpp:35-45).  Called first with
+ * run_start / run_end null it fills run_ptr[n + 1] (CSR offsets); called again with arrays of
+ * run_ptr[n] entries (capacity) it writes the runs.  RDB_EINVAL for a malformed bitmap.  No
+ * counterpart in the reference, which deserializes through CRoaring. */
+int rdb_archive_bitmap_runs(const uint8_t* buf, uint64_t len, const uint64_t* start, const uint64_t* length, uint64_t n,
+                            uint64_t* run_ptr, uint32_t* run_start, uint32_t* run_end, uint64_t capacity);
 int rdb_archive_scan_row_bitmaps(const uint8_t* buf, uint64_t len, uint64_t pos, uint64_t n, uint32_t* row,
                                  uint64_t* bitmap_start, uint64_t* bitmap_length, uint64_t* end);
 /* rdb_table_add_sequence_column_finalized with each row's missing positions given as the runs

@@ -4,7 +4,8 @@
 // the reference's Debug build with ASAN.  Both sanitizers stop the process at their first finding.
 //
 //   sanitize_check host    entry points without device work: the diff of aligned sequences, the
-//                          saved-state scanners on whole, truncated and corrupted buffers, the
+//                          saved-state scanners and bitmap decoder on whole, truncated and
+//                          corrupted buffers, the
 //                          NDJSON and Arrow IPC writers with every capacity below the needed one;
 //   sanitize_check device  on GPU 0: tables built from the same rows by the device builder and by
 //                          the diff-form upload (equal digests), sequence, date, integer,
@@ -153,6 +154,47 @@ void checkArchiveScanners(Rng& rng) {
    CHECK(rdb_archive_scan_containers(buf.data(), buf.size(), buf.size() + 5, 1, position.data(), chunk.data(),
                                      symbol.data(), card.data(), tc.data(), off.data(), nullptr, 0, &pb,
                                      &end) == RDB_EINVAL);
+}
+
+// A portable roaring bitmap with a run container (key 0: [5, 10) and [20, 25)), an array container
+// (key 1: 7, 8, 9) and a bitset container (key 2: its first 5,000 rows), decoded to runs whole and
+// after random damage (every outcome is a result or RDB_EINVAL).
+void checkBitmapRuns(Rng& rng) {
+   std::vector<uint8_t> b;
+   put<uint32_t>(b, 12347u | (2u << 16));  // three containers, with the run cookie
+   put<uint8_t>(b, 0x1);                    // container 0 holds runs
+   for (uint16_t v : {0, 9, 1, 2, 2, 4999}) put<uint16_t>(b, v);  // (key, cardinality - 1)
+   for (uint16_t v : {2, 5, 4, 20, 4}) put<uint16_t>(b, v);        // run count, (start, length - 1)
+   for (uint16_t v : {7, 8, 9}) put<uint16_t>(b, v);
+   for (uint32_t w = 0; w < 1024; ++w) {
+      const uint32_t lo = 64 * w;
+      const uint64_t word = lo + 64 <= 5000 ? ~0ull : (lo >= 5000 ? 0ull : (1ull << (5000 - lo)) - 1);
+      put<uint64_t>(b, word);
+   }
+   const uint64_t start = 0, len = b.size();
+   uint64_t ptr[2] = {};
+   CHECK(rdb_archive_bitmap_runs(b.data(), b.size(), &start, &len, 1, ptr, nullptr, nullptr, 0) == RDB_OK);
+   CHECK(ptr[1] == 4);
+   uint32_t rs[4], re[4];
+   CHECK(rdb_archive_bitmap_runs(b.data(), b.size(), &start, &len, 1, ptr, rs, re, 4) == RDB_OK);
+   CHECK(rs[0] == 5 && re[0] == 10 && rs[1] == 20 && re[1] == 25 && rs[2] == 65543 && re[2] == 65546 &&
+         rs[3] == 131072 && re[3] == 131072 + 5000);
+   for (int trial = 0; trial < 400; ++trial) {
+      std::vector<uint8_t> bad(b.begin(), b.begin() + (trial % 3 == 0 ? rng.below(static_cast<uint32_t>(b.size())) : b.size()));
+      if (trial % 3 != 0) {
+         for (uint32_t k = 1 + rng.below(3); k > 0; --k) bad[rng.below(static_cast<uint32_t>(bad.size()))] = static_cast<uint8_t>(rng.next());
+      }
+      const uint64_t blen = bad.size();
+      uint64_t bp[2] = {};
+      int rc = rdb_archive_bitmap_runs(bad.data(), blen, &start, &blen, 1, bp, nullptr, nullptr, 0);
+      CHECK(rc == RDB_OK || rc == RDB_EINVAL);
+      if (rc == RDB_OK) {
+         std::vector<uint32_t> a(bp[1] + 1), e(bp[1] + 1);
+         CHECK(rdb_archive_bitmap_runs(bad.data(), blen, &start, &blen, 1, bp, a.data(), e.data(), bp[1]) == RDB_OK);
+      }
+   }
+   const uint64_t past = b.size() + 1;
+   CHECK(rdb_archive_bitmap_runs(b.data(), b.size(), &start, &past, 1, ptr, nullptr, nullptr, 0) == RDB_EINVAL);
 }
 
 void checkSinks(Rng& rng) {
@@ -675,6 +717,7 @@ int main(int argc, char** argv) {
    if (mode == "host") {
       checkDiff(rng);
       checkArchiveScanners(rng);
+      checkBitmapRuns(rng);
       checkSinks(rng);
    } else if (mode == "device") {
       if (deviceMode() != 0) return 1;

@@ -31,6 +31,7 @@ SIGNATURES = [
                                                         _vp, _u64, _vp, _vp, _vp, _vp, _vp]),
     ("rdb_archive_scan_containers", _int, [_vp, _u64, _u64, _u64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _vp, _vp]),
     ("rdb_archive_scan_row_bitmaps", _int, [_vp, _u64, _u64, _u64, _vp, _vp, _vp, _vp]),
+    ("rdb_archive_bitmap_runs", _int, [_vp, _u64, _vp, _vp, _u64, _vp, _vp, _vp, _u64]),
     ("rdb_table_add_sequence_column_finalized_runs", _int, [_vp, _cp, _int, _vp, _vp, _u32, _u64, _vp, _vp, _vp, _vp, _vp,
                                                              _vp, _vp, _u64, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("rdb_seq_builder_create", _int, [_vp, _cp, _int, _vp, _u32, ctypes.POINTER(_vp)]),

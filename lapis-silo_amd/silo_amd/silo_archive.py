@@ -261,61 +261,29 @@ def decode_bitmaps(data: np.ndarray, starts: np.ndarray, lens: np.ndarray, name:
     return np.concatenate(ptrs), (np.concatenate(vals) if vals else np.zeros(0, np.uint32))
 
 
-def decode_bitmap_runs(data: np.ndarray, starts: np.ndarray, lens: np.ndarray, name: str = "roaring",
-                       block: int = 4096):
-    """Many portable roaring bitmaps (as decode_bitmaps) to their runs: CSR ptr u64 [n + 1] and the
-    runs [start, end) u32.  Run containers give their runs as written; other containers' values are
-    joined into runs."""
+def decode_bitmap_runs(data: np.ndarray, starts: np.ndarray, lens: np.ndarray, name: str = "roaring"):
+    """Many portable roaring bitmaps (bitmap i at data[starts[i]:starts[i] + lens[i]]) to their
+    maximal runs: CSR ptr u64 [n + 1] and the runs [start, end) u32, decoded natively
+    (rdb_archive_bitmap_runs, two passes: counts, then runs).  Malformed bitmaps raise
+    SiloFormatError."""
+    import ctypes
+
+    from .engine import _p, lib
+
+    data = np.ascontiguousarray(data, np.uint8)
+    starts = np.ascontiguousarray(starts, np.uint64)
+    lens = np.ascontiguousarray(lens, np.uint64)
     n = len(starts)
-    ptrs, rs, re_ = [np.zeros(1, np.uint64)], [], []
-    total = 0
-    for b in range(0, n, block):
-        st = starts[b:b + block].astype(np.int64)
-        ln = lens[b:b + block].astype(np.int64)
-        m = len(st)
-        if np.any(ln < 8):
-            raise SiloFormatError(f"{name}: roaring bitmap too short")
-        u32 = lambda at: _gather_u16(data, at) | (_gather_u16(data, at + 2) << np.uint32(16))
-        cookie = u32(st)
-        # one run container: cookie (size 1), flag byte with the run bit, key, cardinality - 1,
-        # run count, (start, length - 1) pairs
-        run = (cookie == _SERIAL_COOKIE) & (ln >= 11)
-        run &= np.where(run, data[np.where(run, st + 4, 0)] & 1, 0) == 1
-        n_runs = np.where(run, _gather_u16(data, np.where(run, st + 9, 0)).astype(np.int64), 0)
-        run &= ln == 11 + 4 * n_runs
-        counts = np.zeros(m, np.int64)
-        r_idx = np.nonzero(run)[0]
-        pair_at = _ranges(st[r_idx] + 11, n_runs[r_idx] * 4)[::4]
-        key = np.repeat(_gather_u16(data, st[r_idx] + 5).astype(np.int64) << 16, n_runs[r_idx])
-        r_start = _gather_u16(data, pair_at).astype(np.int64) + key
-        r_end = r_start + _gather_u16(data, pair_at + 2).astype(np.int64) + 1
-        counts[r_idx] = n_runs[r_idx]
-        o_idx = np.nonzero(~run)[0]
-        o_start = o_end = np.zeros(0, np.int64)
-        if len(o_idx):
-            optr, ovals = _decode_block(data, st[o_idx], ln[o_idx], name)
-            v = ovals.astype(np.int64)
-            owner = np.repeat(np.arange(len(o_idx)), np.diff(optr.astype(np.int64)))
-            first = np.ones(len(v), bool)
-            first[1:] = (owner[1:] != owner[:-1]) | (v[1:] != v[:-1] + 1)
-            heads = np.nonzero(first)[0]
-            tails = np.concatenate([heads[1:], [len(v)]]) - 1
-            o_start, o_end = v[heads], v[tails] + 1
-            counts[o_idx] = np.bincount(owner[heads], minlength=len(o_idx))
-        ptr = np.zeros(m + 1, np.int64)
-        np.cumsum(counts, out=ptr[1:])
-        out_s = np.zeros(int(ptr[-1]), np.int64)
-        out_e = np.zeros(int(ptr[-1]), np.int64)
-        dest = _ranges(ptr[:-1][r_idx], n_runs[r_idx])
-        out_s[dest], out_e[dest] = r_start, r_end
-        dest = _ranges(ptr[:-1][o_idx], counts[o_idx])
-        out_s[dest], out_e[dest] = o_start, o_end
-        ptrs.append((ptr[1:] + total).astype(np.uint64))
-        rs.append(out_s.astype(np.uint32))
-        re_.append(out_e.astype(np.uint32))
-        total += int(ptr[-1])
-    empty = np.zeros(0, np.uint32)
-    return np.concatenate(ptrs), (np.concatenate(rs) if rs else empty), (np.concatenate(re_) if re_ else empty)
+    ptr = np.zeros(n + 1, np.uint64)
+    base = ctypes.c_void_p(data.ctypes.data)
+    rc = lib().rdb_archive_bitmap_runs(base, len(data), _p(starts), _p(lens), n, _p(ptr), None, None, 0)
+    if rc == 0:
+        total = int(ptr[-1])
+        rs, re_ = np.empty(max(total, 1), np.uint32), np.empty(max(total, 1), np.uint32)
+        rc = lib().rdb_archive_bitmap_runs(base, len(data), _p(starts), _p(lens), n, _p(ptr), _p(rs), _p(re_), total)
+    if rc != 0:
+        raise SiloFormatError(f"{name}: {lib().rdb_last_error(None).decode()}")
+    return ptr, rs[:total], re_[:total]
 
 
 def _decode_block(data: np.ndarray, starts: np.ndarray, lens: np.ndarray, name: str):

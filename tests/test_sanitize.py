@@ -3,7 +3,8 @@
 `lapis-silo_amd/csrc/sanitize_check` is the engine's host sources built with both sanitizers
 (the counterpart of the reference's Debug build with ASAN, CMakeLists.txt:11) and linked with a
 driver of the C ABI.  Its host mode runs the entry points that need no device: the diff of aligned
-sequences, the saved-state scanners on whole, truncated and corrupted buffers, and the NDJSON and
+sequences, the saved-state scanners and bitmap decoder on whole, truncated and corrupted buffers,
+and the NDJSON and
 Arrow IPC writers at every capacity below the one needed.  Any sanitizer finding ends the process
 with a report and a non-zero status."""
 import os

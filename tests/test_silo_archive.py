@@ -201,6 +201,47 @@ def test_portable_roaring_decode_round_trip(runs):
     assert sa.roaring_values(struct.pack("<II", 12346, 0)).size == 0
 
 
+def _runs_of(values: np.ndarray) -> list:
+    """Maximal runs [a, b) of sorted unique values, by brute force."""
+    out = []
+    for v in values.tolist():
+        if out and out[-1][1] == v:
+            out[-1][1] = v + 1
+        else:
+            out.append([v, v + 1])
+    return [tuple(r) for r in out]
+
+
+def test_native_bitmap_runs_match_the_values():
+    """rdb_archive_bitmap_runs (the reader's decode of horizontal bitmaps) gives the maximal runs of
+    the values each bitmap holds, for array, bitset and run containers, several containers per
+    bitmap and runs that cross a container boundary; malformed bitmaps raise SiloFormatError."""
+    from silo_amd import silo_archive as sa
+
+    rng = np.random.default_rng(11)
+    bitmaps, want = [], []
+    for trial in range(60):
+        parts = [rng.integers(0, 1 << 16, size=int(rng.integers(1, 50)), dtype=np.uint32) + (7 << 16),
+                 np.arange((3 << 16) - int(rng.integers(0, 9)), (3 << 16) + int(rng.integers(1, 65536)), dtype=np.uint32),
+                 rng.choice(1 << 16, size=int(rng.integers(4097, 30000)), replace=False).astype(np.uint32) + (9 << 16),
+                 np.zeros(0, np.uint32)]
+        vals = np.unique(np.concatenate(parts[: 1 + trial % 4]))
+        bitmaps.append(encode_portable(vals, trial % 2 == 1))
+        want.append(_runs_of(vals))
+    bitmaps.append(struct.pack("<II", 12346, 0))
+    want.append([])
+    data = np.frombuffer(b"".join(bitmaps), np.uint8)
+    starts = np.cumsum([0] + [len(b) for b in bitmaps[:-1]])
+    ptr, rs, re_ = sa.decode_bitmap_runs(data, starts, np.array([len(b) for b in bitmaps]))
+    for i, w in enumerate(want):
+        got = list(zip(rs[ptr[i]:ptr[i + 1]].tolist(), re_[ptr[i]:ptr[i + 1]].tolist()))
+        assert got == w, i
+    good = encode_portable(np.array([1, 2, 3, 70000], np.uint32), False)
+    for bad in (good[:-1], b"\x00" * 8, good[:4], struct.pack("<II", 12346, 70000)):
+        with pytest.raises(sa.SiloFormatError):
+            sa.decode_bitmap_runs(np.frombuffer(bad, np.uint8), np.array([0]), np.array([len(bad)]))
+
+
 def test_damaged_or_foreign_archives_are_refused(tmp_path):
     from silo_amd import silo_archive as sa
 
