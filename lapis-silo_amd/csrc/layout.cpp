@@ -1310,12 +1310,32 @@ void buildDictColumn(Table& t, DictColumn& col, const uint32_t* ids, const uint8
       std::vector<std::vector<std::vector<uint16_t>>> split(nv);
       std::vector<uint8_t> use(nv, 0);
       parallelFor(nv, [&](size_t v) {
-         std::vector<uint32_t> acc;
-         for (uint32_t w : under[v]) acc.insert(acc.end(), own[w].begin(), own[w].end());
-         if (acc.empty()) return;
-         std::sort(acc.begin(), acc.end());
-         split[v] = t.storeChunks(acc);
+         uint64_t total = 0;
+         for (uint32_t w : under[v]) total += own[w].size();
+         if (total == 0) return;
          use[v] = 1;
+         if (total < (uint64_t{1} << 16)) {
+            std::vector<uint32_t> acc;
+            acc.reserve(total);
+            for (uint32_t w : under[v]) acc.insert(acc.end(), own[w].begin(), own[w].end());
+            std::sort(acc.begin(), acc.end());
+            split[v] = t.storeChunks(acc);
+            return;
+         }
+         // a large subtree (a root's can hold every row): its rows set bits of per-chunk bitsets,
+         // read back in order, instead of sorting them
+         std::vector<uint64_t> bits(static_cast<size_t>(t.n_chunks) * kChunkWords, 0);
+         for (uint32_t w : under[v]) {
+            for (uint32_t g : own[w]) bits[g >> 6] |= 1ULL << (g & 63);  // own rows lie in the table
+         }
+         auto& out = split[v];
+         out.resize(t.n_chunks);
+         for (uint32_t c = 0; c < t.n_chunks; ++c) {
+            const uint64_t* cw = bits.data() + static_cast<size_t>(c) * kChunkWords;
+            for (uint32_t k = 0; k < kChunkWords; ++k) {
+               for (uint64_t x = cw[k]; x; x &= x - 1) out[c].push_back(static_cast<uint16_t>(64 * k + std::countr_zero(x)));
+            }
+         }
       });
       for (uint32_t v = 0; v < nv; ++v) {
          if (use[v]) col.sublineage_bitmap[v] = t.addStoreChunks(std::move(split[v]));
