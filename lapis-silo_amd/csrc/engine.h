@@ -209,7 +209,7 @@ struct SeqColumn {
    std::vector<uint32_t> slot_rows;          // rows per slot over all chunks (planner statistics)
    std::vector<uint32_t> coverage;           // covered rows per position (planner statistics)
    uint64_t row_major_bytes = 0;             // row-major mutation CSR (sparse path)
-   std::vector<uint32_t> row_mut_ptr, row_nrun_ptr;  // host copies over padded rows (traffic accounting)
+   RawVector<uint32_t> row_mut_ptr, row_nrun_ptr;  // host copies over padded rows (traffic accounting)
    DeviceBuffer sparse_counts, sparse_bnd;   // zero between calls; cleared by the sparse finalize
    // wall time (ms) of the index build's phases: checks, adaptation, containers, slots and tiles,
    // row-level arrays, layout checks, upload

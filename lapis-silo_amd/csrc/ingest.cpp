@@ -321,8 +321,8 @@ struct Hasher {
       h *= 0xFF51AFD7ED558CCDULL;
       h ^= h >> 29;
    }
-   template <typename T>
-   void vec(const std::vector<T>& v) {
+   template <typename T, typename A>
+   void vec(const std::vector<T, A>& v) {
       mix(v.size());
       words(v.data(), v.size() * sizeof(T));
    }

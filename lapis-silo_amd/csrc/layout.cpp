@@ -788,7 +788,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    std::vector<uint32_t> light_ptr(n_tiles + 1, 0), heavy_ptr(n_tiles + 1, 0), group_ptr(n_tiles + 1, 0);
    // per (chunk, position): [lo, hi) of the position's light entries, groups and descriptors, and
    // the mask of 4096-row slices that hold any of its rows
-   std::vector<uint32_t> pos_ranges(static_cast<size_t>(t.n_chunks) * L * kPosRecord, 0);
+   RawVector<uint32_t> pos_ranges(static_cast<size_t>(t.n_chunks) * L * kPosRecord);  // zeroed per chunk below
    auto sliceMask = [&](const ChunkContainers& ch, const Desc& d) -> uint32_t {
       const uint32_t type = descType(d.w0), card = descCard(d.w0);
       uint32_t m = 0;
@@ -872,6 +872,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    RawVector<Desc> desc(heavy_ptr[n_tiles]);
    parallelFor(t.n_chunks, [&](size_t c) {
       const auto& ch = chunks[c];
+      std::fill_n(pos_ranges.data() + c * L * kPosRecord, static_cast<size_t>(L) * kPosRecord, 0u);
       size_t k = 0;
       for (uint32_t b = 0; b < n_blocks; ++b) {
          const uint32_t pend = col.block_pos_start[b + 1];
@@ -947,7 +948,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    phase(3);
    // --- row-level arrays (padded to 2^16 rows per chunk) ------------------------------------------
    const size_t padded = static_cast<size_t>(t.n_chunks) << 16;
-   std::vector<uint32_t> cov(padded, 0), pptr(padded + 1, 0);
+   RawVector<uint32_t> cov(padded), pptr(padded + 1);  // every entry written per chunk below
    std::vector<uint64_t> null_bits(static_cast<size_t>(t.n_chunks) * kChunkWords, 0);
    col.chunk_nruns.assign(t.n_chunks, 0);
    if (nptr[n] > 0xFFFFFFFFULL) fail(RDB_ENOMEM, "too many missing runs");
@@ -959,6 +960,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
       const uint64_t off = t.chunk_offset[c];
       const uint32_t size = t.chunk_sizes[c];
       uint64_t muts = 0;
+      std::fill_n(cov.data() + (c << 16), kChunkRows, 0u);
       for (uint32_t r = 0; r < size; ++r) {
          const uint64_t i = off + r;
          const size_t g = (c << 16) | r;
@@ -1017,8 +1019,7 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
    // row-major mutations over padded rows (null rows have none)
    for (uint32_t c = 0; c < t.n_chunks; ++c) chunk_muts[c + 1] += chunk_muts[c];
    if (chunk_muts[t.n_chunks] > 0xFFFFFFFFULL) fail(RDB_ENOMEM, "too many mutations for 32-bit offsets");
-   std::vector<uint32_t> rmptr(padded + 1, 0);
-   RawVector<uint32_t> rmuts(chunk_muts[t.n_chunks]);
+   RawVector<uint32_t> rmptr(padded + 1), rmuts(chunk_muts[t.n_chunks]);
    parallelFor(t.n_chunks, [&](size_t c) {
       uint32_t acc = static_cast<uint32_t>(chunk_muts[c]);
       for (uint32_t r = 0; r < kChunkRows; ++r) {
