@@ -462,7 +462,8 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
 
    // normalized internal missing runs: clipped to [start, end) as insertCoverage does
    // (horizontal_coverage_index.cpp:34-37), sorted and merged
-   std::vector<uint64_t> nptr(n + 1, 0);
+   RawVector<uint64_t> nptr(n + 1);  // every entry written below
+   nptr[0] = 0;
    std::vector<std::vector<uint32_t>> row_runs_tmp(trusted ? 0 : t.n_chunks);
    if (!trusted) parallelFor(t.n_chunks, [&](size_t c) {
       auto& out = row_runs_tmp[c];
@@ -515,21 +516,29 @@ void buildSequenceColumn(Table& t, SeqColumn& col, const uint8_t* is_null, const
          }
       }
    });
-   std::vector<uint32_t> runs_all;
+   RawVector<uint32_t> runs_all;
    if (trusted) {
       // rows of the device diff: runs already internal, sorted and disjoint
       std::copy(nrun_ptr, nrun_ptr + n + 1, nptr.begin());
       runs_all.resize(nptr[n]);
       for (uint64_t k = 0; k < nptr[n]; ++k) runs_all[k] = nrun_start[k] | (nrun_end[k] << 16);
    } else {
-      for (uint64_t i = 0; i < n; ++i) nptr[i + 1] += nptr[i];
-      runs_all.resize(nptr[n]);
-      size_t o = 0;
-      for (auto& v : row_runs_tmp) {
-         std::copy(v.begin(), v.end(), runs_all.begin() + static_cast<ptrdiff_t>(o));
-         o += v.size();
+      // nptr[i + 1] holds row i's run count: offsets per chunk from the chunks' totals, then each
+      // chunk's running sums and its runs in parallel
+      std::vector<uint64_t> chunk_base(t.n_chunks + 1, 0);
+      for (uint32_t c = 0; c < t.n_chunks; ++c) chunk_base[c + 1] = chunk_base[c] + row_runs_tmp[c].size();
+      runs_all.resize(chunk_base[t.n_chunks]);
+      parallelFor(t.n_chunks, [&](size_t c) {
+         uint64_t acc = chunk_base[c];
+         for (uint64_t i = t.chunk_offset[c]; i < t.chunk_offset[c + 1]; ++i) {
+            acc += nptr[i + 1];
+            nptr[i + 1] = acc;
+         }
+         auto& v = row_runs_tmp[c];
+         std::copy(v.begin(), v.end(), runs_all.begin() + static_cast<ptrdiff_t>(chunk_base[c]));
          std::vector<uint32_t>().swap(v);
-      }
+      });
+      if (nptr[n] != chunk_base[t.n_chunks]) fail(RDB_EDEVICE, "internal: missing-run offsets");
    }
 
    phase(0);
